@@ -1,0 +1,164 @@
+"""Benchmark of the plain FP64 c3 + Bayes-energy path on one MI355X.
+
+Workload (the full-size plain configuration): 1000 seeded random point atoms
+in a ball of radius 26, KMAX=38 shells (qmax = pi), intensity expanded to L=18
+(even l), c3 over all 3276 k1>=k2>=k3 triples of K=26 shells on the N=32
+alpha/beta grid (2048 bins), scored against a Poisson histogram of 1e9 counts.
+
+A step is one c3 + energy evaluation of the device-resident structure
+(tp_state_energy: triple products, FP64-MFMA contraction with the fused
+energy epilogue, final reduction).  value = evaluations per second.
+The 2000-step plain annealing run (tp_anneal) gives MC steps/s.
+
+    python bench.py [--steps K] [--warmup W] [--anneal-steps S] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "threephotons.jl_amd"))
+
+METRIC = "c3 evals/s + MC annealing steps/s at K=26 L=18 N=32; time-to-energy vs plain FP64"
+FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (matrix and vector)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def setup(cfg, tp, workload):
+    t0 = time.time()
+    atoms = workload.atoms_in_ball(cfg["natoms"], cfg["radius"], cfg["atoms_seed"])
+    truth = tp.intensity_from_atoms(atoms, cfg["KMAX"], cfg["qmax"], cfg["LMAX"], cfg["L"], cfg["ntheta"], cfg["nphi"])
+    log(f"intensity: {time.time() - t0:.2f}s")
+    t0 = time.time()
+    basis = tp.HIP_calculate_basis(cfg["L"], cfg["LMAX"], cfg["N"], cfg["K"], 0.0, truth.dq())
+    log(f"basis (basislen {basis.basislen}, n_lt {basis.n_lt}, klength {basis.klength}): {time.time() - t0:.2f}s")
+    c3 = tp.FullCorrelation_parallized(truth, basis, return_raw=True)
+    c3ref, _ = workload.poisson_histogram(c3, cfg["N"], cfg["K"], cfg["counts"], cfg["hist_seed"])
+    start = workload.random_start_structure(truth, cfg["L"], cfg["start_seed"])
+    return truth, basis, c3ref, start
+
+
+def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
+    """Reference formulation (dense basislen-inner contraction B*PA with the
+    triple-product stage, correlations.jl:157-193) timed by the oracle on a
+    sample of k-triple columns, scaled to whole evaluations."""
+    from oracle import pyoracle as po
+    ob = po.Basis(cfg["L"], cfg["LMAX"], cfg["N"], cfg["K"])
+    t0 = time.perf_counter()
+    ob.c3_dense(start.coeff, 0, ncols, nthreads=nthreads)
+    t_dense = time.perf_counter() - t0
+    dense_evals = (ncols / ob.klength) / t_dense
+    t0 = time.perf_counter()
+    c3f = ob.c3_factored(start.coeff, nthreads=nthreads)
+    po.energy(c3f, cfg["N"], c3ref, cfg["K"], 1, cfg["K"])
+    t_fact = time.perf_counter() - t0
+    return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
+                sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
+                       f"({t_dense:.1f}s), scaled to whole evaluations; host '{cpu_model()}'",
+                factored_evals_per_s=1.0 / t_fact,
+                factored_sample=f"one full factored evaluation (F.As + energy), {t_fact:.2f}s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--anneal-steps", type=int, default=2000)
+    ap.add_argument("--cpu-cols", type=int, default=192)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the k-triple sharding "
+              "design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import tp_amd as tp
+    from tp_amd import workload
+    cfg = workload.CONFIGS["full"]
+    ctx = tp.HIP_init()
+    truth, basis, c3ref, start = setup(cfg, tp, workload)
+    state = tp.DeviceState(start, basis)
+    state.energy(c3ref)   # histogram upload happens here, outside the timed region
+
+    for _ in range(args.warmup):
+        state.energy(c3ref)
+    ctx.set_timing(True)
+    kern = np.zeros(3)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        E = state.energy(c3ref)
+        kern += ctx.last_timings()
+    elapsed = time.perf_counter() - t0
+    ctx.set_timing(False)
+    evals_per_s = args.steps / elapsed
+    kern_ms = kern / args.steps
+    flops = 2.0 * basis.R * basis.n_lt * basis.klength       # l-triple contraction, per evaluation
+    achieved = flops / (kern_ms[1] * 1e-3) / 1e12
+    log(f"evals/s {evals_per_s:.1f}  kernels ms: triple {kern_ms[0]:.4f} contract {kern_ms[1]:.4f} "
+        f"final {kern_ms[2]:.4f}  E={E:.10g}")
+
+    # plain annealing (2000 steps), device resident
+    astate = tp.DeviceState(start, basis)
+    t0 = time.perf_counter()
+    res = tp.rotate_all_at_once(astate, basis, c3ref, args.anneal_steps, np.pi / 4, 1.01, 0.1, 0.99998, seed=1000,
+                                ntemp=50)
+    t_anneal = time.perf_counter() - t0
+    steps_per_s = (args.anneal_steps + 50 + 1) / t_anneal
+    log(f"anneal: {args.anneal_steps} steps in {t_anneal:.2f}s, E0 {res['E0']:.10g} -> {res['E_final']:.10g}, "
+        f"accepted {int(res['accepted'].sum())}")
+
+    out = {
+        "metric": METRIC,
+        "value": evals_per_s,
+        "unit": "c3+energy evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain FP64 c3 + Bayes energy: 1000 random point atoms (r=26), KMAX=38, L=18, K=26, "
+                        "N=32 -> 3276 k-triples x 2048 bins, 505 l-triples; 1e9-count Poisson histogram",
+            "form": "coupling-form triple products + l-triple contraction c3 = F.As (FP64 MFMA) + fused energy",
+        },
+        "mc_steps_per_s": steps_per_s,
+        "mc_anneal": {"steps": args.anneal_steps, "temperature_probes": 50, "seconds": t_anneal,
+                      "E0": res["E0"], "E_final": res["E_final"], "accepted": int(res["accepted"].sum())},
+        "kernel_ms": {"triple": kern_ms[0], "contract": kern_ms[1], "final": kern_ms[2]},
+        "roofline": {"bound": "mfma", "kernel": "k_contract", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops},
+    }
+    if not args.no_cpu:
+        nthreads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/*
+ * tp.h -- C ABI of libtp.so, the MI355X (gfx950) implementation of the
+ * ThreePhotons.jl three-photon correlation c3(k1,k2,k3,alpha,beta) and its
+ * Bayes log-likelihood energy, as evaluated on every Monte Carlo step of the
+ * rotate_all_at_once annealing.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns 0
+ * on success or a negative TP_E_* code (the message is in tp_last_error()),
+ * and never throws, aborts or exits.  Calls are synchronous: results are in
+ * the caller's buffers when the call returns.  Host pointers are borrowed for
+ * the duration of the call only.  A tp_ctx is used by one host thread at a
+ * time.
+ *
+ * Layouts follow the reference's Julia (column-major) arrays:
+ *   coefficients  complex128 interleaved (re,im), shell k (1-based) starts at
+ *                 element (k-1)*ld, entry (l,m) at seanindex(l,m,LMAX)-1
+ *                 (spherical_harmonics.jl:163-170); ld >= LMAX^2.
+ *   c3raw         R x klength, R = 2N^2, row r = a + (b-1)N, column kt in the
+ *                 order for k1=1:K, k2=1:k1, k3=1:k2 (correlations.jl:179-184,
+ *                 return_raw=true).
+ *   c3ref         N x 2N x Kr x Kr x Kr float64, element (a,b,k3,k2,k1)
+ *                 (datagen.jl:432-475 after loadHistograms).
+ */
+#ifndef TP_H
+#define TP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TP_ABI_VERSION 1
+
+#define TP_OK 0
+#define TP_E_ARG (-1)          /* invalid argument or shape */
+#define TP_E_HIP (-2)          /* HIP runtime error (device missing, launch failure) */
+#define TP_E_OOM (-3)          /* device or host allocation failed */
+#define TP_E_STATE (-4)        /* object used with a mismatching basis/context */
+#define TP_E_UNSUPPORTED (-5)  /* parameter combination not implemented */
+
+typedef struct tp_ctx tp_ctx;
+typedef struct tp_basis tp_basis;
+typedef struct tp_ref tp_ref;
+typedef struct tp_state tp_state;
+
+/* ABI version of the loaded library (== TP_ABI_VERSION). */
+int tp_abi_version(void);
+
+/* Thread-local text of the last error. */
+const char *tp_last_error(void);
+
+/* Open device `device` (HIP ordinal) and create a context with its own stream.
+ * Replaces CUDA_init (cuda.jl:10-21); a failure lets the caller fall back to
+ * the reference's CPU BasisType, as rotation_search does
+ * (determination.jl:210-216). */
+int tp_init(int device, tp_ctx **out);
+void tp_finalize(tp_ctx *ctx);
+
+/* Precompute the basis in FP64 on the host and upload it.
+ * Replaces calculate_basis + CUDA_store_basis / CUDA_calculate_basis
+ * (correlations.jl:87-153, cuda.jl:45-65).  for_intensity=1 selects even l
+ * (lrange = 0:2:L).  lambda != 0 returns TP_E_UNSUPPORTED in this version. */
+int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq,
+                    int for_intensity, tp_basis **out);
+void tp_basis_destroy(tp_basis *b);
+
+/* basislen, number of l-triples, klength = K(K+1)(K+2)/6, R = 2N^2. */
+int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *klength, int64_t *R);
+
+/* c3raw for the coefficients of shells 1..K.
+ * Replaces FullCorrelation_parallized(intensity, basis, minimal, normalize,
+ * return_raw=true) (correlations.jl:179-184, cuda.jl:74-98). */
+int tp_c3(tp_ctx *ctx, const tp_basis *b, const double *coeff, int64_t ld, double *c3_out);
+
+/* Compress a reference histogram for the energy: the slices visited by the
+ * K3_range = k3lo:k3hi walk of energy() (correlations.jl:211-217) are
+ * normalised (q / sum|q|) and kept on the device.  Kr is the shell extent of
+ * the c3ref array.  Mirrors the c3ref compression of new_energy
+ * (cuda.jl:127-139). */
+int tp_ref_create(tp_ctx *ctx, const tp_basis *b, const double *c3ref, int Kr, int k3lo, int k3hi,
+                  tp_ref **out);
+void tp_ref_destroy(tp_ref *r);
+
+/* Bayes energy of host coefficients (shells 1..K used).
+ * Replaces energy(intensity, basis, c3ref, K3_range, measure) with
+ * measure "Bayes" (correlations.jl:201-226); measure 0 = "Bayes", any other
+ * value returns 0.0 as the reference does (correlations.jl:210). */
+int tp_energy(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const double *coeff, int64_t ld,
+              int measure, double *E_out);
+
+/* ---- device-resident structure (the annealing state) -------------------- */
+
+/* Upload nshell shells of coefficients (nshell >= K; the reference rotates all
+ * KMAX shells, determination.jl:236).  Only even l <= L are kept on the
+ * device; the rest of the caller's vector is untouched by tp_state_read. */
+int tp_state_create(tp_ctx *ctx, const tp_basis *b, const double *coeff, int64_t ld, int nshell,
+                    tp_state **out);
+void tp_state_destroy(tp_state *s);
+/* Write the device coefficients back into the caller's (ld-strided) vectors. */
+int tp_state_read(tp_ctx *ctx, const tp_state *s, double *coeff, int64_t ld);
+/* Energy / c3raw of the device-resident coefficients (no host transfer of
+ * coefficients).  These are the per-step calls of rotate_all_at_once. */
+int tp_state_energy(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, double *E_out);
+int tp_state_c3(tp_ctx *ctx, const tp_basis *b, tp_state *s, double *c3_out);
+
+/* Plain Monte Carlo simulated annealing, run entirely on the device.
+ * Replaces rotate_all_at_once (determination.jl:281-338) with
+ * calculate_temperature (:244-261), get_MC_step (:232-241) and evaluate_step
+ * (:491-515); nsteps fixed steps replace the step-size stopping rule.
+ * Random numbers: Philox4x32-10 keyed by `seed`.
+ * Outputs (host, may be NULL): E_trace[nsteps] (E after step i),
+ * Enew_trace[nsteps] (proposal energy), acc_trace[nsteps] (1 = accepted),
+ * scalars[4] = {E0, T0, E_final, T_final}.  The state holds the final
+ * structure. */
+typedef struct tp_anneal_params {
+    int nsteps;
+    int ntemp;                          /* proposals sampled for T0 (reference: 50) */
+    double initial_stepsize;            /* stepsize_l = initial_stepsize*(l-1) */
+    double stepsizefactor;
+    double initial_temperature_factor;
+    double temperature_decay;
+    uint64_t seed;
+} tp_anneal_params;
+
+int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
+              double *E_trace, double *Enew_trace, int32_t *acc_trace, double *scalars);
+
+/* ---- synthetic input ------------------------------------------------------ */
+
+/* Spherical-harmonic coefficients of the intensity of point atoms with unit
+ * form factor, I(q) = |sum_j exp(i q.r_j)|^2, on shells q_k = k*dq,
+ * k = 1..KMAX, by Gauss-Legendre (cos theta) x uniform-phi quadrature with
+ * ntheta x nphi points; entries with l <= lfill are written, the rest zeroed.
+ * Counterpart of getIntensity_cartesian / createSphericalHarmonicsStructure
+ * (structure.jl:74-134) for point atoms. */
+int tp_intensity_from_atoms(tp_ctx *ctx, const double *atoms_xyz, int natoms, int KMAX, double dq, int LMAX,
+                            int lfill, int ntheta, int nphi, double *coeff_out);
+
+/* ---- instrumentation ------------------------------------------------------ */
+
+/* Per-kernel device time (ms) of the last tp_state_energy call, measured
+ * with HIP events on the context stream: [0] coefficient pack/As stage,
+ * [1] contraction + energy partials, [2] final reduction.  Enabled by
+ * tp_set_timing(ctx, 1). */
+int tp_set_timing(tp_ctx *ctx, int on);
+int tp_last_timings(const tp_ctx *ctx, double *ms_out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TP_H */

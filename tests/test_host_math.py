@@ -1,0 +1,62 @@
+"""libtp's host FP64 special functions (basis precompute) against the golden
+vectors: the header is compiled for the host with g++.  CPU only."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import seanindex
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(HERE), "threephotons.jl_amd", "csrc")
+GOLD = os.path.join(HERE, "golden")
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("hm") / "host_math_driver")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", CSRC, "-o", exe, os.path.join(HERE, "host_math_driver.cpp")],
+                   check=True)
+
+    def run(cmds):
+        out = subprocess.run([exe], input="\n".join(cmds) + "\n", capture_output=True, text=True, check=True)
+        return out.stdout.split("\n")
+
+    return run
+
+
+def test_wigner3j_golden(driver):
+    rows = json.load(open(os.path.join(GOLD, "wigner3j.json")))
+    got = driver(["w %d %d %d %d %d %d" % tuple(r[:6]) for r in rows])
+    worst = max(abs(float(g) - r[6]) / max(abs(r[6]), 1e-3) for g, r in zip(got, rows))
+    assert worst < 1e-14, worst
+
+
+def test_sphplm_golden(driver):
+    rows = json.load(open(os.path.join(GOLD, "sphplm.json")))
+    got = driver(["p %d %d %.17g" % (l, m, th) for l, m, th, _ in rows])
+    worst = max(abs(float(g) - r[3]) / max(abs(r[3]), 1e-2) for g, r in zip(got, rows))
+    assert worst < 1e-13, worst
+
+
+def test_seanindex(driver):
+    cmds, want = [], []
+    for L in (1, 7, 25):
+        for l in range(L):
+            for m in range(-l, l + 1):
+                cmds.append(f"s {l} {m} {L}")
+                want.append(seanindex(l, m, L))
+    got = [int(x) for x in driver(cmds)[: len(want)]]
+    assert got == want
+
+
+def test_gauss_legendre(driver):
+    for n in (1, 5, 48, 96):
+        rows = driver([f"g {n}"])[:n]
+        x = np.array([float(r.split()[0]) for r in rows])
+        w = np.array([float(r.split()[1]) for r in rows])
+        xr, wr = np.polynomial.legendre.leggauss(n)
+        assert np.abs(np.sort(x) - xr).max() < 1e-14
+        assert np.abs(w[np.argsort(x)] - wr).max() < 1e-14

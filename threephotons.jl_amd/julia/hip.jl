@@ -1,0 +1,126 @@
+# hip.jl -- MI355X backend for ThreePhotons.jl, included by the module root in
+# place of (or after) cuda.jl, exactly as cuda.jl is included today
+# (src/ThreePhotons.jl:145-150).  Every method calls libtp.so through ccall;
+# no CUDA/CUBLAS package is loaded.
+#
+#   HIP_init()                         replaces CUDA_init            (cuda.jl:10-21)
+#   BasisTypeHip <: AbstractBasisType  replaces BasisTypeCuda        (cuda.jl:24-42)
+#   HIP_calculate_basis(...)           replaces CUDA_calculate_basis (cuda.jl:57-69)
+#   FullCorrelation_parallized(::SphericalHarmonicsVolume, ::BasisTypeHip, ...)   (cuda.jl:74-108)
+#   energy(::SphericalHarmonicsVolume, ::BasisTypeHip, ::C3, ::UnitRange, ...)    (correlations.jl:201-226)
+#
+# Written against Julia 0.6 syntax like the rest of the package.
+
+export BasisTypeHip, HIP_init, HIP_calculate_basis
+
+const libtp = get(ENV, "THREEPHOTONS_LIBTP", joinpath(dirname(@__FILE__), "..", "libtp.so"))
+HIP_enabled = false
+hip_ctx = C_NULL
+
+function hip_check(rc::Cint, what::String)
+    if rc != 0
+        msg = unsafe_string(ccall((:tp_last_error, libtp), Cstring, ()))
+        error("$what failed ($rc): $msg")
+    end
+end
+
+"""Opens device 0 (replaces CUDA_init)."""
+function HIP_init(device::Integer=0)
+    out = Ref{Ptr{Void}}(C_NULL)
+    hip_check(ccall((:tp_init, libtp), Cint, (Cint, Ptr{Ptr{Void}}), device, out), "tp_init")
+    global hip_ctx = out[]
+    global HIP_enabled = true
+    println("Initialization of HIP complete.")
+end
+
+"""Precomputed basis resident on the GPU (replaces BasisTypeCuda)."""
+type BasisTypeHip <: AbstractBasisType
+    handle::Ptr{Void}
+    refs::Dict{Any,Ptr{Void}}
+    basislen::Int64
+    N::Int64
+    L::Int64
+    LMAX::Int64
+    lrange::StepRange
+    ctr::Dict
+    rtc::Dict
+    K::Int64
+    lambda::Float64
+    dq::Float64
+end
+
+function HIP_calculate_basis(L::Int64, LMAX::Int64, N::Int64, K::Int64, lambda::Float64, dq::Float64, forIntensity=true)
+    if !HIP_enabled
+        return calculate_basis(L, LMAX, N, K, lambda, dq, forIntensity)   # the reference CPU path
+    end
+    out = Ref{Ptr{Void}}(C_NULL)
+    hip_check(ccall((:tp_basis_create, libtp), Cint,
+                    (Ptr{Void}, Cint, Cint, Cint, Cint, Cdouble, Cdouble, Cint, Ptr{Ptr{Void}}),
+                    hip_ctx, L, LMAX, N, K, lambda, dq, forIntensity ? 1 : 0, out), "tp_basis_create")
+    bl = Ref{Int64}(0)
+    hip_check(ccall((:tp_basis_info, libtp), Cint, (Ptr{Void}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+                    out[], bl, C_NULL, C_NULL, C_NULL), "tp_basis_info")
+    lrange = forIntensity ? (0:2:L) : (0:L)
+    ctr = Dict(l => Umat(l) for l = lrange)
+    rtc = Dict(l => ctranspose(ctr[l]) for l = lrange)
+    b = BasisTypeHip(out[], Dict{Any,Ptr{Void}}(), bl[], N, L, LMAX, lrange, ctr, rtc, K, lambda, dq)
+    finalizer(b, x -> begin
+        for r in values(x.refs) ccall((:tp_ref_destroy, libtp), Void, (Ptr{Void},), r) end
+        ccall((:tp_basis_destroy, libtp), Void, (Ptr{Void},), x.handle)
+    end)
+    return b
+end
+
+function HIP_calculate_basis(L::Int64, LMAX::Int64, N::Int64, K::Int64, forIntensity=true)
+    return HIP_calculate_basis(L, LMAX, N, K, 0.0, 0.0, forIntensity)
+end
+
+global complexBasis_choice = HIP_calculate_basis
+
+"""Coefficients of shells 1..K as one contiguous LMAX^2 x K complex matrix (no conjugation)."""
+function hip_pack(intensity::SphericalHarmonicsVolume, K::Int64)
+    numcoeff = num_coeff(intensity.LMAX)
+    return Complex{Float64}[intensity.coeff[k][i] for i = 1:numcoeff, k = 1:K]
+end
+
+function FullCorrelation_parallized(intensity::SphericalHarmonicsVolume, basis::BasisTypeHip, minimal::Bool=true, normalize::Bool=false, return_raw::Bool=false)
+    klength = Integer(basis.K*(basis.K+1)*(basis.K+2)/6)
+    coeff = hip_pack(intensity, basis.K)
+    res = zeros(Float64, 2*basis.N^2, klength)
+    hip_check(ccall((:tp_c3, libtp), Cint, (Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Ptr{Float64}),
+                    hip_ctx, basis.handle, coeff, size(coeff, 1), res), "tp_c3")
+    if return_raw return res end
+    t = zeros(Float64, basis.N, 2*basis.N, basis.K, basis.K, basis.K)
+    i = 1
+    for k1=1:basis.K for k2=1:k1 for k3=1:k2
+        t[:,:, k3, k2, k1] = reshape(res[:, i], basis.N, 2*basis.N)
+        i += 1
+    end end end
+    return normalize ? t / sum(abs, t) : t
+end
+
+"""Device copy of the normalised c3ref slices of the K3_range walk (cached per histogram)."""
+function hip_ref(basis::BasisTypeHip, c3ref::C3, K3_range::UnitRange{Int64})
+    key = (object_id(c3ref), K3_range)
+    if !haskey(basis.refs, key)
+        out = Ref{Ptr{Void}}(C_NULL)
+        hip_check(ccall((:tp_ref_create, libtp), Cint, (Ptr{Void}, Ptr{Void}, Ptr{Float64}, Cint, Cint, Cint, Ptr{Ptr{Void}}),
+                        hip_ctx, basis.handle, c3ref, size(c3ref, 3), minimum(K3_range), maximum(K3_range), out),
+                  "tp_ref_create")
+        basis.refs[key] = out[]
+    end
+    return basis.refs[key]
+end
+
+function energy(intensity::SphericalHarmonicsVolume, basis::BasisTypeHip, c3ref::C3, K3_range::UnitRange{Int64}, measure::String="Bayes", negativity_factor::Float64=0.0)
+    ref = hip_ref(basis, c3ref, K3_range)
+    coeff = hip_pack(intensity, basis.K)
+    E = Ref{Float64}(0.0)
+    hip_check(ccall((:tp_energy, libtp), Cint, (Ptr{Void}, Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Cint, Ptr{Float64}),
+                    hip_ctx, basis.handle, ref, coeff, size(coeff, 1), measure == "Bayes" ? 0 : 1, E), "tp_energy")
+    return E[]
+end
+
+function energy(intensity::SphericalHarmonicsVolume, basis::BasisTypeHip, c3ref::C3, measure::String="Bayes", negativity_factor::Float64=0.0)
+    return energy(intensity, basis, c3ref, 1:basis.K, measure, negativity_factor)
+end

@@ -1,0 +1,271 @@
+"""Host-side mirror of the ThreePhotons.jl GPU backend (cuda.jl) over libtp.so.
+
+Names, argument meaning and error behaviour follow the reference's Julia API
+for this path:
+
+  HIP_init()                         ~ CUDA_init            (cuda.jl:10-21)
+  HIP_calculate_basis(...)           ~ CUDA_calculate_basis (cuda.jl:57-65)
+  FullCorrelation_parallized(...)    (cuda.jl:74-108, correlations.jl:179-193)
+  energy(...)                        (correlations.jl:196-226)
+  rotate_all_at_once(...)            (determination.jl:281-338)
+
+Arrays use the reference's index order with 0-based numpy indexing:
+coefficients are a (KMAX, LMAX^2) complex array (row k-1 is intensity.coeff[k]
+in seanindex order), c3 histograms are (N, 2N, K, K, K) arrays indexed
+[a, b, k3, k2, k1].  Every call goes to the gfx950 library; there is no CPU
+fallback here (the reference's own CPU BasisType is the fallback when HIP_init
+fails).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import AnnealParams, TPError, build, check, lib
+
+__all__ = [
+    "TPError", "build", "lib", "SphericalHarmonicsVolume", "HIP_init", "HipContext", "BasisTypeHip",
+    "HIP_calculate_basis", "FullCorrelation_parallized", "energy", "DeviceState", "rotate_all_at_once",
+    "intensity_from_atoms", "seanindex", "tripletFactor", "klength",
+]
+
+
+def seanindex(l, m, LMAX):
+    """1-based position of (l, m) in a length-LMAX^2 vector (spherical_harmonics.jl:163-170)."""
+    B = LMAX
+    if m >= 0:
+        return 1 + m * B - m * (m - 1) // 2 + (l - m)
+    a = B - 1 + m
+    return 1 + B * (B + 1) // 2 + a * (a + 1) // 2 + (l + m)
+
+
+def tripletFactor(k1, k2, k3):
+    """datagen.jl:166-174"""
+    if k1 == k2 == k3:
+        return 1.0
+    if k1 == k2 or k2 == k3 or k1 == k3:
+        return 3.0
+    return 6.0
+
+
+def klength(K):
+    return K * (K + 1) * (K + 2) // 6
+
+
+class SphericalHarmonicsVolume:
+    """Per-shell complex SH coefficients (spherical_harmonics.jl:9-31)."""
+
+    def __init__(self, coeff, LMAX, KMAX, rmax):
+        coeff = np.ascontiguousarray(coeff, dtype=np.complex128)
+        if coeff.shape != (KMAX, LMAX * LMAX):
+            raise ValueError(f"coeff must be (KMAX, LMAX^2) = ({KMAX}, {LMAX * LMAX}), got {coeff.shape}")
+        self.coeff, self.LMAX, self.KMAX, self.rmax = coeff, LMAX, KMAX, float(rmax)
+
+    def dq(self):
+        """spherical_harmonics.jl:102"""
+        return self.rmax / self.KMAX
+
+    def copy(self):
+        return SphericalHarmonicsVolume(self.coeff.copy(), self.LMAX, self.KMAX, self.rmax)
+
+
+class HipContext:
+    """Device context (one HIP stream).  HIP_init() returns the default one."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().tp_init(device, C.byref(h)), "tp_init")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tp_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on=True):
+        check(lib().tp_set_timing(self.h, int(on)), "tp_set_timing")
+
+    def last_timings(self):
+        out = (C.c_double * 3)()
+        check(lib().tp_last_timings(self.h, out, 3), "tp_last_timings")
+        return list(out)
+
+
+_default_ctx = None
+
+
+def HIP_init(device=0):
+    """~ CUDA_init (cuda.jl:10-21): open device 0 and keep it as the default context."""
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = HipContext(device)
+    return _default_ctx
+
+
+def _ctx(ctx):
+    return ctx if ctx is not None else HIP_init()
+
+
+class BasisTypeHip:
+    """~ BasisTypeCuda (cuda.jl:24-42): the precomputed basis, resident on the GPU."""
+
+    def __init__(self, L, LMAX, N, K, lam=0.0, dq=0.0, forIntensity=True, ctx=None):
+        self.ctx = _ctx(ctx)
+        h = C.c_void_p()
+        check(lib().tp_basis_create(self.ctx.h, L, LMAX, N, K, float(lam), float(dq), int(bool(forIntensity)),
+                                    C.byref(h)), "tp_basis_create")
+        self.h = h
+        self.L, self.LMAX, self.N, self.K, self.lam, self.dq = L, LMAX, N, K, float(lam), float(dq)
+        self.lrange = range(0, L + 1, 2 if forIntensity else 1)
+        bl, nl, kl, R = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().tp_basis_info(h, C.byref(bl), C.byref(nl), C.byref(kl), C.byref(R)), "tp_basis_info")
+        self.basislen, self.n_lt, self.klength, self.R = bl.value, nl.value, kl.value, R.value
+        self._refs = {}
+
+    def __del__(self):
+        try:
+            for r in self._refs.values():
+                lib().tp_ref_destroy(r[0])
+            self._refs = {}
+            if getattr(self, "h", None):
+                lib().tp_basis_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def ref(self, c3ref, K3_range=None):
+        """Device copy of the normalised histogram slices visited by energy()'s walk (cached)."""
+        if K3_range is None:
+            K3_range = range(1, self.K + 1)
+        key = (id(c3ref), K3_range.start, K3_range.stop)
+        hit = self._refs.get(key)
+        if hit is not None and hit[1] is c3ref:
+            return hit[0]
+        c3ref = np.asarray(c3ref, dtype=np.float64)
+        if c3ref.ndim != 5 or c3ref.shape[0] != self.N or c3ref.shape[1] != 2 * self.N:
+            raise ValueError("c3ref must be (N, 2N, Kr, Kr, Kr)")
+        Kr = c3ref.shape[2]
+        flat = np.ascontiguousarray(c3ref.ravel(order="F"))
+        h = C.c_void_p()
+        check(lib().tp_ref_create(self.ctx.h, self.h, flat.ctypes.data, Kr, K3_range.start, K3_range.stop - 1,
+                                  C.byref(h)), "tp_ref_create")
+        self._refs[key] = (h, c3ref)
+        return h
+
+
+def HIP_calculate_basis(L, LMAX, N, K, lam=0.0, dq=0.0, forIntensity=True, ctx=None):
+    """~ CUDA_calculate_basis (cuda.jl:57-65)."""
+    return BasisTypeHip(L, LMAX, N, K, lam, dq, forIntensity, ctx)
+
+
+def _coeff_ptr(intensity, basis, nshell=None):
+    c = intensity.coeff if isinstance(intensity, SphericalHarmonicsVolume) else np.asarray(intensity)
+    c = np.ascontiguousarray(c, dtype=np.complex128)
+    need = basis.K if nshell is None else nshell
+    if c.ndim != 2 or c.shape[0] < need or c.shape[1] < basis.LMAX * basis.LMAX:
+        raise ValueError(f"coefficients must be (>= {need}, >= LMAX^2) complex")
+    return c, c.shape[1]
+
+
+def FullCorrelation_parallized(intensity, basis, minimal=True, normalize=False, return_raw=False):
+    """c3 on the GPU (cuda.jl:74-108).  return_raw gives the (R, klength) matrix."""
+    c, ld = _coeff_ptr(intensity, basis)
+    res = np.zeros((basis.klength, basis.R))      # column kt contiguous == Julia R x klength
+    check(lib().tp_c3(basis.ctx.h, basis.h, c.ctypes.data, ld, res.ctypes.data), "tp_c3")
+    res = res.T
+    if return_raw:
+        return res
+    N, K = basis.N, basis.K
+    t = np.zeros((N, 2 * N, K, K, K))
+    i = 0
+    for k1 in range(1, K + 1):
+        for k2 in range(1, k1 + 1):
+            for k3 in range(1, k2 + 1):
+                t[:, :, k3 - 1, k2 - 1, k1 - 1] = res[:, i].reshape(2 * N, N).T
+                i += 1
+    return t / np.abs(t).sum() if normalize else t
+
+
+def energy(intensity, basis, c3ref, K3_range=None, measure="Bayes", negativity_factor=0.0):
+    """Bayes energy (correlations.jl:196-226).  negativity_factor is ignored, as in the reference."""
+    ref = basis.ref(c3ref, K3_range)
+    c, ld = _coeff_ptr(intensity, basis)
+    E = C.c_double()
+    check(lib().tp_energy(basis.ctx.h, basis.h, ref, c.ctypes.data, ld, 0 if measure == "Bayes" else 1,
+                          C.byref(E)), "tp_energy")
+    return E.value
+
+
+class DeviceState:
+    """Device-resident coefficients of all KMAX shells (the annealing state)."""
+
+    def __init__(self, intensity, basis, nshell=None):
+        nshell = intensity.KMAX if (nshell is None and isinstance(intensity, SphericalHarmonicsVolume)) else nshell
+        c, ld = _coeff_ptr(intensity, basis, nshell)
+        nshell = c.shape[0] if nshell is None else nshell
+        h = C.c_void_p()
+        check(lib().tp_state_create(basis.ctx.h, basis.h, c.ctypes.data, ld, nshell, C.byref(h)), "tp_state_create")
+        self.h, self.basis, self.nshell, self.ld = h, basis, nshell, ld
+        self.template = c[:nshell].copy()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().tp_state_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def read(self):
+        out = self.template.copy()
+        check(lib().tp_state_read(self.basis.ctx.h, self.h, out.ctypes.data, self.ld), "tp_state_read")
+        return out
+
+    def energy(self, c3ref, K3_range=None):
+        ref = self.basis.ref(c3ref, K3_range)
+        E = C.c_double()
+        check(lib().tp_state_energy(self.basis.ctx.h, self.basis.h, ref, self.h, C.byref(E)), "tp_state_energy")
+        return E.value
+
+    def c3(self):
+        res = np.zeros((self.basis.klength, self.basis.R))
+        check(lib().tp_state_c3(self.basis.ctx.h, self.basis.h, self.h, res.ctypes.data), "tp_state_c3")
+        return res.T
+
+
+def rotate_all_at_once(state, basis, c3ref, nsteps, initial_stepsize=np.pi / 4, stepsizefactor=1.01,
+                       initial_temperature_factor=0.1, temperature_decay=0.99998, seed=0, ntemp=50,
+                       K3_range=None):
+    """Plain MC simulated annealing on the device (determination.jl:281-338)."""
+    ref = basis.ref(c3ref, K3_range)
+    p = AnnealParams(nsteps, ntemp, initial_stepsize, stepsizefactor, initial_temperature_factor,
+                     temperature_decay, seed)
+    E = np.zeros(max(nsteps, 1))
+    En = np.zeros(max(nsteps, 1))
+    acc = np.zeros(max(nsteps, 1), np.int32)
+    sc = np.zeros(4)
+    check(lib().tp_anneal(basis.ctx.h, basis.h, ref, state.h, C.byref(p), E.ctypes.data, En.ctypes.data,
+                          acc.ctypes.data, sc.ctypes.data), "tp_anneal")
+    return dict(E=E[:nsteps], E_new=En[:nsteps], accepted=acc[:nsteps], E0=sc[0], T0=sc[1], E_final=sc[2],
+                T_final=sc[3])
+
+
+def intensity_from_atoms(atoms, KMAX, qmax, LMAX, lfill, ntheta, nphi, ctx=None):
+    """SH coefficients of point-atom intensity on shells q_k = k*dq, dq = qmax/KMAX.
+
+    As in createSphericalHarmonicsStructure (structure.jl:74-134) the returned
+    volume stores qmax in its rmax field, so dq(volume) = qmax/KMAX
+    (spherical_harmonics.jl:102-104)."""
+    ctx = _ctx(ctx)
+    atoms = np.ascontiguousarray(atoms, dtype=np.float64)
+    out = np.zeros((KMAX, LMAX * LMAX), np.complex128)
+    dq = qmax / KMAX
+    check(lib().tp_intensity_from_atoms(ctx.h, atoms.ctypes.data, atoms.shape[0], KMAX, dq, LMAX, lfill, ntheta,
+                                        nphi, out.ctypes.data), "tp_intensity_from_atoms")
+    return SphericalHarmonicsVolume(out, LMAX, KMAX, qmax)

@@ -1,0 +1,102 @@
+"""Build and load libtp.so (the gfx950 C-ABI library declared in include/tp.h)."""
+import ctypes as C
+import os
+import subprocess
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))   # threephotons.jl_amd/
+REPO_ROOT = os.path.dirname(PKG_ROOT)
+LIB_PATH = os.path.join(PKG_ROOT, "libtp.so")
+SRC = os.path.join(PKG_ROOT, "csrc", "tp_hip.hip")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Wno-unused-value", "-Wno-unused-result",
+    # packed FP32 instructions are kept out of every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose=False):
+    """Compile csrc/tp_hip.hip for gfx950 into threephotons.jl_amd/libtp.so."""
+    tmp = LIB_PATH + ".tmp"
+    cmd = [HIPCC] + HIPCC_FLAGS + ["-o", tmp, SRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
+    os.replace(tmp, LIB_PATH)
+    if verbose:
+        print("built", LIB_PATH)
+    return LIB_PATH
+
+
+_lib = None
+
+
+class TPError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded libtp.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TPError(f"libtp.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    pp = C.POINTER(C.c_void_p)
+    L.tp_abi_version.restype = i32
+    L.tp_last_error.restype = C.c_char_p
+    L.tp_init.argtypes = [i32, pp]
+    L.tp_finalize.argtypes = [vp]
+    L.tp_finalize.restype = None
+    L.tp_basis_create.argtypes = [vp, i32, i32, i32, i32, dbl, dbl, i32, pp]
+    L.tp_basis_destroy.argtypes = [vp]
+    L.tp_basis_destroy.restype = None
+    L.tp_basis_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
+    L.tp_c3.argtypes = [vp, vp, vp, i64, vp]
+    L.tp_ref_create.argtypes = [vp, vp, vp, i32, i32, i32, pp]
+    L.tp_ref_destroy.argtypes = [vp]
+    L.tp_ref_destroy.restype = None
+    L.tp_energy.argtypes = [vp, vp, vp, vp, i64, i32, C.POINTER(dbl)]
+    L.tp_state_create.argtypes = [vp, vp, vp, i64, i32, pp]
+    L.tp_state_destroy.argtypes = [vp]
+    L.tp_state_destroy.restype = None
+    L.tp_state_read.argtypes = [vp, vp, vp, i64]
+    L.tp_state_energy.argtypes = [vp, vp, vp, vp, C.POINTER(dbl)]
+    L.tp_state_c3.argtypes = [vp, vp, vp, vp]
+    L.tp_anneal.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.tp_intensity_from_atoms.argtypes = [vp, vp, i32, i32, dbl, i32, i32, i32, i32, vp]
+    L.tp_set_timing.argtypes = [vp, i32]
+    L.tp_last_timings.argtypes = [vp, C.POINTER(dbl), i32]
+    _lib = L
+    return L
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().tp_last_error().decode(errors="replace")
+        raise TPError(f"{what} failed (rc={rc}): {msg}")
+
+
+class AnnealParams(C.Structure):
+    _fields_ = [
+        ("nsteps", C.c_int),
+        ("ntemp", C.c_int),
+        ("initial_stepsize", C.c_double),
+        ("stepsizefactor", C.c_double),
+        ("initial_temperature_factor", C.c_double),
+        ("temperature_decay", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+# Entry points declared in include/tp.h (checked by the CPU test suite).
+EXPORTS = [
+    "tp_abi_version", "tp_last_error", "tp_init", "tp_finalize", "tp_basis_create", "tp_basis_destroy",
+    "tp_basis_info", "tp_c3", "tp_ref_create", "tp_ref_destroy", "tp_energy", "tp_state_create",
+    "tp_state_destroy", "tp_state_read", "tp_state_energy", "tp_state_c3", "tp_anneal",
+    "tp_intensity_from_atoms", "tp_set_timing", "tp_last_timings",
+]
