@@ -313,8 +313,8 @@ __global__ __launch_bounds__(256) void k_contract(const double *__restrict__ F, 
         }
     }
 
-    // accumulator layout of v_mfma_f64_16x16x4f64: lane holds column l16,
-    // rows 4*lq + v (v = 0..3) of the 16x16 block.
+    // accumulator layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane
+    // holds column l16, rows lq + 4*v (v = 0..3) of the 16x16 block.
 #pragma unroll
     for (int j = 0; j < CB; ++j) {
         const int col = c0 + j * 16 + l16;
@@ -322,10 +322,10 @@ __global__ __launch_bounds__(256) void k_contract(const double *__restrict__ F, 
         double sp = 0.0, sq = 0.0;
 #pragma unroll
         for (int i = 0; i < RB; ++i) {
-            const int rbase = r0 + wave * WM + i * 16 + 4 * lq;
+            const int rbase = r0 + wave * WM + i * 16 + lq;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = rbase + v;
+                const int r = rbase + 4 * v;
                 const double c = acc[i][j][v];
                 if (cok && r < R) {
                     if (WRITE_C3) c3out[(int64_t)col * R + r] = c;
