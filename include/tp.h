@@ -139,10 +139,10 @@ int tp_intensity_from_atoms(tp_ctx *ctx, const double *atoms_xyz, int natoms, in
 
 /* ---- instrumentation ------------------------------------------------------ */
 
-/* Per-kernel device time (ms) of the last tp_state_energy call, measured
- * with HIP events on the context stream: [0] coefficient pack/As stage,
- * [1] contraction + energy partials, [2] final reduction.  Enabled by
- * tp_set_timing(ctx, 1). */
+/* Per-stage device time (ms) of the last tp_energy / tp_state_energy call,
+ * measured with HIP events on the context stream: [0] triple products,
+ * [1] G = H.As contraction, [2] c3 = Bm.G contraction with the energy
+ * partial sums, [3] energy finish.  Enabled by tp_set_timing(ctx, 1). */
 int tp_set_timing(tp_ctx *ctx, int on);
 int tp_last_timings(const tp_ctx *ctx, double *ms_out, int n);
 

@@ -1,0 +1,586 @@
+// tp_kernels.hpp -- gfx950 device code of libtp (included by tp_hip.hip).
+//
+// One c3 + Bayes-energy evaluation (lambda = 0) is four launches:
+//   k_triple   coupling-form triple products As[lt][kt]              (FP64, LDS)
+//   k_gemm     G  = H . As      (folded (m2,mu3) x k-triple)         (FP64 MFMA)
+//   k_gemm     c3 = Bm . G, fused Bayes partial sums per row tile    (FP64 MFMA)
+//   k_final    per-column energies, fixed-order total, optional Metropolis step
+// H[(m2,mu3), lt] = sum_{j in lt} w_j Y(l1,m1) Y(l2,m2) Y(l3,mu3) and
+// Bm[r, (m2,mu3)] = cos(m2 a_r + mu3 b_r), so Bm . H is the l-triple basis F
+// of the reference's slow path (flab, correlations.jl:337-362) and
+// Bm . H . As = B . PA (correlations.jl:182).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));   // promotable to registers, unlike HIP's double2
+
+#define TP_TAG_MOVE 0x4D4F5645u
+#define TP_TAG_ACC 0x41434350u
+#define TP_TAG_TEMP 0x54454D50u
+
+constexpr double kPiD = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals.  Streams are
+// keyed by the run seed; the counter is (j>>1, l-index, step, tag).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    }
+}
+
+__device__ __forceinline__ double u01(uint32_t lo, uint32_t hi)
+{
+    const uint64_t u = ((uint64_t)hi << 32) | lo;
+    return (double)(u >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__device__ inline double dev_normal(uint64_t seed, uint32_t step, uint32_t li, uint32_t j, uint32_t tag)
+{
+    uint32_t c[4] = {j >> 1, li, step, tag};
+    philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double a = u01(c[0], c[1]), b = u01(c[2], c[3]);
+    const double r = sqrt(-2.0 * log(1.0 - a));
+    const double ang = 2.0 * kPiD * b;
+    return (j & 1) ? r * sin(ang) : r * cos(ang);
+}
+
+__device__ inline double dev_uniform(uint64_t seed, uint32_t step, uint32_t tag)
+{
+    uint32_t c[4] = {0u, 0xFFFFFFFFu, step, tag};
+    philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    return u01(c[0], c[1]);
+}
+
+// Double-buffered coefficients: buffer b1 is used when (*sel ^ flip) is set.
+__device__ __forceinline__ const double2 *pick(const double2 *b0, const double2 *b1, const int *sel, int flip)
+{
+    const int s = (sel ? sel[0] : 0) ^ flip;
+    return s ? b1 : b0;
+}
+
+// Gather the compact (even l <= L) coefficients out of LMAX^2-strided vectors.
+__global__ void k_pack(const double2 *__restrict__ raw, int64_t ld, int nshell, const int *__restrict__ sean0,
+                       int n_c, double2 *__restrict__ coef)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nshell * n_c) return;
+    const int k = (int)(i / n_c), c = (int)(i % n_c);
+    coef[i] = raw[(int64_t)k * ld + sean0[c]];
+}
+
+// ---------------------------------------------------------------------------
+// Triple products in coupling form.  Workgroup = one chunk of consecutive
+// l-triples (its (lt,mu3) groups g0..g1 and their weights, staged in LDS);
+// all K shells' coefficients sit in LDS and the workgroup walks every k-pair
+// (k1 >= k2) in batches of PB:
+//   stage 1: T[p][lt,mu3] = sum_{m1} w(l1 m1 l2 m2 l3 mu3) c_k1[l1,m1] c_k2[l2,m2],
+//            m2 = -m1-mu3 (one thread per (pair, group));
+//   stage 2: As[lt][kt(k1,k2,k3)] = Re sum_mu3 T[p][lt,mu3] c_k3[l3,mu3], k3 <= k2
+//            (one thread per (pair, lt, k3), k3 fastest: As rows are written
+//            contiguously).
+// This equals sum_{j in lt} w_j Re(c_k1[i1] c_k2[i2] c_k3[i3])
+// (correlations.jl:165-174, cuda_kernel.cu:30-37) regrouped by mu3.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_triple(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
+                                                 const int *__restrict__ sel, int flip, int n_c, int K,
+                                                 const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
+                                                 const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
+                                                 const int4 *__restrict__ chunks, const int *__restrict__ pairs,
+                                                 int npairs, int PB, int Gmax, int Emax, double *__restrict__ As,
+                                                 int64_t kl_pad)
+{
+    extern __shared__ double2 sm[];
+    const double2 *coef = pick(cb0, cb1, sel, flip);
+    double2 *sc = sm;                                          // [K][n_c]
+    double2 *T = sc + (size_t)K * n_c;                         // [PB][G]
+    double *sw = reinterpret_cast<double *>(T + (size_t)PB * Gmax);   // [Emax]
+    uint32_t *si = reinterpret_cast<uint32_t *>(sw + Emax);            // [Emax]
+    int *sg = reinterpret_cast<int *>(si + Emax);                      // [Gmax + 1]
+    const int4 cw = chunks[blockIdx.x];
+    const int lt0 = cw.x, lt1 = cw.y, g0 = cw.z, g1 = cw.w;
+    const int G = g1 - g0, nlt = lt1 - lt0;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int e0 = gstart[g0], ne = gstart[g1] - e0;
+    for (int i = tid; i < K * n_c; i += bs) sc[i] = coef[i];
+    for (int i = tid; i < ne; i += bs) {
+        sw[i] = ew[e0 + i];
+        si[i] = eidx[e0 + i];
+    }
+    for (int i = tid; i <= G; i += bs) sg[i] = gstart[g0 + i] - e0;
+    __syncthreads();
+    for (int pb = 0; pb < npairs; pb += PB) {
+        const int nb = min(PB, npairs - pb);
+        for (int idx = tid; idx < nb * G; idx += bs) {
+            const int q = idx / G, gl = idx - q * G;
+            const int pk = pairs[pb + q], k1 = pk & 0xffff, k2 = pk >> 16;
+            const double2 *c1 = sc + (k1 - 1) * n_c, *c2 = sc + (k2 - 1) * n_c;
+            const int s = sg[gl], e = sg[gl + 1];
+            double tr0 = 0.0, ti0 = 0.0, tr1 = 0.0, ti1 = 0.0;
+            int j = s;
+            for (; j + 1 < e; j += 2) {
+                const uint32_t ia = si[j], ib = si[j + 1];
+                const double wa = sw[j], wb = sw[j + 1];
+                const double2 a0 = c1[ia & 0xffffu], b0 = c2[ia >> 16];
+                const double2 a1 = c1[ib & 0xffffu], b1 = c2[ib >> 16];
+                tr0 += wa * (a0.x * b0.x - a0.y * b0.y);
+                ti0 += wa * (a0.x * b0.y + a0.y * b0.x);
+                tr1 += wb * (a1.x * b1.x - a1.y * b1.y);
+                ti1 += wb * (a1.x * b1.y + a1.y * b1.x);
+            }
+            if (j < e) {
+                const uint32_t ia = si[j];
+                const double wa = sw[j];
+                const double2 a0 = c1[ia & 0xffffu], b0 = c2[ia >> 16];
+                tr0 += wa * (a0.x * b0.x - a0.y * b0.y);
+                ti0 += wa * (a0.x * b0.y + a0.y * b0.x);
+            }
+            T[idx] = make_double2(tr0 + tr1, ti0 + ti1);
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nb * nlt * K; idx += bs) {
+            const int k3 = idx % K + 1, rest = idx / K;
+            const int q = rest / nlt, lt = lt0 + (rest - q * nlt);
+            const int pk = pairs[pb + q], k1 = pk & 0xffff, k2 = pk >> 16;
+            if (k3 > k2) continue;
+            const int4 li = ltinfo[lt];
+            const double2 *Tl = T + q * G + (li.y - g0);
+            const double2 *c3 = sc + (k3 - 1) * n_c + li.x;
+            const int n = 2 * li.z + 1;
+            double s0 = 0.0, s1 = 0.0;
+            int m = 0;
+            for (; m + 1 < n; m += 2) {
+                const double2 t0 = Tl[m], c0 = c3[m], t1 = Tl[m + 1], c1 = c3[m + 1];
+                s0 += t0.x * c0.x - t0.y * c0.y;
+                s1 += t1.x * c1.x - t1.y * c1.y;
+            }
+            if (m < n) {
+                const double2 t0 = Tl[m], c0 = c3[m];
+                s0 += t0.x * c0.x - t0.y * c0.y;
+            }
+            const int64_t kt = (int64_t)(k1 - 1) * k1 * (k1 + 1) / 6 + (int64_t)(k2 - 1) * k2 / 2 + (k3 - 1);
+            As[(int64_t)lt * kl_pad + kt] = s0 + s1;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA GEMM  C[M x N] = A[M x K] . B[K x N]  (v_mfma_f64_16x16x4f64).
+// A is stored K-major (A[k*lda + m]), B as B[k*ldb + n]; K is a multiple of
+// 16 and the padded operands hold zeros.  A BM x BN tile per workgroup of 4
+// waves arranged WGM x (4/WGM); K staged through LDS 16 at a time with the
+// next slice prefetched into registers while the MFMAs of the current one
+// run.  Workgroups are mapped so that each XCD (block id mod 8) owns
+// consecutive row tiles, whose A rows then stay in that XCD's L2.
+//   OUT = 1: C written column-major out[col*ldo + r] for r < M, col < ncols
+//            (the c3raw layout of return_raw, correlations.jl:182)
+//   OUT = 2: the full padded tile written row-major out[r*ldo + col]
+//   ENERGY : per column, the partial sums over this tile's rows of
+//            p = max(c,1e-30) and qhat*log(p) (correlations.jl:203-220) go to
+//            part[(rt*2+{0,1})*ldp + col]; requires WGM = 4.
+// ---------------------------------------------------------------------------
+template <int BM, int BN, int WGM, int OUT, bool ENERGY>
+__global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int64_t lda,
+                                              const double *__restrict__ B, int64_t ldb, int Kdim, int M, int ncols,
+                                              int n_rt, int n_ct, double *__restrict__ out, int64_t ldo,
+                                              const double *__restrict__ qhat, double *__restrict__ part, int64_t ldp)
+{
+    constexpr int BK = 16;
+    constexpr int WGN = 4 / WGM;
+    constexpr int WTM = BM / WGM, WTN = BN / WGN;
+    constexpr int RB = WTM / 16, CB = WTN / 16;
+    constexpr int A2 = BK * BM / 2 / 256;
+    constexpr int B2 = BK * BN / 2 / 256;
+    static_assert(A2 >= 1 && B2 >= 1 && RB >= 1 && CB >= 1, "tile too small");
+    static_assert(!ENERGY || WGM == 4, "energy epilogue needs all waves on the same columns");
+    __shared__ double sA[BK][BM];
+    __shared__ double sB[BK][BN];
+    __shared__ double red[ENERGY ? 4 : 1][2][BN];
+
+    const int nblk = n_rt * n_ct;
+    int lin = blockIdx.x;
+    if ((nblk & 7) == 0) lin = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+    const int rt = lin / n_ct, ct = lin % n_ct;
+    const int r0 = rt * BM, c0 = ct * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WGM, wn = wave / WGM;
+    const int l16 = lane & 15, lq = lane >> 4;
+
+    dbl4 acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    dbl2 ra[A2], rb[B2];
+#pragma unroll
+    for (int q = 0; q < A2; ++q) {
+        const int i = tid + 256 * q, row = i / (BM / 2), c2 = i % (BM / 2);
+        ra[q] = *reinterpret_cast<const dbl2 *>(A + (int64_t)row * lda + r0 + 2 * c2);
+    }
+#pragma unroll
+    for (int q = 0; q < B2; ++q) {
+        const int i = tid + 256 * q, row = i / (BN / 2), c2 = i % (BN / 2);
+        rb[q] = *reinterpret_cast<const dbl2 *>(B + (int64_t)row * ldb + c0 + 2 * c2);
+    }
+    for (int k0 = 0; k0 < Kdim; k0 += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < A2; ++q) {
+            const int i = tid + 256 * q, row = i / (BM / 2), c2 = i % (BM / 2);
+            *reinterpret_cast<dbl2 *>(&sA[row][2 * c2]) = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < B2; ++q) {
+            const int i = tid + 256 * q, row = i / (BN / 2), c2 = i % (BN / 2);
+            *reinterpret_cast<dbl2 *>(&sB[row][2 * c2]) = rb[q];
+        }
+        __syncthreads();
+        if (k0 + BK < Kdim) {
+#pragma unroll
+            for (int q = 0; q < A2; ++q) {
+                const int i = tid + 256 * q, row = i / (BM / 2), c2 = i % (BM / 2);
+                ra[q] = *reinterpret_cast<const dbl2 *>(A + (int64_t)(k0 + BK + row) * lda + r0 + 2 * c2);
+            }
+#pragma unroll
+            for (int q = 0; q < B2; ++q) {
+                const int i = tid + 256 * q, row = i / (BN / 2), c2 = i % (BN / 2);
+                rb[q] = *reinterpret_cast<const dbl2 *>(B + (int64_t)(k0 + BK + row) * ldb + c0 + 2 * c2);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            double a[RB], b[CB];
+#pragma unroll
+            for (int i = 0; i < RB; ++i) a[i] = sA[kk * 4 + lq][wm * WTM + i * 16 + l16];
+#pragma unroll
+            for (int j = 0; j < CB; ++j) b[j] = sB[kk * 4 + lq][wn * WTN + j * 16 + l16];
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+
+    // accumulator layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane
+    // holds column l16, rows lq + 4*v (v = 0..3) of the 16x16 block.
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+        const int col = c0 + wn * WTN + j * 16 + l16;
+        const bool cok = col < ncols;
+        double sp = 0.0, sq = 0.0;
+#pragma unroll
+        for (int i = 0; i < RB; ++i) {
+            const int rbase = r0 + wm * WTM + i * 16 + lq;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rbase + 4 * v;
+                const double c = acc[i][j][v];
+                if (OUT == 2) out[(int64_t)r * ldo + col] = c;
+                if (cok && r < M) {
+                    if (OUT == 1) out[(int64_t)col * ldo + r] = c;
+                    if (ENERGY) {
+                        const double pv = c > 1e-30 ? c : 1e-30;
+                        const double q = qhat[(int64_t)col * M + r];
+                        sp += pv;
+                        sq += q * log(pv);
+                    }
+                }
+            }
+        }
+        if (ENERGY) {
+            sp += __shfl_xor(sp, 16);
+            sq += __shfl_xor(sq, 16);
+            sp += __shfl_xor(sp, 32);
+            sq += __shfl_xor(sq, 32);
+            if (lq == 0) {
+                red[wave][0][j * 16 + l16] = sp;
+                red[wave][1][j * 16 + l16] = sq;
+            }
+        }
+    }
+    if (ENERGY) {
+        __syncthreads();
+        if (tid < BN) {
+            const double sp = ((red[0][0][tid] + red[1][0][tid]) + red[2][0][tid]) + red[3][0][tid];
+            const double sq = ((red[0][1][tid] + red[1][1][tid]) + red[2][1][tid]) + red[3][1][tid];
+            part[((int64_t)rt * 2 + 0) * ldp + c0 + tid] = sp;
+            part[((int64_t)rt * 2 + 1) * ldp + c0 + tid] = sq;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Energy finish and the annealing step.
+//   E_kt = tF * (qsum * log(S) - Q),  S = sum_rt part0, Q = sum_rt part1,
+// summed per block in a fixed tree, block totals summed in block order by the
+// last block to arrive (one agent-scope fence and atomic per block), so E is
+// bitwise reproducible.  The counter is back at zero at kernel end.
+// mode 0: E -> Eout;  mode 1: also E -> temp[step] (temperature probe);
+// mode 2: also scal[0] = E (initial energy);
+// mode 3: Metropolis step (evaluate_step, determination.jl:491-515, with
+//         T *= decay first, rotate_all_at_once :328): on acceptance the step
+//         sizes grow by `factor`, E is replaced and the proposal buffer
+//         becomes current (*sel ^= 1); otherwise step sizes shrink.
+// ---------------------------------------------------------------------------
+struct McArgs {
+    int mode;
+    uint32_t step;
+    uint64_t seed;
+    double decay, factor;
+    int nl;
+    double *scal;        // {E, T}
+    double *stepsizes;   // [nl]
+    int *sel;            // current buffer index
+    double *trE, *trEn;  // [nsteps]
+    int *trA;            // [nsteps]
+    double *temp;        // [ntemp]
+};
+
+__global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, int n_rt, int64_t ldp, int ncols,
+                                               const double *__restrict__ qsum, const double *__restrict__ tf,
+                                               double *__restrict__ blockpart, unsigned *__restrict__ counter,
+                                               double *__restrict__ Eout, McArgs mc)
+{
+    __shared__ double red[256];
+    __shared__ int s_last;
+    const int tid = threadIdx.x;
+    const int kt = blockIdx.x * 256 + tid;
+    double e = 0.0;
+    if (kt < ncols) {
+        double S = 0.0, Q = 0.0;
+        for (int q = 0; q < n_rt; ++q) {
+            S += part[((int64_t)q * 2 + 0) * ldp + kt];
+            Q += part[((int64_t)q * 2 + 1) * ldp + kt];
+        }
+        e = tf[kt] * (qsum[kt] * log(S) - Q);
+    }
+    red[tid] = e;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) red[tid] += red[tid + s];
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    blockpart[blockIdx.x] = red[0];
+    __threadfence();
+    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev != gridDim.x - 1) return;
+    __threadfence();
+    double E = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b) E += blockpart[b];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    Eout[0] = E;
+    if (mc.mode == 1) {
+        mc.temp[mc.step] = E;
+    } else if (mc.mode == 2) {
+        mc.scal[0] = E;
+    } else if (mc.mode == 3) {
+        const double T = mc.scal[1] * mc.decay;
+        mc.scal[1] = T;
+        const double dE = E - mc.scal[0];
+        const double ap = exp(-dE / T);
+        const double u = dev_uniform(mc.seed, mc.step, TP_TAG_ACC);
+        const int acc = (dE < 0.0 || (dE > 0.0 && u < ap)) ? 1 : 0;
+        if (acc) {
+            for (int i = 0; i < mc.nl; ++i) mc.stepsizes[i] *= mc.factor;
+            mc.scal[0] = E;
+            mc.sel[0] ^= 1;
+        } else {
+            for (int i = 0; i < mc.nl; ++i) mc.stepsizes[i] /= mc.factor;
+        }
+        mc.trE[mc.step - 1] = mc.scal[0];
+        mc.trEn[mc.step - 1] = E;
+        mc.trA[mc.step - 1] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One Monte Carlo proposal (get_MC_step, determination.jl:232-241): for each
+// even l in 2..L (one workgroup per l) a rotation by beta = stepsize_l * z in
+// the plane of a random orthonormal 2-frame (u,v) of R^(2l+1),
+//   R = I + (cos b - 1)(u u^T + v v^T) + sin b (v u^T - u v^T),
+// which is inv(C) G(beta) C of random_rotation_step (utilities.jl:180-190)
+// written through the two rows of C that G touches.  It acts on the complex
+// vectors through the real basis, x <- ctr^H R ctr x, with ctr = Umat(l)
+// (spherical_harmonics.jl:445-457) in its two-entries-per-row form.  Lane i
+// of a wave owns entry m = i - l of one shell; the four waves share the
+// shells.  The current buffer is read, the other one written; workgroup 0
+// also copies the l = 0 block.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_propose(double2 *__restrict__ b0, double2 *__restrict__ b1,
+                                                 const int *__restrict__ sel, int nshell, int n_c,
+                                                 const int *__restrict__ coff_l,
+                                                 const double *__restrict__ stepsizes, uint64_t seed,
+                                                 uint32_t step, uint32_t tag)
+{
+    __shared__ double uv[2][64];
+    __shared__ double sbeta;
+    const int s = sel ? sel[0] : 0;
+    const double2 *cur = s ? b1 : b0;
+    double2 *prop = s ? b0 : b1;
+    const int li = blockIdx.x, l = 2 * (li + 1), n = 2 * l + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (wave == 0) {
+        // the frame: Gram-Schmidt of two Gaussian vectors, lane i = entry i
+        const double g1 = lane < n ? dev_normal(seed, step, (uint32_t)li, (uint32_t)lane, tag) : 0.0;
+        const double g2 = lane < n ? dev_normal(seed, step, (uint32_t)li, (uint32_t)(n + lane), tag) : 0.0;
+        const double u = g1 / sqrt(wave_sum(g1 * g1));
+        const double d = wave_sum(u * g2);
+        const double vt = g2 - d * u;
+        const double v = vt / sqrt(wave_sum(vt * vt));
+        uv[0][lane] = u;
+        uv[1][lane] = v;
+        if (lane == 0) sbeta = stepsizes[li] * dev_normal(seed, step, (uint32_t)li, (uint32_t)(2 * n), tag);
+    }
+    __syncthreads();
+    const double cm = cos(sbeta) - 1.0, sb = sin(sbeta);
+    const double h = 0.70710678118654752440;
+    const int off = coff_l[l];
+    const double u = lane < n ? uv[0][lane] : 0.0, v = lane < n ? uv[1][lane] : 0.0;
+    const int m = lane - l;                      // this lane's order m (valid when lane < n)
+    const int am = m < 0 ? -m : m;
+    const double sg = (am & 1) ? -1.0 : 1.0;
+    for (int k = wave; k < nshell; k += 4) {
+        if (li == 0 && lane == 0) prop[(int64_t)k * n_c + coff_l[0]] = cur[(int64_t)k * n_c + coff_l[0]];
+        const double2 *x = cur + (int64_t)k * n_c + off;
+        double2 *xo = prop + (int64_t)k * n_c + off;
+        double2 xs = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
+        if (lane < n) {
+            xs = x[lane];            // x_m
+            xm = x[2 * l - lane];    // x_{-m}
+        }
+        // y = ctr x:  y_0 = x_0; y_u = (x_u + (-1)^u x_{-u})/sqrt2 (u > 0);
+        //             y_{-u} = (i (-1)^u x_{-u} - i x_u)/sqrt2 (u > 0)
+        double yr, yi;
+        if (m == 0) {
+            yr = xs.x; yi = xs.y;
+        } else if (m > 0) {
+            yr = h * (xs.x + sg * xm.x); yi = h * (xs.y + sg * xm.y);
+        } else {   // here xs = x_{-u}, xm = x_u with u = -m
+            yr = h * (-sg * xs.y + xm.y); yi = h * (sg * xs.x - xm.x);
+        }
+        if (lane >= n) { yr = 0.0; yi = 0.0; }
+        const double ar = wave_sum(u * yr), ai = wave_sum(u * yi);
+        const double br = wave_sum(v * yr), bi = wave_sum(v * yi);
+        yr += cm * (u * ar + v * br) + sb * (v * ar - u * br);
+        yi += cm * (u * ai + v * bi) + sb * (v * ai - u * bi);
+        // x = ctr^H y:  x_m = (y_m + i y_{-m})/sqrt2 ; x_{-m} = (-1)^m (y_m - i y_{-m})/sqrt2
+        const int mir = (lane < n) ? (2 * l - lane) : lane;
+        const double zr = __shfl(yr, mir), zi = __shfl(yi, mir);   // y at -m
+        if (lane < n) {
+            double orr, oi;
+            if (m == 0) {
+                orr = yr; oi = yi;
+            } else if (m > 0) {
+                orr = h * (yr - zi); oi = h * (yi + zr);
+            } else {   // own y is y_{-|m|}, z is y_{|m|}
+                orr = h * sg * (zr + yi); oi = h * sg * (zi - yr);
+            }
+            xo[lane] = make_double2(orr, oi);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic intensity of point atoms (input generator, not the hot path).
+// ---------------------------------------------------------------------------
+// I(k,a,b) = |sum_j exp(i q_k n(a,b).r_j)|^2 on the quadrature grid.
+__global__ __launch_bounds__(256) void k_intensity_grid(const double *__restrict__ atoms, int natoms, int KMAX,
+                                                        double dq, const double *__restrict__ gx, int ntheta,
+                                                        int nphi, double *__restrict__ I)
+{
+    __shared__ double sat[3 * 256];
+    const int64_t npts = (int64_t)KMAX * ntheta * nphi;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double qx = 0, qy = 0, qz = 0;
+    if (i < npts) {
+        const int k = (int)(i / ((int64_t)ntheta * nphi)) + 1;
+        const int a = (int)((i / nphi) % ntheta), b = (int)(i % nphi);
+        const double ct = gx[a], st = sqrt((1.0 - ct) * (1.0 + ct));
+        const double ph = 2.0 * kPiD * b / nphi;
+        const double q = k * dq;
+        qx = q * st * cos(ph); qy = q * st * sin(ph); qz = q * ct;
+    }
+    double re = 0.0, im = 0.0;
+    for (int j0 = 0; j0 < natoms; j0 += 256) {
+        const int nj = min(256, natoms - j0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < 3 * nj; t += blockDim.x) sat[t] = atoms[3 * j0 + t];
+        __syncthreads();
+        for (int j = 0; j < nj; ++j) {
+            double s, c;
+            sincos(qx * sat[3 * j] + qy * sat[3 * j + 1] + qz * sat[3 * j + 2], &s, &c);
+            re += c;
+            im += s;
+        }
+    }
+    if (i < npts) I[i] = re * re + im * im;
+}
+
+// G(k,a,m) = sum_b I(k,a,b) exp(-i m phi_b), m = 0..lfill
+__global__ void k_intensity_phi(const double *__restrict__ I, int KMAX, int ntheta, int nphi, int lfill,
+                                double2 *__restrict__ G)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)KMAX * ntheta * (lfill + 1);
+    if (i >= n) return;
+    const int m = (int)(i % (lfill + 1));
+    const int64_t ka = i / (lfill + 1);
+    const double *row = I + ka * nphi;
+    double re = 0, im = 0;
+    for (int b = 0; b < nphi; ++b) {
+        double s, c;
+        sincospi(2.0 * (double)((int64_t)m * b % nphi) / nphi, &s, &c);
+        re += row[b] * c;
+        im -= row[b] * s;
+    }
+    G[i] = make_double2(re, im);
+}
+
+// c_lm(k) = (2pi/nphi) sum_a w_a sphPlm(l,m,theta_a) G(k,a,m) (m<0 by symmetry)
+__global__ void k_intensity_coeff(const double2 *__restrict__ G, const double *__restrict__ P,
+                                  const double *__restrict__ gw, int KMAX, int ntheta, int nphi, int lfill, int LMAX,
+                                  const int *__restrict__ sean_lm, double2 *__restrict__ coeff)
+{
+    const int nlm = (lfill + 1) * (lfill + 1);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)KMAX * nlm) return;
+    const int k = (int)(i / nlm), q = (int)(i % nlm);
+    const int l = (int)sqrt((double)q), m = q - l * l - l;
+    const int am = m < 0 ? -m : m;
+    double re = 0, im = 0;
+    for (int a = 0; a < ntheta; ++a) {
+        const double y = P[((int64_t)a * (lfill + 1) + l) * (lfill + 1) + am];
+        const double2 g = G[((int64_t)k * ntheta + a) * (lfill + 1) + am];
+        re += gw[a] * y * g.x;
+        im += gw[a] * y * g.y;
+    }
+    const double sc = 2.0 * kPiD / nphi;
+    re *= sc;
+    im *= sc;
+    if (m < 0) {   // c_{l,-|m|} = (-1)^m conj(c_{l,|m|}) for a real function
+        im = -im;
+        if (am & 1) { re = -re; im = -im; }
+    }
+    coeff[(int64_t)k * LMAX * LMAX + sean_lm[q]] = make_double2(re, im);
+}

@@ -92,8 +92,8 @@ class HipContext:
         check(lib().tp_set_timing(self.h, int(on)), "tp_set_timing")
 
     def last_timings(self):
-        out = (C.c_double * 3)()
-        check(lib().tp_last_timings(self.h, out, 3), "tp_last_timings")
+        out = (C.c_double * 4)()
+        check(lib().tp_last_timings(self.h, out, 4), "tp_last_timings")
         return list(out)
 
 
