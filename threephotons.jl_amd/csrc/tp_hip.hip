@@ -60,6 +60,7 @@ struct tp_ctx {
 constexpr int G1_BM = 64, G1_BN = 32, G1_WGM = 2;   // G = H . As
 constexpr int G2_BM = 64, G2_BN = 64, G2_WGM = 4;   // c3 = Bm . G (+ energy)
 constexpr int KL_ALIGN = 64;                        // column padding (multiple of G1_BN and G2_BN)
+constexpr int kMfmaMaxK = 27;                       // k_triple_mfma: at most 16 row blocks of pairs per half
 
 struct tp_basis {
     tp_ctx *ctx = nullptr;
@@ -74,6 +75,11 @@ struct tp_basis {
     int n_rt = 0, n_ct = 0, g1_rt = 0, g1_ct = 0, n_fb = 0;
     int n_groups = 0, nchunks = 0, Gmax = 0, Emax = 0, npairs = 0, PB = 0;
     size_t triple_lds = 0;
+    // MFMA triple products (K <= kMfmaMaxK): work list of (lt, k1 half)
+    bool use_mfma = false;
+    int nwork = 0, Lw = 0, EmaxLt = 0, Prows = 0;
+    size_t mfma_lds = 0;
+    int *d_work = nullptr;         // [nwork] lt | half << 16, most expensive first
     // device tables
     double *d_H = nullptr;         // [nlt_pad][nm_pad64]   H[lt][class]
     double *d_Bm = nullptr;        // [nm_pad16][R_pad]     Bm[class][r]
@@ -156,9 +162,15 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     const bool timing = ctx->timing;
     hipStream_t st = ctx->stream;
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    hipLaunchKernelGGL(k_triple, dim3((unsigned)b->nchunks), dim3(1024), b->triple_lds, st, b0, b1, sel, flip, b->n_c,
-                       b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_chunks, b->d_pairs, b->npairs, b->PB,
-                       b->Gmax, b->Emax, b->d_As, b->kl_pad);
+    if (b->use_mfma) {
+        hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(512), b->mfma_lds, st, b0, b1, sel, flip,
+                           b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_work, b->Lw, b->EmaxLt,
+                           b->Prows, b->d_As, b->kl_pad);
+    } else {
+        hipLaunchKernelGGL(k_triple, dim3((unsigned)b->nchunks), dim3(1024), b->triple_lds, st, b0, b1, sel, flip,
+                           b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_chunks, b->d_pairs,
+                           b->npairs, b->PB, b->Gmax, b->Emax, b->d_As, b->kl_pad);
+    }
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
@@ -276,7 +288,7 @@ void tp_basis_destroy(tp_basis *b)
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
     void *ptrs[] = {b->d_H,   b->d_Bm,  b->d_ew,   b->d_eidx,    b->d_gstart, b->d_ltinfo, b->d_chunks,
                     b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_blk,
-                    b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3};
+                    b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -363,7 +375,7 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
                     }
                 if (cnt == 0) continue;
                 lt_goff.push_back((int)gstart.size());
-                ltinfo.push_back(make_int4(b->coff[l3], (int)gstart.size(), l3, 0));
+                ltinfo.push_back(make_int4(b->coff[l3], (int)gstart.size(), l3, l1 | (l2 << 8)));
                 for (auto &gv : groups) {
                     gstart.push_back((int)ew.size());
                     for (auto &e : gv) {
@@ -440,6 +452,43 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
                                       (int)b->triple_lds);
     }
 
+    // MFMA triple products: one workgroup per (l-triple, half of the k1
+    // range), most expensive first
+    std::vector<int> work;
+    if (K <= kMfmaMaxK) {
+        b->Lw = 2 * L + 1;
+        b->EmaxLt = 0;
+        for (int t = 0; t < nlt; ++t) b->EmaxLt = std::max(b->EmaxLt, lt_entries[t]);
+        const int nh = K > 16 ? 2 : 1;
+        int maxrb = 0;
+        std::vector<std::pair<double, int>> wl;
+        for (int h = 0; h < nh; ++h) {
+            const int k1lo = 16 * h + 1, k1hi = std::min(16 * h + 16, K);
+            const int P = k1hi * (k1hi + 1) / 2 - (k1lo - 1) * k1lo / 2;
+            const int nrb = (P + 15) / 16;
+            maxrb = std::max(maxrb, nrb);
+            for (int t = 0; t < nlt; ++t) {
+                const int n3 = 2 * ltinfo[t].z + 1;
+                const double cost = 2.0 * lt_entries[t] + 8.0 * ((n3 + 7) / 8) * nrb;
+                wl.push_back({-cost, t | (h << 16)});
+            }
+        }
+        std::sort(wl.begin(), wl.end());
+        for (auto &w : wl) work.push_back(w.second);
+        b->nwork = (int)work.size();
+        b->Prows = 16 * maxrb;
+        b->mfma_lds = sizeof(double2) * 3 * (size_t)K * b->Lw + sizeof(double) * 2 * (size_t)b->Prows * 16 +
+                      (sizeof(double) + sizeof(uint32_t)) * (size_t)b->EmaxLt + sizeof(int) * ((size_t)b->Lw + 1) +
+                      sizeof(int) * (size_t)b->Prows + 64;
+        int maxlds = 0;
+        if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
+            maxlds = 65536;
+        b->use_mfma = b->mfma_lds <= (size_t)maxlds && maxrb <= 16;
+        if (b->use_mfma && b->mfma_lds > 65536)
+            (void)hipFuncSetAttribute((const void *)k_triple_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->mfma_lds);
+    }
+
     // padded dims; H and Bm
     b->R_pad = round_up(b->R, G2_BM);
     b->nlt_pad = round_up(nlt, 16);
@@ -473,7 +522,8 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
     };
     if ((rc = up(&b->d_H, H)) || (rc = up(&b->d_Bm, Bm)) || (rc = up(&b->d_ew, ew)) || (rc = up(&b->d_eidx, eidx)) ||
         (rc = up(&b->d_gstart, gstart)) || (rc = up(&b->d_ltinfo, ltinfo)) || (rc = up(&b->d_chunks, chunks)) ||
-        (rc = up(&b->d_pairs, pairs)) || (rc = up(&b->d_sean0, b->sean0)) || (rc = up(&b->d_coff, b->coff))) {
+        (rc = up(&b->d_pairs, pairs)) || (rc = up(&b->d_sean0, b->sean0)) || (rc = up(&b->d_coff, b->coff)) ||
+        (b->use_mfma && (rc = up(&b->d_work, work)))) {
         tp_basis_destroy(b);
         return rc;
     }

@@ -176,6 +176,182 @@ __global__ __launch_bounds__(1024) void k_triple(const double2 *__restrict__ cb0
 }
 
 // ---------------------------------------------------------------------------
+// Triple products on FP64 MFMA (K <= 28).  Workgroup = (l-triple lt, half h
+// of the k1 range: k1 in 16h+1..16h+16), 8 waves.  The l1, l2, l3 blocks of
+// all K shells sit in LDS.  Iteration t handles mu3 = 8t..8t+7, one per wave:
+//   stage 1: [Tr | Ti](k1, k2) = A'.B' over the group's E entries, where
+//            A'[k1][e] = w_e Re c_k1[i1_e], A'[k1][E+e] = w_e Im c_k1[i1_e],
+//            B'[e] = [Re c_k2[i2_e] | Im c_k2[i2_e]],
+//            B'[E+e] = [-Im c_k2[i2_e] | Re c_k2[i2_e]]
+//            (complex T = sum_e a_e b_e as one real GEMM with K-dim 2E);
+//            the (k1 >= k2) entries go to Tf[pair][slot] in LDS, slot =
+//            (mu3 - 8t, re/im);
+//   stage 2: As[pair][k3] += sum_slot Tf[pair][slot] . [Re c_k3[l3,mu3] ; -Im c_k3[l3,mu3]],
+//            accumulated across all iterations in MFMA registers.
+// Tf is double-buffered, so one barrier per iteration suffices.  Same sums
+// as k_triple below, regrouped for MFMA.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
+                                                     const int *__restrict__ sel, int flip, int n_c, int K,
+                                                     const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
+                                                     const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
+                                                     const int *__restrict__ work, int Lw, int Emax, int Prows,
+                                                     double *__restrict__ As, int64_t kl_pad)
+{
+    extern __shared__ double2 sm[];
+    const double2 *coef = pick(cb0, cb1, sel, flip);
+    const int wk = work[blockIdx.x];
+    const int lt = wk & 0xffff, h = wk >> 16;
+    const int4 li = ltinfo[lt];
+    const int l3 = li.z, l1 = li.w & 0xff, l2 = (li.w >> 8) & 0xff;
+    const int n1 = 2 * l1 + 1, n2 = 2 * l2 + 1, n3 = 2 * l3 + 1;
+    const int goff = li.y;
+    // LDS carve-up
+    double2 *c1b = sm;                                   // [K][n1]
+    double2 *c2b = c1b + (size_t)K * Lw;                 // [K][n2]
+    double2 *c3b = c2b + (size_t)K * Lw;                 // [K][n3]
+    double *Tf = reinterpret_cast<double *>(c3b + (size_t)K * Lw);   // [2][Prows][16]
+    double *sw = Tf + 2 * (size_t)Prows * 16;                        // [Emax]
+    uint32_t *si = reinterpret_cast<uint32_t *>(sw + Emax);          // [Emax]
+    int *sg = reinterpret_cast<int *>(si + Emax);                    // [n3 + 1]
+    int *pk = sg + (n3 + 1);                                         // [Prows]
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int co1 = l1 * (l1 - 1) / 2, co2 = l2 * (l2 - 1) / 2, co3 = l3 * (l3 - 1) / 2;   // compact offsets (even l)
+    for (int i = tid; i < K * n1; i += 512) c1b[i] = coef[(size_t)(i / n1) * n_c + co1 + i % n1];
+    for (int i = tid; i < K * n2; i += 512) c2b[i] = coef[(size_t)(i / n2) * n_c + co2 + i % n2];
+    for (int i = tid; i < K * n3; i += 512) c3b[i] = coef[(size_t)(i / n3) * n_c + co3 + i % n3];
+    const int e0 = gstart[goff], ne = gstart[goff + n3] - e0;
+    for (int i = tid; i < ne; i += 512) {
+        sw[i] = ew[e0 + i];
+        const uint32_t id = eidx[e0 + i];
+        si[i] = (uint32_t)((int)(id & 0xffffu) - co1) | ((uint32_t)((int)(id >> 16) - co2) << 16);
+    }
+    for (int i = tid; i <= n3; i += 512) sg[i] = gstart[goff + i] - e0;
+    // pairs (k1 >= k2) of this half, in the reference order, rows of Tf
+    const int k1lo = 16 * h + 1, k1hi = min(16 * h + 16, K);
+    const int off_h = (k1lo - 1) * k1lo / 2;                     // pairs with k1 < k1lo
+    const int P = k1hi * (k1hi + 1) / 2 - off_h;
+    const int nrb = (P + 15) / 16;
+    for (int p = tid; p < nrb * 16; p += 512) {
+        if (p < P) {
+            const int g = p + off_h;
+            int k1 = (int)((1.0 + sqrt(1.0 + 8.0 * g)) * 0.5);
+            while (k1 * (k1 - 1) / 2 > g) --k1;
+            while ((k1 + 1) * k1 / 2 <= g) ++k1;
+            pk[p] = k1 | ((g - k1 * (k1 - 1) / 2 + 1) << 8);
+        } else {
+            pk[p] = 0;
+        }
+    }
+    for (int i = tid; i < 2 * nrb * 16 * 16; i += 512) Tf[i] = 0.0;
+    __syncthreads();
+
+    dbl4 acc2[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc2[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int niter = (n3 + 7) / 8;
+    const int k1s = 16 * h + l16 + 1;   // stage-1 A row of this lane
+    for (int t = 0; t < niter; ++t) {
+        double *T = Tf + (size_t)(t & 1) * Prows * 16;
+        const int mu = 8 * t + wave;     // index mu3 + l3 of this wave's group
+        if (mu < n3) {
+            const int s = sg[mu], E = sg[mu + 1] - s;
+            dbl4 a1[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a1[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < (2 * E + 3) / 4; ++ks) {
+                const int ep = 4 * ks + lq;
+                double av = 0.0, br0 = 0.0, bi0 = 0.0, br1 = 0.0, bi1 = 0.0;
+                if (ep < 2 * E) {
+                    const bool first = ep < E;
+                    const int e = first ? ep : ep - E;
+                    const double w = sw[s + e];
+                    const uint32_t id = si[s + e];
+                    const int i1 = id & 0xffff, i2 = id >> 16;
+                    if (k1s <= K) {
+                        const double2 c = c1b[(k1s - 1) * n1 + i1];
+                        av = w * (first ? c.x : c.y);
+                    }
+                    const int k2a = l16 + 1, k2b = 16 + l16 + 1;
+                    if (k2a <= K) {
+                        const double2 c = c2b[(k2a - 1) * n2 + i2];
+                        br0 = first ? c.x : -c.y;
+                        bi0 = first ? c.y : c.x;
+                    }
+                    if (k2b <= K) {
+                        const double2 c = c2b[(k2b - 1) * n2 + i2];
+                        br1 = first ? c.x : -c.y;
+                        bi1 = first ? c.y : c.x;
+                    }
+                }
+                a1[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br0, a1[0], 0, 0, 0);
+                a1[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br1, a1[1], 0, 0, 0);
+                a1[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi0, a1[2], 0, 0, 0);
+                a1[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi1, a1[3], 0, 0, 0);
+            }
+            // rows k1 = 16h + lq + 4v + 1, columns k2 = 16*(j&1) + l16 + 1
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k2 = 16 * (j & 1) + l16 + 1;
+                const int slot = 2 * wave + (j >> 1);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k1 = 16 * h + lq + 4 * v + 1;
+                    if (k1 <= K && k2 <= k1) {
+                        const int p = (k1 - 1) * k1 / 2 + (k2 - 1) - off_h;
+                        T[p * 16 + slot] = a1[j][v];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // stage 2: blocks b = wave + 8j -> (row block b>>1, k3 block b&1)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = wave + 8 * j;
+            if (b < 2 * nrb) {
+                const int rb = b >> 1, cb = b & 1;
+                const int k3 = 16 * cb + l16 + 1;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    const int slot = 4 * kk + lq;
+                    const double av = T[(16 * rb + l16) * 16 + slot];
+                    const int m3 = 8 * t + (slot >> 1);
+                    double bv = 0.0;
+                    if (m3 < n3 && k3 <= K) {
+                        const double2 c = c3b[(k3 - 1) * n3 + m3];
+                        bv = (slot & 1) ? -c.y : c.x;
+                    }
+                    acc2[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc2[j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // rows = pairs p = 16 rb + lq + 4v, columns k3 = 16 cb + l16 + 1
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int b = wave + 8 * j;
+        if (b < 2 * nrb) {
+            const int rb = b >> 1, cb = b & 1;
+            const int k3 = 16 * cb + l16 + 1;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int p = 16 * rb + lq + 4 * v;
+                if (p < P) {
+                    const int k1 = pk[p] & 0xff, k2 = pk[p] >> 8;
+                    if (k3 <= k2) {
+                        const int64_t kt = (int64_t)(k1 - 1) * k1 * (k1 + 1) / 6 + (int64_t)(k2 - 1) * k2 / 2 + (k3 - 1);
+                        As[(int64_t)lt * kl_pad + kt] = acc2[j][v];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 MFMA GEMM  C[M x N] = A[M x K] . B[K x N]  (v_mfma_f64_16x16x4f64).
 // A is stored K-major (A[k*lda + m]), B as B[k*ldb + n]; K is a multiple of
 // 16 and the padded operands hold zeros.  A BM x BN tile per workgroup of 4
