@@ -54,6 +54,7 @@ struct tp_ctx {
     bool timing = false;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double last_ms[4] = {0, 0, 0, 0};
+    double *h_pin = nullptr;   // pinned host landing buffer for scalar results
 };
 
 // contraction tiles
@@ -197,7 +198,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[3], st));
     if (r) {
-        const unsigned nfb = (unsigned)((ncols + 255) / 256);
+        const unsigned nfb = (unsigned)((ncols + kFinalCols - 1) / kFinalCols);
         hipLaunchKernelGGL(k_final, dim3(nfb), dim3(256), 0, st, b->d_part, b->n_rt, b->kl_pad, ncols, r->d_qsum,
                            r->d_tf, b->d_blk, b->d_counter, Eout, mc);
         TP_HIP(hipGetLastError());
@@ -254,6 +255,12 @@ int tp_init(int device, tp_ctx **out)
             return tp_fail(TP_E_HIP, "hipEventCreate failed");
         }
     }
+    e = hipHostMalloc((void **)&c->h_pin, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c->h_pin = nullptr;
+        tp_finalize(c);
+        return tp_fail(TP_E_HIP, "hipHostMalloc failed");
+    }
     *out = c;
     return 0;
 }
@@ -265,6 +272,7 @@ void tp_finalize(tp_ctx *ctx)
     for (int i = 0; i < 5; ++i)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     delete ctx;
 }
 
@@ -499,7 +507,7 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
     b->n_ct = (int)(b->kl_pad / G2_BN);
     b->g1_rt = (int)(b->nm_pad64 / G1_BM);
     b->g1_ct = (int)(b->kl_pad / G1_BN);
-    b->n_fb = (int)((b->kl_pad + 255) / 256);
+    b->n_fb = (int)((b->kl_pad + kFinalCols - 1) / kFinalCols);
     std::vector<double> H((size_t)b->nlt_pad * b->nm_pad64, 0.0);
     for (int t = 0; t < nlt; ++t)
         for (int c = 0; c < b->n_m; ++c) H[(size_t)t * b->nm_pad64 + c] = Hcols[t][c];
@@ -643,8 +651,9 @@ int tp_energy(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *co
     if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
     if (int rc = upload_coeff(ctx, b, coeff, ld, b->K, b->d_coef, b->d_raw)) return rc;
     if (int rc = enqueue_eval(ctx, b, r, b->d_coef, nullptr, nullptr, 0, nullptr, b->d_out, mc_none())) return rc;
-    TP_HIP(hipMemcpyAsync(E_out, b->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipMemcpyAsync(ctx->h_pin, b->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
+    *E_out = ctx->h_pin[0];
     record_timing(ctx);
     return 0;
 }
@@ -715,8 +724,9 @@ int tp_state_energy(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s
     if (!b || !r || !s || !E_out) return tp_fail(TP_E_ARG, "null argument");
     if (b->ctx != ctx || r->b != b || s->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
     if (int rc = enqueue_eval(ctx, b, r, s->d_cur, nullptr, nullptr, 0, nullptr, b->d_out, mc_none())) return rc;
-    TP_HIP(hipMemcpyAsync(E_out, b->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipMemcpyAsync(ctx->h_pin, b->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
+    *E_out = ctx->h_pin[0];
     record_timing(ctx);
     return 0;
 }

@@ -184,13 +184,51 @@ __global__ __launch_bounds__(1024) void k_triple(const double2 *__restrict__ cb0
 //            B'[e] = [Re c_k2[i2_e] | Im c_k2[i2_e]],
 //            B'[E+e] = [-Im c_k2[i2_e] | Re c_k2[i2_e]]
 //            (complex T = sum_e a_e b_e as one real GEMM with K-dim 2E);
-//            the (k1 >= k2) entries go to Tf[pair][slot] in LDS, slot =
-//            (mu3 - 8t, re/im);
-//   stage 2: As[pair][k3] += sum_slot Tf[pair][slot] . [Re c_k3[l3,mu3] ; -Im c_k3[l3,mu3]],
+//            the (k1 >= k2) entries go to Tf[slot][pair] in LDS, slot =
+//            (mu3 - 8t, re/im) (slot-major keeps lanes on distinct banks);
+//   stage 2: As[pair][k3] += sum_slot Tf[slot][pair] . [Re c_k3[l3,mu3] ; -Im c_k3[l3,mu3]],
 //            accumulated across all iterations in MFMA registers.
 // Tf is double-buffered, so one barrier per iteration suffices.  Same sums
 // as k_triple below, regrouped for MFMA.
 // ---------------------------------------------------------------------------
+// Stage-1 MFMA operands of one k-step: lane (row l16, k lq) gets
+// A'[k1s][4 ks + lq] and B'[4 ks + lq][k2] for the two k2 blocks (Tr and Ti
+// columns).  Zero past 2E or past K.
+struct S1Ops {
+    double a, br0, bi0, br1, bi1;
+};
+
+__device__ __forceinline__ S1Ops s1_fetch(int ks, int E, int s, int K, int k1s, int l16, int lq, int n1, int n2,
+                                          const double *sw, const uint32_t *si, const double2 *c1b,
+                                          const double2 *c2b)
+{
+    S1Ops o = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int ep = 4 * ks + lq;
+    if (ep < 2 * E) {
+        const bool first = ep < E;
+        const int e = first ? ep : ep - E;
+        const double w = sw[s + e];
+        const uint32_t id = si[s + e];
+        const int i1 = id & 0xffff, i2 = id >> 16;
+        if (k1s <= K) {
+            const double2 c = c1b[(k1s - 1) * n1 + i1];
+            o.a = w * (first ? c.x : c.y);
+        }
+        const int k2a = l16 + 1, k2b = 16 + l16 + 1;
+        if (k2a <= K) {
+            const double2 c = c2b[(k2a - 1) * n2 + i2];
+            o.br0 = first ? c.x : -c.y;
+            o.bi0 = first ? c.y : c.x;
+        }
+        if (k2b <= K) {
+            const double2 c = c2b[(k2b - 1) * n2 + i2];
+            o.br1 = first ? c.x : -c.y;
+            o.bi1 = first ? c.y : c.x;
+        }
+    }
+    return o;
+}
+
 __global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
                                                      const int *__restrict__ sel, int flip, int n_c, int K,
                                                      const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
@@ -245,7 +283,7 @@ __global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__
             pk[p] = 0;
         }
     }
-    for (int i = tid; i < 2 * nrb * 16 * 16; i += 512) Tf[i] = 0.0;
+    for (int i = tid; i < 2 * 16 * Prows; i += 512) Tf[i] = 0.0;
     __syncthreads();
 
     dbl4 acc2[4];
@@ -258,38 +296,20 @@ __global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__
         const int mu = 8 * t + wave;     // index mu3 + l3 of this wave's group
         if (mu < n3) {
             const int s = sg[mu], E = sg[mu + 1] - s;
+            const int nks = (2 * E + 3) / 4;
             dbl4 a1[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) a1[j] = dbl4{0.0, 0.0, 0.0, 0.0};
-            for (int ks = 0; ks < (2 * E + 3) / 4; ++ks) {
-                const int ep = 4 * ks + lq;
-                double av = 0.0, br0 = 0.0, bi0 = 0.0, br1 = 0.0, bi1 = 0.0;
-                if (ep < 2 * E) {
-                    const bool first = ep < E;
-                    const int e = first ? ep : ep - E;
-                    const double w = sw[s + e];
-                    const uint32_t id = si[s + e];
-                    const int i1 = id & 0xffff, i2 = id >> 16;
-                    if (k1s <= K) {
-                        const double2 c = c1b[(k1s - 1) * n1 + i1];
-                        av = w * (first ? c.x : c.y);
-                    }
-                    const int k2a = l16 + 1, k2b = 16 + l16 + 1;
-                    if (k2a <= K) {
-                        const double2 c = c2b[(k2a - 1) * n2 + i2];
-                        br0 = first ? c.x : -c.y;
-                        bi0 = first ? c.y : c.x;
-                    }
-                    if (k2b <= K) {
-                        const double2 c = c2b[(k2b - 1) * n2 + i2];
-                        br1 = first ? c.x : -c.y;
-                        bi1 = first ? c.y : c.x;
-                    }
-                }
-                a1[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br0, a1[0], 0, 0, 0);
-                a1[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, br1, a1[1], 0, 0, 0);
-                a1[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi0, a1[2], 0, 0, 0);
-                a1[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bi1, a1[3], 0, 0, 0);
+            // operands of k-step ks are fetched from LDS one step ahead of the MFMAs
+            S1Ops cur = s1_fetch(0, E, s, K, k1s, l16, lq, n1, n2, sw, si, c1b, c2b);
+            for (int ks = 0; ks < nks; ++ks) {
+                S1Ops nxt = cur;
+                if (ks + 1 < nks) nxt = s1_fetch(ks + 1, E, s, K, k1s, l16, lq, n1, n2, sw, si, c1b, c2b);
+                a1[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.br0, a1[0], 0, 0, 0);
+                a1[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.br1, a1[1], 0, 0, 0);
+                a1[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.bi0, a1[2], 0, 0, 0);
+                a1[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.bi1, a1[3], 0, 0, 0);
+                cur = nxt;
             }
             // rows k1 = 16h + lq + 4v + 1, columns k2 = 16*(j&1) + l16 + 1
 #pragma unroll
@@ -301,7 +321,7 @@ __global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__
                     const int k1 = 16 * h + lq + 4 * v + 1;
                     if (k1 <= K && k2 <= k1) {
                         const int p = (k1 - 1) * k1 / 2 + (k2 - 1) - off_h;
-                        T[p * 16 + slot] = a1[j][v];
+                        T[slot * Prows + p] = a1[j][v];
                     }
                 }
             }
@@ -317,7 +337,7 @@ __global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
                     const int slot = 4 * kk + lq;
-                    const double av = T[(16 * rb + l16) * 16 + slot];
+                    const double av = T[slot * Prows + 16 * rb + l16];
                     const int m3 = 8 * t + (slot >> 1);
                     double bv = 0.0;
                     if (m3 < n3 && k3 <= K) {
@@ -525,38 +545,58 @@ struct McArgs {
     double *temp;        // [ntemp]
 };
 
+constexpr int kFinalCols = 32;   // columns per k_final block (8 threads each)
+
 __global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, int n_rt, int64_t ldp, int ncols,
                                                const double *__restrict__ qsum, const double *__restrict__ tf,
                                                double *__restrict__ blockpart, unsigned *__restrict__ counter,
                                                double *__restrict__ Eout, McArgs mc)
 {
-    __shared__ double red[256];
-    __shared__ int s_last;
-    const int tid = threadIdx.x;
-    const int kt = blockIdx.x * 256 + tid;
-    double e = 0.0;
-    if (kt < ncols) {
-        double S = 0.0, Q = 0.0;
-        for (int q = 0; q < n_rt; ++q) {
+    __shared__ double rs[8][kFinalCols], rq[8][kFinalCols], re[kFinalCols];
+    const int tid = threadIdx.x, c = tid % kFinalCols, g = tid / kFinalCols;
+    const int kt = blockIdx.x * kFinalCols + c;
+    double S = 0.0, Q = 0.0;
+    if (kt < ncols)
+        for (int q = g; q < n_rt; q += 8) {
             S += part[((int64_t)q * 2 + 0) * ldp + kt];
             Q += part[((int64_t)q * 2 + 1) * ldp + kt];
         }
-        e = tf[kt] * (qsum[kt] * log(S) - Q);
+    rs[g][c] = S;
+    rq[g][c] = Q;
+    __syncthreads();
+    if (tid < kFinalCols) {
+        double S2 = 0.0, Q2 = 0.0;
+        for (int q = 0; q < 8; ++q) {
+            S2 += rs[q][tid];
+            Q2 += rq[q][tid];
+        }
+        re[tid] = kt < ncols ? tf[kt] * (qsum[kt] * log(S2) - Q2) : 0.0;
     }
-    red[tid] = e;
+    __shared__ int s_last;
+    __shared__ double fin[256];
+    __syncthreads();
+    if (tid == 0) {
+        double eb = 0.0;
+        for (int q = 0; q < kFinalCols; ++q) eb += re[q];
+        blockpart[blockIdx.x] = eb;
+        __threadfence();
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    // last block: all block totals, read in parallel, summed in a fixed tree
+    __threadfence();
+    double e = 0.0;
+    for (unsigned b = tid; b < gridDim.x; b += 256) e += blockpart[b];
+    fin[tid] = e;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) red[tid] += red[tid + s];
+        if (tid < s) fin[tid] += fin[tid + s];
         __syncthreads();
     }
     if (tid != 0) return;
-    blockpart[blockIdx.x] = red[0];
-    __threadfence();
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev != gridDim.x - 1) return;
-    __threadfence();
-    double E = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b) E += blockpart[b];
+    const double E = fin[0];
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     Eout[0] = E;
     if (mc.mode == 1) {
