@@ -68,6 +68,21 @@ void tp_basis_destroy(tp_basis *b);
 /* basislen, number of l-triples, klength = K(K+1)(K+2)/6, R = 2N^2. */
 int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *klength, int64_t *R);
 
+/* Arithmetic of the contraction for every later call on this basis.
+ *   TP_PREC_FP64   (default) FP64 MFMA throughout; the parity path.
+ *   TP_PREC_FP16X3 c3 = F.As with F and As split into FP16 hi/lo pairs after
+ *                  power-of-two scaling; hi.hi + (hi.lo + lo.hi) on FP16 MFMA
+ *                  with FP32 accumulation.  Triple products, the histogram
+ *                  terms and the log-likelihood sum stay FP64.
+ *   TP_PREC_BF16   one BF16 product with FP32 accumulation (comparison mode).
+ * No counterpart in the reference (whose GPU path is FP32 end to end,
+ * cuda.jl:79-88). */
+#define TP_PREC_FP64 0
+#define TP_PREC_FP16X3 1
+#define TP_PREC_BF16 2
+int tp_basis_set_precision(tp_basis *b, int mode);
+int tp_basis_get_precision(const tp_basis *b);
+
 /* c3raw for the coefficients of shells 1..K.
  * Replaces FullCorrelation_parallized(intensity, basis, minimal, normalize,
  * return_raw=true) (correlations.jl:179-184, cuda.jl:74-98). */

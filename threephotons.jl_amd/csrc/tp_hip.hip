@@ -60,7 +60,7 @@ struct tp_ctx {
 // contraction tiles
 constexpr int G1_BM = 64, G1_BN = 32, G1_WGM = 2;   // G = H . As
 constexpr int G2_BM = 64, G2_BN = 64, G2_WGM = 4;   // c3 = Bm . G (+ energy)
-constexpr int KL_ALIGN = 64;                        // column padding (multiple of G1_BN and G2_BN)
+constexpr int KL_ALIGN = 128;                       // column padding (multiple of G1_BN, G2_BN and H_BN)
 constexpr int kMfmaMaxK = 27;                       // k_triple_mfma: at most 16 row blocks of pairs per half
 
 struct tp_basis {
@@ -81,6 +81,15 @@ struct tp_basis {
     int nwork = 0, Lw = 0, EmaxLt = 0, Prows = 0;
     size_t mfma_lds = 0;
     int *d_work = nullptr;         // [nwork] lt | half << 16, most expensive first
+    // mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_BF16)
+    int prec = TP_PREC_FP64;
+    std::vector<double> h_H, h_Bm; // host copies [nlt_pad][nm_pad64], [nm_pad16][R_pad] for building F
+    int ldk = 0, n_rt16 = 0, n_ct16 = 0;
+    int64_t R_pad16 = 0;
+    double F_inv = 1.0;            // 1 / (power-of-two scale of F)
+    uint16_t *d_Fhi = nullptr, *d_Flo = nullptr, *d_Fbf = nullptr;   // [R_pad16][ldk]
+    uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;                      // [kl_pad][ldk]
+    double *d_Ainv = nullptr;      // [kl_pad] 1 / column scale of As
     // device tables
     double *d_H = nullptr;         // [nlt_pad][nm_pad64]   H[lt][class]
     double *d_Bm = nullptr;        // [nm_pad16][R_pad]     Bm[class][r]
@@ -174,13 +183,45 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     }
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+    const int ncols = r ? (int)r->ncols : (int)b->klength;
+    int n_rt_fin = b->n_rt;
+    if (b->prec != TP_PREC_FP64) {
+        // mixed precision: c3 = F . As with 16-bit MFMA operands
+        const double *qh = r ? r->d_qhat : nullptr;
+        const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
+        if (b->prec == TP_PREC_FP16X3) {
+            hipLaunchKernelGGL(k_split_as<1>, dim3((unsigned)(b->kl_pad / 64)), dim3(256), 0, st, b->d_As, b->kl_pad,
+                               (int)b->n_lt, b->ldk, b->d_Ahi, b->d_Alo, b->d_Ainv);
+        } else {
+            hipLaunchKernelGGL(k_split_as<2>, dim3((unsigned)(b->kl_pad / 64)), dim3(256), 0, st, b->d_As, b->kl_pad,
+                               (int)b->n_lt, b->ldk, b->d_Ahi, nullptr, b->d_Ainv);
+        }
+        TP_HIP(hipGetLastError());
+        if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
+        const uint16_t *Fh = b->prec == TP_PREC_FP16X3 ? b->d_Fhi : b->d_Fbf;
+#define TP_G16(MODE, OUT, EN, OUTP)                                                                                \
+    hipLaunchKernelGGL((k_gemm16<MODE, OUT, EN>), grid16, dim3(256), 0, st, Fh, b->d_Flo, (int64_t)b->ldk, b->d_Ahi, \
+                       b->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, b->d_Ainv, (int)b->R, ncols, b->n_rt16,          \
+                       b->n_ct16, OUTP, b->R, qh, b->d_part, b->kl_pad)
+        if (b->prec == TP_PREC_FP16X3) {
+            if (r && c3out) TP_G16(1, 1, true, c3out);
+            else if (r) TP_G16(1, 0, true, nullptr);
+            else TP_G16(1, 1, false, c3out);
+        } else {
+            if (r && c3out) TP_G16(2, 1, true, c3out);
+            else if (r) TP_G16(2, 0, true, nullptr);
+            else TP_G16(2, 1, false, c3out);
+        }
+#undef TP_G16
+        TP_HIP(hipGetLastError());
+        n_rt_fin = b->n_rt16;
+    } else {
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
     hipLaunchKernelGGL((k_gemm<G1_BM, G1_BN, G1_WGM, 2, false>), dim3((unsigned)(b->g1_rt * b->g1_ct)), dim3(256), 0,
                        st, b->d_H, b->nm_pad64, b->d_As, b->kl_pad, (int)b->nlt_pad, (int)b->nm_pad64,
                        (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0);
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
-    const int ncols = r ? (int)r->ncols : (int)b->klength;
     const dim3 grid((unsigned)(b->n_rt * b->n_ct));
     if (r && c3out) {
         hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
@@ -196,10 +237,11 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
                            nullptr, nullptr, (int64_t)0);
     }
     TP_HIP(hipGetLastError());
+    }
     if (timing) TP_HIP(hipEventRecord(ctx->ev[3], st));
     if (r) {
         const unsigned nfb = (unsigned)((ncols + kFinalCols - 1) / kFinalCols);
-        hipLaunchKernelGGL(k_final, dim3(nfb), dim3(256), 0, st, b->d_part, b->n_rt, b->kl_pad, ncols, r->d_qsum,
+        hipLaunchKernelGGL(k_final, dim3(nfb), dim3(256), 0, st, b->d_part, n_rt_fin, b->kl_pad, ncols, r->d_qsum,
                            r->d_tf, b->d_blk, b->d_counter, Eout, mc);
         TP_HIP(hipGetLastError());
     }
@@ -296,7 +338,8 @@ void tp_basis_destroy(tp_basis *b)
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
     void *ptrs[] = {b->d_H,   b->d_Bm,  b->d_ew,   b->d_eidx,    b->d_gstart, b->d_ltinfo, b->d_chunks,
                     b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_blk,
-                    b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work};
+                    b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
+                    b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -508,6 +551,10 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
     b->g1_rt = (int)(b->nm_pad64 / G1_BM);
     b->g1_ct = (int)(b->kl_pad / G1_BN);
     b->n_fb = (int)((b->kl_pad + kFinalCols - 1) / kFinalCols);
+    b->ldk = (int)round_up(nlt, 64);
+    b->R_pad16 = round_up(b->R, H_BM);
+    b->n_rt16 = (int)(b->R_pad16 / H_BM);
+    b->n_ct16 = (int)(b->kl_pad / H_BN);
     std::vector<double> H((size_t)b->nlt_pad * b->nm_pad64, 0.0);
     for (int t = 0; t < nlt; ++t)
         for (int c = 0; c < b->n_m; ++c) H[(size_t)t * b->nm_pad64 + c] = Hcols[t][c];
@@ -550,6 +597,8 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
         tp_basis_destroy(b);
         return tp_fail(TP_E_HIP, "hipMemset failed");
     }
+    b->h_H = std::move(H);
+    b->h_Bm = std::move(Bm);
     *out = b;
     return 0;
 }
@@ -562,6 +611,67 @@ int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *
     if (klength) *klength = b->klength;
     if (R) *R = b->R;
     return 0;
+}
+
+static uint16_t host_f16(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+static double host_f16_val(uint16_t h) { return (double)__builtin_bit_cast(_Float16, h); }
+static uint16_t host_bf16(double x)   // round to nearest even of the float value
+{
+    const float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+int tp_basis_set_precision(tp_basis *b, int mode)
+{
+    if (!b) return tp_fail(TP_E_ARG, "null basis");
+    if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3 && mode != TP_PREC_BF16)
+        return tp_fail(TP_E_ARG, "unknown precision mode");
+    if (int rc = check_ctx(b->ctx)) return rc;
+    if (mode != TP_PREC_FP64 && !b->d_Fhi) {
+        // F[r][lt] = sum_class Bm[class][r] H[lt][class] in FP64, then scaled by a
+        // power of two into [2^13, 2^14) and split into FP16 hi/lo and BF16
+        const int64_t nlt = b->n_lt;
+        std::vector<double> F((size_t)b->R_pad16 * b->ldk, 0.0);
+        double fmax = 0.0;
+        for (int64_t r = 0; r < b->R; ++r)
+            for (int64_t t = 0; t < nlt; ++t) {
+                double s = 0.0;
+                for (int c = 0; c < b->n_m; ++c)
+                    s += b->h_Bm[(size_t)c * b->R_pad + r] * b->h_H[(size_t)t * b->nm_pad64 + c];
+                F[(size_t)r * b->ldk + t] = s;
+                fmax = std::max(fmax, std::fabs(s));
+            }
+        const double sF = fmax > 0.0 ? std::ldexp(1.0, 13 - std::ilogb(fmax)) : 1.0;
+        b->F_inv = 1.0 / sF;
+        std::vector<uint16_t> fh(F.size()), fl(F.size()), fb(F.size());
+        for (size_t i = 0; i < F.size(); ++i) {
+            const double x = F[i] * sF;
+            fh[i] = host_f16(x);
+            fl[i] = host_f16(x - host_f16_val(fh[i]));
+            fb[i] = host_bf16(x);
+        }
+        int rc = 0;
+        auto up16 = [&](uint16_t **d, const std::vector<uint16_t> &v) -> int {
+            if (int r2 = dev_alloc(d, v.size())) return r2;
+            if (hipMemcpy(*d, v.data(), sizeof(uint16_t) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return tp_fail(TP_E_HIP, "hipMemcpy failed");
+            return 0;
+        };
+        if ((rc = up16(&b->d_Fhi, fh)) || (rc = up16(&b->d_Flo, fl)) || (rc = up16(&b->d_Fbf, fb)) ||
+            (rc = dev_alloc(&b->d_Ahi, (size_t)b->kl_pad * b->ldk)) ||
+            (rc = dev_alloc(&b->d_Alo, (size_t)b->kl_pad * b->ldk)) || (rc = dev_alloc(&b->d_Ainv, b->kl_pad)))
+            return rc;
+    }
+    b->prec = mode;
+    return 0;
+}
+
+int tp_basis_get_precision(const tp_basis *b)
+{
+    return b ? b->prec : TP_E_ARG;
 }
 
 int tp_c3(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, double *c3_out)

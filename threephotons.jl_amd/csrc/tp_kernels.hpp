@@ -519,6 +519,230 @@ __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int6
 }
 
 // ---------------------------------------------------------------------------
+// Mixed-precision contraction c3 = F . As (the l-triple basis F = Bm . H,
+// precomputed in FP64 and split on the host).
+//   TP precision 1 (FP16 x3): x = hi + lo with hi = fp16(x), lo = fp16(x - hi);
+//     F.As ~ Fhi.Ahi + (Fhi.Alo + Flo.Ahi), the main product and the two
+//     correction products in separate FP32 accumulators, summed in FP64.
+//   TP precision 2 (BF16): one BF16 product, FP32 accumulation.
+// Operands are scaled by powers of two (exact) to sit in [2^13, 2^14): F by
+// one global factor, As per k-triple column (k_split_as).
+// ---------------------------------------------------------------------------
+typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b16x8 __attribute__((ext_vector_type(8)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f16_bits(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+__device__ __forceinline__ double f16_val(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }
+__device__ __forceinline__ uint16_t bf16_bits(double x) { return __builtin_bit_cast(uint16_t, (__bf16)(float)x); }
+
+// As [lt][kt] (FP64) -> per-column scale s (stored as 1/s) and 16-bit copies
+// in [kt][lt] layout (lt contiguous, row length ldk).  64 columns per block.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As, int64_t kl_pad, int nlt, int ldk,
+                                                  uint16_t *__restrict__ Ahi, uint16_t *__restrict__ Alo,
+                                                  double *__restrict__ inv_scale)
+{
+    __shared__ double tile[64][65];
+    __shared__ double smax[4][64];
+    const int kt0 = blockIdx.x * 64;
+    const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
+    double m = 0.0;
+    for (int lt = g; lt < nlt; lt += 4) m = fmax(m, fabs(As[(int64_t)lt * kl_pad + kt0 + c]));
+    smax[g][c] = m;
+    __syncthreads();
+    if (tid < 64) {
+        const double mm = fmax(fmax(smax[0][tid], smax[1][tid]), fmax(smax[2][tid], smax[3][tid]));
+        double s = 1.0;
+        if (mm > 0.0) s = ldexp(1.0, 13 - ilogb(mm));
+        smax[0][tid] = s;
+        inv_scale[kt0 + tid] = 1.0 / s;
+    }
+    __syncthreads();
+    const int kr = tid >> 2, q = tid & 3;   // write phase: row kt0+kr, 16 consecutive lt
+    const double s = smax[0][kr];
+    for (int lt0 = 0; lt0 < ldk; lt0 += 64) {
+        for (int r = g; r < 64; r += 4) {
+            const int lt = lt0 + r;
+            tile[r][c] = lt < nlt ? As[(int64_t)lt * kl_pad + kt0 + c] : 0.0;
+        }
+        __syncthreads();
+        u16x8 h0, h1, l0, l1;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const double x = tile[q * 16 + i][kr] * s;
+            uint16_t hb, lb = 0;
+            if (MODE == 1) {
+                hb = f16_bits(x);
+                lb = f16_bits(x - f16_val(hb));
+            } else {
+                hb = bf16_bits(x);
+            }
+            if (i < 8) { h0[i] = hb; l0[i] = lb; } else { h1[i - 8] = hb; l1[i - 8] = lb; }
+        }
+        u16x8 *ph = reinterpret_cast<u16x8 *>(Ahi + (int64_t)(kt0 + kr) * ldk + lt0 + q * 16);
+        ph[0] = h0;
+        ph[1] = h1;
+        if (MODE == 1) {
+            u16x8 *pl = reinterpret_cast<u16x8 *>(Alo + (int64_t)(kt0 + kr) * ldk + lt0 + q * 16);
+            pl[0] = l0;
+            pl[1] = l1;
+        }
+        __syncthreads();
+    }
+}
+
+// C[r][kt] = sum_k A[r][k] B[kt][k] (both operands k-contiguous, 16-bit),
+// 128 x 128 tile per workgroup of 4 waves (2 x 2, 64 x 64 each), K staged
+// through LDS 32 at a time (rows padded to 40 halves: conflict-free 16-byte
+// reads), next slice prefetched into registers.  MFMA
+// v_mfma_f32_16x16x32_{f16,bf16}: lane holds A[l%16][8(l/16)..+7],
+// B[8(l/16)..+7][l%16], accumulator rows 4(l/16)+v (measured on gfx950).
+// Epilogue as k_gemm: raw c3 (OUT = 1) and/or per-column partial energy sums.
+constexpr int H_BM = 128, H_BN = 128, H_BK = 32, H_LDS_ROW = 40;
+
+template <int MODE, int OUT, bool ENERGY>
+__global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo,
+                                                int64_t lda, const uint16_t *__restrict__ Bhi,
+                                                const uint16_t *__restrict__ Blo, int64_t ldb, int Kdim,
+                                                double a_inv, const double *__restrict__ b_inv, int M, int ncols,
+                                                int n_rt, int n_ct, double *__restrict__ out, int64_t ldo,
+                                                const double *__restrict__ qhat, double *__restrict__ part,
+                                                int64_t ldp)
+{
+    constexpr int NP = MODE == 1 ? 2 : 1;   // hi (+ lo) planes
+    __shared__ uint16_t sA[NP][H_BM][H_LDS_ROW];
+    __shared__ uint16_t sB[NP][H_BN][H_LDS_ROW];
+    __shared__ double red[2][2][H_BN];
+
+    const int nblk = n_rt * n_ct;
+    int lin = blockIdx.x;
+    if ((nblk & 7) == 0) lin = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+    const int rt = lin / n_ct, ct = lin % n_ct;
+    const int r0 = rt * H_BM, c0 = ct * H_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int l16 = lane & 15, lq = lane >> 4;
+
+    f32x4 acc[4][4], cor[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            cor[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+    // 128 rows x 32 halves = 512 chunks of 16 B per plane and operand: 2 per thread
+    u16x8 pa[NP][2], pb[NP][2];
+    auto gl = [](const uint16_t *base, int64_t ld, int row, int k) {
+        return *reinterpret_cast<const u16x8 *>(base + (int64_t)row * ld + k);
+    };
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
+        pa[0][q] = gl(Ahi, lda, r0 + row, kc);
+        pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
+        if (NP == 2) {
+            pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
+            pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
+        }
+    }
+    for (int k0 = 0; k0 < Kdim; k0 += H_BK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                *reinterpret_cast<u16x8 *>(&sA[p][row][kc]) = pa[p][q];
+                *reinterpret_cast<u16x8 *>(&sB[p][row][kc]) = pb[p][q];
+            }
+        }
+        __syncthreads();
+        if (k0 + H_BK < Kdim) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const int i = tid + 256 * q, row = i >> 2, kc = k0 + H_BK + (i & 3) * 8;
+                pa[0][q] = gl(Ahi, lda, r0 + row, kc);
+                pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
+                if (NP == 2) {
+                    pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
+                    pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
+                }
+            }
+        }
+        u16x8 fa[NP][4], fb[NP][4];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                fa[p][i] = *reinterpret_cast<const u16x8 *>(&sA[p][wm * 64 + i * 16 + l16][8 * lq]);
+                fb[p][i] = *reinterpret_cast<const u16x8 *>(&sB[p][wn * 64 + i * 16 + l16][8 * lq]);
+            }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (MODE == 1) {
+                    const h16x8 ah = __builtin_bit_cast(h16x8, fa[0][i]), bh = __builtin_bit_cast(h16x8, fb[0][j]);
+                    const h16x8 al = __builtin_bit_cast(h16x8, fa[NP - 1][i]);
+                    const h16x8 bl = __builtin_bit_cast(h16x8, fb[NP - 1][j]);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[i][j], 0, 0, 0);
+                    cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, cor[i][j], 0, 0, 0);
+                    cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, cor[i][j], 0, 0, 0);
+                } else {
+                    const b16x8 ah = __builtin_bit_cast(b16x8, fa[0][i]), bh = __builtin_bit_cast(b16x8, fb[0][j]);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[i][j], 0, 0, 0);
+                }
+            }
+    }
+
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = c0 + wn * 64 + j * 16 + l16;
+        const bool cok = col < ncols;
+        const double sc = cok ? a_inv * b_inv[col] : 0.0;
+        double sp = 0.0, sq = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int rbase = r0 + wm * 64 + i * 16 + 4 * lq;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rbase + v;
+                if (cok && r < M) {
+                    const double c = ((double)acc[i][j][v] + (double)cor[i][j][v]) * sc;
+                    if (OUT == 1) out[(int64_t)col * ldo + r] = c;
+                    if (ENERGY) {
+                        const double pv = c > 1e-30 ? c : 1e-30;
+                        const double qv = qhat[(int64_t)col * M + r];
+                        sp += pv;
+                        sq += qv * log(pv);
+                    }
+                }
+            }
+        }
+        if (ENERGY) {
+            sp += __shfl_xor(sp, 16);
+            sq += __shfl_xor(sq, 16);
+            sp += __shfl_xor(sp, 32);
+            sq += __shfl_xor(sq, 32);
+            if (lq == 0) {
+                red[wm][0][wn * 64 + j * 16 + l16] = sp;
+                red[wm][1][wn * 64 + j * 16 + l16] = sq;
+            }
+        }
+    }
+    if (ENERGY) {
+        __syncthreads();
+        if (tid < H_BN) {
+            part[((int64_t)rt * 2 + 0) * ldp + c0 + tid] = red[0][0][tid] + red[1][0][tid];
+            part[((int64_t)rt * 2 + 1) * ldp + c0 + tid] = red[0][1][tid] + red[1][1][tid];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Energy finish and the annealing step.
 //   E_kt = tF * (qsum * log(S) - Q),  S = sum_rt part0, Q = sum_rt part1,
 // summed per block in a fixed tree, block totals summed in block order by the

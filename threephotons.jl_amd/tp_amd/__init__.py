@@ -20,7 +20,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import AnnealParams, TPError, build, check, lib
+from ._lib import PREC_BF16, PREC_FP16X3, PREC_FP64, AnnealParams, TPError, build, check, lib
 
 __all__ = [
     "TPError", "build", "lib", "SphericalHarmonicsVolume", "HIP_init", "HipContext", "BasisTypeHip",
@@ -138,6 +138,11 @@ class BasisTypeHip:
                 self.h = None
         except Exception:
             pass
+
+    def set_precision(self, mode):
+        """'fp64' (default), 'fp16x3' (compensated FP16 MFMA) or 'bf16'."""
+        m = {"fp64": PREC_FP64, "fp16x3": PREC_FP16X3, "bf16": PREC_BF16}[mode] if isinstance(mode, str) else mode
+        check(lib().tp_basis_set_precision(self.h, m), "tp_basis_set_precision")
 
     def ref(self, c3ref, K3_range=None):
         """Device copy of the normalised histogram slices visited by energy()'s walk (cached)."""

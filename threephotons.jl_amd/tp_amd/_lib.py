@@ -69,6 +69,8 @@ def lib():
     L.tp_state_c3.argtypes = [vp, vp, vp, vp]
     L.tp_anneal.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.tp_intensity_from_atoms.argtypes = [vp, vp, i32, i32, dbl, i32, i32, i32, i32, vp]
+    L.tp_basis_set_precision.argtypes = [vp, i32]
+    L.tp_basis_get_precision.argtypes = [vp]
     L.tp_set_timing.argtypes = [vp, i32]
     L.tp_last_timings.argtypes = [vp, C.POINTER(dbl), i32]
     _lib = L
@@ -98,5 +100,8 @@ EXPORTS = [
     "tp_abi_version", "tp_last_error", "tp_init", "tp_finalize", "tp_basis_create", "tp_basis_destroy",
     "tp_basis_info", "tp_c3", "tp_ref_create", "tp_ref_destroy", "tp_energy", "tp_state_create",
     "tp_state_destroy", "tp_state_read", "tp_state_energy", "tp_state_c3", "tp_anneal",
-    "tp_intensity_from_atoms", "tp_set_timing", "tp_last_timings",
+    "tp_intensity_from_atoms", "tp_set_timing", "tp_last_timings", "tp_basis_set_precision",
+    "tp_basis_get_precision",
 ]
+
+PREC_FP64, PREC_FP16X3, PREC_BF16 = 0, 1, 2
