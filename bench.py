@@ -90,6 +90,47 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
                 factored_sample=f"one full factored evaluation (F.As + energy), {t_fact:.2f}s")
 
 
+def precision_study(n):
+    """Mixed-precision c3 and energy against the FP64 path over n random
+    orientations (independent Haar SO(2l+1) per even l, seeds 0..n-1,
+    randomStartStructure, determination.jl:4-17) of the full-size structure.
+    c3 errors: per k-triple slice max_r|dc|/max_r|c3| (max over slices), the
+    RMS over all bins of |dc|/max_r|c3| per slice, and the normwise
+    ||dc||_2/||c3||_2."""
+    import tp_amd as tp
+    from tp_amd import workload
+    cfg = workload.CONFIGS["full"]
+    tp.HIP_init()
+    truth, basis, c3ref, _ = setup(cfg, tp, workload)
+    out = {"metric": "mixed-precision c3 / energy error vs FP64", "orientations": n, "config": "full (L=18,K=26,N=32)"}
+    acc = {m: {"slice_max": [], "rms": [], "norm2": [], "dE_abs": [], "dE_rel": []} for m in ("fp16x3", "bf16")}
+    for seed in range(n):
+        st = workload.random_start_structure(truth, cfg["L"], seed)
+        dev = tp.DeviceState(st, basis)
+        basis.set_precision("fp64")
+        c64 = dev.c3()
+        E64 = dev.energy(c3ref)
+        scale = np.abs(c64).max(axis=0)
+        for m in ("fp16x3", "bf16"):
+            basis.set_precision(m)
+            c = dev.c3()
+            E = dev.energy(c3ref)
+            d = np.abs(c - c64) / scale
+            a = acc[m]
+            a["slice_max"].append(float(d.max()))
+            a["rms"].append(float(np.sqrt((d * d).mean())))
+            a["norm2"].append(float(np.linalg.norm(c - c64) / np.linalg.norm(c64)))
+            a["dE_abs"].append(abs(E - E64))
+            a["dE_rel"].append(abs(E - E64) / abs(E64))
+        basis.set_precision("fp64")
+        log(f"orientation {seed}: fp16x3 {acc['fp16x3']['slice_max'][-1]:.2e}  bf16 {acc['bf16']['slice_max'][-1]:.2e}")
+    for m, a in acc.items():
+        out[m] = {"c3_slice_rel_err_max": max(a["slice_max"]), "c3_slice_rel_err_rms": float(np.sqrt(np.mean(np.square(a["rms"])))),
+                  "c3_norm2_rel_err_max": max(a["norm2"]), "energy_abs_err_max": max(a["dE_abs"]),
+                  "energy_rel_err_max": max(a["dE_rel"])}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +139,13 @@ def main():
     ap.add_argument("--anneal-steps", type=int, default=2000)
     ap.add_argument("--cpu-cols", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision-study", type=int, default=0,
+                    help="instead of the benchmark: c3/energy errors of the mixed-precision modes against FP64 "
+                         "for this many random orientations of the full-size structure")
     args = ap.parse_args()
+    if args.precision_study:
+        precision_study(args.precision_study)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the k-triple sharding "
               "design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -141,6 +188,29 @@ def main():
     log(f"anneal: {args.anneal_steps} steps in {t_anneal:.2f}s, E0 {res['E0']:.10g} -> {res['E_final']:.10g}, "
         f"accepted {int(res['accepted'].sum())}")
 
+    # mixed precision (FP16 hi/lo compensated contraction), same structure and histogram
+    basis.set_precision("fp16x3")
+    E16 = state.energy(c3ref)
+    for _ in range(args.warmup):
+        state.energy(c3ref)
+    ctx.set_timing(True)
+    kern16 = np.zeros(4)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        state.energy(c3ref)
+        kern16 += ctx.last_timings()
+    el16 = time.perf_counter() - t0
+    ctx.set_timing(False)
+    kern16 /= args.steps
+    mstate = tp.DeviceState(start, basis)
+    t0 = time.perf_counter()
+    res16 = tp.rotate_all_at_once(mstate, basis, c3ref, args.anneal_steps, np.pi / 4, 1.01, 0.1, 0.99998, seed=1000,
+                                  ntemp=50)
+    t_anneal16 = time.perf_counter() - t0
+    basis.set_precision("fp64")
+    log(f"fp16x3: evals/s {args.steps / el16:.1f}  stage ms {np.round(kern16, 4)}  |dE| {abs(E16 - E):.3g}  "
+        f"anneal {t_anneal16:.2f}s -> {res16['E_final']:.10g}")
+
     out = {
         "metric": METRIC,
         "value": evals_per_s,
@@ -164,6 +234,13 @@ def main():
                       "E0": res["E0"], "E_final": res["E_final"], "accepted": int(res["accepted"].sum())},
         "kernel_ms": {"triple": kern_ms[0], "gemm_G": kern_ms[1], "gemm_c3_energy": kern_ms[2],
                       "final": kern_ms[3]},
+        "mixed_precision": {
+            "mode": "fp16x3: c3 = F.As on FP16 MFMA, hi.hi + (hi.lo + lo.hi), FP32 accumulation, FP64 energy",
+            "evals_per_s": args.steps / el16, "ms_per_eval": 1e3 * el16 / args.steps,
+            "stage_ms": {"triple": kern16[0], "split": kern16[1], "gemm16_c3_energy": kern16[2], "final": kern16[3]},
+            "gemm16_tflops_algorithmic": 2.0 * basis.R * basis.n_lt * basis.klength / (kern16[2] * 1e-3) / 1e12,
+            "energy_abs_err_vs_fp64": abs(E16 - E), "mc_steps_per_s": (args.anneal_steps + 51) / t_anneal16,
+            "mc_E_final": res16["E_final"]},
         "roofline": {"bound": "mfma", "kernel": "k_gemm c3 = Bm.G (+ fused energy)", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": None, "flops_per_launch": flops_g2,

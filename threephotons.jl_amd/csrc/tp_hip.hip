@@ -189,12 +189,13 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         // mixed precision: c3 = F . As with 16-bit MFMA operands
         const double *qh = r ? r->d_qhat : nullptr;
         const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
+        const dim3 gs((unsigned)(b->kl_pad / 64), (unsigned)(b->ldk / 64));
         if (b->prec == TP_PREC_FP16X3) {
-            hipLaunchKernelGGL(k_split_as<1>, dim3((unsigned)(b->kl_pad / 64)), dim3(256), 0, st, b->d_As, b->kl_pad,
-                               (int)b->n_lt, b->ldk, b->d_Ahi, b->d_Alo, b->d_Ainv);
+            hipLaunchKernelGGL(k_split_as<1>, gs, dim3(256), 0, st, b->d_As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
+                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, b->d_Ahi, b->d_Alo, b->d_Ainv);
         } else {
-            hipLaunchKernelGGL(k_split_as<2>, dim3((unsigned)(b->kl_pad / 64)), dim3(256), 0, st, b->d_As, b->kl_pad,
-                               (int)b->n_lt, b->ldk, b->d_Ahi, nullptr, b->d_Ainv);
+            hipLaunchKernelGGL(k_split_as<2>, gs, dim3(256), 0, st, b->d_As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
+                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, b->d_Ahi, nullptr, b->d_Ainv);
         }
         TP_HIP(hipGetLastError());
         if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
@@ -630,6 +631,8 @@ int tp_basis_set_precision(tp_basis *b, int mode)
     if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3 && mode != TP_PREC_BF16)
         return tp_fail(TP_E_ARG, "unknown precision mode");
     if (int rc = check_ctx(b->ctx)) return rc;
+    if (mode != TP_PREC_FP64 && b->K > 64)
+        return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64");
     if (mode != TP_PREC_FP64 && !b->d_Fhi) {
         // F[r][lt] = sum_class Bm[class][r] H[lt][class] in FP64, then scaled by a
         // power of two into [2^13, 2^14) and split into FP16 hi/lo and BF16

@@ -537,37 +537,66 @@ __device__ __forceinline__ uint16_t f16_bits(double x) { return __builtin_bit_ca
 __device__ __forceinline__ double f16_val(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }
 __device__ __forceinline__ uint16_t bf16_bits(double x) { return __builtin_bit_cast(uint16_t, (__bf16)(float)x); }
 
-// As [lt][kt] (FP64) -> per-column scale s (stored as 1/s) and 16-bit copies
-// in [kt][lt] layout (lt contiguous, row length ldk).  64 columns per block.
+// As [lt][kt] (FP64) -> 16-bit copies in [kt][lt] layout (lt contiguous, row
+// length ldk), one 64 x 64 tile per block, with a per-column power-of-two
+// scale s_kt (stored as 1/s) from the bound
+//   |As(kt, lt)| <= |c_k1,l1| |c_k2,l2| |c_k3,l3| <= n_k1 n_k2 n_k3,
+// n_k = max_l |c_k,l| (the 3j coupling of three unit vectors has modulus <= 1
+// by the orthogonality of the Clebsch-Gordan map), so |As s| < 2^14.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As, int64_t kl_pad, int nlt, int ldk,
-                                                  uint16_t *__restrict__ Ahi, uint16_t *__restrict__ Alo,
+                                                  const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
+                                                  const int *__restrict__ sel, int flip, int n_c, int K, int L,
+                                                  int klength, uint16_t *__restrict__ Ahi, uint16_t *__restrict__ Alo,
                                                   double *__restrict__ inv_scale)
 {
+    const double2 *coef = pick(cb0, cb1, sel, flip);
     __shared__ double tile[64][65];
-    __shared__ double smax[4][64];
-    const int kt0 = blockIdx.x * 64;
+    __shared__ double s_n2[64 * 12];   // |c_k,l|^2, k < K <= 64, even l <= 22
+    __shared__ double s_nk[64];
+    __shared__ double s_sc[64];
+    const int kt0 = blockIdx.x * 64, lt0 = blockIdx.y * 64;
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
-    double m = 0.0;
-    for (int lt = g; lt < nlt; lt += 4) m = fmax(m, fabs(As[(int64_t)lt * kl_pad + kt0 + c]));
-    smax[g][c] = m;
-    __syncthreads();
-    if (tid < 64) {
-        const double mm = fmax(fmax(smax[0][tid], smax[1][tid]), fmax(smax[2][tid], smax[3][tid]));
-        double s = 1.0;
-        if (mm > 0.0) s = ldexp(1.0, 13 - ilogb(mm));
-        smax[0][tid] = s;
-        inv_scale[kt0 + tid] = 1.0 / s;
+    const int nl = L / 2 + 1;
+    for (int i = tid; i < K * nl; i += 256) {
+        const int k = i / nl, l = 2 * (i % nl);
+        const double2 *v = coef + (size_t)k * n_c + l * (l - 1) / 2;
+        double s = 0.0;
+        for (int m = 0; m <= 2 * l; ++m) s += v[m].x * v[m].x + v[m].y * v[m].y;
+        s_n2[i] = s;
     }
     __syncthreads();
+    if (tid < K) {
+        double m = 0.0;
+        for (int l = 0; l < nl; ++l) m = fmax(m, s_n2[tid * nl + l]);
+        s_nk[tid] = sqrt(m);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        double s = 1.0;
+        const int kt = kt0 + tid;
+        if (kt < klength) {
+            // kt -> (k1, k2, k3) in the order k1 >= k2 >= k3 (correlations.jl:72-82)
+            int k1 = 1;
+            while ((int64_t)(k1 + 1) * (k1 + 2) * k1 / 6 <= kt) ++k1;
+            const int r1 = kt - (k1 - 1) * k1 * (k1 + 1) / 6;
+            int k2 = 1;
+            while ((k2 + 1) * k2 / 2 <= r1) ++k2;
+            const int k3 = r1 - (k2 - 1) * k2 / 2 + 1;
+            const double bnd = s_nk[k1 - 1] * s_nk[k2 - 1] * s_nk[k3 - 1];
+            if (bnd > 0.0) s = ldexp(1.0, 13 - ilogb(bnd));
+        }
+        s_sc[tid] = s;
+        if (blockIdx.y == 0) inv_scale[kt] = 1.0 / s;
+    }
     const int kr = tid >> 2, q = tid & 3;   // write phase: row kt0+kr, 16 consecutive lt
-    const double s = smax[0][kr];
-    for (int lt0 = 0; lt0 < ldk; lt0 += 64) {
+    {
         for (int r = g; r < 64; r += 4) {
             const int lt = lt0 + r;
             tile[r][c] = lt < nlt ? As[(int64_t)lt * kl_pad + kt0 + c] : 0.0;
         }
         __syncthreads();
+        const double s = s_sc[kr];
         u16x8 h0, h1, l0, l1;
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -594,13 +623,13 @@ __global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As,
 }
 
 // C[r][kt] = sum_k A[r][k] B[kt][k] (both operands k-contiguous, 16-bit),
-// 128 x 128 tile per workgroup of 4 waves (2 x 2, 64 x 64 each), K staged
+// 128 x 64 tile per workgroup of 4 waves (2 x 2, 64 x 32 each), K staged
 // through LDS 32 at a time (rows padded to 40 halves: conflict-free 16-byte
 // reads), next slice prefetched into registers.  MFMA
 // v_mfma_f32_16x16x32_{f16,bf16}: lane holds A[l%16][8(l/16)..+7],
 // B[8(l/16)..+7][l%16], accumulator rows 4(l/16)+v (measured on gfx950).
 // Epilogue as k_gemm: raw c3 (OUT = 1) and/or per-column partial energy sums.
-constexpr int H_BM = 128, H_BN = 128, H_BK = 32, H_LDS_ROW = 40;
+constexpr int H_BM = 128, H_BN = 64, H_BK = 32, H_LDS_ROW = 40;
 
 template <int MODE, int OUT, bool ENERGY>
 __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo,
@@ -612,6 +641,8 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
                                                 int64_t ldp)
 {
     constexpr int NP = MODE == 1 ? 2 : 1;   // hi (+ lo) planes
+    constexpr int WTM = H_BM / 2, WTN = H_BN / 2, RB = WTM / 16, CB = WTN / 16;
+    constexpr int QA = H_BM * 4 / 256, QB = H_BN * 4 / 256;   // 16-byte chunks per thread per plane
     __shared__ uint16_t sA[NP][H_BM][H_LDS_ROW];
     __shared__ uint16_t sB[NP][H_BN][H_LDS_ROW];
     __shared__ double red[2][2][H_BN];
@@ -625,65 +656,73 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lq = lane >> 4;
 
-    f32x4 acc[4][4], cor[4][4];
+    f32x4 acc[RB][CB], cor[RB][CB];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < RB; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < CB; ++j) {
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
             cor[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
-    // 128 rows x 32 halves = 512 chunks of 16 B per plane and operand: 2 per thread
-    u16x8 pa[NP][2], pb[NP][2];
+    u16x8 pa[NP][QA], pb[NP][QB];
     auto gl = [](const uint16_t *base, int64_t ld, int row, int k) {
         return *reinterpret_cast<const u16x8 *>(base + (int64_t)row * ld + k);
     };
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < QA; ++q) {
         const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
         pa[0][q] = gl(Ahi, lda, r0 + row, kc);
+        if (NP == 2) pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
+    }
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
         pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
-        if (NP == 2) {
-            pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
-            pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
-        }
+        if (NP == 2) pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
     }
     for (int k0 = 0; k0 < Kdim; k0 += H_BK) {
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
 #pragma unroll
-            for (int p = 0; p < NP; ++p) {
-                *reinterpret_cast<u16x8 *>(&sA[p][row][kc]) = pa[p][q];
-                *reinterpret_cast<u16x8 *>(&sB[p][row][kc]) = pb[p][q];
-            }
+            for (int p = 0; p < NP; ++p) *reinterpret_cast<u16x8 *>(&sA[p][row][kc]) = pa[p][q];
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
+#pragma unroll
+            for (int p = 0; p < NP; ++p) *reinterpret_cast<u16x8 *>(&sB[p][row][kc]) = pb[p][q];
         }
         __syncthreads();
         if (k0 + H_BK < Kdim) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < QA; ++q) {
                 const int i = tid + 256 * q, row = i >> 2, kc = k0 + H_BK + (i & 3) * 8;
                 pa[0][q] = gl(Ahi, lda, r0 + row, kc);
+                if (NP == 2) pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
+            }
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const int i = tid + 256 * q, row = i >> 2, kc = k0 + H_BK + (i & 3) * 8;
                 pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
-                if (NP == 2) {
-                    pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
-                    pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
-                }
+                if (NP == 2) pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
             }
         }
-        u16x8 fa[NP][4], fb[NP][4];
+        u16x8 fa[NP][RB], fb[NP][CB];
 #pragma unroll
-        for (int p = 0; p < NP; ++p)
+        for (int p = 0; p < NP; ++p) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                fa[p][i] = *reinterpret_cast<const u16x8 *>(&sA[p][wm * 64 + i * 16 + l16][8 * lq]);
-                fb[p][i] = *reinterpret_cast<const u16x8 *>(&sB[p][wn * 64 + i * 16 + l16][8 * lq]);
-            }
+            for (int i = 0; i < RB; ++i)
+                fa[p][i] = *reinterpret_cast<const u16x8 *>(&sA[p][wm * WTM + i * 16 + l16][8 * lq]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < CB; ++j)
+                fb[p][j] = *reinterpret_cast<const u16x8 *>(&sB[p][wn * WTN + j * 16 + l16][8 * lq]);
+        }
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+        for (int i = 0; i < RB; ++i)
+#pragma unroll
+            for (int j = 0; j < CB; ++j) {
                 if (MODE == 1) {
                     const h16x8 ah = __builtin_bit_cast(h16x8, fa[0][i]), bh = __builtin_bit_cast(h16x8, fb[0][j]);
                     const h16x8 al = __builtin_bit_cast(h16x8, fa[NP - 1][i]);
@@ -699,14 +738,15 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
     }
 
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = c0 + wn * 64 + j * 16 + l16;
+    for (int j = 0; j < CB; ++j) {
+        const int colt = wn * WTN + j * 16 + l16;   // column within the tile
+        const int col = c0 + colt;
         const bool cok = col < ncols;
         const double sc = cok ? a_inv * b_inv[col] : 0.0;
         double sp = 0.0, sq = 0.0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int rbase = r0 + wm * 64 + i * 16 + 4 * lq;
+        for (int i = 0; i < RB; ++i) {
+            const int rbase = r0 + wm * WTM + i * 16 + 4 * lq;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = rbase + v;
@@ -728,8 +768,8 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
             sp += __shfl_xor(sp, 32);
             sq += __shfl_xor(sq, 32);
             if (lq == 0) {
-                red[wm][0][wn * 64 + j * 16 + l16] = sp;
-                red[wm][1][wn * 64 + j * 16 + l16] = sq;
+                red[wm][0][colt] = sp;
+                red[wm][1][colt] = sq;
             }
         }
     }
