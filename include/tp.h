@@ -43,6 +43,7 @@ typedef struct tp_ctx tp_ctx;
 typedef struct tp_basis tp_basis;
 typedef struct tp_ref tp_ref;
 typedef struct tp_state tp_state;
+typedef struct tp_batch tp_batch;
 
 /* ABI version of the loaded library (== TP_ABI_VERSION). */
 int tp_abi_version(void);
@@ -140,6 +141,69 @@ typedef struct tp_anneal_params {
 
 int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
               double *E_trace, double *Enew_trace, int32_t *acc_trace, double *scalars);
+
+/* ---- batches: many evaluations in one graph launch ------------------------- */
+
+/* Bayes energies of n independent evaluations, each with its own basis (all
+ * sharing L, LMAX, N, context and precision; K may differ) and histogram
+ * refs[i] (built on bases[i]).  coeff[i] has K_i shells, ld[i] >= LMAX^2.
+ * Mixed precision (tp_basis_set_precision FP16X3 / BF16 on every basis): the
+ * n evaluations run as one fixed sequence of batched launches, captured once
+ * as a HIP graph and replayed while the same (bases, refs) are passed again.
+ * FP64: the evaluations run one after another on the device.
+ * The counterpart of calling energy() (correlations.jl:201-226) n times. */
+int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_ref *const *refs,
+                    const double *const *coeff, const int64_t *ld, double *E_out);
+
+/* Concurrent annealing runs ("groups"), each with its own shell count
+ * K_g = refs[g]'s basis K <= b's K, histogram refs[g], start coefficients
+ * coeff[g] (K_g shells) and seed.  Every sweep proposes nprop (>= 3)
+ * rotations per run: for each even l >= 2 an independent SO(3) rotation
+ * (uniform axis, Haar-distributed angle <= amax * pi/4) applied to the
+ * l-blocks of all shells through FP64 real Wigner-D matrices, which keeps
+ * every two-photon correlation sum_m c_k1,lm^* c_k2,lm, as the reference's
+ * moves do (get_MC_step, determination.jl:232-241).  All ngroups * nprop
+ * evaluations of a sweep, and the per-run decision, are one replay of a
+ * captured HIP graph.  Decision per run: the lowest-energy proposal is
+ * accepted by the Metropolis rule on its exact energy difference
+ * (evaluate_step, determination.jl:491-515); T decays by `decay` per sweep;
+ * amax grows by `factor` (capped at 1) on acceptance and shrinks otherwise.
+ * b must be in mixed precision.  hist_cap sweeps of per-run energies are kept. */
+int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, const tp_ref *const *refs,
+                    const double *const *coeff, const int64_t *ld, const uint64_t *seeds, int hist_cap,
+                    tp_batch **out);
+void tp_batch_destroy(tp_batch *B);
+/* Time the candidate triple-product work orders and contraction tiles on this
+ * batch (reps launches each) and keep the fastest.  ms_out[6] (may be NULL):
+ * work orders 0,1 then tiles 128x64, 64x64, 128x128, 64x128; config_out[3]
+ * (may be NULL): chosen tile rows, tile columns, work order. */
+int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out);
+/* Initial energies and temperatures: slot 0 of every run is the identity, T0 =
+ * temperature_factor * std(energies of the other slots) (calculate_temperature,
+ * determination.jl:244-261). */
+int tp_batch_probe(tp_batch *B, double temperature_factor);
+/* nsweeps sweeps (graph replays). */
+int tp_batch_run(tp_batch *B, int nsweeps, double decay, double factor);
+/* Per-run state: energy, temperature, angle bound, accepted sweeps, sweeps done. */
+int tp_batch_group(tp_batch *B, int g, double *E, double *T, double *amax, int32_t *nacc, int32_t *nsweep);
+/* Current coefficients of run g written into the caller's vectors (even l <= L
+ * entries of its K_g shells; the rest untouched). */
+int tp_batch_coeff(tp_batch *B, int g, double *coeff, int64_t ld);
+/* Energies after each sweep, [min(n, hist_cap)][ngroups]. */
+int tp_batch_history(tp_batch *B, double *Ehist, int n);
+/* Evaluate given rotations of the current states: quat[ngroups*nprop][L/2][4]
+ * (w,x,y,z per even l >= 2; slot e = g*nprop + p), energies to E_out[ngroups*nprop].
+ * No decision is taken. */
+int tp_batch_eval_quats(tp_batch *B, const double *quat, double *E_out);
+
+/* Rotate the even l >= 2 blocks of nshell shells by quat[L/2][4] (one unit
+ * quaternion per l, w,x,y,z) in FP64, through the real Wigner-D recurrence:
+ * the rotated coefficients describe f(R^-1 r).  Entries other than even l <= L
+ * are copied.  One quaternion for every l rotates the whole intensity, which
+ * leaves c3 unchanged (its SO(3) invariance); rotateStructure
+ * (spherical_harmonics.jl:598-612) is the reference's whole-structure form. */
+int tp_rotate(tp_ctx *ctx, const tp_basis *b, const double *coeff, int64_t ld, int nshell, const double *quat,
+              double *out);
 
 /* ---- synthetic input ------------------------------------------------------ */
 

@@ -41,3 +41,33 @@ def c3ref_from_columns(cols, N, K):
 
 def rel_err(a, b):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300))
+
+
+def quat_to_rot(q):
+    """Rotation matrix of a unit quaternion (w, x, y, z)."""
+    w, x, y, z = np.asarray(q, float) / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def sh_eval(oracle, c, l, LMAX, v):
+    """sum_m c_{l,m} Y_lm(v) for unit vector v, Y_lm = sphPlm(l,m,theta) e^{i m phi}
+    (sphPlm of the oracle: the reference's spherical_harmonics.jl:69-76)."""
+    th, ph = np.arccos(np.clip(v[2], -1, 1)), np.arctan2(v[1], v[0])
+    return sum(c[seanindex(l, m, LMAX) - 1] * oracle.sphPlm(l, m, th) * np.exp(1j * m * ph) for m in range(-l, l + 1))
+
+
+def c2_invariants(c, K, L, LMAX):
+    """sum_m conj(c_{k1,l,m}) c_{k2,l,m} for all k1 <= k2 <= K and even l <= L."""
+    out = []
+    for l in range(0, L + 1, 2):
+        idx = [seanindex(l, m, LMAX) - 1 for m in range(-l, l + 1)]
+        blk = c[:K, idx]
+        out.append(np.conj(blk) @ blk.T)
+    return np.array(out)
+
+
+def random_quats(rng, n):
+    q = rng.normal(size=(n, 4))
+    return q / np.linalg.norm(q, axis=1, keepdims=True)

@@ -14,6 +14,7 @@
 #include "../../include/tp.h"
 #include "tp_host_math.hpp"
 #include "tp_kernels.hpp"
+#include "tp_batch.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -55,6 +56,8 @@ struct tp_ctx {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double last_ms[4] = {0, 0, 0, 0};
     double *h_pin = nullptr;   // pinned host landing buffer for scalar results
+    tp_batch *eb = nullptr;    // plan (and graph) of the last mixed-precision tp_energy_batch
+    std::vector<const void *> eb_sig;
 };
 
 // contraction tiles
@@ -81,6 +84,7 @@ struct tp_basis {
     int nwork = 0, Lw = 0, EmaxLt = 0, Prows = 0;
     size_t mfma_lds = 0;
     int *d_work = nullptr;         // [nwork] lt | half << 16, most expensive first
+    std::vector<int> h_lt_cnt, h_lt_l3;   // per l-triple: basis entries, l3 (work lists of batches)
     // mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_BF16)
     int prec = TP_PREC_FP64;
     std::vector<double> h_H, h_Bm; // host copies [nlt_pad][nm_pad64], [nm_pad16][R_pad] for building F
@@ -201,9 +205,9 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
         const uint16_t *Fh = b->prec == TP_PREC_FP16X3 ? b->d_Fhi : b->d_Fbf;
 #define TP_G16(MODE, OUT, EN, OUTP)                                                                                \
-    hipLaunchKernelGGL((k_gemm16<MODE, OUT, EN>), grid16, dim3(256), 0, st, Fh, b->d_Flo, (int64_t)b->ldk, b->d_Ahi, \
-                       b->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, b->d_Ainv, (int)b->R, ncols, b->n_rt16,          \
-                       b->n_ct16, OUTP, b->R, qh, b->d_part, b->kl_pad)
+    hipLaunchKernelGGL((k_gemm16<H_BM, H_BN, MODE, OUT, EN, false>), grid16, dim3(256), 0, st, Fh, b->d_Flo,          \
+                       (int64_t)b->ldk, b->d_Ahi, b->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, b->d_Ainv, (int)b->R,  \
+                       ncols, b->n_rt16, b->n_ct16, OUTP, b->R, qh, b->d_part, b->kl_pad, nullptr, nullptr)
         if (b->prec == TP_PREC_FP16X3) {
             if (r && c3out) TP_G16(1, 1, true, c3out);
             else if (r) TP_G16(1, 0, true, nullptr);
@@ -267,6 +271,367 @@ static void record_timing(tp_ctx *ctx)
 }
 
 // ---------------------------------------------------------------------------
+// batches: many evaluations (uneven K, own histograms) scored by one fixed
+// sequence of launches, captured as a HIP graph
+// ---------------------------------------------------------------------------
+struct tp_batch {
+    tp_ctx *ctx = nullptr;
+    const tp_basis *b = nullptr;   // tables and F (same L, LMAX, N as every evaluation; K >= theirs)
+    int ngroups = 0, nprop = 0, ne = 0, nl = 0, hist_cap = 0;
+    bool sweep = false;            // annealing runs (rotation proposals) vs plain evaluations
+    std::vector<EvalDesc> h_ev;
+    std::vector<GroupState> h_gs;
+    std::vector<std::vector<std::pair<double, int2>>> items;   // per evaluation: (cost, {e, lt | h << 16})
+    int64_t ncb = 0, ldA = 0, cstride = 0;
+    int nwork = 0;
+    int bm = H_BM, bn = H_BN, order = 0;   // contraction tile, triple-product work order (autotuned)
+    double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    EvalDesc *d_ev = nullptr;
+    GroupState *d_gs = nullptr;
+    int2 *d_work = nullptr;
+    double2 *d_cur = nullptr, *d_prop = nullptr;
+    double *d_quat = nullptr, *d_As = nullptr, *d_sc = nullptr, *d_isc = nullptr, *d_part = nullptr;
+    double *d_E = nullptr, *d_Ehist = nullptr;
+    uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;
+    const double **d_qtab = nullptr;
+    int *d_nvalid = nullptr;
+    double2 *d_raw = nullptr;      // staging of host coefficients (plain evaluations)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double g_decay = 0.0, g_factor = 0.0;   // what the captured sweep graph was built with
+    std::vector<const tp_ref *> refs;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static void batch_free(tp_batch *B)
+{
+    if (!B) return;
+    if (B->ctx) (void)hipSetDevice(B->ctx->device);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    for (auto &e : B->ev)
+        if (e) (void)hipEventDestroy(e);
+    void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As,
+                    B->d_sc,  B->d_isc, B->d_part, B->d_E,    B->d_Ehist,  B->d_Ahi,    B->d_Alo,
+                    (void *)B->d_qtab, B->d_nvalid, B->d_raw};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete B;
+}
+
+// Work items of the MFMA triple products for an evaluation with K shells
+// (the per-(l-triple, k1 half) costs of tp_basis_create).
+static void batch_items(const tp_basis *b, int e, int K, std::vector<std::pair<double, int2>> &out)
+{
+    const int nh = K > 16 ? 2 : 1;
+    for (int h = 0; h < nh; ++h) {
+        const int k1lo = 16 * h + 1, k1hi = std::min(16 * h + 16, K);
+        const int P = k1hi * (k1hi + 1) / 2 - (k1lo - 1) * k1lo / 2;
+        const int nrb = (P + 15) / 16;
+        for (int t = 0; t < (int)b->n_lt; ++t) {
+            const int n3 = 2 * b->h_lt_l3[t] + 1;
+            const double cost = 2.0 * b->h_lt_cnt[t] + 8.0 * ((n3 + 7) / 8) * nrb;
+            out.push_back({cost, make_int2(e, t | (h << 16))});
+        }
+    }
+}
+
+// Upload the work list in the given order: 0 = all items by decreasing cost,
+// 1 = evaluation-major (an evaluation's items adjacent, by decreasing cost).
+static int batch_upload_work(tp_batch *B, int order)
+{
+    std::vector<std::pair<double, int2>> all;
+    for (auto &v : B->items) all.insert(all.end(), v.begin(), v.end());
+    auto cmp_cost = [](const std::pair<double, int2> &a, const std::pair<double, int2> &b) {
+        if (a.first != b.first) return a.first > b.first;
+        if (a.second.x != b.second.x) return a.second.x < b.second.x;
+        return a.second.y < b.second.y;
+    };
+    auto cmp_eval = [&](const std::pair<double, int2> &a, const std::pair<double, int2> &b) {
+        if (a.second.x != b.second.x) return a.second.x < b.second.x;
+        return cmp_cost(a, b);
+    };
+    if (order == 0) std::sort(all.begin(), all.end(), cmp_cost);
+    else std::sort(all.begin(), all.end(), cmp_eval);
+    std::vector<int2> w(all.size());
+    for (size_t i = 0; i < all.size(); ++i) w[i] = all[i].second;
+    if (!B->d_work)
+        if (int rc = dev_alloc(&B->d_work, w.size())) return rc;
+    TP_HIP(hipMemcpy(B->d_work, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice));
+    B->nwork = (int)w.size();
+    B->order = order;
+    return 0;
+}
+
+// Lay out ngroups groups of nprop evaluations each; group g has refs[g]
+// (its histogram; K = refs[g]->b->K).
+static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, const tp_ref *const *refs, bool sweep,
+                       int hist_cap, tp_batch **out)
+{
+    *out = nullptr;
+    if (!b->use_mfma) return tp_fail(TP_E_UNSUPPORTED, "batches need the MFMA triple products (K <= 27)");
+    if (b->prec == TP_PREC_FP64) return tp_fail(TP_E_UNSUPPORTED, "batched contraction runs in mixed precision");
+    if (b->R % 128 != 0) return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs 2N^2 to be a multiple of 128");
+    if (b->mfma_lds > 65536)
+        (void)hipFuncSetAttribute((const void *)k_triple_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)b->mfma_lds);
+    tp_batch *B = new tp_batch();
+    B->ctx = ctx;
+    B->b = b;
+    B->ngroups = ngroups;
+    B->nprop = nprop;
+    B->ne = ngroups * nprop;
+    B->nl = b->L / 2;
+    B->sweep = sweep;
+    B->hist_cap = hist_cap;
+    B->cstride = (int64_t)b->K * b->n_c;
+    B->refs.assign(refs, refs + ngroups);
+    // column blocks
+    std::vector<const double *> qtab;
+    std::vector<int> nvalid;
+    int64_t cb = 0;
+    for (int g = 0; g < ngroups; ++g) {
+        const tp_ref *r = refs[g];
+        const int K = r->b->K;
+        const int64_t kl = (int64_t)K * (K + 1) * (K + 2) / 6, nt = (kl + 63) / 64;
+        for (int p = 0; p < nprop; ++p) {
+            EvalDesc d;
+            std::memset(&d, 0, sizeof(d));
+            d.K = K;
+            d.ncols = (int)r->ncols;
+            d.cbase = (int)cb;
+            d.nprop = nprop;
+            d.p = p;
+            d.group = g;
+            d.qsum = r->d_qsum;
+            d.tf = r->d_tf;
+            B->h_ev.push_back(d);
+        }
+        for (int64_t j = 0; j < nt; ++j)
+            for (int p = 0; p < nprop; ++p) {
+                const int64_t nv = std::max<int64_t>(0, std::min<int64_t>(64, r->ncols - j * 64));
+                qtab.push_back(r->d_qhat + (nv > 0 ? j * 64 * b->R : 0));
+                nvalid.push_back((int)nv);
+            }
+        GroupState s;
+        std::memset(&s, 0, sizeof(s));
+        s.amax = 1.0;
+        s.K = K;
+        s.nprop = nprop;
+        s.e0 = g * nprop;
+        B->h_gs.push_back(s);
+        cb += nt * nprop;
+    }
+    while (cb % 2) {   // even block count: 128-column tiles
+        qtab.push_back(refs[0]->d_qhat);
+        nvalid.push_back(0);
+        ++cb;
+    }
+    B->ncb = cb;
+    B->ldA = cb * 64;
+    int rc = 0;
+    const size_t nrt_max = (size_t)(b->R_pad16 / 64);
+    if ((rc = dev_alloc(&B->d_ev, B->ne)) || (rc = dev_alloc(&B->d_gs, ngroups)) ||
+        (rc = dev_alloc(&B->d_prop, (size_t)B->ne * B->cstride)) ||
+        (sweep && (rc = dev_alloc(&B->d_cur, (size_t)ngroups * B->cstride))) ||
+        (rc = dev_alloc(&B->d_quat, (size_t)B->ne * std::max(1, B->nl) * 4)) ||
+        (rc = dev_alloc(&B->d_As, (size_t)b->n_lt * B->ldA)) || (rc = dev_alloc(&B->d_sc, B->ldA)) ||
+        (rc = dev_alloc(&B->d_isc, B->ldA)) || (rc = dev_alloc(&B->d_part, nrt_max * 2 * B->ldA)) ||
+        (rc = dev_alloc(&B->d_E, B->ne)) || (rc = dev_alloc(&B->d_Ehist, (size_t)std::max(1, hist_cap) * ngroups)) ||
+        (rc = dev_alloc(&B->d_Ahi, (size_t)B->ldA * b->ldk)) ||
+        (b->prec == TP_PREC_FP16X3 && (rc = dev_alloc(&B->d_Alo, (size_t)B->ldA * b->ldk))) ||
+        (rc = dev_alloc(&B->d_qtab, qtab.size())) || (rc = dev_alloc(&B->d_nvalid, nvalid.size())) ||
+        (!sweep && (rc = dev_alloc(&B->d_raw, (size_t)b->K * b->LMAX * b->LMAX)))) {
+        batch_free(B);
+        return rc;
+    }
+    for (int e = 0; e < B->ne; ++e) {
+        EvalDesc &d = B->h_ev[e];
+        d.coef = B->d_prop + (size_t)e * B->cstride;
+        d.src = sweep ? B->d_cur + (size_t)d.group * B->cstride : nullptr;
+    }
+    B->items.resize(B->ne);
+    for (int e = 0; e < B->ne; ++e) batch_items(b, e, B->h_ev[e].K, B->items[e]);
+    hipError_t e = hipMemcpy(B->d_ev, B->h_ev.data(), sizeof(EvalDesc) * B->ne, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_qtab, qtab.data(), sizeof(double *) * qtab.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(B->d_nvalid, nvalid.data(), sizeof(int) * nvalid.size(), hipMemcpyHostToDevice);
+    // As columns that no evaluation writes (tile padding) must read as zero
+    if (e == hipSuccess) e = hipMemset(B->d_As, 0, sizeof(double) * (size_t)b->n_lt * B->ldA);
+    if (e == hipSuccess) e = hipEventCreate(&B->ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&B->ev[1]);
+    if (e != hipSuccess) {
+        batch_free(B);
+        return tp_fail(TP_E_HIP, std::string("batch set-up: ") + hipGetErrorString(e));
+    }
+    if ((rc = batch_upload_work(B, 0))) {
+        batch_free(B);
+        return rc;
+    }
+    *out = B;
+    return 0;
+}
+
+template <int BM, int BN>
+static void batch_gemm_launch(const tp_batch *B, hipStream_t st)
+{
+    const tp_basis *b = B->b;
+    const int n_rt = (int)(b->R_pad16 / BM), n_ct = (int)(B->ldA / BN);
+    const dim3 grid((unsigned)(n_rt * n_ct));
+    if (b->prec == TP_PREC_FP16X3)
+        hipLaunchKernelGGL((k_gemm16<BM, BN, 1, 0, true, true>), grid, dim3(256), 0, st, b->d_Fhi, b->d_Flo,
+                           (int64_t)b->ldk, B->d_Ahi, B->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, B->d_isc, (int)b->R,
+                           0, n_rt, n_ct, nullptr, (int64_t)0, nullptr, B->d_part, B->ldA, B->d_qtab, B->d_nvalid);
+    else
+        hipLaunchKernelGGL((k_gemm16<BM, BN, 2, 0, true, true>), grid, dim3(256), 0, st, b->d_Fbf, nullptr,
+                           (int64_t)b->ldk, B->d_Ahi, nullptr, (int64_t)b->ldk, b->ldk, b->F_inv, B->d_isc, (int)b->R,
+                           0, n_rt, n_ct, nullptr, (int64_t)0, nullptr, B->d_part, B->ldA, B->d_qtab, B->d_nvalid);
+}
+
+static void batch_gemm(const tp_batch *B, hipStream_t st, int bm, int bn)
+{
+    if (bm == 128 && bn == 64) batch_gemm_launch<128, 64>(B, st);
+    else if (bm == 64 && bn == 64) batch_gemm_launch<64, 64>(B, st);
+    else if (bm == 128 && bn == 128) batch_gemm_launch<128, 128>(B, st);
+    else batch_gemm_launch<64, 128>(B, st);
+}
+
+static const int kTiles[4][2] = {{128, 64}, {64, 64}, {128, 128}, {64, 128}};
+
+// stages of one batched evaluation (coefficients already in place)
+static void batch_triple(const tp_batch *B, hipStream_t st)
+{
+    const tp_basis *b = B->b;
+    hipLaunchKernelGGL(k_colscale_batch, dim3((unsigned)B->ne), dim3(256), 0, st, B->d_ev, b->n_c, b->L, B->d_sc,
+                       B->d_isc);
+    hipLaunchKernelGGL(k_triple_batch, dim3((unsigned)B->nwork), dim3(512), b->mfma_lds, st, B->d_ev, B->d_work,
+                       b->n_c, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->Lw, b->EmaxLt, b->Prows, B->d_As,
+                       B->ldA);
+}
+
+static void batch_split(const tp_batch *B, hipStream_t st)
+{
+    const tp_basis *b = B->b;
+    const dim3 gs((unsigned)B->ncb, (unsigned)(b->ldk / 64));
+    if (b->prec == TP_PREC_FP16X3)
+        hipLaunchKernelGGL(k_split_batch<1>, gs, dim3(256), 0, st, B->d_As, B->ldA, (int)b->n_lt, b->ldk, B->d_sc,
+                           B->d_Ahi, B->d_Alo);
+    else
+        hipLaunchKernelGGL(k_split_batch<2>, gs, dim3(256), 0, st, B->d_As, B->ldA, (int)b->n_lt, b->ldk, B->d_sc,
+                           B->d_Ahi, nullptr);
+}
+
+static void batch_final(const tp_batch *B, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_final_batch, dim3((unsigned)B->ne), dim3(256), 0, st, B->d_ev, B->d_part,
+                       (int)(B->b->R_pad16 / B->bm), B->ldA, B->d_E);
+}
+
+static int batch_enqueue_eval(const tp_batch *B, hipStream_t st)
+{
+    batch_triple(B, st);
+    batch_split(B, st);
+    batch_gemm(B, st, B->bm, B->bn);
+    batch_final(B, st);
+    TP_HIP(hipGetLastError());
+    return 0;
+}
+
+// One sweep: proposals (random quaternions, rotation), evaluation, decision.
+static int batch_enqueue_sweep(const tp_batch *B, hipStream_t st, int probe, double decay, double factor, double tfac)
+{
+    const tp_basis *b = B->b;
+    const int nq = B->ne * B->nl;
+    hipLaunchKernelGGL(k_quat_random, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, B->d_ev, B->ne, B->nl,
+                       B->d_gs, probe, B->d_quat);
+    hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)B->nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev, B->d_quat,
+                       B->nl, b->n_c);
+    if (int rc = batch_enqueue_eval(B, st)) return rc;
+    hipLaunchKernelGGL(k_select_batch, dim3((unsigned)B->ngroups), dim3(256), 0, st, B->d_gs, B->d_ev, B->d_E,
+                       B->d_cur, B->cstride, b->n_c, probe, decay, factor, tfac, B->d_Ehist, B->hist_cap,
+                       B->ngroups);
+    TP_HIP(hipGetLastError());
+    return 0;
+}
+
+static int batch_capture(tp_batch *B, double decay, double factor)
+{
+    hipStream_t st = B->ctx->stream;
+    if (B->exec) {
+        TP_HIP(hipGraphExecDestroy(B->exec));
+        B->exec = nullptr;
+    }
+    if (B->graph) {
+        TP_HIP(hipGraphDestroy(B->graph));
+        B->graph = nullptr;
+    }
+    TP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = B->sweep ? batch_enqueue_sweep(B, st, 0, decay, factor, 0.0) : batch_enqueue_eval(B, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return tp_fail(TP_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    B->graph = g;
+    TP_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+    B->g_decay = decay;
+    B->g_factor = factor;
+    return 0;
+}
+
+// Time `reps` launches of a stage on the context stream (ms per launch).
+template <class F>
+static int batch_time(tp_batch *B, int reps, F &&f, double *ms)
+{
+    hipStream_t st = B->ctx->stream;
+    f(st);
+    TP_HIP(hipGetLastError());
+    TP_HIP(hipEventRecord(B->ev[0], st));
+    for (int i = 0; i < reps; ++i) f(st);
+    TP_HIP(hipEventRecord(B->ev[1], st));
+    TP_HIP(hipEventSynchronize(B->ev[1]));
+    TP_HIP(hipGetLastError());
+    float t = 0.f;
+    TP_HIP(hipEventElapsedTime(&t, B->ev[0], B->ev[1]));
+    *ms = t / reps;
+    return 0;
+}
+
+// Stage sizes of the current coefficients: choose the triple-product work
+// order and the contraction tile by timing every candidate on this batch.
+static int batch_autotune(tp_batch *B, int reps)
+{
+    hipStream_t st = B->ctx->stream;
+    double best = 1e30;
+    int best_order = 0;
+    for (int order = 0; order < 2; ++order) {
+        if (int rc = batch_upload_work(B, order)) return rc;
+        double ms = 0;
+        if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_triple(B, s); }, &ms)) return rc;
+        B->tune_ms[order] = ms;
+        if (ms < best) { best = ms; best_order = order; }
+    }
+    if (int rc = batch_upload_work(B, best_order)) return rc;
+    batch_triple(B, st);
+    batch_split(B, st);
+    best = 1e30;
+    int best_t = 0;
+    for (int t = 0; t < 4; ++t) {
+        double ms = 0;
+        if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_gemm(B, s, kTiles[t][0], kTiles[t][1]); }, &ms))
+            return rc;
+        B->tune_ms[2 + t] = ms;
+        if (ms < best) { best = ms; best_t = t; }
+    }
+    B->bm = kTiles[best_t][0];
+    B->bn = kTiles[best_t][1];
+    TP_HIP(hipStreamSynchronize(st));
+    if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -316,6 +681,7 @@ void tp_finalize(tp_ctx *ctx)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
+    batch_free(ctx->eb);
     delete ctx;
 }
 
@@ -600,6 +966,8 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
     }
     b->h_H = std::move(H);
     b->h_Bm = std::move(Bm);
+    b->h_lt_cnt = lt_entries;
+    for (auto &t : ltinfo) b->h_lt_l3.push_back(t.z);
     *out = b;
     return 0;
 }
@@ -1021,6 +1389,274 @@ int tp_intensity_from_atoms(tp_ctx *ctx, const double *atoms_xyz, int natoms, in
     cleanup();
     if (e != hipSuccess) return tp_fail(TP_E_HIP, std::string("intensity kernels: ") + hipGetErrorString(e));
     return 0;
+}
+
+// ---- batches ----------------------------------------------------------------
+static int batch_check_ref(const tp_basis *b, const tp_ref *r)
+{
+    if (!r || !r->b) return tp_fail(TP_E_ARG, "null histogram");
+    const tp_basis *rb = r->b;
+    if (rb->ctx != b->ctx) return tp_fail(TP_E_STATE, "histogram belongs to another context");
+    if (rb->L != b->L || rb->LMAX != b->LMAX || rb->N != b->N)
+        return tp_fail(TP_E_ARG, "batch members must share L, LMAX and N");
+    if (rb->K > b->K) return tp_fail(TP_E_ARG, "batch member has more shells than the batch basis");
+    return 0;
+}
+
+void tp_batch_destroy(tp_batch *B) { batch_free(B); }
+
+int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, const tp_ref *const *refs,
+                    const double *const *coeff, const int64_t *ld, const uint64_t *seeds, int hist_cap,
+                    tp_batch **out)
+{
+    if (!out) return tp_fail(TP_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!b || !refs || !coeff || !ld || !seeds) return tp_fail(TP_E_ARG, "null argument");
+    if (b->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
+    if (ngroups < 1 || nprop < 3 || nprop > 4096 || hist_cap < 0)
+        return tp_fail(TP_E_ARG, "need ngroups >= 1, 3 <= nprop <= 4096, hist_cap >= 0");
+    if (b->L > kRotMaxL || b->L < 2) return tp_fail(TP_E_UNSUPPORTED, "rotation proposals need 2 <= L <= 22");
+    for (int g = 0; g < ngroups; ++g)
+        if (int rc = batch_check_ref(b, refs[g])) return rc;
+    tp_batch *B = nullptr;
+    if (int rc = batch_build(ctx, b, ngroups, nprop, refs, true, hist_cap, &B)) return rc;
+    tp_basis *bm = const_cast<tp_basis *>(b);
+    for (int g = 0; g < ngroups; ++g) {
+        if (int rc = upload_coeff(ctx, bm, coeff[g], ld[g], B->h_gs[g].K, B->d_cur + (size_t)g * B->cstride,
+                                  bm->d_raw)) {
+            batch_free(B);
+            return rc;
+        }
+        B->h_gs[g].seed = seeds[g];
+    }
+    hipError_t e = hipMemcpyAsync(B->d_gs, B->h_gs.data(), sizeof(GroupState) * ngroups, hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        batch_free(B);
+        return tp_fail(TP_E_HIP, std::string("batch upload: ") + hipGetErrorString(e));
+    }
+    *out = B;
+    return 0;
+}
+
+int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out)
+{
+    if (!B || reps < 1) return tp_fail(TP_E_ARG, "null batch or reps < 1");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    hipStream_t st = B->ctx->stream;
+    if (B->sweep) {
+        // proposals of the current states (probe stream) so every stage sees real data
+        const int nq = B->ne * B->nl;
+        hipLaunchKernelGGL(k_quat_random, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, B->d_ev, B->ne,
+                           B->nl, B->d_gs, 1, B->d_quat);
+        hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)B->nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev,
+                           B->d_quat, B->nl, B->b->n_c);
+        TP_HIP(hipGetLastError());
+    }
+    if (int rc = batch_autotune(B, reps)) return rc;
+    if (ms_out)
+        for (int i = 0; i < 6; ++i) ms_out[i] = B->tune_ms[i];
+    if (config_out) {
+        config_out[0] = B->bm;
+        config_out[1] = B->bn;
+        config_out[2] = B->order;
+    }
+    return 0;
+}
+
+int tp_batch_probe(tp_batch *B, double temperature_factor)
+{
+    if (!B || !B->sweep) return tp_fail(TP_E_ARG, "null batch or not an annealing batch");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    if (int rc = batch_enqueue_sweep(B, B->ctx->stream, 1, 1.0, 1.0, temperature_factor)) return rc;
+    TP_HIP(hipStreamSynchronize(B->ctx->stream));
+    return 0;
+}
+
+int tp_batch_run(tp_batch *B, int nsweeps, double decay, double factor)
+{
+    if (!B || !B->sweep || nsweeps < 0) return tp_fail(TP_E_ARG, "null batch, not an annealing batch or nsweeps < 0");
+    if (!(decay > 0.0) || !(factor > 0.0)) return tp_fail(TP_E_ARG, "decay and factor must be positive");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    if (!B->exec || decay != B->g_decay || factor != B->g_factor)
+        if (int rc = batch_capture(B, decay, factor)) return rc;
+    for (int i = 0; i < nsweeps; ++i) TP_HIP(hipGraphLaunch(B->exec, B->ctx->stream));
+    TP_HIP(hipStreamSynchronize(B->ctx->stream));
+    return 0;
+}
+
+int tp_batch_group(tp_batch *B, int g, double *E, double *T, double *amax, int32_t *nacc, int32_t *nsweep)
+{
+    if (!B || g < 0 || g >= B->ngroups) return tp_fail(TP_E_ARG, "null batch or group out of range");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    GroupState s;
+    TP_HIP(hipMemcpy(&s, B->d_gs + g, sizeof(GroupState), hipMemcpyDeviceToHost));
+    if (E) *E = s.E;
+    if (T) *T = s.T;
+    if (amax) *amax = s.amax;
+    if (nacc) *nacc = s.nacc;
+    if (nsweep) *nsweep = s.nsweep;
+    return 0;
+}
+
+int tp_batch_coeff(tp_batch *B, int g, double *coeff, int64_t ld)
+{
+    if (!B || !B->sweep || g < 0 || g >= B->ngroups || !coeff) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    const tp_basis *b = B->b;
+    if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "ld shorter than LMAX^2");
+    const int K = B->h_gs[g].K;
+    std::vector<double2> h((size_t)K * b->n_c);
+    TP_HIP(hipMemcpy(h.data(), B->d_cur + (size_t)g * B->cstride, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < K; ++k)
+        for (int c = 0; c < b->n_c; ++c) {
+            double *d = coeff + 2 * ((int64_t)k * ld + b->sean0[c]);
+            d[0] = h[(size_t)k * b->n_c + c].x;
+            d[1] = h[(size_t)k * b->n_c + c].y;
+        }
+    return 0;
+}
+
+int tp_batch_history(tp_batch *B, double *Ehist, int n)
+{
+    if (!B || !Ehist || n < 0) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    const int m = std::min(n, B->hist_cap);
+    if (m > 0)
+        TP_HIP(hipMemcpy(Ehist, B->d_Ehist, sizeof(double) * (size_t)m * B->ngroups, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int tp_batch_eval_quats(tp_batch *B, const double *quat, double *E_out)
+{
+    if (!B || !B->sweep || !quat || !E_out) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    hipStream_t st = B->ctx->stream;
+    TP_HIP(hipMemcpyAsync(B->d_quat, quat, sizeof(double) * (size_t)B->ne * B->nl * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)B->nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev, B->d_quat,
+                       B->nl, B->b->n_c);
+    TP_HIP(hipGetLastError());
+    if (int rc = batch_enqueue_eval(B, st)) return rc;
+    TP_HIP(hipMemcpyAsync(E_out, B->d_E, sizeof(double) * B->ne, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_ref *const *refs,
+                    const double *const *coeff, const int64_t *ld, double *E_out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (n < 1 || !bases || !refs || !coeff || !ld || !E_out) return tp_fail(TP_E_ARG, "bad argument");
+    int im = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!bases[i] || !refs[i]) return tp_fail(TP_E_ARG, "null basis or histogram");
+        if (bases[i]->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
+        if (refs[i]->b != bases[i]) return tp_fail(TP_E_STATE, "histogram was built on another basis");
+        if (bases[i]->K > bases[im]->K) im = i;
+    }
+    const tp_basis *bm = bases[im];
+    for (int i = 0; i < n; ++i) {
+        if (int rc = batch_check_ref(bm, refs[i])) return rc;
+        if (bases[i]->prec != bm->prec) return tp_fail(TP_E_ARG, "batch members must share the precision mode");
+    }
+    hipStream_t st = ctx->stream;
+    if (bm->prec == TP_PREC_FP64 || !bm->use_mfma || bm->R % 128 != 0) {
+        // one evaluation after another on the device
+        double *dE = nullptr;
+        if (int rc = dev_alloc(&dE, n)) return rc;
+        int rc = 0;
+        for (int i = 0; i < n && !rc; ++i) {
+            tp_basis *b = const_cast<tp_basis *>(bases[i]);
+            rc = upload_coeff(ctx, b, coeff[i], ld[i], b->K, b->d_coef, b->d_raw);
+            if (!rc) rc = enqueue_eval(ctx, b, refs[i], b->d_coef, nullptr, nullptr, 0, nullptr, dE + i, mc_none());
+            if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = tp_fail(TP_E_HIP, "evaluation failed");
+        }
+        if (!rc && hipMemcpy(E_out, dE, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = tp_fail(TP_E_HIP, "hipMemcpy failed");
+        (void)hipFree(dE);
+        return rc;
+    }
+    std::vector<const void *> sig;
+    sig.push_back(bm);
+    for (int i = 0; i < n; ++i) sig.push_back(refs[i]);
+    bool fresh = false;
+    if (!ctx->eb || ctx->eb_sig != sig) {
+        batch_free(ctx->eb);
+        ctx->eb = nullptr;
+        ctx->eb_sig.clear();
+        tp_batch *B = nullptr;
+        if (int rc = batch_build(ctx, bm, n, 1, refs, false, 0, &B)) return rc;
+        ctx->eb = B;
+        ctx->eb_sig = sig;
+        fresh = true;
+    }
+    tp_batch *B = ctx->eb;
+    tp_basis *bmm = const_cast<tp_basis *>(bm);
+    for (int i = 0; i < n; ++i)
+        if (int rc = upload_coeff(ctx, bmm, coeff[i], ld[i], B->h_ev[i].K, B->d_prop + (size_t)i * B->cstride,
+                                  B->d_raw))
+            return rc;
+    if (fresh) {
+        if (int rc = batch_autotune(B, 2)) return rc;
+        if (int rc = batch_capture(B, 0.0, 0.0)) return rc;
+    }
+    TP_HIP(hipGraphLaunch(B->exec, st));
+    TP_HIP(hipMemcpyAsync(E_out, B->d_E, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int tp_rotate(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, int nshell, const double *quat,
+              double *out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!bc || !coeff || !quat || !out || nshell < 1) return tp_fail(TP_E_ARG, "bad argument");
+    if (bc->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
+    if (bc->L > kRotMaxL) return tp_fail(TP_E_UNSUPPORTED, "rotation needs L <= 22");
+    if (ld < (int64_t)bc->LMAX * bc->LMAX) return tp_fail(TP_E_ARG, "ld shorter than LMAX^2");
+    tp_basis *b = const_cast<tp_basis *>(bc);
+    const int nl = b->L / 2;
+    std::memcpy(out, coeff, sizeof(double) * 2 * (size_t)nshell * ld);
+    if (nl == 0) return 0;
+    double2 *src = nullptr, *dst = nullptr;
+    double *dq = nullptr;
+    EvalDesc *dev = nullptr;
+    int rc = 0;
+    if ((rc = dev_alloc(&src, (size_t)nshell * b->n_c)) || (rc = dev_alloc(&dst, (size_t)nshell * b->n_c)) ||
+        (rc = dev_alloc(&dq, (size_t)nl * 4)) || (rc = dev_alloc(&dev, 1))) {
+        (void)hipFree(src); (void)hipFree(dst); (void)hipFree(dq); (void)hipFree(dev);
+        return rc;
+    }
+    EvalDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.src = src;
+    d.coef = dst;
+    d.K = nshell;
+    rc = upload_coeff(ctx, b, coeff, ld, nshell, src, nullptr);
+    std::vector<double2> h((size_t)nshell * b->n_c);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(dq, quat, sizeof(double) * nl * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(dev, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)nl, 1u), dim3(256), 0, ctx->stream, dev, dq, nl, b->n_c);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(h.data(), dst, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = tp_fail(TP_E_HIP, std::string("tp_rotate: ") + hipGetErrorString(e));
+    }
+    if (!rc)
+        for (int k = 0; k < nshell; ++k)
+            for (int c = 0; c < b->n_c; ++c) {
+                double *o = out + 2 * ((int64_t)k * ld + b->sean0[c]);
+                o[0] = h[(size_t)k * b->n_c + c].x;
+                o[1] = h[(size_t)k * b->n_c + c].y;
+            }
+    (void)hipFree(src); (void)hipFree(dst); (void)hipFree(dq); (void)hipFree(dev);
+    return rc;
 }
 
 }  // extern "C"

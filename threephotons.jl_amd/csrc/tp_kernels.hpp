@@ -229,17 +229,21 @@ __device__ __forceinline__ S1Ops s1_fetch(int ks, int E, int s, int K, int k1s, 
     return o;
 }
 
-__global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
-                                                     const int *__restrict__ sel, int flip, int n_c, int K,
-                                                     const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
-                                                     const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
-                                                     const int *__restrict__ work, int Lw, int Emax, int Prows,
-                                                     double *__restrict__ As, int64_t kl_pad)
+// Column of k-triple kt in an As row: kt-tile j = kt / 64 of an evaluation
+// sits in 64-column block cbase + j * nprop + p (evaluations of one batch
+// group interleaved by tile; a single evaluation has cbase = 0, nprop = 1,
+// p = 0, i.e. column kt).
+__device__ __forceinline__ int64_t as_col(int64_t kt, int cbase, int nprop, int p)
 {
-    extern __shared__ double2 sm[];
-    const double2 *coef = pick(cb0, cb1, sel, flip);
-    const int wk = work[blockIdx.x];
-    const int lt = wk & 0xffff, h = wk >> 16;
+    return ((int64_t)cbase + (kt >> 6) * nprop + p) * 64 + (kt & 63);
+}
+
+__device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K, int lt,
+                                                 int h, const double *__restrict__ ew,
+                                                 const uint32_t *__restrict__ eidx, const int *__restrict__ gstart,
+                                                 const int4 *__restrict__ ltinfo, int Lw, int Emax, int Prows,
+                                                 double *__restrict__ As, int64_t ldA, int cbase, int nprop, int p0)
+{
     const int4 li = ltinfo[lt];
     const int l3 = li.z, l1 = li.w & 0xff, l2 = (li.w >> 8) & 0xff;
     const int n1 = 2 * l1 + 1, n2 = 2 * l2 + 1, n3 = 2 * l3 + 1;
@@ -363,12 +367,25 @@ __global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__
                     const int k1 = pk[p] & 0xff, k2 = pk[p] >> 8;
                     if (k3 <= k2) {
                         const int64_t kt = (int64_t)(k1 - 1) * k1 * (k1 + 1) / 6 + (int64_t)(k2 - 1) * k2 / 2 + (k3 - 1);
-                        As[(int64_t)lt * kl_pad + kt] = acc2[j][v];
+                        As[(int64_t)lt * ldA + as_col(kt, cbase, nprop, p0)] = acc2[j][v];
                     }
                 }
             }
         }
     }
+}
+
+__global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
+                                                     const int *__restrict__ sel, int flip, int n_c, int K,
+                                                     const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
+                                                     const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
+                                                     const int *__restrict__ work, int Lw, int Emax, int Prows,
+                                                     double *__restrict__ As, int64_t kl_pad)
+{
+    extern __shared__ double2 sm[];
+    const int wk = work[blockIdx.x];
+    triple_mfma_body(sm, pick(cb0, cb1, sel, flip), n_c, K, wk & 0xffff, wk >> 16, ew, eidx, gstart, ltinfo, Lw, Emax,
+                     Prows, As, kl_pad, 0, 1, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -623,35 +640,40 @@ __global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As,
 }
 
 // C[r][kt] = sum_k A[r][k] B[kt][k] (both operands k-contiguous, 16-bit),
-// 128 x 64 tile per workgroup of 4 waves (2 x 2, 64 x 32 each), K staged
-// through LDS 32 at a time (rows padded to 40 halves: conflict-free 16-byte
-// reads), next slice prefetched into registers.  MFMA
+// BM x BN tile (BM, BN in {64, 128}) per workgroup of 4 waves (2 x 2), K
+// staged through LDS 32 at a time (rows padded to 40 halves: conflict-free
+// 16-byte reads), next slice prefetched into registers.  MFMA
 // v_mfma_f32_16x16x32_{f16,bf16}: lane holds A[l%16][8(l/16)..+7],
 // B[8(l/16)..+7][l%16], accumulator rows 4(l/16)+v (measured on gfx950).
 // Epilogue as k_gemm: raw c3 (OUT = 1) and/or per-column partial energy sums.
+// BATCH: the columns are 64-column blocks of many evaluations; block cb reads
+// its histogram columns from qtab[cb] (nvalid[cb] of them are real), and the
+// tiles run column-tile-major so the row tiles of one column tile share an
+// XCD's L2.
 constexpr int H_BM = 128, H_BN = 64, H_BK = 32, H_LDS_ROW = 40;
 
-template <int MODE, int OUT, bool ENERGY>
+template <int BM, int BN, int MODE, int OUT, bool ENERGY, bool BATCH>
 __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo,
                                                 int64_t lda, const uint16_t *__restrict__ Bhi,
                                                 const uint16_t *__restrict__ Blo, int64_t ldb, int Kdim,
                                                 double a_inv, const double *__restrict__ b_inv, int M, int ncols,
                                                 int n_rt, int n_ct, double *__restrict__ out, int64_t ldo,
                                                 const double *__restrict__ qhat, double *__restrict__ part,
-                                                int64_t ldp)
+                                                int64_t ldp, const double *const *__restrict__ qtab,
+                                                const int *__restrict__ nvalid)
 {
     constexpr int NP = MODE == 1 ? 2 : 1;   // hi (+ lo) planes
-    constexpr int WTM = H_BM / 2, WTN = H_BN / 2, RB = WTM / 16, CB = WTN / 16;
-    constexpr int QA = H_BM * 4 / 256, QB = H_BN * 4 / 256;   // 16-byte chunks per thread per plane
-    __shared__ uint16_t sA[NP][H_BM][H_LDS_ROW];
-    __shared__ uint16_t sB[NP][H_BN][H_LDS_ROW];
-    __shared__ double red[2][2][H_BN];
+    constexpr int WTM = BM / 2, WTN = BN / 2, RB = WTM / 16, CB = WTN / 16;
+    constexpr int QA = BM * 4 / 256, QB = BN * 4 / 256;   // 16-byte chunks per thread per plane
+    __shared__ uint16_t sA[NP][BM][H_LDS_ROW];
+    __shared__ uint16_t sB[NP][BN][H_LDS_ROW];
+    __shared__ double red[2][2][BN];
 
     const int nblk = n_rt * n_ct;
     int lin = blockIdx.x;
     if ((nblk & 7) == 0) lin = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
-    const int rt = lin / n_ct, ct = lin % n_ct;
-    const int r0 = rt * H_BM, c0 = ct * H_BN;
+    const int rt = BATCH ? lin % n_rt : lin / n_ct, ct = BATCH ? lin / n_rt : lin % n_ct;
+    const int r0 = rt * BM, c0 = ct * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lq = lane >> 4;
@@ -741,7 +763,15 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
     for (int j = 0; j < CB; ++j) {
         const int colt = wn * WTN + j * 16 + l16;   // column within the tile
         const int col = c0 + colt;
-        const bool cok = col < ncols;
+        bool cok;
+        const double *qcol = nullptr;
+        if (BATCH) {
+            cok = (col & 63) < nvalid[col >> 6];
+            if (ENERGY && cok) qcol = qtab[col >> 6] + (int64_t)(col & 63) * M;
+        } else {
+            cok = col < ncols;
+            if (ENERGY) qcol = qhat + (int64_t)col * M;
+        }
         const double sc = cok ? a_inv * b_inv[col] : 0.0;
         double sp = 0.0, sq = 0.0;
 #pragma unroll
@@ -755,7 +785,7 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
                     if (OUT == 1) out[(int64_t)col * ldo + r] = c;
                     if (ENERGY) {
                         const double pv = c > 1e-30 ? c : 1e-30;
-                        const double qv = qhat[(int64_t)col * M + r];
+                        const double qv = qcol[r];
                         sp += pv;
                         sq += qv * log(pv);
                     }
@@ -775,7 +805,7 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
     }
     if (ENERGY) {
         __syncthreads();
-        if (tid < H_BN) {
+        if (tid < BN) {
             part[((int64_t)rt * 2 + 0) * ldp + c0 + tid] = red[0][0][tid] + red[1][0][tid];
             part[((int64_t)rt * 2 + 1) * ldp + c0 + tid] = red[0][1][tid] + red[1][1][tid];
         }

@@ -274,3 +274,105 @@ def intensity_from_atoms(atoms, KMAX, qmax, LMAX, lfill, ntheta, nphi, ctx=None)
     check(lib().tp_intensity_from_atoms(ctx.h, atoms.ctypes.data, atoms.shape[0], KMAX, dq, LMAX, lfill, ntheta,
                                         nphi, out.ctypes.data), "tp_intensity_from_atoms")
     return SphericalHarmonicsVolume(out, LMAX, KMAX, qmax)
+
+
+def rotate(intensity, basis, quats, nshell=None):
+    """Rotate the even l >= 2 blocks of every shell by one unit quaternion per l
+    (quats shape (L/2, 4), w,x,y,z; a single (4,) quaternion is used for every l),
+    in FP64 through the real Wigner-D recurrence.  Returns the rotated (nshell, ld)
+    coefficient array; f'(r) = f(R^-1 r)."""
+    c, ld = _coeff_ptr(intensity, basis, nshell)
+    nshell = c.shape[0] if nshell is None else nshell
+    q = np.asarray(quats, dtype=np.float64)
+    if q.ndim == 1:
+        q = np.tile(q, (basis.L // 2, 1))
+    q = np.ascontiguousarray(q.reshape(basis.L // 2, 4))
+    out = np.zeros_like(c[:nshell])
+    check(lib().tp_rotate(basis.ctx.h, basis.h, c.ctypes.data, ld, nshell, q.ctypes.data, out.ctypes.data),
+          "tp_rotate")
+    return out
+
+
+def energy_batch(intensities, bases, c3refs, K3_ranges=None):
+    """Bayes energies of many evaluations in one call (tp_energy_batch): one per
+    (intensity, basis, c3ref); bases share L, LMAX, N and precision, K may differ."""
+    n = len(intensities)
+    K3_ranges = K3_ranges or [None] * n
+    refs = (C.c_void_p * n)(*[bases[i].ref(c3refs[i], K3_ranges[i]).value for i in range(n)])
+    hb = (C.c_void_p * n)(*[b.h.value for b in bases])
+    arrs = [_coeff_ptr(intensities[i], bases[i]) for i in range(n)]
+    ptrs = (C.c_void_p * n)(*[a[0].ctypes.data for a in arrs])
+    lds = (C.c_int64 * n)(*[a[1] for a in arrs])
+    E = np.zeros(n)
+    check(lib().tp_energy_batch(bases[0].ctx.h, n, hb, refs, ptrs, lds, E.ctypes.data), "tp_energy_batch")
+    return E
+
+
+class AnnealBatch:
+    """Concurrent annealing runs scored together (tp_batch_*): run g has its own
+    basis bases[g] (shell count K_g), histogram c3refs[g], start coefficients
+    starts[g] and seed; every sweep scores nprop rotation proposals per run in one
+    graph replay.  `basis` (mixed precision, K >= every K_g) holds the tables."""
+
+    def __init__(self, basis, bases, c3refs, starts, seeds, nprop=64, hist_cap=0):
+        self.basis, self.bases, self.nprop = basis, list(bases), nprop
+        self.ngroups = n = len(bases)
+        refs = (C.c_void_p * n)(*[bases[g].ref(c3refs[g]).value for g in range(n)])
+        arrs = [_coeff_ptr(starts[g], bases[g]) for g in range(n)]
+        self._starts = [a[0][:bases[g].K].copy() for g, a in enumerate(arrs)]
+        ptrs = (C.c_void_p * n)(*[a[0].ctypes.data for a in arrs])
+        lds = (C.c_int64 * n)(*[a[1] for a in arrs])
+        self._lds = [a[1] for a in arrs]
+        sd = (C.c_uint64 * n)(*[int(s) for s in seeds])
+        h = C.c_void_p()
+        check(lib().tp_batch_create(basis.ctx.h, basis.h, n, nprop, refs, ptrs, lds, sd, hist_cap, C.byref(h)),
+              "tp_batch_create")
+        self.h, self.hist_cap = h, hist_cap
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().tp_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def autotune(self, reps=3):
+        ms = np.zeros(6)
+        cfg = np.zeros(3, np.int32)
+        check(lib().tp_batch_autotune(self.h, reps, ms.ctypes.data, cfg.ctypes.data), "tp_batch_autotune")
+        return dict(order_ms=ms[:2].tolist(), tile_ms=dict(zip(["128x64", "64x64", "128x128", "64x128"],
+                                                                ms[2:].tolist())),
+                    tile=(int(cfg[0]), int(cfg[1])), order=int(cfg[2]))
+
+    def probe(self, temperature_factor=0.1):
+        check(lib().tp_batch_probe(self.h, float(temperature_factor)), "tp_batch_probe")
+
+    def run(self, nsweeps, decay=0.99998, factor=1.01):
+        check(lib().tp_batch_run(self.h, int(nsweeps), float(decay), float(factor)), "tp_batch_run")
+
+    def group(self, g):
+        E, T, a = C.c_double(), C.c_double(), C.c_double()
+        na, ns = C.c_int32(), C.c_int32()
+        check(lib().tp_batch_group(self.h, g, C.byref(E), C.byref(T), C.byref(a), C.byref(na), C.byref(ns)),
+              "tp_batch_group")
+        return dict(E=E.value, T=T.value, amax=a.value, nacc=na.value, nsweep=ns.value)
+
+    def coeff(self, g):
+        out = self._starts[g].copy()
+        check(lib().tp_batch_coeff(self.h, g, out.ctypes.data, self._lds[g]), "tp_batch_coeff")
+        return out
+
+    def history(self, n=None):
+        n = self.hist_cap if n is None else min(n, self.hist_cap)
+        out = np.zeros((max(n, 0), self.ngroups))
+        if n > 0:
+            check(lib().tp_batch_history(self.h, out.ctypes.data, n), "tp_batch_history")
+        return out
+
+    def eval_quats(self, quats):
+        q = np.ascontiguousarray(np.asarray(quats, dtype=np.float64).reshape(self.ngroups * self.nprop,
+                                                                             self.basis.L // 2, 4))
+        E = np.zeros(self.ngroups * self.nprop)
+        check(lib().tp_batch_eval_quats(self.h, q.ctypes.data, E.ctypes.data), "tp_batch_eval_quats")
+        return E

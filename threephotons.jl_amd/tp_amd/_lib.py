@@ -73,6 +73,18 @@ def lib():
     L.tp_basis_get_precision.argtypes = [vp]
     L.tp_set_timing.argtypes = [vp, i32]
     L.tp_last_timings.argtypes = [vp, C.POINTER(dbl), i32]
+    L.tp_energy_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.tp_batch_create.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, i32, pp]
+    L.tp_batch_destroy.argtypes = [vp]
+    L.tp_batch_destroy.restype = None
+    L.tp_batch_autotune.argtypes = [vp, i32, vp, vp]
+    L.tp_batch_probe.argtypes = [vp, dbl]
+    L.tp_batch_run.argtypes = [vp, i32, dbl, dbl]
+    L.tp_batch_group.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.tp_batch_coeff.argtypes = [vp, i32, vp, i64]
+    L.tp_batch_history.argtypes = [vp, vp, i32]
+    L.tp_batch_eval_quats.argtypes = [vp, vp, vp]
+    L.tp_rotate.argtypes = [vp, vp, vp, i64, i32, vp, vp]
     _lib = L
     return L
 
@@ -101,7 +113,9 @@ EXPORTS = [
     "tp_basis_info", "tp_c3", "tp_ref_create", "tp_ref_destroy", "tp_energy", "tp_state_create",
     "tp_state_destroy", "tp_state_read", "tp_state_energy", "tp_state_c3", "tp_anneal",
     "tp_intensity_from_atoms", "tp_set_timing", "tp_last_timings", "tp_basis_set_precision",
-    "tp_basis_get_precision",
+    "tp_basis_get_precision", "tp_energy_batch", "tp_batch_create", "tp_batch_destroy", "tp_batch_autotune",
+    "tp_batch_probe", "tp_batch_run", "tp_batch_group", "tp_batch_coeff", "tp_batch_history",
+    "tp_batch_eval_quats", "tp_rotate",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16 = 0, 1, 2
