@@ -155,9 +155,10 @@ int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, cons
 int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_ref *const *refs,
                     const double *const *coeff, const int64_t *ld, double *E_out);
 
-/* Concurrent annealing runs ("groups"), each with its own shell count
- * K_g = refs[g]'s basis K <= b's K, histogram refs[g], start coefficients
- * coeff[g] (K_g shells) and seed.  Every sweep proposes nprop (>= 3)
+/* Concurrent annealing runs ("groups"), each with its own histogram refs[g]
+ * (built on any basis sharing b's L, LMAX, N), shell count K_g = the upper end
+ * of refs[g]'s K3_range (<= b's K), start coefficients coeff[g] (K_g shells)
+ * and seed.  Every sweep proposes nprop (>= 3)
  * rotations per run: for each even l >= 2 an independent SO(3) rotation
  * (uniform axis, Haar-distributed angle <= amax * pi/4) applied to the
  * l-blocks of all shells through FP64 real Wigner-D matrices, which keeps

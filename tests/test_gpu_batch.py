@@ -106,7 +106,7 @@ def _anneal_setup(tp, Ks, nprop, hist_cap=0, L=8, LMAX=25, N=8):
     refs = [_hist(tp, truth[i], L, LMAX, N, K, 20 + i) for i, K in enumerate(Ks)]
     rng = np.random.default_rng(4)
     starts = [tp.rotate(truth[i], bases[i], random_quats(rng, L // 2)) for i in range(len(Ks))]
-    B = tp.AnnealBatch(master, bases, refs, starts, seeds=[1000 + g for g in range(len(Ks))], nprop=nprop,
+    B = tp.AnnealBatch(master, refs, starts, seeds=[1000 + g for g in range(len(Ks))], nprop=nprop,
                        hist_cap=hist_cap)
     return master, bases, refs, starts, B
 

@@ -119,6 +119,7 @@ struct tp_basis {
 
 struct tp_ref {
     const tp_basis *b = nullptr;
+    int klo = 1, khi = 1;      // K3_range of the walk
     int64_t ncols = 0;
     double *d_qhat = nullptr;  // [ncols][R]
     double *d_qsum = nullptr;  // [ncols]
@@ -365,8 +366,8 @@ static int batch_upload_work(tp_batch *B, int order)
 
 // Lay out ngroups groups of nprop evaluations each; group g has refs[g]
 // (its histogram; K = refs[g]->b->K).
-static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, const tp_ref *const *refs, bool sweep,
-                       int hist_cap, tp_batch **out)
+static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, const tp_ref *const *refs,
+                       const int *Kg, bool sweep, int hist_cap, tp_batch **out)
 {
     *out = nullptr;
     if (!b->use_mfma) return tp_fail(TP_E_UNSUPPORTED, "batches need the MFMA triple products (K <= 27)");
@@ -392,7 +393,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     int64_t cb = 0;
     for (int g = 0; g < ngroups; ++g) {
         const tp_ref *r = refs[g];
-        const int K = r->b->K;
+        const int K = Kg[g];
         const int64_t kl = (int64_t)K * (K + 1) * (K + 2) / 6, nt = (kl + 63) / 64;
         for (int p = 0; p < nprop; ++p) {
             EvalDesc d;
@@ -1104,6 +1105,8 @@ int tp_ref_create(tp_ctx *ctx, const tp_basis *b, const double *c3ref, int Kr, i
     }
     tp_ref *r = new tp_ref();
     r->b = b;
+    r->klo = k3lo;
+    r->khi = k3hi;
     r->ncols = ncols;
     if (dev_alloc(&r->d_qhat, qhat.size()) || dev_alloc(&r->d_qsum, ncols) || dev_alloc(&r->d_tf, ncols)) {
         tp_ref_destroy(r);
@@ -1392,14 +1395,16 @@ int tp_intensity_from_atoms(tp_ctx *ctx, const double *atoms_xyz, int natoms, in
 }
 
 // ---- batches ----------------------------------------------------------------
-static int batch_check_ref(const tp_basis *b, const tp_ref *r)
+// r must be usable with b's tables for an evaluation with K shells
+static int batch_check_ref(const tp_basis *b, const tp_ref *r, int K)
 {
     if (!r || !r->b) return tp_fail(TP_E_ARG, "null histogram");
     const tp_basis *rb = r->b;
     if (rb->ctx != b->ctx) return tp_fail(TP_E_STATE, "histogram belongs to another context");
     if (rb->L != b->L || rb->LMAX != b->LMAX || rb->N != b->N)
         return tp_fail(TP_E_ARG, "batch members must share L, LMAX and N");
-    if (rb->K > b->K) return tp_fail(TP_E_ARG, "batch member has more shells than the batch basis");
+    if (K > b->K) return tp_fail(TP_E_ARG, "batch member has more shells than the batch basis");
+    if (r->ncols > (int64_t)K * (K + 1) * (K + 2) / 6) return tp_fail(TP_E_ARG, "histogram walk longer than K allows");
     return 0;
 }
 
@@ -1417,10 +1422,14 @@ int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, cons
     if (ngroups < 1 || nprop < 3 || nprop > 4096 || hist_cap < 0)
         return tp_fail(TP_E_ARG, "need ngroups >= 1, 3 <= nprop <= 4096, hist_cap >= 0");
     if (b->L > kRotMaxL || b->L < 2) return tp_fail(TP_E_UNSUPPORTED, "rotation proposals need 2 <= L <= 22");
-    for (int g = 0; g < ngroups; ++g)
-        if (int rc = batch_check_ref(b, refs[g])) return rc;
+    std::vector<int> Kg(ngroups);
+    for (int g = 0; g < ngroups; ++g) {
+        if (!refs[g]) return tp_fail(TP_E_ARG, "null histogram");
+        Kg[g] = refs[g]->khi;   // shells that enter the run's energy
+        if (int rc = batch_check_ref(b, refs[g], Kg[g])) return rc;
+    }
     tp_batch *B = nullptr;
-    if (int rc = batch_build(ctx, b, ngroups, nprop, refs, true, hist_cap, &B)) return rc;
+    if (int rc = batch_build(ctx, b, ngroups, nprop, refs, Kg.data(), true, hist_cap, &B)) return rc;
     tp_basis *bm = const_cast<tp_basis *>(b);
     for (int g = 0; g < ngroups; ++g) {
         if (int rc = upload_coeff(ctx, bm, coeff[g], ld[g], B->h_gs[g].K, B->d_cur + (size_t)g * B->cstride,
@@ -1557,8 +1566,10 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
         if (bases[i]->K > bases[im]->K) im = i;
     }
     const tp_basis *bm = bases[im];
+    std::vector<int> Kb(n);
     for (int i = 0; i < n; ++i) {
-        if (int rc = batch_check_ref(bm, refs[i])) return rc;
+        Kb[i] = bases[i]->K;
+        if (int rc = batch_check_ref(bm, refs[i], Kb[i])) return rc;
         if (bases[i]->prec != bm->prec) return tp_fail(TP_E_ARG, "batch members must share the precision mode");
     }
     hipStream_t st = ctx->stream;
@@ -1587,7 +1598,7 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
         ctx->eb = nullptr;
         ctx->eb_sig.clear();
         tp_batch *B = nullptr;
-        if (int rc = batch_build(ctx, bm, n, 1, refs, false, 0, &B)) return rc;
+        if (int rc = batch_build(ctx, bm, n, 1, refs, Kb.data(), false, 0, &B)) return rc;
         ctx->eb = B;
         ctx->eb_sig = sig;
         fresh = true;

@@ -310,16 +310,18 @@ def energy_batch(intensities, bases, c3refs, K3_ranges=None):
 
 class AnnealBatch:
     """Concurrent annealing runs scored together (tp_batch_*): run g has its own
-    basis bases[g] (shell count K_g), histogram c3refs[g], start coefficients
-    starts[g] and seed; every sweep scores nprop rotation proposals per run in one
-    graph replay.  `basis` (mixed precision, K >= every K_g) holds the tables."""
+    histogram c3refs[g] (shape (N, 2N, K_g, K_g, K_g); shell count K_g), start
+    coefficients starts[g] and seed; every sweep scores nprop rotation proposals
+    per run in one graph replay.  `basis` (mixed precision, K >= every K_g) holds
+    the tables; the histograms are walked over K3_range 1:K_g on it."""
 
-    def __init__(self, basis, bases, c3refs, starts, seeds, nprop=64, hist_cap=0):
-        self.basis, self.bases, self.nprop = basis, list(bases), nprop
-        self.ngroups = n = len(bases)
-        refs = (C.c_void_p * n)(*[bases[g].ref(c3refs[g]).value for g in range(n)])
-        arrs = [_coeff_ptr(starts[g], bases[g]) for g in range(n)]
-        self._starts = [a[0][:bases[g].K].copy() for g, a in enumerate(arrs)]
+    def __init__(self, basis, c3refs, starts, seeds, nprop=64, hist_cap=0):
+        self.basis, self.nprop = basis, nprop
+        self.ngroups = n = len(c3refs)
+        self.Ks = [np.asarray(c3refs[g]).shape[2] for g in range(n)]
+        refs = (C.c_void_p * n)(*[basis.ref(c3refs[g], range(1, self.Ks[g] + 1)).value for g in range(n)])
+        arrs = [_coeff_ptr(starts[g], basis, self.Ks[g]) for g in range(n)]
+        self._starts = [a[0][:self.Ks[g]].copy() for g, a in enumerate(arrs)]
         ptrs = (C.c_void_p * n)(*[a[0].ctypes.data for a in arrs])
         lds = (C.c_int64 * n)(*[a[1] for a in arrs])
         self._lds = [a[1] for a in arrs]
