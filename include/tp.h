@@ -44,6 +44,7 @@ typedef struct tp_basis tp_basis;
 typedef struct tp_ref tp_ref;
 typedef struct tp_state tp_state;
 typedef struct tp_batch tp_batch;
+typedef struct tp_mlp tp_mlp;
 
 /* ABI version of the loaded library (== TP_ABI_VERSION). */
 int tp_abi_version(void);
@@ -205,6 +206,48 @@ int tp_batch_eval_quats(tp_batch *B, const double *quat, double *E_out);
  * (spherical_harmonics.jl:598-612) is the reference's whole-structure form. */
 int tp_rotate(tp_ctx *ctx, const tp_basis *b, const double *coeff, int64_t ld, int nshell, const double *quat,
               double *out);
+
+/* ---- learned rotation proposer --------------------------------------------- */
+/* No counterpart in the reference, whose moves come only from
+ * random_rotation_step (utilities.jl:180-190).  An MLP 192 -> 256 -> 256 -> 256
+ * (ReLU) -> 64 in BF16 with FP32 accumulation and FP32 master weights / Adam:
+ * input = the 190 real-basis coefficients of one shell (even l <= 18, scaled
+ * to norm 8), log10 T, 0; output rows 0..35 = one quaternion (w,x,y,z) per
+ * even l = 2..18 (rows 36..63 are held at zero).  Parameters flat, as
+ * W1 [256][192], b1 [256], W2 [256][256], b2, W3 [256][256], b3, W4 [64][256],
+ * b4 [64] (tp_mlp_nparam() floats).  Training minimises
+ * (1/B) sum_b sum_{j<36} (y_bj - t_bj)^2 with Adam (0.9, 0.999, 1e-8) on
+ * minibatches drawn with replacement (Philox keyed by seed). */
+int tp_mlp_nparam(void);
+int tp_mlp_create(tp_ctx *ctx, uint64_t seed, tp_mlp **out);
+void tp_mlp_destroy(tp_mlp *m);
+int tp_mlp_get_params(tp_mlp *m, float *out);
+/* Also resets the Adam state. */
+int tp_mlp_set_params(tp_mlp *m, const float *in);
+/* Y[n][64] for inputs X[n][192]. */
+int tp_mlp_forward(tp_mlp *m, const float *X, int n, float *Y);
+/* Loss and its gradient (FP32, flat like the parameters) on n rows (n a
+ * multiple of 64); nothing is updated. */
+int tp_mlp_loss_grad(tp_mlp *m, const float *X, const float *T, int n, double *loss, float *grad);
+/* steps Adam steps on host data (X[n][192], T[n][36]); loss_trace[steps] may be NULL. */
+int tp_mlp_train(tp_mlp *m, const float *X, const float *T, int n, int steps, int batch, float lr, uint64_t seed,
+                 float *loss_trace);
+/* The same on a batch's move log (the data never leaves the device). */
+int tp_mlp_train_log(tp_mlp *m, tp_batch *B, int steps, int batch, float lr, uint64_t seed, float *loss_trace);
+
+/* Keep a move log of up to `capacity` samples: after every sweep, each proposal
+ * that passes the Metropolis test on its own against its run's current energy
+ * adds one sample per shell of the run (that shell's features in the state the
+ * move starts from, and the proposal's quaternions with w >= 0). */
+int tp_batch_log(tp_batch *B, int capacity);
+int tp_batch_log_count(tp_batch *B, int32_t *n);
+int tp_batch_log_read(tp_batch *B, int first, int n, float *X, float *T);
+/* The last nlearn (< nprop) slots of every run take the network's quaternions
+ * (input: shell j*K_g/nlearn of the run's state and its T), each composed with
+ * a random rotation of angle <= noise * amax * pi/4; the other slots stay
+ * random.  Acceptance is unchanged (exact energies).  nlearn = 0 switches the
+ * network off. */
+int tp_batch_set_proposer(tp_batch *B, tp_mlp *m, int nlearn, double noise);
 
 /* ---- synthetic input ------------------------------------------------------ */
 

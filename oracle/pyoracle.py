@@ -38,6 +38,9 @@ def lib():
         L.or_wigner3j.argtypes = [C.c_int] * 6
         L.or_sphPlm.restype = C.c_double
         L.or_sphPlm.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.or_mlp_nparam.restype = C.c_int
+        L.or_mlp_loss_grad.restype = C.c_double
+        L.or_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.or_basis_create.restype = C.c_void_p
         L.or_basis_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int]
         L.or_basis_free.argtypes = [C.c_void_p]
@@ -271,3 +274,20 @@ def intensity_from_atoms(atoms, KMAX, dq, LMAX, lfill, ntheta, nphi, nthreads=0)
     lib().or_intensity_from_atoms(atoms, atoms.shape[0], KMAX, dq, LMAX, lfill, ntheta, nphi,
                                   out.view(np.float64).reshape(KMAX, -1), nthreads)
     return out
+
+
+def mlp_nparam():
+    return lib().or_mlp_nparam()
+
+
+def mlp_loss_grad(params, X, T, emulate=True, want_grad=True):
+    """Loss, gradient (FP64) and outputs of the proposer MLP (or_mlp_loss_grad)."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    B = X.shape[0]
+    g = np.zeros(p.size) if want_grad else None
+    Y = np.zeros((B, 64))
+    loss = lib().or_mlp_loss_grad(p.ctypes.data, B, X.ctypes.data, T.ctypes.data, int(emulate),
+                                  g.ctypes.data if want_grad else None, Y.ctypes.data)
+    return loss, g, Y

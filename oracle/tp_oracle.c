@@ -963,3 +963,119 @@ void or_intensity_from_atoms(const double *atoms /* natoms x 3 */, int natoms, i
     }
     free(gx); free(gw);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Learned proposer: MLP 192 -> 256 -> 256 -> 256 (ReLU) -> 64, MSE on the   */
+/* first 36 outputs.  No counterpart in the reference (the reference        */
+/* proposes moves only by random_rotation_step, utilities.jl:180-190); this */
+/* is the checker of the device's forward/backward kernels.                 */
+/* Parameters flat: W1 [256][192], b1 [256], W2 [256][256], b2, W3, b3,      */
+/* W4 [64][256], b4 [64].  emulate != 0 rounds, as the device does, the      */
+/* inputs, the weights, every activation after ReLU, dY and every masked     */
+/* hidden gradient to bfloat16 (round to nearest even of the float value);  */
+/* accumulation is FP64 here (FP32 on the device).                          */
+/* Pinned by central finite differences of the non-emulated loss.           */
+/* ------------------------------------------------------------------------ */
+#define OR_MLP_IN 192
+#define OR_MLP_HID 256
+#define OR_MLP_OUT 64
+#define OR_MLP_Q 36
+
+static double or_bf16(double x)
+{
+    float f = (float)x;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+int or_mlp_nparam(void)
+{
+    return OR_MLP_HID * OR_MLP_IN + OR_MLP_HID + 2 * (OR_MLP_HID * OR_MLP_HID + OR_MLP_HID) +
+           OR_MLP_OUT * OR_MLP_HID + OR_MLP_OUT;
+}
+
+/* loss = (1/B) sum_b sum_{j<36} (y_bj - t_bj)^2; grad (may be NULL) gets
+ * d loss / d params; Y (may be NULL) gets the [B][64] outputs. */
+double or_mlp_loss_grad(const float *params, int B, const float *X /* [B][192] */, const float *T /* [B][36] */,
+                        int emulate, double *grad, double *Y)
+{
+    const int nin[4] = {OR_MLP_IN, OR_MLP_HID, OR_MLP_HID, OR_MLP_HID};
+    const int nout[4] = {OR_MLP_HID, OR_MLP_HID, OR_MLP_HID, OR_MLP_OUT};
+    int off[4];
+    off[0] = 0;
+    for (int L = 1; L < 4; ++L) off[L] = off[L - 1] + nout[L - 1] * nin[L - 1] + nout[L - 1];
+    double *W[4], *bvec[4];
+    for (int L = 0; L < 4; ++L) {
+        W[L] = (double *)malloc(sizeof(double) * nout[L] * nin[L]);
+        bvec[L] = (double *)malloc(sizeof(double) * nout[L]);
+        for (int i = 0; i < nout[L] * nin[L]; ++i) {
+            const double w = params[off[L] + i];
+            W[L][i] = emulate ? or_bf16(w) : w;
+        }
+        for (int i = 0; i < nout[L]; ++i) bvec[L][i] = params[off[L] + nout[L] * nin[L] + i];
+    }
+    /* activations a[0] = x, a[1..3] hidden (post ReLU), a[4] = y */
+    double *a[5];
+    a[0] = (double *)malloc(sizeof(double) * B * OR_MLP_IN);
+    for (int i = 0; i < B * OR_MLP_IN; ++i) a[0][i] = emulate ? or_bf16(X[i]) : X[i];
+    for (int L = 0; L < 4; ++L) {
+        a[L + 1] = (double *)malloc(sizeof(double) * B * nout[L]);
+        for (int b = 0; b < B; ++b)
+            for (int o = 0; o < nout[L]; ++o) {
+                double s = 0.0;
+                for (int i = 0; i < nin[L]; ++i) s += a[L][b * nin[L] + i] * W[L][o * nin[L] + i];
+                s += bvec[L][o];
+                if (L < 3) {
+                    s = s > 0.0 ? s : 0.0;
+                    if (emulate) s = or_bf16(s);
+                } else if (emulate) {
+                    s = (double)(float)s;
+                }
+                a[L + 1][b * nout[L] + o] = s;
+            }
+    }
+    double loss = 0.0;
+    double *d = (double *)calloc((size_t)B * OR_MLP_OUT, sizeof(double));
+    for (int b = 0; b < B; ++b)
+        for (int j = 0; j < OR_MLP_Q; ++j) {
+            const double e = a[4][b * OR_MLP_OUT + j] - T[b * OR_MLP_Q + j];
+            loss += e * e / B;
+            const double g = 2.0 * e / B;
+            d[b * OR_MLP_OUT + j] = emulate ? or_bf16(g) : g;
+        }
+    if (Y) memcpy(Y, a[4], sizeof(double) * B * OR_MLP_OUT);
+    if (grad) {
+        for (int L = 3; L >= 0; --L) {
+            double *gW = grad + off[L], *gb = grad + off[L] + nout[L] * nin[L];
+            for (int o = 0; o < nout[L]; ++o) {
+                double sb = 0.0;
+                for (int b = 0; b < B; ++b) sb += d[b * nout[L] + o];
+                gb[o] = sb;
+                for (int i = 0; i < nin[L]; ++i) {
+                    double s = 0.0;
+                    for (int b = 0; b < B; ++b) s += d[b * nout[L] + o] * a[L][b * nin[L] + i];
+                    gW[o * nin[L] + i] = s;
+                }
+            }
+            if (L == 0) break;
+            double *dn = (double *)malloc(sizeof(double) * B * nin[L]);
+            for (int b = 0; b < B; ++b)
+                for (int i = 0; i < nin[L]; ++i) {
+                    double s = 0.0;
+                    for (int o = 0; o < nout[L]; ++o) s += d[b * nout[L] + o] * W[L][o * nin[L] + i];
+                    if (!(a[L][b * nin[L] + i] > 0.0)) s = 0.0;   /* ReLU derivative of the layer below */
+                    dn[b * nin[L] + i] = emulate ? or_bf16(s) : s;
+                }
+            free(d);
+            d = dn;
+        }
+    }
+    free(d);
+    for (int L = 0; L < 5; ++L) free(a[L]);
+    for (int L = 0; L < 4; ++L) { free(W[L]); free(bvec[L]); }
+    return loss;
+}

@@ -1,0 +1,100 @@
+"""The learned proposer's hand-written BF16 kernels against the oracle
+(or_mlp_loss_grad with the device's bf16 rounding points emulated; its
+gradient is pinned by finite differences in test_oracle_mlp.py), training,
+and the move log / learned slots of a sweep batch."""
+import numpy as np
+import pytest
+
+from helpers import c2_invariants
+
+pytestmark = pytest.mark.gpu
+
+# FP32 accumulation (device) vs FP64 (oracle) moves an occasional bf16
+# rounding by one ulp (2^-8 relative); normwise that stays far below 1e-3.
+GRAD_TOL = 1e-3
+
+
+@pytest.fixture(scope="module")
+def tp():
+    import tp_amd
+    tp_amd.HIP_init()
+    return tp_amd
+
+
+def _data(rng, n):
+    X = rng.normal(size=(n, 192)).astype(np.float32)
+    X[:, 191] = 0.0
+    T = rng.normal(size=(n, 36)).astype(np.float32) * 0.5
+    return X, T
+
+
+def test_forward_and_grad_match_oracle(tp, oracle):
+    m = tp.MlpProposer(seed=3)
+    p = m.params()
+    assert p.size == oracle.mlp_nparam()
+    rng = np.random.default_rng(0)
+    X, T = _data(rng, 128)
+    loss, g = m.loss_grad(X, T)
+    lo, go, Yo = oracle.mlp_loss_grad(p, X, T, emulate=True)
+    Y = m.forward(X)
+    assert np.abs(Y[:, :36] - Yo[:, :36]).max() <= 1e-3 * np.abs(Yo[:, :36]).max()
+    assert np.all(Y[:, 36:] == 0)
+    assert abs(loss - lo) <= 1e-4 * lo
+    bounds = np.cumsum([0, 256 * 192, 256, 65536, 256, 65536, 256, 64 * 256, 64])
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        err = np.linalg.norm(g[a:b] - go[a:b]) / max(np.linalg.norm(go[a:b]), 1e-30)
+        print(f"block {a}:{b} rel err {err:.2e}")
+        assert err < GRAD_TOL
+
+
+def test_set_params_roundtrip_and_training(tp):
+    m = tp.MlpProposer(seed=1)
+    p = m.params()
+    m.set_params(p)
+    assert np.array_equal(m.params(), p)
+    rng = np.random.default_rng(5)
+    n = 4096
+    X, _ = _data(rng, n)
+    A = rng.normal(size=(192, 36)).astype(np.float32) / np.sqrt(192)
+    T = np.tanh(X @ A).astype(np.float32)          # a learnable target
+    tr = m.train(X, T, steps=400, batch=256, lr=2e-3, seed=7)
+    print("loss", tr[:3], tr[-3:])
+    assert np.all(np.isfinite(tr))
+    assert tr[-20:].mean() < 0.3 * tr[:5].mean()
+
+
+def test_move_log_and_learned_slots(tp):
+    from test_gpu_batch import _anneal_setup
+    Ks, nprop = [5, 8, 6], 8
+    L, LMAX = 8, 25
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, hist_cap=40)
+    B.log(100000)
+    B.probe(0.5)
+    B.run(10, decay=0.999, factor=1.01)
+    n = B.log_count()
+    print("logged", n)
+    assert n > 0 and n % 1 == 0
+    X, T = B.log_read(0, min(n, 500))
+    feat = X[:, :190]
+    # L = 8: 45 real-basis entries of norm 8, the rest zero
+    assert np.allclose(np.linalg.norm(feat[:, :45], axis=1), 8.0, rtol=2e-2)
+    assert np.all(feat[:, 45:] == 0) and np.all(X[:, 191] == 0)
+    q = T.reshape(-1, 9, 4)[:, :L // 2]
+    assert np.allclose(np.linalg.norm(q, axis=2), 1.0, atol=1e-6)
+    assert np.all(q[:, :, 0] >= 0)
+    m = tp.MlpProposer(seed=2)
+    tr = m.train_log(B, steps=50, batch=128, lr=1e-3, seed=3)
+    assert np.all(np.isfinite(tr))
+    B.set_proposer(m, 3, noise=0.25)
+    B.run(10, decay=0.999, factor=1.01)
+    for b in bases:
+        b.set_precision("fp16x3")
+    for g in range(len(Ks)):
+        st = B.group(g)
+        assert st["nsweep"] == 20
+        c = B.coeff(g)
+        Es = tp.energy(c, bases[g], refs[g])
+        assert abs(st["E"] - Es) <= 1e-10 * abs(Es)
+        a, b2 = c2_invariants(starts[g], Ks[g], L, LMAX), c2_invariants(c, Ks[g], L, LMAX)
+        assert np.abs(a - b2).max() < 1e-11 * np.abs(a).max()
+    assert B.log_count() >= n

@@ -15,6 +15,7 @@
 #include "tp_host_math.hpp"
 #include "tp_kernels.hpp"
 #include "tp_batch.hpp"
+#include "tp_mlp.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -45,6 +46,17 @@ static int dev_alloc(T **p, size_t count)
 }
 
 static int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+static uint16_t host_f16(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+static double host_f16_val(uint16_t h) { return (double)__builtin_bit_cast(_Float16, h); }
+static uint16_t host_bf16(double x)   // round to nearest even of the float value
+{
+    const float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
 
 // ---------------------------------------------------------------------------
 // objects
@@ -302,6 +314,17 @@ struct tp_batch {
     double g_decay = 0.0, g_factor = 0.0;   // what the captured sweep graph was built with
     std::vector<const tp_ref *> refs;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    // move log (training data of the learned proposer)
+    int log_cap = 0;
+    uint16_t *d_Xlog = nullptr;    // [log_cap][192]
+    float *d_Tlog = nullptr;       // [log_cap][36]
+    int *d_logn = nullptr;         // [1] samples logged
+    // learned proposals: the last nlearn slots of every run
+    struct tp_mlp *mlp = nullptr;
+    int nlearn = 0, Mi = 0;
+    double noise = 0.0;
+    uint16_t *d_Xi = nullptr, *d_Hi[3] = {nullptr, nullptr, nullptr};
+    float *d_Yi = nullptr;
 };
 
 static void batch_free(tp_batch *B)
@@ -314,7 +337,8 @@ static void batch_free(tp_batch *B)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As,
                     B->d_sc,  B->d_isc, B->d_part, B->d_E,    B->d_Ehist,  B->d_Ahi,    B->d_Alo,
-                    (void *)B->d_qtab, B->d_nvalid, B->d_raw};
+                    (void *)B->d_qtab, B->d_nvalid, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn,
+                    B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete B;
@@ -537,16 +561,23 @@ static int batch_enqueue_eval(const tp_batch *B, hipStream_t st)
     return 0;
 }
 
-// One sweep: proposals (random quaternions, rotation), evaluation, decision.
+static void batch_learned_quats(const tp_batch *B, hipStream_t st);   // learned slots (defined with the MLP code)
+
+// One sweep: proposals (random quaternions; learned ones in the last nlearn
+// slots), rotation, evaluation, move log, decision.
 static int batch_enqueue_sweep(const tp_batch *B, hipStream_t st, int probe, double decay, double factor, double tfac)
 {
     const tp_basis *b = B->b;
     const int nq = B->ne * B->nl;
     hipLaunchKernelGGL(k_quat_random, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, B->d_ev, B->ne, B->nl,
                        B->d_gs, probe, B->d_quat);
+    if (!probe && B->mlp && B->nlearn > 0) batch_learned_quats(B, st);
     hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)B->nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev, B->d_quat,
                        B->nl, b->n_c);
     if (int rc = batch_enqueue_eval(B, st)) return rc;
+    if (!probe && B->log_cap > 0)
+        hipLaunchKernelGGL(k_log_moves, dim3(1), dim3(1024), 0, st, B->d_ev, B->ne, B->nl, B->d_gs, B->d_E, B->d_quat,
+                           b->n_c, B->d_Xlog, B->d_Tlog, B->d_logn, B->log_cap);
     hipLaunchKernelGGL(k_select_batch, dim3((unsigned)B->ngroups), dim3(256), 0, st, B->d_gs, B->d_ev, B->d_E,
                        B->d_cur, B->cstride, b->n_c, probe, decay, factor, tfac, B->d_Ehist, B->hist_cap,
                        B->ngroups);
@@ -630,6 +661,269 @@ static int batch_autotune(tp_batch *B, int reps)
     TP_HIP(hipStreamSynchronize(st));
     if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// learned proposer (MLP, tp_mlp.hpp)
+// ---------------------------------------------------------------------------
+struct tp_mlp {
+    tp_ctx *ctx = nullptr;
+    float *d_p = nullptr, *d_m = nullptr, *d_v = nullptr, *d_g = nullptr;   // [kMlpNParam]
+    uint16_t *d_W[4] = {nullptr, nullptr, nullptr, nullptr}, *d_Wt[4] = {nullptr, nullptr, nullptr, nullptr};
+    int *d_t = nullptr;            // Adam steps taken
+    // activations / gradients for up to Bcap rows
+    int Bcap = 0;
+    uint16_t *X = nullptr, *Xt = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *Ht[3] = {nullptr, nullptr, nullptr};
+    uint16_t *dY = nullptr, *dYt = nullptr, *dH[3] = {nullptr, nullptr, nullptr}, *dHt[3] = {nullptr, nullptr, nullptr};
+    float *Y = nullptr, *T = nullptr, *lpart = nullptr, *trace = nullptr;
+    int trace_cap = 0;
+    // host-supplied training data
+    uint16_t *d_Xdata = nullptr;
+    float *d_Tdata = nullptr;
+    int *d_count = nullptr;
+    int ndata_cap = 0;
+    // captured training step
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void *g_src = nullptr;
+    int g_B = 0;
+    float g_lr = 0.f;
+    uint64_t g_seed = 0;
+};
+
+static int mlp_bias_off(int L) { return kMlpOffW[L] + kMlpRows[L] * kMlpCols[L]; }
+
+static void mlp_free_acts(tp_mlp *m)
+{
+    void *ptrs[] = {m->X, m->Xt, m->H[0], m->H[1], m->H[2], m->Ht[0], m->Ht[1], m->Ht[2], m->dY, m->dYt,
+                    m->dH[0], m->dH[1], m->dH[2], m->dHt[0], m->dHt[1], m->dHt[2], m->Y, m->T, m->lpart};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    m->X = m->Xt = m->dY = m->dYt = nullptr;
+    for (int i = 0; i < 3; ++i) m->H[i] = m->Ht[i] = m->dH[i] = m->dHt[i] = nullptr;
+    m->Y = m->T = m->lpart = nullptr;
+    m->Bcap = 0;
+}
+
+static int mlp_ensure(tp_mlp *m, int B)
+{
+    if (B <= m->Bcap) return 0;
+    mlp_free_acts(m);
+    if (m->exec) { (void)hipGraphExecDestroy(m->exec); m->exec = nullptr; }
+    if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+    int rc = 0;
+    const size_t b = (size_t)B;
+    if ((rc = dev_alloc(&m->X, b * kMlpIn)) || (rc = dev_alloc(&m->Xt, b * kMlpIn)) ||
+        (rc = dev_alloc(&m->dY, b * kMlpOut)) || (rc = dev_alloc(&m->dYt, b * kMlpOut)) ||
+        (rc = dev_alloc(&m->Y, b * kMlpOut)) || (rc = dev_alloc(&m->T, b * kMlpQ)) || (rc = dev_alloc(&m->lpart, b)))
+        return rc;
+    for (int i = 0; i < 3; ++i)
+        if ((rc = dev_alloc(&m->H[i], b * kMlpHid)) || (rc = dev_alloc(&m->Ht[i], b * kMlpHid)) ||
+            (rc = dev_alloc(&m->dH[i], b * kMlpHid)) || (rc = dev_alloc(&m->dHt[i], b * kMlpHid)))
+            return rc;
+    TP_HIP(hipMemset(m->X, 0, sizeof(uint16_t) * b * kMlpIn));
+    m->Bcap = B;
+    return 0;
+}
+
+static MlpEpi epi_none()
+{
+    MlpEpi e;
+    std::memset(&e, 0, sizeof(e));
+    e.scale = 1.f;
+    return e;
+}
+
+static void mlp_gemm(hipStream_t st, const uint16_t *A, int64_t lda, const uint16_t *B, int64_t ldb, int M, int N, int K,
+                     const MlpEpi &e)
+{
+    hipLaunchKernelGGL(k_mlp_gemm, dim3((unsigned)((M / 64) * (N / 64))), dim3(256), 0, st, A, lda, B, ldb, M, N, K, e);
+}
+
+// Forward pass of M rows of X (BF16 [M][192]); with `train` the transposed
+// activations are kept for the backward pass.  Hidden layers in H (BF16),
+// outputs in Y (FP32 [M][64]).
+static void mlp_forward(const tp_mlp *m, hipStream_t st, const uint16_t *X, int M, uint16_t *const H[3],
+                        uint16_t *const Ht[3], float *Y)
+{
+    const uint16_t *in = X;
+    int64_t ldin = kMlpIn;
+    for (int L = 0; L < 3; ++L) {
+        MlpEpi e = epi_none();
+        e.bias = m->d_p + mlp_bias_off(L);
+        e.relu = 1;
+        e.outb = H[L];
+        e.ldo = kMlpHid;
+        if (Ht) { e.outt = Ht[L]; e.ldt = M; }
+        mlp_gemm(st, in, ldin, m->d_W[L], kMlpCols[L], M, kMlpHid, kMlpCols[L], e);
+        in = H[L];
+        ldin = kMlpHid;
+    }
+    MlpEpi e = epi_none();
+    e.bias = m->d_p + mlp_bias_off(3);
+    e.outf = Y;
+    e.ldf = kMlpOut;
+    mlp_gemm(st, in, ldin, m->d_W[3], kMlpHid, M, kMlpOut, kMlpHid, e);
+}
+
+// Loss and gradient of B rows (X, Xt, T already in place) into d_g.
+static void mlp_backward(const tp_mlp *m, hipStream_t st, int B)
+{
+    hipLaunchKernelGGL(k_mlp_loss, dim3((unsigned)((B + 3) / 4)), dim3(256), 0, st, m->Y, m->T, B, m->dY, m->dYt,
+                       m->lpart);
+    const uint16_t *dcur = m->dY, *dcurt = m->dYt;
+    int ncur = kMlpOut;
+    for (int L = 3; L >= 0; --L) {
+        // dW_L [rows][cols] = dcur^T . act_L^T   (act_0 = X)
+        const uint16_t *actt = L == 0 ? m->Xt : m->Ht[L - 1];
+        MlpEpi e = epi_none();
+        e.outf = m->d_g + kMlpOffW[L];
+        e.ldf = kMlpCols[L];
+        mlp_gemm(st, dcurt, B, actt, B, kMlpRows[L], kMlpCols[L], B, e);
+        hipLaunchKernelGGL(k_mlp_rowsum, dim3((unsigned)((kMlpRows[L] + 3) / 4)), dim3(256), 0, st, dcurt, (int64_t)B,
+                           B, m->d_g + mlp_bias_off(L), kMlpRows[L]);
+        if (L == 0) break;
+        // dH_{L-1} = (dcur . W_L) * [H_{L-1} > 0]
+        MlpEpi e2 = epi_none();
+        e2.mask = m->H[L - 1];
+        e2.ldm = kMlpHid;
+        e2.outb = m->dH[L - 1];
+        e2.ldo = kMlpHid;
+        e2.outt = m->dHt[L - 1];
+        e2.ldt = B;
+        mlp_gemm(st, dcur, ncur, m->d_Wt[L], kMlpRows[L], B, kMlpCols[L], kMlpRows[L], e2);
+        dcur = m->dH[L - 1];
+        dcurt = m->dHt[L - 1];
+        ncur = kMlpHid;
+    }
+}
+
+static MlpWeights mlp_weights(const tp_mlp *m)
+{
+    MlpWeights w;
+    for (int L = 0; L < 4; ++L) { w.W[L] = m->d_W[L]; w.Wt[L] = m->d_Wt[L]; }
+    return w;
+}
+
+// Refresh the BF16 copies from the master parameters (an Adam step with lr 0
+// and zero moments would do the same; this is a plain conversion).
+static int mlp_upload(tp_mlp *m, const std::vector<float> &p)
+{
+    TP_HIP(hipMemcpy(m->d_p, p.data(), sizeof(float) * kMlpNParam, hipMemcpyHostToDevice));
+    for (int L = 0; L < 4; ++L) {
+        const int R = kMlpRows[L], Cc = kMlpCols[L];
+        std::vector<uint16_t> w((size_t)R * Cc), wt((size_t)R * Cc);
+        for (int r = 0; r < R; ++r)
+            for (int c = 0; c < Cc; ++c) {
+                const uint16_t h = host_bf16(p[(size_t)kMlpOffW[L] + (size_t)r * Cc + c]);
+                w[(size_t)r * Cc + c] = h;
+                wt[(size_t)c * R + r] = h;
+            }
+        TP_HIP(hipMemcpy(m->d_W[L], w.data(), sizeof(uint16_t) * w.size(), hipMemcpyHostToDevice));
+        TP_HIP(hipMemcpy(m->d_Wt[L], wt.data(), sizeof(uint16_t) * wt.size(), hipMemcpyHostToDevice));
+    }
+    TP_HIP(hipMemset(m->d_m, 0, sizeof(float) * kMlpNParam));
+    TP_HIP(hipMemset(m->d_v, 0, sizeof(float) * kMlpNParam));
+    TP_HIP(hipMemset(m->d_t, 0, sizeof(int)));
+    return 0;
+}
+
+static void mlp_free(tp_mlp *m)
+{
+    if (!m) return;
+    if (m->ctx) (void)hipSetDevice(m->ctx->device);
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    mlp_free_acts(m);
+    void *ptrs[] = {m->d_p, m->d_m, m->d_v, m->d_g, m->d_t, m->trace, m->d_Xdata, m->d_Tdata, m->d_count};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int L = 0; L < 4; ++L) {
+        if (m->d_W[L]) (void)hipFree(m->d_W[L]);
+        if (m->d_Wt[L]) (void)hipFree(m->d_Wt[L]);
+    }
+    delete m;
+}
+
+__global__ void k_mlp_losssum(const float *lpart, int B, const int *tstep, float *trace, int cap)
+{
+    if (threadIdx.x != 0) return;
+    float s = 0.f;
+    for (int b = 0; b < B; ++b) s += lpart[b];
+    const int t = tstep[0];
+    if (t < cap) trace[t] = s;
+}
+
+// One training step on data rows (Xd, Td, count): gather a minibatch, forward,
+// backward, Adam, loss to trace[t], t += 1.
+static void mlp_train_step(tp_mlp *m, hipStream_t st, const uint16_t *Xd, const float *Td, const int *count, int B,
+                           float lr, uint64_t seed)
+{
+    hipLaunchKernelGGL(k_mlp_gather, dim3((unsigned)B), dim3(256), 0, st, Xd, Td, count, seed, m->d_t, B, m->X, m->Xt,
+                       m->T);
+    mlp_forward(m, st, m->X, B, m->H, m->Ht, m->Y);
+    mlp_backward(m, st, B);
+    hipLaunchKernelGGL(k_mlp_losssum, dim3(1), dim3(64), 0, st, m->lpart, B, m->d_t, m->trace, m->trace_cap);
+    hipLaunchKernelGGL(k_mlp_adam, dim3((unsigned)((kMlpNParam + 255) / 256)), dim3(256), 0, st, m->d_p, m->d_m,
+                       m->d_v, m->d_g, m->d_t, lr, 0.9f, 0.999f, 1e-8f, mlp_weights(m));
+    hipLaunchKernelGGL(k_mlp_tick, dim3(1), dim3(1), 0, st, m->d_t);
+}
+
+static int mlp_train(tp_mlp *m, const uint16_t *Xd, const float *Td, const int *count, int steps, int B, float lr,
+                     uint64_t seed, float *loss_trace)
+{
+    hipStream_t st = m->ctx->stream;
+    if (int rc = mlp_ensure(m, B)) return rc;
+    int t0 = 0;
+    TP_HIP(hipMemcpy(&t0, m->d_t, sizeof(int), hipMemcpyDeviceToHost));
+    if (m->trace_cap < t0 + steps) {
+        if (m->trace) TP_HIP(hipFree(m->trace));
+        m->trace = nullptr;
+        m->trace_cap = 0;
+        if (int rc = dev_alloc(&m->trace, (size_t)(t0 + steps) * 2)) return rc;
+        m->trace_cap = (t0 + steps) * 2;
+        if (m->exec) { TP_HIP(hipGraphExecDestroy(m->exec)); m->exec = nullptr; }
+        if (m->graph) { TP_HIP(hipGraphDestroy(m->graph)); m->graph = nullptr; }
+    }
+    if (!m->exec || m->g_src != (const void *)Xd || m->g_B != B || m->g_lr != lr || m->g_seed != seed) {
+        if (m->exec) { TP_HIP(hipGraphExecDestroy(m->exec)); m->exec = nullptr; }
+        if (m->graph) { TP_HIP(hipGraphDestroy(m->graph)); m->graph = nullptr; }
+        TP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        mlp_train_step(m, st, Xd, Td, count, B, lr, seed);
+        hipGraph_t g = nullptr;
+        TP_HIP(hipStreamEndCapture(st, &g));
+        m->graph = g;
+        TP_HIP(hipGraphInstantiate(&m->exec, g, nullptr, nullptr, 0));
+        m->g_src = Xd;
+        m->g_B = B;
+        m->g_lr = lr;
+        m->g_seed = seed;
+    }
+    for (int i = 0; i < steps; ++i) TP_HIP(hipGraphLaunch(m->exec, st));
+    if (loss_trace)
+        TP_HIP(hipMemcpyAsync(loss_trace, m->trace + t0, sizeof(float) * steps, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+static uint64_t splitmix64(uint64_t &s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static void batch_learned_quats(const tp_batch *B, hipStream_t st)
+{
+    const tp_basis *b = B->b;
+    hipLaunchKernelGGL(k_mlp_infer_features, dim3((unsigned)B->nlearn, (unsigned)B->ngroups), dim3(64), 0, st,
+                       B->d_gs, B->d_cur, B->cstride, b->n_c, B->nlearn, B->d_Xi);
+    uint16_t *const H[3] = {B->d_Hi[0], B->d_Hi[1], B->d_Hi[2]};
+    mlp_forward(B->mlp, st, B->d_Xi, B->Mi, H, nullptr, B->d_Yi);
+    const int nq = B->ne * B->nl;
+    hipLaunchKernelGGL(k_mlp_to_quat, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, B->d_ev, B->ne, B->nl,
+                       B->d_gs, B->d_Yi, B->noise, B->d_quat);
 }
 
 // ---------------------------------------------------------------------------
@@ -983,16 +1277,6 @@ int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *
     return 0;
 }
 
-static uint16_t host_f16(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
-static double host_f16_val(uint16_t h) { return (double)__builtin_bit_cast(_Float16, h); }
-static uint16_t host_bf16(double x)   // round to nearest even of the float value
-{
-    const float f = (float)x;
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
 
 int tp_basis_set_precision(tp_basis *b, int mode)
 {
@@ -1668,6 +1952,234 @@ int tp_rotate(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, 
             }
     (void)hipFree(src); (void)hipFree(dst); (void)hipFree(dq); (void)hipFree(dev);
     return rc;
+}
+
+// ---- learned proposer -------------------------------------------------------
+int tp_mlp_nparam(void) { return kMlpNParam; }
+
+void tp_mlp_destroy(tp_mlp *m) { mlp_free(m); }
+
+int tp_mlp_create(tp_ctx *ctx, uint64_t seed, tp_mlp **out)
+{
+    if (!out) return tp_fail(TP_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (int rc = check_ctx(ctx)) return rc;
+    tp_mlp *m = new tp_mlp();
+    m->ctx = ctx;
+    int rc = 0;
+    if ((rc = dev_alloc(&m->d_p, kMlpNParam)) || (rc = dev_alloc(&m->d_m, kMlpNParam)) ||
+        (rc = dev_alloc(&m->d_v, kMlpNParam)) || (rc = dev_alloc(&m->d_g, kMlpNParam)) || (rc = dev_alloc(&m->d_t, 1))) {
+        mlp_free(m);
+        return rc;
+    }
+    for (int L = 0; L < 4; ++L)
+        if ((rc = dev_alloc(&m->d_W[L], (size_t)kMlpRows[L] * kMlpCols[L])) ||
+            (rc = dev_alloc(&m->d_Wt[L], (size_t)kMlpRows[L] * kMlpCols[L]))) {
+            mlp_free(m);
+            return rc;
+        }
+    // He-normal weights (Box-Muller on splitmix64), zero biases; the output
+    // layer starts small with bias = identity quaternion (w = 1) per l
+    std::vector<float> p(kMlpNParam, 0.f);
+    uint64_t st = seed;
+    for (int L = 0; L < 4; ++L) {
+        const double sd = std::sqrt(2.0 / kMlpCols[L]) * (L == 3 ? 0.1 : 1.0);
+        for (int r = 0; r < kMlpRows[L]; ++r)
+            for (int c = 0; c < kMlpCols[L]; ++c) {
+                const double u1 = ((splitmix64(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+                const double u2 = ((splitmix64(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+                const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * tph::kPi * u2);
+                if (L == 3 && r >= kMlpQ) continue;
+                p[(size_t)kMlpOffW[L] + (size_t)r * kMlpCols[L] + c] = (float)(sd * z);
+            }
+    }
+    for (int li = 0; li < kMlpQ / 4; ++li) p[(size_t)mlp_bias_off(3) + 4 * li] = 1.f;
+    if ((rc = mlp_upload(m, p))) {
+        mlp_free(m);
+        return rc;
+    }
+    *out = m;
+    return 0;
+}
+
+int tp_mlp_get_params(tp_mlp *m, float *out)
+{
+    if (!m || !out) return tp_fail(TP_E_ARG, "null argument");
+    if (int rc = check_ctx(m->ctx)) return rc;
+    TP_HIP(hipMemcpy(out, m->d_p, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int tp_mlp_set_params(tp_mlp *m, const float *in)
+{
+    if (!m || !in) return tp_fail(TP_E_ARG, "null argument");
+    if (int rc = check_ctx(m->ctx)) return rc;
+    std::vector<float> p(in, in + kMlpNParam);
+    for (int r = kMlpQ; r < kMlpOut; ++r) {   // output padding stays zero
+        for (int c = 0; c < kMlpHid; ++c) p[(size_t)kMlpOffW[3] + (size_t)r * kMlpHid + c] = 0.f;
+        p[(size_t)mlp_bias_off(3) + r] = 0.f;
+    }
+    return mlp_upload(m, p);
+}
+
+// host rows [n][192] float -> device BF16 X and X^T (rows padded to a multiple of 64)
+static int mlp_put_rows(tp_mlp *m, const float *X, int n, int B)
+{
+    std::vector<uint16_t> xb((size_t)B * kMlpIn, 0), xt((size_t)B * kMlpIn, 0);
+    for (int r = 0; r < n; ++r)
+        for (int i = 0; i < kMlpIn; ++i) {
+            const uint16_t h = host_bf16(X[(size_t)r * kMlpIn + i]);
+            xb[(size_t)r * kMlpIn + i] = h;
+            xt[(size_t)i * B + r] = h;
+        }
+    TP_HIP(hipMemcpy(m->X, xb.data(), sizeof(uint16_t) * xb.size(), hipMemcpyHostToDevice));
+    TP_HIP(hipMemcpy(m->Xt, xt.data(), sizeof(uint16_t) * xt.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int tp_mlp_forward(tp_mlp *m, const float *X, int n, float *Y)
+{
+    if (!m || !X || !Y || n < 1) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(m->ctx)) return rc;
+    const int B = (int)round_up(n, 64);
+    if (int rc = mlp_ensure(m, B)) return rc;
+    if (int rc = mlp_put_rows(m, X, n, B)) return rc;
+    hipStream_t st = m->ctx->stream;
+    mlp_forward(m, st, m->X, B, m->H, m->Ht, m->Y);
+    TP_HIP(hipGetLastError());
+    TP_HIP(hipMemcpyAsync(Y, m->Y, sizeof(float) * (size_t)n * kMlpOut, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int tp_mlp_loss_grad(tp_mlp *m, const float *X, const float *T, int n, double *loss, float *grad)
+{
+    if (!m || !X || !T || n < 1 || n % 64) return tp_fail(TP_E_ARG, "bad argument (n must be a positive multiple of 64)");
+    if (int rc = check_ctx(m->ctx)) return rc;
+    if (int rc = mlp_ensure(m, n)) return rc;
+    if (int rc = mlp_put_rows(m, X, n, n)) return rc;
+    hipStream_t st = m->ctx->stream;
+    TP_HIP(hipMemcpyAsync(m->T, T, sizeof(float) * (size_t)n * kMlpQ, hipMemcpyHostToDevice, st));
+    mlp_forward(m, st, m->X, n, m->H, m->Ht, m->Y);
+    mlp_backward(m, st, n);
+    TP_HIP(hipGetLastError());
+    std::vector<float> lp(n);
+    TP_HIP(hipMemcpyAsync(lp.data(), m->lpart, sizeof(float) * n, hipMemcpyDeviceToHost, st));
+    if (grad) TP_HIP(hipMemcpyAsync(grad, m->d_g, sizeof(float) * kMlpNParam, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    if (loss) {
+        double s = 0.0;
+        for (float v : lp) s += v;
+        *loss = s;
+    }
+    return 0;
+}
+
+int tp_mlp_train(tp_mlp *m, const float *X, const float *T, int n, int steps, int batch, float lr, uint64_t seed,
+                 float *loss_trace)
+{
+    if (!m || !X || !T || n < 1 || steps < 0 || batch < 64 || batch % 64)
+        return tp_fail(TP_E_ARG, "bad argument (batch must be a positive multiple of 64)");
+    if (int rc = check_ctx(m->ctx)) return rc;
+    if (n > m->ndata_cap) {
+        if (m->d_Xdata) (void)hipFree(m->d_Xdata);
+        if (m->d_Tdata) (void)hipFree(m->d_Tdata);
+        m->d_Xdata = nullptr;
+        m->d_Tdata = nullptr;
+        m->ndata_cap = 0;
+        if (int rc = dev_alloc(&m->d_Xdata, (size_t)n * kMlpIn)) return rc;
+        if (int rc = dev_alloc(&m->d_Tdata, (size_t)n * kMlpQ)) return rc;
+        m->ndata_cap = n;
+    }
+    if (!m->d_count)
+        if (int rc = dev_alloc(&m->d_count, 1)) return rc;
+    std::vector<uint16_t> xb((size_t)n * kMlpIn);
+    for (size_t i = 0; i < xb.size(); ++i) xb[i] = host_bf16(X[i]);
+    TP_HIP(hipMemcpy(m->d_Xdata, xb.data(), sizeof(uint16_t) * xb.size(), hipMemcpyHostToDevice));
+    TP_HIP(hipMemcpy(m->d_Tdata, T, sizeof(float) * (size_t)n * kMlpQ, hipMemcpyHostToDevice));
+    TP_HIP(hipMemcpy(m->d_count, &n, sizeof(int), hipMemcpyHostToDevice));
+    return mlp_train(m, m->d_Xdata, m->d_Tdata, m->d_count, steps, batch, lr, seed, loss_trace);
+}
+
+int tp_mlp_train_log(tp_mlp *m, tp_batch *B, int steps, int batch, float lr, uint64_t seed, float *loss_trace)
+{
+    if (!m || !B || steps < 0 || batch < 64 || batch % 64) return tp_fail(TP_E_ARG, "bad argument");
+    if (!B->log_cap) return tp_fail(TP_E_STATE, "the batch keeps no move log (tp_batch_log)");
+    if (int rc = check_ctx(m->ctx)) return rc;
+    return mlp_train(m, B->d_Xlog, B->d_Tlog, B->d_logn, steps, batch, lr, seed, loss_trace);
+}
+
+int tp_batch_log(tp_batch *B, int capacity)
+{
+    if (!B || !B->sweep || capacity < 1) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    if (B->log_cap) return tp_fail(TP_E_STATE, "move log already enabled");
+    if (int rc = dev_alloc(&B->d_Xlog, (size_t)capacity * kMlpIn)) return rc;
+    if (int rc = dev_alloc(&B->d_Tlog, (size_t)capacity * kMlpQ)) return rc;
+    if (int rc = dev_alloc(&B->d_logn, 1)) return rc;
+    TP_HIP(hipMemset(B->d_logn, 0, sizeof(int)));
+    B->log_cap = capacity;
+    if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
+    return 0;
+}
+
+int tp_batch_log_count(tp_batch *B, int32_t *n)
+{
+    if (!B || !n) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    *n = 0;
+    if (B->log_cap) TP_HIP(hipMemcpy(n, B->d_logn, sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int tp_batch_log_read(tp_batch *B, int first, int n, float *X, float *T)
+{
+    if (!B || !B->log_cap || first < 0 || n < 0 || first + n > B->log_cap) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    std::vector<uint16_t> xb((size_t)n * kMlpIn);
+    TP_HIP(hipMemcpy(xb.data(), B->d_Xlog + (size_t)first * kMlpIn, sizeof(uint16_t) * xb.size(), hipMemcpyDeviceToHost));
+    if (T) TP_HIP(hipMemcpy(T, B->d_Tlog + (size_t)first * kMlpQ, sizeof(float) * (size_t)n * kMlpQ, hipMemcpyDeviceToHost));
+    if (X)
+        for (size_t i = 0; i < xb.size(); ++i) {
+            const uint32_t u = (uint32_t)xb[i] << 16;
+            float f;
+            std::memcpy(&f, &u, 4);
+            X[i] = f;
+        }
+    return 0;
+}
+
+int tp_batch_set_proposer(tp_batch *B, tp_mlp *m, int nlearn, double noise)
+{
+    if (!B || !B->sweep || nlearn < 0 || nlearn >= B->nprop || noise < 0.0) return tp_fail(TP_E_ARG, "bad argument");
+    if (nlearn > 0 && !m) return tp_fail(TP_E_ARG, "learned slots need a network");
+    if (m && m->ctx != B->ctx) return tp_fail(TP_E_STATE, "network belongs to another context");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    const int Mi = (int)round_up((int64_t)B->ngroups * std::max(1, nlearn), 64);
+    if (Mi > B->Mi) {
+        void *ptrs[] = {B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        B->d_Xi = nullptr; B->d_Hi[0] = B->d_Hi[1] = B->d_Hi[2] = nullptr; B->d_Yi = nullptr;
+        B->Mi = 0;
+        int rc = 0;
+        if ((rc = dev_alloc(&B->d_Xi, (size_t)Mi * kMlpIn)) || (rc = dev_alloc(&B->d_Hi[0], (size_t)Mi * kMlpHid)) ||
+            (rc = dev_alloc(&B->d_Hi[1], (size_t)Mi * kMlpHid)) || (rc = dev_alloc(&B->d_Hi[2], (size_t)Mi * kMlpHid)) ||
+            (rc = dev_alloc(&B->d_Yi, (size_t)Mi * kMlpOut)))
+            return rc;
+        TP_HIP(hipMemset(B->d_Xi, 0, sizeof(uint16_t) * (size_t)Mi * kMlpIn));
+        B->Mi = Mi;
+    }
+    B->mlp = nlearn > 0 ? m : nullptr;
+    B->nlearn = nlearn;
+    B->noise = noise;
+    std::vector<GroupState> gs(B->ngroups);
+    TP_HIP(hipMemcpy(gs.data(), B->d_gs, sizeof(GroupState) * B->ngroups, hipMemcpyDeviceToHost));
+    for (auto &g : gs) g.nlearn = nlearn;
+    TP_HIP(hipMemcpy(B->d_gs, gs.data(), sizeof(GroupState) * B->ngroups, hipMemcpyHostToDevice));
+    for (auto &g : B->h_gs) g.nlearn = nlearn;
+    if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
+    return 0;
 }
 
 }  // extern "C"

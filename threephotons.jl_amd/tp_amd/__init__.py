@@ -378,3 +378,90 @@ class AnnealBatch:
         E = np.zeros(self.ngroups * self.nprop)
         check(lib().tp_batch_eval_quats(self.h, q.ctypes.data, E.ctypes.data), "tp_batch_eval_quats")
         return E
+
+
+def _ab_log(self, capacity):
+    check(lib().tp_batch_log(self.h, int(capacity)), "tp_batch_log")
+
+
+def _ab_log_count(self):
+    n = C.c_int32()
+    check(lib().tp_batch_log_count(self.h, C.byref(n)), "tp_batch_log_count")
+    return n.value
+
+
+def _ab_log_read(self, first, n):
+    X = np.zeros((n, 192), np.float32)
+    T = np.zeros((n, 36), np.float32)
+    check(lib().tp_batch_log_read(self.h, first, n, X.ctypes.data, T.ctypes.data), "tp_batch_log_read")
+    return X, T
+
+
+def _ab_set_proposer(self, mlp, nlearn, noise=0.25):
+    check(lib().tp_batch_set_proposer(self.h, mlp.h if mlp is not None else None, int(nlearn), float(noise)),
+          "tp_batch_set_proposer")
+    self._mlp = mlp
+
+
+AnnealBatch.log = _ab_log
+AnnealBatch.log_count = _ab_log_count
+AnnealBatch.log_read = _ab_log_read
+AnnealBatch.set_proposer = _ab_set_proposer
+
+
+class MlpProposer:
+    """The learned rotation proposer (tp_mlp_*): BF16 MLP 192 -> 256x3 -> 64
+    whose first 36 outputs are one quaternion per even l = 2..18."""
+
+    def __init__(self, seed=0, ctx=None):
+        self.ctx = _ctx(ctx)
+        h = C.c_void_p()
+        check(lib().tp_mlp_create(self.ctx.h, int(seed), C.byref(h)), "tp_mlp_create")
+        self.h = h
+        self.nparam = lib().tp_mlp_nparam()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().tp_mlp_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        check(lib().tp_mlp_get_params(self.h, p.ctypes.data), "tp_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().tp_mlp_set_params(self.h, p.ctypes.data), "tp_mlp_set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 64), np.float32)
+        check(lib().tp_mlp_forward(self.h, X.ctypes.data, X.shape[0], Y.ctypes.data), "tp_mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_double()
+        check(lib().tp_mlp_loss_grad(self.h, X.ctypes.data, T.ctypes.data, X.shape[0], C.byref(loss), g.ctypes.data),
+              "tp_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, X, T, steps, batch=512, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        tr = np.zeros(max(steps, 1), np.float32)
+        check(lib().tp_mlp_train(self.h, X.ctypes.data, T.ctypes.data, X.shape[0], steps, batch, lr, seed,
+                                 tr.ctypes.data), "tp_mlp_train")
+        return tr[:steps]
+
+    def train_log(self, anneal_batch, steps, batch=512, lr=1e-3, seed=0):
+        tr = np.zeros(max(steps, 1), np.float32)
+        check(lib().tp_mlp_train_log(self.h, anneal_batch.h, steps, batch, lr, seed, tr.ctypes.data),
+              "tp_mlp_train_log")
+        return tr[:steps]

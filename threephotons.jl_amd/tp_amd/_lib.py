@@ -85,6 +85,20 @@ def lib():
     L.tp_batch_history.argtypes = [vp, vp, i32]
     L.tp_batch_eval_quats.argtypes = [vp, vp, vp]
     L.tp_rotate.argtypes = [vp, vp, vp, i64, i32, vp, vp]
+    L.tp_mlp_nparam.restype = i32
+    L.tp_mlp_create.argtypes = [vp, C.c_uint64, pp]
+    L.tp_mlp_destroy.argtypes = [vp]
+    L.tp_mlp_destroy.restype = None
+    L.tp_mlp_get_params.argtypes = [vp, vp]
+    L.tp_mlp_set_params.argtypes = [vp, vp]
+    L.tp_mlp_forward.argtypes = [vp, vp, i32, vp]
+    L.tp_mlp_loss_grad.argtypes = [vp, vp, vp, i32, C.POINTER(dbl), vp]
+    L.tp_mlp_train.argtypes = [vp, vp, vp, i32, i32, i32, C.c_float, C.c_uint64, vp]
+    L.tp_mlp_train_log.argtypes = [vp, vp, i32, i32, C.c_float, C.c_uint64, vp]
+    L.tp_batch_log.argtypes = [vp, i32]
+    L.tp_batch_log_count.argtypes = [vp, vp]
+    L.tp_batch_log_read.argtypes = [vp, i32, i32, vp, vp]
+    L.tp_batch_set_proposer.argtypes = [vp, vp, i32, dbl]
     _lib = L
     return L
 
@@ -115,7 +129,9 @@ EXPORTS = [
     "tp_intensity_from_atoms", "tp_set_timing", "tp_last_timings", "tp_basis_set_precision",
     "tp_basis_get_precision", "tp_energy_batch", "tp_batch_create", "tp_batch_destroy", "tp_batch_autotune",
     "tp_batch_probe", "tp_batch_run", "tp_batch_group", "tp_batch_coeff", "tp_batch_history",
-    "tp_batch_eval_quats", "tp_rotate",
+    "tp_batch_eval_quats", "tp_rotate", "tp_mlp_nparam", "tp_mlp_create", "tp_mlp_destroy", "tp_mlp_get_params",
+    "tp_mlp_set_params", "tp_mlp_forward", "tp_mlp_loss_grad", "tp_mlp_train", "tp_mlp_train_log", "tp_batch_log",
+    "tp_batch_log_count", "tp_batch_log_read", "tp_batch_set_proposer",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16 = 0, 1, 2
