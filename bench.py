@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "threephotons.jl_amd"))
 
 METRIC = "c3 evals/s + MC annealing steps/s at K=26 L=18 N=32; time-to-energy vs plain FP64"
+SWEEP_FACTOR = 1.05   # angle-bound growth/shrink per sweep of the batched annealing (a sweep stands for many steps)
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (matrix and vector)
 HBM_PEAK_GBS = 8000.0
 
@@ -149,9 +150,9 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
     t_tune = time.perf_counter() - t0
     B.probe(0.1)
     E0 = [B.group(g)["E"] for g in range(len(Ks))]
-    B.run(2)   # graph capture and first replays, untimed
+    B.run(2, factor=SWEEP_FACTOR)   # graph capture and first replays, untimed
     t0 = time.perf_counter()
-    B.run(nsweeps)
+    B.run(nsweeps, factor=SWEEP_FACTOR)
     el = time.perf_counter() - t0
     st = [B.group(g) for g in range(len(Ks))]
     cols = sum(K * (K + 1) * (K + 2) // 6 for K in Ks) * nprop
@@ -171,8 +172,100 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
         "gemm16_tflops_algorithmic": flops / (tile_ms * 1e-3) / 1e12,
         "triple_ms": min(tune["order_ms"]), "gemm16_ms": tile_ms,
         "E0": E0, "E_final": [s["E"] for s in st], "accepted": [s["nacc"] for s in st],
-        "mode": "fp16x3 contraction, FP64 rotations/energy; best-of-nprop + Metropolis per sweep",
+        "mode": "fp16x3 contraction, FP64 rotations/energy; random slots = the reference's plane rotation in a "
+                "Haar frame (|b| <= pi/4); best-of-nprop + Metropolis per sweep; angle factor %.2f/sweep" % SWEEP_FACTOR,
     }
+
+
+def _time_to(Es, t_of, target):
+    """First time at which the energy trace reaches target (None if never)."""
+    idx = np.nonzero(np.asarray(Es) <= target)[0]
+    return None if idx.size == 0 else float(t_of(int(idx[0])))
+
+
+def learned_leg(cfg, tp, workload, truth, basis, c3ref, start, plain, data_sweeps, train_steps, cmp_sweeps,
+                tte_sweeps, nprop=64, nlearn=48, seeds=tuple(range(1000, 1020))):
+    """Learned proposer: (1) move log from the concurrent runs' sweeps
+    (random proposals only), (2) device training of the MLP on the log,
+    (3) the same runs continued with and without the learned slots
+    (nlearn of nprop), (4) time-to-energy on the plain run's own problem
+    (K=26, its start structure and histogram) against the plain FP64 anneal."""
+    Ks = [int(np.random.default_rng(s).integers(10, 27)) for s in seeds]
+    starts = [workload.random_start_structure(truth, cfg["L"], s) for s in seeds]
+    refs = [np.ascontiguousarray(c3ref[:, :, :K, :K, :K]) for K in Ks]
+    basis.set_precision("fp16x3")
+    out = {"nprop": nprop, "nlearn": nlearn}
+    # (1) data
+    B = tp.AnnealBatch(basis, refs, starts, seeds, nprop=nprop, hist_cap=data_sweeps + 8)
+    B.set_family("quat")   # random SO(3) moves: their accepted quaternions are the training labels
+    cap = 1_200_000
+    B.log(cap)
+    B.probe(0.1)
+    t0 = time.perf_counter()
+    B.run(data_sweeps, factor=SWEEP_FACTOR)
+    t_data = time.perf_counter() - t0
+    nlog = B.log_count()
+    out["data"] = {"sweeps": data_sweeps, "samples": nlog, "seconds": t_data,
+                   "samples_are": "(move that passes Metropolis alone, shell) pairs"}
+    log(f"learned: {nlog} logged samples from {data_sweeps} sweeps in {t_data:.1f}s")
+    # (2) training
+    mlp = tp.MlpProposer(seed=1)
+    t0 = time.perf_counter()
+    tr = mlp.train_log(B, train_steps, batch=512, lr=1e-3, seed=2)
+    t_train = time.perf_counter() - t0
+    out["train"] = {"steps": train_steps, "batch": 512, "seconds": t_train, "samples_seen": train_steps * 512,
+                    "loss_first": float(tr[:20].mean()), "loss_last": float(tr[-20:].mean())}
+    log(f"learned: trained {train_steps} steps in {t_train:.2f}s, loss {tr[:20].mean():.4f} -> {tr[-20:].mean():.4f}")
+    del B
+    # (3) continuation with / without the network from the same start
+    res = {}
+    for name, nl in (("random", 0), ("learned", nlearn)):
+        Bc = tp.AnnealBatch(basis, refs, starts, seeds, nprop=nprop, hist_cap=cmp_sweeps + 8)
+        Bc.probe(0.1)
+        if nl:
+            Bc.set_proposer(mlp, nl, noise=0.25)
+        Bc.run(1, factor=SWEEP_FACTOR)
+        t0 = time.perf_counter()
+        Bc.run(cmp_sweeps, factor=SWEEP_FACTOR)
+        el = time.perf_counter() - t0
+        st = [Bc.group(g) for g in range(len(Ks))]
+        h = Bc.history()
+        res[name] = {"ms_per_sweep": 1e3 * el / cmp_sweeps, "E_final": [x["E"] for x in st],
+                     "accepted": [x["nacc"] for x in st], "mean_dE": float(np.mean(h[cmp_sweeps] - h[0]))}
+    out["compare"] = res
+    out["compare"]["gain_mean_dE"] = res["learned"]["mean_dE"] / res["random"]["mean_dE"] \
+        if res["random"]["mean_dE"] != 0 else None
+    log(f"learned: mean dE over {cmp_sweeps} sweeps random {res['random']['mean_dE']:.3f} "
+        f"learned {res['learned']['mean_dE']:.3f}")
+    # (4) time-to-energy on the plain run's problem
+    E_plain = plain["E_final"]
+    t_plain = plain["seconds"]
+    tte = {}
+    for name, nl in (("batched_fp16x3_random", 0), ("batched_fp16x3_learned", nlearn)):
+        Bt = tp.AnnealBatch(basis, [c3ref], [start], [1000], nprop=nprop, hist_cap=tte_sweeps + 8)
+        Bt.probe(0.1)
+        if nl:
+            Bt.set_proposer(mlp, nl, noise=0.25)
+        t0 = time.perf_counter()
+        Bt.run(tte_sweeps, factor=SWEEP_FACTOR)
+        el = time.perf_counter() - t0
+        h = Bt.history()[:tte_sweeps, 0]
+        per = el / tte_sweeps
+        tte[name] = {"sweeps": tte_sweeps, "ms_per_sweep": 1e3 * per, "E_final": float(h[-1]),
+                     "t_within_0.1pct": _time_to(h, lambda i: (i + 1) * per, E_plain * 1.001),
+                     "t_reach_plain_final": _time_to(h, lambda i: (i + 1) * per, E_plain)}
+    Ep = np.asarray(plain["E_trace"])
+    per_step = t_plain / len(Ep)
+    tte["plain_fp64"] = {"steps": len(Ep), "seconds": t_plain, "E_final": E_plain,
+                         "t_within_0.1pct": _time_to(Ep, lambda i: (i + 1) * per_step, E_plain * 1.001),
+                         "t_reach_plain_final": t_plain}
+    lf = tte["batched_fp16x3_learned"]["t_reach_plain_final"]
+    tte["speedup_reach_plain_final"] = (t_plain / lf) if lf else None
+    tte["E0"] = float(plain["E0"])
+    out["time_to_energy"] = tte
+    log(f"learned: time-to-energy {tte}")
+    basis.set_precision("fp64")
+    return out
 
 
 def main():
@@ -184,6 +277,11 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-sweeps", type=int, default=20, help="sweeps of the concurrent-runs leg (0: skip)")
+    ap.add_argument("--learned-data-sweeps", type=int, default=100,
+                    help="sweeps logged for the learned proposer (0: skip the learned leg)")
+    ap.add_argument("--learned-train-steps", type=int, default=2000)
+    ap.add_argument("--learned-cmp-sweeps", type=int, default=40)
+    ap.add_argument("--learned-tte-sweeps", type=int, default=200)
     ap.add_argument("--precision-study", type=int, default=0,
                     help="instead of the benchmark: c3/energy errors of the mixed-precision modes against FP64 "
                          "for this many random orientations of the full-size structure")
@@ -260,6 +358,11 @@ def main():
     if args.batch_sweeps > 0:
         batch = batch_leg(cfg, tp, workload, truth, basis, c3ref, args.batch_sweeps)
         basis.set_precision("fp64")
+    learned = None
+    if args.learned_data_sweeps > 0:
+        plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
+        learned = learned_leg(cfg, tp, workload, truth, basis, c3ref, start, plain, args.learned_data_sweeps,
+                              args.learned_train_steps, args.learned_cmp_sweeps, args.learned_tte_sweeps)
 
     out = {
         "metric": METRIC,
@@ -298,6 +401,8 @@ def main():
     }
     if batch is not None:
         out["batched"] = batch
+    if learned is not None:
+        out["learned"] = learned
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)

@@ -156,15 +156,22 @@ int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, cons
 int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_ref *const *refs,
                     const double *const *coeff, const int64_t *ld, double *E_out);
 
+#define TP_MOVE_QUAT 0    /* random slots: an SO(3) rotation per even l (uniform axis, Haar angle) */
+#define TP_MOVE_PLANE 1   /* random slots: the reference's move, a rotation in the plane of a
+                           * Haar-random 2-frame of R^(2l+1) (default) */
+
 /* Concurrent annealing runs ("groups"), each with its own histogram refs[g]
  * (built on any basis sharing b's L, LMAX, N), shell count K_g = the upper end
  * of refs[g]'s K3_range (<= b's K), start coefficients coeff[g] (K_g shells)
  * and seed.  Every sweep proposes nprop (>= 3)
- * rotations per run: for each even l >= 2 an independent SO(3) rotation
- * (uniform axis, Haar-distributed angle <= amax * pi/4) applied to the
- * l-blocks of all shells through FP64 real Wigner-D matrices, which keeps
- * every two-photon correlation sum_m c_k1,lm^* c_k2,lm, as the reference's
- * moves do (get_MC_step, determination.jl:232-241).  All ngroups * nprop
+ * rotations per run, each acting on the l-blocks (even l >= 2) of all shells
+ * in FP64, which keeps every two-photon correlation sum_m c_k1,lm^* c_k2,lm
+ * as the reference's moves do (get_MC_step, determination.jl:232-241):
+ * TP_MOVE_PLANE (default) the reference's move, a rotation by
+ * b = amax (pi/4)(l-1) z (z normal, clipped to |b| <= pi/4) in the plane of
+ * a Haar-random 2-frame of R^(2l+1); TP_MOVE_QUAT an SO(3) rotation per l
+ * (uniform axis, Haar-distributed angle <= amax pi/4) through the real
+ * Wigner-D recurrence.  All ngroups * nprop
  * evaluations of a sweep, and the per-run decision, are one replay of a
  * captured HIP graph.  Decision per run: the lowest-energy proposal is
  * accepted by the Metropolis rule on its exact energy difference
@@ -184,6 +191,11 @@ int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out);
  * temperature_factor * std(energies of the other slots) (calculate_temperature,
  * determination.jl:244-261). */
 int tp_batch_probe(tp_batch *B, double temperature_factor);
+/* Move family of the random slots (TP_MOVE_PLANE or TP_MOVE_QUAT). */
+int tp_batch_set_family(tp_batch *B, int family);
+/* Angle bound of every run's random proposals, as a fraction of pi/4
+ * (0 < amax <= 1; 1 at creation); it then adapts by `factor` per sweep. */
+int tp_batch_set_amax(tp_batch *B, double amax);
 /* nsweeps sweeps (graph replays). */
 int tp_batch_run(tp_batch *B, int nsweeps, double decay, double factor);
 /* Per-run state: energy, temperature, angle bound, accepted sweeps, sweeps done. */
@@ -235,8 +247,9 @@ int tp_mlp_train(tp_mlp *m, const float *X, const float *T, int n, int steps, in
 /* The same on a batch's move log (the data never leaves the device). */
 int tp_mlp_train_log(tp_mlp *m, tp_batch *B, int steps, int batch, float lr, uint64_t seed, float *loss_trace);
 
-/* Keep a move log of up to `capacity` samples: after every sweep, each proposal
- * that passes the Metropolis test on its own against its run's current energy
+/* Keep a move log of up to `capacity` samples: after every sweep, each
+ * quaternion proposal (learned slots; random slots of TP_MOVE_QUAT) that
+ * passes the Metropolis test on its own against its run's current energy
  * adds one sample per shell of the run (that shell's features in the state the
  * move starts from, and the proposal's quaternions with w >= 0). */
 int tp_batch_log(tp_batch *B, int capacity);

@@ -128,10 +128,12 @@ def test_batch_eval_quats_matches_single(tp):
             assert abs(E[e] - Es) <= 1e-10 * abs(Es), (g, p, E[e], Es)
 
 
-def test_batch_sweeps(tp):
+@pytest.mark.parametrize("family", ["plane", "quat"])
+def test_batch_sweeps(tp, family):
     Ks, nprop, nsw = [5, 9, 7], 16, 25
     L, LMAX = 8, 25
     master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, hist_cap=nsw)
+    B.set_family(family)
     cfg = B.autotune(reps=2)
     assert cfg["tile"] in [(128, 64), (64, 64), (128, 128), (64, 128)] and cfg["order"] in (0, 1)
     B.probe(0.1)
@@ -158,6 +160,7 @@ def test_batch_sweeps(tp):
         assert st["E"] <= g0[g]["E"] + 5 * g0[g]["T"]
     # deterministic: a second batch with the same seeds follows the same path
     _, _, _, _, B2 = _anneal_setup(tp, Ks, nprop, hist_cap=nsw)
+    B2.set_family(family)
     B2.probe(0.1)
     B2.run(nsw, decay=0.999, factor=1.01)
     # (its contraction tile may differ from the autotuned one: same values up to summation order)

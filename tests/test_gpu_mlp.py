@@ -68,6 +68,7 @@ def test_move_log_and_learned_slots(tp):
     Ks, nprop = [5, 8, 6], 8
     L, LMAX = 8, 25
     master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, hist_cap=40)
+    B.set_family("quat")   # random quaternion moves: the log's training data
     B.log(100000)
     B.probe(0.5)
     B.run(10, decay=0.999, factor=1.01)

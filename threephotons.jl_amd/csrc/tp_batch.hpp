@@ -118,16 +118,67 @@ __global__ __launch_bounds__(256) void k_quat_random(const EvalDesc *__restrict_
 constexpr int kRotMaxL = 22;
 constexpr int kRotShellChunk = 16;
 
+// Move families of the random slots.
+#define TP_MOVE_QUAT 0    // SO(3) rotation per l (D^l of a quaternion)
+#define TP_MOVE_PLANE 1   // rotation in the plane of a Haar-random 2-frame of R^(2l+1)
+
+// With gs != nullptr and family == TP_MOVE_PLANE, the random slots (p <
+// nprop - nlearn; in probe mode all but slot 0) instead get the reference's
+// move (get_MC_step / random_rotation_step, determination.jl:232-241,
+// utilities.jl:180-190, written as in k_propose):
+//   D = I + (cos b - 1)(u u^T + v v^T) + sin b (v u^T - u v^T),
+// (u, v) Gram-Schmidt of two Gaussian vectors, b = amax (pi/4)(l-1) z clipped
+// to [-pi/4, pi/4] (the reference's step sizes (pi/4)(l-1), determination.jl:295).
+// A Haar-random frame is Haar-random in any fixed orthonormal basis, so the
+// real basis used here serves as well as the reference's Umat basis.
 __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict__ ev, const double *__restrict__ quat,
-                                                      int nl, int n_c)
+                                                      int nl, int n_c, const GroupState *__restrict__ gs, int family,
+                                                      int probe)
 {
     __shared__ double Dm[2][(2 * kRotMaxL + 1) * (2 * kRotMaxL + 1)];
     __shared__ double R1[9];
     __shared__ double2 xs[kRotShellChunk][2 * kRotMaxL + 1], xo[kRotShellChunk][2 * kRotMaxL + 1];
+    __shared__ double uv[2][64];
+    __shared__ double s_beta;
     const int li = blockIdx.x, e = blockIdx.y, l = 2 * (li + 1);
     const int tid = threadIdx.x;
     const EvalDesc d = ev[e];
-    if (tid == 0) {
+    bool plane = false;
+    GroupState g;
+    if (gs && family == TP_MOVE_PLANE) {
+        g = gs[d.group];
+        plane = probe ? d.p != 0 : d.p < d.nprop - g.nlearn;
+    }
+    if (plane) {
+        const int n = 2 * l + 1;
+        const uint32_t step = probe ? 0xFFFFFFFFu : (uint32_t)g.nsweep;
+        const uint32_t lid = (uint32_t)li | ((uint32_t)d.p << 8);
+        if (tid < 64) {
+            const int lane = tid;
+            const double g1 = lane < n ? dev_normal(g.seed, step, lid, (uint32_t)lane, TP_TAG_MOVE) : 0.0;
+            const double g2 = lane < n ? dev_normal(g.seed, step, lid, (uint32_t)(n + lane), TP_TAG_MOVE) : 0.0;
+            const double u = g1 / sqrt(wave_sum(g1 * g1));
+            const double dd = wave_sum(u * g2);
+            const double vt = g2 - dd * u;
+            const double v = vt / sqrt(wave_sum(vt * vt));
+            uv[0][lane] = lane < n ? u : 0.0;
+            uv[1][lane] = lane < n ? v : 0.0;
+            if (lane == 0) {
+                const double q4 = 0.25 * kPiD;
+                double b = g.amax * q4 * (l - 1) * dev_normal(g.seed, step, lid, (uint32_t)(2 * n), TP_TAG_MOVE);
+                s_beta = b > q4 ? q4 : (b < -q4 ? -q4 : b);
+            }
+        }
+        __syncthreads();
+        const double cm = cos(s_beta) - 1.0, sb = sin(s_beta);
+        for (int idx = tid; idx < n * n; idx += 256) {
+            const int r = idx / n, c = idx % n;
+            const double ur = uv[0][r], vr = uv[1][r], uc = uv[0][c], vc = uv[1][c];
+            Dm[0][idx] = (r == c ? 1.0 : 0.0) + cm * (ur * uc + vr * vc) + sb * (vr * uc - ur * vc);
+        }
+        __syncthreads();
+    }
+    if (!plane && tid == 0) {
         const double *q4 = quat + ((size_t)e * nl + li) * 4;
         double w = q4[0], x = q4[1], y = q4[2], z = q4[3];
         const double nq = sqrt(w * w + x * x + y * y + z * z);
@@ -140,9 +191,10 @@ __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict
             for (int b = 0; b < 3; ++b) R1[a * 3 + b] = Rc[cix[a]][cix[b]];
     }
     __syncthreads();
-    for (int i = tid; i < 9; i += 256) Dm[1][i] = R1[i];
+    if (!plane)
+        for (int i = tid; i < 9; i += 256) Dm[1][i] = R1[i];
     __syncthreads();
-    for (int lp = 2; lp <= l; ++lp) {
+    for (int lp = 2; lp <= (plane ? 1 : l); ++lp) {
         const double *pv = Dm[(lp - 1) & 1];
         double *cur = Dm[lp & 1];
         const int n = 2 * lp + 1, np = 2 * lp - 1;
@@ -178,7 +230,7 @@ __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict
         }
         __syncthreads();
     }
-    const double *D = Dm[l & 1];
+    const double *D = Dm[0];   // l is even
     const int n = 2 * l + 1, off = l * (l - 1) / 2;
     const double h = 0.70710678118654752440;
     if (li == 0)

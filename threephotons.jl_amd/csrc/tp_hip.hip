@@ -298,6 +298,7 @@ struct tp_batch {
     int64_t ncb = 0, ldA = 0, cstride = 0;
     int nwork = 0;
     int bm = H_BM, bn = H_BN, order = 0;   // contraction tile, triple-product work order (autotuned)
+    int family = TP_MOVE_PLANE;            // move family of the random slots
     double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     EvalDesc *d_ev = nullptr;
     GroupState *d_gs = nullptr;
@@ -573,11 +574,11 @@ static int batch_enqueue_sweep(const tp_batch *B, hipStream_t st, int probe, dou
                        B->d_gs, probe, B->d_quat);
     if (!probe && B->mlp && B->nlearn > 0) batch_learned_quats(B, st);
     hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)B->nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev, B->d_quat,
-                       B->nl, b->n_c);
+                       B->nl, b->n_c, B->d_gs, B->family, probe);
     if (int rc = batch_enqueue_eval(B, st)) return rc;
     if (!probe && B->log_cap > 0)
         hipLaunchKernelGGL(k_log_moves, dim3(1), dim3(1024), 0, st, B->d_ev, B->ne, B->nl, B->d_gs, B->d_E, B->d_quat,
-                           b->n_c, B->d_Xlog, B->d_Tlog, B->d_logn, B->log_cap);
+                           b->n_c, B->d_Xlog, B->d_Tlog, B->d_logn, B->log_cap, B->family);
     hipLaunchKernelGGL(k_select_batch, dim3((unsigned)B->ngroups), dim3(256), 0, st, B->d_gs, B->d_ev, B->d_E,
                        B->d_cur, B->cstride, b->n_c, probe, decay, factor, tfac, B->d_Ehist, B->hist_cap,
                        B->ngroups);
@@ -1745,7 +1746,7 @@ int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out)
         hipLaunchKernelGGL(k_quat_random, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, B->d_ev, B->ne,
                            B->nl, B->d_gs, 1, B->d_quat);
         hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)B->nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev,
-                           B->d_quat, B->nl, B->b->n_c);
+                           B->d_quat, B->nl, B->b->n_c, B->d_gs, B->family, 1);
         TP_HIP(hipGetLastError());
     }
     if (int rc = batch_autotune(B, reps)) return rc;
@@ -1829,7 +1830,7 @@ int tp_batch_eval_quats(tp_batch *B, const double *quat, double *E_out)
     hipStream_t st = B->ctx->stream;
     TP_HIP(hipMemcpyAsync(B->d_quat, quat, sizeof(double) * (size_t)B->ne * B->nl * 4, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)B->nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev, B->d_quat,
-                       B->nl, B->b->n_c);
+                       B->nl, B->b->n_c, nullptr, TP_MOVE_QUAT, 0);
     TP_HIP(hipGetLastError());
     if (int rc = batch_enqueue_eval(B, st)) return rc;
     TP_HIP(hipMemcpyAsync(E_out, B->d_E, sizeof(double) * B->ne, hipMemcpyDeviceToHost, st));
@@ -1935,7 +1936,8 @@ int tp_rotate(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, 
         hipError_t e = hipMemcpyAsync(dq, quat, sizeof(double) * nl * 4, hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(dev, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)nl, 1u), dim3(256), 0, ctx->stream, dev, dq, nl, b->n_c);
+            hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)nl, 1u), dim3(256), 0, ctx->stream, dev, dq, nl, b->n_c,
+                               nullptr, TP_MOVE_QUAT, 0);
             e = hipGetLastError();
         }
         if (e == hipSuccess)
@@ -2178,6 +2180,26 @@ int tp_batch_set_proposer(tp_batch *B, tp_mlp *m, int nlearn, double noise)
     for (auto &g : gs) g.nlearn = nlearn;
     TP_HIP(hipMemcpy(B->d_gs, gs.data(), sizeof(GroupState) * B->ngroups, hipMemcpyHostToDevice));
     for (auto &g : B->h_gs) g.nlearn = nlearn;
+    if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
+    return 0;
+}
+
+int tp_batch_set_amax(tp_batch *B, double amax)
+{
+    if (!B || !B->sweep || !(amax > 0.0) || amax > 1.0) return tp_fail(TP_E_ARG, "need 0 < amax <= 1");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    std::vector<GroupState> gs(B->ngroups);
+    TP_HIP(hipMemcpy(gs.data(), B->d_gs, sizeof(GroupState) * B->ngroups, hipMemcpyDeviceToHost));
+    for (auto &g : gs) g.amax = amax;
+    TP_HIP(hipMemcpy(B->d_gs, gs.data(), sizeof(GroupState) * B->ngroups, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int tp_batch_set_family(tp_batch *B, int family)
+{
+    if (!B || !B->sweep || (family != TP_MOVE_QUAT && family != TP_MOVE_PLANE)) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    B->family = family;
     if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
     return 0;
 }

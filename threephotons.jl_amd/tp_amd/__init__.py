@@ -403,6 +403,18 @@ def _ab_set_proposer(self, mlp, nlearn, noise=0.25):
     self._mlp = mlp
 
 
+def _ab_set_amax(self, amax):
+    check(lib().tp_batch_set_amax(self.h, float(amax)), "tp_batch_set_amax")
+
+
+def _ab_set_family(self, family):
+    """'plane' (the reference's move, default) or 'quat' (SO(3) per l)."""
+    f = {"quat": 0, "plane": 1}[family] if isinstance(family, str) else int(family)
+    check(lib().tp_batch_set_family(self.h, f), "tp_batch_set_family")
+
+
+AnnealBatch.set_family = _ab_set_family
+AnnealBatch.set_amax = _ab_set_amax
 AnnealBatch.log = _ab_log
 AnnealBatch.log_count = _ab_log_count
 AnnealBatch.log_read = _ab_log_read
