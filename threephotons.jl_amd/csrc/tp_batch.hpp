@@ -14,6 +14,7 @@
 #pragma once
 
 #include "tp_kernels.hpp"
+#include "tp_triple16.hpp"
 
 #define TP_TAG_QAXIS 0x51415849u
 #define TP_TAG_QANGLE 0x51414E47u
@@ -295,11 +296,29 @@ __global__ __launch_bounds__(512) void k_triple_batch(const EvalDesc *__restrict
                      d.cbase, d.nprop, d.p);
 }
 
+// Mixed-precision triple products of a batch (triple16_body); shell scales
+// of evaluation e at shsc + e * kst.
+__global__ __launch_bounds__(512) void k_triple16_batch(const EvalDesc *__restrict__ ev, const int2 *__restrict__ work,
+                                                        const double *__restrict__ shsc, int kst, int n_c,
+                                                        const float *__restrict__ ew,
+                                                        const uint32_t *__restrict__ eidx,
+                                                        const int *__restrict__ gstart,
+                                                        const int4 *__restrict__ ltinfo, int Lw, int Emax, int Prows,
+                                                        double *__restrict__ As, int64_t ldA)
+{
+    extern __shared__ __align__(16) char smem16b[];
+    const int2 w = work[blockIdx.x];
+    const EvalDesc &d = ev[w.x];
+    triple16_body(smem16b, d.coef, shsc + (size_t)w.x * kst, n_c, d.K, w.y & 0xffff, w.y >> 16, ew, eidx, gstart,
+                  ltinfo, Lw, Emax, Prows, As, ldA, d.cbase, d.nprop, d.p);
+}
+
 // Per-column power-of-two scales of an evaluation's As columns (the bound of
-// k_split_as), one workgroup per evaluation; the padding columns of its last
-// tile get scale 0.
+// k_split_as) and its shell scales (triple16_body), one workgroup per
+// evaluation; the padding columns of its last tile get scale 0.
 __global__ __launch_bounds__(256) void k_colscale_batch(const EvalDesc *__restrict__ ev, int n_c, int L,
-                                                        double *__restrict__ sc, double *__restrict__ isc)
+                                                        double *__restrict__ sc, double *__restrict__ isc,
+                                                        double *__restrict__ shsc, int kst)
 {
     __shared__ double s_n2[64 * 12];
     __shared__ double s_nk[64];
@@ -316,7 +335,9 @@ __global__ __launch_bounds__(256) void k_colscale_batch(const EvalDesc *__restri
     if (tid < K) {
         double m = 0.0;
         for (int l = 0; l < nl; ++l) m = fmax(m, s_n2[tid * nl + l]);
-        s_nk[tid] = sqrt(m);
+        const double nk = sqrt(m);
+        s_nk[tid] = nk;
+        shsc[(size_t)blockIdx.x * kst + tid] = nk > 0.0 ? ldexp(1.0, 12 - ilogb(nk)) : 1.0;
     }
     __syncthreads();
     const int klength = K * (K + 1) * (K + 2) / 6, ncol = (klength + 63) / 64 * 64;

@@ -14,6 +14,7 @@
 #include "../../include/tp.h"
 #include "tp_host_math.hpp"
 #include "tp_kernels.hpp"
+#include "tp_triple16.hpp"
 #include "tp_batch.hpp"
 #include "tp_mlp.hpp"
 
@@ -94,7 +95,12 @@ struct tp_basis {
     // MFMA triple products (K <= kMfmaMaxK): work list of (lt, k1 half)
     bool use_mfma = false;
     int nwork = 0, Lw = 0, EmaxLt = 0, Prows = 0;
-    size_t mfma_lds = 0;
+    size_t mfma_lds = 0, t16_lds = 0;   // dynamic LDS of k_triple_mfma, k_triple16
+    int EmaxLt8 = 0;                    // entries of an l-triple with every group padded to a multiple of 8
+    float *d_ew8 = nullptr;             // [..] weights, groups padded to multiples of 8 with zeros
+    uint32_t *d_eidx8 = nullptr;        // [..]
+    int *d_gstart8 = nullptr;           // [n_groups + 1]
+    double *d_shsc = nullptr;           // [K] shell scales of the mixed-precision triple products
     int *d_work = nullptr;         // [nwork] lt | half << 16, most expensive first
     std::vector<int> h_lt_cnt, h_lt_l3;   // per l-triple: basis entries, l3 (work lists of batches)
     // mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_BF16)
@@ -189,7 +195,14 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     const bool timing = ctx->timing;
     hipStream_t st = ctx->stream;
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    if (b->use_mfma) {
+    if (b->use_mfma && b->prec != TP_PREC_FP64) {
+        // mixed precision: compensated FP16 triple products
+        hipLaunchKernelGGL(k_shell_scales, dim3(1), dim3(256), 0, st, b0, b1, sel, flip, b->K, b->n_c, b->L,
+                           b->d_shsc);
+        hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
+                           b->d_shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
+                           b->EmaxLt8, b->Prows, b->d_As, b->kl_pad);
+    } else if (b->use_mfma) {
         hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(512), b->mfma_lds, st, b0, b1, sel, flip,
                            b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_work, b->Lw, b->EmaxLt,
                            b->Prows, b->d_As, b->kl_pad);
@@ -305,7 +318,7 @@ struct tp_batch {
     int2 *d_work = nullptr;
     double2 *d_cur = nullptr, *d_prop = nullptr;
     double *d_quat = nullptr, *d_As = nullptr, *d_sc = nullptr, *d_isc = nullptr, *d_part = nullptr;
-    double *d_E = nullptr, *d_Ehist = nullptr;
+    double *d_E = nullptr, *d_Ehist = nullptr, *d_shsc = nullptr;   // d_shsc [ne][K] shell scales
     uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;
     const double **d_qtab = nullptr;
     int *d_nvalid = nullptr;
@@ -338,7 +351,7 @@ static void batch_free(tp_batch *B)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As,
                     B->d_sc,  B->d_isc, B->d_part, B->d_E,    B->d_Ehist,  B->d_Ahi,    B->d_Alo,
-                    (void *)B->d_qtab, B->d_nvalid, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn,
+                    (void *)B->d_qtab, B->d_nvalid, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn, B->d_shsc,
                     B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -464,6 +477,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         (rc = dev_alloc(&B->d_As, (size_t)b->n_lt * B->ldA)) || (rc = dev_alloc(&B->d_sc, B->ldA)) ||
         (rc = dev_alloc(&B->d_isc, B->ldA)) || (rc = dev_alloc(&B->d_part, nrt_max * 2 * B->ldA)) ||
         (rc = dev_alloc(&B->d_E, B->ne)) || (rc = dev_alloc(&B->d_Ehist, (size_t)std::max(1, hist_cap) * ngroups)) ||
+        (rc = dev_alloc(&B->d_shsc, (size_t)B->ne * b->K)) ||
         (rc = dev_alloc(&B->d_Ahi, (size_t)B->ldA * b->ldk)) ||
         (b->prec == TP_PREC_FP16X3 && (rc = dev_alloc(&B->d_Alo, (size_t)B->ldA * b->ldk))) ||
         (rc = dev_alloc(&B->d_qtab, qtab.size())) || (rc = dev_alloc(&B->d_nvalid, nvalid.size())) ||
@@ -528,10 +542,10 @@ static void batch_triple(const tp_batch *B, hipStream_t st)
 {
     const tp_basis *b = B->b;
     hipLaunchKernelGGL(k_colscale_batch, dim3((unsigned)B->ne), dim3(256), 0, st, B->d_ev, b->n_c, b->L, B->d_sc,
-                       B->d_isc);
-    hipLaunchKernelGGL(k_triple_batch, dim3((unsigned)B->nwork), dim3(512), b->mfma_lds, st, B->d_ev, B->d_work,
-                       b->n_c, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->Lw, b->EmaxLt, b->Prows, B->d_As,
-                       B->ldA);
+                       B->d_isc, B->d_shsc, b->K);
+    hipLaunchKernelGGL(k_triple16_batch, dim3((unsigned)B->nwork), dim3(512), b->t16_lds, st, B->d_ev, B->d_work,
+                       B->d_shsc, b->K, b->n_c, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->Lw, b->EmaxLt8,
+                       b->Prows, B->d_As, B->ldA);
 }
 
 static void batch_split(const tp_batch *B, hipStream_t st)
@@ -1002,7 +1016,7 @@ void tp_basis_destroy(tp_basis *b)
     void *ptrs[] = {b->d_H,   b->d_Bm,  b->d_ew,   b->d_eidx,    b->d_gstart, b->d_ltinfo, b->d_chunks,
                     b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_blk,
                     b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
-                    b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv};
+                    b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1103,6 +1117,20 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
             }
     gstart.push_back((int)ew.size());
     const int nlt = (int)lt_entries.size();
+    // the same groups padded to multiples of 8 entries (zero weight, the
+    // group's first index) for the FP16 triple products
+    std::vector<float> ew8;
+    std::vector<uint32_t> eidx8;
+    std::vector<int> gstart8;
+    for (size_t g = 0; g + 1 < gstart.size(); ++g) {
+        gstart8.push_back((int)ew8.size());
+        const int a = gstart[g], n = gstart[g + 1] - gstart[g], n8 = (n + 7) / 8 * 8;
+        for (int i = 0; i < n8; ++i) {
+            ew8.push_back(i < n ? (float)ew[a + i] : 0.f);
+            eidx8.push_back(eidx[i < n ? a + i : a]);
+        }
+    }
+    gstart8.push_back((int)ew8.size());
     b->basislen = basislen;
     b->n_lt = nlt;
     b->n_groups = (int)gstart.size() - 1;
@@ -1197,10 +1225,22 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
         int maxlds = 0;
         if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
             maxlds = 65536;
-        b->use_mfma = b->mfma_lds <= (size_t)maxlds && maxrb <= 16;
+        b->EmaxLt8 = 0;
+        for (int t = 0; t < nlt; ++t) {
+            const int g0 = ltinfo[t].y, n3 = 2 * ltinfo[t].z + 1;
+            b->EmaxLt8 = std::max(b->EmaxLt8, gstart8[g0 + n3] - gstart8[g0]);
+        }
+        b->t16_lds = triple16_lds(K, b->Lw, b->EmaxLt8, b->Prows);
+        b->use_mfma = b->mfma_lds <= (size_t)maxlds && b->t16_lds <= (size_t)maxlds && maxrb <= 16;
         if (b->use_mfma && b->mfma_lds > 65536)
             (void)hipFuncSetAttribute((const void *)k_triple_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)b->mfma_lds);
+        if (b->use_mfma && b->t16_lds > 65536) {
+            (void)hipFuncSetAttribute((const void *)k_triple16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->t16_lds);
+            (void)hipFuncSetAttribute((const void *)k_triple16_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->t16_lds);
+        }
     }
 
     // padded dims; H and Bm
@@ -1241,7 +1281,8 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
     if ((rc = up(&b->d_H, H)) || (rc = up(&b->d_Bm, Bm)) || (rc = up(&b->d_ew, ew)) || (rc = up(&b->d_eidx, eidx)) ||
         (rc = up(&b->d_gstart, gstart)) || (rc = up(&b->d_ltinfo, ltinfo)) || (rc = up(&b->d_chunks, chunks)) ||
         (rc = up(&b->d_pairs, pairs)) || (rc = up(&b->d_sean0, b->sean0)) || (rc = up(&b->d_coff, b->coff)) ||
-        (b->use_mfma && (rc = up(&b->d_work, work)))) {
+        (b->use_mfma && (rc = up(&b->d_work, work))) || (rc = up(&b->d_ew8, ew8)) || (rc = up(&b->d_eidx8, eidx8)) ||
+        (rc = up(&b->d_gstart8, gstart8))) {
         tp_basis_destroy(b);
         return rc;
     }
@@ -1249,7 +1290,8 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
         (rc = dev_alloc(&b->d_G, (size_t)b->nm_pad64 * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_part, (size_t)b->n_rt * 2 * b->kl_pad)) || (rc = dev_alloc(&b->d_blk, b->n_fb)) ||
         (rc = dev_alloc(&b->d_counter, 1)) || (rc = dev_alloc(&b->d_out, 8)) ||
-        (rc = dev_alloc(&b->d_raw, (size_t)K * LMAX * LMAX)) || (rc = dev_alloc(&b->d_coef, (size_t)K * b->n_c))) {
+        (rc = dev_alloc(&b->d_raw, (size_t)K * LMAX * LMAX)) || (rc = dev_alloc(&b->d_coef, (size_t)K * b->n_c)) ||
+        (rc = dev_alloc(&b->d_shsc, (size_t)K))) {
         tp_basis_destroy(b);
         return rc;
     }
