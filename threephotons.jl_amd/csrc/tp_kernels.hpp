@@ -200,7 +200,7 @@ struct S1Ops {
 
 __device__ __forceinline__ S1Ops s1_fetch(int ks, int E, int s, int K, int k1s, int l16, int lq, int n1, int n2,
                                           const double *sw, const uint32_t *si, const double2 *c1b,
-                                          const double2 *c2b)
+                                          const double2 *c2b, bool two)
 {
     S1Ops o = {0.0, 0.0, 0.0, 0.0, 0.0};
     const int ep = 4 * ks + lq;
@@ -220,7 +220,7 @@ __device__ __forceinline__ S1Ops s1_fetch(int ks, int E, int s, int K, int k1s, 
             o.br0 = first ? c.x : -c.y;
             o.bi0 = first ? c.y : c.x;
         }
-        if (k2b <= K) {
+        if (two && k2b <= K) {
             const double2 c = c2b[(k2b - 1) * n2 + i2];
             o.br1 = first ? c.x : -c.y;
             o.bi1 = first ? c.y : c.x;
@@ -293,6 +293,7 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
     dbl4 acc2[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc2[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int nob = h ? 2 * nrb : nrb;   // stage-2 output blocks
     const int niter = (n3 + 7) / 8;
     const int k1s = 16 * h + l16 + 1;   // stage-1 A row of this lane
     for (int t = 0; t < niter; ++t) {
@@ -305,19 +306,22 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
 #pragma unroll
             for (int j = 0; j < 4; ++j) a1[j] = dbl4{0.0, 0.0, 0.0, 0.0};
             // operands of k-step ks are fetched from LDS one step ahead of the MFMAs
-            S1Ops cur = s1_fetch(0, E, s, K, k1s, l16, lq, n1, n2, sw, si, c1b, c2b);
+            S1Ops cur = s1_fetch(0, E, s, K, k1s, l16, lq, n1, n2, sw, si, c1b, c2b, h > 0);
             for (int ks = 0; ks < nks; ++ks) {
                 S1Ops nxt = cur;
-                if (ks + 1 < nks) nxt = s1_fetch(ks + 1, E, s, K, k1s, l16, lq, n1, n2, sw, si, c1b, c2b);
+                if (ks + 1 < nks) nxt = s1_fetch(ks + 1, E, s, K, k1s, l16, lq, n1, n2, sw, si, c1b, c2b, h > 0);
                 a1[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.br0, a1[0], 0, 0, 0);
-                a1[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.br1, a1[1], 0, 0, 0);
                 a1[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.bi0, a1[2], 0, 0, 0);
-                a1[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.bi1, a1[3], 0, 0, 0);
+                if (h > 0) {   // k1 <= 16 needs no k2 > 16
+                    a1[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.br1, a1[1], 0, 0, 0);
+                    a1[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(cur.a, cur.bi1, a1[3], 0, 0, 0);
+                }
                 cur = nxt;
             }
             // rows k1 = 16h + lq + 4v + 1, columns k2 = 16*(j&1) + l16 + 1
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
+                if ((j & 1) && h == 0) continue;
                 const int k2 = 16 * (j & 1) + l16 + 1;
                 const int slot = 2 * wave + (j >> 1);
 #pragma unroll
@@ -331,12 +335,13 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
             }
         }
         __syncthreads();
-        // stage 2: blocks b = wave + 8j -> (row block b>>1, k3 block b&1)
+        // stage 2: blocks b = wave + 8j -> (row block b>>1, k3 block b&1); for
+        // h = 0 only k3 block 0 (k3 <= k2 <= 16): b -> (b, 0)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int b = wave + 8 * j;
-            if (b < 2 * nrb) {
-                const int rb = b >> 1, cb = b & 1;
+            if (b < nob) {
+                const int rb = h ? b >> 1 : b, cb = h ? b & 1 : 0;
                 const int k3 = 16 * cb + l16 + 1;
 #pragma unroll
                 for (int kk = 0; kk < 4; ++kk) {
@@ -357,8 +362,8 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int b = wave + 8 * j;
-        if (b < 2 * nrb) {
-            const int rb = b >> 1, cb = b & 1;
+        if (b < nob) {
+            const int rb = h ? b >> 1 : b, cb = h ? b & 1 : 0;
             const int k3 = 16 * cb + l16 + 1;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {

@@ -153,7 +153,10 @@ __device__ __forceinline__ void triple16_body(char *smem, const double2 *__restr
 #pragma unroll
     for (int j = 0; j < 4; ++j) a2[j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int k1s = 16 * h + l16 + 1;   // stage-1 A row of this lane
-    const bool okA = k1s <= K, okB0 = l16 + 1 <= K, okB1 = l16 + 17 <= K;
+    // k1 <= 16 (h = 0): every pair has k2, k3 <= 16, so the second k2 / k3 block is skipped
+    const bool two = h > 0;
+    const bool okA = k1s <= K, okB0 = l16 + 1 <= K, okB1 = two && l16 + 17 <= K;
+    const int nob = two ? 2 * nrb : nrb;   // stage-2 output blocks (pair block, k3 block)
     const int nch = (n3 + 15) / 16;
     for (int ch = 0; ch < nch; ++ch) {
         // ---- stage 1: mu3 groups 16 ch + wave and 16 ch + 8 + wave
@@ -206,6 +209,7 @@ __device__ __forceinline__ void triple16_body(char *smem, const double2 *__restr
                 const h16x8 hah = __builtin_bit_cast(h16x8, ah), hal = __builtin_bit_cast(h16x8, al);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
+                    if ((q & 1) && !two) continue;
                     const h16x8 hbh = __builtin_bit_cast(h16x8, bh[q]), hbl = __builtin_bit_cast(h16x8, bl[q]);
                     th[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hal, hbh, th[q], 0, 0, 0);
                     th[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(hah, hbl, th[q], 0, 0, 0);
@@ -216,6 +220,7 @@ __device__ __forceinline__ void triple16_body(char *smem, const double2 *__restr
             const int sb = 2 * (mu - 16 * ch);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
+                if ((q & 1) && !two) continue;
                 const int k2 = 16 * (q & 1) + l16 + 1;
                 const int slot = sb + (q >> 1);
 #pragma unroll
@@ -231,10 +236,11 @@ __device__ __forceinline__ void triple16_body(char *smem, const double2 *__restr
         __syncthreads();
         // ---- stage 2: As[pair][k3] += T 2^-13 . [Re c_k3 ; -Im c_k3] over this chunk's 32 slots
         {
-            // output blocks b = wave + 8 j all have k3 block cb = wave & 1: one B operand per wave
+            // output blocks b = wave + 8 j all have k3 block cb = wave & 1 (0 when h = 0):
+            // one B operand per wave
             u16x8 b3h = {}, b3l = {};
             {
-                const int k3 = 16 * (wave & 1) + l16 + 1;
+                const int k3 = 16 * (two ? (wave & 1) : 0) + l16 + 1;
 #pragma unroll
                 for (int jj = 0; jj < 8; ++jj) {
                     const int m3 = 16 * ch + 4 * lq + (jj >> 1);
@@ -250,8 +256,8 @@ __device__ __forceinline__ void triple16_body(char *smem, const double2 *__restr
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int b = wave + 8 * j;
-                if (b < 2 * nrb) {
-                    const int rb = b >> 1;
+                if (b < nob) {
+                    const int rb = two ? b >> 1 : b;
                     const float4 *tp = reinterpret_cast<const float4 *>(Tf + (16 * rb + l16) * kT16Row + 8 * lq);
                     const float4 t0 = tp[0], t1 = tp[1];
                     const float tv[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
@@ -270,8 +276,8 @@ __device__ __forceinline__ void triple16_body(char *smem, const double2 *__restr
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int b = wave + 8 * j;
-        if (b < 2 * nrb) {
-            const int rb = b >> 1, cb = b & 1;
+        if (b < nob) {
+            const int rb = two ? b >> 1 : b, cb = two ? b & 1 : 0;
             const int k3 = 16 * cb + l16 + 1;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
