@@ -160,7 +160,7 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
     ms_sweep = 1e3 * el / nsweeps
     kl_total = cols
     flops = 2.0 * basis.R * basis.n_lt * kl_total
-    tile_ms = tune["tile_ms"]["%dx%d" % tune["tile"]]
+    tile_ms = tune["tile_ms"]["%dx%d/%s" % (tune["tile"][0], tune["tile"][1], tune["schedule"])]
     log(f"batch: {len(Ks)} runs K={Ks}, {ne} evals ({cols} k-triple columns) per sweep: {ms_sweep:.2f} ms/sweep, "
         f"{ne / (ms_sweep * 1e-3):.0f} evals/s; tune {tune}")
     return {

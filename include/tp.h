@@ -182,10 +182,12 @@ int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, cons
                     const double *const *coeff, const int64_t *ld, const uint64_t *seeds, int hist_cap,
                     tp_batch **out);
 void tp_batch_destroy(tp_batch *B);
-/* Time the candidate triple-product work orders and contraction tiles on this
- * batch (reps launches each) and keep the fastest.  ms_out[6] (may be NULL):
- * work orders 0,1 then tiles 128x64, 64x64, 128x128, 64x128; config_out[3]
- * (may be NULL): chosen tile rows, tile columns, work order. */
+/* Time the candidate triple-product work orders and contraction kernels on
+ * this batch (reps launches each) and keep the fastest.  ms_out[10] (may be
+ * NULL): work orders 0 (by cost), 1 (evaluation-major), then tiles 128x64,
+ * 64x64, 128x128, 64x128 with parallel row tiles, then the same four with
+ * row-walking workgroups; config_out[4] (may be NULL): chosen tile rows, tile
+ * columns, work order, schedule (1 parallel, 2 row-walking). */
 int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out);
 /* Initial energies and temperatures: slot 0 of every run is the identity, T0 =
  * temperature_factor * std(energies of the other slots) (calculate_temperature,

@@ -136,6 +136,7 @@ def test_batch_sweeps(tp, family):
     B.set_family(family)
     cfg = B.autotune(reps=2)
     assert cfg["tile"] in [(128, 64), (64, 64), (128, 128), (64, 128)] and cfg["order"] in (0, 1)
+    assert cfg["schedule"] in ("rows", "walk")
     B.probe(0.1)
     g0 = [B.group(g) for g in range(len(Ks))]
     for b in bases:

@@ -197,7 +197,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
     if (b->use_mfma && b->prec != TP_PREC_FP64) {
         // mixed precision: compensated FP16 triple products
-        hipLaunchKernelGGL(k_shell_scales, dim3(1), dim3(256), 0, st, b0, b1, sel, flip, b->K, b->n_c, b->L,
+        hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)b->K), dim3(256), 0, st, b0, b1, sel, flip, b->n_c, b->L,
                            b->d_shsc);
         hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
                            b->d_shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
@@ -231,7 +231,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
         const uint16_t *Fh = b->prec == TP_PREC_FP16X3 ? b->d_Fhi : b->d_Fbf;
 #define TP_G16(MODE, OUT, EN, OUTP)                                                                                \
-    hipLaunchKernelGGL((k_gemm16<H_BM, H_BN, MODE, OUT, EN, false>), grid16, dim3(256), 0, st, Fh, b->d_Flo,          \
+    hipLaunchKernelGGL((k_gemm16<H_BM, H_BN, MODE, OUT, EN, 0>), grid16, dim3(256), 0, st, Fh, b->d_Flo,          \
                        (int64_t)b->ldk, b->d_Ahi, b->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, b->d_Ainv, (int)b->R,  \
                        ncols, b->n_rt16, b->n_ct16, OUTP, b->R, qh, b->d_part, b->kl_pad, nullptr, nullptr)
         if (b->prec == TP_PREC_FP16X3) {
@@ -310,9 +310,9 @@ struct tp_batch {
     std::vector<std::vector<std::pair<double, int2>>> items;   // per evaluation: (cost, {e, lt | h << 16})
     int64_t ncb = 0, ldA = 0, cstride = 0;
     int nwork = 0;
-    int bm = H_BM, bn = H_BN, order = 0;   // contraction tile, triple-product work order (autotuned)
+    int gcand = 0, order = 0;              // contraction candidate (kTiles), triple-product work order (autotuned)
     int family = TP_MOVE_PLANE;            // move family of the random slots
-    double tune_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double tune_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     EvalDesc *d_ev = nullptr;
     GroupState *d_gs = nullptr;
     int2 *d_work = nullptr;
@@ -511,31 +511,47 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     return 0;
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int WALK>
 static void batch_gemm_launch(const tp_batch *B, hipStream_t st)
 {
     const tp_basis *b = B->b;
     const int n_rt = (int)(b->R_pad16 / BM), n_ct = (int)(B->ldA / BN);
-    const dim3 grid((unsigned)(n_rt * n_ct));
+    // WALK = 2: one workgroup per column tile walking all row tiles; 1: one per tile
+    const dim3 grid((unsigned)(WALK == 2 ? n_ct : n_rt * n_ct));
     if (b->prec == TP_PREC_FP16X3)
-        hipLaunchKernelGGL((k_gemm16<BM, BN, 1, 0, true, true>), grid, dim3(256), 0, st, b->d_Fhi, b->d_Flo,
+        hipLaunchKernelGGL((k_gemm16<BM, BN, 1, 0, true, WALK>), grid, dim3(256), 0, st, b->d_Fhi, b->d_Flo,
                            (int64_t)b->ldk, B->d_Ahi, B->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, B->d_isc, (int)b->R,
                            0, n_rt, n_ct, nullptr, (int64_t)0, nullptr, B->d_part, B->ldA, B->d_qtab, B->d_nvalid);
     else
-        hipLaunchKernelGGL((k_gemm16<BM, BN, 2, 0, true, true>), grid, dim3(256), 0, st, b->d_Fbf, nullptr,
+        hipLaunchKernelGGL((k_gemm16<BM, BN, 2, 0, true, WALK>), grid, dim3(256), 0, st, b->d_Fbf, nullptr,
                            (int64_t)b->ldk, B->d_Ahi, nullptr, (int64_t)b->ldk, b->ldk, b->F_inv, B->d_isc, (int)b->R,
                            0, n_rt, n_ct, nullptr, (int64_t)0, nullptr, B->d_part, B->ldA, B->d_qtab, B->d_nvalid);
 }
 
-static void batch_gemm(const tp_batch *B, hipStream_t st, int bm, int bn)
+// Contraction candidates: tile (rows x columns) and schedule (1: parallel row
+// tiles, 2: row-walking workgroups).
+static const int kTiles[8][3] = {{128, 64, 1}, {64, 64, 1}, {128, 128, 1}, {64, 128, 1},
+                                 {128, 64, 2}, {64, 64, 2}, {128, 128, 2}, {64, 128, 2}};
+
+static void batch_gemm(const tp_batch *B, hipStream_t st, int cand)
 {
-    if (bm == 128 && bn == 64) batch_gemm_launch<128, 64>(B, st);
-    else if (bm == 64 && bn == 64) batch_gemm_launch<64, 64>(B, st);
-    else if (bm == 128 && bn == 128) batch_gemm_launch<128, 128>(B, st);
-    else batch_gemm_launch<64, 128>(B, st);
+    switch (cand) {
+    case 0: batch_gemm_launch<128, 64, 1>(B, st); break;
+    case 1: batch_gemm_launch<64, 64, 1>(B, st); break;
+    case 2: batch_gemm_launch<128, 128, 1>(B, st); break;
+    case 3: batch_gemm_launch<64, 128, 1>(B, st); break;
+    case 4: batch_gemm_launch<128, 64, 2>(B, st); break;
+    case 5: batch_gemm_launch<64, 64, 2>(B, st); break;
+    case 6: batch_gemm_launch<128, 128, 2>(B, st); break;
+    default: batch_gemm_launch<64, 128, 2>(B, st); break;
+    }
 }
 
-static const int kTiles[4][2] = {{128, 64}, {64, 64}, {128, 128}, {64, 128}};
+// energy-sum rows the chosen contraction leaves in part
+static int batch_part_rows(const tp_batch *B)
+{
+    return kTiles[B->gcand][2] == 2 ? 1 : (int)(B->b->R_pad16 / kTiles[B->gcand][0]);
+}
 
 // stages of one batched evaluation (coefficients already in place)
 static void batch_triple(const tp_batch *B, hipStream_t st)
@@ -563,14 +579,14 @@ static void batch_split(const tp_batch *B, hipStream_t st)
 static void batch_final(const tp_batch *B, hipStream_t st)
 {
     hipLaunchKernelGGL(k_final_batch, dim3((unsigned)B->ne), dim3(256), 0, st, B->d_ev, B->d_part,
-                       (int)(B->b->R_pad16 / B->bm), B->ldA, B->d_E);
+                       batch_part_rows(B), B->ldA, B->d_E);
 }
 
 static int batch_enqueue_eval(const tp_batch *B, hipStream_t st)
 {
     batch_triple(B, st);
     batch_split(B, st);
-    batch_gemm(B, st, B->bm, B->bn);
+    batch_gemm(B, st, B->gcand);
     batch_final(B, st);
     TP_HIP(hipGetLastError());
     return 0;
@@ -664,15 +680,13 @@ static int batch_autotune(tp_batch *B, int reps)
     batch_split(B, st);
     best = 1e30;
     int best_t = 0;
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < 8; ++t) {
         double ms = 0;
-        if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_gemm(B, s, kTiles[t][0], kTiles[t][1]); }, &ms))
-            return rc;
+        if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_gemm(B, s, t); }, &ms)) return rc;
         B->tune_ms[2 + t] = ms;
         if (ms < best) { best = ms; best_t = t; }
     }
-    B->bm = kTiles[best_t][0];
-    B->bn = kTiles[best_t][1];
+    B->gcand = best_t;
     TP_HIP(hipStreamSynchronize(st));
     if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
     return 0;
@@ -1327,8 +1341,8 @@ int tp_basis_set_precision(tp_basis *b, int mode)
     if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3 && mode != TP_PREC_BF16)
         return tp_fail(TP_E_ARG, "unknown precision mode");
     if (int rc = check_ctx(b->ctx)) return rc;
-    if (mode != TP_PREC_FP64 && b->K > 64)
-        return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64");
+    if (mode != TP_PREC_FP64 && (b->K > 64 || b->n_c > 1024))
+        return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64 and L <= 44");
     if (mode != TP_PREC_FP64 && !b->d_Fhi) {
         // F[r][lt] = sum_class Bm[class][r] H[lt][class] in FP64, then scaled by a
         // power of two into [2^13, 2^14) and split into FP16 hi/lo and BF16
@@ -1793,11 +1807,12 @@ int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out)
     }
     if (int rc = batch_autotune(B, reps)) return rc;
     if (ms_out)
-        for (int i = 0; i < 6; ++i) ms_out[i] = B->tune_ms[i];
+        for (int i = 0; i < 10; ++i) ms_out[i] = B->tune_ms[i];
     if (config_out) {
-        config_out[0] = B->bm;
-        config_out[1] = B->bn;
+        config_out[0] = kTiles[B->gcand][0];
+        config_out[1] = kTiles[B->gcand][1];
         config_out[2] = B->order;
+        config_out[3] = kTiles[B->gcand][2];
     }
     return 0;
 }

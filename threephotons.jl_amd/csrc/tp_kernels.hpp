@@ -645,19 +645,26 @@ __global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As,
 }
 
 // C[r][kt] = sum_k A[r][k] B[kt][k] (both operands k-contiguous, 16-bit),
-// BM x BN tile (BM, BN in {64, 128}) per workgroup of 4 waves (2 x 2), K
-// staged through LDS 32 at a time (rows padded to 40 halves: conflict-free
-// 16-byte reads), next slice prefetched into registers.  MFMA
+// BM x BN tile (BM, BN in {64, 128}) per step of a workgroup of 4 waves
+// (2 x 2), K staged through LDS 32 at a time (rows padded to 40 halves:
+// conflict-free 16-byte reads), next slice prefetched into registers.  MFMA
 // v_mfma_f32_16x16x32_{f16,bf16}: lane holds A[l%16][8(l/16)..+7],
 // B[8(l/16)..+7][l%16], accumulator rows 4(l/16)+v (measured on gfx950).
+// FP16 hi/lo: hi.hi and (hi.lo + lo.hi) in two FP32 accumulators, summed in
+// FP64 (the corrections then round at 2^-11 of the main sum's scale).
 // Epilogue as k_gemm: raw c3 (OUT = 1) and/or per-column partial energy sums.
-// BATCH: the columns are 64-column blocks of many evaluations; block cb reads
-// its histogram columns from qtab[cb] (nvalid[cb] of them are real), and the
-// tiles run column-tile-major so the row tiles of one column tile share an
-// XCD's L2.
+// BATCH > 0: the columns are 64-column blocks of many evaluations; block cb
+// reads its histogram columns from qtab[cb] (nvalid[cb] of them are real).
+//   BATCH = 1: one workgroup per (row tile, column tile), column-tile-major
+//              (the row tiles of a column tile share an XCD's L2);
+//   BATCH = 2: one workgroup per column tile walking all row tiles (its B tile
+//              read once; the energy sums it writes, part row 0, are whole
+//              columns).
+// Either way the column tiles that share histogram columns are neighbours in
+// the (XCD-grouped) tile order.
 constexpr int H_BM = 128, H_BN = 64, H_BK = 32, H_LDS_ROW = 40;
 
-template <int BM, int BN, int MODE, int OUT, bool ENERGY, bool BATCH>
+template <int BM, int BN, int MODE, int OUT, bool ENERGY, int BATCH>
 __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi, const uint16_t *__restrict__ Alo,
                                                 int64_t lda, const uint16_t *__restrict__ Bhi,
                                                 const uint16_t *__restrict__ Blo, int64_t ldb, int Kdim,
@@ -674,145 +681,161 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
     __shared__ uint16_t sB[NP][BN][H_LDS_ROW];
     __shared__ double red[2][2][BN];
 
-    const int nblk = n_rt * n_ct;
+    const int nblk = BATCH == 2 ? n_ct : n_rt * n_ct;
     int lin = blockIdx.x;
     if ((nblk & 7) == 0) lin = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
-    const int rt = BATCH ? lin % n_rt : lin / n_ct, ct = BATCH ? lin / n_rt : lin % n_ct;
-    const int r0 = rt * BM, c0 = ct * BN;
+    const int ct = BATCH == 2 ? lin : (BATCH == 1 ? lin / n_rt : lin % n_ct);
+    const int rt_first = BATCH == 2 ? 0 : (BATCH == 1 ? lin % n_rt : lin / n_ct);
+    const int rt_last = BATCH == 2 ? n_rt : rt_first + 1;
+    const int c0 = ct * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lq = lane >> 4;
-
-    f32x4 acc[RB][CB], cor[RB][CB];
-#pragma unroll
-    for (int i = 0; i < RB; ++i)
-#pragma unroll
-        for (int j = 0; j < CB; ++j) {
-            acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-            cor[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
-    u16x8 pa[NP][QA], pb[NP][QB];
     auto gl = [](const uint16_t *base, int64_t ld, int row, int k) {
         return *reinterpret_cast<const u16x8 *>(base + (int64_t)row * ld + k);
     };
+
+    // per-column data of this lane's CB columns
+    bool cok[CB];
+    const double *qcol[CB];
+    double sc[CB], sp[CB], sq[CB];
 #pragma unroll
-    for (int q = 0; q < QA; ++q) {
-        const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
-        pa[0][q] = gl(Ahi, lda, r0 + row, kc);
-        if (NP == 2) pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
+    for (int j = 0; j < CB; ++j) {
+        const int col = c0 + wn * WTN + j * 16 + l16;
+        qcol[j] = nullptr;
+        if (BATCH) {
+            cok[j] = (col & 63) < nvalid[col >> 6];
+            if (ENERGY && cok[j]) qcol[j] = qtab[col >> 6] + (int64_t)(col & 63) * M;
+        } else {
+            cok[j] = col < ncols;
+            if (ENERGY) qcol[j] = qhat + (int64_t)col * M;
+        }
+        sc[j] = cok[j] ? a_inv * b_inv[col] : 0.0;
+        sp[j] = 0.0;
+        sq[j] = 0.0;
     }
-#pragma unroll
-    for (int q = 0; q < QB; ++q) {
-        const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
-        pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
-        if (NP == 2) pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
-    }
-    for (int k0 = 0; k0 < Kdim; k0 += H_BK) {
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < QA; ++q) {
-            const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
-#pragma unroll
-            for (int p = 0; p < NP; ++p) *reinterpret_cast<u16x8 *>(&sA[p][row][kc]) = pa[p][q];
-        }
-#pragma unroll
-        for (int q = 0; q < QB; ++q) {
-            const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
-#pragma unroll
-            for (int p = 0; p < NP; ++p) *reinterpret_cast<u16x8 *>(&sB[p][row][kc]) = pb[p][q];
-        }
-        __syncthreads();
-        if (k0 + H_BK < Kdim) {
-#pragma unroll
-            for (int q = 0; q < QA; ++q) {
-                const int i = tid + 256 * q, row = i >> 2, kc = k0 + H_BK + (i & 3) * 8;
-                pa[0][q] = gl(Ahi, lda, r0 + row, kc);
-                if (NP == 2) pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
-            }
-#pragma unroll
-            for (int q = 0; q < QB; ++q) {
-                const int i = tid + 256 * q, row = i >> 2, kc = k0 + H_BK + (i & 3) * 8;
-                pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
-                if (NP == 2) pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
-            }
-        }
-        u16x8 fa[NP][RB], fb[NP][CB];
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-#pragma unroll
-            for (int i = 0; i < RB; ++i)
-                fa[p][i] = *reinterpret_cast<const u16x8 *>(&sA[p][wm * WTM + i * 16 + l16][8 * lq]);
-#pragma unroll
-            for (int j = 0; j < CB; ++j)
-                fb[p][j] = *reinterpret_cast<const u16x8 *>(&sB[p][wn * WTN + j * 16 + l16][8 * lq]);
-        }
+
+    for (int rt = rt_first; rt < rt_last; ++rt) {
+        const int r0 = rt * BM;
+        f32x4 acc[RB][CB], cor[RB][CB];
 #pragma unroll
         for (int i = 0; i < RB; ++i)
 #pragma unroll
             for (int j = 0; j < CB; ++j) {
-                if (MODE == 1) {
-                    const h16x8 ah = __builtin_bit_cast(h16x8, fa[0][i]), bh = __builtin_bit_cast(h16x8, fb[0][j]);
-                    const h16x8 al = __builtin_bit_cast(h16x8, fa[NP - 1][i]);
-                    const h16x8 bl = __builtin_bit_cast(h16x8, fb[NP - 1][j]);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[i][j], 0, 0, 0);
-                    cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, cor[i][j], 0, 0, 0);
-                    cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, cor[i][j], 0, 0, 0);
-                } else {
-                    const b16x8 ah = __builtin_bit_cast(b16x8, fa[0][i]), bh = __builtin_bit_cast(b16x8, fb[0][j]);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[i][j], 0, 0, 0);
+                acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+                cor[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
+        u16x8 pa[NP][QA], pb[NP][QB];
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
+            pa[0][q] = gl(Ahi, lda, r0 + row, kc);
+            if (NP == 2) pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
+            pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
+            if (NP == 2) pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
+        }
+        for (int k0 = 0; k0 < Kdim; k0 += H_BK) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) *reinterpret_cast<u16x8 *>(&sA[p][row][kc]) = pa[p][q];
+            }
+#pragma unroll
+            for (int q = 0; q < QB; ++q) {
+                const int i = tid + 256 * q, row = i >> 2, kc = (i & 3) * 8;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) *reinterpret_cast<u16x8 *>(&sB[p][row][kc]) = pb[p][q];
+            }
+            __syncthreads();
+            if (k0 + H_BK < Kdim) {
+#pragma unroll
+                for (int q = 0; q < QA; ++q) {
+                    const int i = tid + 256 * q, row = i >> 2, kc = k0 + H_BK + (i & 3) * 8;
+                    pa[0][q] = gl(Ahi, lda, r0 + row, kc);
+                    if (NP == 2) pa[NP - 1][q] = gl(Alo, lda, r0 + row, kc);
+                }
+#pragma unroll
+                for (int q = 0; q < QB; ++q) {
+                    const int i = tid + 256 * q, row = i >> 2, kc = k0 + H_BK + (i & 3) * 8;
+                    pb[0][q] = gl(Bhi, ldb, c0 + row, kc);
+                    if (NP == 2) pb[NP - 1][q] = gl(Blo, ldb, c0 + row, kc);
                 }
             }
-    }
-
+            u16x8 fa[NP][RB], fb[NP][CB];
 #pragma unroll
-    for (int j = 0; j < CB; ++j) {
-        const int colt = wn * WTN + j * 16 + l16;   // column within the tile
-        const int col = c0 + colt;
-        bool cok;
-        const double *qcol = nullptr;
-        if (BATCH) {
-            cok = (col & 63) < nvalid[col >> 6];
-            if (ENERGY && cok) qcol = qtab[col >> 6] + (int64_t)(col & 63) * M;
-        } else {
-            cok = col < ncols;
-            if (ENERGY) qcol = qhat + (int64_t)col * M;
+            for (int p = 0; p < NP; ++p) {
+#pragma unroll
+                for (int i = 0; i < RB; ++i)
+                    fa[p][i] = *reinterpret_cast<const u16x8 *>(&sA[p][wm * WTM + i * 16 + l16][8 * lq]);
+#pragma unroll
+                for (int j = 0; j < CB; ++j)
+                    fb[p][j] = *reinterpret_cast<const u16x8 *>(&sB[p][wn * WTN + j * 16 + l16][8 * lq]);
+            }
+#pragma unroll
+            for (int i = 0; i < RB; ++i)
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    if (MODE == 1) {
+                        const h16x8 ah = __builtin_bit_cast(h16x8, fa[0][i]), bh = __builtin_bit_cast(h16x8, fb[0][j]);
+                        const h16x8 al = __builtin_bit_cast(h16x8, fa[NP - 1][i]);
+                        const h16x8 bl = __builtin_bit_cast(h16x8, fb[NP - 1][j]);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[i][j], 0, 0, 0);
+                        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, cor[i][j], 0, 0, 0);
+                        cor[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, cor[i][j], 0, 0, 0);
+                    } else {
+                        const b16x8 ah = __builtin_bit_cast(b16x8, fa[0][i]), bh = __builtin_bit_cast(b16x8, fb[0][j]);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[i][j], 0, 0, 0);
+                    }
+                }
         }
-        const double sc = cok ? a_inv * b_inv[col] : 0.0;
-        double sp = 0.0, sq = 0.0;
+        // epilogue of this row tile
 #pragma unroll
-        for (int i = 0; i < RB; ++i) {
-            const int rbase = r0 + wm * WTM + i * 16 + 4 * lq;
+        for (int j = 0; j < CB; ++j) {
+            const int col = c0 + wn * WTN + j * 16 + l16;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = rbase + v;
-                if (cok && r < M) {
-                    const double c = ((double)acc[i][j][v] + (double)cor[i][j][v]) * sc;
-                    if (OUT == 1) out[(int64_t)col * ldo + r] = c;
-                    if (ENERGY) {
-                        const double pv = c > 1e-30 ? c : 1e-30;
-                        const double qv = qcol[r];
-                        sp += pv;
-                        sq += qv * log(pv);
+            for (int i = 0; i < RB; ++i) {
+                const int rbase = r0 + wm * WTM + i * 16 + 4 * lq;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = rbase + v;
+                    if (cok[j] && r < M) {
+                        const double c = ((double)acc[i][j][v] + (MODE == 1 ? (double)cor[i][j][v] : 0.0)) * sc[j];
+                        if (OUT == 1) out[(int64_t)col * ldo + r] = c;
+                        if (ENERGY) {
+                            const double pv = c > 1e-30 ? c : 1e-30;
+                            sp[j] += pv;
+                            sq[j] += qcol[j][r] * log(pv);
+                        }
                     }
                 }
             }
         }
-        if (ENERGY) {
-            sp += __shfl_xor(sp, 16);
-            sq += __shfl_xor(sq, 16);
-            sp += __shfl_xor(sp, 32);
-            sq += __shfl_xor(sq, 32);
-            if (lq == 0) {
-                red[wm][0][colt] = sp;
-                red[wm][1][colt] = sq;
-            }
-        }
     }
     if (ENERGY) {
+        const int rt_out = BATCH == 2 ? 0 : rt_first;
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+            const int colt = wn * WTN + j * 16 + l16;
+            double a = sp[j], b = sq[j];
+            a += __shfl_xor(a, 16);
+            b += __shfl_xor(b, 16);
+            a += __shfl_xor(a, 32);
+            b += __shfl_xor(b, 32);
+            if (lq == 0) {
+                red[wm][0][colt] = a;
+                red[wm][1][colt] = b;
+            }
+        }
         __syncthreads();
         if (tid < BN) {
-            part[((int64_t)rt * 2 + 0) * ldp + c0 + tid] = red[0][0][tid] + red[1][0][tid];
-            part[((int64_t)rt * 2 + 1) * ldp + c0 + tid] = red[0][1][tid] + red[1][1][tid];
+            part[((int64_t)rt_out * 2 + 0) * ldp + c0 + tid] = red[0][0][tid] + red[1][0][tid];
+            part[((int64_t)rt_out * 2 + 1) * ldp + c0 + tid] = red[0][1][tid] + red[1][1][tid];
         }
     }
 }

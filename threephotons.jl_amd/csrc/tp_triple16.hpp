@@ -27,33 +27,31 @@ __device__ __forceinline__ void split_h(float x, int j, h16x8 &hi, h16x8 &lo)
     lo[j] = (_Float16)(x - (float)h);
 }
 
-// Shell scales s_k (powers of two) of an evaluation's coefficients: one block.
-__device__ inline void shell_scales_block(const double2 *__restrict__ coef, int K, int n_c, int L,
-                                          double *__restrict__ out, double *s_n2 /* LDS [K * (L/2+1)] */)
-{
-    const int tid = threadIdx.x, nl = L / 2 + 1;
-    for (int i = tid; i < K * nl; i += blockDim.x) {
-        const int k = i / nl, l = 2 * (i % nl);
-        const double2 *v = coef + (size_t)k * n_c + l * (l - 1) / 2;
-        double s = 0.0;
-        for (int m = 0; m <= 2 * l; ++m) s += v[m].x * v[m].x + v[m].y * v[m].y;
-        s_n2[i] = s;
-    }
-    __syncthreads();
-    for (int k = tid; k < K; k += blockDim.x) {
-        double m = 0.0;
-        for (int l = 0; l < nl; ++l) m = fmax(m, s_n2[k * nl + l]);
-        const double nk = sqrt(m);
-        out[k] = nk > 0.0 ? ldexp(1.0, 12 - ilogb(nk)) : 1.0;
-    }
-}
-
+// One workgroup per shell k: |c_k,l|^2 per block l in parallel, then the
+// power of two s_k.
 __global__ __launch_bounds__(256) void k_shell_scales(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
-                                                      const int *__restrict__ sel, int flip, int K, int n_c, int L,
+                                                      const int *__restrict__ sel, int flip, int n_c, int L,
                                                       double *__restrict__ out)
 {
-    __shared__ double s_n2[64 * 12];
-    shell_scales_block(pick(cb0, cb1, sel, flip), K, n_c, L, out, s_n2);
+    __shared__ double s_e[1024];
+    __shared__ double s_l[32];
+    const double2 *c = pick(cb0, cb1, sel, flip) + (size_t)blockIdx.x * n_c;
+    const int tid = threadIdx.x, nl = L / 2 + 1;
+    for (int i = tid; i < n_c; i += 256) s_e[i] = c[i].x * c[i].x + c[i].y * c[i].y;
+    __syncthreads();
+    if (tid < nl) {
+        const int l = 2 * tid;
+        double s = 0.0;
+        for (int m = 0; m <= 2 * l; ++m) s += s_e[l * (l - 1) / 2 + m];
+        s_l[tid] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double m = 0.0;
+        for (int l = 0; l < nl; ++l) m = fmax(m, s_l[l]);
+        const double nk = sqrt(m);
+        out[blockIdx.x] = nk > 0.0 ? ldexp(1.0, 12 - ilogb(nk)) : 1.0;
+    }
 }
 
 // Bytes of dynamic LDS of triple16_body.

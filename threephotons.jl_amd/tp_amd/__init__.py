@@ -340,12 +340,13 @@ class AnnealBatch:
             pass
 
     def autotune(self, reps=3):
-        ms = np.zeros(6)
-        cfg = np.zeros(3, np.int32)
+        ms = np.zeros(10)
+        cfg = np.zeros(4, np.int32)
         check(lib().tp_batch_autotune(self.h, reps, ms.ctypes.data, cfg.ctypes.data), "tp_batch_autotune")
-        return dict(order_ms=ms[:2].tolist(), tile_ms=dict(zip(["128x64", "64x64", "128x128", "64x128"],
-                                                                ms[2:].tolist())),
-                    tile=(int(cfg[0]), int(cfg[1])), order=int(cfg[2]))
+        names = [f"{t}/{s}" for s in ("rows", "walk") for t in ("128x64", "64x64", "128x128", "64x128")]
+        return dict(order_ms=ms[:2].tolist(), tile_ms=dict(zip(names, ms[2:].tolist())),
+                    tile=(int(cfg[0]), int(cfg[1])), order=int(cfg[2]),
+                    schedule="walk" if cfg[3] == 2 else "rows")
 
     def probe(self, temperature_factor=0.1):
         check(lib().tp_batch_probe(self.h, float(temperature_factor)), "tp_batch_probe")
