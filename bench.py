@@ -46,6 +46,25 @@ def folded_classes(L):
     return n
 
 
+def lt_n3_sum(L):
+    """Sum over l-triples (even l, triangle rule) of 2 l3 + 1."""
+    s = 0
+    for l1 in range(0, L + 1, 2):
+        for l2 in range(0, L + 1, 2):
+            for l3 in range(abs(l1 - l2), min(l1 + l2, L) + 1, 2):
+                s += 2 * l3 + 1
+    return s
+
+
+# HBM bytes per launch of the plain path's dominant kernel (k_triple_mfma) at
+# the full-size workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
+# dispatch), scaled by what the same counters read for a 256 MiB device copy
+# (FETCH_SIZE 0.5x, WRITE_SIZE 1.0x of the bytes moved); the summary with the
+# commands is profiles/hbm_counters_full_L18_K26_N32.md.
+TRIPLE_FETCH_KIB, TRIPLE_WRITE_KIB = 2500.0, 12924.8
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -197,7 +216,6 @@ def learned_leg(cfg, tp, workload, truth, basis, c3ref, start, plain, data_sweep
     out = {"nprop": nprop, "nlearn": nlearn}
     # (1) data
     B = tp.AnnealBatch(basis, refs, starts, seeds, nprop=nprop, hist_cap=data_sweeps + 8)
-    B.set_family("quat")   # random SO(3) moves: their accepted quaternions are the training labels
     cap = 1_200_000
     B.log(cap)
     B.probe(0.1)
@@ -317,7 +335,10 @@ def main():
     n_m = folded_classes(cfg["L"])
     flops_g1 = 2.0 * n_m * basis.n_lt * basis.klength         # G = H . As
     flops_g2 = 2.0 * basis.R * n_m * basis.klength            # c3 = Bm . G
-    achieved = flops_g2 / (kern_ms[2] * 1e-3) / 1e12
+    K = cfg["K"]
+    # coupling-form triple products: complex MACs (8 flop) of stage 1 and stage 2
+    flops_tri = 8.0 * (K * (K + 1) // 2 * basis.basislen + basis.klength * lt_n3_sum(cfg["L"]))
+    achieved = flops_tri / (kern_ms[0] * 1e-3) / 1e12
     log(f"evals/s {evals_per_s:.1f}  stage ms: triple {kern_ms[0]:.4f} G {kern_ms[1]:.4f} c3+energy "
         f"{kern_ms[2]:.4f} final {kern_ms[3]:.4f}  E={E:.10g}")
 
@@ -394,9 +415,12 @@ def main():
             "gemm16_tflops_algorithmic": 2.0 * basis.R * basis.n_lt * basis.klength / (kern16[2] * 1e-3) / 1e12,
             "energy_abs_err_vs_fp64": abs(E16 - E), "mc_steps_per_s": (args.anneal_steps + 51) / t_anneal16,
             "mc_E_final": res16["E_final"]},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm c3 = Bm.G (+ fused energy)", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "flops_per_launch": flops_g2,
+        "roofline": {"bound": "mfma", "kernel": "k_triple_mfma (coupling-form triple products, FP64 MFMA)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": TRIPLE_FETCH_KIB * 1024 * FETCH_SCALE + TRIPLE_WRITE_KIB * 1024 * WRITE_SCALE,
+                     "flops_per_launch": flops_tri, "launch_ms": kern_ms[0],
+                     "gemm_c3_energy_tflops": flops_g2 / (kern_ms[2] * 1e-3) / 1e12,
                      "gemm_G_tflops": flops_g1 / (kern_ms[1] * 1e-3) / 1e12},
     }
     if batch is not None:

@@ -65,6 +65,15 @@ void tp_finalize(tp_ctx *ctx);
  * (lrange = 0:2:L).  lambda != 0 returns TP_E_UNSUPPORTED in this version. */
 int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq,
                     int for_intensity, tp_basis **out);
+/* The same from a basis the caller already holds: wignerlist[basislen] and
+ * indices (9 x basislen, column j = [sean(l1,m1), sean(l2,m2), sean(l3,-m3),
+ * l1, m1, l2, m2, l3, -m3]) of the reference's BasisType (correlations.jl:35-63,
+ * 87-153); the caller's weights are used as given.  pacombos (9 x ncombos) may
+ * be NULL; when given its length must be a multiple of klength.  Replaces
+ * CUDA_store_basis (cuda.jl:45-55).  Even l (for_intensity) only. */
+int tp_basis_from_host(tp_ctx *ctx, const double *wignerlist, const int64_t *indices, int64_t basislen,
+                       const int64_t *pacombos, int64_t ncombos, int N, int L, int LMAX, int K, double lambda,
+                       double dq, tp_basis **out);
 void tp_basis_destroy(tp_basis *b);
 
 /* basislen, number of l-triples, klength = K(K+1)(K+2)/6, R = 2N^2. */
@@ -250,18 +259,20 @@ int tp_mlp_train(tp_mlp *m, const float *X, const float *T, int n, int steps, in
 int tp_mlp_train_log(tp_mlp *m, tp_batch *B, int steps, int batch, float lr, uint64_t seed, float *loss_trace);
 
 /* Keep a move log of up to `capacity` samples: after every sweep, each
- * quaternion proposal (learned slots; random slots of TP_MOVE_QUAT) that
- * passes the Metropolis test on its own against its run's current energy
+ * proposal that passes the Metropolis test on its own against its run's current energy
  * adds one sample per shell of the run (that shell's features in the state the
- * move starts from, and the proposal's quaternions with w >= 0). */
+ * move starts from, and the proposal's quaternions with w >= 0; a plane move
+ * is recorded as (cos |b|/2, sin |b|/2, 0, 0) of its angle b per l). */
 int tp_batch_log(tp_batch *B, int capacity);
 int tp_batch_log_count(tp_batch *B, int32_t *n);
 int tp_batch_log_read(tp_batch *B, int first, int n, float *X, float *T);
 /* The last nlearn (< nprop) slots of every run take the network's quaternions
  * (input: shell j*K_g/nlearn of the run's state and its T), each composed with
  * a random rotation of angle <= noise * amax * pi/4; the other slots stay
- * random.  Acceptance is unchanged (exact energies).  nlearn = 0 switches the
- * network off. */
+ * random.  With TP_MOVE_QUAT the quaternion is the SO(3) rotation of block l;
+ * with TP_MOVE_PLANE its angle is the angle of a plane rotation of block l in
+ * a Haar-random plane (the reference's move with a learned step).  Acceptance
+ * is unchanged (exact energies).  nlearn = 0 switches the network off. */
 int tp_batch_set_proposer(tp_batch *B, tp_mlp *m, int nlearn, double noise);
 
 /* ---- synthetic input ------------------------------------------------------ */

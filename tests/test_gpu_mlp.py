@@ -99,3 +99,69 @@ def test_move_log_and_learned_slots(tp):
         a, b2 = c2_invariants(starts[g], Ks[g], L, LMAX), c2_invariants(c, Ks[g], L, LMAX)
         assert np.abs(a - b2).max() < 1e-11 * np.abs(a).max()
     assert B.log_count() >= n
+
+
+def test_plane_family_log_labels(tp):
+    """With the reference's plane moves, the log records each l-block's angle b
+    as (cos |b|/2, sin |b|/2, 0, 0) with |b| <= pi/4, and learned slots (whose
+    quaternion sets the angle) keep every invariant of the sweep."""
+    from test_gpu_batch import _anneal_setup
+    Ks, nprop = [6, 9], 8
+    L, LMAX = 8, 25
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, hist_cap=30)
+    B.log(50000)
+    B.probe(0.5)
+    B.run(10, decay=0.999, factor=1.05)
+    n = B.log_count()
+    assert n > 0
+    _, T = B.log_read(0, n)
+    q = T.reshape(-1, 9, 4)[:, :L // 2]
+    assert np.all(q[:, :, 2:] == 0)
+    assert np.allclose(np.linalg.norm(q, axis=2), 1.0, atol=1e-6)
+    assert np.all(q[:, :, 0] >= np.cos(np.pi / 8) - 1e-7)
+    m = tp.MlpProposer(seed=4)
+    m.train_log(B, steps=30, batch=128, lr=1e-3, seed=5)
+    B.set_proposer(m, 4, noise=0.25)
+    B.run(10, decay=0.999, factor=1.05)
+    for b in bases:
+        b.set_precision("fp16x3")
+    for g in range(len(Ks)):
+        st = B.group(g)
+        c = B.coeff(g)
+        Es = tp.energy(c, bases[g], refs[g])
+        assert abs(st["E"] - Es) <= 1e-10 * abs(Es)
+        a, b2 = c2_invariants(starts[g], Ks[g], L, LMAX), c2_invariants(c, Ks[g], L, LMAX)
+        assert np.abs(a - b2).max() < 1e-11 * np.abs(a).max()
+
+
+def test_learned_vs_random_gain(tp):
+    """What the trained network gains over the method without it, measured on
+    the same runs from the same start: mean energy change over the same number
+    of sweeps with 3/4 of the slots learned (plane moves whose angles the
+    network sets) against random slots only.  Both runs must stay exact (the
+    acceptance uses the exact energy either way); the gain is reported."""
+    from test_gpu_batch import _anneal_setup
+    Ks, nprop, nsw = [6, 9, 8], 16, 20
+    # training data: moves of the random method
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, hist_cap=60)
+    B.log(200000)
+    B.probe(0.5)
+    B.run(40, decay=0.999, factor=1.05)
+    m = tp.MlpProposer(seed=11)
+    tr = m.train_log(B, steps=300, batch=256, lr=1e-3, seed=12)
+    assert tr[-20:].mean() < tr[:20].mean()
+    dE = {}
+    for name, nl in (("random", 0), ("learned", 12)):
+        _, _, _, _, Bc = _anneal_setup(tp, Ks, nprop, hist_cap=nsw + 2)
+        Bc.probe(0.5)
+        if nl:
+            Bc.set_proposer(m, nl, noise=0.25)
+        Bc.run(nsw, decay=0.999, factor=1.05)
+        h = Bc.history()
+        E0 = np.array([Bc.group(g)["E"] for g in range(len(Ks))])
+        dE[name] = float(np.mean(h[nsw - 1] - h[0]))
+        assert np.all(np.isfinite(h[:nsw]))
+        assert np.allclose(h[nsw - 1], E0)
+    gain = dE["learned"] / dE["random"] if dE["random"] != 0 else float("nan")
+    print(f"mean dE over {nsw} sweeps: random {dE['random']:.4f}, learned {dE['learned']:.4f}, ratio {gain:.3f}")
+    assert dE["random"] <= 0 and dE["learned"] <= 0   # best-of-N annealing goes downhill

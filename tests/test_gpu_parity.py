@@ -187,3 +187,27 @@ def test_full_size_properties(tp, oracle):
     E = tp.energy(vol, b, ref)
     assert abs(E - Emin) / abs(Emin) < E_TOL
     say("full-size: energy ok", E)
+
+
+def test_basis_from_host_tables(tp, oracle):
+    """tp_basis_from_host (CUDA_store_basis) on the oracle's wignerlist/indices
+    gives the same c3 as tp_basis_create, and uses the weights it is given."""
+    L, LMAX, N, K = 6, 8, 6, 4
+    ob = oracle.Basis(L, LMAX, N, K)
+    w, ind, _ = ob.tables()
+    pac = ob.pacombos()
+    c = random_real_coeff(K, L, LMAX, seed=77, c00=3.0)
+    b0 = tp.HIP_calculate_basis(L, LMAX, N, K)
+    b1 = tp.HIP_store_basis(w, ind, N, L, LMAX, K, pacombos=pac)
+    assert (b1.basislen, b1.n_lt, b1.klength) == (b0.basislen, b0.n_lt, b0.klength)
+    c0 = tp.FullCorrelation_parallized(c, b0, return_raw=True)
+    c1 = tp.FullCorrelation_parallized(c, b1, return_raw=True)
+    assert rel_err(c1, c0) < 1e-14
+    w2 = w.copy()
+    w2[len(w2) // 2] *= 1.5
+    b2 = tp.HIP_store_basis(w2, ind, N, L, LMAX, K)
+    assert rel_err(tp.FullCorrelation_parallized(c, b2, return_raw=True), c0) > 1e-6
+    bad = ind.copy()
+    bad[3, 4] += 1   # m1 no longer matches its seanindex
+    with pytest.raises(tp.TPError):
+        tp.HIP_store_basis(w, bad, N, L, LMAX, K)

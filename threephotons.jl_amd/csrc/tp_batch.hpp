@@ -123,16 +123,20 @@ constexpr int kRotShellChunk = 16;
 #define TP_MOVE_QUAT 0    // SO(3) rotation per l (D^l of a quaternion)
 #define TP_MOVE_PLANE 1   // rotation in the plane of a Haar-random 2-frame of R^(2l+1)
 
-// With gs != nullptr and family == TP_MOVE_PLANE, the random slots (p <
-// nprop - nlearn; in probe mode all but slot 0) instead get the reference's
-// move (get_MC_step / random_rotation_step, determination.jl:232-241,
-// utilities.jl:180-190, written as in k_propose):
+// With gs != nullptr and family == TP_MOVE_PLANE, every slot (in probe mode
+// all but slot 0) instead gets the reference's move (get_MC_step /
+// random_rotation_step, determination.jl:232-241, utilities.jl:180-190,
+// written as in k_propose):
 //   D = I + (cos b - 1)(u u^T + v v^T) + sin b (v u^T - u v^T),
-// (u, v) Gram-Schmidt of two Gaussian vectors, b = amax (pi/4)(l-1) z clipped
-// to [-pi/4, pi/4] (the reference's step sizes (pi/4)(l-1), determination.jl:295).
+// (u, v) Gram-Schmidt of two Gaussian vectors; random slots take
+// b = amax (pi/4)(l-1) z (the reference's step sizes (pi/4)(l-1),
+// determination.jl:295), learned slots (p >= nprop - nlearn) the angle of
+// their quaternion, 2 atan2(|(x,y,z)|, |w|), with a random sign; b is clipped
+// to [-pi/4, pi/4].  The angle actually used is written back as the
+// quaternion (cos |b|/2, sin |b|/2, 0, 0), the label the move log records.
 // A Haar-random frame is Haar-random in any fixed orthonormal basis, so the
 // real basis used here serves as well as the reference's Umat basis.
-__global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict__ ev, const double *__restrict__ quat,
+__global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict__ ev, double *__restrict__ quat,
                                                       int nl, int n_c, const GroupState *__restrict__ gs, int family,
                                                       int probe)
 {
@@ -144,11 +148,12 @@ __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict
     const int li = blockIdx.x, e = blockIdx.y, l = 2 * (li + 1);
     const int tid = threadIdx.x;
     const EvalDesc d = ev[e];
-    bool plane = false;
+    bool plane = false, learned = false;
     GroupState g;
     if (gs && family == TP_MOVE_PLANE) {
         g = gs[d.group];
-        plane = probe ? d.p != 0 : d.p < d.nprop - g.nlearn;
+        plane = !(probe && d.p == 0);
+        learned = !probe && d.p >= d.nprop - g.nlearn;
     }
     if (plane) {
         const int n = 2 * l + 1;
@@ -166,8 +171,21 @@ __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict
             uv[1][lane] = lane < n ? v : 0.0;
             if (lane == 0) {
                 const double q4 = 0.25 * kPiD;
-                double b = g.amax * q4 * (l - 1) * dev_normal(g.seed, step, lid, (uint32_t)(2 * n), TP_TAG_MOVE);
-                s_beta = b > q4 ? q4 : (b < -q4 ? -q4 : b);
+                const double z = dev_normal(g.seed, step, lid, (uint32_t)(2 * n), TP_TAG_MOVE);
+                double b;
+                double *q = quat + ((size_t)e * nl + li) * 4;
+                if (learned) {
+                    const double vn = sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+                    b = 2.0 * atan2(vn, fabs(q[0])) * (z < 0.0 ? -1.0 : 1.0);
+                } else {
+                    b = g.amax * q4 * (l - 1) * z;
+                }
+                b = b > q4 ? q4 : (b < -q4 ? -q4 : b);
+                s_beta = b;
+                q[0] = cos(0.5 * fabs(b));
+                q[1] = sin(0.5 * fabs(b));
+                q[2] = 0.0;
+                q[3] = 0.0;
             }
         }
         __syncthreads();

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/tp.h"
@@ -608,7 +609,7 @@ static int batch_enqueue_sweep(const tp_batch *B, hipStream_t st, int probe, dou
     if (int rc = batch_enqueue_eval(B, st)) return rc;
     if (!probe && B->log_cap > 0)
         hipLaunchKernelGGL(k_log_moves, dim3(1), dim3(1024), 0, st, B->d_ev, B->ne, B->nl, B->d_gs, B->d_E, B->d_quat,
-                           b->n_c, B->d_Xlog, B->d_Tlog, B->d_logn, B->log_cap, B->family);
+                           b->n_c, B->d_Xlog, B->d_Tlog, B->d_logn, B->log_cap);
     hipLaunchKernelGGL(k_select_batch, dim3((unsigned)B->ngroups), dim3(256), 0, st, B->d_gs, B->d_ev, B->d_E,
                        B->d_cur, B->cstride, b->n_c, probe, decay, factor, tfac, B->d_Ehist, B->hist_cap,
                        B->ngroups);
@@ -1036,8 +1037,49 @@ void tp_basis_destroy(tp_basis *b)
     delete b;
 }
 
+// Weights supplied by the caller (tp_basis_from_host): (l1,l2,l3,m1,m2) -> w.
+static uint64_t wkey(int l1, int l2, int l3, int m1, int m2)
+{
+    return ((((uint64_t)l1 * 64 + l2) * 64 + l3) * 128 + (uint64_t)(m1 + 64)) * 128 + (uint64_t)(m2 + 64);
+}
+
+static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq, int for_intensity,
+                       const std::unordered_map<uint64_t, double> *hw, int64_t expect_len, tp_basis **out);
+
 int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq, int for_intensity,
                     tp_basis **out)
+{
+    return basis_build(ctx, L, LMAX, N, K, lambda, dq, for_intensity, nullptr, -1, out);
+}
+
+int tp_basis_from_host(tp_ctx *ctx, const double *wignerlist, const int64_t *indices, int64_t basislen,
+                       const int64_t *pacombos, int64_t ncombos, int N, int L, int LMAX, int K, double lambda,
+                       double dq, tp_basis **out)
+{
+    if (!out) return tp_fail(TP_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (!wignerlist || !indices || basislen < 1) return tp_fail(TP_E_ARG, "null or empty basis arrays");
+    std::unordered_map<uint64_t, double> hw;
+    hw.reserve((size_t)basislen * 2);
+    for (int64_t j = 0; j < basislen; ++j) {
+        const int64_t *t = indices + 9 * j;   // [sean1, sean2, sean3, l1, m1, l2, m2, l3, -m3]
+        const int l1 = (int)t[3], m1 = (int)t[4], l2 = (int)t[5], m2 = (int)t[6], l3 = (int)t[7], mu3 = (int)t[8];
+        if (l1 < 0 || l2 < 0 || l3 < 0 || l1 > L || l2 > L || l3 > L || std::abs(m1) > l1 || std::abs(m2) > l2 ||
+            m1 + m2 + mu3 != 0 || t[0] != tph::seanindex(l1, m1, LMAX) || t[1] != tph::seanindex(l2, m2, LMAX) ||
+            t[2] != tph::seanindex(l3, mu3, LMAX))
+            return tp_fail(TP_E_ARG, "indices column " + std::to_string(j) + " is not a basis entry");
+        hw[wkey(l1, l2, l3, m1, m2)] = wignerlist[j];
+    }
+    if (pacombos && ncombos > 0) {
+        // PAcombos columns [k1,k2,k3,kt,l1,l2,l3,jstart,mcombos] cover every (k-triple, l-triple)
+        const int64_t kl = (int64_t)K * (K + 1) * (K + 2) / 6;
+        if (ncombos % kl != 0) return tp_fail(TP_E_ARG, "PAcombos length is not a multiple of klength");
+    }
+    return basis_build(ctx, L, LMAX, N, K, lambda, dq, 1, &hw, basislen, out);
+}
+
+static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq, int for_intensity,
+                       const std::unordered_map<uint64_t, double> *hw, int64_t expect_len, tp_basis **out)
 {
     if (!out) return tp_fail(TP_E_ARG, "null output pointer");
     *out = nullptr;
@@ -1107,8 +1149,15 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
                     for (int m2 = -l2; m2 <= l2; ++m2) {
                         const int mu3 = -(m1 + m2);
                         if (mu3 < -l3 || mu3 > l3) continue;
-                        const double w = tph::wigner3j(l1, m1, l2, m2, l3, mu3);
-                        if (!(std::fabs(w) > thr)) continue;
+                        double w;
+                        if (hw) {   // the caller's weights, on the reference's own enumeration
+                            const auto it = hw->find(wkey(l1, l2, l3, m1, m2));
+                            if (it == hw->end()) continue;
+                            w = it->second;
+                        } else {
+                            w = tph::wigner3j(l1, m1, l2, m2, l3, mu3);
+                            if (!(std::fabs(w) > thr)) continue;
+                        }
                         groups[mu3 + l3].push_back({w, b->coff[l1] + m1 + l1, b->coff[l2] + m2 + l2});
                         // P at theta = pi/2 vanishes for odd m (Y(l,m,pi/2) = 0 when l+m is odd)
                         if (!(m1 & 1) && !(m2 & 1))
@@ -1131,6 +1180,10 @@ int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, d
             }
     gstart.push_back((int)ew.size());
     const int nlt = (int)lt_entries.size();
+    if (hw && basislen != expect_len) {
+        delete b;
+        return tp_fail(TP_E_ARG, "host basis has entries outside the enumeration of (L, LMAX)");
+    }
     // the same groups padded to multiples of 8 entries (zero weight, the
     // group's first index) for the FP16 triple products
     std::vector<float> ew8;

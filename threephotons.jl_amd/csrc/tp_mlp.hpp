@@ -276,11 +276,12 @@ __global__ __launch_bounds__(256) void k_mlp_to_quat(const EvalDesc *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Move log: after a sweep's evaluations, quaternion proposals that pass the
-// Metropolis test on their own against the run's current energy (dE < 0, or u < exp(-dE/T)
+// Move log: after a sweep's evaluations, proposals that pass the Metropolis
+// test on their own against the run's current energy (dE < 0, or u < exp(-dE/T)
 // with u from the run's stream, tag LOG) are logged, one sample per shell of
 // the run: features of that shell of the state the move starts from, and the
-// proposal's quaternions (w >= 0 form).  One workgroup walks all proposals in
+// proposal's quaternions (w >= 0 form; for plane moves the angle quaternions
+// that k_rotate_batch writes back).  One workgroup walks all proposals in
 // order, so the log is deterministic.  Runs before k_select_batch.
 // ---------------------------------------------------------------------------
 #define TP_TAG_LOG 0x4C4F4721u
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(1024) void k_log_moves(const EvalDesc *__restrict__
                                                     const GroupState *__restrict__ gs, const double *__restrict__ Eprop,
                                                     const double *__restrict__ quat, int n_c,
                                                     uint16_t *__restrict__ Xlog, float *__restrict__ Tlog,
-                                                    int *__restrict__ count, int cap, int family)
+                                                    int *__restrict__ count, int cap)
 {
     __shared__ int s_off[1024];
     __shared__ int s_base;
@@ -306,9 +307,7 @@ __global__ __launch_bounds__(1024) void k_log_moves(const EvalDesc *__restrict__
             double u0, u1;
             philox_u01x2(g.seed, (uint32_t)d.p, 0u, (uint32_t)g.nsweep, TP_TAG_LOG, u0, u1);
             const bool ok = dE < 0.0 || (dE > 0.0 && g.T > 0.0 && u0 < exp(-dE / g.T));
-            // plane moves (random slots of the plane family) have no quaternion
-            const bool quat_move = !(family == TP_MOVE_PLANE && d.p < d.nprop - g.nlearn);
-            take = (ok && quat_move) ? d.K : 0;
+            take = ok ? d.K : 0;
         }
         s_off[tid] = take;
         __syncthreads();

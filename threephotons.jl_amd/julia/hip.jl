@@ -124,3 +124,52 @@ end
 function energy(intensity::SphericalHarmonicsVolume, basis::BasisTypeHip, c3ref::C3, measure::String="Bayes", negativity_factor::Float64=0.0)
     return energy(intensity, basis, c3ref, 1:basis.K, measure, negativity_factor)
 end
+
+"""Switch a basis to the mixed-precision contraction (1 = FP16 hi/lo, 2 = BF16, 0 = FP64)."""
+function hip_set_precision(basis::BasisTypeHip, mode::Integer)
+    hip_check(ccall((:tp_basis_set_precision, libtp), Cint, (Ptr{Void}, Cint), basis.handle, mode),
+              "tp_basis_set_precision")
+end
+
+"""Many rotate_all_at_once runs in one process: run g starts from intensities[g],
+scores against c3refs[g] over shells 1:size(c3refs[g], 3) on `basis` (mixed precision,
+K >= every run's), and every sweep evaluates `nprop` proposals per run in one graph launch.
+Returns the final energies; the runs' structures are read back into `intensities`."""
+function rotate_all_at_once_batched(intensities::Vector{SphericalHarmonicsVolume}, basis::BasisTypeHip,
+                                    c3refs::Vector{C3}, seeds::Vector{UInt64}, nsweeps::Int;
+                                    nprop::Int=64, temperature_factor::Float64=0.1,
+                                    decay::Float64=0.99998, factor::Float64=1.05)
+    n = length(intensities)
+    refs = Ptr{Void}[hip_ref(basis, c3refs[g], 1:size(c3refs[g], 3)) for g in 1:n]
+    coeffs = [hip_pack(intensities[g], size(c3refs[g], 3)) for g in 1:n]
+    ptrs = Ptr{Complex{Float64}}[pointer(c) for c in coeffs]
+    lds = Int64[size(c, 1) for c in coeffs]
+    batch = Ref{Ptr{Void}}(C_NULL)
+    hip_check(ccall((:tp_batch_create, libtp), Cint,
+                    (Ptr{Void}, Ptr{Void}, Cint, Cint, Ptr{Ptr{Void}}, Ptr{Ptr{Complex{Float64}}}, Ptr{Int64},
+                     Ptr{UInt64}, Cint, Ptr{Ptr{Void}}),
+                    hip_ctx, basis.handle, n, nprop, refs, ptrs, lds, seeds, nsweeps, batch), "tp_batch_create")
+    try
+        hip_check(ccall((:tp_batch_probe, libtp), Cint, (Ptr{Void}, Cdouble), batch[], temperature_factor),
+                  "tp_batch_probe")
+        hip_check(ccall((:tp_batch_run, libtp), Cint, (Ptr{Void}, Cint, Cdouble, Cdouble), batch[], nsweeps, decay,
+                        factor), "tp_batch_run")
+        E = zeros(n)
+        for g in 1:n
+            Eg = Ref{Float64}(0.0)
+            hip_check(ccall((:tp_batch_group, libtp), Cint, (Ptr{Void}, Cint, Ptr{Float64}, Ptr{Float64},
+                            Ptr{Float64}, Ptr{Int32}, Ptr{Int32}), batch[], g - 1, Eg, C_NULL, C_NULL, C_NULL,
+                            C_NULL), "tp_batch_group")
+            E[g] = Eg[]
+            out = copy(coeffs[g])
+            hip_check(ccall((:tp_batch_coeff, libtp), Cint, (Ptr{Void}, Cint, Ptr{Complex{Float64}}, Int64),
+                            batch[], g - 1, out, size(out, 1)), "tp_batch_coeff")
+            for k in 1:size(out, 2)
+                intensities[g].coeff[k][:] = out[:, k]
+            end
+        end
+        return E
+    finally
+        ccall((:tp_batch_destroy, libtp), Void, (Ptr{Void},), batch[])
+    end
+end

@@ -115,11 +115,18 @@ def _ctx(ctx):
 class BasisTypeHip:
     """~ BasisTypeCuda (cuda.jl:24-42): the precomputed basis, resident on the GPU."""
 
-    def __init__(self, L, LMAX, N, K, lam=0.0, dq=0.0, forIntensity=True, ctx=None):
+    def __init__(self, L, LMAX, N, K, lam=0.0, dq=0.0, forIntensity=True, ctx=None, _host=None):
         self.ctx = _ctx(ctx)
         h = C.c_void_p()
-        check(lib().tp_basis_create(self.ctx.h, L, LMAX, N, K, float(lam), float(dq), int(bool(forIntensity)),
-                                    C.byref(h)), "tp_basis_create")
+        if _host is None:
+            check(lib().tp_basis_create(self.ctx.h, L, LMAX, N, K, float(lam), float(dq), int(bool(forIntensity)),
+                                        C.byref(h)), "tp_basis_create")
+        else:
+            w, ind, pac = _host
+            pp = pac.ctypes.data if pac is not None else None
+            npc = pac.shape[0] if pac is not None else 0
+            check(lib().tp_basis_from_host(self.ctx.h, w.ctypes.data, ind.ctypes.data, w.shape[0], pp, npc, N, L,
+                                           LMAX, K, float(lam), float(dq), C.byref(h)), "tp_basis_from_host")
         self.h = h
         self.L, self.LMAX, self.N, self.K, self.lam, self.dq = L, LMAX, N, K, float(lam), float(dq)
         self.lrange = range(0, L + 1, 2 if forIntensity else 1)
@@ -167,6 +174,15 @@ class BasisTypeHip:
 def HIP_calculate_basis(L, LMAX, N, K, lam=0.0, dq=0.0, forIntensity=True, ctx=None):
     """~ CUDA_calculate_basis (cuda.jl:57-65)."""
     return BasisTypeHip(L, LMAX, N, K, lam, dq, forIntensity, ctx)
+
+
+def HIP_store_basis(wignerlist, indices, N, L, LMAX, K, pacombos=None, lam=0.0, dq=0.0, ctx=None):
+    """~ CUDA_store_basis (cuda.jl:45-55): a basis from the caller's wignerlist
+    and indices (basislen x 9 rows [sean1, sean2, sean3, l1, m1, l2, m2, l3, -m3])."""
+    w = np.ascontiguousarray(wignerlist, dtype=np.float64)
+    ind = np.ascontiguousarray(indices, dtype=np.int64).reshape(w.shape[0], 9)
+    pac = None if pacombos is None else np.ascontiguousarray(pacombos, dtype=np.int64).reshape(-1, 9)
+    return BasisTypeHip(L, LMAX, N, K, lam, dq, True, ctx, _host=(w, ind, pac))
 
 
 def _coeff_ptr(intensity, basis, nshell=None):

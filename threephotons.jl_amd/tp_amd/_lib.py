@@ -53,6 +53,7 @@ def lib():
     L.tp_finalize.argtypes = [vp]
     L.tp_finalize.restype = None
     L.tp_basis_create.argtypes = [vp, i32, i32, i32, i32, dbl, dbl, i32, pp]
+    L.tp_basis_from_host.argtypes = [vp, vp, vp, i64, vp, i64, i32, i32, i32, i32, dbl, dbl, pp]
     L.tp_basis_destroy.argtypes = [vp]
     L.tp_basis_destroy.restype = None
     L.tp_basis_info.argtypes = [vp, C.POINTER(i64), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64)]
@@ -134,7 +135,7 @@ EXPORTS = [
     "tp_batch_eval_quats", "tp_rotate", "tp_mlp_nparam", "tp_mlp_create", "tp_mlp_destroy", "tp_mlp_get_params",
     "tp_mlp_set_params", "tp_mlp_forward", "tp_mlp_loss_grad", "tp_mlp_train", "tp_mlp_train_log", "tp_batch_log",
     "tp_batch_log_count", "tp_batch_log_read", "tp_batch_set_proposer", "tp_batch_set_amax",
-    "tp_batch_set_family",
+    "tp_batch_set_family", "tp_basis_from_host",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16 = 0, 1, 2
