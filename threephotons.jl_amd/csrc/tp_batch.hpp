@@ -297,23 +297,8 @@ __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict
 
 // ---------------------------------------------------------------------------
 // Triple products of a batch: work item = (evaluation, l-triple | half << 16),
-// most expensive first; same body as k_triple_mfma with the evaluation's own
-// coefficients, shell count and column blocks.
+// most expensive first.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void k_triple_batch(const EvalDesc *__restrict__ ev, const int2 *__restrict__ work,
-                                                      int n_c, const double *__restrict__ ew,
-                                                      const uint32_t *__restrict__ eidx,
-                                                      const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
-                                                      int Lw, int Emax, int Prows, double *__restrict__ As,
-                                                      int64_t ldA)
-{
-    extern __shared__ double2 sm[];
-    const int2 w = work[blockIdx.x];
-    const EvalDesc &d = ev[w.x];
-    triple_mfma_body(sm, d.coef, n_c, d.K, w.y & 0xffff, w.y >> 16, ew, eidx, gstart, ltinfo, Lw, Emax, Prows, As, ldA,
-                     d.cbase, d.nprop, d.p);
-}
-
 // Mixed-precision triple products of a batch (triple16_body); shell scales
 // of evaluation e at shsc + e * kst.
 __global__ __launch_bounds__(512) void k_triple16_batch(const EvalDesc *__restrict__ ev, const int2 *__restrict__ work,

@@ -204,7 +204,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
                            b->d_shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
                            b->EmaxLt8, b->Prows, b->d_As, b->kl_pad);
     } else if (b->use_mfma) {
-        hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(512), b->mfma_lds, st, b0, b1, sel, flip,
+        hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(1024), b->mfma_lds, st, b0, b1, sel, flip,
                            b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_work, b->Lw, b->EmaxLt,
                            b->Prows, b->d_As, b->kl_pad);
     } else {
@@ -412,9 +412,6 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     if (!b->use_mfma) return tp_fail(TP_E_UNSUPPORTED, "batches need the MFMA triple products (K <= 27)");
     if (b->prec == TP_PREC_FP64) return tp_fail(TP_E_UNSUPPORTED, "batched contraction runs in mixed precision");
     if (b->R % 128 != 0) return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs 2N^2 to be a multiple of 128");
-    if (b->mfma_lds > 65536)
-        (void)hipFuncSetAttribute((const void *)k_triple_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)b->mfma_lds);
     tp_batch *B = new tp_batch();
     B->ctx = ctx;
     B->b = b;
@@ -1286,7 +1283,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         for (auto &w : wl) work.push_back(w.second);
         b->nwork = (int)work.size();
         b->Prows = 16 * maxrb;
-        b->mfma_lds = sizeof(double2) * 3 * (size_t)K * b->Lw + sizeof(double) * 2 * (size_t)b->Prows * 16 +
+        b->mfma_lds = sizeof(double2) * 3 * (size_t)K * b->Lw + sizeof(double) * 32 * (size_t)b->Prows +
                       (sizeof(double) + sizeof(uint32_t)) * (size_t)b->EmaxLt + sizeof(int) * ((size_t)b->Lw + 1) +
                       sizeof(int) * (size_t)b->Prows + 64;
         int maxlds = 0;

@@ -177,19 +177,21 @@ __global__ __launch_bounds__(1024) void k_triple(const double2 *__restrict__ cb0
 
 // ---------------------------------------------------------------------------
 // Triple products on FP64 MFMA (K <= 28).  Workgroup = (l-triple lt, half h
-// of the k1 range: k1 in 16h+1..16h+16), 8 waves.  The l1, l2, l3 blocks of
-// all K shells sit in LDS.  Iteration t handles mu3 = 8t..8t+7, one per wave:
+// of the k1 range: k1 in 16h+1..16h+16), 16 waves (the LDS allows one
+// workgroup per CU; 16 waves give 4 per SIMD to hide the LDS latency).  The
+// l1, l2, l3 blocks of all K shells sit in LDS.  Iteration t handles
+// mu3 = 16t..16t+15, one per wave:
 //   stage 1: [Tr | Ti](k1, k2) = A'.B' over the group's E entries, where
 //            A'[k1][e] = w_e Re c_k1[i1_e], A'[k1][E+e] = w_e Im c_k1[i1_e],
 //            B'[e] = [Re c_k2[i2_e] | Im c_k2[i2_e]],
 //            B'[E+e] = [-Im c_k2[i2_e] | Re c_k2[i2_e]]
 //            (complex T = sum_e a_e b_e as one real GEMM with K-dim 2E);
 //            the (k1 >= k2) entries go to Tf[slot][pair] in LDS, slot =
-//            (mu3 - 8t, re/im) (slot-major keeps lanes on distinct banks);
+//            (mu3 - 16t, re/im) (slot-major keeps lanes on distinct banks);
 //   stage 2: As[pair][k3] += sum_slot Tf[slot][pair] . [Re c_k3[l3,mu3] ; -Im c_k3[l3,mu3]],
 //            accumulated across all iterations in MFMA registers.
-// Tf is double-buffered, so one barrier per iteration suffices.  Same sums
-// as k_triple below, regrouped for MFMA.
+// For k1 <= 16 the k2 > 16 and k3 > 16 blocks are skipped.  Same sums as
+// k_triple below, regrouped for MFMA.
 // ---------------------------------------------------------------------------
 // Stage-1 MFMA operands of one k-step: lane (row l16, k lq) gets
 // A'[k1s][4 ks + lq] and B'[4 ks + lq][k2] for the two k2 blocks (Tr and Ti
@@ -252,31 +254,32 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
     double2 *c1b = sm;                                   // [K][n1]
     double2 *c2b = c1b + (size_t)K * Lw;                 // [K][n2]
     double2 *c3b = c2b + (size_t)K * Lw;                 // [K][n3]
-    double *Tf = reinterpret_cast<double *>(c3b + (size_t)K * Lw);   // [2][Prows][16]
-    double *sw = Tf + 2 * (size_t)Prows * 16;                        // [Emax]
+    double *Tf = reinterpret_cast<double *>(c3b + (size_t)K * Lw);   // [32 slots][Prows]
+    double *sw = Tf + 32 * (size_t)Prows;                            // [Emax]
     uint32_t *si = reinterpret_cast<uint32_t *>(sw + Emax);          // [Emax]
     int *sg = reinterpret_cast<int *>(si + Emax);                    // [n3 + 1]
     int *pk = sg + (n3 + 1);                                         // [Prows]
 
+    constexpr int NT = 1024;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int co1 = l1 * (l1 - 1) / 2, co2 = l2 * (l2 - 1) / 2, co3 = l3 * (l3 - 1) / 2;   // compact offsets (even l)
-    for (int i = tid; i < K * n1; i += 512) c1b[i] = coef[(size_t)(i / n1) * n_c + co1 + i % n1];
-    for (int i = tid; i < K * n2; i += 512) c2b[i] = coef[(size_t)(i / n2) * n_c + co2 + i % n2];
-    for (int i = tid; i < K * n3; i += 512) c3b[i] = coef[(size_t)(i / n3) * n_c + co3 + i % n3];
+    for (int i = tid; i < K * n1; i += NT) c1b[i] = coef[(size_t)(i / n1) * n_c + co1 + i % n1];
+    for (int i = tid; i < K * n2; i += NT) c2b[i] = coef[(size_t)(i / n2) * n_c + co2 + i % n2];
+    for (int i = tid; i < K * n3; i += NT) c3b[i] = coef[(size_t)(i / n3) * n_c + co3 + i % n3];
     const int e0 = gstart[goff], ne = gstart[goff + n3] - e0;
-    for (int i = tid; i < ne; i += 512) {
+    for (int i = tid; i < ne; i += NT) {
         sw[i] = ew[e0 + i];
         const uint32_t id = eidx[e0 + i];
         si[i] = (uint32_t)((int)(id & 0xffffu) - co1) | ((uint32_t)((int)(id >> 16) - co2) << 16);
     }
-    for (int i = tid; i <= n3; i += 512) sg[i] = gstart[goff + i] - e0;
+    for (int i = tid; i <= n3; i += NT) sg[i] = gstart[goff + i] - e0;
     // pairs (k1 >= k2) of this half, in the reference order, rows of Tf
     const int k1lo = 16 * h + 1, k1hi = min(16 * h + 16, K);
     const int off_h = (k1lo - 1) * k1lo / 2;                     // pairs with k1 < k1lo
     const int P = k1hi * (k1hi + 1) / 2 - off_h;
     const int nrb = (P + 15) / 16;
-    for (int p = tid; p < nrb * 16; p += 512) {
+    for (int p = tid; p < nrb * 16; p += NT) {
         if (p < P) {
             const int g = p + off_h;
             int k1 = (int)((1.0 + sqrt(1.0 + 8.0 * g)) * 0.5);
@@ -287,18 +290,17 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
             pk[p] = 0;
         }
     }
-    for (int i = tid; i < 2 * 16 * Prows; i += 512) Tf[i] = 0.0;
+    for (int i = tid; i < 32 * Prows; i += NT) Tf[i] = 0.0;
     __syncthreads();
 
-    dbl4 acc2[4];
+    dbl4 acc2[2];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc2[j] = dbl4{0.0, 0.0, 0.0, 0.0};
-    const int nob = h ? 2 * nrb : nrb;   // stage-2 output blocks
-    const int niter = (n3 + 7) / 8;
+    for (int j = 0; j < 2; ++j) acc2[j] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int nob = h ? 2 * nrb : nrb;   // stage-2 output blocks (<= 32)
+    const int niter = (n3 + 15) / 16;
     const int k1s = 16 * h + l16 + 1;   // stage-1 A row of this lane
     for (int t = 0; t < niter; ++t) {
-        double *T = Tf + (size_t)(t & 1) * Prows * 16;
-        const int mu = 8 * t + wave;     // index mu3 + l3 of this wave's group
+        const int mu = 16 * t + wave;    // index mu3 + l3 of this wave's group
         if (mu < n3) {
             const int s = sg[mu], E = sg[mu + 1] - s;
             const int nks = (2 * E + 3) / 4;
@@ -329,25 +331,25 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
                     const int k1 = 16 * h + lq + 4 * v + 1;
                     if (k1 <= K && k2 <= k1) {
                         const int p = (k1 - 1) * k1 / 2 + (k2 - 1) - off_h;
-                        T[slot * Prows + p] = a1[j][v];
+                        Tf[slot * Prows + p] = a1[j][v];
                     }
                 }
             }
         }
         __syncthreads();
-        // stage 2: blocks b = wave + 8j -> (row block b>>1, k3 block b&1); for
+        // stage 2: blocks b = wave + 16j -> (row block b>>1, k3 block b&1); for
         // h = 0 only k3 block 0 (k3 <= k2 <= 16): b -> (b, 0)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int b = wave + 8 * j;
+        for (int j = 0; j < 2; ++j) {
+            const int b = wave + 16 * j;
             if (b < nob) {
                 const int rb = h ? b >> 1 : b, cb = h ? b & 1 : 0;
                 const int k3 = 16 * cb + l16 + 1;
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
+                for (int kk = 0; kk < 8; ++kk) {
                     const int slot = 4 * kk + lq;
-                    const double av = T[slot * Prows + 16 * rb + l16];
-                    const int m3 = 8 * t + (slot >> 1);
+                    const double av = Tf[slot * Prows + 16 * rb + l16];
+                    const int m3 = 16 * t + (slot >> 1);
                     double bv = 0.0;
                     if (m3 < n3 && k3 <= K) {
                         const double2 c = c3b[(k3 - 1) * n3 + m3];
@@ -357,11 +359,12 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
                 }
             }
         }
+        __syncthreads();
     }
     // rows = pairs p = 16 rb + lq + 4v, columns k3 = 16 cb + l16 + 1
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int b = wave + 8 * j;
+    for (int j = 0; j < 2; ++j) {
+        const int b = wave + 16 * j;
         if (b < nob) {
             const int rb = h ? b >> 1 : b, cb = h ? b & 1 : 0;
             const int k3 = 16 * cb + l16 + 1;
@@ -380,7 +383,7 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
     }
 }
 
-__global__ __launch_bounds__(512) void k_triple_mfma(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
+__global__ __launch_bounds__(1024) void k_triple_mfma(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
                                                      const int *__restrict__ sel, int flip, int n_c, int K,
                                                      const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
                                                      const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
