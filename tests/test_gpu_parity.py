@@ -31,13 +31,16 @@ def say(*a):
 
 CASES = [
     # (L, LMAX, N, K)       tiny, ragged (R and klength not tile multiples), the small config,
-    # two k1 halves of the MFMA triple products (K=20), and K > 27 (the VALU triple-product kernel)
+    # two k1 halves of the MFMA triple products (K=20), K > 27 (the VALU triple-product kernel),
+    # N = 32 (the separated-grid contraction at its largest) and N > 32 (the GEMM contraction)
     (4, 6, 4, 3),
     (6, 9, 5, 5),
     (2, 3, 3, 2),
     (8, 25, 16, 8),
     (6, 8, 4, 20),
     (4, 6, 3, 29),
+    (4, 6, 32, 4),
+    (4, 6, 40, 3),
 ]
 
 

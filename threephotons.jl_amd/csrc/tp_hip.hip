@@ -116,6 +116,11 @@ struct tp_basis {
     // device tables
     double *d_H = nullptr;         // [nlt_pad][nm_pad64]   H[lt][class]
     double *d_Bm = nullptr;        // [nm_pad16][R_pad]     Bm[class][r]
+    // separated grid (k_c3sep): classes of one m2 contiguous
+    int n_m2 = 0, jpad = 0, sep_kb = 0, njc = 0;
+    int *d_m2start = nullptr;      // [n_m2 + 1]
+    double *d_TA = nullptr;        // [2][n_m][jpad]  cos / sin(mu3 b_j)
+    double *d_TB = nullptr;        // [round_up(N,16)][sep_kb]  cos(m2 a_i), -sin(m2 a_i)
     double *d_ew = nullptr;        // [basislen] weights, sorted (lt, mu3, m1)
     uint32_t *d_eidx = nullptr;    // [basislen] compact i1 | i2 << 16
     int *d_gstart = nullptr;       // [n_groups + 1]
@@ -254,21 +259,37 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
                        (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0);
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
-    const dim3 grid((unsigned)(b->n_rt * b->n_ct));
-    if (r && c3out) {
-        hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
-                           b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
-                           r->d_qhat, b->d_part, b->kl_pad);
-    } else if (r) {
-        hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 0, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
-                           b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, nullptr, b->R,
-                           r->d_qhat, b->d_part, b->kl_pad);
+    if (b->N > 32) {
+        // c3 = Bm . G as a GEMM (k_c3sep keeps at most 32 values of alpha per workgroup)
+        const dim3 grid((unsigned)(b->n_rt * b->n_ct));
+        if (r && c3out)
+            hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
+                               b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
+                               r->d_qhat, b->d_part, b->kl_pad);
+        else if (r)
+            hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 0, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
+                               b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, nullptr, b->R,
+                               r->d_qhat, b->d_part, b->kl_pad);
+        else
+            hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, false>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
+                               b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
+                               nullptr, nullptr, (int64_t)0);
+        TP_HIP(hipGetLastError());
+        n_rt_fin = b->n_rt;
     } else {
-        hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, false>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
-                           b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
-                           nullptr, nullptr, (int64_t)0);
-    }
+    // c3 = Bm . G on the separated grid, energy fused
+    const dim3 grid((unsigned)((ncols + 15) / 16), (unsigned)b->njc);
+    const int64_t tap = (int64_t)b->n_m * b->jpad;
+#define TP_SEP(OUT, EN, OUTP, QH)                                                                                   \
+    hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), 0, st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
+                       tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, b->d_part, b->kl_pad, OUTP, b->R)
+    if (r && c3out) TP_SEP(1, true, c3out, r->d_qhat);
+    else if (r) TP_SEP(0, true, nullptr, r->d_qhat);
+    else TP_SEP(1, false, c3out, nullptr);
+#undef TP_SEP
     TP_HIP(hipGetLastError());
+    n_rt_fin = b->njc;
+    }
     }
     if (timing) TP_HIP(hipEventRecord(ctx->ev[3], st));
     if (r) {
@@ -1028,7 +1049,8 @@ void tp_basis_destroy(tp_basis *b)
     void *ptrs[] = {b->d_H,   b->d_Bm,  b->d_ew,   b->d_eidx,    b->d_gstart, b->d_ltinfo, b->d_chunks,
                     b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_blk,
                     b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
-                    b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8};
+                    b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
+                    b->d_m2start, b->d_TA, b->d_TB};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1334,6 +1356,32 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
                 Bm[(size_t)c * b->R_pad + ia + (size_t)jb * N] = std::cos(cls_m2[c] * a + cls_mu3[c] * be);
             }
 
+    // separated-grid tables (k_c3sep)
+    b->n_m2 = L / 2 + 1;
+    b->jpad = (int)round_up(2 * N, 16);
+    b->sep_kb = (int)round_up(2 * b->n_m2, 4);
+    b->njc = b->jpad / 16;
+    std::vector<int> m2start(b->n_m2 + 1, b->n_m);
+    for (int c = b->n_m - 1; c >= 0; --c) m2start[cls_m2[c] / 2] = c;
+    for (int i = b->n_m2 - 1; i >= 0; --i)
+        if (m2start[i] > m2start[i + 1]) m2start[i] = m2start[i + 1];
+    std::vector<double> TA((size_t)2 * b->n_m * b->jpad, 0.0);
+    for (int c = 0; c < b->n_m; ++c)
+        for (int j = 0; j < 2 * N; ++j) {
+            const double be = (j + 0.5) * tph::kPi / N;
+            TA[(size_t)c * b->jpad + j] = std::cos(cls_mu3[c] * be);
+            TA[(size_t)(b->n_m + c) * b->jpad + j] = std::sin(cls_mu3[c] * be);
+        }
+    const int nip = (int)round_up(N, 16);
+    std::vector<double> TB((size_t)nip * b->sep_kb, 0.0);
+    for (int i = 0; i < N; ++i) {
+        const double a = (i + 0.5) * tph::kPi / N;
+        for (int m2i = 0; m2i < b->n_m2; ++m2i) {
+            TB[(size_t)i * b->sep_kb + 2 * m2i] = std::cos(2 * m2i * a);
+            TB[(size_t)i * b->sep_kb + 2 * m2i + 1] = -std::sin(2 * m2i * a);
+        }
+    }
+
     int rc = 0;
     auto up = [&](auto **dptr, const auto &vec) -> int {
         using T = typename std::remove_reference<decltype(vec)>::type::value_type;
@@ -1346,13 +1394,15 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         (rc = up(&b->d_gstart, gstart)) || (rc = up(&b->d_ltinfo, ltinfo)) || (rc = up(&b->d_chunks, chunks)) ||
         (rc = up(&b->d_pairs, pairs)) || (rc = up(&b->d_sean0, b->sean0)) || (rc = up(&b->d_coff, b->coff)) ||
         (b->use_mfma && (rc = up(&b->d_work, work))) || (rc = up(&b->d_ew8, ew8)) || (rc = up(&b->d_eidx8, eidx8)) ||
-        (rc = up(&b->d_gstart8, gstart8))) {
+        (rc = up(&b->d_gstart8, gstart8)) || (rc = up(&b->d_m2start, m2start)) || (rc = up(&b->d_TA, TA)) ||
+        (rc = up(&b->d_TB, TB))) {
         tp_basis_destroy(b);
         return rc;
     }
     if ((rc = dev_alloc(&b->d_As, (size_t)b->nlt_pad * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_G, (size_t)b->nm_pad64 * b->kl_pad)) ||
-        (rc = dev_alloc(&b->d_part, (size_t)b->n_rt * 2 * b->kl_pad)) || (rc = dev_alloc(&b->d_blk, b->n_fb)) ||
+        (rc = dev_alloc(&b->d_part, (size_t)std::max(b->n_rt, b->njc) * 2 * b->kl_pad)) ||
+        (rc = dev_alloc(&b->d_blk, b->n_fb)) ||
         (rc = dev_alloc(&b->d_counter, 1)) || (rc = dev_alloc(&b->d_out, 8)) ||
         (rc = dev_alloc(&b->d_raw, (size_t)K * LMAX * LMAX)) || (rc = dev_alloc(&b->d_coef, (size_t)K * b->n_c)) ||
         (rc = dev_alloc(&b->d_shsc, (size_t)K))) {

@@ -1125,3 +1125,136 @@ __global__ void k_intensity_coeff(const double2 *__restrict__ G, const double *_
     }
     coeff[(int64_t)k * LMAX * LMAX + sean_lm[q]] = make_double2(re, im);
 }
+
+// ---------------------------------------------------------------------------
+// c3 = Bm . G by separation of the (alpha, beta) grid, FP64 MFMA.
+// Bm[r,(m2,mu3)] = cos(m2 a_i + mu3 b_j) = cos(m2 a_i) cos(mu3 b_j) - sin(m2 a_i) sin(mu3 b_j)
+// with r = i + N j, so per k-triple column
+//   stage A: C[m2][j] = sum_{mu3} cos(mu3 b_j) G[(m2,mu3)],  S[m2][j] = sum_{mu3} sin(mu3 b_j) G[(m2,mu3)]
+//   stage B: c3[i][j] = sum_{m2} cos(m2 a_i) C[m2][j] - sin(m2 a_i) S[m2][j]
+// (2.136.2N + 2.2 n_m2.N.2N flop per column instead of 2.R.136).  Workgroup
+// = (16 columns, 16 values of j); classes of one m2 are contiguous
+// (m2start).  Tables: TA[cs][class][j] = cos / sin(mu3_class b_j) (ld jpad),
+// TB[i][2 m2i + cs] = cos(m2 a_i) / -sin(m2 a_i) (ld kb).  Energy partial
+// sums of the workgroup's rows go to part row blockIdx.y (as k_gemm does per
+// row tile); OUT = 1 also writes c3 (column-major, ld ldo).
+// ---------------------------------------------------------------------------
+constexpr int kSepMaxKB = 24;   // 2 (L/2 + 1) padded to 4, L <= 22
+
+template <int OUT, bool ENERGY>
+__global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
+                                               int n_m2, const double *__restrict__ TA, int64_t ta_plane, int jpad,
+                                               const double *__restrict__ TB, int kb, int N, int ncols,
+                                               const double *__restrict__ qhat, double *__restrict__ part,
+                                               int64_t ldp, double *__restrict__ out, int64_t ldo)
+{
+    // 8 waves; every global operand a wave needs is requested before it is
+    // used (the kernel is latency-bound: one residency wave of workgroups)
+    constexpr int NW = 8, MAXB = 4;   // stage-B blocks per wave: 16 j x (N/16) i-blocks / 8 waves (N <= 32)
+    __shared__ double CS[kSepMaxKB][16][17];   // [2 m2i + cs][j local][col], padded row
+    __shared__ double red[NW][2][16];
+    const int c0 = blockIdx.x * 16, jc = blockIdx.y, j0 = jc * 16;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int R = 2 * N * N, twoN = 2 * N;
+    const int nib = (N + 15) / 16, nob = 16 * nib;
+    // histogram values of this wave's stage-B outputs (transposed blocks:
+    // lane -> column c0 + lq + 4v, row i = 16 ib + l16)
+    double qv[MAXB][4];
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        const int ob = wave + NW * t;
+        const int jl = ob / nib, ib = ob % nib, j = j0 + jl, i = 16 * ib + l16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int col = c0 + lq + 4 * v;
+            qv[t][v] = (ENERGY && ob < nob && j < twoN && i < N && col < ncols) ? qhat[(int64_t)col * R + i + N * j] : 0.0;
+        }
+    }
+    // padding rows of CS (k' >= 2 n_m2) stay zero
+    for (int i = tid; i < (kb - 2 * n_m2) * 256; i += 512) {
+        const int k = 2 * n_m2 + i / 256, r = (i / 16) % 16, c = i % 16;
+        CS[k][r][c] = 0.0;
+    }
+    // ---- stage A: blocks (m2i, cs), rows j local, K = classes of m2 (<= 24)
+    for (int b = wave; b < 2 * n_m2; b += NW) {
+        const int m2i = b >> 1, cs = b & 1;
+        const int cb = m2start[m2i], ce = m2start[m2i + 1];
+        const double *ta = TA + (int64_t)cs * ta_plane;
+        double av[6], gv[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int cls = cb + 4 * q + lq;
+            const bool ok = cls < ce;
+            av[q] = ok ? ta[(int64_t)cls * jpad + j0 + l16] : 0.0;
+            gv[q] = ok ? G[(int64_t)cls * ldg + c0 + l16] : 0.0;
+        }
+        dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (cb + 4 * q < ce) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], gv[q], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) CS[b][lq + 4 * v][l16] = acc[v];
+    }
+    __syncthreads();
+    // ---- stage B: blocks (j local, i block), K = 2 n_m2 (padded to kb), computed
+    // transposed (rows = columns kt, cols = i) so that a lane's c3 values run
+    // along contiguous rows r of the histogram column
+    double sp[4] = {0.0, 0.0, 0.0, 0.0}, sq[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < MAXB; ++t) {
+        const int ob = wave + NW * t;
+        if (ob >= nob) break;
+        const int jl = ob / nib, ib = ob % nib;
+        const int j = j0 + jl;
+        if (j >= twoN) continue;   // wave-uniform
+        double tb[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) tb[q] = 4 * q < kb ? TB[(int64_t)(16 * ib + l16) * kb + 4 * q + lq] : 0.0;
+        dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (4 * q < kb) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CS[4 * q + lq][jl][l16], tb[q], acc, 0, 0, 0);
+        const int i = 16 * ib + l16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int col = c0 + lq + 4 * v;
+            if (col < ncols && i < N) {
+                const int r = i + N * j;
+                const double c = acc[v];
+                if (OUT == 1) out[(int64_t)col * ldo + r] = c;
+                if (ENERGY) {
+                    const double pv = c > 1e-30 ? c : 1e-30;
+                    sp[v] += pv;
+                    sq[v] += qv[t][v] * log(pv);
+                }
+            }
+        }
+    }
+    if (ENERGY) {
+        // column c0 + lq + 4v: sum over the 16 lanes of the same lq
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                sp[v] += __shfl_xor(sp[v], o);
+                sq[v] += __shfl_xor(sq[v], o);
+            }
+            if (l16 == 0) {
+                red[wave][0][lq + 4 * v] = sp[v];
+                red[wave][1][lq + 4 * v] = sq[v];
+            }
+        }
+        __syncthreads();
+        if (tid < 16 && c0 + tid < ncols) {
+            double a = 0.0, bsum = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                a += red[w][0][tid];
+                bsum += red[w][1][tid];
+            }
+            part[((int64_t)jc * 2 + 0) * ldp + c0 + tid] = a;
+            part[((int64_t)jc * 2 + 1) * ldp + c0 + tid] = bsum;
+        }
+    }
+}
