@@ -4,9 +4,23 @@
 //   p l m theta           -> sphPlm
 //   s l m LMAX            -> seanindex
 //   g n                   -> n Gauss-Legendre nodes and weights
+//   l n                   -> worst error (ulp of the exact value, long double
+//                            reference) of tp_fastlog over n arguments
+#include <cmath>
 #include <cstdio>
+#include <random>
 #include <vector>
+#include "tp_fastlog.hpp"
 #include "tp_host_math.hpp"
+
+static double ulp_err(double x, const double *tab)
+{
+    const long double ref = logl((long double)x);
+    const double got = tp_fastlog(x, tab);
+    const double rd = (double)ref;
+    const double u = std::nextafter(std::fabs(rd), INFINITY) - std::fabs(rd);
+    return (double)(fabsl((long double)got - ref) / (long double)u);
+}
 
 int main()
 {
@@ -25,6 +39,26 @@ int main()
             int l, m, L;
             if (std::scanf("%d %d %d", &l, &m, &L) != 3) return 1;
             std::printf("%lld\n", (long long)tph::seanindex(l, m, L));
+        } else if (c == 'l') {
+            int n;
+            if (std::scanf("%d", &n) != 1) return 1;
+            double tab[kLogTabSize];
+            tp_fastlog_table(tab);
+            std::mt19937_64 rng(12345);
+            std::uniform_real_distribution<double> U(0.0, 1.0);
+            double worst = 0.0;
+            for (int i = 0; i < n; ++i) {
+                double x;
+                switch (i % 4) {
+                case 0: x = std::exp(-69.0 + 138.0 * U(rng)); break;             // 1e-30 .. 1e30
+                case 1: x = 1.0 + (U(rng) - 0.5) * 0.03125; break;               // around 1
+                case 2: x = std::ldexp(1.0 + U(rng), (int)(U(rng) * 80) - 40); break;
+                default: x = std::ldexp(1.0 + (std::floor(U(rng) * 128) + (U(rng) < 0.5 ? 0.0 : 1.0)) / 128.0, 3) *
+                             (1.0 + (U(rng) - 0.5) * 1e-15); break;              // table-interval edges
+                }
+                worst = std::fmax(worst, ulp_err(x, tab));
+            }
+            std::printf("%.6g\n", worst);
         } else if (c == 'g') {
             int n;
             if (std::scanf("%d", &n) != 1) return 1;

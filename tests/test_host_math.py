@@ -60,3 +60,10 @@ def test_gauss_legendre(driver):
         xr, wr = np.polynomial.legendre.leggauss(n)
         assert np.abs(np.sort(x) - xr).max() < 1e-14
         assert np.abs(w[np.argsort(x)] - wr).max() < 1e-14
+
+
+def test_fastlog_ulp(driver):
+    """The energy epilogues' table-driven log (csrc/tp_fastlog.hpp) against a
+    long double reference over 1e-30..1e30, around 1 and at table edges."""
+    worst = float(driver(["l 400000"])[0])
+    assert worst < 4.0, worst

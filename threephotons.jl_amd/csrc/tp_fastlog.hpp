@@ -1,0 +1,89 @@
+// tp_fastlog.hpp -- table-driven FP64 natural logarithm for the energy
+// epilogues (the Bayes sum needs one log per histogram bin per evaluation).
+//
+// Method: P. T. P. Tang, "Table-driven implementation of the logarithm
+// function in IEEE floating-point arithmetic", ACM TOMS 16(4), 1990.
+// For normal positive x = 2^e m, m in [1, 2): j = the top 7 bits of m's
+// fraction, c_j = 1 + (j + 1/2)/128, inv_j = RN(1/c_j), t_j = -log(inv_j),
+// and for c_j >= sqrt2 the reduction is to m/2 in [1/sqrt2, 1): t_j -= ln2,
+// a_j = 1 (so e ln2 and t_j never cancel by more than a factor 2); tables
+// built on the host in long double by tp_fastlog_table, with ln 2 split as
+// hi + lo, hi having 21 significant bits so that (e + a_j) hi is exact;
+// r = m inv_j - 1 by one FMA (|r| < 2^-8, rounding error < 2^-60), and
+//   log x = (e + a_j) ln2 + t_j + log1p(r),
+// log1p by its Taylor series (to r^8 here, truncation < 2^-66).  Near 1
+// (|x - 1| < 2^-7, where e ln2 + t_j would cancel) log1p(x - 1) is summed
+// directly to r^10.  Error: a few ulp (checked against libm in
+// tests/test_host_math.py).  Non-positive, infinite, NaN and subnormal
+// arguments go to the library log.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#ifdef __HIPCC__
+#define TP_HD __host__ __device__
+#else
+#define TP_HD
+#endif
+
+constexpr int kLogTab = 128;
+constexpr int kLogTabSize = 3 * kLogTab + 2;   // inv_j, t_j, a_j, ln2 hi, ln2 lo
+
+inline void tp_fastlog_table(double *tab)
+{
+    const long double ln2 = logl(2.0L);
+    for (int j = 0; j < kLogTab; ++j) {
+        const long double c = 1.0L + (j + 0.5L) / kLogTab;
+        const double inv = (double)(1.0L / c);
+        const bool up = c >= sqrtl(2.0L);
+        tab[j] = inv;
+        tab[kLogTab + j] = (double)(-logl((long double)inv) - (up ? ln2 : 0.0L));
+        tab[2 * kLogTab + j] = up ? 1.0 : 0.0;
+    }
+    double hi = (double)ln2;
+    uint64_t b;
+    std::memcpy(&b, &hi, 8);
+    b &= ~0xFFFFFFFFull;   // keep 21 significant bits
+    std::memcpy(&hi, &b, 8);
+    tab[3 * kLogTab] = hi;
+    tab[3 * kLogTab + 1] = (double)(ln2 - (long double)hi);
+}
+
+TP_HD inline double tp_fastlog(double x, const double *tab)
+{
+    uint64_t bits;
+    std::memcpy(&bits, &x, 8);
+    const int ex = (int)((bits >> 52) & 0x7FF);
+    if (ex == 0 || ex == 0x7FF || (bits >> 63)) return std::log(x);   // 0, subnormal, inf/NaN, negative
+    const double d = x - 1.0;                                        // exact for x in [1/2, 2]
+    if (std::fabs(d) < 0.0078125) {
+        double p = -0.1;
+        p = std::fma(p, d, 1.0 / 9.0);
+        p = std::fma(p, d, -0.125);
+        p = std::fma(p, d, 1.0 / 7.0);
+        p = std::fma(p, d, -1.0 / 6.0);
+        p = std::fma(p, d, 0.2);
+        p = std::fma(p, d, -0.25);
+        p = std::fma(p, d, 1.0 / 3.0);
+        p = std::fma(p, d, -0.5);
+        return std::fma(p * d, d, d);
+    }
+    const int e = ex - 1023;
+    const uint64_t mb = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+    double m;
+    std::memcpy(&m, &mb, 8);
+    const int j = (int)((bits >> 45) & 0x7F);
+    const double r = std::fma(m, tab[j], -1.0);
+    double p = -0.125;
+    p = std::fma(p, r, 1.0 / 7.0);
+    p = std::fma(p, r, -1.0 / 6.0);
+    p = std::fma(p, r, 0.2);
+    p = std::fma(p, r, -0.25);
+    p = std::fma(p, r, 1.0 / 3.0);
+    p = std::fma(p, r, -0.5);
+    p = std::fma(p * r, r, r);
+    const double ee = (double)e + tab[2 * kLogTab + j];
+    return std::fma(ee, tab[3 * kLogTab], tab[kLogTab + j]) + std::fma(ee, tab[3 * kLogTab + 1], p);
+}

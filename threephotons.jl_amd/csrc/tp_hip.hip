@@ -121,6 +121,7 @@ struct tp_basis {
     int *d_m2start = nullptr;      // [n_m2 + 1]
     double *d_TA = nullptr;        // [2][n_m][jpad]  cos / sin(mu3 b_j)
     double *d_TB = nullptr;        // [round_up(N,16)][sep_kb]  cos(m2 a_i), -sin(m2 a_i)
+    double *d_logtab = nullptr;    // [kLogTabSize] tables of tp_fastlog
     double *d_ew = nullptr;        // [basislen] weights, sorted (lt, mu3, m1)
     uint32_t *d_eidx = nullptr;    // [basislen] compact i1 | i2 << 16
     int *d_gstart = nullptr;       // [n_groups + 1]
@@ -256,7 +257,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
     hipLaunchKernelGGL((k_gemm<G1_BM, G1_BN, G1_WGM, 2, false>), dim3((unsigned)(b->g1_rt * b->g1_ct)), dim3(256), 0,
                        st, b->d_H, b->nm_pad64, b->d_As, b->kl_pad, (int)b->nlt_pad, (int)b->nm_pad64,
-                       (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0);
+                       (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
     if (b->N > 32) {
@@ -265,15 +266,15 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         if (r && c3out)
             hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
                                b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
-                               r->d_qhat, b->d_part, b->kl_pad);
+                               r->d_qhat, b->d_part, b->kl_pad, b->d_logtab);
         else if (r)
             hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 0, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
                                b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, nullptr, b->R,
-                               r->d_qhat, b->d_part, b->kl_pad);
+                               r->d_qhat, b->d_part, b->kl_pad, b->d_logtab);
         else
             hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, false>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
                                b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
-                               nullptr, nullptr, (int64_t)0);
+                               nullptr, nullptr, (int64_t)0, nullptr);
         TP_HIP(hipGetLastError());
         n_rt_fin = b->n_rt;
     } else {
@@ -282,7 +283,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     const int64_t tap = (int64_t)b->n_m * b->jpad;
 #define TP_SEP(OUT, EN, OUTP, QH)                                                                                   \
     hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), 0, st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
-                       tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, b->d_part, b->kl_pad, OUTP, b->R)
+                       tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, b->d_part, b->kl_pad, OUTP, b->R,       \
+                       b->d_logtab)
     if (r && c3out) TP_SEP(1, true, c3out, r->d_qhat);
     else if (r) TP_SEP(0, true, nullptr, r->d_qhat);
     else TP_SEP(1, false, c3out, nullptr);
@@ -1050,7 +1052,7 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_blk,
                     b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
-                    b->d_m2start, b->d_TA, b->d_TB};
+                    b->d_m2start, b->d_TA, b->d_TB, b->d_logtab};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1382,6 +1384,9 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         }
     }
 
+    std::vector<double> logtab(kLogTabSize);
+    tp_fastlog_table(logtab.data());
+
     int rc = 0;
     auto up = [&](auto **dptr, const auto &vec) -> int {
         using T = typename std::remove_reference<decltype(vec)>::type::value_type;
@@ -1395,7 +1400,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         (rc = up(&b->d_pairs, pairs)) || (rc = up(&b->d_sean0, b->sean0)) || (rc = up(&b->d_coff, b->coff)) ||
         (b->use_mfma && (rc = up(&b->d_work, work))) || (rc = up(&b->d_ew8, ew8)) || (rc = up(&b->d_eidx8, eidx8)) ||
         (rc = up(&b->d_gstart8, gstart8)) || (rc = up(&b->d_m2start, m2start)) || (rc = up(&b->d_TA, TA)) ||
-        (rc = up(&b->d_TB, TB))) {
+        (rc = up(&b->d_TB, TB)) || (rc = up(&b->d_logtab, logtab))) {
         tp_basis_destroy(b);
         return rc;
     }

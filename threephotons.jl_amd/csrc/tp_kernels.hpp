@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "tp_fastlog.hpp"
+
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2 __attribute__((ext_vector_type(2)));   // promotable to registers, unlike HIP's double2
 
@@ -415,8 +417,13 @@ template <int BM, int BN, int WGM, int OUT, bool ENERGY>
 __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int64_t lda,
                                               const double *__restrict__ B, int64_t ldb, int Kdim, int M, int ncols,
                                               int n_rt, int n_ct, double *__restrict__ out, int64_t ldo,
-                                              const double *__restrict__ qhat, double *__restrict__ part, int64_t ldp)
+                                              const double *__restrict__ qhat, double *__restrict__ part, int64_t ldp,
+                                              const double *__restrict__ logtab)
 {
+    __shared__ double s_lt[ENERGY ? kLogTabSize : 1];
+    if (ENERGY)
+        for (int i = threadIdx.x; i < kLogTabSize; i += blockDim.x) s_lt[i] = logtab[i];
+    __syncthreads();
     constexpr int BK = 16;
     constexpr int WGN = 4 / WGM;
     constexpr int WTM = BM / WGM, WTN = BN / WGN;
@@ -516,7 +523,7 @@ __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int6
                         const double pv = c > 1e-30 ? c : 1e-30;
                         const double q = qhat[(int64_t)col * M + r];
                         sp += pv;
-                        sq += q * log(pv);
+                        sq += q * tp_fastlog(pv, s_lt);
                     }
                 }
             }
@@ -813,7 +820,7 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
                         if (ENERGY) {
                             const double pv = c > 1e-30 ? c : 1e-30;
                             sp[j] += pv;
-                            sq[j] += qcol[j][r] * log(pv);
+                            sq[j] += qcol[j][r] * log(pv);   // (tp_fastlog's table costs this kernel occupancy)
                         }
                     }
                 }
@@ -1146,8 +1153,12 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
                                                int n_m2, const double *__restrict__ TA, int64_t ta_plane, int jpad,
                                                const double *__restrict__ TB, int kb, int N, int ncols,
                                                const double *__restrict__ qhat, double *__restrict__ part,
-                                               int64_t ldp, double *__restrict__ out, int64_t ldo)
+                                               int64_t ldp, double *__restrict__ out, int64_t ldo,
+                                               const double *__restrict__ logtab)
 {
+    __shared__ double s_lt[ENERGY ? kLogTabSize : 1];
+    if (ENERGY)   // ordered before the epilogue by the barrier after stage A
+        for (int i = threadIdx.x; i < kLogTabSize; i += 512) s_lt[i] = logtab[i];
     // 8 waves; every global operand a wave needs is requested before it is
     // used (the kernel is latency-bound: one residency wave of workgroups)
     constexpr int NW = 8, MAXB = 4;   // stage-B blocks per wave: 16 j x (N/16) i-blocks / 8 waves (N <= 32)
@@ -1226,7 +1237,7 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
                 if (ENERGY) {
                     const double pv = c > 1e-30 ? c : 1e-30;
                     sp[v] += pv;
-                    sq[v] += qv[t][v] * log(pv);
+                    sq[v] += qv[t][v] * tp_fastlog(pv, s_lt);
                 }
             }
         }
