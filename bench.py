@@ -370,6 +370,16 @@ def main():
         state.energy(c3ref)
     el16 = time.perf_counter() - t0
     kern16 = stage_times(ctx, state, c3ref, args.steps)
+    basis.set_precision("triple16")
+    Et16 = state.energy(c3ref)
+    for _ in range(args.warmup):
+        state.energy(c3ref)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        state.energy(c3ref)
+    elt16 = time.perf_counter() - t0
+    kernt16 = stage_times(ctx, state, c3ref, args.steps)
+    basis.set_precision("fp16x3")
     mstate = tp.DeviceState(start, basis)
     t0 = time.perf_counter()
     res16 = tp.rotate_all_at_once(mstate, basis, c3ref, args.anneal_steps, np.pi / 4, 1.01, 0.1, 0.99998, seed=1000,
@@ -418,7 +428,11 @@ def main():
             "stage_ms": {"triple": kern16[0], "split": kern16[1], "gemm16_c3_energy": kern16[2], "final": kern16[3]},
             "gemm16_tflops_algorithmic": 2.0 * basis.R * basis.n_lt * basis.klength / (kern16[2] * 1e-3) / 1e12,
             "energy_abs_err_vs_fp64": abs(E16 - E), "mc_steps_per_s": (args.anneal_steps + 51) / t_anneal16,
-            "mc_E_final": res16["E_final"]},
+            "mc_E_final": res16["E_final"],
+            "triple16": {"mode": "compensated FP16 triple products, FP64 separated-grid contraction",
+                         "evals_per_s": args.steps / elt16, "energy_abs_err_vs_fp64": abs(Et16 - E),
+                         "stage_ms": {"triple": kernt16[0], "gemm_G": kernt16[1], "c3_energy": kernt16[2],
+                                      "final": kernt16[3]}}},
         "roofline": {"bound": "mfma", "kernel": "k_triple_mfma (coupling-form triple products, FP64 MFMA)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,

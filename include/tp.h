@@ -86,11 +86,15 @@ int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *
  *                  with FP32 accumulation.  Triple products, the histogram
  *                  terms and the log-likelihood sum stay FP64.
  *   TP_PREC_BF16   one BF16 product with FP32 accumulation (comparison mode).
- * No counterpart in the reference (whose GPU path is FP32 end to end,
- * cuda.jl:79-88). */
+ *   TP_PREC_TRIPLE16 compensated FP16 triple products (as in FP16X3) with the
+ *                  FP64 contraction: the fastest single evaluation.
+ * FP16X3, BF16 and TRIPLE16 compute the triple products on FP16 MFMA with
+ * hi/lo compensation.  No counterpart in the reference (whose GPU path is FP32
+ * end to end, cuda.jl:79-88). */
 #define TP_PREC_FP64 0
 #define TP_PREC_FP16X3 1
 #define TP_PREC_BF16 2
+#define TP_PREC_TRIPLE16 3
 int tp_basis_set_precision(tp_basis *b, int mode);
 int tp_basis_get_precision(const tp_basis *b);
 

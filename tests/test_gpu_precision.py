@@ -38,6 +38,9 @@ def test_fp16x3_c3_and_energy(tp, L, LMAX, N, K):
     E16 = tp.energy(vol, b, ref)
     b.set_precision("bf16")
     cbf = tp.FullCorrelation_parallized(vol, b, return_raw=True)
+    b.set_precision("triple16")
+    ct16 = tp.FullCorrelation_parallized(vol, b, return_raw=True)
+    Et16 = tp.energy(vol, b, ref)
     b.set_precision("fp64")
     again = tp.FullCorrelation_parallized(vol, b, return_raw=True)
     assert np.array_equal(again, c64)            # switching modes leaves the FP64 path untouched
@@ -47,6 +50,10 @@ def test_fp16x3_c3_and_energy(tp, L, LMAX, N, K):
     assert ebf > 10 * e16                         # one BF16 product is not enough for 1e-5
     assert ebf < 5e-2
     assert abs(E16 - E64) / abs(E64) < 1e-6
+    et = slice_err(ct16, c64)                    # triple products alone in compensated FP16
+    print(f"   triple16 slice err {et:.2e}, |dE| {abs(Et16 - E64):.2e}")
+    assert et < C3_TOL_FP16X3
+    assert abs(Et16 - E64) / abs(E64) < 1e-6
 
 
 def test_fp16x3_full_size(tp):

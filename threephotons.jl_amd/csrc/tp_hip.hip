@@ -222,7 +222,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     if (timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     const int ncols = r ? (int)r->ncols : (int)b->klength;
     int n_rt_fin = b->n_rt;
-    if (b->prec != TP_PREC_FP64) {
+    if (b->prec == TP_PREC_FP16X3 || b->prec == TP_PREC_BF16) {
         // mixed precision: c3 = F . As with 16-bit MFMA operands
         const double *qh = r ? r->d_qhat : nullptr;
         const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
@@ -433,7 +433,8 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
 {
     *out = nullptr;
     if (!b->use_mfma) return tp_fail(TP_E_UNSUPPORTED, "batches need the MFMA triple products (K <= 27)");
-    if (b->prec == TP_PREC_FP64) return tp_fail(TP_E_UNSUPPORTED, "batched contraction runs in mixed precision");
+    if (b->prec != TP_PREC_FP16X3 && b->prec != TP_PREC_BF16)
+        return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs TP_PREC_FP16X3 or TP_PREC_BF16");
     if (b->R % 128 != 0) return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs 2N^2 to be a multiple of 128");
     tp_batch *B = new tp_batch();
     B->ctx = ctx;
@@ -1443,12 +1444,12 @@ int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *
 int tp_basis_set_precision(tp_basis *b, int mode)
 {
     if (!b) return tp_fail(TP_E_ARG, "null basis");
-    if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3 && mode != TP_PREC_BF16)
+    if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3 && mode != TP_PREC_BF16 && mode != TP_PREC_TRIPLE16)
         return tp_fail(TP_E_ARG, "unknown precision mode");
     if (int rc = check_ctx(b->ctx)) return rc;
     if (mode != TP_PREC_FP64 && (b->K > 64 || b->n_c > 1024))
         return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64 and L <= 44");
-    if (mode != TP_PREC_FP64 && !b->d_Fhi) {
+    if ((mode == TP_PREC_FP16X3 || mode == TP_PREC_BF16) && !b->d_Fhi) {
         // F[r][lt] = sum_class Bm[class][r] H[lt][class] in FP64, then scaled by a
         // power of two into [2^13, 2^14) and split into FP16 hi/lo and BF16
         const int64_t nlt = b->n_lt;
@@ -2020,7 +2021,7 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
         if (bases[i]->prec != bm->prec) return tp_fail(TP_E_ARG, "batch members must share the precision mode");
     }
     hipStream_t st = ctx->stream;
-    if (bm->prec == TP_PREC_FP64 || !bm->use_mfma || bm->R % 128 != 0) {
+    if ((bm->prec != TP_PREC_FP16X3 && bm->prec != TP_PREC_BF16) || !bm->use_mfma || bm->R % 128 != 0) {
         // one evaluation after another on the device
         double *dE = nullptr;
         if (int rc = dev_alloc(&dE, n)) return rc;

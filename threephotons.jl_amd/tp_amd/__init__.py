@@ -147,8 +147,11 @@ class BasisTypeHip:
             pass
 
     def set_precision(self, mode):
-        """'fp64' (default), 'fp16x3' (compensated FP16 MFMA) or 'bf16'."""
-        m = {"fp64": PREC_FP64, "fp16x3": PREC_FP16X3, "bf16": PREC_BF16}[mode] if isinstance(mode, str) else mode
+        """'fp64' (default), 'fp16x3' (compensated FP16 MFMA triple products and
+        contraction), 'bf16', or 'triple16' (compensated FP16 triple products,
+        FP64 contraction)."""
+        m = {"fp64": PREC_FP64, "fp16x3": PREC_FP16X3, "bf16": PREC_BF16,
+             "triple16": 3}[mode] if isinstance(mode, str) else mode
         check(lib().tp_basis_set_precision(self.h, m), "tp_basis_set_precision")
 
     def ref(self, c3ref, K3_range=None):

@@ -138,5 +138,5 @@ EXPORTS = [
     "tp_batch_set_family", "tp_basis_from_host",
 ]
 
-PREC_FP64, PREC_FP16X3, PREC_BF16 = 0, 1, 2
+PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
 MOVE_QUAT, MOVE_PLANE = 0, 1
