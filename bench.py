@@ -102,24 +102,53 @@ def setup(cfg, tp, workload):
 
 
 def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
-    """Reference formulation (dense basislen-inner contraction B*PA with the
-    triple-product stage, correlations.jl:157-193) timed by the oracle on a
-    sample of k-triple columns, scaled to whole evaluations."""
+    """The oracle (C, FP64, OpenMP over `nthreads` host threads) in the two forms:
+    the reference formulation (dense basislen-inner contraction B*PA with the
+    triple-product stage, correlations.jl:157-193) on a sample of k-triple
+    columns, scaled to whole evaluations; the factorised formulation (the GPU's
+    algorithm) on three whole evaluations; the MC move; and the small
+    configuration in full."""
     from oracle import pyoracle as po
+    from tp_amd import workload
     ob = po.Basis(cfg["L"], cfg["LMAX"], cfg["N"], cfg["K"])
     t0 = time.perf_counter()
     ob.c3_dense(start.coeff, 0, ncols, nthreads=nthreads)
     t_dense = time.perf_counter() - t0
     dense_evals = (ncols / ob.klength) / t_dense
+    t_fact = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        c3f = ob.c3_factored(start.coeff, nthreads=nthreads)
+        po.energy(c3f, cfg["N"], c3ref, cfg["K"], 1, cfg["K"])
+        t_fact.append(time.perf_counter() - t0)
+    steps = np.array([np.pi / 4 * (l - 1) for l in range(2, cfg["L"] + 1, 2)])
     t0 = time.perf_counter()
-    c3f = ob.c3_factored(start.coeff, nthreads=nthreads)
-    po.energy(c3f, cfg["N"], c3ref, cfg["K"], 1, cfg["K"])
-    t_fact = time.perf_counter() - t0
+    for i in range(20):
+        po.mc_propose(1000, i + 1, 0x4D4F5645, cfg["L"], steps, start.coeff, cfg["LMAX"])
+    t_move = (time.perf_counter() - t0) / 20
+    # the small configuration, both forms in full
+    sm = workload.CONFIGS["small"]
+    atoms = workload.atoms_in_ball(sm["natoms"], sm["radius"], sm["atoms_seed"])
+    coeff0 = po.intensity_from_atoms(atoms, sm["KMAX"], sm["qmax"] / sm["KMAX"], sm["LMAX"], sm["L"], sm["ntheta"],
+                                     sm["nphi"], nthreads=nthreads)
+    ob0 = po.Basis(sm["L"], sm["LMAX"], sm["N"], sm["K"])
+    t0 = time.perf_counter()
+    c3d0 = ob0.c3_dense(coeff0, nthreads=nthreads)
+    t_d0 = time.perf_counter() - t0
+    ref0, _ = workload.poisson_histogram(c3d0, sm["N"], sm["K"], sm["counts"], sm["hist_seed"])
+    t0 = time.perf_counter()
+    c3f0 = ob0.c3_factored(coeff0, nthreads=nthreads)
+    E0 = po.energy(c3f0, sm["N"], ref0, sm["K"], 1, sm["K"])
+    t_f0 = time.perf_counter() - t0
+    tf = float(np.median(t_fact))
     return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
                 sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
                        f"({t_dense:.1f}s), scaled to whole evaluations; host '{cpu_model()}'",
-                factored_evals_per_s=1.0 / t_fact,
-                factored_sample=f"one full factored evaluation (F.As + energy), {t_fact:.2f}s")
+                factored_evals_per_s=1.0 / tf,
+                factored_sample=f"3 full factorised evaluations (F.As + energy), median {tf:.2f}s",
+                mc_steps_per_s_factored=1.0 / (tf + t_move), move_seconds=t_move,
+                small_config={"dense_evals_per_s": 1.0 / t_d0, "factored_evals_per_s": 1.0 / t_f0,
+                              "energy": E0, "klength": ob0.klength, "R": 2 * sm["N"] ** 2})
 
 
 def precision_study(n):
@@ -304,7 +333,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--anneal-steps", type=int, default=2000)
-    ap.add_argument("--cpu-cols", type=int, default=192)
+    ap.add_argument("--cpu-cols", type=int, default=1600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-sweeps", type=int, default=20, help="sweeps of the concurrent-runs leg (0: skip)")
     ap.add_argument("--learned-data-sweeps", type=int, default=100,
@@ -342,7 +371,10 @@ def main():
     kern_ms = stage_times(ctx, state, c3ref, args.steps)
     n_m = folded_classes(cfg["L"])
     flops_g1 = 2.0 * n_m * basis.n_lt * basis.klength         # G = H . As
-    flops_g2 = 2.0 * basis.R * n_m * basis.klength            # c3 = Bm . G
+    # c3 = Bm . G on the separated (alpha, beta) grid: stage A (C, S over mu3 at
+    # 2N values of beta) and stage B (over m2 = 0, 2, .., L at every bin)
+    n_m2 = cfg["L"] // 2 + 1
+    flops_g2 = (8.0 * n_m * cfg["N"] + 4.0 * n_m2 * basis.R) * basis.klength
     K = cfg["K"]
     # coupling-form triple products: complex MACs (8 flop) of stage 1 and stage 2
     flops_tri = 8.0 * (K * (K + 1) // 2 * basis.basislen + basis.klength * lt_n3_sum(cfg["L"]))
@@ -438,7 +470,7 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRIPLE_FETCH_KIB * 1024 * FETCH_SCALE + TRIPLE_WRITE_KIB * 1024 * WRITE_SCALE,
                      "flops_per_launch": flops_tri, "launch_ms": kern_ms[0],
-                     "gemm_c3_energy_tflops": flops_g2 / (kern_ms[2] * 1e-3) / 1e12,
+                     "c3sep_energy_tflops": flops_g2 / (kern_ms[2] * 1e-3) / 1e12,
                      "gemm_G_tflops": flops_g1 / (kern_ms[1] * 1e-3) / 1e12},
     }
     if batch is not None:

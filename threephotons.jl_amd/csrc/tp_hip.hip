@@ -1728,7 +1728,7 @@ int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, cons
     if (p->initial_temperature_factor > 0.0 && p->ntemp > 1) {
         mc.mode = 1;
         for (int i = 0; i < p->ntemp; ++i) {
-            hipLaunchKernelGGL(k_propose, dim3(nl), dim3(256), 0, st, s->d_cur, s->d_prop, s->d_sel, s->nshell, b->n_c,
+            hipLaunchKernelGGL(k_propose, dim3(nl, (unsigned)((s->nshell + 3) / 4)), dim3(256), 0, st, s->d_cur, s->d_prop, s->d_sel, s->nshell, b->n_c,
                                b->d_coff, s->d_step, p->seed, (uint32_t)i, TP_TAG_TEMP);
             mc.step = (uint32_t)i;
             if ((rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 1, nullptr, b->d_out, mc))) {
@@ -1753,7 +1753,7 @@ int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, cons
     // the hot loop: propose, evaluate the proposal buffer, Metropolis in k_final
     mc.mode = 3;
     for (int i = 1; i <= p->nsteps; ++i) {
-        hipLaunchKernelGGL(k_propose, dim3(nl), dim3(256), 0, st, s->d_cur, s->d_prop, s->d_sel, s->nshell, b->n_c,
+        hipLaunchKernelGGL(k_propose, dim3(nl, (unsigned)((s->nshell + 3) / 4)), dim3(256), 0, st, s->d_cur, s->d_prop, s->d_sel, s->nshell, b->n_c,
                            b->d_coff, s->d_step, p->seed, (uint32_t)i, TP_TAG_MOVE);
         mc.step = (uint32_t)i;
         if ((rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 1, nullptr, b->d_out, mc))) {

@@ -965,8 +965,10 @@ __global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, 
 // vectors through the real basis, x <- ctr^H R ctr x, with ctr = Umat(l)
 // (spherical_harmonics.jl:445-457) in its two-entries-per-row form.  Lane i
 // of a wave owns entry m = i - l of one shell; the four waves share the
-// shells.  The current buffer is read, the other one written; workgroup 0
-// also copies the l = 0 block.
+// shells of the workgroup's chunk (blockIdx.y: 4 shells per workgroup, so
+// the launch has L/2 x nshell/4 workgroups; each recomputes its l's frame
+// from the same Philox counters).  The current buffer is read, the other one
+// written; the l = 2 workgroups also copy the l = 0 block.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -1008,7 +1010,7 @@ __global__ __launch_bounds__(256) void k_propose(double2 *__restrict__ b0, doubl
     const int m = lane - l;                      // this lane's order m (valid when lane < n)
     const int am = m < 0 ? -m : m;
     const double sg = (am & 1) ? -1.0 : 1.0;
-    for (int k = wave; k < nshell; k += 4) {
+    for (int k = 4 * blockIdx.y + wave; k < nshell && k < 4 * blockIdx.y + 4; k += 4) {
         if (li == 0 && lane == 0) prop[(int64_t)k * n_c + coff_l[0]] = cur[(int64_t)k * n_c + coff_l[0]];
         const double2 *x = cur + (int64_t)k * n_c + off;
         double2 *xo = prop + (int64_t)k * n_c + off;
@@ -1139,7 +1141,8 @@ __global__ void k_intensity_coeff(const double2 *__restrict__ G, const double *_
 // with r = i + N j, so per k-triple column
 //   stage A: C[m2][j] = sum_{mu3} cos(mu3 b_j) G[(m2,mu3)],  S[m2][j] = sum_{mu3} sin(mu3 b_j) G[(m2,mu3)]
 //   stage B: c3[i][j] = sum_{m2} cos(m2 a_i) C[m2][j] - sin(m2 a_i) S[m2][j]
-// (2.136.2N + 2.2 n_m2.N.2N flop per column instead of 2.R.136).  Workgroup
+// (8 n_m N + 4 n_m2 R flop per column instead of 2 R n_m; L = 18, N = 32:
+// 1.17e5 instead of 5.57e5).  Workgroup
 // = (16 columns, 16 values of j); classes of one m2 are contiguous
 // (m2start).  Tables: TA[cs][class][j] = cos / sin(mu3_class b_j) (ld jpad),
 // TB[i][2 m2i + cs] = cos(m2 a_i) / -sin(m2 a_i) (ld kb).  Energy partial
