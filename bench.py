@@ -164,7 +164,8 @@ def precision_study(n):
     tp.HIP_init()
     truth, basis, c3ref, _ = setup(cfg, tp, workload)
     out = {"metric": "mixed-precision c3 / energy error vs FP64", "orientations": n, "config": "full (L=18,K=26,N=32)"}
-    acc = {m: {"slice_max": [], "rms": [], "norm2": [], "dE_abs": [], "dE_rel": []} for m in ("fp16x3", "bf16")}
+    modes = ("fp16x3", "triple16", "bf16")
+    acc = {m: {"slice_max": [], "rms": [], "norm2": [], "dE_abs": [], "dE_rel": []} for m in modes}
     for seed in range(n):
         st = workload.random_start_structure(truth, cfg["L"], seed)
         dev = tp.DeviceState(st, basis)
@@ -172,7 +173,7 @@ def precision_study(n):
         c64 = dev.c3()
         E64 = dev.energy(c3ref)
         scale = np.abs(c64).max(axis=0)
-        for m in ("fp16x3", "bf16"):
+        for m in modes:
             basis.set_precision(m)
             c = dev.c3()
             E = dev.energy(c3ref)
@@ -184,7 +185,7 @@ def precision_study(n):
             a["dE_abs"].append(abs(E - E64))
             a["dE_rel"].append(abs(E - E64) / abs(E64))
         basis.set_precision("fp64")
-        log(f"orientation {seed}: fp16x3 {acc['fp16x3']['slice_max'][-1]:.2e}  bf16 {acc['bf16']['slice_max'][-1]:.2e}")
+        log(f"orientation {seed}: " + "  ".join(f"{m} {acc[m]['slice_max'][-1]:.2e}" for m in modes))
     for m, a in acc.items():
         out[m] = {"c3_slice_rel_err_max": max(a["slice_max"]), "c3_slice_rel_err_rms": float(np.sqrt(np.mean(np.square(a["rms"])))),
                   "c3_norm2_rel_err_max": max(a["norm2"]), "energy_abs_err_max": max(a["dE_abs"]),
@@ -452,7 +453,7 @@ def main():
         "mc_steps_per_s": steps_per_s,
         "mc_anneal": {"steps": args.anneal_steps, "temperature_probes": 50, "seconds": t_anneal,
                       "E0": res["E0"], "E_final": res["E_final"], "accepted": int(res["accepted"].sum())},
-        "kernel_ms": {"triple": kern_ms[0], "gemm_G": kern_ms[1], "gemm_c3_energy": kern_ms[2],
+        "kernel_ms": {"triple": kern_ms[0], "gemm_G": kern_ms[1], "c3sep_energy": kern_ms[2],
                       "final": kern_ms[3]},
         "mixed_precision": {
             "mode": "fp16x3: c3 = F.As on FP16 MFMA, hi.hi + (hi.lo + lo.hi), FP32 accumulation, FP64 energy",
