@@ -169,6 +169,18 @@ int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, cons
 int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_ref *const *refs,
                     const double *const *coeff, const int64_t *ld, double *E_out);
 
+/* The Bayes energy (FP64, as tp_energy with measure 0) and its gradient with
+ * respect to the coefficients of the first K shells:
+ *   grad[2 (k ld + s)] + i grad[2 (k ld + s) + 1] = dE/dRe c_k[s] + i dE/dIm c_k[s],
+ * every entry treated as an independent complex variable (entries outside
+ * even l <= L are zero).  grad_out has K x ld complex entries, the layout of
+ * coeff.  No counterpart in the reference, which evaluates energy()
+ * (correlations.jl:201-226) only; it is that function's chain rule through
+ * the contraction and the triple products (:165-193).  Needs N <= 32, L <= 22
+ * and K <= 27 (TP_E_UNSUPPORTED otherwise). */
+int tp_energy_grad(tp_ctx *ctx, const tp_basis *basis, const tp_ref *ref, const double *coeff, int64_t ld,
+                   double *E_out, double *grad_out);
+
 #define TP_MOVE_QUAT 0    /* random slots: an SO(3) rotation per even l (uniform axis, Haar angle) */
 #define TP_MOVE_PLANE 1   /* random slots: the reference's move, a rotation in the plane of a
                            * Haar-random 2-frame of R^(2l+1) (default) */

@@ -55,6 +55,8 @@ def lib():
         L.or_c3_factored.argtypes = [C.c_void_p, d_p, C.c_int64, d_p, C.c_int]
         L.or_c3_slice.argtypes = [d_p, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_double, C.c_double, d_p]
+        L.or_energy_grad.restype = C.c_double
+        L.or_energy_grad.argtypes = [C.c_void_p, d_p, C.c_int64, d_p, C.c_int, C.c_int, C.c_int, d_p, C.c_int]
         L.or_energy.restype = C.c_double
         L.or_energy.argtypes = [d_p, C.c_int, C.c_int64, d_p, C.c_int, C.c_int, C.c_int]
         L.or_umat.argtypes = [C.c_int, d_p]
@@ -159,6 +161,17 @@ class Basis:
         if lib().or_c3_factored(self.h, coeff, coeff.shape[1] // 2, out, nthreads) != 0:
             raise ValueError("factored form needs lambda = 0")
         return out.reshape(self.klength, self.R).T
+
+    def energy_grad(self, coeff, c3ref, k3lo=1, k3hi=None, nthreads=0):
+        """(E, gamma): the Bayes energy and gamma_k[i] = dE/dRe c + i dE/dIm c
+        for the first K shells (complex, shaped like coeff[:K])."""
+        c = _coeff_arr(np.asarray(coeff)[:self.K])
+        cref = np.ascontiguousarray(np.asarray(c3ref, dtype=np.float64).ravel(order="F"))
+        Kr = np.asarray(c3ref).shape[2]
+        g = np.zeros_like(c)
+        E = lib().or_energy_grad(self.h, c, c.shape[1] // 2, cref, Kr, k3lo, self.K if k3hi is None else k3hi, g,
+                                 nthreads)
+        return E, g.view(np.complex128)
 
 
 def _coeff_arr(coeff):

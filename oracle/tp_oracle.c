@@ -562,6 +562,92 @@ double or_energy(const double *c3raw, int N, int64_t ncols, const double *c3ref,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Gradient of the Bayes energy with respect to the coefficients (no         */
+/* counterpart in the reference, which only evaluates the energy; it is the  */
+/* chain rule through energy() (correlations.jl:201-226), the factored        */
+/* contraction and the triple products (:165-174)):                           */
+/*   D[r,i]  = dE/dc3raw[r,i] = tF (sum_r' qhat_r' / S - qhat_r / p_r)        */
+/*             for c3raw[r,i] > 1e-30 (0 where the clamp is active),          */
+/*             qhat = q / sum|q|, S = sum_r p_r;                              */
+/*   gAs(kt,lt) = sum_r F[r,lt] D[r,kt];                                      */
+/*   gamma_k[i] = dE/dRe c_k[i] + i dE/dIm c_k[i] accumulates, for every     */
+/*   entry j of lt and kt = (k1,k2,k3), gAs w_j conj(product of the other two */
+/*   factors) in each of the three slots (d Re(abc) = Re(conj(conj(bc)) da)). */
+/* Every coefficient entry is treated as an independent complex variable.    */
+/* grad: K shells x ld complex (interleaved), zeroed here.  Returns E.       */
+/* ------------------------------------------------------------------------ */
+double or_energy_grad(const or_basis *b, const double *coeff, int64_t ld, const double *c3ref, int Kr, int k3lo,
+                      int k3hi, double *grad, int nthreads)
+{
+    if (b->lambda != 0.0) return NAN;
+    const int64_t R = b->R, nlt = b->n_lt, KL = b->klength;
+    double *c3 = (double *)malloc(sizeof(double) * R * KL);
+    double *F = (double *)malloc(sizeof(double) * R * nlt);
+    double *D = (double *)calloc((size_t)R * KL, sizeof(double));
+    double *gA = (double *)malloc(sizeof(double) * KL * nlt);
+    or_c3_factored(b, coeff, ld, c3, nthreads);
+    or_factor_F(b, F);
+    const double E = or_energy(c3, b->N, KL, c3ref, Kr, k3lo, k3hi);
+    /* D along energy()'s column walk */
+    int64_t i = 0;
+    for (int k1 = k3lo; k1 <= k3hi; ++k1)
+        for (int k2 = k3lo; k2 <= k1; ++k2)
+            for (int k3 = k3lo; k3 <= k2; ++k3, ++i) {
+                const double *pc = c3 + i * R;
+                const double *q = c3ref + (int64_t)R * ((k3 - 1) + (int64_t)Kr * ((k2 - 1) + (int64_t)Kr * (k1 - 1)));
+                double sp = 0.0, sq = 0.0, qs = 0.0;
+                for (int64_t r = 0; r < R; ++r) {
+                    sp += pc[r] > 1e-30 ? pc[r] : 1e-30;
+                    sq += fabs(q[r]);
+                }
+                for (int64_t r = 0; r < R; ++r) qs += q[r] / sq;
+                const double tf = or_tripletFactor(k1, k2, k3);
+                for (int64_t r = 0; r < R; ++r)
+                    D[i * R + r] = pc[r] > 1e-30 ? tf * (qs / sp - (q[r] / sq) / pc[r]) : 0.0;
+            }
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(static)
+    for (int64_t kt = 0; kt < KL; ++kt)
+        for (int64_t t = 0; t < nlt; ++t) {
+            double s = 0.0;
+            for (int64_t r = 0; r < R; ++r) s += F[t * R + r] * D[kt * R + r];
+            gA[kt * nlt + t] = s;
+        }
+    memset(grad, 0, sizeof(double) * 2 * (size_t)b->K * ld);
+    for (int64_t kt = 0; kt < KL; ++kt) {
+        const int64_t *k = b->kt_k + 3 * kt;
+        const double *c1 = coeff + 2 * (k[0] - 1) * ld, *c2 = coeff + 2 * (k[1] - 1) * ld,
+                     *c3c = coeff + 2 * (k[2] - 1) * ld;
+        double *g1 = grad + 2 * (k[0] - 1) * ld, *g2 = grad + 2 * (k[1] - 1) * ld, *g3 = grad + 2 * (k[2] - 1) * ld;
+        for (int64_t t = 0; t < nlt; ++t) {
+            const double ga = gA[kt * nlt + t];
+            if (ga == 0.0) continue;
+            const int64_t j0 = b->lt[5 * t + 4] - 1, mc = b->lt[5 * t + 3];
+            for (int64_t n = 0; n < mc; ++n) {
+                const int64_t j = j0 + n;
+                const int64_t *d = b->ind + 9 * j;
+                const int64_t a1 = 2 * (d[0] - 1), a2 = 2 * (d[1] - 1), a3 = 2 * (d[2] - 1);
+                const double ar = c1[a1], ai = c1[a1 + 1], br = c2[a2], bi = c2[a2 + 1], cr = c3c[a3], ci = c3c[a3 + 1];
+                const double g = ga * b->w[j];
+                /* slot 1: conj(b c) */
+                g1[a1] += g * (br * cr - bi * ci);
+                g1[a1 + 1] -= g * (br * ci + bi * cr);
+                /* slot 2: conj(a c) */
+                g2[a2] += g * (ar * cr - ai * ci);
+                g2[a2 + 1] -= g * (ar * ci + ai * cr);
+                /* slot 3: conj(a b) */
+                g3[a3] += g * (ar * br - ai * bi);
+                g3[a3 + 1] -= g * (ar * bi + ai * br);
+            }
+        }
+    }
+    free(c3); free(F); free(D); free(gA);
+    return E;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Umat: spherical_harmonics.jl:445-457.  out: (2l+1)^2 complex, row-major   */
 /* [u+l][m+l], interleaved re/im.                                             */
 /* ------------------------------------------------------------------------ */

@@ -432,3 +432,81 @@ def test_anneal_oracle_semantics(oracle):
     # the final structure's energy is the final E
     c3 = b.c3_factored(r["coeff"])
     assert oracle.energy(c3, N, ref, K, 1, K) == pytest.approx(r["E_final"], rel=1e-13)
+
+
+# ---------------------------------------------------------------- energy gradient
+def _grad_case(oracle, seed=5):
+    L, LMAX, N, K = 6, 8, 4, 3
+    c = random_real_coeff(K, L, LMAX, seed, c00=6.0)
+    b = oracle.Basis(L, LMAX, N, K)
+    other = random_real_coeff(K, L, LMAX, seed + 1, c00=6.0)
+    p = np.maximum(b.c3_factored(other), 1e-30)
+    ref = c3ref_from_columns(p / p.sum(), N, K)
+    return b, c, ref, N, K
+
+
+def test_energy_grad_directional_fd(oracle):
+    """Re sum conj(gamma) . delta equals the central difference of the energy
+    along delta (every coefficient entry an independent complex variable)."""
+    b, c, ref, N, K = _grad_case(oracle)
+    E, g = b.energy_grad(c, ref)
+    assert E == pytest.approx(_energy(oracle, b.c3_factored(c), N, ref, K), rel=1e-14)
+    rng = np.random.default_rng(9)
+    for _ in range(3):
+        d = rng.normal(size=c.shape) + 1j * rng.normal(size=c.shape)
+        h = 1e-5
+        Ep = _energy(oracle, b.c3_factored(c + h * d), N, ref, K)
+        Em = _energy(oracle, b.c3_factored(c - h * d), N, ref, K)
+        fd = (Ep - Em) / (2 * h)
+        an = float(np.real(np.vdot(g, d)))
+        assert an == pytest.approx(fd, rel=1e-6, abs=1e-9)
+
+
+def test_energy_grad_single_entries(oracle):
+    """Real and imaginary parts of single entries of each slot's l (the gradient
+    is not symmetric in c_{l,m} and c_{l,-m}: both enter the triple products)."""
+    b, c, ref, N, K = _grad_case(oracle, seed=11)
+    _, g = b.energy_grad(c, ref)
+    h = 1e-5
+    for (k, l, m) in [(0, 2, 1), (1, 4, -3), (2, 6, 6), (2, 0, 0)]:
+        i = seanindex(l, m, 8) - 1
+        for unit in (1.0, 1j):
+            d = np.zeros_like(c)
+            d[k, i] = unit
+            fd = (_energy(oracle, b.c3_factored(c + h * d), N, ref, K) -
+                  _energy(oracle, b.c3_factored(c - h * d), N, ref, K)) / (2 * h)
+            an = g[k, i].real if unit == 1.0 else g[k, i].imag
+            assert an == pytest.approx(fd, rel=1e-5, abs=1e-8)
+
+
+def test_energy_grad_zero_at_gibbs_minimum(oracle):
+    """With the histogram equal to the structure's own normalised c3 the energy
+    is at its minimum over positive c3, so the gradient vanishes."""
+    L, LMAX, N, K = 6, 8, 4, 3
+    c = random_real_coeff(K, L, LMAX, 2, c00=6.0)
+    b = oracle.Basis(L, LMAX, N, K)
+    p = np.maximum(b.c3_factored(c), 1e-30)
+    ref = c3ref_from_columns(p / p.sum(), N, K)
+    _, g = b.energy_grad(c, ref)
+    _, g2 = b.energy_grad(c, _grad_case(oracle)[2])
+    assert np.abs(g).max() <= 1e-12 * np.abs(g2).max()
+
+
+def test_energy_grad_orthogonal_to_so3(oracle):
+    """A common SO(3) rotation of all shells leaves c3 and E unchanged, so the
+    gradient has no component along it."""
+    b, c, ref, N, K = _grad_case(oracle, seed=21)
+    _, g = b.energy_grad(c, ref)
+    eps = 1e-5
+
+    def rotated(e):
+        rc = c.copy()
+        for l in range(0, 7, 2):
+            D = oracle.rotation_matrix(0.3 * e, 0.7 * e, -0.2 * e, l)
+            idx = [seanindex(l, m, 8) - 1 for m in range(-l, l + 1)]
+            rc[:, idx] = c[:, idx] @ D.T
+        return rc
+
+    d = (rotated(eps) - rotated(-eps)) / (2 * eps)   # the generator applied to c, to O(eps^2)
+    assert np.linalg.norm(d) > 1.0
+    assert abs(np.real(np.vdot(g, d))) <= 1e-8 * np.linalg.norm(g) * np.linalg.norm(d)

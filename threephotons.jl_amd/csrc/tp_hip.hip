@@ -18,6 +18,7 @@
 #include "tp_triple16.hpp"
 #include "tp_batch.hpp"
 #include "tp_mlp.hpp"
+#include "tp_grad.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -118,6 +119,7 @@ struct tp_basis {
     double *d_Bm = nullptr;        // [nm_pad16][R_pad]     Bm[class][r]
     // separated grid (k_c3sep): classes of one m2 contiguous
     int n_m2 = 0, jpad = 0, sep_kb = 0, njc = 0;
+    bool use_sep = false;          // N <= 32, 2(L/2+1) <= 24 and <= 24 classes per m2
     int *d_m2start = nullptr;      // [n_m2 + 1]
     double *d_TA = nullptr;        // [2][n_m][jpad]  cos / sin(mu3 b_j)
     double *d_TB = nullptr;        // [round_up(N,16)][sep_kb]  cos(m2 a_i), -sin(m2 a_i)
@@ -140,6 +142,19 @@ struct tp_basis {
     double2 *d_raw = nullptr;      // [K][LMAX^2] staging of host coefficients
     double2 *d_coef = nullptr;     // [K][n_c] compact coefficients of host-input calls
     double *d_c3 = nullptr;        // [klength][R] c3raw output staging (lazy)
+    // energy gradient (tp_energy_grad), built on first use
+    std::vector<int4> h_ltinfo;    // host copy of d_ltinfo
+    bool grad_ready = false;
+    int nlt_pad64 = 0, gLw = 0, gEmax = 0;
+    size_t bwd_lds = 0;
+    double *d_HT4 = nullptr;       // [njc * nm_pad64][nlt_pad64]  H^T stacked njc times
+    double *d_gGp = nullptr;       // [njc * nm_pad64][kl_pad]     dE/dG per block of 16 values of j
+    double *d_gA = nullptr;        // [nlt_pad64][kl_pad]          dE/dAs
+    double2 *d_gpart = nullptr;    // [n_lt][3][K][gLw]            per (l-triple, slot) gradient blocks
+    double2 *d_grad = nullptr;     // [K][n_c]                     dE/dRe c + i dE/dIm c
+    int2 *d_lofc = nullptr;        // [n_c] {l, m + l}
+    int *d_lstart = nullptr;       // [L/2 + 2]
+    int *d_llist = nullptr;        // [3 n_lt] lt * 3 + slot, grouped by the slot's l
 };
 
 struct tp_ref {
@@ -197,12 +212,13 @@ static int upload_coeff(tp_ctx *ctx, tp_basis *b, const double *coeff, int64_t l
 // With r == nullptr the raw c3 goes to c3out; otherwise the energy is formed
 // and k_final applies `mc`.
 static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const double2 *b0, const double2 *b1,
-                        const int *sel, int flip, double *c3out, double *Eout, const McArgs &mc)
+                        const int *sel, int flip, double *c3out, double *Eout, const McArgs &mc, bool force64 = false)
 {
     const bool timing = ctx->timing;
     hipStream_t st = ctx->stream;
+    const int prec = force64 ? TP_PREC_FP64 : b->prec;
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    if (b->use_mfma && b->prec != TP_PREC_FP64) {
+    if (b->use_mfma && prec != TP_PREC_FP64) {
         // mixed precision: compensated FP16 triple products
         hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)b->K), dim3(256), 0, st, b0, b1, sel, flip, b->n_c, b->L,
                            b->d_shsc);
@@ -222,12 +238,12 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     if (timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     const int ncols = r ? (int)r->ncols : (int)b->klength;
     int n_rt_fin = b->n_rt;
-    if (b->prec == TP_PREC_FP16X3 || b->prec == TP_PREC_BF16) {
+    if (prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16) {
         // mixed precision: c3 = F . As with 16-bit MFMA operands
         const double *qh = r ? r->d_qhat : nullptr;
         const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
         const dim3 gs((unsigned)(b->kl_pad / 64), (unsigned)(b->ldk / 64));
-        if (b->prec == TP_PREC_FP16X3) {
+        if (prec == TP_PREC_FP16X3) {
             hipLaunchKernelGGL(k_split_as<1>, gs, dim3(256), 0, st, b->d_As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
                                sel, flip, b->n_c, b->K, b->L, (int)b->klength, b->d_Ahi, b->d_Alo, b->d_Ainv);
         } else {
@@ -236,12 +252,12 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         }
         TP_HIP(hipGetLastError());
         if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
-        const uint16_t *Fh = b->prec == TP_PREC_FP16X3 ? b->d_Fhi : b->d_Fbf;
+        const uint16_t *Fh = prec == TP_PREC_FP16X3 ? b->d_Fhi : b->d_Fbf;
 #define TP_G16(MODE, OUT, EN, OUTP)                                                                                \
     hipLaunchKernelGGL((k_gemm16<H_BM, H_BN, MODE, OUT, EN, 0>), grid16, dim3(256), 0, st, Fh, b->d_Flo,          \
                        (int64_t)b->ldk, b->d_Ahi, b->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, b->d_Ainv, (int)b->R,  \
                        ncols, b->n_rt16, b->n_ct16, OUTP, b->R, qh, b->d_part, b->kl_pad, nullptr, nullptr)
-        if (b->prec == TP_PREC_FP16X3) {
+        if (prec == TP_PREC_FP16X3) {
             if (r && c3out) TP_G16(1, 1, true, c3out);
             else if (r) TP_G16(1, 0, true, nullptr);
             else TP_G16(1, 1, false, c3out);
@@ -260,8 +276,9 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
                        (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
-    if (b->N > 32) {
-        // c3 = Bm . G as a GEMM (k_c3sep keeps at most 32 values of alpha per workgroup)
+    if (!b->use_sep) {
+        // c3 = Bm . G as a GEMM (k_c3sep keeps at most 32 values of alpha per
+        // workgroup and at most 24 classes per m2 and 24 rows of (m2, cos/sin))
         const dim3 grid((unsigned)(b->n_rt * b->n_ct));
         if (r && c3out)
             hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
@@ -282,7 +299,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     const dim3 grid((unsigned)((ncols + 15) / 16), (unsigned)b->njc);
     const int64_t tap = (int64_t)b->n_m * b->jpad;
 #define TP_SEP(OUT, EN, OUTP, QH)                                                                                   \
-    hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), 0, st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
+    hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), c3sep_lds_bytes(b->sep_kb), st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
                        tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, b->d_part, b->kl_pad, OUTP, b->R,       \
                        b->d_logtab)
     if (r && c3out) TP_SEP(1, true, c3out, r->d_qhat);
@@ -1053,7 +1070,8 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_blk,
                     b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
-                    b->d_m2start, b->d_TA, b->d_TB, b->d_logtab};
+                    b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
+                    b->d_grad, b->d_lofc, b->d_lstart, b->d_llist};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1368,6 +1386,9 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     for (int c = b->n_m - 1; c >= 0; --c) m2start[cls_m2[c] / 2] = c;
     for (int i = b->n_m2 - 1; i >= 0; --i)
         if (m2start[i] > m2start[i + 1]) m2start[i] = m2start[i + 1];
+    int max_cls = 0;
+    for (int i = 0; i < b->n_m2; ++i) max_cls = std::max(max_cls, m2start[i + 1] - m2start[i]);
+    b->use_sep = N <= 32 && b->sep_kb <= kSepMaxKB && max_cls <= 24;
     std::vector<double> TA((size_t)2 * b->n_m * b->jpad, 0.0);
     for (int c = 0; c < b->n_m; ++c)
         for (int j = 0; j < 2 * N; ++j) {
@@ -1405,6 +1426,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         tp_basis_destroy(b);
         return rc;
     }
+    b->h_ltinfo = ltinfo;
     if ((rc = dev_alloc(&b->d_As, (size_t)b->nlt_pad * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_G, (size_t)b->nm_pad64 * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_part, (size_t)std::max(b->n_rt, b->njc) * 2 * b->kl_pad)) ||
@@ -1586,6 +1608,121 @@ int tp_energy(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *co
     TP_HIP(hipStreamSynchronize(ctx->stream));
     *E_out = ctx->h_pin[0];
     record_timing(ctx);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Energy gradient (no counterpart in the reference; see tp_grad.hpp)
+// ---------------------------------------------------------------------------
+static int grad_setup(tp_ctx *ctx, tp_basis *b)
+{
+    if (b->grad_ready) return 0;
+    if (!b->use_sep || !b->use_mfma)
+        return tp_fail(TP_E_UNSUPPORTED, "the energy gradient needs N <= 32, L <= 22 and K <= 27");
+    b->gLw = 2 * b->L + 1;
+    b->gEmax = b->EmaxLt;
+    b->bwd_lds = triple_bwd_lds_bytes(b->K, b->gLw, b->klength, b->gEmax);
+    int maxlds = 0;
+    if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
+        maxlds = 65536;
+    if (b->bwd_lds > (size_t)maxlds)
+        return tp_fail(TP_E_UNSUPPORTED, "the energy gradient's triple-product workgroup does not fit the LDS");
+    if (b->bwd_lds > 65536)
+        TP_HIP(hipFuncSetAttribute((const void *)k_triple_bwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)b->bwd_lds));
+    b->nlt_pad64 = (int)round_up(b->n_lt, 64);
+    const int nlt = (int)b->n_lt, nmp = (int)b->nm_pad64, njc = b->njc, K = b->K;
+    std::vector<double> HT4((size_t)njc * nmp * b->nlt_pad64, 0.0);
+    for (int jc = 0; jc < njc; ++jc)
+        for (int c = 0; c < nmp; ++c)
+            for (int t = 0; t < nlt; ++t)
+                HT4[((size_t)jc * nmp + c) * b->nlt_pad64 + t] = b->h_H[(size_t)t * b->nm_pad64 + c];
+    // reduction lists: for each even l, the (l-triple, slot) blocks whose slot has order l
+    const int nl = b->L / 2 + 1;
+    std::vector<int> lstart(nl + 1, 0), llist;
+    for (int li = 0; li < nl; ++li) {
+        lstart[li] = (int)llist.size();
+        for (int t = 0; t < nlt; ++t) {
+            const int4 q = b->h_ltinfo[t];
+            const int ls[3] = {q.w & 0xff, (q.w >> 8) & 0xff, q.z};
+            for (int sl = 0; sl < 3; ++sl)
+                if (ls[sl] == 2 * li) llist.push_back(t * 3 + sl);
+        }
+    }
+    lstart[nl] = (int)llist.size();
+    std::vector<int2> lofc(b->n_c);
+    for (int l = 0; l <= b->L; l += 2)
+        for (int m = 0; m <= 2 * l; ++m) lofc[b->coff[l] + m] = make_int2(l, m);
+    int rc = 0;
+    if ((rc = dev_alloc(&b->d_HT4, HT4.size())) || (rc = dev_alloc(&b->d_gGp, (size_t)njc * nmp * b->kl_pad)) ||
+        (rc = dev_alloc(&b->d_gA, (size_t)b->nlt_pad64 * b->kl_pad)) ||
+        (rc = dev_alloc(&b->d_gpart, (size_t)nlt * 3 * K * b->gLw)) || (rc = dev_alloc(&b->d_grad, (size_t)K * b->n_c)) ||
+        (rc = dev_alloc(&b->d_lofc, lofc.size())) || (rc = dev_alloc(&b->d_lstart, lstart.size())) ||
+        (rc = dev_alloc(&b->d_llist, llist.size())))
+        return rc;
+    TP_HIP(hipMemcpy(b->d_HT4, HT4.data(), sizeof(double) * HT4.size(), hipMemcpyHostToDevice));
+    TP_HIP(hipMemset(b->d_gGp, 0, sizeof(double) * (size_t)njc * nmp * b->kl_pad));   // rows / columns never written stay 0
+    TP_HIP(hipMemcpy(b->d_lofc, lofc.data(), sizeof(int2) * lofc.size(), hipMemcpyHostToDevice));
+    TP_HIP(hipMemcpy(b->d_lstart, lstart.data(), sizeof(int) * lstart.size(), hipMemcpyHostToDevice));
+    TP_HIP(hipMemcpy(b->d_llist, llist.data(), sizeof(int) * llist.size(), hipMemcpyHostToDevice));
+    b->grad_ready = true;
+    return 0;
+}
+
+// dE/dc of the evaluation just enqueued (FP64, with d_G and the energy
+// partials of k_c3sep still in place) into b->d_grad.
+static int enqueue_grad(tp_ctx *ctx, tp_basis *b, const tp_ref *r, const double2 *coef)
+{
+    hipStream_t st = ctx->stream;
+    const int ncols = (int)r->ncols;
+    const int64_t tap = (int64_t)b->n_m * b->jpad;
+    hipLaunchKernelGGL(k_c3sep_bwd, dim3((unsigned)((ncols + 15) / 16), (unsigned)b->njc), dim3(512),
+                       c3sep_lds_bytes(b->sep_kb), st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad,
+                       b->d_TB, b->sep_kb, b->N, ncols, r->d_qhat, r->d_qsum, r->d_tf, b->d_part, b->kl_pad, b->njc,
+                       b->n_m, b->d_gGp, b->kl_pad, (int)b->nm_pad64);
+    TP_HIP(hipGetLastError());
+    const int n_rt = b->nlt_pad64 / G1_BM, n_ct = (int)(b->kl_pad / G1_BN);
+    hipLaunchKernelGGL((k_gemm<G1_BM, G1_BN, G1_WGM, 2, false>), dim3((unsigned)(n_rt * n_ct)), dim3(256), 0, st,
+                       b->d_HT4, (int64_t)b->nlt_pad64, b->d_gGp, b->kl_pad, (int)(b->njc * b->nm_pad64),
+                       b->nlt_pad64, (int)b->kl_pad, n_rt, n_ct, b->d_gA, b->kl_pad, nullptr, nullptr, (int64_t)0,
+                       nullptr);
+    TP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_triple_bwd, dim3((unsigned)b->n_lt), dim3(1024), b->bwd_lds, st, coef, b->n_c, b->K, b->d_ew,
+                       b->d_eidx, b->d_gstart, b->d_ltinfo, b->gLw, b->gEmax, b->d_gA, b->kl_pad, (int)b->klength,
+                       b->d_gpart);
+    TP_HIP(hipGetLastError());
+    const int n = b->K * b->n_c;
+    hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b->d_gpart, b->K, b->gLw,
+                       b->n_c, b->d_lofc, b->d_lstart, b->d_llist, b->d_grad);
+    TP_HIP(hipGetLastError());
+    return 0;
+}
+
+int tp_energy_grad(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *coeff, int64_t ld, double *E_out,
+                   double *grad_out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!bc || !r || !E_out || !grad_out || !coeff) return tp_fail(TP_E_ARG, "null argument");
+    tp_basis *b = const_cast<tp_basis *>(bc);
+    if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
+    if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "coefficient vector shorter than LMAX^2");
+    if (int rc = grad_setup(ctx, b)) return rc;
+    if (int rc = upload_coeff(ctx, b, coeff, ld, b->K, b->d_coef, b->d_raw)) return rc;
+    if (int rc = enqueue_eval(ctx, b, r, b->d_coef, nullptr, nullptr, 0, nullptr, b->d_out, mc_none(), true))
+        return rc;
+    if (int rc = enqueue_grad(ctx, b, r, b->d_coef)) return rc;
+    std::vector<double2> g((size_t)b->K * b->n_c);
+    TP_HIP(hipMemcpyAsync(ctx->h_pin, b->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipMemcpyAsync(g.data(), b->d_grad, sizeof(double2) * g.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    *E_out = ctx->h_pin[0];
+    std::memset(grad_out, 0, sizeof(double) * 2 * (size_t)b->K * ld);
+    for (int k = 0; k < b->K; ++k)
+        for (int c = 0; c < b->n_c; ++c) {
+            const size_t o = 2 * ((size_t)k * ld + b->sean0[c]);
+            grad_out[o] = g[(size_t)k * b->n_c + c].x;
+            grad_out[o + 1] = g[(size_t)k * b->n_c + c].y;
+        }
     return 0;
 }
 

@@ -1151,6 +1151,9 @@ __global__ void k_intensity_coeff(const double2 *__restrict__ G, const double *_
 // ---------------------------------------------------------------------------
 constexpr int kSepMaxKB = 24;   // 2 (L/2 + 1) padded to 4, L <= 22
 
+// Dynamic LDS of k_c3sep: CS[kb][16][17] doubles.
+inline size_t c3sep_lds_bytes(int kb) { return sizeof(double) * (size_t)kb * 16 * 17; }
+
 template <int OUT, bool ENERGY>
 __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
                                                int n_m2, const double *__restrict__ TA, int64_t ta_plane, int jpad,
@@ -1160,35 +1163,26 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
                                                const double *__restrict__ logtab)
 {
     __shared__ double s_lt[ENERGY ? kLogTabSize : 1];
+    __shared__ double red[8][2][16];
+    extern __shared__ double cs_dyn[];
+    // CS[k][j local][col] = cs_dyn[(k * 16 + j) * 17 + col] (padded row)
+    auto CS = [&](int k, int r, int c) -> double & { return cs_dyn[(k * 16 + r) * 17 + c]; };
     if (ENERGY)   // ordered before the epilogue by the barrier after stage A
         for (int i = threadIdx.x; i < kLogTabSize; i += 512) s_lt[i] = logtab[i];
-    // 8 waves; every global operand a wave needs is requested before it is
-    // used (the kernel is latency-bound: one residency wave of workgroups)
+    // 8 waves.  The histogram values (HBM) are requested after stage A's
+    // operands: loads complete in order, so issuing them first would make
+    // stage A wait for them.  Three workgroups fit a CU (dynamic LDS sized kb).
     constexpr int NW = 8, MAXB = 4;   // stage-B blocks per wave: 16 j x (N/16) i-blocks / 8 waves (N <= 32)
-    __shared__ double CS[kSepMaxKB][16][17];   // [2 m2i + cs][j local][col], padded row
-    __shared__ double red[NW][2][16];
     const int c0 = blockIdx.x * 16, jc = blockIdx.y, j0 = jc * 16;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar table loads
     const int l16 = lane & 15, lq = lane >> 4;
     const int R = 2 * N * N, twoN = 2 * N;
     const int nib = (N + 15) / 16, nob = 16 * nib;
-    // histogram values of this wave's stage-B outputs (transposed blocks:
-    // lane -> column c0 + lq + 4v, row i = 16 ib + l16)
-    double qv[MAXB][4];
-#pragma unroll
-    for (int t = 0; t < MAXB; ++t) {
-        const int ob = wave + NW * t;
-        const int jl = ob / nib, ib = ob % nib, j = j0 + jl, i = 16 * ib + l16;
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int col = c0 + lq + 4 * v;
-            qv[t][v] = (ENERGY && ob < nob && j < twoN && i < N && col < ncols) ? qhat[(int64_t)col * R + i + N * j] : 0.0;
-        }
-    }
     // padding rows of CS (k' >= 2 n_m2) stay zero
     for (int i = tid; i < (kb - 2 * n_m2) * 256; i += 512) {
         const int k = 2 * n_m2 + i / 256, r = (i / 16) % 16, c = i % 16;
-        CS[k][r][c] = 0.0;
+        CS(k, r, c) = 0.0;
     }
     // ---- stage A: blocks (m2i, cs), rows j local, K = classes of m2 (<= 24)
     for (int b = wave; b < 2 * n_m2; b += NW) {
@@ -1208,7 +1202,37 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
         for (int q = 0; q < 6; ++q)
             if (cb + 4 * q < ce) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], gv[q], acc, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) CS[b][lq + 4 * v][l16] = acc[v];
+        for (int v = 0; v < 4; ++v) CS(b, lq + 4 * v, l16) = acc[v];
+    }
+    // stage-B operands: the i block of a wave's outputs is the same for all its
+    // blocks (NW is a multiple of nib), so one TB slice serves them all
+    // All loads below are unconditional from clamped (in-range) addresses,
+    // with out-of-range values zeroed afterwards: loads under a branch would
+    // make the first stage-B MFMA wait for every histogram load.
+    const int ibw = wave % nib;
+    double tb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int kk = 4 * q + lq;
+        tb[q] = TB[(int64_t)(16 * ibw + l16) * kb + min(kk, kb - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+        if (4 * q + lq >= kb) tb[q] = 0.0;
+    // histogram values of this wave's stage-B outputs (transposed blocks:
+    // lane -> column c0 + lq + 4v, row i = 16 ib + l16)
+    double qv[MAXB][4];
+    if (ENERGY) {
+#pragma unroll
+        for (int t = 0; t < MAXB; ++t) {
+            const int ob = wave + NW * t;
+            const int jl = ob / nib, ib = ob % nib, j = min(j0 + jl, twoN - 1), i = min(16 * ib + l16, N - 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int col = min(c0 + lq + 4 * v, ncols - 1);
+                qv[t][v] = qhat[(int64_t)col * R + i + N * j];
+            }
+        }
     }
     __syncthreads();
     // ---- stage B: blocks (j local, i block), K = 2 n_m2 (padded to kb), computed
@@ -1222,13 +1246,10 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
         const int jl = ob / nib, ib = ob % nib;
         const int j = j0 + jl;
         if (j >= twoN) continue;   // wave-uniform
-        double tb[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) tb[q] = 4 * q < kb ? TB[(int64_t)(16 * ib + l16) * kb + 4 * q + lq] : 0.0;
         dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int q = 0; q < 6; ++q)
-            if (4 * q < kb) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CS[4 * q + lq][jl][l16], tb[q], acc, 0, 0, 0);
+            if (4 * q < kb) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CS(4 * q + lq, jl, l16), tb[q], acc, 0, 0, 0);
         const int i = 16 * ib + l16;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {

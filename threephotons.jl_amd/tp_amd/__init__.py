@@ -312,6 +312,19 @@ def rotate(intensity, basis, quats, nshell=None):
     return out
 
 
+def energy_grad(intensity, basis, c3ref, K3_range=None):
+    """(E, gamma): the FP64 Bayes energy (as energy()) and its gradient
+    gamma[k, s] = dE/dRe c_k[s] + i dE/dIm c_k[s] for the first K shells
+    (tp_energy_grad; every coefficient entry an independent complex variable)."""
+    c, ld = _coeff_ptr(intensity, basis)
+    ref = basis.ref(c3ref, K3_range)
+    E = C.c_double()
+    g = np.zeros((basis.K, ld), np.complex128)
+    check(lib().tp_energy_grad(basis.ctx.h, basis.h, ref.value, c.ctypes.data, ld, C.byref(E), g.ctypes.data),
+          "tp_energy_grad")
+    return E.value, g
+
+
 def energy_batch(intensities, bases, c3refs, K3_ranges=None):
     """Bayes energies of many evaluations in one call (tp_energy_batch): one per
     (intensity, basis, c3ref); bases share L, LMAX, N and precision, K may differ."""
