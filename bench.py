@@ -244,6 +244,40 @@ def _time_to(Es, t_of, target):
     return None if idx.size == 0 else float(t_of(int(idx[0])))
 
 
+def gradient_leg(tp, basis, c3ref, start, plain, nsteps, ncand=16):
+    """Gradient-guided moves (tp_amd.descent) on the plain run's problem (K=26,
+    its start structure and histogram): per step one FP64 energy + gradient
+    (tp_energy_grad), a ladder of ncand step lengths along the Riemannian
+    steepest descent on every SO(2l+1) scored in one batched FP16 hi/lo
+    evaluation, the best accepted on its FP64 energy.  The candidate batch's
+    plan and graph are built before the timed region (as the batched runs'
+    graphs are)."""
+    from tp_amd import descent
+    basis.set_precision("fp16x3")
+    tp.energy_batch([start.coeff] * ncand, [basis] * ncand, [c3ref] * ncand)
+    tp.energy_grad(start.coeff, basis, c3ref)
+    E_plain, t_plain = plain["E_final"], plain["seconds"]
+    r = descent.gradient_anneal(start, basis, c3ref, nsteps, ncand=ncand)
+    h = r["history"]
+    ts, Es = np.array([t for t, _ in h]), np.array([e for _, e in h])
+    per = ts[-1] / max(1, len(h) - 1)
+    reach = _time_to(Es, lambda i: ts[i], E_plain)
+    within = _time_to(Es, lambda i: ts[i], E_plain * 1.001)
+    at_plain = Es[np.searchsorted(ts, t_plain, side="right") - 1]
+    out = {"mode": "FP64 energy + gradient (tp_energy_grad) per step; %d step lengths along exp(-eta Omega_l) "
+                   "scored in one batched FP16x3 evaluation; best accepted on its FP64 energy (T = 0)" % ncand,
+           "steps": len(h) - 1, "ms_per_step": 1e3 * per, "E0": float(Es[0]), "E_final": float(Es[-1]),
+           "E_trace": [float(e) for e in Es], "step_lengths": r["step_lengths"],
+           "t_reach_plain_final": reach, "t_within_0.1pct": within,
+           "E_at_plain_seconds": float(at_plain), "plain_seconds": t_plain, "plain_E_final": E_plain,
+           "speedup_reach_plain_final": (t_plain / reach) if reach else None}
+    log(f"gradient moves: {len(h) - 1} steps, {1e3 * per:.2f} ms/step, E {Es[0]:.4f} -> {Es[-1]:.4f}; "
+        f"reaches the plain run's final {E_plain:.4f} at {1e3 * reach if reach else float('nan'):.2f} ms "
+        f"(plain: {1e3 * t_plain:.1f} ms)")
+    basis.set_precision("fp64")
+    return out
+
+
 def learned_leg(cfg, tp, workload, truth, basis, c3ref, start, plain, data_sweeps, train_steps, cmp_sweeps,
                 tte_sweeps, nprop=64, nlearn=48, seeds=tuple(range(1000, 1020))):
     """Learned proposer: (1) move log from the concurrent runs' sweeps
@@ -342,6 +376,7 @@ def main():
     ap.add_argument("--learned-train-steps", type=int, default=2000)
     ap.add_argument("--learned-cmp-sweeps", type=int, default=40)
     ap.add_argument("--learned-tte-sweeps", type=int, default=200)
+    ap.add_argument("--grad-steps", type=int, default=30, help="steps of the gradient-guided moves (0: skip)")
     ap.add_argument("--precision-study", type=int, default=0,
                     help="instead of the benchmark: c3/energy errors of the mixed-precision modes against FP64 "
                          "for this many random orientations of the full-size structure")
@@ -426,9 +461,12 @@ def main():
     if args.batch_sweeps > 0:
         batch = batch_leg(cfg, tp, workload, truth, basis, c3ref, args.batch_sweeps)
         basis.set_precision("fp64")
+    plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
+    grad = None
+    if args.grad_steps > 0:
+        grad = gradient_leg(tp, basis, c3ref, start, plain, args.grad_steps)
     learned = None
     if args.learned_data_sweeps > 0:
-        plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
         learned = learned_leg(cfg, tp, workload, truth, basis, c3ref, start, plain, args.learned_data_sweeps,
                               args.learned_train_steps, args.learned_cmp_sweeps, args.learned_tte_sweeps)
 
@@ -478,6 +516,8 @@ def main():
         out["batched"] = batch
     if learned is not None:
         out["learned"] = learned
+    if grad is not None:
+        out["gradient_moves"] = grad
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)

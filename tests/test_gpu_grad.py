@@ -87,3 +87,32 @@ def test_energy_grad_rotation_fd(tp):
         an = float(np.real(np.vdot(g, d)))
         say(f"  l={l}: analytic {an:.10g}  fd {fd:.10g}")
         assert an == pytest.approx(fd, rel=1e-5, abs=1e-7 * abs(E))
+
+
+def test_gradient_moves_beat_random_walk(tp):
+    """Gradient-guided moves (tp_amd.descent) only ever lower the energy at
+    T = 0 and, for the same number of energy evaluations (a gradient counted
+    as three), end far lower than the reference's random-walk annealing."""
+    from tp_amd import descent, workload
+    L, LMAX, N, K = 8, 10, 8, 6
+    truth = random_real_coeff(K, L, LMAX, 31, c00=6.0, decay=0.05)
+    b = tp.HIP_calculate_basis(L, LMAX, N, K)
+    p = np.maximum(tp.FullCorrelation_parallized(truth, b, return_raw=True), 1e-30)
+    ref = c3ref_from_columns(p / p.sum(), N, K)
+    g = np.random.default_rng(7)
+    start = truth.copy()
+    for l in range(2, L + 1, 2):
+        start = workload.rotate_l(start, l, workload.haar_rotation(2 * l + 1, g), LMAX)
+    E0 = tp.energy(start, b, ref)
+    nsteps, ncand = 12, 8
+    r = descent.gradient_anneal(start, b, ref, nsteps, ncand=ncand)
+    Es = [e for _, e in r["history"]]
+    say(f"  gradient: {Es[0]:.6f} -> {Es[-1]:.6f} in {len(Es) - 1} steps, step lengths "
+        f"{np.round(r['step_lengths'], 3)}")
+    assert all(Es[i + 1] <= Es[i] for i in range(len(Es) - 1))
+    assert Es[0] == E0
+    evals = nsteps * (ncand + 3)
+    st = tp.DeviceState(tp.SphericalHarmonicsVolume(start, LMAX, K, float(K)), b)
+    plain = tp.rotate_all_at_once(st, b, ref, evals, np.pi / 4, 1.01, 0.1, 0.99998, seed=3, ntemp=10)
+    say(f"  random walk, {evals} steps: {plain['E0']:.6f} -> {plain['E_final']:.6f}")
+    assert E0 - Es[-1] > 2.0 * (E0 - plain["E_final"])

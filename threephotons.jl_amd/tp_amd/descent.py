@@ -1,0 +1,114 @@
+"""Gradient-guided annealing moves.
+
+The reference's move (get_MC_step, determination.jl:232-241) rotates every
+even-l block of all shells by an element of SO(2l+1) in the real basis, which
+leaves the two-photon invariants unchanged; it picks the rotation at random.
+Here the rotation follows the energy's steepest descent on that same group:
+with x_k = Umat(l) c_k,l (real) and dE/dx_k = Re(Umat(l) gamma_k,l) from the
+device gradient (tp_energy_grad), the Riemannian gradient on SO(2l+1) is the
+skew part Omega_l of sum_k (dE/dx_k) x_k^T, and a move is R_l = exp(-eta
+Omega_l) for every l at once.  Each step scores a ladder of step lengths eta
+in one batched evaluation (the basis's precision: FP16 hi/lo in the intended
+use), takes the best, and accepts it by Metropolis on its exact FP64 energy
+(evaluate_step, determination.jl:491-515, with T *= decay per step as in
+rotate_all_at_once :328).  The rotations are small dense matrices built on
+the host (as the reference builds its Haar rotations on the host); every c3,
+energy and gradient runs on the device.
+"""
+import time
+
+import numpy as np
+
+from . import energy_batch, energy_grad, seanindex
+from .workload import umat
+
+
+class GradientMoves:
+    """Per-l real-basis transforms and the rotation algebra of the moves."""
+
+    def __init__(self, basis):
+        self.basis = basis
+        self.ls = list(range(2, basis.L + 1, 2))
+        self.U = {l: umat(l) for l in self.ls}
+        self.idx = {l: np.array([seanindex(l, m, basis.LMAX) - 1 for m in range(-l, l + 1)]) for l in self.ls}
+
+    def riemannian(self, c, gamma):
+        """{l: Omega_l}, the skew-symmetric steepest-ascent direction of E on SO(2l+1)."""
+        K = gamma.shape[0]
+        out = {}
+        for l in self.ls:
+            U, ix = self.U[l], self.idx[l]
+            x = (c[:K, ix] @ U.T).real
+            gx = (gamma[:, ix] @ U.T).real
+            M = gx.T @ x
+            out[l] = 0.5 * (M - M.T)
+        return out
+
+    @staticmethod
+    def norm(om):
+        return float(np.sqrt(sum((A * A).sum() for A in om.values())))
+
+    def rotations(self, om, etas):
+        """[{l: exp(-eta Omega_l)} for eta in etas], through the eigenvectors of the Hermitian i Omega_l."""
+        out = [dict() for _ in etas]
+        for l, A in om.items():
+            w, V = np.linalg.eigh(1j * A)
+            Vh = V.conj().T
+            for s, eta in enumerate(etas):
+                out[s][l] = ((V * np.exp(1j * eta * w)) @ Vh).real
+        return out
+
+    def apply(self, c, rots):
+        """Coefficients with every shell's l-block rotated: x <- R x in the real basis."""
+        out = c.copy()
+        for l, R in rots.items():
+            U, ix = self.U[l], self.idx[l]
+            rot = U.conj().T @ R.astype(np.complex128) @ U
+            out[:, ix] = c[:, ix] @ rot.T
+        return out
+
+
+def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, seed=0, K3_range=None,
+                    target=None, time_limit=None):
+    """Anneal `coeff` (KMAX x LMAX^2 complex, all shells rotated) with gradient
+    moves.  Candidates are scored in the basis's current precision; the chosen
+    one is accepted on its FP64 energy.  Stops after nsteps, when E <= target,
+    or after time_limit seconds.  Returns a dict with the final coefficients and
+    energy, the (seconds, E) history (history[0] = start) and the step lengths."""
+    gm = GradientMoves(basis)
+    rng = np.random.default_rng(seed)
+    c = np.array(coeff.coeff if hasattr(coeff, "coeff") else coeff, dtype=np.complex128, copy=True)
+    t0 = time.perf_counter()
+    E, g = energy_grad(c, basis, c3ref, K3_range)
+    hist, steps, accepted = [(0.0, E)], [], 0
+    scale = None
+    ladder = 2.0 ** ((np.arange(ncand) - ncand // 2) / 2.0)
+    bases, refs, rngs = [basis] * ncand, [c3ref] * ncand, [K3_range] * ncand
+    T = T0
+    for step in range(nsteps):
+        om = gm.riemannian(c, g)
+        gn = gm.norm(om)
+        if gn == 0.0:
+            break
+        if scale is None:
+            scale = 1.0 / gn
+        etas = scale * ladder
+        cands = [gm.apply(c, R) for R in gm.rotations(om, etas)]
+        Ec = energy_batch(cands, bases, refs, rngs)
+        k = int(np.argmin(Ec))
+        En, gnew = energy_grad(cands[k], basis, c3ref, K3_range)
+        T *= decay
+        dE = En - E
+        acc = dE < 0.0 or (T > 0.0 and rng.uniform() < np.exp(-dE / T))
+        if acc:
+            c, E, g = cands[k], En, gnew
+            scale = etas[k]
+            accepted += 1
+        else:
+            scale = etas[k] / 4.0
+        steps.append(float(etas[k] * gn))
+        now = time.perf_counter() - t0
+        hist.append((now, E))
+        if (target is not None and E <= target) or (time_limit is not None and now >= time_limit):
+            break
+    return {"coeff": c, "E": E, "history": hist, "step_lengths": steps, "accepted": accepted}
