@@ -154,6 +154,10 @@ class BasisTypeHip:
              "triple16": 3}[mode] if isinstance(mode, str) else mode
         check(lib().tp_basis_set_precision(self.h, m), "tp_basis_set_precision")
 
+    def precision(self):
+        """The current precision mode (PREC_* code)."""
+        return int(lib().tp_basis_get_precision(self.h))
+
     def ref(self, c3ref, K3_range=None):
         """Device copy of the normalised histogram slices visited by energy()'s walk (cached)."""
         if K3_range is None:

@@ -19,7 +19,7 @@ import time
 
 import numpy as np
 
-from . import energy_batch, energy_grad, seanindex
+from . import PREC_FP64, energy, energy_batch, energy_grad, seanindex
 from .workload import umat
 
 
@@ -48,8 +48,24 @@ class GradientMoves:
     def norm(om):
         return float(np.sqrt(sum((A * A).sum() for A in om.values())))
 
+    def candidates(self, c, om, etas):
+        """The coefficients moved by exp(-eta Omega_l) on every l, one array per
+        eta.  With i Omega_l = V diag(w) V^H (Hermitian) and W = Umat(l)^H V, the
+        complex-basis rotation is W diag(exp(i eta w)) W^H; all etas of one l
+        are formed in one batched product."""
+        etas = np.asarray(etas, dtype=np.float64)
+        out = np.repeat(c[None, :, :], len(etas), axis=0)
+        for l, A in om.items():
+            U, ix = self.U[l], self.idx[l]
+            w, V = np.linalg.eigh(1j * A)
+            W = U.conj().T @ V
+            ph = np.exp(1j * etas[:, None] * w[None, :])             # (S, n)
+            rot = (W[None, :, :] * ph[:, None, :]) @ W.conj().T       # (S, n, n)
+            out[:, :, ix] = c[None, :, ix] @ np.swapaxes(rot, 1, 2)
+        return out
+
     def rotations(self, om, etas):
-        """[{l: exp(-eta Omega_l)} for eta in etas], through the eigenvectors of the Hermitian i Omega_l."""
+        """[{l: exp(-eta Omega_l)} for eta in etas] (real basis)."""
         out = [dict() for _ in etas]
         for l, A in om.items():
             w, V = np.linalg.eigh(1j * A)
@@ -68,13 +84,29 @@ class GradientMoves:
         return out
 
 
+def _energy64(c, basis, c3ref, K3_range):
+    """FP64 energy whatever the basis's precision (the acceptance energy)."""
+    prec = basis.precision()
+    if prec == PREC_FP64:
+        return energy(c, basis, c3ref, K3_range)
+    basis.set_precision(PREC_FP64)
+    try:
+        return energy(c, basis, c3ref, K3_range)
+    finally:
+        basis.set_precision(prec)
+
+
 def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, seed=0, K3_range=None,
                     target=None, time_limit=None):
     """Anneal `coeff` (KMAX x LMAX^2 complex, all shells rotated) with gradient
-    moves.  Candidates are scored in the basis's current precision; the chosen
-    one is accepted on its FP64 energy.  Stops after nsteps, when E <= target,
-    or after time_limit seconds.  Returns a dict with the final coefficients and
-    energy, the (seconds, E) history (history[0] = start) and the step lengths."""
+    moves.  Per step: the FP64 energy gradient of the current state (only after
+    it changed), ncand step lengths along exp(-eta Omega_l) scored in one
+    batched evaluation in the basis's current precision, and the best one
+    accepted by Metropolis on its FP64 energy.  The ladder of step lengths is
+    re-centred on the accepted one (a quarter of it after a rejection).  Stops
+    after nsteps, when E <= target, or after time_limit seconds.  Returns the
+    final coefficients and energy, the (seconds, E) history (history[0] =
+    start) and the step lengths eta |Omega|."""
     gm = GradientMoves(basis)
     rng = np.random.default_rng(seed)
     c = np.array(coeff.coeff if hasattr(coeff, "coeff") else coeff, dtype=np.complex128, copy=True)
@@ -86,6 +118,8 @@ def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, se
     bases, refs, rngs = [basis] * ncand, [c3ref] * ncand, [K3_range] * ncand
     T = T0
     for step in range(nsteps):
+        if g is None:
+            _, g = energy_grad(c, basis, c3ref, K3_range)
         om = gm.riemannian(c, g)
         gn = gm.norm(om)
         if gn == 0.0:
@@ -93,15 +127,15 @@ def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, se
         if scale is None:
             scale = 1.0 / gn
         etas = scale * ladder
-        cands = [gm.apply(c, R) for R in gm.rotations(om, etas)]
-        Ec = energy_batch(cands, bases, refs, rngs)
+        cands = gm.candidates(c, om, etas)
+        Ec = energy_batch(list(cands), bases, refs, rngs)
         k = int(np.argmin(Ec))
-        En, gnew = energy_grad(cands[k], basis, c3ref, K3_range)
+        En = _energy64(cands[k], basis, c3ref, K3_range)
         T *= decay
         dE = En - E
         acc = dE < 0.0 or (T > 0.0 and rng.uniform() < np.exp(-dE / T))
         if acc:
-            c, E, g = cands[k], En, gnew
+            c, E, g = np.array(cands[k]), En, None
             scale = etas[k]
             accepted += 1
         else:
