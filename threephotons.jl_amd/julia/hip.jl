@@ -125,6 +125,20 @@ function energy(intensity::SphericalHarmonicsVolume, basis::BasisTypeHip, c3ref:
     return energy(intensity, basis, c3ref, 1:basis.K, measure, negativity_factor)
 end
 
+"""The Bayes energy (FP64) and its gradient: g[:, k] holds dE/dRe c_k + i dE/dIm c_k
+for shells k = 1..K in the seanindex order of intensity.coeff[k] (tp_energy_grad; no
+counterpart in the reference, which evaluates energy() only)."""
+function hip_energy_grad(intensity::SphericalHarmonicsVolume, basis::BasisTypeHip, c3ref::C3, K3_range::UnitRange{Int64}=1:basis.K)
+    ref = hip_ref(basis, c3ref, K3_range)
+    coeff = hip_pack(intensity, basis.K)
+    E = Ref{Float64}(0.0)
+    g = zeros(Complex{Float64}, size(coeff))
+    hip_check(ccall((:tp_energy_grad, libtp), Cint,
+                    (Ptr{Void}, Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Ptr{Float64}, Ptr{Complex{Float64}}),
+                    hip_ctx, basis.handle, ref, coeff, size(coeff, 1), E, g), "tp_energy_grad")
+    return E[], g
+end
+
 """Switch a basis to the mixed-precision contraction (1 = FP16 hi/lo, 2 = BF16, 0 = FP64)."""
 function hip_set_precision(basis::BasisTypeHip, mode::Integer)
     hip_check(ccall((:tp_basis_set_precision, libtp), Cint, (Ptr{Void}, Cint), basis.handle, mode),
