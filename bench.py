@@ -193,6 +193,39 @@ def precision_study(n):
     print(json.dumps(out), flush=True)
 
 
+def gradient_profile(ncalls, ncand=16, reps=10):
+    """Instead of the benchmark: the parts of a gradient move on the full-size
+    workload, timed from the host (each after one untimed call), then ncalls
+    more gradient calls (the kernel-trace target of the gradient profile)."""
+    import tp_amd as tp
+    from tp_amd import descent, workload
+    cfg = workload.CONFIGS["full"]
+    tp.HIP_init()
+    truth, basis, c3ref, start = setup(cfg, tp, workload)
+    basis.set_precision("fp16x3")
+    c = start.coeff.copy()
+    gm = descent.GradientMoves(basis)
+
+    def timed(f):
+        r = f()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = f()
+        return 1e3 * (time.perf_counter() - t0) / reps, r
+
+    t_grad, (_, g) = timed(lambda: tp.energy_grad(c, basis, c3ref))
+    t_riem, om = timed(lambda: gm.riemannian(c, g))
+    etas = np.geomspace(0.1, 10.0, ncand) / gm.norm(om)
+    t_cand, cands = timed(lambda: gm.candidates(c, om, etas, basis.K))
+    t_batch, _ = timed(lambda: tp.energy_batch(list(cands), [basis] * ncand, [c3ref] * ncand))
+    t_e64, _ = timed(lambda: descent._energy64(cands[0], basis, c3ref, None))
+    for _ in range(ncalls):
+        tp.energy_grad(c, basis, c3ref)
+    print(json.dumps({"metric": "gradient move parts (ms, host-timed)", "energy_grad": t_grad, "riemannian": t_riem,
+                      "candidates_x%d" % ncand: t_cand, "energy_batch_fp16x3_x%d" % ncand: t_batch,
+                      "energy_fp64": t_e64}), flush=True)
+
+
 def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=tuple(range(1000, 1020))):
     """Concurrent annealing runs scored together: run r (seed 1000+r) has
     K_r ~ U{10..26} shells (drawn from its seed), starts from
@@ -377,12 +410,17 @@ def main():
     ap.add_argument("--learned-cmp-sweeps", type=int, default=40)
     ap.add_argument("--learned-tte-sweeps", type=int, default=200)
     ap.add_argument("--grad-steps", type=int, default=30, help="steps of the gradient-guided moves (0: skip)")
+    ap.add_argument("--grad-profile", type=int, default=0,
+                    help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
                     help="instead of the benchmark: c3/energy errors of the mixed-precision modes against FP64 "
                          "for this many random orientations of the full-size structure")
     args = ap.parse_args()
     if args.precision_study:
         precision_study(args.precision_study)
+        return
+    if args.grad_profile:
+        gradient_profile(args.grad_profile)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the k-triple sharding "

@@ -182,10 +182,11 @@ __global__ __launch_bounds__(1024) void k_triple_bwd(const double2 *__restrict__
                                                     const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
                                                     const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
                                                     int Lw, int Emax, const double *__restrict__ gA, int64_t ldga,
-                                                    int klength, double2 *__restrict__ gpart)
+                                                    int klength, const int *__restrict__ order,
+                                                    double2 *__restrict__ gpart)
 {
     extern __shared__ double2 smb[];
-    const int lt = blockIdx.x;
+    const int lt = order[blockIdx.x];   // most expensive l-triples first
     const int4 li = ltinfo[lt];
     const int l3 = li.z, l1 = li.w & 0xff, l2 = (li.w >> 8) & 0xff;
     const int n1 = 2 * l1 + 1, n2 = 2 * l2 + 1, n3 = 2 * l3 + 1;
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(1024) void k_triple_bwd(const double2 *__restrict__
 
     const int tid = threadIdx.x;
     constexpr int NT = 1024;
+    const int npr = (np + 63) & ~63;   // first thread of the A group (2 npr <= NT - 32: K <= 27)
     const int co1 = l1 * (l1 - 1) / 2, co2 = l2 * (l2 - 1) / 2, co3 = l3 * (l3 - 1) / 2;
     for (int i = tid; i < K * n1; i += NT) c1b[(i / n1) * Lw + i % n1] = coef[(size_t)(i / n1) * n_c + co1 + i % n1];
     for (int i = tid; i < K * n2; i += NT) c2b[(i / n2) * Lw + i % n2] = coef[(size_t)(i / n2) * n_c + co2 + i % n2];
@@ -244,28 +246,52 @@ __global__ __launch_bounds__(1024) void k_triple_bwd(const double2 *__restrict__
         }
         if (mu == n3) break;
         const int s0 = sg[mu], E = sg[mu + 1] - s0;
-        // phase 1: T and A of every pair
-        for (int p = tid; p < np; p += NT) {
+        // phase 1: T of every pair (threads [0, np)) and A of every pair
+        // (threads [npr, npr + np)); two accumulators per sum for ILP
+        if (tid < np) {
+            const int p = tid;
             const int k1 = pk[p] & 0xff, k2 = pk[p] >> 8;
             const double2 *a = c1b + (k1 - 1) * Lw, *b = c2b + (k2 - 1) * Lw;
-            double tr = 0.0, ti = 0.0;
-            for (int e = 0; e < E; ++e) {
-                const uint32_t id = si[s0 + e];
-                const double w = sw[s0 + e];
-                const double2 x = a[id & 0xffffu], y = b[id >> 16];
-                tr += w * (x.x * y.x - x.y * y.y);
-                ti += w * (x.x * y.y + x.y * y.x);
+            double tr0 = 0.0, ti0 = 0.0, tr1 = 0.0, ti1 = 0.0;
+            int e = 0;
+            for (; e + 1 < E; e += 2) {
+                const uint32_t id0 = si[s0 + e], id1 = si[s0 + e + 1];
+                const double w0 = sw[s0 + e], w1 = sw[s0 + e + 1];
+                const double2 x0 = a[id0 & 0xffffu], y0 = b[id0 >> 16], x1 = a[id1 & 0xffffu], y1 = b[id1 >> 16];
+                tr0 += w0 * (x0.x * y0.x - x0.y * y0.y);
+                ti0 += w0 * (x0.x * y0.y + x0.y * y0.x);
+                tr1 += w1 * (x1.x * y1.x - x1.y * y1.y);
+                ti1 += w1 * (x1.x * y1.y + x1.y * y1.x);
             }
-            double ar = 0.0, ai = 0.0;
+            if (e < E) {
+                const uint32_t id0 = si[s0 + e];
+                const double w0 = sw[s0 + e];
+                const double2 x0 = a[id0 & 0xffffu], y0 = b[id0 >> 16];
+                tr0 += w0 * (x0.x * y0.x - x0.y * y0.y);
+                ti0 += w0 * (x0.x * y0.y + x0.y * y0.x);
+            }
+            Ts[p] = make_double2(tr0 + tr1, ti0 + ti1);
+        } else if (tid >= npr && tid < npr + np) {
+            const int p = tid - npr;
+            const int k1 = pk[p] & 0xff, k2 = pk[p] >> 8;
+            double ar0 = 0.0, ai0 = 0.0, ar1 = 0.0, ai1 = 0.0;
             const int kb = ktof(k1, k2, 1);
-            for (int k3 = 1; k3 <= k2; ++k3) {
-                const double g = sgA[kb + k3 - 1];
-                const double2 c = c3b[(k3 - 1) * Lw + mu];
-                ar += g * c.x;
-                ai += g * c.y;
+            int k3 = 1;
+            for (; k3 + 1 <= k2; k3 += 2) {
+                const double g0 = sgA[kb + k3 - 1], g1v = sgA[kb + k3];
+                const double2 c0 = c3b[(k3 - 1) * Lw + mu], c1 = c3b[k3 * Lw + mu];
+                ar0 += g0 * c0.x;
+                ai0 += g0 * c0.y;
+                ar1 += g1v * c1.x;
+                ai1 += g1v * c1.y;
             }
-            Ts[p] = make_double2(tr, ti);
-            Ap[p] = make_double2(ar, ai);
+            if (k3 <= k2) {
+                const double g0 = sgA[kb + k3 - 1];
+                const double2 c0 = c3b[(k3 - 1) * Lw + mu];
+                ar0 += g0 * c0.x;
+                ai0 += g0 * c0.y;
+            }
+            Ap[p] = make_double2(ar0 + ar1, ai0 + ai1);
         }
         __syncthreads();
         // phase 2a: slots 1 and 2, thread (entry e, shell k)
@@ -324,22 +350,34 @@ __global__ __launch_bounds__(1024) void k_triple_bwd(const double2 *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// gamma[k][c] = sum over the (l-triple, slot) blocks whose slot has order
-// l(c), in the fixed order of the list (deterministic).  list[lstart[l/2] ..
-// lstart[l/2 + 1]) holds lt * 3 + slot; lofc[c] = {l, m + l}.
+// gamma[k][l, m] = sum over the (l-triple, slot) blocks whose slot has order
+// l, in the fixed order of the list (deterministic).  One wave per (k, l),
+// lanes over m = 0..2l; list[lstart[l/2] .. lstart[l/2 + 1]) holds
+// lt * 3 + slot; coff[l] is the compact offset of order l.
 // ---------------------------------------------------------------------------
-__global__ void k_grad_reduce(const double2 *__restrict__ gpart, int K, int Lw, int n_c, const int2 *__restrict__ lofc,
-                              const int *__restrict__ lstart, const int *__restrict__ list, double2 *__restrict__ grad)
+__global__ __launch_bounds__(64) void k_grad_reduce(const double2 *__restrict__ gpart, int K, int Lw, int n_c,
+                                                    const int *__restrict__ coff, const int *__restrict__ lstart,
+                                                    const int *__restrict__ list, double2 *__restrict__ grad)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= K * n_c) return;
-    const int k = i / n_c, c = i % n_c;
-    const int2 lm = lofc[c];
+    const int k = blockIdx.x, li = blockIdx.y, l = 2 * li, m = threadIdx.x;
+    if (m > 2 * l) return;
+    const int q0 = lstart[li], q1 = lstart[li + 1];
     double sr = 0.0, si = 0.0;
-    for (int q = lstart[lm.x / 2]; q < lstart[lm.x / 2 + 1]; ++q) {
-        const double2 v = gpart[((size_t)list[q] * K + k) * Lw + lm.y];
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {   // four independent loads in flight
+        const double2 v0 = gpart[((size_t)list[q] * K + k) * Lw + m];
+        const double2 v1 = gpart[((size_t)list[q + 1] * K + k) * Lw + m];
+        const double2 v2 = gpart[((size_t)list[q + 2] * K + k) * Lw + m];
+        const double2 v3 = gpart[((size_t)list[q + 3] * K + k) * Lw + m];
+        sr += v0.x; si += v0.y;
+        sr += v1.x; si += v1.y;
+        sr += v2.x; si += v2.y;
+        sr += v3.x; si += v3.y;
+    }
+    for (; q < q1; ++q) {
+        const double2 v = gpart[((size_t)list[q] * K + k) * Lw + m];
         sr += v.x;
         si += v.y;
     }
-    grad[i] = make_double2(sr, si);
+    grad[(size_t)k * n_c + coff[l] + m] = make_double2(sr, si);
 }

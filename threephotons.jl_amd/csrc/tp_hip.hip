@@ -152,9 +152,9 @@ struct tp_basis {
     double *d_gA = nullptr;        // [nlt_pad64][kl_pad]          dE/dAs
     double2 *d_gpart = nullptr;    // [n_lt][3][K][gLw]            per (l-triple, slot) gradient blocks
     double2 *d_grad = nullptr;     // [K][n_c]                     dE/dRe c + i dE/dIm c
-    int2 *d_lofc = nullptr;        // [n_c] {l, m + l}
     int *d_lstart = nullptr;       // [L/2 + 2]
     int *d_llist = nullptr;        // [3 n_lt] lt * 3 + slot, grouped by the slot's l
+    int *d_bwd_order = nullptr;    // [n_lt] l-triples by decreasing backward cost
 };
 
 struct tp_ref {
@@ -1071,7 +1071,7 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
-                    b->d_grad, b->d_lofc, b->d_lstart, b->d_llist};
+                    b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1650,20 +1650,27 @@ static int grad_setup(tp_ctx *ctx, tp_basis *b)
         }
     }
     lstart[nl] = (int)llist.size();
-    std::vector<int2> lofc(b->n_c);
-    for (int l = 0; l <= b->L; l += 2)
-        for (int m = 0; m <= 2 * l; ++m) lofc[b->coff[l] + m] = make_int2(l, m);
+    // backward work order: per mu3 the pair sums (entries) and the k3 / slot sums
+    std::vector<std::pair<int64_t, int>> cost(nlt);
+    const int np = K * (K + 1) / 2;
+    for (int t = 0; t < nlt; ++t) {
+        const int4 q = b->h_ltinfo[t];
+        cost[t] = {-((int64_t)b->h_lt_cnt[t] * (np + 2 * K) + (int64_t)(2 * q.z + 1) * 3 * np), t};
+    }
+    std::sort(cost.begin(), cost.end());
+    std::vector<int> order(nlt);
+    for (int t = 0; t < nlt; ++t) order[t] = cost[t].second;
     int rc = 0;
     if ((rc = dev_alloc(&b->d_HT4, HT4.size())) || (rc = dev_alloc(&b->d_gGp, (size_t)njc * nmp * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_gA, (size_t)b->nlt_pad64 * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_gpart, (size_t)nlt * 3 * K * b->gLw)) || (rc = dev_alloc(&b->d_grad, (size_t)K * b->n_c)) ||
-        (rc = dev_alloc(&b->d_lofc, lofc.size())) || (rc = dev_alloc(&b->d_lstart, lstart.size())) ||
+        (rc = dev_alloc(&b->d_lstart, lstart.size())) || (rc = dev_alloc(&b->d_bwd_order, order.size())) ||
         (rc = dev_alloc(&b->d_llist, llist.size())))
         return rc;
     TP_HIP(hipMemcpy(b->d_HT4, HT4.data(), sizeof(double) * HT4.size(), hipMemcpyHostToDevice));
     TP_HIP(hipMemset(b->d_gGp, 0, sizeof(double) * (size_t)njc * nmp * b->kl_pad));   // rows / columns never written stay 0
-    TP_HIP(hipMemcpy(b->d_lofc, lofc.data(), sizeof(int2) * lofc.size(), hipMemcpyHostToDevice));
     TP_HIP(hipMemcpy(b->d_lstart, lstart.data(), sizeof(int) * lstart.size(), hipMemcpyHostToDevice));
+    TP_HIP(hipMemcpy(b->d_bwd_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
     TP_HIP(hipMemcpy(b->d_llist, llist.data(), sizeof(int) * llist.size(), hipMemcpyHostToDevice));
     b->grad_ready = true;
     return 0;
@@ -1689,11 +1696,10 @@ static int enqueue_grad(tp_ctx *ctx, tp_basis *b, const tp_ref *r, const double2
     TP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_triple_bwd, dim3((unsigned)b->n_lt), dim3(1024), b->bwd_lds, st, coef, b->n_c, b->K, b->d_ew,
                        b->d_eidx, b->d_gstart, b->d_ltinfo, b->gLw, b->gEmax, b->d_gA, b->kl_pad, (int)b->klength,
-                       b->d_gpart);
+                       b->d_bwd_order, b->d_gpart);
     TP_HIP(hipGetLastError());
-    const int n = b->K * b->n_c;
-    hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, b->d_gpart, b->K, b->gLw,
-                       b->n_c, b->d_lofc, b->d_lstart, b->d_llist, b->d_grad);
+    hipLaunchKernelGGL(k_grad_reduce, dim3((unsigned)b->K, (unsigned)(b->L / 2 + 1)), dim3(64), 0, st, b->d_gpart, b->K,
+                       b->gLw, b->n_c, b->d_coff, b->d_lstart, b->d_llist, b->d_grad);
     TP_HIP(hipGetLastError());
     return 0;
 }

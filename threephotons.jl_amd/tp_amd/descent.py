@@ -48,20 +48,40 @@ class GradientMoves:
     def norm(om):
         return float(np.sqrt(sum((A * A).sum() for A in om.values())))
 
-    def candidates(self, c, om, etas):
-        """The coefficients moved by exp(-eta Omega_l) on every l, one array per
-        eta.  With i Omega_l = V diag(w) V^H (Hermitian) and W = Umat(l)^H V, the
-        complex-basis rotation is W diag(exp(i eta w)) W^H; all etas of one l
-        are formed in one batched product."""
+    def _block_rotations(self, om, etas):
+        """{l: (S, n, n)} complex-basis rotations exp(-eta Omega_l) for all etas:
+        with i Omega_l = V diag(w) V^H (Hermitian) and W = Umat(l)^H V, the
+        rotation is W diag(exp(i eta w)) W^H (one batched product per l)."""
         etas = np.asarray(etas, dtype=np.float64)
-        out = np.repeat(c[None, :, :], len(etas), axis=0)
+        out = {}
         for l, A in om.items():
-            U, ix = self.U[l], self.idx[l]
             w, V = np.linalg.eigh(1j * A)
-            W = U.conj().T @ V
-            ph = np.exp(1j * etas[:, None] * w[None, :])             # (S, n)
-            rot = (W[None, :, :] * ph[:, None, :]) @ W.conj().T       # (S, n, n)
-            out[:, :, ix] = c[None, :, ix] @ np.swapaxes(rot, 1, 2)
+            W = self.U[l].conj().T @ V
+            ph = np.exp(1j * etas[:, None] * w[None, :])
+            out[l] = (W[None, :, :] * ph[:, None, :]) @ W.conj().T
+        return out
+
+    def candidates(self, c, om, etas, nshell=None):
+        """(S, nshell, LMAX^2): the coefficients moved by exp(-eta Omega_l) on
+        every l, one slice per eta.  Only the entries the energy reads (even
+        l <= L) are set; the others are left unset (the library never reads
+        them).  Use moved() for a complete state."""
+        nshell = c.shape[0] if nshell is None else nshell
+        rots = self._block_rotations(om, etas)
+        out = np.empty((len(etas), nshell, c.shape[1]), np.complex128)
+        out[:, :, 0] = c[None, :nshell, 0]   # l = 0 (seanindex 1) is never rotated
+        for l, rot in rots.items():
+            ix = self.idx[l]
+            out[:, :, ix] = c[None, :nshell, ix] @ np.swapaxes(rot, 1, 2)
+        return out
+
+    def moved(self, c, om, eta):
+        """The complete coefficient array (every shell and entry) moved by exp(-eta Omega_l)."""
+        rots = self._block_rotations(om, [eta])
+        out = c.copy()
+        for l, rot in rots.items():
+            ix = self.idx[l]
+            out[:, ix] = c[:, ix] @ rot[0].T
         return out
 
     def rotations(self, om, etas):
@@ -127,7 +147,7 @@ def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, se
         if scale is None:
             scale = 1.0 / gn
         etas = scale * ladder
-        cands = gm.candidates(c, om, etas)
+        cands = gm.candidates(c, om, etas, basis.K)
         Ec = energy_batch(list(cands), bases, refs, rngs)
         k = int(np.argmin(Ec))
         En = _energy64(cands[k], basis, c3ref, K3_range)
@@ -135,7 +155,7 @@ def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, se
         dE = En - E
         acc = dE < 0.0 or (T > 0.0 and rng.uniform() < np.exp(-dE / T))
         if acc:
-            c, E, g = np.array(cands[k]), En, None
+            c, E, g = gm.moved(c, om, etas[k]), En, None
             scale = etas[k]
             accepted += 1
         else:
