@@ -76,7 +76,9 @@ struct tp_ctx {
 };
 
 // contraction tiles
-constexpr int G1_BM = 64, G1_BN = 32, G1_WGM = 2;   // G = H . As
+constexpr int G1_BM = 64, G1_BN = 32, G1_WGM = 2;   // tile of the gradient's gAs = H^T . gG
+constexpr int GF_BM = 32, GF_BN = 32, GF_WGM = 2;   // G = H . As: rows = classes (136 at L = 18), small tiles
+                                                    // so that the launch has several waves of workgroups
 constexpr int G2_BM = 64, G2_BN = 64, G2_WGM = 4;   // c3 = Bm . G (+ energy)
 constexpr int KL_ALIGN = 128;                       // column padding (multiple of G1_BN, G2_BN and H_BN)
 constexpr int kMfmaMaxK = 27;                       // k_triple_mfma: at most 16 row blocks of pairs per half
@@ -271,8 +273,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         n_rt_fin = b->n_rt16;
     } else {
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
-    hipLaunchKernelGGL((k_gemm<G1_BM, G1_BN, G1_WGM, 2, false>), dim3((unsigned)(b->g1_rt * b->g1_ct)), dim3(256), 0,
-                       st, b->d_H, b->nm_pad64, b->d_As, b->kl_pad, (int)b->nlt_pad, (int)b->nm_pad64,
+    hipLaunchKernelGGL((k_gemm<GF_BM, GF_BN, GF_WGM, 2, false>), dim3((unsigned)(b->g1_rt * b->g1_ct)), dim3(256), 0,
+                       st, b->d_H, b->nm_pad64, b->d_As, b->kl_pad, (int)b->nlt_pad, (int)(b->g1_rt * GF_BM),
                        (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
@@ -1358,8 +1360,8 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     b->nm_pad16 = round_up(b->n_m, 16);
     b->n_rt = (int)(b->R_pad / G2_BM);
     b->n_ct = (int)(b->kl_pad / G2_BN);
-    b->g1_rt = (int)(b->nm_pad64 / G1_BM);
-    b->g1_ct = (int)(b->kl_pad / G1_BN);
+    b->g1_rt = (int)((b->n_m + GF_BM - 1) / GF_BM);   // rows of G past them stay unwritten (never read)
+    b->g1_ct = (int)(b->kl_pad / GF_BN);
     b->n_fb = (int)((b->kl_pad + kFinalCols - 1) / kFinalCols);
     b->ldk = (int)round_up(nlt, 64);
     b->R_pad16 = round_up(b->R, H_BM);
