@@ -1,10 +1,12 @@
 // tp_kernels.hpp -- gfx950 device code of libtp (included by tp_hip.hip).
 //
 // One c3 + Bayes-energy evaluation (lambda = 0) is four launches:
-//   k_triple   coupling-form triple products As[lt][kt]              (FP64, LDS)
-//   k_gemm     G  = H . As      (folded (m2,mu3) x k-triple)         (FP64 MFMA)
-//   k_gemm     c3 = Bm . G, fused Bayes partial sums per row tile    (FP64 MFMA)
-//   k_final    per-column energies, fixed-order total, optional Metropolis step
+//   k_triple_mfma  coupling-form triple products As[lt][kt]                 (FP64 MFMA, LDS)
+//                  (k_triple: the VALU form for K > 27)
+//   k_gemm         G  = H . As      (folded (m2,mu3) x k-triple)            (FP64 MFMA)
+//   k_c3sep        c3 = Bm . G on the separated (alpha, beta) grid, fused
+//                  Bayes partial sums (k_gemm with the energy epilogue for N > 32)
+//   k_final        per-column energies, fixed-order total, optional Metropolis step
 // H[(m2,mu3), lt] = sum_{j in lt} w_j Y(l1,m1) Y(l2,m2) Y(l3,mu3) and
 // Bm[r, (m2,mu3)] = cos(m2 a_r + mu3 b_r), so Bm . H is the l-triple basis F
 // of the reference's slow path (flab, correlations.jl:337-362) and
