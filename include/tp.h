@@ -208,11 +208,13 @@ int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, cons
                     tp_batch **out);
 void tp_batch_destroy(tp_batch *B);
 /* Time the candidate triple-product work orders and contraction kernels on
- * this batch (reps launches each) and keep the fastest.  ms_out[10] (may be
+ * this batch (reps launches each) and keep the fastest.  ms_out[14] (may be
  * NULL): work orders 0 (by cost), 1 (evaluation-major), then tiles 128x64,
- * 64x64, 128x128, 64x128 with parallel row tiles, then the same four with
- * row-walking workgroups; config_out[4] (may be NULL): chosen tile rows, tile
- * columns, work order, schedule (1 parallel, 2 row-walking). */
+ * 64x64, 128x128, 64x128 with parallel row tiles, the same four with
+ * row-walking workgroups, and the same four with the row tiles partitioned
+ * over the XCDs (1e30 where a shape does not divide by 8); config_out[4] (may
+ * be NULL): chosen tile rows, tile columns, work order, schedule (1 parallel,
+ * 2 row-walking, 3 XCD-partitioned rows). */
 int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out);
 /* Initial energies and temperatures: slot 0 of every run is the identity, T0 =
  * temperature_factor * std(energies of the other slots) (calculate_temperature,
@@ -223,6 +225,9 @@ int tp_batch_set_family(tp_batch *B, int family);
 /* Angle bound of every run's random proposals, as a fraction of pi/4
  * (0 < amax <= 1; 1 at creation); it then adapts by `factor` per sweep. */
 int tp_batch_set_amax(tp_batch *B, double amax);
+/* Use contraction candidate `cand` (0..11, the order of tp_batch_autotune's
+ * ms_out[2..13]) instead of the tuned one, re-capturing the sweep graph. */
+int tp_batch_set_contraction(tp_batch *B, int cand);
 /* nsweeps sweeps (graph replays). */
 int tp_batch_run(tp_batch *B, int nsweeps, double decay, double factor);
 /* Per-run state: energy, temperature, angle bound, accepted sweeps, sweeps done. */

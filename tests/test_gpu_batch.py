@@ -136,7 +136,7 @@ def test_batch_sweeps(tp, family):
     B.set_family(family)
     cfg = B.autotune(reps=2)
     assert cfg["tile"] in [(128, 64), (64, 64), (128, 128), (64, 128)] and cfg["order"] in (0, 1)
-    assert cfg["schedule"] in ("rows", "walk")
+    assert cfg["schedule"] in ("rows", "walk", "xcdrows")
     B.probe(0.1)
     g0 = [B.group(g) for g in range(len(Ks))]
     for b in bases:
@@ -166,3 +166,25 @@ def test_batch_sweeps(tp, family):
     B2.run(nsw, decay=0.999, factor=1.01)
     # (its contraction tile may differ from the autotuned one: same values up to summation order)
     np.testing.assert_allclose(B2.history(), hist, rtol=1e-10)
+
+
+def test_batch_every_contraction_candidate(tp):
+    """Every contraction candidate of the autotuner, forced in turn, gives the
+    single evaluations' energies (N = 32: 2048 rows, so the XCD-partitioned
+    schedule applies to every tile)."""
+    Ks, nprop, L, N = [4, 7, 5], 4, 4, 32
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, L=L, LMAX=6, N=N)
+    rng = np.random.default_rng(12)
+    q = np.stack([random_quats(rng, L // 2) for _ in range(len(Ks) * nprop)])
+    for b in bases:
+        b.set_precision("fp16x3")
+    Es = []
+    for g in range(len(Ks)):
+        for p in range(nprop):
+            Es.append(tp.energy(tp.rotate(starts[g], bases[g], q[g * nprop + p]), bases[g], refs[g]))
+    Es = np.array(Es)
+    for cand in range(12):
+        print("candidate", cand, flush=True)
+        B.set_contraction(cand)
+        E = B.eval_quats(q)
+        assert np.all(np.abs(E - Es) <= 1e-10 * np.abs(Es)), (cand, np.abs(E - Es).max())

@@ -77,6 +77,7 @@ struct tp_ctx {
 
 // contraction tiles
 constexpr int G1_BM = 64, G1_BN = 32, G1_WGM = 2;   // tile of the gradient's gAs = H^T . gG
+constexpr int kNumTiles = 12;                       // contraction candidates of the batch autotuner
 constexpr int GF_BM = 32, GF_BN = 32, GF_WGM = 2;   // G = H . As: rows = classes (136 at L = 18), small tiles
                                                     // so that the launch has several waves of workgroups
 constexpr int G2_BM = 64, G2_BN = 64, G2_WGM = 4;   // c3 = Bm . G (+ energy)
@@ -355,7 +356,7 @@ struct tp_batch {
     int nwork = 0;
     int gcand = 0, order = 0;              // contraction candidate (kTiles), triple-product work order (autotuned)
     int family = TP_MOVE_PLANE;            // move family of the random slots
-    double tune_ms[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double tune_ms[2 + kNumTiles] = {};
     EvalDesc *d_ev = nullptr;
     GroupState *d_gs = nullptr;
     int2 *d_work = nullptr;
@@ -557,7 +558,8 @@ static void batch_gemm_launch(const tp_batch *B, hipStream_t st)
 {
     const tp_basis *b = B->b;
     const int n_rt = (int)(b->R_pad16 / BM), n_ct = (int)(B->ldA / BN);
-    // WALK = 2: one workgroup per column tile walking all row tiles; 1: one per tile
+    // WALK = 2: one workgroup per column tile walking all row tiles; 1: one per
+    // tile; 3: one per tile, row tiles partitioned over the XCDs
     const dim3 grid((unsigned)(WALK == 2 ? n_ct : n_rt * n_ct));
     if (b->prec == TP_PREC_FP16X3)
         hipLaunchKernelGGL((k_gemm16<BM, BN, 1, 0, true, WALK>), grid, dim3(256), 0, st, b->d_Fhi, b->d_Flo,
@@ -571,8 +573,9 @@ static void batch_gemm_launch(const tp_batch *B, hipStream_t st)
 
 // Contraction candidates: tile (rows x columns) and schedule (1: parallel row
 // tiles, 2: row-walking workgroups).
-static const int kTiles[8][3] = {{128, 64, 1}, {64, 64, 1}, {128, 128, 1}, {64, 128, 1},
-                                 {128, 64, 2}, {64, 64, 2}, {128, 128, 2}, {64, 128, 2}};
+static const int kTiles[kNumTiles][3] = {{128, 64, 1}, {64, 64, 1}, {128, 128, 1}, {64, 128, 1},
+                                         {128, 64, 2}, {64, 64, 2}, {128, 128, 2}, {64, 128, 2},
+                                         {128, 64, 3}, {64, 64, 3}, {128, 128, 3}, {64, 128, 3}};
 
 static void batch_gemm(const tp_batch *B, hipStream_t st, int cand)
 {
@@ -584,7 +587,11 @@ static void batch_gemm(const tp_batch *B, hipStream_t st, int cand)
     case 4: batch_gemm_launch<128, 64, 2>(B, st); break;
     case 5: batch_gemm_launch<64, 64, 2>(B, st); break;
     case 6: batch_gemm_launch<128, 128, 2>(B, st); break;
-    default: batch_gemm_launch<64, 128, 2>(B, st); break;
+    case 7: batch_gemm_launch<64, 128, 2>(B, st); break;
+    case 8: batch_gemm_launch<128, 64, 3>(B, st); break;
+    case 9: batch_gemm_launch<64, 64, 3>(B, st); break;
+    case 10: batch_gemm_launch<128, 128, 3>(B, st); break;
+    default: batch_gemm_launch<64, 128, 3>(B, st); break;
     }
 }
 
@@ -721,8 +728,16 @@ static int batch_autotune(tp_batch *B, int reps)
     batch_split(B, st);
     best = 1e30;
     int best_t = 0;
-    for (int t = 0; t < 8; ++t) {
+    const int nct64 = (int)(B->ldA / 64);
+    for (int t = 0; t < kNumTiles; ++t) {
         double ms = 0;
+        if (kTiles[t][2] == 3) {   // the XCD partition needs n_rt and the grid divisible by 8
+            const int nrt = (int)(B->b->R_pad16 / kTiles[t][0]), nct = (int)(B->ldA / kTiles[t][1]);
+            if (nrt % 8 != 0 || (nrt * nct) % 8 != 0 || nct64 < 1) {
+                B->tune_ms[2 + t] = 1e30;
+                continue;
+            }
+        }
         if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_gemm(B, s, t); }, &ms)) return rc;
         B->tune_ms[2 + t] = ms;
         if (ms < best) { best = ms; best_t = t; }
@@ -2058,7 +2073,7 @@ int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out)
     }
     if (int rc = batch_autotune(B, reps)) return rc;
     if (ms_out)
-        for (int i = 0; i < 10; ++i) ms_out[i] = B->tune_ms[i];
+        for (int i = 0; i < 2 + kNumTiles; ++i) ms_out[i] = B->tune_ms[i];
     if (config_out) {
         config_out[0] = kTiles[B->gcand][0];
         config_out[1] = kTiles[B->gcand][1];
@@ -2500,6 +2515,21 @@ int tp_batch_set_amax(tp_batch *B, double amax)
     TP_HIP(hipMemcpy(gs.data(), B->d_gs, sizeof(GroupState) * B->ngroups, hipMemcpyDeviceToHost));
     for (auto &g : gs) g.amax = amax;
     TP_HIP(hipMemcpy(B->d_gs, gs.data(), sizeof(GroupState) * B->ngroups, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int tp_batch_set_contraction(tp_batch *B, int cand)
+{
+    if (!B || cand < 0 || cand >= kNumTiles) return tp_fail(TP_E_ARG, "contraction candidate out of range");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    if (kTiles[cand][2] == 3) {
+        const int nrt = (int)(B->b->R_pad16 / kTiles[cand][0]), nct = (int)(B->ldA / kTiles[cand][1]);
+        if (nrt % 8 != 0 || (nrt * nct) % 8 != 0)
+            return tp_fail(TP_E_UNSUPPORTED, "the XCD-partitioned schedule needs row and tile counts divisible by 8");
+    }
+    B->gcand = cand;
+    TP_HIP(hipStreamSynchronize(B->ctx->stream));
+    if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
     return 0;
 }
 

@@ -671,7 +671,8 @@ __global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As,
 //              (the row tiles of a column tile share an XCD's L2);
 //   BATCH = 2: one workgroup per column tile walking all row tiles (its B tile
 //              read once; the energy sums it writes, part row 0, are whole
-//              columns).
+//              columns);
+//   BATCH = 3: as 1, with the row tiles partitioned over the 8 XCDs.
 // Either way the column tiles that share histogram columns are neighbours in
 // the (XCD-grouped) tile order.
 constexpr int H_BM = 128, H_BN = 64, H_BK = 32, H_LDS_ROW = 40;
@@ -696,8 +697,19 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
     const int nblk = BATCH == 2 ? n_ct : n_rt * n_ct;
     int lin = blockIdx.x;
     if ((nblk & 7) == 0) lin = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
-    const int ct = BATCH == 2 ? lin : (BATCH == 1 ? lin / n_rt : lin % n_ct);
-    const int rt_first = BATCH == 2 ? 0 : (BATCH == 1 ? lin % n_rt : lin / n_ct);
+    int ct, rt_first;
+    if (BATCH == 3) {
+        // the XCD of workgroup id (id mod 8) owns row tiles [x n_rt/8, (x+1) n_rt/8)
+        // for every column tile: its L2 keeps those rows of A; the column tiles
+        // of one k-triple tile (all proposals) run back to back and share the
+        // histogram tile (host: n_rt % 8 == 0 and nblk % 8 == 0)
+        const int rpx = n_rt >> 3, x = blockIdx.x & 7, i = blockIdx.x >> 3;
+        rt_first = x * rpx + i % rpx;
+        ct = i / rpx;
+    } else {
+        ct = BATCH == 2 ? lin : (BATCH == 1 ? lin / n_rt : lin % n_ct);
+        rt_first = BATCH == 2 ? 0 : (BATCH == 1 ? lin % n_rt : lin / n_ct);
+    }
     const int rt_last = BATCH == 2 ? n_rt : rt_first + 1;
     const int c0 = ct * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;

@@ -376,13 +376,17 @@ class AnnealBatch:
             pass
 
     def autotune(self, reps=3):
-        ms = np.zeros(10)
+        ms = np.zeros(14)
         cfg = np.zeros(4, np.int32)
         check(lib().tp_batch_autotune(self.h, reps, ms.ctypes.data, cfg.ctypes.data), "tp_batch_autotune")
-        names = [f"{t}/{s}" for s in ("rows", "walk") for t in ("128x64", "64x64", "128x128", "64x128")]
+        names = [f"{t}/{s}" for s in ("rows", "walk", "xcdrows") for t in ("128x64", "64x64", "128x128", "64x128")]
         return dict(order_ms=ms[:2].tolist(), tile_ms=dict(zip(names, ms[2:].tolist())),
                     tile=(int(cfg[0]), int(cfg[1])), order=int(cfg[2]),
-                    schedule="walk" if cfg[3] == 2 else "rows")
+                    schedule={1: "rows", 2: "walk", 3: "xcdrows"}[int(cfg[3])])
+
+    def set_contraction(self, cand):
+        """Fix the contraction candidate (0..11 in autotune()'s tile_ms order)."""
+        check(lib().tp_batch_set_contraction(self.h, int(cand)), "tp_batch_set_contraction")
 
     def probe(self, temperature_factor=0.1):
         check(lib().tp_batch_probe(self.h, float(temperature_factor)), "tp_batch_probe")

@@ -102,6 +102,7 @@ def lib():
     L.tp_batch_log_read.argtypes = [vp, i32, i32, vp, vp]
     L.tp_batch_set_proposer.argtypes = [vp, vp, i32, dbl]
     L.tp_batch_set_amax.argtypes = [vp, dbl]
+    L.tp_batch_set_contraction.argtypes = [vp, i32]
     L.tp_batch_set_family.argtypes = [vp, i32]
     _lib = L
     return L
@@ -135,7 +136,7 @@ EXPORTS = [
     "tp_batch_probe", "tp_batch_run", "tp_batch_group", "tp_batch_coeff", "tp_batch_history",
     "tp_batch_eval_quats", "tp_rotate", "tp_mlp_nparam", "tp_mlp_create", "tp_mlp_destroy", "tp_mlp_get_params",
     "tp_mlp_set_params", "tp_mlp_forward", "tp_mlp_loss_grad", "tp_mlp_train", "tp_mlp_train_log", "tp_batch_log",
-    "tp_batch_log_count", "tp_batch_log_read", "tp_batch_set_proposer", "tp_batch_set_amax",
+    "tp_batch_log_count", "tp_batch_log_read", "tp_batch_set_proposer", "tp_batch_set_amax", "tp_batch_set_contraction",
     "tp_batch_set_family", "tp_basis_from_host", "tp_energy_grad",
 ]
 
