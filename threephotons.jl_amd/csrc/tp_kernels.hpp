@@ -342,7 +342,9 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
         }
         __syncthreads();
         // stage 2: blocks b = wave + 16j -> (row block b>>1, k3 block b&1); for
-        // h = 0 only k3 block 0 (k3 <= k2 <= 16): b -> (b, 0)
+        // h = 0 only k3 block 0 (k3 <= k2 <= 16): b -> (b, 0).  k-steps kk
+        // cover mu3 offsets 2kk, 2kk+1: only those below n3 are run
+        const int kkn = (min(16, n3 - 16 * t) + 1) >> 1;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int b = wave + 16 * j;
@@ -351,6 +353,7 @@ __device__ __forceinline__ void triple_mfma_body(double2 *sm, const double2 *__r
                 const int k3 = 16 * cb + l16 + 1;
 #pragma unroll
                 for (int kk = 0; kk < 8; ++kk) {
+                    if (kk >= kkn) break;   // uniform over the workgroup
                     const int slot = 4 * kk + lq;
                     const double av = Tf[slot * Prows + 16 * rb + l16];
                     const int m3 = 16 * t + (slot >> 1);
