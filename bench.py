@@ -61,7 +61,7 @@ def lt_n3_sum(L):
 # dispatch), scaled by what the same counters read for a 256 MiB device copy
 # (FETCH_SIZE 0.5x, WRITE_SIZE 1.0x of the bytes moved); the summary with the
 # commands is profiles/hbm_counters_full_L18_K26_N32.md.
-TRIPLE_FETCH_KIB, TRIPLE_WRITE_KIB = 2482.6, 12924.8
+TRIPLE_FETCH_KIB, TRIPLE_WRITE_KIB = 2478.1, 12924.8
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 

@@ -181,6 +181,23 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
 int tp_energy_grad(tp_ctx *ctx, const tp_basis *basis, const tp_ref *ref, const double *coeff, int64_t ld,
                    double *E_out, double *grad_out);
 
+/* Photon-pair and photon-triplet histograms of sparse images (the data the
+ * energy is scored against).  tp_hist_create ≙ the zeroed c1/c2/c3 of
+ * generateHistogram (datagen.jl:363-372) with dq = qcut/K2; tp_hist_add ≙
+ * histogramCorrelationsInPicture_alltoall (datagen.jl:228-282) applied to
+ * npictures pictures: picture i is photons[3 offsets[i] .. 3 offsets[i+1])
+ * (x, y, z per photon; offsets[0] = 0; at most 512 photons per picture);
+ * tp_hist_read returns c1[K2], c2[N,K2,K2] and c3[N,2N,K3,K3,K3] in the
+ * reference's (column-major) layouts, weighted as the reference weights them,
+ * and optionally the integer counts behind them (n1/n2/n3, same layouts; any
+ * output may be NULL).  Counts are exact 64-bit sums, so repeated or split
+ * calls give identical histograms. */
+typedef struct tp_hist tp_hist;
+int tp_hist_create(tp_ctx *ctx, int N, int K2, int K3, double dq, tp_hist **out);
+int tp_hist_add(tp_hist *h, const double *photons, const int64_t *offsets, int npictures);
+int tp_hist_read(tp_hist *h, double *c1, double *c2, double *c3, uint64_t *n1, uint64_t *n2, uint64_t *n3);
+void tp_hist_destroy(tp_hist *h);
+
 #define TP_MOVE_QUAT 0    /* random slots: an SO(3) rotation per even l (uniform axis, Haar angle) */
 #define TP_MOVE_PLANE 1   /* random slots: the reference's move, a rotation in the plane of a
                            * Haar-random 2-frame of R^(2l+1) (default) */

@@ -57,6 +57,12 @@ def lib():
                                   C.c_double, C.c_double, d_p]
         L.or_energy_grad.restype = C.c_double
         L.or_energy_grad.argtypes = [C.c_void_p, d_p, C.c_int64, d_p, C.c_int, C.c_int, C.c_int, d_p, C.c_int]
+        L.or_angle_between.restype = C.c_double
+        L.or_angle_between.argtypes = [d_p, d_p]
+        L.or_hist_pictures.restype = None
+        L.or_hist_pictures.argtypes = [d_p, C.POINTER(C.c_int64), C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       d_p, d_p, d_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]
         L.or_energy.restype = C.c_double
         L.or_energy.argtypes = [d_p, C.c_int, C.c_int64, d_p, C.c_int, C.c_int, C.c_int]
         L.or_umat.argtypes = [C.c_int, d_p]
@@ -193,6 +199,29 @@ def energy(c3raw, N, c3ref, Kr, k3lo, k3hi):
     craw = np.ascontiguousarray(c3raw.T).ravel()
     cref = np.ascontiguousarray(np.asarray(c3ref, dtype=np.float64).ravel(order="F"))
     return lib().or_energy(craw, N, c3raw.shape[1], cref, Kr, k3lo, k3hi)
+
+
+def angle_between(p1, p2):
+    return lib().or_angle_between(np.ascontiguousarray(p1, dtype=np.float64), np.ascontiguousarray(p2, dtype=np.float64))
+
+
+def hist_pictures(photons, offsets, dq, N, K2, K3):
+    """histogramCorrelationsInPicture_alltoall over pictures: photons (n, 3),
+    picture i = photons[offsets[i]:offsets[i+1]].  Returns c1 (K2,), c2 (N, K2, K2),
+    c3 (N, 2N, K3, K3, K3) indexed as the reference ([ai, k2, k1], [ai, bi, k3, k2, k1]
+    zero-based) and the matching integer counts n1, n2, n3."""
+    ph = np.ascontiguousarray(photons, dtype=np.float64)
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    c1, c2, c3 = np.zeros(K2), np.zeros(N * K2 * K2), np.zeros(2 * N * N * K3 ** 3)
+    n1 = np.zeros(K2, np.uint64)
+    n2 = np.zeros(N * K2 * K2, np.uint64)
+    n3 = np.zeros(2 * N * N * K3 ** 3, np.uint64)
+    u64 = C.POINTER(C.c_uint64)
+    lib().or_hist_pictures(ph, off.ctypes.data_as(C.POINTER(C.c_int64)), len(off) - 1, dq, N, K2, K3, c1, c2, c3,
+                           n1.ctypes.data_as(u64), n2.ctypes.data_as(u64), n3.ctypes.data_as(u64))
+    s2, s3 = (N, K2, K2), (N, 2 * N, K3, K3, K3)
+    return (c1, c2.reshape(s2, order="F"), c3.reshape(s3, order="F"), n1, n2.reshape(s2, order="F"),
+            n3.reshape(s3, order="F"))
 
 
 def umat(l):

@@ -648,6 +648,120 @@ double or_energy_grad(const or_basis *b, const double *coeff, int64_t ld, const 
 }
 
 /* ------------------------------------------------------------------------ */
+/* Photon-pair and photon-triplet histograms of sparse images:               */
+/* histogramCorrelationsInPicture_alltoall (datagen.jl:228-282) with         */
+/* angle_between (utilities.jl:61-72), doubletFactor (datagen.jl:177-183),   */
+/* tripletFactor (:166-174).  Per picture: keep the photons with             */
+/* k = round(|p|/dq) in 1..K2 (round = nearest, ties to even, as Julia's     */
+/* round), stable-sort them by |p| ascending, then for i > j > k (so         */
+/* k1 >= k2 >= k3):                                                          */
+/*   c1[k1] += 1/k1;                                                         */
+/*   alpha = angle_between(p_i, p_j) in [0, 2pi); alpha_s = 2pi - alpha if   */
+/*   alpha > pi; ai = mod(floor(alpha_s/da), N) + 1, da = pi/N;             */
+/*   c2[ai,k2,k1] += 1/(doubletFactor k1 k2);                                */
+/*   if k1,k2,k3 <= K3: beta = angle_between(p_i, p_k), beta = 2pi - beta if */
+/*   alpha > pi; bi = mod(floor(beta/da), 2N) + 1;                           */
+/*   c3[ai,bi,k3,k2,k1] += 1/(tripletFactor k1 k2 k3).                       */
+/* angle_between returns 0 when the normalised dot product is exactly 0      */
+/* (orthogonal photons), as the reference does.  |p| is sqrt(x^2+y^2+z^2)    */
+/* (Julia's norm rescales by max|x_i| first; the two agree but in the last   */
+/* bit).  Layouts (Julia column-major): c1[K2]; c2[N,K2,K2]; c3[N,2N,K3,K3,K3]; */
+/* n1/n2/n3 (may be NULL) count the contributions with the same indices.     */
+/* ------------------------------------------------------------------------ */
+static double or_doubletFactor(int k1, int k2) { return k1 != k2 ? 2.0 : 1.0; }
+
+double or_angle_between(const double *p1, const double *p2)
+{
+    const double n1 = sqrt(p1[0] * p1[0] + p1[1] * p1[1] + p1[2] * p1[2]);
+    const double n2 = sqrt(p2[0] * p2[0] + p2[1] * p2[1] + p2[2] * p2[2]);
+    double val = (p1[0] * p2[0] + p1[1] * p2[1] + p1[2] * p2[2]) / (n1 * n2);
+    val = val < -1.0 ? -1.0 : (val > 1.0 ? 1.0 : val);
+    if (fabs(val) > 0.0) {
+        double angle = acos(val);
+        if (p1[0] * p2[1] - p1[1] * p2[0] > 0) angle = 2.0 * OR_PI - angle;
+        return angle;
+    }
+    return 0.0;
+}
+
+static int64_t or_fmod_floor(double x, int64_t m)
+{
+    int64_t f = (int64_t)floor(x);
+    int64_t r = f % m;
+    return r < 0 ? r + m : r;
+}
+
+void or_hist_pictures(const double *photons /* [n][3] */, const int64_t *offsets /* [npic+1] */, int npic,
+                      double dq, int N, int K2, int K3, double *c1, double *c2, double *c3, uint64_t *n1,
+                      uint64_t *n2, uint64_t *n3)
+{
+    const double da = OR_PI / N;
+    for (int pic = 0; pic < npic; ++pic) {
+        const double *ph = photons + 3 * offsets[pic];
+        const int64_t np = offsets[pic + 1] - offsets[pic];
+        int64_t *idx = (int64_t *)malloc(sizeof(int64_t) * (np > 0 ? np : 1));
+        double *nrm = (double *)malloc(sizeof(double) * (np > 0 ? np : 1));
+        int *kk = (int *)malloc(sizeof(int) * (np > 0 ? np : 1));
+        int64_t l = 0;
+        for (int64_t i = 0; i < np; ++i) {
+            const double *p = ph + 3 * i;
+            const double nm = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+            const double kr = nearbyint(nm / dq);
+            if (kr >= 1.0 && kr <= (double)K2) {
+                idx[l] = i;
+                nrm[l] = nm;
+                kk[l] = (int)kr;
+                ++l;
+            }
+        }
+        if (l >= 3) {
+            /* stable insertion sort by norm ascending */
+            for (int64_t a = 1; a < l; ++a) {
+                const int64_t ti = idx[a];
+                const double tn = nrm[a];
+                const int tk = kk[a];
+                int64_t b = a - 1;
+                while (b >= 0 && nrm[b] > tn) {
+                    idx[b + 1] = idx[b]; nrm[b + 1] = nrm[b]; kk[b + 1] = kk[b];
+                    --b;
+                }
+                idx[b + 1] = ti; nrm[b + 1] = tn; kk[b + 1] = tk;
+            }
+            for (int64_t i = 0; i < l; ++i) {
+                const double *p1 = ph + 3 * idx[i];
+                const int k1 = kk[i];
+                c1[k1 - 1] += 1.0 / k1;
+                if (n1) n1[k1 - 1] += 1;
+                for (int64_t j = 0; j < i; ++j) {
+                    const double *p2 = ph + 3 * idx[j];
+                    const int k2 = kk[j];
+                    const double alpha = or_angle_between(p1, p2);
+                    const double alpha_s = alpha > OR_PI ? 2.0 * OR_PI - alpha : alpha;
+                    const int64_t ai = or_fmod_floor(alpha_s / da, N);
+                    const int64_t o2 = ai + (int64_t)N * ((k2 - 1) + (int64_t)K2 * (k1 - 1));
+                    c2[o2] += 1.0 / (or_doubletFactor(k1, k2) * k1 * k2);
+                    if (n2) n2[o2] += 1;
+                    for (int64_t k = 0; k < j; ++k) {
+                        const int k3 = kk[k];
+                        if (k1 <= K3 && k2 <= K3 && k3 <= K3) {
+                            const double *p3 = ph + 3 * idx[k];
+                            double beta = or_angle_between(p1, p3);
+                            if (alpha > OR_PI) beta = 2.0 * OR_PI - beta;
+                            const int64_t bi = or_fmod_floor(beta / da, 2 * (int64_t)N);
+                            const int64_t o3 = ai + (int64_t)N * (bi + 2 * (int64_t)N * ((k3 - 1) +
+                                               (int64_t)K3 * ((k2 - 1) + (int64_t)K3 * (k1 - 1))));
+                            c3[o3] += 1.0 / (or_tripletFactor(k1, k2, k3) * k1 * k2 * k3);
+                            if (n3) n3[o3] += 1;
+                        }
+                    }
+                }
+            }
+        }
+        free(idx); free(nrm); free(kk);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Umat: spherical_harmonics.jl:445-457.  out: (2l+1)^2 complex, row-major   */
 /* [u+l][m+l], interleaved re/im.                                             */
 /* ------------------------------------------------------------------------ */

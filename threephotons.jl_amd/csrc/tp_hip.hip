@@ -19,6 +19,7 @@
 #include "tp_batch.hpp"
 #include "tp_mlp.hpp"
 #include "tp_grad.hpp"
+#include "tp_hist.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -1746,6 +1747,148 @@ int tp_energy_grad(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const doubl
             grad_out[o] = g[(size_t)k * b->n_c + c].x;
             grad_out[o + 1] = g[(size_t)k * b->n_c + c].y;
         }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Photon-pair / photon-triplet histograms (tp_hist.hpp)
+// ---------------------------------------------------------------------------
+struct tp_hist {
+    tp_ctx *ctx = nullptr;
+    int N = 0, K2 = 0, K3 = 0;
+    double dq = 0.0;
+    int64_t nkt = 0;                 // k-triples k1 >= k2 >= k3 of 1..K3
+    unsigned long long *d_n1 = nullptr, *d_n2 = nullptr, *d_n3 = nullptr;   // [K2], [N K2 K2], [nkt][2N][N]
+    double *d_ph = nullptr;          // staging of the photons
+    int64_t *d_off = nullptr;        // staging of the picture offsets
+    int64_t ph_cap = 0, off_cap = 0;
+    int64_t pictures = 0;
+};
+
+void tp_hist_destroy(tp_hist *h)
+{
+    if (!h) return;
+    if (h->ctx) (void)hipSetDevice(h->ctx->device);
+    void *ptrs[] = {h->d_n1, h->d_n2, h->d_n3, h->d_ph, h->d_off};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete h;
+}
+
+int tp_hist_create(tp_ctx *ctx, int N, int K2, int K3, double dq, tp_hist **out)
+{
+    if (!out) return tp_fail(TP_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (int rc = check_ctx(ctx)) return rc;
+    if (N < 1 || K2 < 1 || K3 < 1 || K3 > K2 || !(dq > 0.0)) return tp_fail(TP_E_ARG, "need N, K2 >= K3 >= 1 and dq > 0");
+    tp_hist *h = new tp_hist();
+    h->ctx = ctx;
+    h->N = N;
+    h->K2 = K2;
+    h->K3 = K3;
+    h->dq = dq;
+    h->nkt = (int64_t)K3 * (K3 + 1) * (K3 + 2) / 6;
+    int rc = 0;
+    if ((rc = dev_alloc(&h->d_n1, (size_t)K2)) || (rc = dev_alloc(&h->d_n2, (size_t)N * K2 * K2)) ||
+        (rc = dev_alloc(&h->d_n3, (size_t)h->nkt * 2 * N * N))) {
+        tp_hist_destroy(h);
+        return rc;
+    }
+    hipError_t e = hipMemset(h->d_n1, 0, sizeof(unsigned long long) * K2);
+    if (e == hipSuccess) e = hipMemset(h->d_n2, 0, sizeof(unsigned long long) * (size_t)N * K2 * K2);
+    if (e == hipSuccess) e = hipMemset(h->d_n3, 0, sizeof(unsigned long long) * (size_t)h->nkt * 2 * N * N);
+    if (e != hipSuccess) {
+        tp_hist_destroy(h);
+        return tp_fail(TP_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    }
+    *out = h;
+    return 0;
+}
+
+int tp_hist_add(tp_hist *h, const double *photons, const int64_t *offsets, int npictures)
+{
+    if (!h || !offsets || npictures < 0) return tp_fail(TP_E_ARG, "bad argument");
+    if (npictures == 0) return 0;
+    if (int rc = check_ctx(h->ctx)) return rc;
+    if (offsets[0] != 0) return tp_fail(TP_E_ARG, "offsets[0] must be 0");
+    for (int i = 0; i < npictures; ++i) {
+        const int64_t n = offsets[i + 1] - offsets[i];
+        if (n < 0) return tp_fail(TP_E_ARG, "offsets must not decrease");
+        if (n > kHistMaxPhotons) return tp_fail(TP_E_UNSUPPORTED, "a picture holds more than 512 photons");
+    }
+    const int64_t nph = offsets[npictures];
+    if (nph > 0 && !photons) return tp_fail(TP_E_ARG, "null photons");
+    if (3 * nph > h->ph_cap) {
+        if (h->d_ph) TP_HIP(hipFree(h->d_ph));
+        h->d_ph = nullptr;
+        h->ph_cap = 0;
+        if (int rc = dev_alloc(&h->d_ph, (size_t)std::max<int64_t>(3 * nph, 3))) return rc;
+        h->ph_cap = std::max<int64_t>(3 * nph, 3);
+    }
+    if (npictures + 1 > h->off_cap) {
+        if (h->d_off) TP_HIP(hipFree(h->d_off));
+        h->d_off = nullptr;
+        h->off_cap = 0;
+        if (int rc = dev_alloc(&h->d_off, (size_t)npictures + 1)) return rc;
+        h->off_cap = npictures + 1;
+    }
+    tp_ctx *ctx = h->ctx;
+    hipStream_t st = ctx->stream;
+    if (nph > 0) TP_HIP(hipMemcpyAsync(h->d_ph, photons, sizeof(double) * 3 * nph, hipMemcpyHostToDevice, st));
+    TP_HIP(hipMemcpyAsync(h->d_off, offsets, sizeof(int64_t) * (npictures + 1), hipMemcpyHostToDevice, st));
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
+    hipLaunchKernelGGL(k_hist_pictures, dim3((unsigned)npictures), dim3(kHistThreads), 0, st, h->d_ph, h->d_off, h->dq,
+                       h->N, h->K2, h->K3, h->d_n1, h->d_n2, h->d_n3);
+    TP_HIP(hipGetLastError());
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+    TP_HIP(hipStreamSynchronize(st));
+    if (ctx->timing) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->last_ms[0] = t;
+        for (int i = 1; i < 4; ++i) ctx->last_ms[i] = 0.0;
+    }
+    h->pictures += npictures;
+    return 0;
+}
+
+int tp_hist_read(tp_hist *h, double *c1, double *c2, double *c3, uint64_t *n1_out, uint64_t *n2_out,
+                 uint64_t *n3_out)
+{
+    if (!h) return tp_fail(TP_E_ARG, "null histogram");
+    if (int rc = check_ctx(h->ctx)) return rc;
+    const int N = h->N, K2 = h->K2, K3 = h->K3, R = 2 * N * N;
+    std::vector<unsigned long long> n1(K2), n2((size_t)N * K2 * K2), n3((size_t)h->nkt * R);
+    TP_HIP(hipMemcpy(n1.data(), h->d_n1, sizeof(unsigned long long) * n1.size(), hipMemcpyDeviceToHost));
+    TP_HIP(hipMemcpy(n2.data(), h->d_n2, sizeof(unsigned long long) * n2.size(), hipMemcpyDeviceToHost));
+    TP_HIP(hipMemcpy(n3.data(), h->d_n3, sizeof(unsigned long long) * n3.size(), hipMemcpyDeviceToHost));
+    for (int k1 = 1; k1 <= K2; ++k1) {
+        if (c1) c1[k1 - 1] = (double)n1[k1 - 1] * (1.0 / k1);
+        if (n1_out) n1_out[k1 - 1] = n1[k1 - 1];
+        for (int k2 = 1; k2 <= K2; ++k2) {
+            const double w = 1.0 / ((k1 != k2 ? 2.0 : 1.0) * k1 * k2);
+            for (int a = 0; a < N; ++a) {
+                const size_t o = a + (size_t)N * ((k2 - 1) + (size_t)K2 * (k1 - 1));
+                if (c2) c2[o] = (double)n2[o] * w;
+                if (n2_out) n2_out[o] = n2[o];
+            }
+        }
+    }
+    const size_t full = (size_t)R * K3 * K3 * K3;
+    if (c3) std::memset(c3, 0, sizeof(double) * full);
+    if (n3_out) std::memset(n3_out, 0, sizeof(uint64_t) * full);
+    int64_t kt = 0;
+    for (int k1 = 1; k1 <= K3; ++k1)
+        for (int k2 = 1; k2 <= k1; ++k2)
+            for (int k3 = 1; k3 <= k2; ++k3, ++kt) {
+                const double tf = (k1 == k2 && k2 == k3) ? 1.0 : ((k1 == k2 || k2 == k3 || k1 == k3) ? 3.0 : 6.0);
+                const double w = 1.0 / (tf * k1 * k2 * k3);
+                const size_t o = (size_t)R * ((k3 - 1) + (size_t)K3 * ((k2 - 1) + (size_t)K3 * (k1 - 1)));
+                for (int r = 0; r < R; ++r) {
+                    const unsigned long long c = n3[(size_t)kt * R + r];
+                    if (c3) c3[o + r] = (double)c * w;
+                    if (n3_out) n3_out[o + r] = c;
+                }
+            }
     return 0;
 }
 

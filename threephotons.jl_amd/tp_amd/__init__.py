@@ -329,6 +329,47 @@ def energy_grad(intensity, basis, c3ref, K3_range=None):
     return E.value, g
 
 
+class PhotonHistogram:
+    """Photon-pair / photon-triplet histograms of sparse images on the device
+    (tp_hist_*): add() is histogramCorrelationsInPicture_alltoall
+    (datagen.jl:228-282) over many pictures; read() gives c1, c2, c3 with the
+    reference's index order (c2[ai, k2, k1], c3[ai, bi, k3, k2, k1], zero-based)
+    and the integer counts behind them."""
+
+    def __init__(self, N, K2, K3, dq, ctx=None):
+        self.ctx = ctx or HIP_init()
+        self.N, self.K2, self.K3, self.dq = N, K2, K3, float(dq)
+        h = C.c_void_p()
+        check(lib().tp_hist_create(self.ctx.h, N, K2, K3, self.dq, C.byref(h)), "tp_hist_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().tp_hist_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def add(self, photons, offsets):
+        """photons (n, 3); picture i = photons[offsets[i]:offsets[i+1]]."""
+        ph = np.ascontiguousarray(photons, dtype=np.float64).reshape(-1, 3)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        check(lib().tp_hist_add(self.h, ph.ctypes.data, off.ctypes.data, len(off) - 1), "tp_hist_add")
+
+    def read(self):
+        N, K2, K3 = self.N, self.K2, self.K3
+        c1, c2, c3 = np.zeros(K2), np.zeros(N * K2 * K2), np.zeros(2 * N * N * K3 ** 3)
+        n1 = np.zeros(K2, np.uint64)
+        n2 = np.zeros(N * K2 * K2, np.uint64)
+        n3 = np.zeros(2 * N * N * K3 ** 3, np.uint64)
+        check(lib().tp_hist_read(self.h, c1.ctypes.data, c2.ctypes.data, c3.ctypes.data, n1.ctypes.data,
+                                 n2.ctypes.data, n3.ctypes.data), "tp_hist_read")
+        s2, s3 = (N, K2, K2), (N, 2 * N, K3, K3, K3)
+        return {"c1": c1, "c2": c2.reshape(s2, order="F"), "c3": c3.reshape(s3, order="F"), "n1": n1,
+                "n2": n2.reshape(s2, order="F"), "n3": n3.reshape(s3, order="F")}
+
+
 def energy_batch(intensities, bases, c3refs, K3_ranges=None):
     """Bayes energies of many evaluations in one call (tp_energy_batch): one per
     (intensity, basis, c3ref); bases share L, LMAX, N and precision, K may differ."""

@@ -76,6 +76,11 @@ def lib():
     L.tp_last_timings.argtypes = [vp, C.POINTER(dbl), i32]
     L.tp_energy_batch.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.tp_energy_grad.argtypes = [vp, vp, vp, vp, i64, vp, vp]
+    L.tp_hist_create.argtypes = [vp, i32, i32, i32, dbl, pp]
+    L.tp_hist_add.argtypes = [vp, vp, vp, i32]
+    L.tp_hist_read.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.tp_hist_destroy.argtypes = [vp]
+    L.tp_hist_destroy.restype = None
     L.tp_batch_create.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, i32, pp]
     L.tp_batch_destroy.argtypes = [vp]
     L.tp_batch_destroy.restype = None
@@ -138,6 +143,7 @@ EXPORTS = [
     "tp_mlp_set_params", "tp_mlp_forward", "tp_mlp_loss_grad", "tp_mlp_train", "tp_mlp_train_log", "tp_batch_log",
     "tp_batch_log_count", "tp_batch_log_read", "tp_batch_set_proposer", "tp_batch_set_amax", "tp_batch_set_contraction",
     "tp_batch_set_family", "tp_basis_from_host", "tp_energy_grad",
+    "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3

@@ -105,3 +105,20 @@ def poisson_histogram(c3raw, N, K, total, seed):
                 i += 1
     h = np.maximum(h, 1e-30)
     return h / h.sum(), counts
+
+
+def beamstop_pictures(npic, nphot, qcut, seed, beamstop_width=0.0):
+    """Sparse pictures as beamstop_shot makes them (datagen.jl:146-160): nphot
+    uniform points of [-qcut, qcut]^2 (z = 0) per picture, those with |x| or |y|
+    inside the beamstop and those beyond |p| = qcut dropped.  Returns (photons
+    (n, 3), offsets (npic + 1,))."""
+    g = rng(seed)
+    pts = 2.0 * qcut * (g.uniform(size=(npic, nphot, 2)) - 0.5)
+    keep = (np.abs(pts[..., 0]) > beamstop_width) & (np.abs(pts[..., 1]) > beamstop_width) & \
+           (np.hypot(pts[..., 0], pts[..., 1]) <= qcut)
+    counts = keep.sum(axis=1)
+    ph = np.zeros((int(counts.sum()), 3))
+    ph[:, :2] = pts[keep]
+    off = np.zeros(npic + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    return ph, off
