@@ -140,6 +140,12 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
     c3f0 = ob0.c3_factored(coeff0, nthreads=nthreads)
     E0 = po.energy(c3f0, sm["N"], ref0, sm["K"], 1, sm["K"])
     t_f0 = time.perf_counter() - t0
+    # photon-triplet histograms: the oracle on 1024 pictures of the histogram workload
+    hc = HIST_CFG
+    ph, off = workload.beamstop_pictures(1024, hc["nphot"], hc["qcut"], hc["seed"], beamstop_width=hc["qcut"] / 20.0)
+    t0 = time.perf_counter()
+    po.hist_pictures(ph, off, hc["qcut"] / hc["K2"], hc["N"], hc["K2"], hc["K3"])
+    t_hist = time.perf_counter() - t0
     tf = float(np.median(t_fact))
     return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
                 sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
@@ -147,6 +153,8 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
                 factored_evals_per_s=1.0 / tf,
                 factored_sample=f"3 full factorised evaluations (F.As + energy), median {tf:.2f}s",
                 mc_steps_per_s_factored=1.0 / (tf + t_move), move_seconds=t_move,
+                histogram_pictures_per_s=1024 / t_hist,
+                histogram_sample="or_hist_pictures (1 thread) on 1024 pictures of the histogram workload",
                 small_config={"dense_evals_per_s": 1.0 / t_d0, "factored_evals_per_s": 1.0 / t_f0,
                               "energy": E0, "klength": ob0.klength, "R": 2 * sm["N"] ** 2})
 
@@ -275,6 +283,40 @@ def _time_to(Es, t_of, target):
     """First time at which the energy trace reaches target (None if never)."""
     idx = np.nonzero(np.asarray(Es) <= target)[0]
     return None if idx.size == 0 else float(t_of(int(idx[0])))
+
+
+HIST_CFG = dict(N=32, K2=38, K3=26, qcut=38.0, nphot=100, seed=5)   # beamstop-only pictures of run_generate_histograms.jl
+
+
+def hist_leg(tp, workload, npic):
+    """Photon-pair / photon-triplet histograms (tp_hist_*) of npic synthetic
+    pictures: 100 uniform incident points of [-qcut, qcut]^2 per picture, the
+    beamstop (qcut/20) and |p| > qcut dropped (beamstop_shot, datagen.jl:146-160;
+    the beamstop-only runs of jobs/run_generate_histograms.jl:327), N = 32,
+    K2 = 38, K3 = 26, dq = qcut / K2 = 1.  Device time of the histogram kernel
+    (HIP events; photons already on the device) and the call including the
+    photon upload."""
+    c = HIST_CFG
+    ph, off = workload.beamstop_pictures(npic, c["nphot"], c["qcut"], c["seed"], beamstop_width=c["qcut"] / 20.0)
+    ctx = tp.HIP_init()
+    h = tp.PhotonHistogram(c["N"], c["K2"], c["K3"], c["qcut"] / c["K2"])
+    h.add(ph[:off[64]], off[:65])   # warm-up
+    h = tp.PhotonHistogram(c["N"], c["K2"], c["K3"], c["qcut"] / c["K2"])
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    h.add(ph, off)
+    wall = time.perf_counter() - t0
+    kern_ms = float(ctx.last_timings()[0])
+    ctx.set_timing(False)
+    r = h.read()
+    n1, n2, n3 = int(r["n1"].sum()), int(r["n2"].sum()), int(r["n3"].sum())
+    out = {"workload": "%d pictures x %d incident points (beamstop-only), N=32, K2=38, K3=26" % (npic, c["nphot"]),
+           "photons": n1, "pairs": n2, "triplets": n3, "kernel_ms": kern_ms, "call_s": wall,
+           "pictures_per_s": npic / (kern_ms * 1e-3), "triplets_per_s": n3 / (kern_ms * 1e-3),
+           "pictures_per_s_with_upload": npic / wall}
+    log(f"histograms: {npic} pictures ({n1} photons, {n3} triplets) in {kern_ms:.2f} ms of kernel "
+        f"({out['pictures_per_s']:.3g} pictures/s, {out['triplets_per_s']:.3g} triplets/s); call {1e3 * wall:.1f} ms")
+    return out
 
 
 def gradient_leg(tp, basis, c3ref, start, plain, nsteps, ncand=16):
@@ -410,6 +452,7 @@ def main():
     ap.add_argument("--learned-cmp-sweeps", type=int, default=40)
     ap.add_argument("--learned-tte-sweeps", type=int, default=200)
     ap.add_argument("--grad-steps", type=int, default=30, help="steps of the gradient-guided moves (0: skip)")
+    ap.add_argument("--hist-pictures", type=int, default=32768, help="pictures of the histogram leg (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -499,6 +542,7 @@ def main():
     if args.batch_sweeps > 0:
         batch = batch_leg(cfg, tp, workload, truth, basis, c3ref, args.batch_sweeps)
         basis.set_precision("fp64")
+    hist = hist_leg(tp, workload, args.hist_pictures) if args.hist_pictures > 0 else None
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:
@@ -556,6 +600,8 @@ def main():
         out["learned"] = learned
     if grad is not None:
         out["gradient_moves"] = grad
+    if hist is not None:
+        out["histograms"] = hist
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)
