@@ -139,6 +139,39 @@ function hip_energy_grad(intensity::SphericalHarmonicsVolume, basis::BasisTypeHi
     return E[], g
 end
 
+"""histogramCorrelationsInPicture_alltoall (datagen.jl:228-282) for many pictures at once on the
+GPU (tp_hist_*): the pictures' pair and triplet counts are added to c1, c2, c3 with the reference's
+weights.  lambda is unused there as in the reference."""
+function histogramCorrelationsInPicture_alltoall_hip(pictures::Vector{Vector{Vector{Float64}}}, c1::C1, c2::C2, c3::C3,
+                                                      dq::Float64, N::Int64, K2::Int64, K3::Int64, lambda::Float64=0.0)
+    h = Ref{Ptr{Void}}(C_NULL)
+    hip_check(ccall((:tp_hist_create, libtp), Cint, (Ptr{Void}, Cint, Cint, Cint, Cdouble, Ptr{Ptr{Void}}),
+                    hip_ctx, N, K2, K3, dq, h), "tp_hist_create")
+    try
+        n = sum(length, pictures)
+        ph = zeros(Float64, 3, n)
+        off = zeros(Int64, length(pictures) + 1)
+        i = 0
+        for (a, pic) in enumerate(pictures)
+            for p in pic
+                i += 1
+                ph[:, i] = p
+            end
+            off[a + 1] = i
+        end
+        hip_check(ccall((:tp_hist_add, libtp), Cint, (Ptr{Void}, Ptr{Float64}, Ptr{Int64}, Cint),
+                        h[], ph, off, length(pictures)), "tp_hist_add")
+        d1 = zeros(K2); d2 = zeros(N, K2, K2); d3 = zeros(N, 2*N, K3, K3, K3)
+        hip_check(ccall((:tp_hist_read, libtp), Cint,
+                        (Ptr{Void}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{UInt64}, Ptr{UInt64}, Ptr{UInt64}),
+                        h[], d1, d2, d3, C_NULL, C_NULL, C_NULL), "tp_hist_read")
+        c1 .+= d1; c2 .+= d2; c3 .+= d3
+    finally
+        ccall((:tp_hist_destroy, libtp), Void, (Ptr{Void},), h[])
+    end
+    return nothing
+end
+
 """Switch a basis to the mixed-precision contraction (1 = FP16 hi/lo, 2 = BF16, 0 = FP64)."""
 function hip_set_precision(basis::BasisTypeHip, mode::Integer)
     hip_check(ccall((:tp_basis_set_precision, libtp), Cint, (Ptr{Void}, Cint), basis.handle, mode),
