@@ -146,6 +146,14 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
     t0 = time.perf_counter()
     po.hist_pictures(ph, off, hc["qcut"] / hc["K2"], hc["N"], hc["K2"], hc["K3"])
     t_hist = time.perf_counter() - t0
+    # two-photon correlation and retrieveSolution (numpy oracle, 38 shells of the start structure)
+    from oracle import c2_oracle
+    t0 = time.perf_counter()
+    c2s = c2_oracle.two_photons(start.coeff, cfg["L"], cfg["LMAX"], cfg["N"], start.KMAX)
+    t_c2 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    c2_oracle.retrieve_solution(c2s, cfg["L"], cfg["LMAX"], 1, start.KMAX)
+    t_ret = time.perf_counter() - t0
     tf = float(np.median(t_fact))
     return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
                 sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
@@ -155,6 +163,8 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
                 mc_steps_per_s_factored=1.0 / (tf + t_move), move_seconds=t_move,
                 histogram_pictures_per_s=1024 / t_hist,
                 histogram_sample="or_hist_pictures (1 thread) on 1024 pictures of the histogram workload",
+                two_photons_s=t_c2, retrieve_s=t_ret,
+                two_photon_sample="numpy oracle: twoPhotons of the 38-shell start structure and retrieveSolution of it",
                 small_config={"dense_evals_per_s": 1.0 / t_d0, "factored_evals_per_s": 1.0 / t_f0,
                               "energy": E0, "klength": ob0.klength, "R": 2 * sm["N"] ** 2})
 
@@ -319,6 +329,41 @@ def hist_leg(tp, workload, npic):
     return out
 
 
+def c2_leg(tp, truth, basis, reps):
+    """Two-photon correlation of the full structure (twoPhotons, correlations.jl:229-243:
+    38 shells, L = 18, N = 32) on the device, and retrieveSolution
+    (correlations.jl:262-319, lambda = 0, host) of that c2 over shells 1..38.
+    Kernel time by HIP events; the call includes the coefficient upload and the
+    c2 download.  The round-trip residual |c2(retrieved) - c2| / max|c2| is
+    reported: it is not 0 for this structure, which has l > 2k - 2 content on
+    its inner shells that the reference's fit leaves out by design."""
+    K2 = truth.KMAX
+    ctx = tp.HIP_init()
+    c2 = tp.twoPhotons(truth, basis, K2)
+    ctx.set_timing(True)
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tp.twoPhotons(truth, basis, K2)
+        ks.append(ctx.last_timings()[0])
+    call = (time.perf_counter() - t0) / reps
+    ctx.set_timing(False)
+    t0 = time.perf_counter()
+    vol, info = tp.retrieveSolution(c2, basis.L, truth.LMAX, K2, truth.rmax, return_info=True)
+    t_ret = time.perf_counter() - t0
+    back = tp.twoPhotons(vol, basis, K2)
+    resid = float(np.abs(back - c2).max() / np.abs(c2).max())
+    kern = float(np.mean(ks))
+    out_bytes = 8.0 * basis.N * K2 * K2
+    out = {"workload": f"twoPhotons of {K2} shells, L={basis.L}, N={basis.N}; retrieveSolution over 1:{K2}",
+           "kernel_ms": kern, "call_ms": 1e3 * call, "c2_write_GBps": out_bytes / (kern * 1e-3) / 1e9,
+           "retrieve_s": t_ret, "retrieve_flipped": info["flipped"], "retrieve_negativity": info["negativity"],
+           "round_trip_residual": resid}
+    log(f"two-photon: kernel {1e3 * kern:.1f} us, call {1e3 * call:.2f} ms; retrieveSolution {1e3 * t_ret:.1f} ms "
+        f"(flipped {info['flipped']}, round-trip residual {resid:.3g})")
+    return out
+
+
 def gradient_leg(tp, basis, c3ref, start, plain, nsteps, ncand=16):
     """Gradient-guided moves (tp_amd.descent) on the plain run's problem (K=26,
     its start structure and histogram): per step one FP64 energy + gradient
@@ -453,6 +498,7 @@ def main():
     ap.add_argument("--learned-tte-sweeps", type=int, default=200)
     ap.add_argument("--grad-steps", type=int, default=30, help="steps of the gradient-guided moves (0: skip)")
     ap.add_argument("--hist-pictures", type=int, default=32768, help="pictures of the histogram leg (0: skip)")
+    ap.add_argument("--c2-reps", type=int, default=20, help="timed twoPhotons calls of the two-photon leg (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -543,6 +589,7 @@ def main():
         batch = batch_leg(cfg, tp, workload, truth, basis, c3ref, args.batch_sweeps)
         basis.set_precision("fp64")
     hist = hist_leg(tp, workload, args.hist_pictures) if args.hist_pictures > 0 else None
+    c2 = c2_leg(tp, truth, basis, args.c2_reps) if args.c2_reps > 0 else None
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:
@@ -602,6 +649,8 @@ def main():
         out["gradient_moves"] = grad
     if hist is not None:
         out["histograms"] = hist
+    if c2 is not None:
+        out["two_photon"] = c2
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)

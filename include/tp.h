@@ -198,6 +198,28 @@ int tp_hist_add(tp_hist *h, const double *photons, const int64_t *offsets, int n
 int tp_hist_read(tp_hist *h, double *c1, double *c2, double *c3, uint64_t *n1, uint64_t *n2, uint64_t *n3);
 void tp_hist_destroy(tp_hist *h);
 
+/* Two-photon correlation of a structure: twoPhotons (correlations.jl:229-243),
+ * lambda = 0.  coeff: K2 shells of ld complex (interleaved) coefficients in the
+ * seanindex layout; the l range and N are the basis's.  c2_out[N,K2,K2] in the
+ * reference's (column-major) layout, c2[a,k2,k1] = sum_{l even <= L}
+ * Re(c_k1,l . c_k2,l) P_l(cos alpha_a) / (4 pi), alpha_a = (a + 1/2) pi / N;
+ * minimal != 0 computes only k2 <= k1 (0 elsewhere); normalize != 0 divides by
+ * sum |c2| instead of by 4 pi.  One kernel launch (timed by tp_set_timing). */
+int tp_two_photons(tp_ctx *ctx, const tp_basis *basis, const double *coeff, int64_t ld, int K2, int minimal,
+                   int normalize, double *c2_out);
+
+/* A start structure from a two-photon histogram: retrieveSolution
+ * (correlations.jl:262-319), lambda = 0, host code (no GPU call).  c2[N,K2dim,K2dim]
+ * in the reference's layout; shells k2lo..k2hi (1-based) are fitted, with
+ * L < (k2hi - k2lo + 1)/2 and L <= LMAX - 1.  coeff_out: k2hi shells of ld
+ * complex (interleaved) coefficients, zero outside the fitted shells and the
+ * even l <= L.  Eigenvector signs, which LAPACK leaves open, are fixed so that
+ * each eigenvector's entry of largest magnitude is positive.  flipped (may be
+ * NULL) = 1 if the negativity check changed every sign; negativity (may be
+ * NULL) = the summed shell negativity before that change. */
+int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi, int L, int LMAX,
+                         double *coeff_out, int64_t ld, int *flipped, double *negativity);
+
 #define TP_MOVE_QUAT 0    /* random slots: an SO(3) rotation per even l (uniform axis, Haar angle) */
 #define TP_MOVE_PLANE 1   /* random slots: the reference's move, a rotation in the plane of a
                            * Haar-random 2-frame of R^(2l+1) (default) */

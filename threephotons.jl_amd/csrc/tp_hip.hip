@@ -20,6 +20,8 @@
 #include "tp_mlp.hpp"
 #include "tp_grad.hpp"
 #include "tp_hist.hpp"
+#include "tp_c2.hpp"
+#include "tp_retrieve.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -159,6 +161,11 @@ struct tp_basis {
     int *d_lstart = nullptr;       // [L/2 + 2]
     int *d_llist = nullptr;        // [3 n_lt] lt * 3 + slot, grouped by the slot's l
     int *d_bwd_order = nullptr;    // [n_lt] l-triples by decreasing backward cost
+    // two-photon correlation (tp_two_photons), grown on demand
+    int64_t c2_shells = 0, c2_len = 0;
+    double2 *d_c2coef = nullptr;   // [c2_shells][n_c]
+    double2 *d_c2raw = nullptr;    // [c2_shells][LMAX^2]
+    double *d_c2 = nullptr;        // [c2_len] = N K2 K2
 };
 
 struct tp_ref {
@@ -1089,7 +1096,7 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
-                    b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order};
+                    b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1889,6 +1896,72 @@ int tp_hist_read(tp_hist *h, double *c1, double *c2, double *c3, uint64_t *n1_ou
                     if (n3_out) n3_out[o + r] = c;
                 }
             }
+    return 0;
+}
+
+int tp_two_photons(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, int K2, int minimal,
+                   int normalize, double *c2_out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!bc || !c2_out) return tp_fail(TP_E_ARG, "null argument");
+    tp_basis *b = const_cast<tp_basis *>(bc);
+    if (b->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
+    if (K2 < 1) return tp_fail(TP_E_ARG, "need K2 >= 1");
+    const int N = b->N, nl = b->L / b->lstep + 1;
+    const size_t lds = two_photons_lds_bytes(b->n_c, N, nl);
+    if (lds > 64 * 1024) return tp_fail(TP_E_UNSUPPORTED, "two-photon tables exceed 64 KiB of LDS (N or L too large)");
+    const int64_t len = (int64_t)N * K2 * K2;
+    if (K2 > b->c2_shells) {
+        if (b->d_c2coef) TP_HIP(hipFree(b->d_c2coef));
+        if (b->d_c2raw) TP_HIP(hipFree(b->d_c2raw));
+        b->d_c2coef = nullptr;
+        b->d_c2raw = nullptr;
+        b->c2_shells = 0;
+        if (int rc = dev_alloc(&b->d_c2coef, (size_t)K2 * b->n_c)) return rc;
+        if (int rc = dev_alloc(&b->d_c2raw, (size_t)K2 * b->LMAX * b->LMAX)) return rc;
+        b->c2_shells = K2;
+    }
+    if (len > b->c2_len) {
+        if (b->d_c2) TP_HIP(hipFree(b->d_c2));
+        b->d_c2 = nullptr;
+        b->c2_len = 0;
+        if (int rc = dev_alloc(&b->d_c2, (size_t)len)) return rc;
+        b->c2_len = len;
+    }
+    // upload_coeff stages through d_c2raw only for ld == LMAX^2 and K2 <= K
+    // shells; otherwise it allocates its own staging buffer.
+    if (int rc = upload_coeff(ctx, b, coeff, ld, K2, b->d_c2coef, K2 <= b->K ? b->d_c2raw : nullptr)) return rc;
+    hipStream_t st = ctx->stream;
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
+    hipLaunchKernelGGL(k_two_photons, dim3((unsigned)K2, (unsigned)((K2 + kC2Chunk - 1) / kC2Chunk)), dim3(256), lds, st,
+                       b->d_c2coef, b->n_c, b->d_coff, b->L,
+                       b->lstep, N, K2, minimal ? 1 : 0, normalize ? 1.0 : 1.0 / (4.0 * tph::kPi), b->d_c2);
+    TP_HIP(hipGetLastError());
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+    TP_HIP(hipMemcpyAsync(c2_out, b->d_c2, sizeof(double) * (size_t)len, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    if (ctx->timing) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->last_ms[0] = t;
+        for (int i = 1; i < 4; ++i) ctx->last_ms[i] = 0.0;
+    }
+    if (normalize) {   // c / sum(abs, c)
+        double s = 0.0;
+        for (int64_t i = 0; i < len; ++i) s += std::fabs(c2_out[i]);
+        for (int64_t i = 0; i < len; ++i) c2_out[i] /= s;
+    }
+    return 0;
+}
+
+int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi, int L, int LMAX, double *coeff_out,
+                         int64_t ld, int *flipped, double *negativity)
+{
+    if (!c2 || !coeff_out) return tp_fail(TP_E_ARG, "null argument");
+    if (N < 1 || K2dim < 1 || LMAX < 1) return tp_fail(TP_E_ARG, "need N, K2, LMAX >= 1");
+    const std::string err = tpc2::retrieve_solution(c2, N, K2dim, k2lo, k2hi, L, LMAX,
+                                                    reinterpret_cast<std::complex<double> *>(coeff_out), ld, flipped,
+                                                    negativity);
+    if (!err.empty()) return tp_fail(TP_E_ARG, err);
     return 0;
 }
 

@@ -25,7 +25,7 @@ from ._lib import PREC_BF16, PREC_FP16X3, PREC_FP64, AnnealParams, TPError, buil
 __all__ = [
     "TPError", "build", "lib", "SphericalHarmonicsVolume", "HIP_init", "HipContext", "BasisTypeHip",
     "HIP_calculate_basis", "FullCorrelation_parallized", "energy", "DeviceState", "rotate_all_at_once",
-    "intensity_from_atoms", "seanindex", "tripletFactor", "klength",
+    "intensity_from_atoms", "seanindex", "tripletFactor", "klength", "twoPhotons", "retrieveSolution",
 ]
 
 
@@ -368,6 +368,41 @@ class PhotonHistogram:
         s2, s3 = (N, K2, K2), (N, 2 * N, K3, K3, K3)
         return {"c1": c1, "c2": c2.reshape(s2, order="F"), "c3": c3.reshape(s3, order="F"), "n1": n1,
                 "n2": n2.reshape(s2, order="F"), "n3": n3.reshape(s3, order="F")}
+
+
+def twoPhotons(volume, basis, K2, minimal=False, normalize=False):
+    """Two-photon correlation c2 (N, K2, K2), indexed [a, k2, k1] (zero-based),
+    of shells 1..K2 of `volume` on the GPU (tp_two_photons; correlations.jl:229-243,
+    lambda = 0).  The l range and N are the basis's."""
+    c, ld = _coeff_ptr(volume, basis, K2)
+    out = np.zeros(basis.N * K2 * K2)
+    check(lib().tp_two_photons(basis.ctx.h, basis.h, c.ctypes.data, ld, K2, int(bool(minimal)),
+                               int(bool(normalize)), out.ctypes.data), "tp_two_photons")
+    return out.reshape((basis.N, K2, K2), order="F")
+
+
+def retrieveSolution(c2, L, LMAX, K2_range, qmax, lam=0.0, return_info=False):
+    """A start structure from a two-photon correlation (tp_retrieve_solution;
+    correlations.jl:262-319): a SphericalHarmonicsVolume of max(K2_range)
+    shells whose c2 matches c2 ([a, k2, k1]) on K2_range.  K2_range is an int
+    (1..K2) or a range of 1-based shells; only lambda = 0 is supported."""
+    if lam != 0.0:
+        raise NotImplementedError("retrieveSolution: only lambda = 0 (flat detector)")
+    rng = range(1, K2_range + 1) if isinstance(K2_range, int) else K2_range
+    lo, hi = min(rng), max(rng)
+    c2 = np.asarray(c2, dtype=np.float64)
+    N, K2a, K2b = c2.shape
+    if K2a != K2b:
+        raise ValueError("c2 must be (N, K2, K2)")
+    flat = np.ascontiguousarray(c2.ravel(order="F"))
+    out = np.zeros((hi, LMAX * LMAX), np.complex128)
+    flipped, neg = C.c_int(), C.c_double()
+    check(lib().tp_retrieve_solution(flat.ctypes.data, N, K2a, lo, hi, L, LMAX, out.ctypes.data, LMAX * LMAX,
+                                     C.byref(flipped), C.byref(neg)), "tp_retrieve_solution")
+    vol = SphericalHarmonicsVolume(out, LMAX, hi, qmax)
+    if return_info:
+        return vol, {"flipped": bool(flipped.value), "negativity": neg.value}
+    return vol
 
 
 def energy_batch(intensities, bases, c3refs, K3_ranges=None):

@@ -81,6 +81,8 @@ def lib():
     L.tp_hist_read.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.tp_hist_destroy.argtypes = [vp]
     L.tp_hist_destroy.restype = None
+    L.tp_two_photons.argtypes = [vp, vp, vp, i64, i32, i32, i32, vp]
+    L.tp_retrieve_solution.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp, i64, vp, vp]
     L.tp_batch_create.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, i32, pp]
     L.tp_batch_destroy.argtypes = [vp]
     L.tp_batch_destroy.restype = None
@@ -143,7 +145,7 @@ EXPORTS = [
     "tp_mlp_set_params", "tp_mlp_forward", "tp_mlp_loss_grad", "tp_mlp_train", "tp_mlp_train_log", "tp_batch_log",
     "tp_batch_log_count", "tp_batch_log_read", "tp_batch_set_proposer", "tp_batch_set_amax", "tp_batch_set_contraction",
     "tp_batch_set_family", "tp_basis_from_host", "tp_energy_grad",
-    "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy",
+    "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy", "tp_two_photons", "tp_retrieve_solution",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
