@@ -8,10 +8,12 @@
 #   HIP_calculate_basis(...)           replaces CUDA_calculate_basis (cuda.jl:57-69)
 #   FullCorrelation_parallized(::SphericalHarmonicsVolume, ::BasisTypeHip, ...)   (cuda.jl:74-108)
 #   energy(::SphericalHarmonicsVolume, ::BasisTypeHip, ::C3, ::UnitRange, ...)    (correlations.jl:201-226)
+#   twoPhotons(::SphericalHarmonicsVolume, ::BasisTypeHip, K2, ...)               (correlations.jl:229-243)
+#   retrieveSolution_hip(c2, L, LMAX, K2_range, qmax)                             (correlations.jl:262-319)
 #
 # Written against Julia 0.6 syntax like the rest of the package.
 
-export BasisTypeHip, HIP_init, HIP_calculate_basis
+export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip
 
 const libtp = get(ENV, "THREEPHOTONS_LIBTP", joinpath(dirname(@__FILE__), "..", "libtp.so"))
 HIP_enabled = false
@@ -170,6 +172,34 @@ function histogramCorrelationsInPicture_alltoall_hip(pictures::Vector{Vector{Vec
         ccall((:tp_hist_destroy, libtp), Void, (Ptr{Void},), h[])
     end
     return nothing
+end
+
+"""twoPhotons (correlations.jl:229-243) on the GPU: c2[:, k2, k1] of shells 1:K2 over the basis's
+l range and alpha grid (lambda = 0)."""
+function twoPhotons(volume::SphericalHarmonicsVolume, basis::BasisTypeHip, K2::Int64, minimal::Bool=false, normalize::Bool=false)
+    coeff = hip_pack(volume, K2)
+    c = zeros(Float64, basis.N, K2, K2)
+    hip_check(ccall((:tp_two_photons, libtp), Cint,
+                    (Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Cint, Cint, Cint, Ptr{Float64}),
+                    hip_ctx, basis.handle, coeff, size(coeff, 1), K2, minimal, normalize, c), "tp_two_photons")
+    return c
+end
+
+"""retrieveSolution (correlations.jl:262-319) through libtp (host code, lambda = 0 only):
+the start structure whose two-photon correlation matches c2 on K2_range."""
+function retrieveSolution_hip(c2::C2, L::Int64, LMAX::Int64, K2_range::UnitRange{Int64}, qmax::Float64, lambda::Float64=0.0)
+    @assert lambda == 0.0
+    N, K2dim, _ = size(c2)
+    hi = maximum(K2_range)
+    coeff = zeros(Complex{Float64}, LMAX^2, hi)
+    flipped = Ref{Cint}(0); neg = Ref{Cdouble}(0.0)
+    hip_check(ccall((:tp_retrieve_solution, libtp), Cint,
+                    (Ptr{Float64}, Cint, Cint, Cint, Cint, Cint, Cint, Ptr{Complex{Float64}}, Int64, Ptr{Cint}, Ptr{Cdouble}),
+                    Array{Float64}(c2), N, K2dim, minimum(K2_range), hi, L, LMAX, coeff, LMAX^2, flipped, neg),
+              "tp_retrieve_solution")
+    intensity = SphericalHarmonicsVolume(LMAX, hi, qmax)
+    for k = 1:hi intensity.coeff[k][:] = coeff[:, k] end
+    return intensity
 end
 
 """Switch a basis to the mixed-precision contraction (1 = FP16 hi/lo, 2 = BF16, 0 = FP64)."""
