@@ -154,6 +154,13 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
     t0 = time.perf_counter()
     c2_oracle.retrieve_solution(c2s, cfg["L"], cfg["LMAX"], 1, start.KMAX)
     t_ret = time.perf_counter() - t0
+    # rotational fit: the oracle's similarity of two rotated copies (38 shells, L = 18, LMAX = 25)
+    from oracle import fit_oracle
+    t0 = time.perf_counter()
+    for e in ((0.3, 0.2, 0.1), (1.0, 2.0, 0.5)):
+        rs = fit_oracle.rotate_structure(start.coeff, *e, start.KMAX, range(2, cfg["L"] + 1, 2), cfg["LMAX"])
+        fit_oracle.similarity(start.coeff, rs, cfg["LMAX"], 1, start.KMAX)
+    t_fit = (time.perf_counter() - t0) / 2
     tf = float(np.median(t_fact))
     return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
                 sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
@@ -165,6 +172,8 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
                 histogram_sample="or_hist_pictures (1 thread) on 1024 pictures of the histogram workload",
                 two_photons_s=t_c2, retrieve_s=t_ret,
                 two_photon_sample="numpy oracle: twoPhotons of the 38-shell start structure and retrieveSolution of it",
+                fit_rotations_per_s=1.0 / t_fit,
+                fit_sample="numpy oracle: rotateStructure + similarity of 2 rotations, 38 shells, L=18, LMAX=25",
                 small_config={"dense_evals_per_s": 1.0 / t_d0, "factored_evals_per_s": 1.0 / t_f0,
                               "energy": E0, "klength": ob0.klength, "R": 2 * sm["N"] ** 2})
 
@@ -364,6 +373,43 @@ def c2_leg(tp, truth, basis, reps):
     return out
 
 
+def fit_leg(tp, truth, num):
+    """fitStructures_full (data_processing.jl:26-46) on the device: the 38-shell
+    structure against a rotated copy of itself, L = 18, LMAX = 25, similarity
+    over shells 1..38 for every one of (num+1)^3 Euler triples of the
+    reference's default ranges; the rotation and similarity kernels timed with
+    HIP events, the call end to end (quaternions, upload, both kernels, copy)."""
+    from tp_amd import fit
+    L, K = 18, truth.KMAX
+    b = tp.HIP_calculate_basis(L, truth.LMAX, 2, 1)
+    t, p, g = np.pi / 4, np.pi / 2, 7 * np.pi / 20      # a point of the num = 20 grid
+    vol = fit.rotateStructure(truth, -t, -g, -p, K, L, b)
+    ctx = tp.HIP_init()
+    sf = fit.StructureFit(b, truth, K).set_volume(vol)
+    sf.similarity(np.zeros((64, 3)))
+    th = np.linspace(0.0, np.pi / 2, num + 1)
+    ph = np.linspace(0.0, np.pi, num + 1)
+    T, P, G = np.meshgrid(th, ph, ph, indexing="ij")
+    euler = np.stack([T.ravel(), P.ravel(), G.ravel()], axis=1)
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    sc = sf.similarity(euler)
+    wall = time.perf_counter() - t0
+    rot_ms, sim_ms = ctx.last_timings()[:2]
+    ctx.set_timing(False)
+    i = int(np.argmax(sc))
+    n = len(euler)
+    B = truth.LMAX
+    flops = 8.0 * 2 * B * (B * (B + 1) / 2) * K * n      # 4 FMAs per (theta_j, |m|, l) per shell
+    out = {"workload": f"fitStructures_full, num={num}: {n} rotations x {K} shells, L={L}, LMAX={B}",
+           "rotations": n, "call_s": wall, "rotations_per_s": n / wall, "rotate_kernel_ms": rot_ms,
+           "similarity_kernel_ms": sim_ms, "similarity_tflops": flops / (sim_ms * 1e-3) / 1e12,
+           "best": float(sc[i]), "best_euler": [float(x) for x in euler[i]]}
+    log(f"fit: {n} rotations in {1e3 * wall:.1f} ms ({n / wall:.0f}/s; rotate {rot_ms:.2f} ms, similarity "
+        f"{sim_ms:.2f} ms = {out['similarity_tflops']:.1f} TFLOP/s), best {sc[i]:.12f} at {np.round(euler[i], 4)}")
+    return out
+
+
 def gradient_leg(tp, basis, c3ref, start, plain, nsteps, ncand=16):
     """Gradient-guided moves (tp_amd.descent) on the plain run's problem (K=26,
     its start structure and histogram): per step one FP64 energy + gradient
@@ -499,6 +545,7 @@ def main():
     ap.add_argument("--grad-steps", type=int, default=30, help="steps of the gradient-guided moves (0: skip)")
     ap.add_argument("--hist-pictures", type=int, default=32768, help="pictures of the histogram leg (0: skip)")
     ap.add_argument("--c2-reps", type=int, default=20, help="timed twoPhotons calls of the two-photon leg (0: skip)")
+    ap.add_argument("--fit-num", type=int, default=20, help="grid divisions of the rotational-fit leg (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -590,6 +637,7 @@ def main():
         basis.set_precision("fp64")
     hist = hist_leg(tp, workload, args.hist_pictures) if args.hist_pictures > 0 else None
     c2 = c2_leg(tp, truth, basis, args.c2_reps) if args.c2_reps > 0 else None
+    fitr = fit_leg(tp, truth, args.fit_num) if args.fit_num > 0 else None
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:
@@ -651,6 +699,8 @@ def main():
         out["histograms"] = hist
     if c2 is not None:
         out["two_photon"] = c2
+    if fitr is not None:
+        out["rotational_fit"] = fitr
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)

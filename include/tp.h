@@ -220,6 +220,24 @@ int tp_two_photons(tp_ctx *ctx, const tp_basis *basis, const double *coeff, int6
 int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi, int L, int LMAX,
                          double *coeff_out, int64_t ld, int *flipped, double *negativity);
 
+/* Rotational fit of a structure to a reference: the scoring inside
+ * fitStructures_full / fitStructures_random (data_processing.jl:26-79).
+ * tp_fit_create takes the reference (k_hi shells of ld complex coefficients),
+ * the shell range k_lo..k_hi (1-based) and a basis for L and LMAX (2 <= L <= 22,
+ * LMAX <= 32); tp_fit_set_volume sets the structure to rotate (k_hi shells);
+ * tp_fit_similarity writes, for each of n Euler triples (theta, phi, gamma),
+ * sc_out[i] = similarity(reference, rotateStructure(volume, theta, phi, gamma,
+ * k_hi, 2:2:L), k_lo:k_hi) (spherical_harmonics.jl:426-435, 598-612): the mean
+ * over the shells of Pearson's correlation of the real surfaces on the
+ * (2 LMAX)^2 grid of the inverse transform.  With timing on, tp_last_timings
+ * gives the rotation and similarity kernel times of the last call. */
+typedef struct tp_fit tp_fit;
+int tp_fit_create(tp_ctx *ctx, const tp_basis *basis, const double *reference, int64_t ld, int k_lo, int k_hi,
+                  tp_fit **out);
+int tp_fit_set_volume(tp_fit *f, const double *coeff, int64_t ld);
+int tp_fit_similarity(tp_fit *f, int n, const double *euler, double *sc_out);
+void tp_fit_destroy(tp_fit *f);
+
 #define TP_MOVE_QUAT 0    /* random slots: an SO(3) rotation per even l (uniform axis, Haar angle) */
 #define TP_MOVE_PLANE 1   /* random slots: the reference's move, a rotation in the plane of a
                            * Haar-random 2-frame of R^(2l+1) (default) */

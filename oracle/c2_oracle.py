@@ -56,20 +56,30 @@ def _fix_sign(V):
     return V * s
 
 
+_YTAB = {}
+
+
+def _ylm_table(LMAX):
+    """Ylm at every point of the (2 LMAX)^2 grid, rows in seanindex order (cached)."""
+    if LMAX not in _YTAB:
+        S = 2 * LMAX
+        th = np.pi * (2 * np.arange(S) + 1) / (4 * LMAX)
+        ph = 2 * np.pi * np.arange(S) / S
+        T, P = np.meshgrid(th, ph, indexing="ij")
+        Y = np.zeros((LMAX * LMAX, S, S), np.complex128)
+        for l in range(LMAX):
+            for m in range(-l, l + 1):
+                Y[seanindex(l, m, LMAX) - 1] = sph_harm_y(l, m, T, P)
+        _YTAB[LMAX] = Y
+    return _YTAB[LMAX]
+
+
 def surface(c, LMAX):
     """Values of one shell's series sum c_lm Ylm on the (2 LMAX)^2 grid
-    theta_j = pi (2j+1)/(4 LMAX), phi_i = 2 pi i/(2 LMAX)."""
-    S = 2 * LMAX
-    th = np.pi * (2 * np.arange(S) + 1) / (4 * LMAX)
-    ph = 2 * np.pi * np.arange(S) / S
-    T, P = np.meshgrid(th, ph, indexing="ij")
-    f = np.zeros((S, S), np.complex128)
-    for l in range(LMAX):
-        for m in range(-l, l + 1):
-            v = c[seanindex(l, m, LMAX) - 1]
-            if v != 0:
-                f += v * sph_harm_y(l, m, T, P)
-    return f
+    theta_j = pi (2j+1)/(4 LMAX), phi_i = 2 pi i/(2 LMAX) (Ylm from scipy's
+    sph_harm_y at each grid point)."""
+    Y = _ylm_table(LMAX)
+    return np.tensordot(np.asarray(c[:LMAX * LMAX], dtype=np.complex128), Y, axes=(0, 0))
 
 
 def negativity(coeff, LMAX, lo, hi):

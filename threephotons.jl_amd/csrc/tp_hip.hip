@@ -22,6 +22,7 @@
 #include "tp_hist.hpp"
 #include "tp_c2.hpp"
 #include "tp_retrieve.hpp"
+#include "tp_fit.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -1962,6 +1963,213 @@ int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi,
                                                     reinterpret_cast<std::complex<double> *>(coeff_out), ld, flipped,
                                                     negativity);
     if (!err.empty()) return tp_fail(TP_E_ARG, err);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// rotational fit (fitStructures_*, data_processing.jl:26-79)
+// ---------------------------------------------------------------------------
+struct tp_fit {
+    tp_ctx *ctx = nullptr;
+    tp_basis *b = nullptr;
+    int K = 0, klo = 0, nk = 0;       // shells 1..K rotated, similarity over klo+1..K
+    double *d_P = nullptr;            // [LMAX][LMAX][2 LMAX] sphPlm(l, m, theta_j) at ((m LMAX + l) 2 LMAX + j)
+    int *d_sean = nullptr;            // [LMAX^2] seanindex - 1 of (l, m) at l^2 + l + m
+    double2 *d_gref = nullptr;        // [nk][2 LMAX - 1][2 LMAX] reference g_j(m)
+    double *d_refsum = nullptr;       // [nk][2] sum Re h, sum (Re h)^2
+    double2 *d_vol = nullptr;         // [K][LMAX^2] the structure (full layout)
+    double2 *d_volc = nullptr;        // [K][n_c] the structure (compact)
+    bool have_vol = false;
+    int cap = 0;                      // rotations per chunk the buffers hold
+    double2 *d_rot = nullptr;         // [cap][K][n_c]
+    double *d_quat = nullptr;         // [cap][L/2][4]
+    EvalDesc *d_ev = nullptr;         // [cap]
+    double *d_cor = nullptr;          // [cap][nk]
+};
+
+void tp_fit_destroy(tp_fit *f)
+{
+    if (!f) return;
+    if (f->ctx) (void)hipSetDevice(f->ctx->device);
+    void *ptrs[] = {f->d_P, f->d_sean, f->d_gref, f->d_refsum, f->d_vol, f->d_volc, f->d_rot, f->d_quat, f->d_ev,
+                    f->d_cor};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete f;
+}
+
+// Copy `nshell` shells of host coefficients (stride ld complex) to a device
+// [nshell][LMAX^2] array.
+static int upload_full(tp_ctx *ctx, const double *coeff, int64_t ld, int nshell, int LMAX, double2 *dst)
+{
+    const size_t w = sizeof(double2) * (size_t)LMAX * LMAX;
+    TP_HIP(hipMemcpy2DAsync(dst, w, coeff, sizeof(double2) * (size_t)ld, w, (size_t)nshell, hipMemcpyHostToDevice,
+                            ctx->stream));
+    return 0;
+}
+
+int tp_fit_create(tp_ctx *ctx, const tp_basis *bc, const double *reference, int64_t ld, int k_lo, int k_hi,
+                  tp_fit **out)
+{
+    if (!out) return tp_fail(TP_E_ARG, "null output pointer");
+    *out = nullptr;
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!bc || !reference) return tp_fail(TP_E_ARG, "null argument");
+    if (bc->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
+    tp_basis *b = const_cast<tp_basis *>(bc);
+    const int LMAX = b->LMAX;
+    if (b->L < 2 || b->L > kRotMaxL) return tp_fail(TP_E_UNSUPPORTED, "rotational fit needs 2 <= L <= 22");
+    if (LMAX > kFitMaxLMAX) return tp_fail(TP_E_UNSUPPORTED, "rotational fit needs LMAX <= 32");
+    if (k_lo < 1 || k_hi < k_lo) return tp_fail(TP_E_ARG, "need 1 <= k_lo <= k_hi");
+    if (ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "ld shorter than LMAX^2");
+    tp_fit *f = new tp_fit();
+    f->ctx = ctx;
+    f->b = b;
+    f->K = k_hi;
+    f->klo = k_lo - 1;
+    f->nk = k_hi - k_lo + 1;
+    const int S = 2 * LMAX, nmm = 2 * LMAX - 1;
+    std::vector<double> P((size_t)S * LMAX * LMAX, 0.0);
+    for (int j = 0; j < S; ++j) {
+        const double th = tph::kPi * (2 * j + 1) / (4.0 * LMAX);
+        for (int m = 0; m < LMAX; ++m)
+            for (int l = m; l < LMAX; ++l) P[((size_t)m * LMAX + l) * S + j] = tph::sphPlm(l, m, th);
+    }
+    std::vector<int> sean((size_t)LMAX * LMAX);
+    for (int l = 0; l < LMAX; ++l)
+        for (int m = -l; m <= l; ++m) sean[l * l + l + m] = (int)(tph::seanindex(l, m, LMAX) - 1);
+    int rc = 0;
+    if ((rc = dev_alloc(&f->d_P, P.size())) || (rc = dev_alloc(&f->d_sean, sean.size())) ||
+        (rc = dev_alloc(&f->d_gref, (size_t)f->nk * S * nmm)) || (rc = dev_alloc(&f->d_refsum, (size_t)2 * f->nk)) ||
+        (rc = dev_alloc(&f->d_vol, (size_t)k_hi * LMAX * LMAX)) || (rc = dev_alloc(&f->d_volc, (size_t)k_hi * b->n_c))) {
+        tp_fit_destroy(f);
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    hipError_t e = hipMemcpyAsync(f->d_P, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(f->d_sean, sean.data(), sizeof(int) * sean.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        const size_t w = sizeof(double2) * (size_t)LMAX * LMAX;
+        e = hipMemcpy2DAsync(f->d_vol, w, reference, sizeof(double2) * (size_t)ld, w, (size_t)k_hi,
+                             hipMemcpyHostToDevice, st);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fit_similarity, dim3((unsigned)f->nk, 1u), dim3(kFitThreads), 0, st, f->d_vol,
+                           (int64_t)LMAX * LMAX, (const double2 *)nullptr, b->n_c, b->d_coff, b->L, LMAX, f->K, f->klo,
+                           f->nk, f->d_sean, f->d_P, f->d_gref, f->d_refsum, (double *)nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        tp_fit_destroy(f);
+        return tp_fail(TP_E_HIP, std::string("tp_fit_create: ") + hipGetErrorString(e));
+    }
+    *out = f;
+    return 0;
+}
+
+int tp_fit_set_volume(tp_fit *f, const double *coeff, int64_t ld)
+{
+    if (!f || !coeff) return tp_fail(TP_E_ARG, "null argument");
+    if (int rc = check_ctx(f->ctx)) return rc;
+    const int LMAX = f->b->LMAX;
+    if (ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "ld shorter than LMAX^2");
+    if (int rc = upload_full(f->ctx, coeff, ld, f->K, LMAX, f->d_vol)) return rc;
+    if (int rc = upload_coeff(f->ctx, f->b, coeff, ld, f->K, f->d_volc, nullptr)) return rc;
+    TP_HIP(hipStreamSynchronize(f->ctx->stream));
+    f->have_vol = true;
+    return 0;
+}
+
+// Unit quaternion (w, x, y, z) of R = Rz(phi) Ry(-theta) Rz(gamma): the
+// rotation rotationMatrix(theta, phi, gamma, l) applies to the coefficients
+// (spherical_harmonics.jl:589-595; the rotated structure is f(R^-1 r)).
+static void euler_quat(double theta, double phi, double gamma, double *q)
+{
+    const double a[4] = {std::cos(0.5 * phi), 0.0, 0.0, std::sin(0.5 * phi)};
+    const double bq[4] = {std::cos(-0.5 * theta), 0.0, std::sin(-0.5 * theta), 0.0};
+    const double c[4] = {std::cos(0.5 * gamma), 0.0, 0.0, std::sin(0.5 * gamma)};
+    auto mul = [](const double *u, const double *v, double *o) {
+        o[0] = u[0] * v[0] - u[1] * v[1] - u[2] * v[2] - u[3] * v[3];
+        o[1] = u[0] * v[1] + u[1] * v[0] + u[2] * v[3] - u[3] * v[2];
+        o[2] = u[0] * v[2] - u[1] * v[3] + u[2] * v[0] + u[3] * v[1];
+        o[3] = u[0] * v[3] + u[1] * v[2] - u[2] * v[1] + u[3] * v[0];
+    };
+    double t[4];
+    mul(a, bq, t);
+    mul(t, c, q);
+}
+
+int tp_fit_similarity(tp_fit *f, int n, const double *euler, double *sc_out)
+{
+    if (!f || (n > 0 && (!euler || !sc_out)) || n < 0) return tp_fail(TP_E_ARG, "bad argument");
+    if (n == 0) return 0;
+    if (int rc = check_ctx(f->ctx)) return rc;
+    if (!f->have_vol) return tp_fail(TP_E_STATE, "tp_fit_set_volume first");
+    tp_ctx *ctx = f->ctx;
+    tp_basis *b = f->b;
+    hipStream_t st = ctx->stream;
+    const int nl = b->L / 2, chunk = std::min(n, 2048);
+    if (chunk > f->cap) {
+        void *ptrs[] = {f->d_rot, f->d_quat, f->d_ev, f->d_cor};
+        for (void *p : ptrs)
+            if (p) TP_HIP(hipFree(p));
+        f->d_rot = nullptr; f->d_quat = nullptr; f->d_ev = nullptr; f->d_cor = nullptr;
+        f->cap = 0;
+        int rc = 0;
+        if ((rc = dev_alloc(&f->d_rot, (size_t)chunk * f->K * b->n_c)) ||
+            (rc = dev_alloc(&f->d_quat, (size_t)chunk * nl * 4)) || (rc = dev_alloc(&f->d_ev, (size_t)chunk)) ||
+            (rc = dev_alloc(&f->d_cor, (size_t)chunk * f->nk)))
+            return rc;
+        f->cap = chunk;
+    }
+    std::vector<double> quat((size_t)chunk * nl * 4);
+    std::vector<EvalDesc> ev(chunk);
+    std::vector<double> cor((size_t)chunk * f->nk);
+    float t_rot = 0.f, t_sim = 0.f;
+    for (int r0 = 0; r0 < n; r0 += chunk) {
+        const int c = std::min(chunk, n - r0);
+        for (int i = 0; i < c; ++i) {
+            double q[4];
+            euler_quat(euler[3 * (r0 + i)], euler[3 * (r0 + i) + 1], euler[3 * (r0 + i) + 2], q);
+            for (int li = 0; li < nl; ++li) std::memcpy(&quat[((size_t)i * nl + li) * 4], q, sizeof(q));
+            EvalDesc &d = ev[i];
+            std::memset(&d, 0, sizeof(d));
+            d.src = f->d_volc;
+            d.coef = f->d_rot + (size_t)i * f->K * b->n_c;
+            d.K = f->K;
+        }
+        TP_HIP(hipMemcpyAsync(f->d_quat, quat.data(), sizeof(double) * (size_t)c * nl * 4, hipMemcpyHostToDevice, st));
+        TP_HIP(hipMemcpyAsync(f->d_ev, ev.data(), sizeof(EvalDesc) * (size_t)c, hipMemcpyHostToDevice, st));
+        if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
+        hipLaunchKernelGGL(k_rotate_batch, dim3((unsigned)nl, (unsigned)c), dim3(256), 0, st, f->d_ev, f->d_quat, nl,
+                           b->n_c, (const GroupState *)nullptr, TP_MOVE_QUAT, 0);
+        TP_HIP(hipGetLastError());
+        if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+        hipLaunchKernelGGL(k_fit_similarity, dim3((unsigned)f->nk, (unsigned)c), dim3(kFitThreads), 0, st, f->d_vol,
+                           (int64_t)b->LMAX * b->LMAX, (const double2 *)f->d_rot, b->n_c, b->d_coff, b->L, b->LMAX,
+                           f->K, f->klo, f->nk, f->d_sean, f->d_P, f->d_gref, f->d_refsum, f->d_cor);
+        TP_HIP(hipGetLastError());
+        if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
+        TP_HIP(hipMemcpyAsync(cor.data(), f->d_cor, sizeof(double) * (size_t)c * f->nk, hipMemcpyDeviceToHost, st));
+        TP_HIP(hipStreamSynchronize(st));
+        if (ctx->timing) {
+            float a = 0.f, bb = 0.f;
+            if (hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]) == hipSuccess) t_rot += a;
+            if (hipEventElapsedTime(&bb, ctx->ev[1], ctx->ev[2]) == hipSuccess) t_sim += bb;
+        }
+        for (int i = 0; i < c; ++i) {
+            double s = 0.0;
+            for (int kk = 0; kk < f->nk; ++kk) s += cor[(size_t)i * f->nk + kk];
+            sc_out[r0 + i] = s / f->nk;
+        }
+    }
+    if (ctx->timing) {
+        ctx->last_ms[0] = t_rot;
+        ctx->last_ms[1] = t_sim;
+        ctx->last_ms[2] = ctx->last_ms[3] = 0.0;
+    }
     return 0;
 }
 

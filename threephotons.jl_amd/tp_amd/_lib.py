@@ -83,6 +83,11 @@ def lib():
     L.tp_hist_destroy.restype = None
     L.tp_two_photons.argtypes = [vp, vp, vp, i64, i32, i32, i32, vp]
     L.tp_retrieve_solution.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp, i64, vp, vp]
+    L.tp_fit_create.argtypes = [vp, vp, vp, i64, i32, i32, pp]
+    L.tp_fit_set_volume.argtypes = [vp, vp, i64]
+    L.tp_fit_similarity.argtypes = [vp, i32, vp, vp]
+    L.tp_fit_destroy.argtypes = [vp]
+    L.tp_fit_destroy.restype = None
     L.tp_batch_create.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, i32, pp]
     L.tp_batch_destroy.argtypes = [vp]
     L.tp_batch_destroy.restype = None
@@ -146,6 +151,7 @@ EXPORTS = [
     "tp_batch_log_count", "tp_batch_log_read", "tp_batch_set_proposer", "tp_batch_set_amax", "tp_batch_set_contraction",
     "tp_batch_set_family", "tp_basis_from_host", "tp_energy_grad",
     "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy", "tp_two_photons", "tp_retrieve_solution",
+    "tp_fit_create", "tp_fit_set_volume", "tp_fit_similarity", "tp_fit_destroy",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
