@@ -10,10 +10,11 @@
 #   energy(::SphericalHarmonicsVolume, ::BasisTypeHip, ::C3, ::UnitRange, ...)    (correlations.jl:201-226)
 #   twoPhotons(::SphericalHarmonicsVolume, ::BasisTypeHip, K2, ...)               (correlations.jl:229-243)
 #   retrieveSolution_hip(c2, L, LMAX, K2_range, qmax)                             (correlations.jl:262-319)
+#   fitStructures_full_hip(volume, reference, num, K_range, L, basis)              (data_processing.jl:26-46)
 #
 # Written against Julia 0.6 syntax like the rest of the package.
 
-export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip
+export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip, fitStructures_full_hip
 
 const libtp = get(ENV, "THREEPHOTONS_LIBTP", joinpath(dirname(@__FILE__), "..", "libtp.so"))
 HIP_enabled = false
@@ -200,6 +201,33 @@ function retrieveSolution_hip(c2::C2, L::Int64, LMAX::Int64, K2_range::UnitRange
     intensity = SphericalHarmonicsVolume(LMAX, hi, qmax)
     for k = 1:hi intensity.coeff[k][:] = coeff[:, k] end
     return intensity
+end
+
+"""fitStructures_full (data_processing.jl:26-46) with every candidate rotation's similarity
+computed by libtp in one call (tp_fit_*).  basis: a BasisTypeHip with the same L and LMAX."""
+function fitStructures_full_hip(volume::SphericalHarmonicsVolume, reference::SphericalHarmonicsVolume, num::Int64,
+                                K_range::UnitRange{Int64}, L::Int64, basis::BasisTypeHip, ta::Float64=0.0,
+                                tb::Float64=float(pi)/2, pa::Float64=0.0, pb::Float64=float(pi), ga::Float64=0.0,
+                                gb::Float64=float(pi))
+    K = maximum(K_range)
+    angles = [(theta, phi, gamma) for theta = ta:(tb-ta)/num:tb for phi = pa:(pb-pa)/num:pb for gamma = ga:(gb-ga)/num:gb]
+    euler = Float64[a[i] for i = 1:3, a in angles]
+    f = Ref{Ptr{Void}}(C_NULL)
+    hip_check(ccall((:tp_fit_create, libtp), Cint, (Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Cint, Cint, Ptr{Ptr{Void}}),
+                    hip_ctx, basis.handle, hip_pack(reference, K), num_coeff(reference.LMAX), minimum(K_range), K, f),
+              "tp_fit_create")
+    sc = zeros(Float64, length(angles))
+    try
+        hip_check(ccall((:tp_fit_set_volume, libtp), Cint, (Ptr{Void}, Ptr{Complex{Float64}}, Int64),
+                        f[], hip_pack(volume, K), num_coeff(volume.LMAX)), "tp_fit_set_volume")
+        hip_check(ccall((:tp_fit_similarity, libtp), Cint, (Ptr{Void}, Cint, Ptr{Float64}, Ptr{Float64}),
+                        f[], length(angles), euler, sc), "tp_fit_similarity")
+    finally
+        ccall((:tp_fit_destroy, libtp), Void, (Ptr{Void},), f[])
+    end
+    i = indmax(sc)
+    bestt, bestp, bestg = angles[i]
+    return (rotateStructure(volume, bestt, bestp, bestg, K, 2:2:L), sc[i], bestt, bestp, bestg)
 end
 
 """Switch a basis to the mixed-precision contraction (1 = FP16 hi/lo, 2 = BF16, 0 = FP64)."""
