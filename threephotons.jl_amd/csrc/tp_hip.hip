@@ -283,7 +283,9 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         n_rt_fin = b->n_rt16;
     } else {
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
-    hipLaunchKernelGGL((k_gemm<GF_BM, GF_BN, GF_WGM, 2, false>), dim3((unsigned)(b->g1_rt * b->g1_ct)), dim3(256), 0,
+    // XCD-grouped column tiles: each As column block is fetched from HBM once
+    hipLaunchKernelGGL((k_gemm<GF_BM, GF_BN, GF_WGM, 2, false, true>),
+                       dim3((unsigned)(8 * ((b->g1_ct + 7) / 8) * b->g1_rt)), dim3(256), 0,
                        st, b->d_H, b->nm_pad64, b->d_As, b->kl_pad, (int)b->nlt_pad, (int)(b->g1_rt * GF_BM),
                        (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
     TP_HIP(hipGetLastError());

@@ -418,7 +418,7 @@ __global__ __launch_bounds__(1024) void k_triple_mfma(const double2 *__restrict_
 //            p = max(c,1e-30) and qhat*log(p) (correlations.jl:203-220) go to
 //            part[(rt*2+{0,1})*ldp + col]; requires WGM = 4.
 // ---------------------------------------------------------------------------
-template <int BM, int BN, int WGM, int OUT, bool ENERGY>
+template <int BM, int BN, int WGM, int OUT, bool ENERGY, bool XCDC = false>
 __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int64_t lda,
                                               const double *__restrict__ B, int64_t ldb, int Kdim, int M, int ncols,
                                               int n_rt, int n_ct, double *__restrict__ out, int64_t ldo,
@@ -426,9 +426,6 @@ __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int6
                                               const double *__restrict__ logtab)
 {
     __shared__ double s_lt[ENERGY ? kLogTabSize : 1];
-    if (ENERGY)
-        for (int i = threadIdx.x; i < kLogTabSize; i += blockDim.x) s_lt[i] = logtab[i];
-    __syncthreads();
     constexpr int BK = 16;
     constexpr int WGN = 4 / WGM;
     constexpr int WTM = BM / WGM, WTN = BN / WGN;
@@ -442,9 +439,25 @@ __global__ __launch_bounds__(256) void k_gemm(const double *__restrict__ A, int6
     __shared__ double red[ENERGY ? 4 : 1][2][BN];
 
     const int nblk = n_rt * n_ct;
-    int lin = blockIdx.x;
-    if ((nblk & 7) == 0) lin = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
-    const int rt = lin / n_ct, ct = lin % n_ct;
+    int rt, ct;
+    if (XCDC) {
+        // XCD x (= block id mod 8) owns column tiles x cpx .. x cpx + cpx - 1
+        // with all their row tiles, consecutive in its dispatch order, so each
+        // B column block is fetched from HBM once, into that XCD's L2; the
+        // grid is 8 cpx n_rt (blocks past n_ct exit)
+        const int cpx = (n_ct + 7) >> 3, i = blockIdx.x >> 3;
+        ct = (blockIdx.x & 7) * cpx + i / n_rt;
+        rt = i % n_rt;
+        if (ct >= n_ct) return;   // whole workgroup, before any barrier
+    } else {
+        int lin = blockIdx.x;
+        if ((nblk & 7) == 0) lin = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+        rt = lin / n_ct;
+        ct = lin % n_ct;
+    }
+    if (ENERGY)
+        for (int i = threadIdx.x; i < kLogTabSize; i += blockDim.x) s_lt[i] = logtab[i];
+    __syncthreads();
     const int r0 = rt * BM, c0 = ct * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave % WGM, wn = wave / WGM;
