@@ -75,6 +75,8 @@ struct tp_ctx {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double last_ms[4] = {0, 0, 0, 0};
     double *h_pin = nullptr;   // pinned host landing buffer for scalar results
+    double *h_res = nullptr;   // pinned, mapped, coherent host buffer the final kernel writes the energy into
+    double *d_res = nullptr;   // its device address
     tp_batch *eb = nullptr;    // plan (and graph) of the last mixed-precision tp_energy_batch
     std::vector<const void *> eb_sig;
 };
@@ -1060,6 +1062,14 @@ int tp_init(int device, tp_ctx **out)
         tp_finalize(c);
         return tp_fail(TP_E_HIP, "hipHostMalloc failed");
     }
+    // the energy of a synchronous evaluation is stored by k_final directly in
+    // host memory: no copy (a blit kernel for small copies) after the evaluation
+    e = hipHostMalloc((void **)&c->h_res, 64, hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&c->d_res, c->h_res, 0);
+    if (e != hipSuccess) {
+        tp_finalize(c);
+        return tp_fail(TP_E_HIP, std::string("mapped result buffer: ") + hipGetErrorString(e));
+    }
     *out = c;
     return 0;
 }
@@ -1072,6 +1082,7 @@ void tp_finalize(tp_ctx *ctx)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
+    if (ctx->h_res) (void)hipHostFree(ctx->h_res);
     batch_free(ctx->eb);
     delete ctx;
 }
@@ -1631,10 +1642,9 @@ int tp_energy(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *co
     tp_basis *b = const_cast<tp_basis *>(bc);
     if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
     if (int rc = upload_coeff(ctx, b, coeff, ld, b->K, b->d_coef, b->d_raw)) return rc;
-    if (int rc = enqueue_eval(ctx, b, r, b->d_coef, nullptr, nullptr, 0, nullptr, b->d_out, mc_none())) return rc;
-    TP_HIP(hipMemcpyAsync(ctx->h_pin, b->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (int rc = enqueue_eval(ctx, b, r, b->d_coef, nullptr, nullptr, 0, nullptr, ctx->d_res, mc_none())) return rc;
     TP_HIP(hipStreamSynchronize(ctx->stream));
-    *E_out = ctx->h_pin[0];
+    *E_out = ctx->h_res[0];
     record_timing(ctx);
     return 0;
 }
@@ -2240,10 +2250,9 @@ int tp_state_energy(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s
     if (int rc = check_ctx(ctx)) return rc;
     if (!b || !r || !s || !E_out) return tp_fail(TP_E_ARG, "null argument");
     if (b->ctx != ctx || r->b != b || s->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
-    if (int rc = enqueue_eval(ctx, b, r, s->d_cur, nullptr, nullptr, 0, nullptr, b->d_out, mc_none())) return rc;
-    TP_HIP(hipMemcpyAsync(ctx->h_pin, b->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (int rc = enqueue_eval(ctx, b, r, s->d_cur, nullptr, nullptr, 0, nullptr, ctx->d_res, mc_none())) return rc;
     TP_HIP(hipStreamSynchronize(ctx->stream));
-    *E_out = ctx->h_pin[0];
+    *E_out = ctx->h_res[0];
     record_timing(ctx);
     return 0;
 }
