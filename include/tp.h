@@ -231,6 +231,29 @@ int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi,
  * over the shells of Pearson's correlation of the real surfaces on the
  * (2 LMAX)^2 grid of the inverse transform.  With timing on, tp_last_timings
  * gives the rotation and similarity kernel times of the last call. */
+/* Spherical-harmonics transforms on s2kit's (2 LMAX)^2 grid (theta_j = pi (2j+1) /
+ * (4 LMAX), phi_i = 2 pi i / (2 LMAX), samples theta-major), with
+ * Ylm = sphPlm e^{i m phi}: tp_sh_surface = getSurfaceVolume (backtransform,
+ * spherical_harmonics.jl:139-153), K shells of ld complex coefficients ->
+ * surf[K][(2 LMAX)^2] complex; tp_sh_coeff = getSphericalHarmonicsVolume
+ * (transform, :127-160) with Driscoll-Healy weights, exact for band-limited
+ * surfaces.  tp_sph_bessel = forward (forward != 0) / backward
+ * (sphBesselTransform, :310-377): every (l, m) radial profile of K shells
+ * through the reference's Hankel quadrature.  LMAX <= 32. */
+int tp_sh_surface(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld, double *surf);
+int tp_sh_coeff(tp_ctx *ctx, int LMAX, int K, const double *surf, double *coeff, int64_t ld);
+int tp_sph_bessel(tp_ctx *ctx, int LMAX, int K, int forward, const double *coeff, int64_t ld, double *out);
+
+/* Phase retrieval: sRAAR(intensity, iterations, beta0, beta_max, tau; cutoff_factor)
+ * (phases.jl:31-95) on the device.  intensity: K shells of ld complex
+ * coefficients; phases: the initial random phases [K][(2 LMAX)^2] in the
+ * surface sample order, or NULL to draw them uniformly in [0, 2 pi) from seed.
+ * coeff_out: the density's K shells of ld complex coefficients; diff_out (may
+ * be NULL): the per-iteration real-space change sum|density - tmp_u| / sum|density|. */
+int tp_sraar(tp_ctx *ctx, int LMAX, int K, const double *intensity, int64_t ld, int iterations, double beta0,
+             double beta_max, double tau, double cutoff_factor, const double *phases, uint64_t seed,
+             double *coeff_out, double *diff_out);
+
 typedef struct tp_fit tp_fit;
 int tp_fit_create(tp_ctx *ctx, const tp_basis *basis, const double *reference, int64_t ld, int k_lo, int k_hi,
                   tp_fit **out);

@@ -1,0 +1,90 @@
+"""SH transforms, spherical Bessel transforms and sRAAR on the device
+(tp_sh_surface, tp_sh_coeff, tp_sph_bessel, tp_sraar) against the oracle
+(oracle/phase_oracle.py, pinned in test_oracle_phase.py).
+
+Bar: transforms within 1e-12 of max |oracle|; sRAAR (nonlinear projections
+iterated) within 1e-8 of max |oracle| on the coefficients and 1e-8 relative on
+the per-iteration change, from the same initial phases."""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import random_real_coeff
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tp():
+    import tp_amd
+    tp_amd.HIP_init()
+    return tp_amd
+
+
+@pytest.fixture(scope="module")
+def po(oracle):
+    from oracle import phase_oracle
+    return phase_oracle
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("LMAX,K", [(4, 3), (9, 5), (25, 4)])
+def test_sh_transforms_match_oracle(tp, po, LMAX, K):
+    from tp_amd import phases
+    say(f"case LMAX={LMAX} K={K}")
+    c = random_real_coeff(K, LMAX - 1, LMAX, LMAX + K, even=False)
+    c[0] += 0.25j * random_real_coeff(1, LMAX - 1, LMAX, 99, even=False)[0]
+    vol = tp.SphericalHarmonicsVolume(c, LMAX, K, 10.0)
+    surf = phases.getSurfaceVolume(vol)
+    ref = np.array([po.surface(c[k], LMAX).ravel() for k in range(K)])
+    err = np.abs(surf - ref).max() / np.abs(ref).max()
+    say(f"  synthesis rel err {err:.2e}")
+    assert err <= 1e-12
+    # analysis of an arbitrary (not band-limited) surface, and the round trip
+    rough = ref + np.random.default_rng(LMAX).normal(size=ref.shape)
+    got = phases.getSphericalHarmonicsVolume(rough, LMAX, K, 10.0).coeff
+    want = np.array([po.coefficients(rough[k], LMAX) for k in range(K)])
+    err = np.abs(got - want).max() / np.abs(want).max()
+    say(f"  analysis rel err {err:.2e}")
+    assert err <= 1e-12
+    back = phases.getSphericalHarmonicsVolume(surf, LMAX, K, 10.0).coeff
+    assert np.abs(back - c).max() <= 1e-12 * np.abs(c).max()
+
+
+@pytest.mark.parametrize("LMAX,K", [(5, 8), (25, 38)])
+def test_sph_bessel_matches_oracle(tp, po, LMAX, K):
+    from tp_amd import phases
+    say(f"case LMAX={LMAX} K={K}")
+    c = random_real_coeff(K, LMAX - 1, LMAX, 7 * K, even=False)
+    vol = tp.SphericalHarmonicsVolume(c, LMAX, K, float(K))
+    for fwd in (True, False):
+        got = (phases.forward if fwd else phases.backward)(vol)
+        want = po.sph_bessel(c, LMAX, fwd)
+        err = np.abs(got.coeff - want).max() / np.abs(want).max()
+        say(f"  forward={fwd} rel err {err:.2e}")
+        assert err <= 1e-12
+        assert got.rmax == pytest.approx(np.pi / (vol.rmax / K))
+
+
+@pytest.mark.parametrize("LMAX,K,iters,cut", [(5, 6, 0, 0.5), (5, 6, 12, 0.5), (8, 10, 25, 0.3)])
+def test_sraar_matches_oracle(tp, po, LMAX, K, iters, cut):
+    from tp_amd import phases
+    say(f"case LMAX={LMAX} K={K} iterations={iters} cutoff={cut}")
+    I = random_real_coeff(K, LMAX - 1, LMAX, 5 + K, even=True, c00=5.0, decay=0.1)
+    ph = np.random.default_rng(K).uniform(0, 2 * np.pi, (K, 4 * LMAX * LMAX))
+    vol = tp.SphericalHarmonicsVolume(I, LMAX, K, float(K))
+    got, diff = phases.sRAAR(vol, iters, 0.75, 0.9, 30.0, cut, phases=ph, return_diff=True)
+    want, wdiff = po.sraar(I, LMAX, iters, 0.75, 0.9, 30.0, cut, ph)
+    err = np.abs(got.coeff - want).max() / np.abs(want).max()
+    say(f"  coeff rel err {err:.2e}; diffs {np.round(diff[:5], 4)}")
+    assert err <= 1e-8
+    assert len(diff) == iters
+    if iters:
+        np.testing.assert_allclose(diff, wdiff, rtol=1e-8)
+    Koff = int(np.ceil(cut * K))
+    assert np.all(got.coeff[Koff:] == 0)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <random>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,6 +24,7 @@
 #include "tp_c2.hpp"
 #include "tp_retrieve.hpp"
 #include "tp_fit.hpp"
+#include "tp_phase.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -2182,6 +2184,292 @@ int tp_fit_similarity(tp_fit *f, int n, const double *euler, double *sc_out)
         ctx->last_ms[1] = t_sim;
         ctx->last_ms[2] = ctx->last_ms[3] = 0.0;
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// SH transforms, spherical Bessel transform and sRAAR (phases.jl:31-95)
+// ---------------------------------------------------------------------------
+struct PhTables {
+    int LMAX = 0, K = 0;
+    double *d_P = nullptr;    // [LMAX][LMAX][2 LMAX] sphPlm(l, m, theta_j) at (m LMAX + l) 2 LMAX + j
+    double *d_W = nullptr;    // [2 LMAX] (2 pi / 2 LMAX) w_j, Driscoll-Healy weights
+    double2 *d_tw = nullptr;  // [2 LMAX] e^{2 pi i r / 2 LMAX}
+    double *d_Hf = nullptr;   // [LMAX][K][K] forward spherical Bessel matrices
+    double *d_Hb = nullptr;   // [LMAX][K][K] backward
+    ~PhTables()
+    {
+        void *ptrs[] = {d_P, d_W, d_tw, d_Hf, d_Hb};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+// frdr weights of the reference's Hankel quadrature (spherical_harmonics.jl:296-307):
+// w_j = pi/4 res_j^1.5 with res from the sample positions x_1..x_K
+static std::vector<double> ph_frdr(const std::vector<double> &x)
+{
+    const int n = (int)x.size();
+    std::vector<double> res(n);
+    res[0] = std::sqrt(0.5) * (x[0] + x[1]) * (x[0] + x[1]);
+    for (int j = 1; j < n - 1; ++j) res[j] = (x[j + 1] - x[j - 1]) * (x[j - 1] + 2 * x[j] + x[j + 1]);
+    res[n - 1] = 4 * x[n - 1] * x[n - 1] - (x[n - 2] + x[n - 1]) * (x[n - 2] + x[n - 1]);
+    for (int j = 0; j < n; ++j) res[j] = tph::kPi / 4 * std::pow(res[j], 1.5);
+    return res;
+}
+
+static int ph_tables(tp_ctx *ctx, int LMAX, int K, bool bessel, PhTables &t)
+{
+    if (LMAX < 1 || LMAX > kPhMaxLMAX) return tp_fail(TP_E_UNSUPPORTED, "need 1 <= LMAX <= 32");
+    if (K < 1 || (bessel && K < 2)) return tp_fail(TP_E_ARG, "need K >= 1 (>= 2 for the Bessel transform)");
+    t.LMAX = LMAX;
+    t.K = K;
+    const int S = 2 * LMAX;
+    // the shell's surface or coefficients and the phi sums sit in LDS: beyond
+    // 64 KiB (LMAX > 22) the kernels need the larger dynamic allowance
+    TP_HIP(hipFuncSetAttribute((const void *)k_sh_analysis, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sh_analysis_lds_bytes(kPhMaxLMAX)));
+    TP_HIP(hipFuncSetAttribute((const void *)k_sh_synth, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)sh_synth_lds_bytes(kPhMaxLMAX)));
+    std::vector<double> P((size_t)LMAX * LMAX * S, 0.0), W(S);
+    std::vector<double2> tw(S);
+    for (int j = 0; j < S; ++j) {
+        const double th = tph::kPi * (2 * j + 1) / (4.0 * LMAX);
+        for (int m = 0; m < LMAX; ++m)
+            for (int l = m; l < LMAX; ++l) P[((size_t)m * LMAX + l) * S + j] = tph::sphPlm(l, m, th);
+        // Driscoll-Healy: w_j = (2/B) sin(theta_j) sum_{k<B} sin((2k+1) theta_j) / (2k+1)
+        double sk = 0.0;
+        for (int k = 0; k < LMAX; ++k) sk += std::sin((2 * k + 1) * th) / (2 * k + 1);
+        W[j] = (2.0 * tph::kPi / S) * (2.0 / LMAX) * std::sin(th) * sk;
+        tw[j] = make_double2(std::cos(2.0 * tph::kPi * j / S), std::sin(2.0 * tph::kPi * j / S));
+    }
+    int rc = 0;
+    if ((rc = dev_alloc(&t.d_P, P.size())) || (rc = dev_alloc(&t.d_W, W.size())) || (rc = dev_alloc(&t.d_tw, tw.size())))
+        return rc;
+    TP_HIP(hipMemcpyAsync(t.d_P, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice, ctx->stream));
+    TP_HIP(hipMemcpyAsync(t.d_W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, ctx->stream));
+    TP_HIP(hipMemcpyAsync(t.d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (bessel) {
+        // sphBesselTransform (spherical_harmonics.jl:310-377): I[k][r] = J_{l+1/2}(pi k r / K);
+        // forward out[k] = i^l sqrt(1/k) sum_r I[k][r] frdr(r)_r in[r], r = 1..K;
+        // backward out[r] = (-i)^l sqrt(1/r) sum_k I[k][r] frdr(q)_k in[k] / (2 pi)^2, q = pi k / K
+        std::vector<double> xr(K), xq(K);
+        for (int i = 0; i < K; ++i) {
+            xr[i] = i + 1.0;
+            xq[i] = tph::kPi / K * (i + 1.0);
+        }
+        const std::vector<double> wr = ph_frdr(xr), wq = ph_frdr(xq);
+        std::vector<double> Hf((size_t)LMAX * K * K), Hb((size_t)LMAX * K * K);
+        for (int l = 0; l < LMAX; ++l)
+            for (int k = 0; k < K; ++k)
+                for (int r = 0; r < K; ++r) {
+                    const double I = std::cyl_bessel_j(l + 0.5, xq[k] * xr[r]);
+                    Hf[((size_t)l * K + k) * K + r] = std::sqrt(1.0 / (k + 1)) * I * wr[r];
+                    Hb[((size_t)l * K + r) * K + k] = std::sqrt(1.0 / (r + 1)) * I * wq[k] / (4.0 * tph::kPi * tph::kPi);
+                }
+        if ((rc = dev_alloc(&t.d_Hf, Hf.size())) || (rc = dev_alloc(&t.d_Hb, Hb.size()))) return rc;
+        TP_HIP(hipMemcpyAsync(t.d_Hf, Hf.data(), sizeof(double) * Hf.size(), hipMemcpyHostToDevice, ctx->stream));
+        TP_HIP(hipMemcpyAsync(t.d_Hb, Hb.data(), sizeof(double) * Hb.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// host coefficients (seanindex layout, stride ld) <-> packed (l^2 + l + m)
+static std::vector<double2> ph_pack(const double *coeff, int64_t ld, int K, int LMAX)
+{
+    std::vector<double2> out((size_t)K * LMAX * LMAX);
+    for (int k = 0; k < K; ++k)
+        for (int l = 0; l < LMAX; ++l)
+            for (int m = -l; m <= l; ++m) {
+                const int64_t s = tph::seanindex(l, m, LMAX) - 1;
+                out[(size_t)k * LMAX * LMAX + l * l + l + m] =
+                    make_double2(coeff[2 * ((size_t)k * ld + s)], coeff[2 * ((size_t)k * ld + s) + 1]);
+            }
+    return out;
+}
+
+static void ph_unpack(const std::vector<double2> &in, int K, int LMAX, double *coeff, int64_t ld)
+{
+    std::memset(coeff, 0, sizeof(double) * 2 * (size_t)K * ld);
+    for (int k = 0; k < K; ++k)
+        for (int l = 0; l < LMAX; ++l)
+            for (int m = -l; m <= l; ++m) {
+                const int64_t s = tph::seanindex(l, m, LMAX) - 1;
+                const double2 v = in[(size_t)k * LMAX * LMAX + l * l + l + m];
+                coeff[2 * ((size_t)k * ld + s)] = v.x;
+                coeff[2 * ((size_t)k * ld + s) + 1] = v.y;
+            }
+}
+
+static void ph_synth(tp_ctx *ctx, const PhTables &t, const double2 *c, double2 *surf)
+{
+    hipLaunchKernelGGL(k_sh_synth, dim3((unsigned)t.K), dim3(kPhThreads), sh_synth_lds_bytes(t.LMAX), ctx->stream, c,
+                       t.LMAX, t.d_P, t.d_tw, surf);
+}
+
+static void ph_analysis(tp_ctx *ctx, const PhTables &t, const double2 *surf, double2 *c)
+{
+    hipLaunchKernelGGL(k_sh_analysis, dim3((unsigned)t.K), dim3(kPhThreads), sh_analysis_lds_bytes(t.LMAX),
+                       ctx->stream, surf, t.LMAX, t.d_P, t.d_W, t.d_tw, c);
+}
+
+static void ph_bessel(tp_ctx *ctx, const PhTables &t, const double2 *in, int forward, double2 *out)
+{
+    const int n = t.K * t.LMAX * t.LMAX;
+    hipLaunchKernelGGL(k_sph_bessel, dim3((unsigned)((n + kPhThreads - 1) / kPhThreads)), dim3(kPhThreads), 0,
+                       ctx->stream, in, t.LMAX, t.K, forward ? t.d_Hf : t.d_Hb, forward, out);
+}
+
+extern "C++" {
+template <typename T>
+struct DevBuf {   // device buffer freed with its scope
+    T *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+}
+
+int tp_sh_surface(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld, double *surf)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!coeff || !surf || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
+    PhTables t;
+    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const size_t NP = (size_t)4 * LMAX * LMAX;
+    DevBuf<double2> c, f;
+    if (int rc = dev_alloc(&c.p, (size_t)K * LMAX * LMAX)) return rc;
+    if (int rc = dev_alloc(&f.p, (size_t)K * NP)) return rc;
+    const std::vector<double2> h = ph_pack(coeff, ld, K, LMAX);
+    TP_HIP(hipMemcpyAsync(c.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    ph_synth(ctx, t, c.p, f.p);
+    TP_HIP(hipGetLastError());
+    TP_HIP(hipMemcpyAsync(surf, f.p, sizeof(double2) * K * NP, hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tp_sh_coeff(tp_ctx *ctx, int LMAX, int K, const double *surf, double *coeff, int64_t ld)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!coeff || !surf || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
+    PhTables t;
+    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const size_t NP = (size_t)4 * LMAX * LMAX;
+    DevBuf<double2> c, f;
+    if (int rc = dev_alloc(&c.p, (size_t)K * LMAX * LMAX)) return rc;
+    if (int rc = dev_alloc(&f.p, (size_t)K * NP)) return rc;
+    TP_HIP(hipMemcpyAsync(f.p, surf, sizeof(double2) * K * NP, hipMemcpyHostToDevice, ctx->stream));
+    ph_analysis(ctx, t, f.p, c.p);
+    TP_HIP(hipGetLastError());
+    std::vector<double2> h((size_t)K * LMAX * LMAX);
+    TP_HIP(hipMemcpyAsync(h.data(), c.p, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    ph_unpack(h, K, LMAX, coeff, ld);
+    return 0;
+}
+
+int tp_sph_bessel(tp_ctx *ctx, int LMAX, int K, int forward, const double *coeff, int64_t ld, double *out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!coeff || !out || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
+    PhTables t;
+    if (int rc = ph_tables(ctx, LMAX, K, true, t)) return rc;
+    DevBuf<double2> a, b;
+    if (int rc = dev_alloc(&a.p, (size_t)K * LMAX * LMAX)) return rc;
+    if (int rc = dev_alloc(&b.p, (size_t)K * LMAX * LMAX)) return rc;
+    const std::vector<double2> h = ph_pack(coeff, ld, K, LMAX);
+    TP_HIP(hipMemcpyAsync(a.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    ph_bessel(ctx, t, a.p, forward, b.p);
+    TP_HIP(hipGetLastError());
+    std::vector<double2> r(h.size());
+    TP_HIP(hipMemcpyAsync(r.data(), b.p, sizeof(double2) * r.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    ph_unpack(r, K, LMAX, out, ld);
+    return 0;
+}
+
+int tp_sraar(tp_ctx *ctx, int LMAX, int K, const double *intensity, int64_t ld, int iterations, double beta0,
+             double beta_max, double tau, double cutoff_factor, const double *phases, uint64_t seed,
+             double *coeff_out, double *diff_out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!intensity || !coeff_out || ld < (int64_t)LMAX * LMAX || iterations < 0)
+        return tp_fail(TP_E_ARG, "bad argument");
+    PhTables t;
+    if (int rc = ph_tables(ctx, LMAX, K, true, t)) return rc;
+    hipStream_t st = ctx->stream;
+    const int NP = 4 * LMAX * LMAX;
+    const int64_t n = (int64_t)K * NP;
+    const int nq = LMAX * LMAX;
+    const int Koff = (int)std::ceil(cutoff_factor * K);
+    const unsigned nb = (unsigned)((n + kPhThreads - 1) / kPhThreads);
+    DevBuf<double2> C1, C2, F, dens, tmpu, tmp1, tmp2;
+    DevBuf<double> amp, ph, part, diff;
+    int rc = 0;
+    if ((rc = dev_alloc(&C1.p, (size_t)K * nq)) || (rc = dev_alloc(&C2.p, (size_t)K * nq)) ||
+        (rc = dev_alloc(&F.p, (size_t)n)) || (rc = dev_alloc(&dens.p, (size_t)n)) || (rc = dev_alloc(&tmpu.p, (size_t)n)) ||
+        (rc = dev_alloc(&tmp1.p, (size_t)n)) || (rc = dev_alloc(&tmp2.p, (size_t)n)) || (rc = dev_alloc(&amp.p, (size_t)n)) ||
+        (rc = dev_alloc(&ph.p, (size_t)n)) || (rc = dev_alloc(&part.p, (size_t)2 * nb)) ||
+        (rc = dev_alloc(&diff.p, (size_t)std::max(iterations, 1))))
+        return rc;
+    {
+        const std::vector<double2> h = ph_pack(intensity, ld, K, LMAX);
+        TP_HIP(hipMemcpyAsync(C1.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, st));
+        std::vector<double> hp((size_t)n);
+        if (phases) {
+            std::memcpy(hp.data(), phases, sizeof(double) * (size_t)n);
+        } else {   // pi * 2 * rand per sample (phases.jl:48-52)
+            std::mt19937_64 g(seed);
+            std::uniform_real_distribution<double> u(0.0, 1.0);
+            for (auto &v : hp) v = tph::kPi * 2 * u(g);
+        }
+        TP_HIP(hipMemcpyAsync(ph.p, hp.data(), sizeof(double) * hp.size(), hipMemcpyHostToDevice, st));
+        TP_HIP(hipStreamSynchronize(st));   // the host staging vectors end here
+    }
+    // B(x) -> out: forward (analysis, Bessel, synthesis), modulus, backward
+    auto projB = [&](const double2 *x, double2 *out) {
+        ph_analysis(ctx, t, x, C1.p);
+        ph_bessel(ctx, t, C1.p, 1, C2.p);
+        ph_synth(ctx, t, C2.p, F.p);
+        hipLaunchKernelGGL(k_raar_modulus, dim3(nb), dim3(kPhThreads), 0, st, F.p, amp.p, n);
+        ph_analysis(ctx, t, F.p, C1.p);
+        ph_bessel(ctx, t, C1.p, 0, C2.p);
+        ph_synth(ctx, t, C2.p, out);
+    };
+    // amplitudes and random phases; density = backward(fourier); tmp1 = 2 B(density) - density
+    ph_synth(ctx, t, C1.p, tmp2.p);
+    hipLaunchKernelGGL(k_raar_init, dim3(nb), dim3(kPhThreads), 0, st, tmp2.p, ph.p, n, amp.p, F.p);
+    ph_analysis(ctx, t, F.p, C1.p);
+    ph_bessel(ctx, t, C1.p, 0, C2.p);
+    ph_synth(ctx, t, C2.p, dens.p);
+    projB(dens.p, tmp2.p);
+    hipLaunchKernelGGL(k_raar_reflect, dim3(nb), dim3(kPhThreads), 0, st, tmp2.p, dens.p, n, tmp1.p);
+    TP_HIP(hipGetLastError());
+    double2 *d = dens.p, *u = tmpu.p;
+    for (int it = 1; it <= iterations; ++it) {
+        const double e3 = std::exp(std::pow(-it / tau, 3.0));
+        const double beta = e3 * beta0 + (1.0 - e3) * beta_max;
+        hipLaunchKernelGGL(k_raar_update, dim3(nb), dim3(kPhThreads), 0, st, tmp1.p, d, n, NP, Koff, beta, u, part.p);
+        hipLaunchKernelGGL(k_raar_diff, dim3(1), dim3(64), 0, st, part.p, (int)nb, diff.p, it - 1);
+        projB(u, tmp2.p);
+        std::swap(d, u);   // density = tmp_u
+        hipLaunchKernelGGL(k_raar_reflect, dim3(nb), dim3(kPhThreads), 0, st, tmp2.p, d, n, tmp1.p);
+        TP_HIP(hipGetLastError());
+    }
+    // density = A(tmp2) (tmp2 = tmp1 when no iteration ran, phases.jl:59); its coefficients
+    hipLaunchKernelGGL(k_raar_support, dim3(nb), dim3(kPhThreads), 0, st, iterations > 0 ? tmp2.p : tmp1.p, n, NP, Koff,
+                       u);
+    ph_analysis(ctx, t, u, C1.p);
+    TP_HIP(hipGetLastError());
+    std::vector<double2> h((size_t)K * nq);
+    TP_HIP(hipMemcpyAsync(h.data(), C1.p, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, st));
+    if (diff_out && iterations > 0)
+        TP_HIP(hipMemcpyAsync(diff_out, diff.p, sizeof(double) * iterations, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    ph_unpack(h, K, LMAX, coeff_out, ld);
     return 0;
 }
 

@@ -88,6 +88,10 @@ def lib():
     L.tp_fit_similarity.argtypes = [vp, i32, vp, vp]
     L.tp_fit_destroy.argtypes = [vp]
     L.tp_fit_destroy.restype = None
+    L.tp_sh_surface.argtypes = [vp, i32, i32, vp, i64, vp]
+    L.tp_sh_coeff.argtypes = [vp, i32, i32, vp, vp, i64]
+    L.tp_sph_bessel.argtypes = [vp, i32, i32, i32, vp, i64, vp]
+    L.tp_sraar.argtypes = [vp, i32, i32, vp, i64, i32, dbl, dbl, dbl, dbl, vp, C.c_uint64, vp, vp]
     L.tp_batch_create.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, i32, pp]
     L.tp_batch_destroy.argtypes = [vp]
     L.tp_batch_destroy.restype = None
@@ -152,6 +156,7 @@ EXPORTS = [
     "tp_batch_set_family", "tp_basis_from_host", "tp_energy_grad",
     "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy", "tp_two_photons", "tp_retrieve_solution",
     "tp_fit_create", "tp_fit_set_volume", "tp_fit_similarity", "tp_fit_destroy",
+    "tp_sh_surface", "tp_sh_coeff", "tp_sph_bessel", "tp_sraar",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
