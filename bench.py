@@ -161,6 +161,15 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
         rs = fit_oracle.rotate_structure(start.coeff, *e, start.KMAX, range(2, cfg["L"] + 1, 2), cfg["LMAX"])
         fit_oracle.similarity(start.coeff, rs, cfg["LMAX"], 1, start.KMAX)
     t_fit = (time.perf_counter() - t0) / 2
+    # phase retrieval: the oracle's sRAAR iterations on the 38-shell start structure (LMAX = 25)
+    from oracle import phase_oracle
+    php = np.random.default_rng(1).uniform(0, 2 * np.pi, (start.KMAX, 4 * cfg["LMAX"] ** 2))
+    t0 = time.perf_counter()
+    phase_oracle.sraar(start.coeff, cfg["LMAX"], 1, 0.75, 0.9, 350.0, 0.3, php)
+    t_ph1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    phase_oracle.sraar(start.coeff, cfg["LMAX"], 3, 0.75, 0.9, 350.0, 0.3, php)
+    t_ph = (time.perf_counter() - t0 - t_ph1) / 2
     tf = float(np.median(t_fact))
     return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
                 sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
@@ -174,6 +183,8 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
                 two_photon_sample="numpy oracle: twoPhotons of the 38-shell start structure and retrieveSolution of it",
                 fit_rotations_per_s=1.0 / t_fit,
                 fit_sample="numpy oracle: rotateStructure + similarity of 2 rotations, 38 shells, L=18, LMAX=25",
+                sraar_ms_per_iteration=1e3 * t_ph,
+                sraar_sample="numpy oracle: sRAAR with 3 and with 1 iterations, 38 shells, LMAX=25; the difference / 2",
                 small_config={"dense_evals_per_s": 1.0 / t_d0, "factored_evals_per_s": 1.0 / t_f0,
                               "energy": E0, "klength": ob0.klength, "R": 2 * sm["N"] ** 2})
 
@@ -410,6 +421,26 @@ def fit_leg(tp, truth, num):
     return out
 
 
+def phase_leg(tp, truth, iters):
+    """sRAAR (phases.jl:31-95) of the 38-shell intensity (LMAX = 25, 50 x 50
+    surface grid per shell) with the reference's post-processing settings
+    (data_processing.jl:508: beta 0.75 -> 0.9, tau = 350, cutoff_factor 0.3),
+    every transform, projection and reflection on the device; initial phases
+    drawn from a fixed seed."""
+    from tp_amd import phases
+    phases.sRAAR(truth, 2, 0.75, 0.9, 350.0, 0.3, seed=1)
+    t0 = time.perf_counter()
+    vol, diff = phases.sRAAR(truth, iters, 0.75, 0.9, 350.0, 0.3, seed=1, return_diff=True)
+    wall = time.perf_counter() - t0
+    out = {"workload": f"sRAAR of the {truth.KMAX}-shell intensity, LMAX={truth.LMAX}, {iters} iterations "
+                       f"(beta 0.75 -> 0.9, tau 350, cutoff 0.3)",
+           "seconds": wall, "ms_per_iteration": 1e3 * wall / max(iters, 1),
+           "diff_first": float(diff[0]) if iters else None, "diff_last": float(diff[-1]) if iters else None}
+    log(f"sRAAR: {iters} iterations in {wall:.3f} s ({out['ms_per_iteration']:.3f} ms each); "
+        f"change {out['diff_first']:.3g} -> {out['diff_last']:.3g}")
+    return out
+
+
 def gradient_leg(tp, basis, c3ref, start, plain, nsteps, ncand=16):
     """Gradient-guided moves (tp_amd.descent) on the plain run's problem (K=26,
     its start structure and histogram): per step one FP64 energy + gradient
@@ -546,6 +577,7 @@ def main():
     ap.add_argument("--hist-pictures", type=int, default=32768, help="pictures of the histogram leg (0: skip)")
     ap.add_argument("--c2-reps", type=int, default=20, help="timed twoPhotons calls of the two-photon leg (0: skip)")
     ap.add_argument("--fit-num", type=int, default=20, help="grid divisions of the rotational-fit leg (0: skip)")
+    ap.add_argument("--sraar-iters", type=int, default=1001, help="iterations of the phase-retrieval leg (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -638,6 +670,7 @@ def main():
     hist = hist_leg(tp, workload, args.hist_pictures) if args.hist_pictures > 0 else None
     c2 = c2_leg(tp, truth, basis, args.c2_reps) if args.c2_reps > 0 else None
     fitr = fit_leg(tp, truth, args.fit_num) if args.fit_num > 0 else None
+    phase = phase_leg(tp, truth, args.sraar_iters) if args.sraar_iters > 0 else None
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:
@@ -701,6 +734,8 @@ def main():
         out["two_photon"] = c2
     if fitr is not None:
         out["rotational_fit"] = fitr
+    if phase is not None:
+        out["phase_retrieval"] = phase
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)
