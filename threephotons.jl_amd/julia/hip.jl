@@ -11,10 +11,11 @@
 #   twoPhotons(::SphericalHarmonicsVolume, ::BasisTypeHip, K2, ...)               (correlations.jl:229-243)
 #   retrieveSolution_hip(c2, L, LMAX, K2_range, qmax)                             (correlations.jl:262-319)
 #   fitStructures_full_hip(volume, reference, num, K_range, L, basis)              (data_processing.jl:26-46)
+#   sRAAR_hip(intensity, iterations, beta0, beta_max, tau; cutoff_factor)          (phases.jl:31-95)
 #
 # Written against Julia 0.6 syntax like the rest of the package.
 
-export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip, fitStructures_full_hip
+export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip, fitStructures_full_hip, sRAAR_hip
 
 const libtp = get(ENV, "THREEPHOTONS_LIBTP", joinpath(dirname(@__FILE__), "..", "libtp.so"))
 HIP_enabled = false
@@ -228,6 +229,25 @@ function fitStructures_full_hip(volume::SphericalHarmonicsVolume, reference::Sph
     i = indmax(sc)
     bestt, bestp, bestg = angles[i]
     return (rotateStructure(volume, bestt, bestp, bestg, K, 2:2:L), sc[i], bestt, bestp, bestg)
+end
+
+"""sRAAR (phases.jl:31-95) through libtp: every transform, projection and reflection on the GPU.
+phases: initial phases ((2 LMAX)^2 x KMAX, surface sample order) or nothing (drawn from `seed`)."""
+function sRAAR_hip(intensity::SphericalHarmonicsVolume, iterations::Integer, beta0::Float64=0.75,
+                   beta_max::Float64=0.75, tau::Float64=100.0; cutoff_factor::Float64=0.5,
+                   phases::Union{Void, Matrix{Float64}}=nothing, seed::Integer=0)
+    K, LMAX = intensity.KMAX, intensity.LMAX
+    coeff = hip_pack(intensity, K)
+    out = zeros(Complex{Float64}, size(coeff))
+    ph = phases == nothing ? C_NULL : phases
+    hip_check(ccall((:tp_sraar, libtp), Cint,
+                    (Ptr{Void}, Cint, Cint, Ptr{Complex{Float64}}, Int64, Cint, Cdouble, Cdouble, Cdouble, Cdouble,
+                     Ptr{Float64}, UInt64, Ptr{Complex{Float64}}, Ptr{Float64}),
+                    hip_ctx, LMAX, K, coeff, size(coeff, 1), iterations, beta0, beta_max, tau, cutoff_factor,
+                    ph, UInt64(seed), out, C_NULL), "tp_sraar")
+    density = SphericalHarmonicsVolume(LMAX, K, qmax(intensity))
+    for k = 1:K density.coeff[k][:] = out[:, k] end
+    return density
 end
 
 """Switch a basis to the mixed-precision contraction (1 = FP16 hi/lo, 2 = BF16, 0 = FP64)."""
