@@ -2197,9 +2197,10 @@ struct PhTables {
     double2 *d_tw = nullptr;  // [2 LMAX] e^{2 pi i r / 2 LMAX}
     double *d_Hf = nullptr;   // [LMAX][K][K] forward spherical Bessel matrices
     double *d_Hb = nullptr;   // [LMAX][K][K] backward
+    double2 *d_G = nullptr;   // [K][2 LMAX][2 LMAX - 1] phi sums of the analysis
     ~PhTables()
     {
-        void *ptrs[] = {d_P, d_W, d_tw, d_Hf, d_Hb};
+        void *ptrs[] = {d_P, d_W, d_tw, d_Hf, d_Hb, d_G};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     }
@@ -2225,12 +2226,6 @@ static int ph_tables(tp_ctx *ctx, int LMAX, int K, bool bessel, PhTables &t)
     t.LMAX = LMAX;
     t.K = K;
     const int S = 2 * LMAX;
-    // the shell's surface or coefficients and the phi sums sit in LDS: beyond
-    // 64 KiB (LMAX > 22) the kernels need the larger dynamic allowance
-    TP_HIP(hipFuncSetAttribute((const void *)k_sh_analysis, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)sh_analysis_lds_bytes(kPhMaxLMAX)));
-    TP_HIP(hipFuncSetAttribute((const void *)k_sh_synth, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)sh_synth_lds_bytes(kPhMaxLMAX)));
     std::vector<double> P((size_t)LMAX * LMAX * S, 0.0), W(S);
     std::vector<double2> tw(S);
     for (int j = 0; j < S; ++j) {
@@ -2244,7 +2239,8 @@ static int ph_tables(tp_ctx *ctx, int LMAX, int K, bool bessel, PhTables &t)
         tw[j] = make_double2(std::cos(2.0 * tph::kPi * j / S), std::sin(2.0 * tph::kPi * j / S));
     }
     int rc = 0;
-    if ((rc = dev_alloc(&t.d_P, P.size())) || (rc = dev_alloc(&t.d_W, W.size())) || (rc = dev_alloc(&t.d_tw, tw.size())))
+    if ((rc = dev_alloc(&t.d_P, P.size())) || (rc = dev_alloc(&t.d_W, W.size())) || (rc = dev_alloc(&t.d_tw, tw.size())) ||
+        (rc = dev_alloc(&t.d_G, (size_t)K * S * (2 * LMAX - 1))))
         return rc;
     TP_HIP(hipMemcpyAsync(t.d_P, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipMemcpyAsync(t.d_W, W.data(), sizeof(double) * W.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -2304,14 +2300,19 @@ static void ph_unpack(const std::vector<double2> &in, int K, int LMAX, double *c
 
 static void ph_synth(tp_ctx *ctx, const PhTables &t, const double2 *c, double2 *surf)
 {
-    hipLaunchKernelGGL(k_sh_synth, dim3((unsigned)t.K), dim3(kPhThreads), sh_synth_lds_bytes(t.LMAX), ctx->stream, c,
-                       t.LMAX, t.d_P, t.d_tw, surf);
+    const unsigned nch = (unsigned)((2 * t.LMAX + kPhRings - 1) / kPhRings);
+    hipLaunchKernelGGL(k_sh_synth, dim3((unsigned)t.K, nch), dim3(kPhThreads), sh_synth_lds_bytes(t.LMAX), ctx->stream,
+                       c, t.LMAX, t.d_P, t.d_tw, surf);
 }
 
 static void ph_analysis(tp_ctx *ctx, const PhTables &t, const double2 *surf, double2 *c)
 {
-    hipLaunchKernelGGL(k_sh_analysis, dim3((unsigned)t.K), dim3(kPhThreads), sh_analysis_lds_bytes(t.LMAX),
-                       ctx->stream, surf, t.LMAX, t.d_P, t.d_W, t.d_tw, c);
+    const unsigned nch = (unsigned)((2 * t.LMAX + kPhRings - 1) / kPhRings);
+    hipLaunchKernelGGL(k_sh_rings, dim3((unsigned)t.K, nch), dim3(kPhThreads), sh_rings_lds_bytes(t.LMAX), ctx->stream,
+                       surf, t.LMAX, t.d_tw, t.d_G);
+    const int n = t.K * t.LMAX * t.LMAX;
+    hipLaunchKernelGGL(k_sh_analysis, dim3((unsigned)((n + kPhThreads - 1) / kPhThreads)), dim3(kPhThreads), 0,
+                       ctx->stream, t.d_G, t.LMAX, t.K, t.d_P, t.d_W, c);
 }
 
 static void ph_bessel(tp_ctx *ctx, const PhTables &t, const double2 *in, int forward, double2 *out)

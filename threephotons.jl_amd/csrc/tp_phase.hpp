@@ -11,25 +11,27 @@
 //     (sphBesselTransform, spherical_harmonics.jl:296-377), a K x K real
 //     matrix per l and the factor (+-i)^l.
 // Coefficients on the device: [K][LMAX^2] complex, (l, m) at l^2 + l + m;
-// surfaces: [K][(2 LMAX)^2] complex.  One workgroup per shell for the
-// transforms (the phi sums by a twiddle table in LDS, the theta sums against
-// the sphPlm table P[m][l][j]).
+// surfaces: [K][(2 LMAX)^2] complex.  The transforms run one workgroup per
+// (shell, 5 rings theta_j): the phi sums by a twiddle table in LDS, the theta
+// sums against the sphPlm table P[m][l][j]; the analysis's theta sums run
+// one thread per (shell, l, m) over the rings' phi sums in HBM.
 #pragma once
 
 constexpr int kPhThreads = 256;
 constexpr int kPhMaxLMAX = 32;
+constexpr int kPhRings = 5;   // rings theta_j per workgroup of the transforms
 
-// LDS bytes of k_sh_synth: the shell's coefficients and g[j][m].
+// LDS bytes of k_sh_synth: the shell's coefficients, g of its rings, twiddles.
 inline size_t sh_synth_lds_bytes(int LMAX)
 {
     const int S = 2 * LMAX;
-    return sizeof(double2) * ((size_t)LMAX * LMAX + (size_t)S * (2 * LMAX - 1) + S);
+    return sizeof(double2) * ((size_t)LMAX * LMAX + (size_t)kPhRings * (2 * LMAX - 1) + S);
 }
-// LDS bytes of k_sh_analysis: the shell's surface and G[j][m].
-inline size_t sh_analysis_lds_bytes(int LMAX)
+// LDS bytes of k_sh_rings: the workgroup's rings of the surface, twiddles.
+inline size_t sh_rings_lds_bytes(int LMAX)
 {
     const int S = 2 * LMAX;
-    return sizeof(double2) * ((size_t)S * S + (size_t)S * (2 * LMAX - 1) + S);
+    return sizeof(double2) * ((size_t)kPhRings * S + S);
 }
 
 // e^{2 pi i (m i mod S) / S} from the table tw[0..S)
@@ -40,22 +42,23 @@ __device__ __forceinline__ double2 ph_tw(const double2 *tw, int m, int i, int S)
     return tw[r];
 }
 
-// coefficients -> surface of shell blockIdx.x
+// coefficients -> surface: workgroup (shell blockIdx.x, rings kPhRings
+// blockIdx.y .. + kPhRings - 1)
 __global__ __launch_bounds__(kPhThreads) void k_sh_synth(const double2 *__restrict__ c, int LMAX,
                                                          const double *__restrict__ P,
                                                          const double2 *__restrict__ twg, double2 *__restrict__ surf)
 {
     extern __shared__ double2 ph_lds[];
     const int S = 2 * LMAX, nq = LMAX * LMAX, nm = 2 * LMAX - 1;
-    double2 *cs = ph_lds, *g = cs + nq, *tw = g + (size_t)S * nm;
-    const int k = blockIdx.x, tid = threadIdx.x;
+    double2 *cs = ph_lds, *g = cs + nq, *tw = g + (size_t)kPhRings * nm;
+    const int k = blockIdx.x, j0 = blockIdx.y * kPhRings, nr = min(kPhRings, S - j0), tid = threadIdx.x;
     for (int q = tid; q < nq; q += kPhThreads) cs[q] = c[(size_t)k * nq + q];
     for (int i = tid; i < S; i += kPhThreads) tw[i] = twg[i];
     __syncthreads();
-    // g[j][m] = sum_l c_lm sphPlm(l, m, theta_j)  (items (|m|, j), j fastest)
-    for (int it = tid; it < S * LMAX; it += kPhThreads) {
-        const int mm = it / S, j = it - mm * S;
-        const double *pj = P + (size_t)mm * LMAX * S + j;
+    // g[j][m] = sum_l c_lm sphPlm(l, m, theta_j)  (items (|m|, ring), ring fastest)
+    for (int it = tid; it < nr * LMAX; it += kPhThreads) {
+        const int mm = it / nr, jl = it - mm * nr;
+        const double *pj = P + (size_t)mm * LMAX * S + j0 + jl;
         double pr = 0.0, pi = 0.0, mr = 0.0, mi = 0.0;
         for (int l = mm; l < LMAX; ++l) {
             const double p = pj[(size_t)l * S];
@@ -66,64 +69,71 @@ __global__ __launch_bounds__(kPhThreads) void k_sh_synth(const double2 *__restri
             mi += xm.y * p;
         }
         const double sg = (mm & 1) ? -1.0 : 1.0;   // sphPlm(l, -m) = (-1)^m sphPlm(l, m)
-        g[(size_t)j * nm + LMAX - 1 + mm] = make_double2(pr, pi);
-        if (mm > 0) g[(size_t)j * nm + LMAX - 1 - mm] = make_double2(sg * mr, sg * mi);
+        g[(size_t)jl * nm + LMAX - 1 + mm] = make_double2(pr, pi);
+        if (mm > 0) g[(size_t)jl * nm + LMAX - 1 - mm] = make_double2(sg * mr, sg * mi);
     }
     __syncthreads();
     // f[j][i] = sum_m g[j][m] e^{i m phi_i}
-    for (int p = tid; p < S * S; p += kPhThreads) {
-        const int j = p / S, i = p - j * S;
-        const double2 *gj = g + (size_t)j * nm;
+    for (int p = tid; p < nr * S; p += kPhThreads) {
+        const int jl = p / S, i = p - jl * S;
+        const double2 *gj = g + (size_t)jl * nm;
         double fr = 0.0, fi = 0.0;
         for (int m = -(LMAX - 1); m <= LMAX - 1; ++m) {
             const double2 a = gj[m + LMAX - 1], e = ph_tw(tw, m, i, S);
             fr += a.x * e.x - a.y * e.y;
             fi += a.x * e.y + a.y * e.x;
         }
-        surf[(size_t)k * S * S + p] = make_double2(fr, fi);
+        surf[(size_t)k * S * S + (size_t)(j0 + jl) * S + i] = make_double2(fr, fi);
     }
 }
 
-// surface -> coefficients of shell blockIdx.x; W[j] = (2 pi / S) w_j
-__global__ __launch_bounds__(kPhThreads) void k_sh_analysis(const double2 *__restrict__ surf, int LMAX,
-                                                            const double *__restrict__ P,
-                                                            const double *__restrict__ W,
-                                                            const double2 *__restrict__ twg, double2 *__restrict__ c)
+// surface -> coefficients, step 1: G[k][j][m] = sum_i f[j][i] e^{-i m phi_i}
+// for the workgroup's rings (shell blockIdx.x, rings from kPhRings blockIdx.y)
+__global__ __launch_bounds__(kPhThreads) void k_sh_rings(const double2 *__restrict__ surf, int LMAX,
+                                                         const double2 *__restrict__ twg, double2 *__restrict__ G)
 {
     extern __shared__ double2 ph_lds[];
-    const int S = 2 * LMAX, nq = LMAX * LMAX, nm = 2 * LMAX - 1;
-    double2 *fs = ph_lds, *G = fs + (size_t)S * S, *tw = G + (size_t)S * nm;
-    const int k = blockIdx.x, tid = threadIdx.x;
-    for (int p = tid; p < S * S; p += kPhThreads) fs[p] = surf[(size_t)k * S * S + p];
+    const int S = 2 * LMAX, nm = 2 * LMAX - 1;
+    double2 *fs = ph_lds, *tw = fs + (size_t)kPhRings * S;
+    const int k = blockIdx.x, j0 = blockIdx.y * kPhRings, nr = min(kPhRings, S - j0), tid = threadIdx.x;
+    for (int p = tid; p < nr * S; p += kPhThreads) fs[p] = surf[(size_t)k * S * S + (size_t)j0 * S + p];
     for (int i = tid; i < S; i += kPhThreads) tw[i] = twg[i];
     __syncthreads();
-    // G[j][m] = sum_i f[j][i] e^{-i m phi_i}
-    for (int it = tid; it < S * nm; it += kPhThreads) {
-        const int j = it / nm, m = it - j * nm - (LMAX - 1);
-        const double2 *fj = fs + (size_t)j * S;
+    for (int it = tid; it < nr * nm; it += kPhThreads) {
+        const int jl = it / nm, m = it - jl * nm - (LMAX - 1);
+        const double2 *fj = fs + (size_t)jl * S;
         double gr = 0.0, gi = 0.0;
         for (int i = 0; i < S; ++i) {
             const double2 a = fj[i], e = ph_tw(tw, m, i, S);   // times conj(e)
             gr += a.x * e.x + a.y * e.y;
             gi += a.y * e.x - a.x * e.y;
         }
-        G[it] = make_double2(gr, gi);
+        G[((size_t)k * S + j0 + jl) * nm + m + LMAX - 1] = make_double2(gr, gi);
     }
-    __syncthreads();
-    // c_lm = sum_j W_j sphPlm(l, m, theta_j) G[j][m]
-    for (int q = tid; q < nq; q += kPhThreads) {
-        const int l = (int)sqrt((double)q + 0.5), m = q - l * l - l, am = m < 0 ? -m : m;
-        const double *pl = P + ((size_t)am * LMAX + l) * S;
-        double cr = 0.0, ci = 0.0;
-        for (int j = 0; j < S; ++j) {
-            const double wp = W[j] * pl[j];
-            const double2 a = G[(size_t)j * nm + m + LMAX - 1];
-            cr += wp * a.x;
-            ci += wp * a.y;
-        }
-        const double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
-        c[(size_t)k * nq + q] = make_double2(sg * cr, sg * ci);
+}
+
+// surface -> coefficients, step 2: c_lm = sum_j W_j sphPlm(l, m, theta_j) G[j][m];
+// one thread per (shell, l, m); W[j] = (2 pi / S) w_j
+__global__ __launch_bounds__(kPhThreads) void k_sh_analysis(const double2 *__restrict__ G, int LMAX, int K,
+                                                            const double *__restrict__ P,
+                                                            const double *__restrict__ W, double2 *__restrict__ c)
+{
+    const int S = 2 * LMAX, nq = LMAX * LMAX, nm = 2 * LMAX - 1;
+    const int t = blockIdx.x * kPhThreads + threadIdx.x;
+    if (t >= K * nq) return;
+    const int k = t / nq, q = t - k * nq;
+    const int l = (int)sqrt((double)q + 0.5), m = q - l * l - l, am = m < 0 ? -m : m;
+    const double *pl = P + ((size_t)am * LMAX + l) * S;
+    const double2 *Gk = G + (size_t)k * S * nm + m + LMAX - 1;
+    double cr = 0.0, ci = 0.0;
+    for (int j = 0; j < S; ++j) {
+        const double wp = W[j] * pl[j];
+        const double2 a = Gk[(size_t)j * nm];
+        cr += wp * a.x;
+        ci += wp * a.y;
     }
+    const double sg = (m < 0 && (am & 1)) ? -1.0 : 1.0;
+    c[(size_t)k * nq + q] = make_double2(sg * cr, sg * ci);
 }
 
 // out[ko][q] = (+-i)^l sum_ki H[l][ko][ki] in[ki][q], l of q; one thread per (ko, q)
