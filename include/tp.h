@@ -231,6 +231,33 @@ int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi,
  * over the shells of Pearson's correlation of the real surfaces on the
  * (2 LMAX)^2 grid of the inverse transform.  With timing on, tp_last_timings
  * gives the rotation and similarity kernel times of the last call. */
+/* Synthetic sparse pictures: pointsPerOrientation (datagen.jl:88-144) for
+ * npictures orientations at once.  The intensity volume is given by its
+ * coefficients (K shells of ld complex, shell spacing rmax / K) and sampled
+ * through its (2 LMAX)^2 surfaces with the reference's trilinear
+ * interpolation (getVolumeInterpolated, spherical_harmonics.jl:484-530) and
+ * envelope bound 1.3 max(surface).  Picture i uses the random streams of
+ * (seed, first_picture + i): a Haar rotation (returned row-major in
+ * rotations[9 i], if not NULL), photons_per_image Gaussian incident photons of
+ * width envelope_sigma, one acceptance uniform per photon.  Accepted photons,
+ * in incident order and at their detector positions (x, y, 0), go to
+ * photons[(i capacity + j) 3 ..] for j < capacity; counts[i] is their number,
+ * which may exceed capacity (the excess is dropped). */
+typedef struct tp_picture_params {
+    double qcut, envelope_sigma, lambda, beamstop_width;
+    int photons_per_image;
+    int capacity;
+    uint64_t seed;
+} tp_picture_params;
+int tp_pictures(tp_ctx *ctx, int LMAX, int K, double rmax, const double *coeff, int64_t ld, const tp_picture_params *p,
+                int npictures, int first_picture, double *photons, int32_t *counts, double *rotations);
+/* generateHistogram's inner loop (datagen.jl:391-415) on the device: the
+ * pictures of tp_pictures (capacity <= 512) histogrammed into h as
+ * tp_hist_add does, without leaving the device; overflow (may be NULL) = the
+ * number of accepted photons dropped beyond capacity. */
+int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double *coeff, int64_t ld,
+                          const tp_picture_params *p, int npictures, int first_picture, int64_t *overflow);
+
 /* Spherical-harmonics transforms on s2kit's (2 LMAX)^2 grid (theta_j = pi (2j+1) /
  * (4 LMAX), phi_i = 2 pi i / (2 LMAX), samples theta-major), with
  * Ylm = sphPlm e^{i m phi}: tp_sh_surface = getSurfaceVolume (backtransform,

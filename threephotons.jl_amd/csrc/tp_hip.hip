@@ -25,6 +25,7 @@
 #include "tp_retrieve.hpp"
 #include "tp_fit.hpp"
 #include "tp_phase.hpp"
+#include "tp_pictures.hpp"
 
 // ---------------------------------------------------------------------------
 // errors
@@ -1859,8 +1860,8 @@ int tp_hist_add(tp_hist *h, const double *photons, const int64_t *offsets, int n
     if (nph > 0) TP_HIP(hipMemcpyAsync(h->d_ph, photons, sizeof(double) * 3 * nph, hipMemcpyHostToDevice, st));
     TP_HIP(hipMemcpyAsync(h->d_off, offsets, sizeof(int64_t) * (npictures + 1), hipMemcpyHostToDevice, st));
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    hipLaunchKernelGGL(k_hist_pictures, dim3((unsigned)npictures), dim3(kHistThreads), 0, st, h->d_ph, h->d_off, h->dq,
-                       h->N, h->K2, h->K3, h->d_n1, h->d_n2, h->d_n3);
+    hipLaunchKernelGGL(k_hist_pictures, dim3((unsigned)npictures), dim3(kHistThreads), 0, st, h->d_ph, h->d_off,
+                       (const int *)nullptr, 0, h->dq, h->N, h->K2, h->K3, h->d_n1, h->d_n2, h->d_n3);
     TP_HIP(hipGetLastError());
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     TP_HIP(hipStreamSynchronize(st));
@@ -2471,6 +2472,131 @@ int tp_sraar(tp_ctx *ctx, int LMAX, int K, const double *intensity, int64_t ld, 
         TP_HIP(hipMemcpyAsync(diff_out, diff.p, sizeof(double) * iterations, hipMemcpyDeviceToHost, st));
     TP_HIP(hipStreamSynchronize(st));
     ph_unpack(h, K, LMAX, coeff_out, ld);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// synthetic pictures (pointsPerOrientation, datagen.jl:88-144) and the
+// histogram generation driver over them (generateHistogram, datagen.jl:363-415)
+// ---------------------------------------------------------------------------
+// The shells' surfaces (getSurfaceVolume) on the device and the envelope bound
+// maxIntensity = 1.3 max Re surf (datagen.jl:89-92 for a SurfaceVolume).
+static int pic_surfaces(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld, DevBuf<double2> &surf,
+                        double &maxI)
+{
+    PhTables t;
+    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const size_t NP = (size_t)4 * LMAX * LMAX;
+    DevBuf<double2> c;
+    if (int rc = dev_alloc(&c.p, (size_t)K * LMAX * LMAX)) return rc;
+    if (int rc = dev_alloc(&surf.p, (size_t)K * NP)) return rc;
+    const std::vector<double2> h = ph_pack(coeff, ld, K, LMAX);
+    TP_HIP(hipMemcpyAsync(c.p, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    ph_synth(ctx, t, c.p, surf.p);
+    TP_HIP(hipGetLastError());
+    std::vector<double2> hs((size_t)K * NP);
+    TP_HIP(hipMemcpyAsync(hs.data(), surf.p, sizeof(double2) * hs.size(), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    double mx = -INFINITY;
+    for (const double2 &v : hs) mx = std::max(mx, v.x);
+    maxI = mx * 1.3;
+    return 0;
+}
+
+static int pic_check(const tp_picture_params *p, int LMAX, int K, double rmax, int npictures)
+{
+    if (!p) return tp_fail(TP_E_ARG, "null parameters");
+    if (LMAX < 1 || LMAX > kPhMaxLMAX || K < 1 || !(rmax > 0.0)) return tp_fail(TP_E_ARG, "need 1 <= LMAX <= 32, K >= 1, rmax > 0");
+    if (npictures < 0 || p->photons_per_image < 0 || p->capacity < 1) return tp_fail(TP_E_ARG, "bad picture counts");
+    if (!(p->envelope_sigma > 0.0) || !(p->qcut > 0.0)) return tp_fail(TP_E_ARG, "need qcut, envelope_sigma > 0");
+    return 0;
+}
+
+int tp_pictures(tp_ctx *ctx, int LMAX, int K, double rmax, const double *coeff, int64_t ld, const tp_picture_params *p,
+                int npictures, int first_picture, double *photons, int32_t *counts, double *rotations)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (int rc = pic_check(p, LMAX, K, rmax, npictures)) return rc;
+    if (!coeff || !photons || !counts || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
+    if (npictures == 0) return 0;
+    DevBuf<double2> surf;
+    double maxI = 0.0;
+    if (int rc = pic_surfaces(ctx, LMAX, K, coeff, ld, surf, maxI)) return rc;
+    DevBuf<double> dph, drot;
+    DevBuf<int> dcnt;
+    if (int rc = dev_alloc(&dph.p, (size_t)npictures * p->capacity * 3)) return rc;
+    if (int rc = dev_alloc(&dcnt.p, (size_t)npictures)) return rc;
+    if (rotations)
+        if (int rc = dev_alloc(&drot.p, (size_t)npictures * 9)) return rc;
+    hipStream_t st = ctx->stream;
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
+    hipLaunchKernelGGL(k_pictures, dim3((unsigned)npictures), dim3(kPicThreads), 0, st, surf.p, LMAX, K, rmax / K, maxI,
+                       p->qcut, p->envelope_sigma, p->photons_per_image, p->lambda, p->beamstop_width, p->seed,
+                       first_picture, p->capacity, dph.p, dcnt.p, drot.p);
+    TP_HIP(hipGetLastError());
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+    TP_HIP(hipMemcpyAsync(photons, dph.p, sizeof(double) * (size_t)npictures * p->capacity * 3, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipMemcpyAsync(counts, dcnt.p, sizeof(int) * (size_t)npictures, hipMemcpyDeviceToHost, st));
+    if (rotations) TP_HIP(hipMemcpyAsync(rotations, drot.p, sizeof(double) * 9 * (size_t)npictures, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    if (ctx->timing) {
+        float tm = 0;
+        if (hipEventElapsedTime(&tm, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->last_ms[0] = tm;
+        for (int i = 1; i < 4; ++i) ctx->last_ms[i] = 0.0;
+    }
+    return 0;
+}
+
+int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double *coeff, int64_t ld,
+                          const tp_picture_params *p, int npictures, int first_picture, int64_t *overflow)
+{
+    if (!h) return tp_fail(TP_E_ARG, "null histogram");
+    tp_ctx *ctx = h->ctx;
+    if (int rc = check_ctx(ctx)) return rc;
+    if (int rc = pic_check(p, LMAX, K, rmax, npictures)) return rc;
+    if (!coeff || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
+    if (p->capacity > kHistMaxPhotons) return tp_fail(TP_E_UNSUPPORTED, "capacity above 512 photons per picture");
+    if (npictures == 0) return 0;
+    DevBuf<double2> surf;
+    double maxI = 0.0;
+    if (int rc = pic_surfaces(ctx, LMAX, K, coeff, ld, surf, maxI)) return rc;
+    const int batch = std::min(npictures, 8192);
+    DevBuf<double> dph;
+    DevBuf<int> dcnt;
+    if (int rc = dev_alloc(&dph.p, (size_t)batch * p->capacity * 3)) return rc;
+    if (int rc = dev_alloc(&dcnt.p, (size_t)batch)) return rc;
+    hipStream_t st = ctx->stream;
+    std::vector<int> hc(batch);
+    int64_t over = 0;
+    float t_gen = 0.f, t_hist = 0.f;
+    for (int b0 = 0; b0 < npictures; b0 += batch) {
+        const int nb = std::min(batch, npictures - b0);
+        if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
+        hipLaunchKernelGGL(k_pictures, dim3((unsigned)nb), dim3(kPicThreads), 0, st, surf.p, LMAX, K, rmax / K, maxI,
+                           p->qcut, p->envelope_sigma, p->photons_per_image, p->lambda, p->beamstop_width, p->seed,
+                           first_picture + b0, p->capacity, dph.p, dcnt.p, (double *)nullptr);
+        TP_HIP(hipGetLastError());
+        if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+        hipLaunchKernelGGL(k_hist_pictures, dim3((unsigned)nb), dim3(kHistThreads), 0, st, dph.p, (const int64_t *)nullptr,
+                           dcnt.p, p->capacity, h->dq, h->N, h->K2, h->K3, h->d_n1, h->d_n2, h->d_n3);
+        TP_HIP(hipGetLastError());
+        if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
+        TP_HIP(hipMemcpyAsync(hc.data(), dcnt.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, st));
+        TP_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < nb; ++i) over += std::max(0, hc[i] - p->capacity);
+        if (ctx->timing) {
+            float a = 0.f, c = 0.f;
+            if (hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]) == hipSuccess) t_gen += a;
+            if (hipEventElapsedTime(&c, ctx->ev[1], ctx->ev[2]) == hipSuccess) t_hist += c;
+        }
+    }
+    if (ctx->timing) {
+        ctx->last_ms[0] = t_gen;
+        ctx->last_ms[1] = t_hist;
+        ctx->last_ms[2] = ctx->last_ms[3] = 0.0;
+    }
+    h->pictures += npictures;
+    if (overflow) *overflow = over;
     return 0;
 }
 

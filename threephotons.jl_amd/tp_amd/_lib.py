@@ -92,6 +92,8 @@ def lib():
     L.tp_sh_coeff.argtypes = [vp, i32, i32, vp, vp, i64]
     L.tp_sph_bessel.argtypes = [vp, i32, i32, i32, vp, i64, vp]
     L.tp_sraar.argtypes = [vp, i32, i32, vp, i64, i32, dbl, dbl, dbl, dbl, vp, C.c_uint64, vp, vp]
+    L.tp_pictures.argtypes = [vp, i32, i32, dbl, vp, i64, vp, i32, i32, vp, vp, vp]
+    L.tp_hist_add_generated.argtypes = [vp, i32, i32, dbl, vp, i64, vp, i32, i32, vp]
     L.tp_batch_create.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, i32, pp]
     L.tp_batch_destroy.argtypes = [vp]
     L.tp_batch_destroy.restype = None
@@ -130,6 +132,18 @@ def check(rc, what=""):
         raise TPError(f"{what} failed (rc={rc}): {msg}")
 
 
+class PictureParams(C.Structure):
+    _fields_ = [
+        ("qcut", C.c_double),
+        ("envelope_sigma", C.c_double),
+        ("lam", C.c_double),
+        ("beamstop_width", C.c_double),
+        ("photons_per_image", C.c_int),
+        ("capacity", C.c_int),
+        ("seed", C.c_uint64),
+    ]
+
+
 class AnnealParams(C.Structure):
     _fields_ = [
         ("nsteps", C.c_int),
@@ -156,7 +170,7 @@ EXPORTS = [
     "tp_batch_set_family", "tp_basis_from_host", "tp_energy_grad",
     "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy", "tp_two_photons", "tp_retrieve_solution",
     "tp_fit_create", "tp_fit_set_volume", "tp_fit_similarity", "tp_fit_destroy",
-    "tp_sh_surface", "tp_sh_coeff", "tp_sph_bessel", "tp_sraar",
+    "tp_sh_surface", "tp_sh_coeff", "tp_sph_bessel", "tp_sraar", "tp_pictures", "tp_hist_add_generated",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
