@@ -170,6 +170,14 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
     t0 = time.perf_counter()
     phase_oracle.sraar(start.coeff, cfg["LMAX"], 3, 0.75, 0.9, 350.0, 0.3, php)
     t_ph = (time.perf_counter() - t0 - t_ph1) / 2
+    # picture synthesis: the oracle on 8 pictures of the same workload
+    from oracle import picture_oracle
+    surf = picture_oracle.surfaces(start.coeff, cfg["LMAX"])
+    t0 = time.perf_counter()
+    for p in range(8):
+        picture_oracle.points_per_orientation(surf, cfg["LMAX"], start.KMAX, start.rmax, start.rmax, start.rmax / 3.0,
+                                              1000, 0.0, 0.0, 7, p)
+    t_pic = (time.perf_counter() - t0) / 8
     tf = float(np.median(t_fact))
     return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
                 sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
@@ -184,6 +192,8 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
                 fit_rotations_per_s=1.0 / t_fit,
                 fit_sample="numpy oracle: rotateStructure + similarity of 2 rotations, 38 shells, L=18, LMAX=25",
                 sraar_ms_per_iteration=1e3 * t_ph,
+                picture_pictures_per_s=1.0 / t_pic,
+                picture_sample="numpy oracle: pointsPerOrientation of 8 pictures, 1000 incident photons each",
                 sraar_sample="numpy oracle: sRAAR with 3 and with 1 iterations, 38 shells, LMAX=25; the difference / 2",
                 small_config={"dense_evals_per_s": 1.0 / t_d0, "factored_evals_per_s": 1.0 / t_f0,
                               "energy": E0, "klength": ob0.klength, "R": 2 * sm["N"] ** 2})
@@ -421,6 +431,36 @@ def fit_leg(tp, truth, num):
     return out
 
 
+def generate_leg(tp, truth, npic):
+    """generateHistogram (datagen.jl:363-415) on the device: npic pictures of
+    the 38-shell intensity (pointsPerOrientation: 1000 incident photons,
+    envelope qcut/3, qcut = the volume's qmax, lambda = 0, as in
+    jobs/run_generate_histograms.jl) histogrammed at N = 32, K2 = 38, K3 = 26
+    without leaving the device; picture synthesis and histogram kernels timed
+    with HIP events."""
+    from tp_amd import pictures
+    ctx = tp.HIP_init()
+    qcut = truth.rmax
+    h = tp.PhotonHistogram(32, 38, 26, qcut / 38)
+    pictures.generate_histograms(h, truth, qcut, 256, photons_per_image=1000, lam=0.0, capacity=512, seed=3)
+    h = tp.PhotonHistogram(32, 38, 26, qcut / 38)
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    over = pictures.generate_histograms(h, truth, qcut, npic, photons_per_image=1000, lam=0.0, capacity=512, seed=7)
+    wall = time.perf_counter() - t0
+    gen_ms, hist_ms = ctx.last_timings()[:2]
+    ctx.set_timing(False)
+    r = h.read()
+    n1, n3 = int(r["n1"].sum()), int(r["n3"].sum())
+    out = {"workload": f"{npic} pictures of the {truth.KMAX}-shell intensity (LMAX={truth.LMAX}), 1000 incident "
+                       f"photons, N=32, K2=38, K3=26, lambda=0",
+           "call_s": wall, "pictures_per_s": npic / wall, "synthesis_ms": gen_ms, "histogram_ms": hist_ms,
+           "photons": n1, "photons_per_picture": n1 / npic, "triplets": n3, "overflow": over}
+    log(f"generateHistogram: {npic} pictures in {wall:.3f} s ({npic / wall:.3g}/s; synthesis {gen_ms:.1f} ms, "
+        f"histograms {hist_ms:.1f} ms), {n1 / npic:.1f} photons/picture, {n3} triplets")
+    return out
+
+
 def phase_leg(tp, truth, iters):
     """sRAAR (phases.jl:31-95) of the 38-shell intensity (LMAX = 25, 50 x 50
     surface grid per shell) with the reference's post-processing settings
@@ -578,6 +618,7 @@ def main():
     ap.add_argument("--c2-reps", type=int, default=20, help="timed twoPhotons calls of the two-photon leg (0: skip)")
     ap.add_argument("--fit-num", type=int, default=20, help="grid divisions of the rotational-fit leg (0: skip)")
     ap.add_argument("--sraar-iters", type=int, default=1001, help="iterations of the phase-retrieval leg (0: skip)")
+    ap.add_argument("--gen-pictures", type=int, default=65536, help="pictures of the histogram-generation leg (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -671,6 +712,7 @@ def main():
     c2 = c2_leg(tp, truth, basis, args.c2_reps) if args.c2_reps > 0 else None
     fitr = fit_leg(tp, truth, args.fit_num) if args.fit_num > 0 else None
     phase = phase_leg(tp, truth, args.sraar_iters) if args.sraar_iters > 0 else None
+    gen = generate_leg(tp, truth, args.gen_pictures) if args.gen_pictures > 0 else None
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:
@@ -736,6 +778,8 @@ def main():
         out["rotational_fit"] = fitr
     if phase is not None:
         out["phase_retrieval"] = phase
+    if gen is not None:
+        out["generated_histograms"] = gen
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)
