@@ -239,8 +239,13 @@ int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi,
  * envelope bound 1.3 max(surface).  Picture i uses the random streams of
  * (seed, first_picture + i): a Haar rotation (returned row-major in
  * rotations[9 i], if not NULL), photons_per_image Gaussian incident photons of
- * width envelope_sigma, one acceptance uniform per photon.  Accepted photons,
- * in incident order and at their detector positions (x, y, 0), go to
+ * width envelope_sigma, one acceptance uniform per photon; with particles > 1
+ * and noise, the segments of generateHistogram's loop (datagen.jl:399-407):
+ * each particle's photons, then its noise photons from the isotropic noise
+ * volume of get_noise_volume (datagen.jl:296-304) with the same rotation
+ * (pointsPerOrientation's incident_photon_variance for the particles only;
+ * beamstop_only replaces all of them by one beamstop shot, datagen.jl:408-411).
+ * Accepted photons, in incident order and at their detector positions (x, y, 0), go to
  * photons[(i capacity + j) 3 ..] for j < capacity; counts[i] is their number,
  * which may exceed capacity (the excess is dropped). */
 typedef struct tp_picture_params {
@@ -248,6 +253,11 @@ typedef struct tp_picture_params {
     int photons_per_image;
     int capacity;
     uint64_t seed;
+    int particles;        /* number_particles (>= 1): one rotation and set of photons each */
+    int noise_photons;    /* GaussianNoise photons per particle (0: no noise) */
+    double noise_sigma;   /* GaussianNoise sigma: noise volume c00(k) = gaussian(k dq; sigma), envelope 1.05 sigma */
+    int incident_photon_variance; /* > 0: each particle's incident count is round(Normal(photons_per_image, this)), >= 0 */
+    int beamstop_only;    /* nonzero: beamstop_shot (datagen.jl:147-161) instead of the particles, identity rotation */
 } tp_picture_params;
 int tp_pictures(tp_ctx *ctx, int LMAX, int K, double rmax, const double *coeff, int64_t ld, const tp_picture_params *p,
                 int npictures, int first_picture, double *photons, int32_t *counts, double *rotations);

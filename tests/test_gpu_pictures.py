@@ -81,3 +81,55 @@ def test_capacity_overflow_is_reported(tp):
         pictures.pointsPerOrientation(vol, 1.0, 3, photons_per_image=800, capacity=2, seed=1)
     h = tp.PhotonHistogram(8, 8, 6, 1.0 / 8)
     assert pictures.generate_histograms(h, vol, 1.0, 3, photons_per_image=800, capacity=2, seed=1) > 0
+
+
+def test_particles_and_noise_match_oracle(tp, pic):
+    """Two particles per picture, each followed by Gaussian noise photons
+    (generateHistogram's segments), against the oracle on the same streams."""
+    from tp_amd import pictures
+    LMAX, K, qcut = 6, 10, 1.0
+    vol = _volume(tp, LMAX, K, qcut, 77)
+    kw = dict(photons_per_image=150, lam=0.0, capacity=512, seed=9, particles=2, noise_photons=100,
+              noise_sigma=0.4)
+    got, _ = pictures.pointsPerOrientation(vol, qcut, 4, **kw)
+    surf = pic.surfaces(vol.coeff, LMAX)
+    nsurf = pic.surfaces(pic.noise_coefficients(K, LMAX, qcut, 0.4), LMAX)
+    for i in range(4):
+        want = pic.picture(surf, nsurf, LMAX, K, qcut, qcut, qcut / 3.0, 150, 0.0, 0.0, 9, i, 2, 100, 0.4)
+        say(f"  picture {i}: {len(got[i])} photons (oracle {len(want)})")
+        assert len(got[i]) == len(want) and len(want) > 0
+        assert np.abs(got[i] - want).max() <= 1e-13
+
+    # the fused generation draws the same segments
+    h1 = tp.PhotonHistogram(8, 10, 8, qcut / 10)
+    off = np.concatenate([[0], np.cumsum([len(p) for p in got])]).astype(np.int64)
+    h1.add(np.vstack(got), off)
+    h2 = tp.PhotonHistogram(8, 10, 8, qcut / 10)
+    assert pictures.generate_histograms(h2, vol, qcut, 4, **kw) == 0
+    a, b = h1.read(), h2.read()
+    for key in ("n1", "n2", "n3"):
+        assert np.array_equal(a[key], b[key])
+
+
+def test_photon_variance_and_beamstop_shots_match_oracle(tp, pic):
+    """incident_photon_variance (per-particle counts) and beamstop_only
+    pictures against the oracle on the same streams."""
+    from tp_amd import pictures
+    LMAX, K, qcut = 6, 10, 1.0
+    vol = _volume(tp, LMAX, K, qcut, 78)
+    got, _ = pictures.pointsPerOrientation(vol, qcut, 5, photons_per_image=200, lam=0.5, capacity=512, seed=13,
+                                           first_picture=3, particles=2, incident_photon_variance=40)
+    surf = pic.surfaces(vol.coeff, LMAX)
+    for i in range(5):
+        want = pic.picture(surf, None, LMAX, K, qcut, qcut, qcut / 3.0, 200, 0.5, 0.0, 13, 3 + i, 2, 0, 1.0,
+                           variance=40)
+        say(f"  picture {3 + i}: {len(got[i])} photons (oracle {len(want)})")
+        assert len(got[i]) == len(want) and len(want) > 0
+        assert np.abs(got[i] - want).max() <= 1e-13
+    shots, rot = pictures.pointsPerOrientation(vol, qcut, 3, photons_per_image=300, capacity=512, seed=2,
+                                               beamstop_width=0.1, beamstop_only=True)
+    assert np.array_equal(rot, np.broadcast_to(np.eye(3), (3, 3, 3)))
+    for i in range(3):
+        want = pic.beamstop_shot(qcut, 300, 0.1, 2, i)
+        assert len(shots[i]) == len(want) > 0
+        assert np.abs(shots[i] - want).max() <= 1e-15

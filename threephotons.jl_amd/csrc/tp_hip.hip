@@ -2503,13 +2503,62 @@ static int pic_surfaces(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64
     return 0;
 }
 
+// Surfaces of the noise volume of get_noise_volume (datagen.jl:296-304): every
+// coefficient deleted, c00(k) = gaussian_distribution(k dr; 0, sigma)
+// (utilities.jl:129-133).
+static int pic_noise_surfaces(tp_ctx *ctx, int LMAX, int K, double rmax, double sigma, DevBuf<double2> &surf,
+                              double &maxI)
+{
+    std::vector<double> c((size_t)2 * K * LMAX * LMAX, 0.0);
+    const double s2 = 2.0 * sigma * sigma;
+    for (int k = 1; k <= K; ++k) {
+        const double q = k * (rmax / K);
+        c[(size_t)2 * (k - 1) * LMAX * LMAX] = 1 / std::sqrt(tph::kPi * s2) * std::exp(-(q * q) / s2);
+    }
+    return pic_surfaces(ctx, LMAX, K, c.data(), (int64_t)LMAX * LMAX, surf, maxI);
+}
+
 static int pic_check(const tp_picture_params *p, int LMAX, int K, double rmax, int npictures)
 {
     if (!p) return tp_fail(TP_E_ARG, "null parameters");
+    if (p->particles < 1 || p->noise_photons < 0 || (p->noise_photons > 0 && !(p->noise_sigma > 0.0)))
+        return tp_fail(TP_E_ARG, "need particles >= 1 and, with noise photons, noise_sigma > 0");
     if (LMAX < 1 || LMAX > kPhMaxLMAX || K < 1 || !(rmax > 0.0)) return tp_fail(TP_E_ARG, "need 1 <= LMAX <= 32, K >= 1, rmax > 0");
-    if (npictures < 0 || p->photons_per_image < 0 || p->capacity < 1) return tp_fail(TP_E_ARG, "bad picture counts");
+    if (npictures < 0 || p->photons_per_image < 0 || p->capacity < 1 || p->incident_photon_variance < 0)
+        return tp_fail(TP_E_ARG, "bad picture counts");
+    // photon indices 3 i + 1 of a segment stay below 2^31 (a count normal beyond 20 deviations never comes)
+    if ((double)p->photons_per_image + 20.0 * p->incident_photon_variance > (double)(1 << 28) ||
+        p->noise_photons > (1 << 28))
+        return tp_fail(TP_E_UNSUPPORTED, "more than 2^28 incident photons per segment");
     if (!(p->envelope_sigma > 0.0) || !(p->qcut > 0.0)) return tp_fail(TP_E_ARG, "need qcut, envelope_sigma > 0");
     return 0;
+}
+
+static PicArgs pic_args(const tp_picture_params *p, int LMAX, int K, double rmax, const double2 *surf, double maxI,
+                        const double2 *nsurf, double nmaxI, int first)
+{
+    PicArgs a;
+    a.surf = surf;
+    a.nsurf = nsurf;
+    a.LMAX = LMAX;
+    a.K = K;
+    a.dr = rmax / K;
+    a.maxI = maxI;
+    a.nmaxI = nmaxI;
+    a.qcut = p->qcut;
+    a.sigma = p->envelope_sigma;
+    a.nsigma = 1.05 * p->noise_sigma;   // datagen.jl:404
+    a.lambda = p->lambda;
+    a.bw = p->beamstop_width;
+    a.n_inc = p->photons_per_image;
+    a.n_noise = p->beamstop_only ? 0 : p->noise_photons;
+    a.variance = p->incident_photon_variance;
+    a.particles = p->particles;
+    a.beamstop_only = p->beamstop_only ? 1 : 0;
+    a.cap = p->capacity;
+    a.first = first;
+    a.seed = p->seed;
+    return a;
 }
 
 int tp_pictures(tp_ctx *ctx, int LMAX, int K, double rmax, const double *coeff, int64_t ld, const tp_picture_params *p,
@@ -2519,9 +2568,11 @@ int tp_pictures(tp_ctx *ctx, int LMAX, int K, double rmax, const double *coeff, 
     if (int rc = pic_check(p, LMAX, K, rmax, npictures)) return rc;
     if (!coeff || !photons || !counts || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
     if (npictures == 0) return 0;
-    DevBuf<double2> surf;
-    double maxI = 0.0;
+    DevBuf<double2> surf, nsurf;
+    double maxI = 0.0, nmaxI = 0.0;
     if (int rc = pic_surfaces(ctx, LMAX, K, coeff, ld, surf, maxI)) return rc;
+    if (p->noise_photons > 0 && !p->beamstop_only)
+        if (int rc = pic_noise_surfaces(ctx, LMAX, K, rmax, p->noise_sigma, nsurf, nmaxI)) return rc;
     DevBuf<double> dph, drot;
     DevBuf<int> dcnt;
     if (int rc = dev_alloc(&dph.p, (size_t)npictures * p->capacity * 3)) return rc;
@@ -2530,9 +2581,8 @@ int tp_pictures(tp_ctx *ctx, int LMAX, int K, double rmax, const double *coeff, 
         if (int rc = dev_alloc(&drot.p, (size_t)npictures * 9)) return rc;
     hipStream_t st = ctx->stream;
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    hipLaunchKernelGGL(k_pictures, dim3((unsigned)npictures), dim3(kPicThreads), 0, st, surf.p, LMAX, K, rmax / K, maxI,
-                       p->qcut, p->envelope_sigma, p->photons_per_image, p->lambda, p->beamstop_width, p->seed,
-                       first_picture, p->capacity, dph.p, dcnt.p, drot.p);
+    hipLaunchKernelGGL(k_pictures, dim3((unsigned)npictures), dim3(kPicThreads), 0, st,
+                       pic_args(p, LMAX, K, rmax, surf.p, maxI, nsurf.p, nmaxI, first_picture), dph.p, dcnt.p, drot.p);
     TP_HIP(hipGetLastError());
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     TP_HIP(hipMemcpyAsync(photons, dph.p, sizeof(double) * (size_t)npictures * p->capacity * 3, hipMemcpyDeviceToHost, st));
@@ -2557,9 +2607,11 @@ int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double
     if (!coeff || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
     if (p->capacity > kHistMaxPhotons) return tp_fail(TP_E_UNSUPPORTED, "capacity above 512 photons per picture");
     if (npictures == 0) return 0;
-    DevBuf<double2> surf;
-    double maxI = 0.0;
+    DevBuf<double2> surf, nsurf;
+    double maxI = 0.0, nmaxI = 0.0;
     if (int rc = pic_surfaces(ctx, LMAX, K, coeff, ld, surf, maxI)) return rc;
+    if (p->noise_photons > 0 && !p->beamstop_only)
+        if (int rc = pic_noise_surfaces(ctx, LMAX, K, rmax, p->noise_sigma, nsurf, nmaxI)) return rc;
     const int batch = std::min(npictures, 8192);
     DevBuf<double> dph;
     DevBuf<int> dcnt;
@@ -2572,9 +2624,9 @@ int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double
     for (int b0 = 0; b0 < npictures; b0 += batch) {
         const int nb = std::min(batch, npictures - b0);
         if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-        hipLaunchKernelGGL(k_pictures, dim3((unsigned)nb), dim3(kPicThreads), 0, st, surf.p, LMAX, K, rmax / K, maxI,
-                           p->qcut, p->envelope_sigma, p->photons_per_image, p->lambda, p->beamstop_width, p->seed,
-                           first_picture + b0, p->capacity, dph.p, dcnt.p, (double *)nullptr);
+        hipLaunchKernelGGL(k_pictures, dim3((unsigned)nb), dim3(kPicThreads), 0, st,
+                           pic_args(p, LMAX, K, rmax, surf.p, maxI, nsurf.p, nmaxI, first_picture + b0), dph.p, dcnt.p,
+                           (double *)nullptr);
         TP_HIP(hipGetLastError());
         if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
         hipLaunchKernelGGL(k_hist_pictures, dim3((unsigned)nb), dim3(kHistThreads), 0, st, dph.p, (const int64_t *)nullptr,

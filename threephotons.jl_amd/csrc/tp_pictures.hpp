@@ -19,15 +19,38 @@
 #define TP_TAG_PROT 0x50524f54u   // picture rotation normals
 #define TP_TAG_PPHO 0x5050484fu   // incident photon normals
 #define TP_TAG_PACC 0x50414343u   // acceptance uniforms
+#define TP_TAG_PCNT 0x50434e54u   // photon-count normals (incident_photon_variance)
+#define TP_TAG_PBST 0x50425354u   // beamstop-shot uniforms
 
 constexpr int kPicThreads = 256;
 
-// uniform number i of the stream (seed, picture, tag)
-__device__ inline double pic_uniform(uint64_t seed, uint32_t pic, uint32_t i)
+// What one launch of k_pictures draws (passed by value).
+struct PicArgs {
+    const double2 *surf;    // (K, (2 LMAX)^2) intensity surfaces
+    const double2 *nsurf;   // noise surfaces (n_noise > 0)
+    int LMAX, K;
+    double dr;              // shell spacing rmax / K
+    double maxI, nmaxI;     // envelope bounds 1.3 max(surface)
+    double qcut, sigma, nsigma, lambda, bw;
+    int n_inc, n_noise, variance, particles, beamstop_only, cap, first;
+    uint64_t seed;
+};
+
+// acceptance uniform of photon i of segment seg of picture pic
+__device__ inline double pic_uniform(uint64_t seed, uint32_t pic, uint32_t i, uint32_t seg)
 {
-    uint32_t c[4] = {i, 0xFFFFFFFEu, pic, TP_TAG_PACC};
+    uint32_t c[4] = {i, 0xFFFFFFFEu - seg, pic, TP_TAG_PACC};
     philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     return u01(c[0], c[1]);
+}
+
+// incident photons of segment seg: round(Normal(n, variance)) (ties to even,
+// as Julia's round), at least 0
+__device__ inline int pic_count(uint64_t seed, uint32_t pic, uint32_t seg, int n, int variance)
+{
+    if (variance <= 0) return n;
+    const double v = rint((double)n + (double)variance * dev_normal(seed, pic, seg, 0, TP_TAG_PCNT));
+    return v > 0.0 ? (int)v : 0;
 }
 
 // surface index (0-based) of ang_to_i (spherical_harmonics.jl:213-221)
@@ -77,78 +100,108 @@ __device__ inline double pic_interp(const double2 *__restrict__ surf, int LMAX, 
            V011 * s0[i11].x + V110 * s1[i10].x + V111 * s1[i11].x;
 }
 
-// One workgroup per picture first + blockIdx.x: accepted photons to
-// ph[(blockIdx.x cap + j) 3 + (0,1,2)] for j < cap, their number to cnt (it may
-// exceed cap: the host reports the overflow), R to rot (row-major) if given.
-__global__ __launch_bounds__(kPicThreads) void k_pictures(const double2 *__restrict__ surf, int LMAX, int K, double dr,
-                                                          double maxI, double qcut, double sigma, int n_inc,
-                                                          double lambda, double bw, uint64_t seed, int first, int cap,
-                                                          double *__restrict__ ph, int *__restrict__ cnt,
-                                                          double *__restrict__ rot)
+// One workgroup per picture a.first + blockIdx.x, made of segments in the
+// reference's order (generateHistogram, datagen.jl:399-407): for each of the
+// `particles` particles, its photons from the intensity (segment 2p) and, if
+// n_noise > 0, noise photons from the noise surfaces with the same rotation
+// (segment 2p + 1, envelope nsigma).  Particle p's rotation uses the normals
+// of index 0xFFFFFFFF - p; segment s the photon normals of index s, the
+// uniforms with word 0xFFFFFFFE - s and, with variance > 0, its photon count
+// from the count normal of index s (noise segments keep n_noise).  With
+// beamstop_only, one segment of beamstop_shot (datagen.jl:147-161): photons
+// uniform on [-qcut, qcut]^2 (two uniforms of the Philox block {i,
+// 0xFFFFFFFE, picture, PBST}) kept outside the beamstop and inside |p| <= qcut,
+// identity rotation.  Accepted photons go to ph[(blockIdx.x cap + j) 3 +
+// (0,1,2)] for j < cap, their number to cnt (it may exceed cap: the host
+// reports the overflow), particle 0's rotation to rot (row-major) if given.
+__global__ __launch_bounds__(kPicThreads) void k_pictures(const PicArgs a, double *__restrict__ ph,
+                                                          int *__restrict__ cnt, double *__restrict__ rot)
 {
 #pragma clang fp contract(off)
     __shared__ double R[9];
     __shared__ int s_base, s_wave[kPicThreads / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t pic = (uint32_t)(first + (int)blockIdx.x);
-    if (tid == 0) {
-        double w = dev_normal(seed, pic, 0xFFFFFFFFu, 0, TP_TAG_PROT), x = dev_normal(seed, pic, 0xFFFFFFFFu, 1, TP_TAG_PROT);
-        double y = dev_normal(seed, pic, 0xFFFFFFFFu, 2, TP_TAG_PROT), z = dev_normal(seed, pic, 0xFFFFFFFFu, 3, TP_TAG_PROT);
-        const double nq = sqrt(w * w + x * x + y * y + z * z);
-        w /= nq; x /= nq; y /= nq; z /= nq;
-        R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - w * z);     R[2] = 2 * (x * z + w * y);
-        R[3] = 2 * (x * y + w * z);     R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - w * x);
-        R[6] = 2 * (x * z - w * y);     R[7] = 2 * (y * z + w * x);     R[8] = 1 - 2 * (x * x + y * y);
-        s_base = 0;
-        if (rot)
-            for (int i = 0; i < 9; ++i) rot[(size_t)blockIdx.x * 9 + i] = R[i];
-    }
-    __syncthreads();
-    // log of the envelope density at 0 (MvNormal with standard deviations sigma)
-    const double lg0 = -1.5 * log(2.0 * kPiD) - 3.0 * log(sigma);
-    const double max_gauss = exp(lg0);
-    for (int i0 = 0; i0 < n_inc; i0 += kPicThreads) {
-        const int i = i0 + tid;
-        bool acc = false;
-        double x = 0.0, y = 0.0;
-        if (i < n_inc) {
-            x = sigma * dev_normal(seed, pic, 0u, (uint32_t)(3 * i), TP_TAG_PPHO);
-            y = sigma * dev_normal(seed, pic, 0u, (uint32_t)(3 * i + 1), TP_TAG_PPHO);
-            if (fabs(x) > bw && fabs(y) > bw) {
-                const double qn = sqrt(x * x + y * y);
-                if (qn <= qcut) {
-                    const double th = acos(qn * lambda / (4 * kPiD));
-                    const double c = cos(th), s = sin(th);
-                    const double e0 = s * x, e1 = s * y, e2 = c * qn;
-                    const double r0 = R[0] * e0 + R[1] * e1 + R[2] * e2;
-                    const double r1 = R[3] * e0 + R[4] * e1 + R[5] * e2;
-                    const double r2 = R[6] * e0 + R[7] * e1 + R[8] * e2;
-                    const double I = pic_interp(surf, LMAX, K, dr, r0, r1, r2);
-                    const double gx = exp(lg0 - 0.5 * ((r0 / sigma) * (r0 / sigma) + (r1 / sigma) * (r1 / sigma) +
-                                                       (r2 / sigma) * (r2 / sigma)));
-                    const double ratio = I / (maxI / max_gauss * gx);
-                    acc = pic_uniform(seed, pic, (uint32_t)i) < ratio;
+    const uint32_t pic = (uint32_t)(a.first + (int)blockIdx.x);
+    const uint64_t seed = a.seed;
+    if (tid == 0) s_base = 0;
+    const int nseg = a.beamstop_only ? 1 : a.particles * (a.n_noise > 0 ? 2 : 1);
+    for (int seg = 0; seg < nseg; ++seg) {
+        const int part = a.n_noise > 0 ? seg >> 1 : seg;
+        const bool noise = !a.beamstop_only && a.n_noise > 0 && (seg & 1);
+        __syncthreads();   // R of the previous segment is no longer read
+        if (tid == 0 && !noise) {
+            if (a.beamstop_only) {
+                for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+            } else {
+                const uint32_t rl = 0xFFFFFFFFu - (uint32_t)part;
+                double w = dev_normal(seed, pic, rl, 0, TP_TAG_PROT), x = dev_normal(seed, pic, rl, 1, TP_TAG_PROT);
+                double y = dev_normal(seed, pic, rl, 2, TP_TAG_PROT), z = dev_normal(seed, pic, rl, 3, TP_TAG_PROT);
+                const double nq = sqrt(w * w + x * x + y * y + z * z);
+                w /= nq; x /= nq; y /= nq; z /= nq;
+                R[0] = 1 - 2 * (y * y + z * z); R[1] = 2 * (x * y - w * z);     R[2] = 2 * (x * z + w * y);
+                R[3] = 2 * (x * y + w * z);     R[4] = 1 - 2 * (x * x + z * z); R[5] = 2 * (y * z - w * x);
+                R[6] = 2 * (x * z - w * y);     R[7] = 2 * (y * z + w * x);     R[8] = 1 - 2 * (x * x + y * y);
+            }
+            if (rot && part == 0)
+                for (int i = 0; i < 9; ++i) rot[(size_t)blockIdx.x * 9 + i] = R[i];
+        }
+        __syncthreads();
+        const double2 *sv = noise ? a.nsurf : a.surf;
+        const double sg = noise ? a.nsigma : a.sigma, mI = noise ? a.nmaxI : a.maxI;
+        const int n = noise ? a.n_noise
+                            : (a.beamstop_only ? a.n_inc : pic_count(seed, pic, (uint32_t)seg, a.n_inc, a.variance));
+        // log of the envelope density at 0 (MvNormal with standard deviations sg)
+        const double lg0 = -1.5 * log(2.0 * kPiD) - 3.0 * log(sg);
+        const double max_gauss = exp(lg0);
+        for (int i0 = 0; i0 < n; i0 += kPicThreads) {
+            const int i = i0 + tid;
+            bool acc = false;
+            double x = 0.0, y = 0.0;
+            if (i < n && a.beamstop_only) {
+                uint32_t c[4] = {(uint32_t)i, 0xFFFFFFFEu, pic, TP_TAG_PBST};
+                philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+                x = 2 * a.qcut * (u01(c[0], c[1]) - 0.5);
+                y = 2 * a.qcut * (u01(c[2], c[3]) - 0.5);
+                acc = fabs(x) > a.bw && fabs(y) > a.bw && sqrt(x * x + y * y) <= a.qcut;
+            } else if (i < n) {
+                x = sg * dev_normal(seed, pic, (uint32_t)seg, (uint32_t)(3 * i), TP_TAG_PPHO);
+                y = sg * dev_normal(seed, pic, (uint32_t)seg, (uint32_t)(3 * i + 1), TP_TAG_PPHO);
+                if (fabs(x) > a.bw && fabs(y) > a.bw) {
+                    const double qn = sqrt(x * x + y * y);
+                    if (qn <= a.qcut) {
+                        const double th = acos(qn * a.lambda / (4 * kPiD));
+                        const double c = cos(th), s = sin(th);
+                        const double e0 = s * x, e1 = s * y, e2 = c * qn;
+                        const double r0 = R[0] * e0 + R[1] * e1 + R[2] * e2;
+                        const double r1 = R[3] * e0 + R[4] * e1 + R[5] * e2;
+                        const double r2 = R[6] * e0 + R[7] * e1 + R[8] * e2;
+                        const double I = pic_interp(sv, a.LMAX, a.K, a.dr, r0, r1, r2);
+                        const double gx = exp(lg0 - 0.5 * ((r0 / sg) * (r0 / sg) + (r1 / sg) * (r1 / sg) +
+                                                           (r2 / sg) * (r2 / sg)));
+                        const double ratio = I / (mI / max_gauss * gx);
+                        acc = pic_uniform(seed, pic, (uint32_t)i, (uint32_t)seg) < ratio;
+                    }
                 }
             }
+            // in-order compaction of the accepted photons of this chunk
+            const uint64_t m = __ballot(acc);
+            const int before = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) s_wave[wave] = __popcll(m);
+            __syncthreads();
+            int off = s_base;
+            for (int w = 0; w < wave; ++w) off += s_wave[w];
+            const int pos = off + before;
+            if (acc && pos < a.cap) {
+                double *o = ph + ((size_t)blockIdx.x * a.cap + pos) * 3;
+                o[0] = x;
+                o[1] = y;
+                o[2] = 0.0;
+            }
+            __syncthreads();
+            if (tid == 0)
+                for (int w = 0; w < kPicThreads / 64; ++w) s_base += s_wave[w];
+            __syncthreads();
         }
-        // in-order compaction of the accepted photons of this chunk
-        const uint64_t m = __ballot(acc);
-        const int before = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) s_wave[wave] = __popcll(m);
-        __syncthreads();
-        int off = s_base;
-        for (int w = 0; w < wave; ++w) off += s_wave[w];
-        const int pos = off + before;
-        if (acc && pos < cap) {
-            double *o = ph + ((size_t)blockIdx.x * cap + pos) * 3;
-            o[0] = x;
-            o[1] = y;
-            o[2] = 0.0;
-        }
-        __syncthreads();
-        if (tid == 0)
-            for (int w = 0; w < kPicThreads / 64; ++w) s_base += s_wave[w];
-        __syncthreads();
     }
     if (tid == 0) cnt[blockIdx.x] = s_base;
 }

@@ -141,6 +141,11 @@ class PictureParams(C.Structure):
         ("photons_per_image", C.c_int),
         ("capacity", C.c_int),
         ("seed", C.c_uint64),
+        ("particles", C.c_int),
+        ("noise_photons", C.c_int),
+        ("noise_sigma", C.c_double),
+        ("incident_photon_variance", C.c_int),
+        ("beamstop_only", C.c_int),
     ]
 
 

@@ -15,9 +15,11 @@ from . import HIP_init, SphericalHarmonicsVolume, check, lib
 from ._lib import PictureParams
 
 
-def _params(qcut, envelope_sigma, photons_per_image, lam, beamstop_width, capacity, seed):
+def _params(qcut, envelope_sigma, photons_per_image, lam, beamstop_width, capacity, seed, particles=1,
+            noise_photons=0, noise_sigma=1.0, incident_photon_variance=0, beamstop_only=False):
     return PictureParams(float(qcut), float(qcut / 3.0 if envelope_sigma is None else envelope_sigma), float(lam),
-                         float(beamstop_width), int(photons_per_image), int(capacity), int(seed))
+                         float(beamstop_width), int(photons_per_image), int(capacity), int(seed), int(particles),
+                         int(noise_photons), float(noise_sigma), int(incident_photon_variance), int(bool(beamstop_only)))
 
 
 def _coeff(volume):
@@ -27,12 +29,18 @@ def _coeff(volume):
 
 
 def pointsPerOrientation(volume, qcut, npictures, envelope_sigma=None, photons_per_image=1000, lam=1.0,
-                         beamstop_width=0.0, capacity=512, seed=0, first_picture=0, ctx=None):
+                         beamstop_width=0.0, capacity=512, seed=0, first_picture=0, particles=1, noise_photons=0,
+                         noise_sigma=1.0, incident_photon_variance=0, beamstop_only=False, ctx=None):
     """(pictures, rotations): pictures[i] = (n_i, 3) accepted photons of picture
-    first_picture + i (detector positions, z = 0), rotations (npictures, 3, 3)."""
+    first_picture + i (detector positions, z = 0), rotations (npictures, 3, 3)
+    of the first particle.  particles / noise_photons / noise_sigma /
+    incident_photon_variance / beamstop_only as generateHistogram's
+    number_particles, GaussianNoise(gamma > 0, sigma, photons),
+    incident_photon_variance and beamstop_only."""
     ctx = ctx or HIP_init()
     c = _coeff(volume)
-    p = _params(qcut, envelope_sigma, photons_per_image, lam, beamstop_width, capacity, seed)
+    p = _params(qcut, envelope_sigma, photons_per_image, lam, beamstop_width, capacity, seed, particles,
+                noise_photons, noise_sigma, incident_photon_variance, beamstop_only)
     ph = np.zeros((npictures, capacity, 3))
     cnt = np.zeros(npictures, np.int32)
     rot = np.zeros((npictures, 3, 3))
@@ -44,11 +52,13 @@ def pointsPerOrientation(volume, qcut, npictures, envelope_sigma=None, photons_p
 
 
 def generate_histograms(hist, volume, qcut, npictures, envelope_sigma=None, photons_per_image=1000, lam=1.0,
-                        beamstop_width=0.0, capacity=512, seed=0, first_picture=0):
+                        beamstop_width=0.0, capacity=512, seed=0, first_picture=0, particles=1, noise_photons=0,
+                        noise_sigma=1.0, incident_photon_variance=0, beamstop_only=False):
     """Add npictures synthetic pictures of `volume` to the PhotonHistogram
     `hist`; returns the number of accepted photons dropped beyond capacity."""
     c = _coeff(volume)
-    p = _params(qcut, envelope_sigma, photons_per_image, lam, beamstop_width, capacity, seed)
+    p = _params(qcut, envelope_sigma, photons_per_image, lam, beamstop_width, capacity, seed, particles,
+                noise_photons, noise_sigma, incident_photon_variance, beamstop_only)
     over = C.c_int64()
     check(lib().tp_hist_add_generated(hist.h, volume.LMAX, volume.KMAX, volume.rmax, c.ctypes.data, c.shape[1],
                                       C.byref(p), npictures, first_picture, C.byref(over)), "tp_hist_add_generated")
