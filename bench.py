@@ -178,6 +178,13 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
         picture_oracle.points_per_orientation(surf, cfg["LMAX"], start.KMAX, start.rmax, start.rmax, start.rmax / 3.0,
                                               1000, 0.0, 0.0, 7, p)
     t_pic = (time.perf_counter() - t0) / 8
+    # structure: the oracle's density sampling of crambin on 4 of the 38 shells (LMAX = 35), scaled to 38
+    from oracle import structure_oracle
+    from tp_amd import structure
+    sat = structure.atom_table(structure.loadPDB(STRUCTURE_PDB))
+    t0 = time.perf_counter()
+    structure_oracle.density_surface(sat, 35, 4, 4.0)
+    t_dens = (time.perf_counter() - t0) * 38 / 4
     tf = float(np.median(t_fact))
     return dict(value=dense_evals, unit="c3+energy evals/s", cores=nthreads, kind="port",
                 sample=f"dense reference formulation on {ncols} of {ob.klength} k-triple columns "
@@ -194,6 +201,8 @@ def cpu_baseline(cfg, start, c3ref, ncols, nthreads):
                 sraar_ms_per_iteration=1e3 * t_ph,
                 picture_pictures_per_s=1.0 / t_pic,
                 picture_sample="numpy oracle: pointsPerOrientation of 8 pictures, 1000 incident photons each",
+                structure_density_s=t_dens,
+                structure_sample="numpy oracle: crambin density sampling (670 atoms, LMAX=35) on 4 shells, x 38/4",
                 sraar_sample="numpy oracle: sRAAR with 3 and with 1 iterations, 38 shells, LMAX=25; the difference / 2",
                 small_config={"dense_evals_per_s": 1.0 / t_d0, "factored_evals_per_s": 1.0 / t_f0,
                               "energy": E0, "klength": ob0.klength, "R": 2 * sm["N"] ** 2})
@@ -325,6 +334,7 @@ def _time_to(Es, t_of, target):
     return None if idx.size == 0 else float(t_of(int(idx[0])))
 
 
+STRUCTURE_PDB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "crambin_atoms.pdb")
 HIST_CFG = dict(N=32, K2=38, K3=26, qcut=38.0, nphot=100, seed=5)   # beamstop-only pictures of run_generate_histograms.jl
 
 
@@ -458,6 +468,40 @@ def generate_leg(tp, truth, npic):
            "photons": n1, "photons_per_picture": n1 / npic, "triplets": n3, "overflow": over}
     log(f"generateHistogram: {npic} pictures in {wall:.3f} s ({npic / wall:.3g}/s; synthesis {gen_ms:.1f} ms, "
         f"histograms {hist_ms:.1f} ms), {n1 / npic:.1f} photons/picture, {n3} triplets")
+    return out
+
+
+def structure_leg(tp, reps):
+    """createSphericalHarmonicsStructure (structure.jl:124-134) of crambin
+    (the ATOM records the reference's structure test loads, 670 atoms) at
+    LMAX = 35, KMAX = 38, rmax = 38, as the histogram-generation runs make
+    their intensity: density sampling, SH analysis, spherical Bessel
+    transform, squared modulus and analysis again, all on the device (HIP
+    events), and the density-sampling kernel alone."""
+    from tp_amd import structure
+    ctx = tp.HIP_init()
+    LMAX, K, rmax = 35, 38, 38.0
+    atoms = structure.atom_table(structure.loadPDB(STRUCTURE_PDB))
+    structure.createSphericalHarmonicsStructure(atoms, LMAX, K, rmax)
+    ctx.set_timing(True)
+    dev, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        structure.createSphericalHarmonicsStructure(atoms, LMAX, K, rmax)
+        wall.append(time.perf_counter() - t0)
+        dev.append(ctx.last_timings()[0])
+    dens = []
+    for _ in range(reps):
+        structure.getDensitySurfaces(atoms, LMAX, K, rmax)
+        dens.append(ctx.last_timings()[0])
+    ctx.set_timing(False)
+    d_ms = float(np.median(dens))
+    pairs = len(atoms) * K * (2 * LMAX) ** 2
+    out = {"workload": f"crambin ({len(atoms)} atoms), LMAX={LMAX}, KMAX={K}, rmax={rmax}",
+           "call_ms": 1e3 * float(np.median(wall)), "device_ms": float(np.median(dev)),
+           "density_kernel_ms": d_ms, "atom_sample_pairs_per_s": pairs / (d_ms * 1e-3)}
+    log(f"createSphericalHarmonicsStructure: call {out['call_ms']:.2f} ms, device {out['device_ms']:.3f} ms "
+        f"(density sampling {d_ms:.3f} ms, {out['atom_sample_pairs_per_s']:.3g} atom-sample pairs/s)")
     return out
 
 
@@ -619,6 +663,7 @@ def main():
     ap.add_argument("--fit-num", type=int, default=20, help="grid divisions of the rotational-fit leg (0: skip)")
     ap.add_argument("--sraar-iters", type=int, default=1001, help="iterations of the phase-retrieval leg (0: skip)")
     ap.add_argument("--gen-pictures", type=int, default=65536, help="pictures of the histogram-generation leg (0: skip)")
+    ap.add_argument("--structure-reps", type=int, default=5, help="timed calls of the structure leg (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -713,6 +758,7 @@ def main():
     fitr = fit_leg(tp, truth, args.fit_num) if args.fit_num > 0 else None
     phase = phase_leg(tp, truth, args.sraar_iters) if args.sraar_iters > 0 else None
     gen = generate_leg(tp, truth, args.gen_pictures) if args.gen_pictures > 0 else None
+    struct = structure_leg(tp, args.structure_reps) if args.structure_reps > 0 else None
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:
@@ -780,6 +826,8 @@ def main():
         out["phase_retrieval"] = phase
     if gen is not None:
         out["generated_histograms"] = gen
+    if struct is not None:
+        out["structure"] = struct
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)

@@ -276,7 +276,7 @@ int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double
  * (transform, :127-160) with Driscoll-Healy weights, exact for band-limited
  * surfaces.  tp_sph_bessel = forward (forward != 0) / backward
  * (sphBesselTransform, :310-377): every (l, m) radial profile of K shells
- * through the reference's Hankel quadrature.  LMAX <= 32. */
+ * through the reference's Hankel quadrature.  LMAX <= 48. */
 int tp_sh_surface(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld, double *surf);
 int tp_sh_coeff(tp_ctx *ctx, int LMAX, int K, const double *surf, double *coeff, int64_t ld);
 int tp_sph_bessel(tp_ctx *ctx, int LMAX, int K, int forward, const double *coeff, int64_t ld, double *out);
@@ -423,6 +423,25 @@ int tp_batch_set_proposer(tp_batch *B, tp_mlp *m, int nlearn, double noise);
  * (structure.jl:74-134) for point atoms. */
 int tp_intensity_from_atoms(tp_ctx *ctx, const double *atoms_xyz, int natoms, int KMAX, double dq, int LMAX,
                             int lfill, int ntheta, int nphi, double *coeff_out);
+
+/* Electron density of Gaussian atoms, getDensity_cartesian (structure.jl:74-87),
+ * sampled as getDensity does (spherical_harmonics.jl:234-244): shells
+ * r_k = k rmax / K, k = 1..K, at the (2 LMAX)^2 points of i_to_ang (:203-210).
+ * atoms: natoms rows (x, y, z, R, B), R and B the atom type's form factor
+ * (the reference's atomfactorlist; atoms of type "1" left out by the caller);
+ * rho(p) = sum over atoms with |p - a|^2 < 5 of B / (R sqrt(2 pi))^3
+ * exp(-|p - a|^2 / (2 R^2)).  surf: [K][(2 LMAX)^2] complex (imaginary 0). */
+int tp_density_surface(tp_ctx *ctx, const double *atoms, int natoms, int LMAX, int K, double rmax, double *surf);
+/* absoluteSquare (spherical_harmonics.jl:393-397): per shell, the coefficients
+ * of |f|^2 with f the surface of the shell's coefficients.  LMAX <= 48. */
+int tp_absolute_square(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld, double *out);
+/* createSphericalHarmonicsStructure (structure.jl:124-134) after loadPDB, on
+ * the device: density = getDensity (sampled as tp_density_surface, then
+ * transformed), fourier = forward(density) (sphBesselTransform), intensity =
+ * absoluteSquare(fourier); each K shells of ld complex coefficients, any of
+ * them may be NULL.  LMAX <= 48, K >= 2. */
+int tp_structure_from_atoms(tp_ctx *ctx, const double *atoms, int natoms, int LMAX, int K, double rmax,
+                            double *density, double *fourier, double *intensity, int64_t ld);
 
 /* ---- instrumentation ------------------------------------------------------ */
 

@@ -4,6 +4,7 @@
 //   p l m theta           -> sphPlm
 //   s l m LMAX            -> seanindex
 //   g n                   -> n Gauss-Legendre nodes and weights
+//   j n x                 -> J_{l+1/2}(x), l = 0..n-1 (bessel_half_all)
 //   l n                   -> worst error (ulp of the exact value, long double
 //                            reference) of tp_fastlog over n arguments
 #include <cmath>
@@ -39,6 +40,13 @@ int main()
             int l, m, L;
             if (std::scanf("%d %d %d", &l, &m, &L) != 3) return 1;
             std::printf("%lld\n", (long long)tph::seanindex(l, m, L));
+        } else if (c == 'j') {
+            int n;
+            double x;
+            if (std::scanf("%d %lf", &n, &x) != 2 || n < 1) return 1;
+            std::vector<double> J((size_t)n);
+            tph::bessel_half_all(x, n, J.data());
+            for (int l = 0; l < n; ++l) std::printf("%.17g%c", J[(size_t)l], l + 1 < n ? ' ' : '\n');
         } else if (c == 'l') {
             int n;
             if (std::scanf("%d", &n) != 1) return 1;

@@ -67,3 +67,32 @@ def test_fastlog_ulp(driver):
     long double reference over 1e-30..1e30, around 1 and at table edges."""
     worst = float(driver(["l 400000"])[0])
     assert worst < 4.0, worst
+
+
+def _bessel(driver, n, xs):
+    return [np.array([float(v) for v in ln.split()]) for ln in driver([f"j {n} {x:.17g}" for x in xs])[:len(xs)]]
+
+
+def test_half_integer_bessel_closed_forms(driver):
+    """J_{1/2} = sqrt(2/(pi x)) sin x, J_{3/2} = sqrt(2/(pi x)) (sin x / x - cos x),
+    J_{5/2} = sqrt(2/(pi x)) ((3/x^2 - 1) sin x - 3 cos x / x), at the Hankel
+    grid's arguments pi k r / K (including the zeros of sin near k r = K n)."""
+    xs = sorted({np.pi / K * k * r for K in (5, 26, 38) for k in range(1, K + 1) for r in range(1, K + 1, 3)})
+    for x, J in zip(xs, _bessel(driver, 3, xs)):
+        f = np.sqrt(2 / (np.pi * x))
+        s, c = np.sin(x), np.cos(x)
+        want = [f * s, f * (s / x - c), f * ((3 / x ** 2 - 1) * s - 3 * c / x)]
+        assert np.allclose(J, want, rtol=0, atol=1e-15 * max(1.0, 3 / x ** 2)), x
+
+
+def test_half_integer_bessel_high_orders(driver):
+    """Orders up to 47.5 against mpmath at 30 digits: within 1e-16 absolute
+    and 1e-14 relative wherever |J| > 1e-6."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 30
+    rng = np.random.default_rng(5)
+    xs = list(np.pi / 38 * rng.integers(1, 39, 40) * rng.integers(1, 39, 40)) + [0.05, 1e-3, 150.0]
+    for x, J in zip(xs, _bessel(driver, 48, xs)):
+        for l in range(0, 48, 3):
+            t = float(mpmath.besselj(l + 0.5, mpmath.mpf(float(x))))
+            assert abs(J[l] - t) <= 1e-16 + (1e-14 * abs(t) if abs(t) > 1e-6 else 0.0), (l, x, J[l], t)

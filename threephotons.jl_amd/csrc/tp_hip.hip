@@ -25,6 +25,7 @@
 #include "tp_retrieve.hpp"
 #include "tp_fit.hpp"
 #include "tp_phase.hpp"
+#include "tp_structure.hpp"
 #include "tp_pictures.hpp"
 
 // ---------------------------------------------------------------------------
@@ -2222,7 +2223,7 @@ static std::vector<double> ph_frdr(const std::vector<double> &x)
 
 static int ph_tables(tp_ctx *ctx, int LMAX, int K, bool bessel, PhTables &t)
 {
-    if (LMAX < 1 || LMAX > kPhMaxLMAX) return tp_fail(TP_E_UNSUPPORTED, "need 1 <= LMAX <= 32");
+    if (LMAX < 1 || LMAX > kPhMaxLMAX) return tp_fail(TP_E_UNSUPPORTED, "need 1 <= LMAX <= 48");
     if (K < 1 || (bessel && K < 2)) return tp_fail(TP_E_ARG, "need K >= 1 (>= 2 for the Bessel transform)");
     t.LMAX = LMAX;
     t.K = K;
@@ -2256,14 +2257,16 @@ static int ph_tables(tp_ctx *ctx, int LMAX, int K, bool bessel, PhTables &t)
             xq[i] = tph::kPi / K * (i + 1.0);
         }
         const std::vector<double> wr = ph_frdr(xr), wq = ph_frdr(xq);
-        std::vector<double> Hf((size_t)LMAX * K * K), Hb((size_t)LMAX * K * K);
-        for (int l = 0; l < LMAX; ++l)
-            for (int k = 0; k < K; ++k)
-                for (int r = 0; r < K; ++r) {
-                    const double I = std::cyl_bessel_j(l + 0.5, xq[k] * xr[r]);
+        std::vector<double> Hf((size_t)LMAX * K * K), Hb((size_t)LMAX * K * K), J((size_t)LMAX);
+        for (int k = 0; k < K; ++k)
+            for (int r = 0; r < K; ++r) {
+                tph::bessel_half_all(xq[k] * xr[r], LMAX, J.data());   // every order at this argument
+                for (int l = 0; l < LMAX; ++l) {
+                    const double I = J[(size_t)l];
                     Hf[((size_t)l * K + k) * K + r] = std::sqrt(1.0 / (k + 1)) * I * wr[r];
                     Hb[((size_t)l * K + r) * K + k] = std::sqrt(1.0 / (r + 1)) * I * wq[k] / (4.0 * tph::kPi * tph::kPi);
                 }
+            }
         if ((rc = dev_alloc(&t.d_Hf, Hf.size())) || (rc = dev_alloc(&t.d_Hb, Hb.size()))) return rc;
         TP_HIP(hipMemcpyAsync(t.d_Hf, Hf.data(), sizeof(double) * Hf.size(), hipMemcpyHostToDevice, ctx->stream));
         TP_HIP(hipMemcpyAsync(t.d_Hb, Hb.data(), sizeof(double) * Hb.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -2390,6 +2393,129 @@ int tp_sph_bessel(tp_ctx *ctx, int LMAX, int K, int forward, const double *coeff
     TP_HIP(hipMemcpyAsync(r.data(), b.p, sizeof(double2) * r.size(), hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
     ph_unpack(r, K, LMAX, out, ld);
+    return 0;
+}
+
+// Atoms (x, y, z, R, B) -> the density kernel's rows; R, B > 0.
+static int str_atoms(const double *atoms, int natoms, std::vector<StrAtom> &out)
+{
+    out.resize((size_t)natoms);
+    for (int a = 0; a < natoms; ++a) {
+        const double *r = atoms + (size_t)5 * a;
+        if (!(r[3] > 0.0) || !(r[4] > 0.0)) return tp_fail(TP_E_ARG, "atom form factors need R > 0 and B > 0");
+        const double R = r[3], s = R * std::sqrt(2.0 * tph::kPi);
+        out[(size_t)a] = StrAtom{r[0], r[1], r[2], r[4] / (s * s * s), 2.0 * R * R};
+    }
+    return 0;
+}
+
+static void str_density(tp_ctx *ctx, const StrAtom *d_atoms, int natoms, int LMAX, int K, double rmax, double2 *surf)
+{
+    const int64_t n = (int64_t)K * 4 * LMAX * LMAX;
+    hipLaunchKernelGGL(k_density_surface, dim3((unsigned)((n + kStrThreads - 1) / kStrThreads)), dim3(kStrThreads), 0,
+                       ctx->stream, d_atoms, natoms, LMAX, K, rmax / K, surf);
+}
+
+int tp_density_surface(tp_ctx *ctx, const double *atoms, int natoms, int LMAX, int K, double rmax, double *surf)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if ((!atoms && natoms > 0) || natoms < 0 || !surf || LMAX < 1 || K < 1 || !(rmax > 0.0))
+        return tp_fail(TP_E_ARG, "bad argument");
+    std::vector<StrAtom> h;
+    if (int rc = str_atoms(atoms, natoms, h)) return rc;
+    const size_t n = (size_t)K * 4 * LMAX * LMAX;
+    DevBuf<StrAtom> da;
+    DevBuf<double2> f;
+    if (natoms > 0)
+        if (int rc = dev_alloc(&da.p, h.size())) return rc;
+    if (int rc = dev_alloc(&f.p, n)) return rc;
+    if (natoms > 0)
+        TP_HIP(hipMemcpyAsync(da.p, h.data(), sizeof(StrAtom) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    str_density(ctx, da.p, natoms, LMAX, K, rmax, f.p);
+    TP_HIP(hipGetLastError());
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    TP_HIP(hipMemcpyAsync(surf, f.p, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->timing) {
+        float tm = 0;
+        if (hipEventElapsedTime(&tm, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->last_ms[0] = tm;
+        for (int i = 1; i < 4; ++i) ctx->last_ms[i] = 0.0;
+    }
+    return 0;
+}
+
+int tp_absolute_square(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld, double *out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!coeff || !out || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
+    PhTables t;
+    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const size_t nq = (size_t)K * LMAX * LMAX, n = (size_t)K * 4 * LMAX * LMAX;
+    DevBuf<double2> c, f;
+    if (int rc = dev_alloc(&c.p, nq)) return rc;
+    if (int rc = dev_alloc(&f.p, n)) return rc;
+    const std::vector<double2> h = ph_pack(coeff, ld, K, LMAX);
+    TP_HIP(hipMemcpyAsync(c.p, h.data(), sizeof(double2) * nq, hipMemcpyHostToDevice, ctx->stream));
+    ph_synth(ctx, t, c.p, f.p);
+    hipLaunchKernelGGL(k_abs_square, dim3((unsigned)((n + kStrThreads - 1) / kStrThreads)), dim3(kStrThreads), 0,
+                       ctx->stream, f.p, (int64_t)n);
+    ph_analysis(ctx, t, f.p, c.p);
+    TP_HIP(hipGetLastError());
+    std::vector<double2> r(nq);
+    TP_HIP(hipMemcpyAsync(r.data(), c.p, sizeof(double2) * nq, hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    ph_unpack(r, K, LMAX, out, ld);
+    return 0;
+}
+
+int tp_structure_from_atoms(tp_ctx *ctx, const double *atoms, int natoms, int LMAX, int K, double rmax,
+                            double *density, double *fourier, double *intensity, int64_t ld)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if ((!atoms && natoms > 0) || natoms < 0 || !(rmax > 0.0) || ld < (int64_t)LMAX * LMAX)
+        return tp_fail(TP_E_ARG, "bad argument");
+    std::vector<StrAtom> h;
+    if (int rc = str_atoms(atoms, natoms, h)) return rc;
+    PhTables t;
+    if (int rc = ph_tables(ctx, LMAX, K, true, t)) return rc;
+    const size_t nq = (size_t)K * LMAX * LMAX, n = (size_t)K * 4 * LMAX * LMAX;
+    DevBuf<StrAtom> da;
+    DevBuf<double2> f, cd, cf, ci;
+    if (natoms > 0)
+        if (int rc = dev_alloc(&da.p, h.size())) return rc;
+    if (int rc = dev_alloc(&f.p, n)) return rc;
+    if (int rc = dev_alloc(&cd.p, nq)) return rc;
+    if (int rc = dev_alloc(&cf.p, nq)) return rc;
+    if (int rc = dev_alloc(&ci.p, nq)) return rc;
+    hipStream_t st = ctx->stream;
+    if (natoms > 0) TP_HIP(hipMemcpyAsync(da.p, h.data(), sizeof(StrAtom) * h.size(), hipMemcpyHostToDevice, st));
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
+    str_density(ctx, da.p, natoms, LMAX, K, rmax, f.p);   // getDensity: sample
+    ph_analysis(ctx, t, f.p, cd.p);                       //             and transform
+    ph_bessel(ctx, t, cd.p, 1, cf.p);                     // forward
+    ph_synth(ctx, t, cf.p, f.p);                          // absoluteSquare
+    hipLaunchKernelGGL(k_abs_square, dim3((unsigned)((n + kStrThreads - 1) / kStrThreads)), dim3(kStrThreads), 0, st,
+                       f.p, (int64_t)n);
+    ph_analysis(ctx, t, f.p, ci.p);
+    TP_HIP(hipGetLastError());
+    if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+    double *outs[3] = {density, fourier, intensity};
+    const double2 *devs[3] = {cd.p, cf.p, ci.p};
+    std::vector<double2> r[3];
+    for (int o = 0; o < 3; ++o)
+        if (outs[o]) {
+            r[o].resize(nq);
+            TP_HIP(hipMemcpyAsync(r[o].data(), devs[o], sizeof(double2) * nq, hipMemcpyDeviceToHost, st));
+        }
+    TP_HIP(hipStreamSynchronize(st));
+    for (int o = 0; o < 3; ++o)
+        if (outs[o]) ph_unpack(r[o], K, LMAX, outs[o], ld);
+    if (ctx->timing) {
+        float tm = 0;
+        if (hipEventElapsedTime(&tm, ctx->ev[0], ctx->ev[1]) == hipSuccess) ctx->last_ms[0] = tm;
+        for (int i = 1; i < 4; ++i) ctx->last_ms[i] = 0.0;
+    }
     return 0;
 }
 
@@ -2523,7 +2649,7 @@ static int pic_check(const tp_picture_params *p, int LMAX, int K, double rmax, i
     if (!p) return tp_fail(TP_E_ARG, "null parameters");
     if (p->particles < 1 || p->noise_photons < 0 || (p->noise_photons > 0 && !(p->noise_sigma > 0.0)))
         return tp_fail(TP_E_ARG, "need particles >= 1 and, with noise photons, noise_sigma > 0");
-    if (LMAX < 1 || LMAX > kPhMaxLMAX || K < 1 || !(rmax > 0.0)) return tp_fail(TP_E_ARG, "need 1 <= LMAX <= 32, K >= 1, rmax > 0");
+    if (LMAX < 1 || LMAX > kPhMaxLMAX || K < 1 || !(rmax > 0.0)) return tp_fail(TP_E_ARG, "need 1 <= LMAX <= 48, K >= 1, rmax > 0");
     if (npictures < 0 || p->photons_per_image < 0 || p->capacity < 1 || p->incident_photon_variance < 0)
         return tp_fail(TP_E_ARG, "bad picture counts");
     // photon indices 3 i + 1 of a segment stay below 2^31 (a count normal beyond 20 deviations never comes)

@@ -129,4 +129,31 @@ inline void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w
     }
 }
 
+// Bessel functions of half-integer order, J[l] = J_{l+1/2}(x) for l = 0..n-1,
+// x > 0: the spherical j_l by Miller's downward recurrence
+// j_{l-1} = (2l+1)/x j_l - j_{l+1} (stable downwards for every order), started
+// far above max(n, x) and normalised by the closed form of j_0 = sin x / x or
+// j_1 = sin x / x^2 - cos x / x, whichever is larger; J_{l+1/2} = sqrt(2x/pi) j_l.
+// Long double throughout: the unnormalised values stay far inside its range.
+inline void bessel_half_all(double x, int n, double *J)
+{
+    const double mx = n > x ? (double)n : x;
+    const int top = n + 40 + (int)(mx + std::sqrt(40.0 * mx));
+    const int ns = n > 2 ? n : 2;
+    std::vector<long double> j((size_t)ns, 0.0L);
+    const long double xl = x;
+    long double jp = 0.0L, jc = 1.0L;   // j_{l+1}, j_l (unnormalised), l = top
+    for (int l = top; l >= 1; --l) {
+        const long double jm = (2.0L * l + 1.0L) / xl * jc - jp;
+        jp = jc;
+        jc = jm;   // j_{l-1}
+        if (l - 1 < ns) j[(size_t)(l - 1)] = jc;
+    }
+    const long double sx = sinl(xl), cx = cosl(xl);
+    const long double j0 = sx / xl, j1 = sx / (xl * xl) - cx / xl;
+    const long double sc = fabsl(j0) >= fabsl(j1) ? j0 / j[0] : j1 / j[1];
+    const long double f = sqrtl(2.0L * xl / (long double)kPi);
+    for (int l = 0; l < n; ++l) J[l] = (double)(f * sc * j[(size_t)l]);
+}
+
 }  // namespace tph

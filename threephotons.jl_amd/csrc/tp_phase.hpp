@@ -18,7 +18,7 @@
 #pragma once
 
 constexpr int kPhThreads = 256;
-constexpr int kPhMaxLMAX = 32;
+constexpr int kPhMaxLMAX = 48;   // k_sh_synth then holds <= 46 KB of LDS
 constexpr int kPhRings = 5;   // rings theta_j per workgroup of the transforms
 
 // LDS bytes of k_sh_synth: the shell's coefficients, g of its rings, twiddles.

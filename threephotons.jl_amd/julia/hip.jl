@@ -12,10 +12,12 @@
 #   retrieveSolution_hip(c2, L, LMAX, K2_range, qmax)                             (correlations.jl:262-319)
 #   fitStructures_full_hip(volume, reference, num, K_range, L, basis)              (data_processing.jl:26-46)
 #   sRAAR_hip(intensity, iterations, beta0, beta_max, tau; cutoff_factor)          (phases.jl:31-95)
+#   createSphericalHarmonicsStructure_hip(pdb, LMAX, KMAX, rmax)                    (structure.jl:124-134)
 #
 # Written against Julia 0.6 syntax like the rest of the package.
 
-export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip, fitStructures_full_hip, sRAAR_hip
+export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip, fitStructures_full_hip, sRAAR_hip,
+       createSphericalHarmonicsStructure_hip
 
 const libtp = get(ENV, "THREEPHOTONS_LIBTP", joinpath(dirname(@__FILE__), "..", "libtp.so"))
 HIP_enabled = false
@@ -297,4 +299,32 @@ function rotate_all_at_once_batched(intensities::Vector{SphericalHarmonicsVolume
     finally
         ccall((:tp_batch_destroy, libtp), Void, (Ptr{Void},), batch[])
     end
+end
+
+"""createSphericalHarmonicsStructure (structure.jl:124-134) with getDensity,
+forward and absoluteSquare on the device: (density, fourier, intensity)."""
+function createSphericalHarmonicsStructure_hip(moleculepdb, LMAX::Int64, KMAX::Int64, rmax::Float64)
+    rows = filter(atom -> atom.t != "1", loadPDB(moleculepdb))
+    atoms = zeros(Float64, 5, length(rows))
+    for (i, atom) in enumerate(rows)
+        ff = atomfactorlist[atom.t]
+        atoms[:, i] = [atom.pos[1], atom.pos[2], atom.pos[3], ff.R, ff.B]
+    end
+    n = num_coeff(LMAX)
+    d = zeros(Complex{Float64}, n, KMAX)
+    f = zeros(Complex{Float64}, n, KMAX)
+    I = zeros(Complex{Float64}, n, KMAX)
+    hip_check(ccall((:tp_structure_from_atoms, libtp), Cint,
+                    (Ptr{Void}, Ptr{Float64}, Cint, Cint, Cint, Cdouble, Ptr{Complex{Float64}},
+                     Ptr{Complex{Float64}}, Ptr{Complex{Float64}}, Int64),
+                    hip_ctx, atoms, size(atoms, 2), LMAX, KMAX, rmax, d, f, I, n), "tp_structure_from_atoms")
+    density = SphericalHarmonicsVolume(LMAX, KMAX, rmax)
+    fourier = SphericalHarmonicsVolume(LMAX, KMAX, qmax(KMAX, rmax))
+    intensity = SphericalHarmonicsVolume(LMAX, KMAX, qmax(KMAX, rmax))
+    for k = 1:KMAX
+        density.coeff[k][:] = d[:, k]
+        fourier.coeff[k][:] = f[:, k]
+        intensity.coeff[k][:] = I[:, k]
+    end
+    return density, fourier, intensity
 end

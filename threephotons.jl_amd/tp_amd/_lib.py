@@ -91,6 +91,9 @@ def lib():
     L.tp_sh_surface.argtypes = [vp, i32, i32, vp, i64, vp]
     L.tp_sh_coeff.argtypes = [vp, i32, i32, vp, vp, i64]
     L.tp_sph_bessel.argtypes = [vp, i32, i32, i32, vp, i64, vp]
+    L.tp_density_surface.argtypes = [vp, vp, i32, i32, i32, dbl, vp]
+    L.tp_absolute_square.argtypes = [vp, i32, i32, vp, i64, vp]
+    L.tp_structure_from_atoms.argtypes = [vp, vp, i32, i32, i32, dbl, vp, vp, vp, i64]
     L.tp_sraar.argtypes = [vp, i32, i32, vp, i64, i32, dbl, dbl, dbl, dbl, vp, C.c_uint64, vp, vp]
     L.tp_pictures.argtypes = [vp, i32, i32, dbl, vp, i64, vp, i32, i32, vp, vp, vp]
     L.tp_hist_add_generated.argtypes = [vp, i32, i32, dbl, vp, i64, vp, i32, i32, vp]
@@ -176,6 +179,7 @@ EXPORTS = [
     "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy", "tp_two_photons", "tp_retrieve_solution",
     "tp_fit_create", "tp_fit_set_volume", "tp_fit_similarity", "tp_fit_destroy",
     "tp_sh_surface", "tp_sh_coeff", "tp_sph_bessel", "tp_sraar", "tp_pictures", "tp_hist_add_generated",
+    "tp_density_surface", "tp_absolute_square", "tp_structure_from_atoms",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
