@@ -62,7 +62,12 @@ void tp_finalize(tp_ctx *ctx);
 /* Precompute the basis in FP64 on the host and upload it.
  * Replaces calculate_basis + CUDA_store_basis / CUDA_calculate_basis
  * (correlations.jl:87-153, cuda.jl:45-65).  for_intensity=1 selects even l
- * (lrange = 0:2:L).  lambda != 0 returns TP_E_UNSUPPORTED in this version. */
+ * (lrange = 0:2:L).  lambda dq != 0 is the curved Ewald sphere: the
+ * Legendre factors are taken at theta_k = acos(k lambda dq / (4 pi))
+ * (correlations.jl:94), which needs |K lambda dq| <= 4 pi; such a basis
+ * evaluates in FP64 only (tp_c3, tp_energy, tp_state_*, tp_anneal,
+ * tp_energy_batch), and mixed precision, batched runs, the energy gradient
+ * and tp_two_photons return TP_E_UNSUPPORTED for it. */
 int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq,
                     int for_intensity, tp_basis **out);
 /* The same from a basis the caller already holds: wignerlist[basislen] and

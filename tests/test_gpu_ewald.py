@@ -1,0 +1,92 @@
+"""The curved Ewald sphere (lambda dq != 0) on the device against the oracle's
+dense B.PA form, whose Legendre factors at theta_k = acos(k lambda dq / 4 pi)
+are pinned by the exact orientational average in test_oracle.py.  Includes the
+reference's own GPU test (tests/test_cuda.jl: the crambin intensity of
+test_structure.jl, K = 15, L = 12, N = 16, LMAX = 25, lambda = 2.5, dq of the
+intensity), whose bar is rel-L1 < 1e-5; the bar here is the FP64 one, 1e-10."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import c3ref_from_columns, random_real_coeff, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+CRAMBIN = os.path.join(os.path.dirname(__file__), "golden", "crambin_atoms.pdb")
+
+
+@pytest.fixture(scope="module")
+def tp():
+    import tp_amd
+    tp_amd.HIP_init()
+    return tp_amd
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+CASES = [
+    # (L, LMAX, N, K, lambda, dq): the separated-grid contraction with odd m (N <= 32, small L),
+    # two k1 halves of the triple products (K = 20), the GEMM contraction (N > 32)
+    (4, 6, 4, 3, 2.5, 1.0),
+    (6, 9, 5, 5, 1.0, 0.8),
+    (4, 6, 32, 4, 2.0, 1.0),
+    (6, 8, 4, 20, 0.5, 0.3),
+    (4, 6, 40, 3, 2.0, 1.0),
+]
+
+
+@pytest.mark.parametrize("L,LMAX,N,K,lam,dq", CASES)
+def test_c3_and_energy_match_oracle(tp, oracle, L, LMAX, N, K, lam, dq):
+    say("case", L, LMAX, N, K, lam, dq)
+    c = random_real_coeff(K, L, LMAX, seed=L * 10 + K, c00=4.0)
+    b = tp.HIP_calculate_basis(L, LMAX, N, K, lam, dq)
+    ob = oracle.Basis(L, LMAX, N, K, lam=lam, dq=dq)
+    vol = tp.SphericalHarmonicsVolume(c, LMAX, K, 1.0)
+    got = tp.FullCorrelation_parallized(vol, b, return_raw=True)
+    want = ob.c3_dense(c, nthreads=8)
+    err = rel_err(got, want)
+    say(f"  c3 {err:.2e}")
+    assert err < TOL
+    p = np.maximum(want, 1e-30)
+    rng = np.random.default_rng(2)
+    ref = c3ref_from_columns(rng.poisson(1e4 * p / p.sum() * p.shape[1]).astype(float), N, K)
+    ref = np.maximum(ref, 1e-30)
+    ref /= ref.sum()
+    E, Eo = tp.energy(vol, b, ref), oracle.energy(want, N, ref, K, 1, K)
+    say(f"  energy {abs(E - Eo) / abs(Eo):.2e}")
+    assert abs(E - Eo) / abs(Eo) < TOL
+
+
+def test_reference_gpu_test_on_crambin(tp, oracle):
+    """tests/test_cuda.jl: calculate_basis(L, LMAX, N, K, 2.5, dq(intensity)) and
+    FullCorrelation_parallized of the crambin intensity, GPU against CPU."""
+    from tp_amd import structure
+    L, LMAX, N, K, lam = 12, 25, 16, 15, 2.5
+    _, _, intensity = structure.createSphericalHarmonicsStructure(CRAMBIN, LMAX, 38, 38.0)
+    dq = intensity.dq()
+    assert dq == pytest.approx(np.pi / 38)
+    b = tp.HIP_calculate_basis(L, LMAX, N, K, lam, dq)
+    ob = oracle.Basis(L, LMAX, N, K, lam=lam, dq=dq)
+    got = tp.FullCorrelation_parallized(intensity, b, return_raw=True)
+    want = ob.c3_dense(intensity.coeff[:K], nthreads=8)
+    l1 = np.abs(got - want).sum() / np.abs(want).sum()
+    say(f"  rel-L1 {l1:.2e}, max {rel_err(got, want):.2e}")
+    assert l1 < 1e-5 and rel_err(got, want) < TOL
+    # the energy against c3ref = ones(N, 2N, K, K, K), the reference's commented-out check
+    ref = np.ones((N, 2 * N, K, K, K))
+    E, Eo = tp.energy(intensity, b, ref), oracle.energy(want, N, ref, K, 1, K)
+    assert abs(E - Eo) / abs(Eo) < TOL
+
+
+def test_curved_basis_limits(tp):
+    b = tp.HIP_calculate_basis(4, 6, 4, 3, 2.5, 1.0)
+    for mode in ("fp16x3", "bf16", "triple16"):
+        with pytest.raises(RuntimeError, match="lambda"):
+            b.set_precision(mode)
+    with pytest.raises(RuntimeError, match="acos"):
+        tp.HIP_calculate_basis(4, 6, 4, 3, 10.0, 1.0)   # 3 * 10 / (4 pi) > 1

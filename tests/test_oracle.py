@@ -138,6 +138,19 @@ def test_c3_three_formulations_agree(oracle):
     assert rel_err(s, d) < 1e-13
 
 
+def test_c3_curved_ewald_dense_equals_slice(oracle):
+    """lambda != 0: the dense B.PA form (Legendre factors at theta_k =
+    acos(k lambda dq / 4 pi), odd m included) equals the slow flab form."""
+    c, L, LMAX, N, K = _tiny(seed=4)
+    lam, dq = 2.5, 1.0
+    b = oracle.Basis(L, LMAX, N, K, lam=lam, dq=dq)
+    d = b.c3_dense(c)
+    s = np.stack([oracle.c3_slice(c, N, L, LMAX, *kt, dq=dq, lam=lam) for kt in ktriples(K)], axis=1)
+    assert rel_err(s, d) < 1e-13
+    flat = oracle.Basis(L, LMAX, N, K).c3_dense(c)
+    assert rel_err(d, flat) > 1e-3   # the curvature changes c3
+
+
 def test_c3_small_config_dense_vs_factored(oracle):
     c = random_real_coeff(8, 8, 25, 3, decay=0.2)
     b = oracle.Basis(8, 25, 16, 8)
@@ -150,12 +163,15 @@ def _Y(oracle, l, m, th, ph):
     return oracle.sphPlm(l, m, th) * np.exp(1j * m * ph)
 
 
-def test_c3_equals_orientational_average(oracle):
-    """c3(a,b) = (1/8pi^2) int_SO(3) I(R n1) I(R n2) I(R n3) dR, n_i in the
-    theta=pi/2 plane at azimuths 0, alpha, beta; exact quadrature over Euler
-    angles for a band-limited intensity."""
+@pytest.mark.parametrize("lam,dq", [(0.0, 0.0), (2.5, 1.0)])
+def test_c3_equals_orientational_average(oracle, lam, dq):
+    """c3(a,b) = (1/8pi^2) int_SO(3) I(R n1) I(R n2) I(R n3) dR, n_i at polar
+    angles theta_ki = acos(k_i lambda dq / 4 pi) (the Ewald sphere of
+    detector_to_Ewald_sphere, datagen.jl:33-39; pi/2 for lambda = 0) and
+    azimuths 0, alpha, beta; exact quadrature over Euler angles for a
+    band-limited intensity."""
     c, L, LMAX, N, K = _tiny(seed=5)
-    b = oracle.Basis(L, LMAX, N, K)
+    b = oracle.Basis(L, LMAX, N, K, lam=lam, dq=dq)
     d = b.c3_dense(c)
 
     def I(k, n):
@@ -177,10 +193,10 @@ def test_c3_equals_orientational_average(oracle):
     na = 14
     kts = ktriples(K)
     for kt, ia, jb in [((3, 2, 2), 1, 5), ((2, 2, 1), 0, 7), ((3, 3, 3), 3, 2)]:
-        n1 = np.array([1.0, 0, 0])
         a, bb = (ia + 0.5) * math.pi / N, (jb + 0.5) * math.pi / N
-        n2 = np.array([math.cos(a), math.sin(a), 0])
-        n3 = np.array([math.cos(bb), math.sin(bb), 0])
+        th = [math.acos(k * lam * dq / (4 * math.pi)) for k in kt]
+        n1, n2, n3 = (np.array([math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)])
+                      for t, p in zip(th, (0.0, a, bb)))
         acc = 0.0
         for xb, wb in zip(gx, gw):
             for ai in range(na):

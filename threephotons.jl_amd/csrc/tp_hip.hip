@@ -26,6 +26,7 @@
 #include "tp_fit.hpp"
 #include "tp_phase.hpp"
 #include "tp_structure.hpp"
+#include "tp_ewald.hpp"
 #include "tp_pictures.hpp"
 
 // ---------------------------------------------------------------------------
@@ -102,7 +103,14 @@ struct tp_basis {
     int n_c = 0;                 // compact coefficients per shell (l in lrange, m = -l..l)
     std::vector<int> coff;       // compact offset of order l (indexed by l)
     std::vector<int> sean0;      // [n_c] seanindex-1 of each compact entry
-    int n_m = 0;                 // folded even (m2,mu3) classes
+    int n_m = 0;                 // folded (m2,mu3) classes (even m only when lambda dq = 0)
+    // curved Ewald sphere (lambda dq != 0): G formed by k_glam
+    bool lam = false;
+    int lam_yrow = 0;
+    double *d_lamY = nullptr;      // [(L+1)(2L+1)][K] sphPlm(l, m, theta_k)
+    int *d_ktk = nullptr;          // [klength] k1 | k2 << 10 | k3 << 20
+    LamEnt *d_lament = nullptr;    // entries of every class, class-major
+    int *d_lamcs = nullptr;        // [n_m + 1] first entry of each class
     int64_t R_pad = 0, nlt_pad = 0, kl_pad = 0, nm_pad64 = 0, nm_pad16 = 0;
     int n_rt = 0, n_ct = 0, g1_rt = 0, g1_ct = 0, n_fb = 0;
     int n_groups = 0, nchunks = 0, Gmax = 0, Emax = 0, npairs = 0, PB = 0;
@@ -288,12 +296,20 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         TP_HIP(hipGetLastError());
         n_rt_fin = b->n_rt16;
     } else {
+    if (b->lam) {
+        // curved Ewald sphere: the class weights depend on the k-triple
+        hipLaunchKernelGGL(k_glam, dim3((unsigned)(b->kl_pad / kLamThreads + (b->kl_pad % kLamThreads != 0)),
+                                        (unsigned)((b->n_m + kLamClasses - 1) / kLamClasses)),
+                           dim3(kLamThreads), 0, st, b->d_As, b->kl_pad, b->d_ktk, (int)b->klength, b->d_lamY,
+                           b->d_lament, b->d_lamcs, b->n_m, b->d_G, b->kl_pad);
+    } else {
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
     // XCD-grouped column tiles: each As column block is fetched from HBM once
     hipLaunchKernelGGL((k_gemm<GF_BM, GF_BN, GF_WGM, 2, false, true>),
                        dim3((unsigned)(8 * ((b->g1_ct + 7) / 8) * b->g1_rt)), dim3(256), 0,
                        st, b->d_H, b->nm_pad64, b->d_As, b->kl_pad, (int)b->nlt_pad, (int)(b->g1_rt * GF_BM),
                        (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
+    }
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
     if (!b->use_sep) {
@@ -1114,7 +1130,8 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
-                    b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2};
+                    b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2,
+                    b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1169,8 +1186,9 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     if (int rc = check_ctx(ctx)) return rc;
     if (L < 0 || L > 22 || LMAX < L + 1 || N < 1 || N > 256 || K < 1 || K > 255)
         return tp_fail(TP_E_ARG, "basis parameters out of range (0<=L<=22, LMAX>L, 1<=N<=256, 1<=K<=255)");
-    if (lambda != 0.0)
-        return tp_fail(TP_E_UNSUPPORTED, "lambda != 0 (curved Ewald sphere) is not implemented in this version");
+    // shell angles theta_k = acos(k lambda dq / (4 pi)) (correlations.jl:94) must exist for k <= K
+    if (!(std::fabs(K * lambda * dq / (4 * tph::kPi)) <= 1.0))
+        return tp_fail(TP_E_ARG, "shell angles acos(k lambda dq / (4 pi)) undefined: need |K lambda dq| <= 4 pi");
     if (!for_intensity)
         return tp_fail(TP_E_UNSUPPORTED, "for_intensity = false (odd l) is not implemented in this version");
     tp_basis *b = new tp_basis();
@@ -1178,6 +1196,9 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     b->L = L; b->LMAX = LMAX; b->N = N; b->K = K; b->lambda = lambda; b->dq = dq;
     b->for_intensity = 1;
     b->lstep = 2;
+    b->lam = lambda * dq != 0.0;
+    // classes of (m2, mu3): even m only on the flat sphere, where sphPlm(l, m, pi/2) = 0 for odd l + m
+    const int mstep = b->lam ? 1 : 2;
     b->R = 2LL * N * N;
     b->klength = (int64_t)K * (K + 1) * (K + 2) / 6;
     // compact layout
@@ -1189,13 +1210,13 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     for (int l = 0; l <= L; l += b->lstep)
         for (int m = -l; m <= l; ++m) b->sean0[b->coff[l] + m + l] = (int)(tph::seanindex(l, m, LMAX) - 1);
 
-    // folded even (m2, mu3) classes: cos(m2 a + mu3 b) is the same for
+    // folded (m2, mu3) classes: cos(m2 a + mu3 b) is the same for
     // (m2,mu3) and (-m2,-mu3); representative has m2 > 0, or m2 == 0 and mu3 >= 0
     const int W = 2 * L + 1;
     std::vector<int> cls((size_t)W * W, -1);
     std::vector<int> cls_m2, cls_mu3;
-    for (int m2 = 0; m2 <= L; m2 += 2)
-        for (int mu3 = -L; mu3 <= L; mu3 += 2) {
+    for (int m2 = 0; m2 <= L; m2 += mstep)
+        for (int mu3 = -L; mu3 <= L; mu3 += mstep) {
             if (m2 == 0 && mu3 < 0) continue;
             if (std::abs(m2 + mu3) > L) continue;
             const int c = (int)cls_m2.size();
@@ -1243,7 +1264,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
                         }
                         groups[mu3 + l3].push_back({w, b->coff[l1] + m1 + l1, b->coff[l2] + m2 + l2});
                         // P at theta = pi/2 vanishes for odd m (Y(l,m,pi/2) = 0 when l+m is odd)
-                        if (!(m1 & 1) && !(m2 & 1))
+                        if (!b->lam && !(m1 & 1) && !(m2 & 1))
                             hcol[cls[(size_t)(m2 + L) * W + (mu3 + L)]] += w * Y(l1, m1) * Y(l2, m2) * Y(l3, mu3);
                         ++cnt;
                     }
@@ -1266,6 +1287,55 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     if (hw && basislen != expect_len) {
         delete b;
         return tp_fail(TP_E_ARG, "host basis has entries outside the enumeration of (L, LMAX)");
+    }
+    // curved Ewald sphere: per-class entries over the l-triples, the shells'
+    // sphPlm rows and the k-triple decoding of k_glam
+    std::vector<LamEnt> lament;
+    std::vector<int> lamcs, ktk;
+    std::vector<double> lamY;
+    if (b->lam) {
+        b->lam_yrow = (L + 1) * W;
+        lamY.assign((size_t)K * b->lam_yrow, 0.0);
+        for (int k = 1; k <= K; ++k) {
+            const double thk = std::acos(k * lambda * dq / (4 * tph::kPi));
+            for (int l = 0; l <= L; ++l)
+                for (int m = -l; m <= l; ++m) lamY[(size_t)(l * W + m + L) * K + (k - 1)] = tph::sphPlm(l, m, thk);
+        }
+        for (int c = 0; c < b->n_m; ++c) {
+            const size_t first = lament.size();
+            lamcs.push_back((int)first);
+            const int m2 = cls_m2[c], mu3 = cls_mu3[c], m1 = -(m2 + mu3);
+            for (int t = 0; t < nlt; ++t) {
+                const int l3 = ltinfo[t].z, l1 = ltinfo[t].w & 255, l2 = ltinfo[t].w >> 8;
+                if (std::abs(m1) > l1 || std::abs(m2) > l2 || std::abs(mu3) > l3) continue;
+                double w;
+                if (hw) {
+                    const auto it = hw->find(wkey(l1, l2, l3, m1, m2));
+                    if (it == hw->end()) continue;
+                    w = it->second;
+                } else {
+                    w = tph::wigner3j(l1, m1, l2, m2, l3, mu3);
+                    if (!(std::fabs(w) > thr)) continue;
+                }
+                // (-m1, -m2, -mu3) gives the same term: 3j and the sphPlm signs cancel for even l1 + l2 + l3
+                const double mult = (m2 == 0 && mu3 == 0) ? 1.0 : 2.0;
+                lament.push_back(LamEnt{mult * w, t, (l1 * W + m1 + L) * K, (l2 * W + m2 + L) * K,
+                                        (l3 * W + mu3 + L) * K, 0});
+            }
+            // (l3, l2, l1) order within the class (the rows y3, y2, y1 grow with l)
+            std::sort(lament.begin() + first, lament.end(), [](const LamEnt &x, const LamEnt &y) {
+                return x.y3 != y.y3 ? x.y3 < y.y3 : (x.y2 != y.y2 ? x.y2 < y.y2 : x.y1 < y.y1);
+            });
+            for (size_t i = first; i < lament.size(); ++i) {
+                const bool n3 = i == first || lament[i].y3 != lament[i - 1].y3;
+                const bool n2 = n3 || lament[i].y2 != lament[i - 1].y2;
+                lament[i].fl = (n3 ? 1 : 0) | (n2 ? 2 : 0);
+            }
+        }
+        lamcs.push_back((int)lament.size());
+        for (int k1 = 1; k1 <= K; ++k1)
+            for (int k2 = 1; k2 <= k1; ++k2)
+                for (int k3 = 1; k3 <= k2; ++k3) ktk.push_back(k1 | (k2 << 10) | (k3 << 20));
     }
     // the same groups padded to multiples of 8 entries (zero weight, the
     // group's first index) for the FP16 triple products
@@ -1421,12 +1491,12 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             }
 
     // separated-grid tables (k_c3sep)
-    b->n_m2 = L / 2 + 1;
+    b->n_m2 = L / mstep + 1;
     b->jpad = (int)round_up(2 * N, 16);
     b->sep_kb = (int)round_up(2 * b->n_m2, 4);
     b->njc = b->jpad / 16;
     std::vector<int> m2start(b->n_m2 + 1, b->n_m);
-    for (int c = b->n_m - 1; c >= 0; --c) m2start[cls_m2[c] / 2] = c;
+    for (int c = b->n_m - 1; c >= 0; --c) m2start[cls_m2[c] / mstep] = c;
     for (int i = b->n_m2 - 1; i >= 0; --i)
         if (m2start[i] > m2start[i + 1]) m2start[i] = m2start[i + 1];
     int max_cls = 0;
@@ -1444,8 +1514,8 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     for (int i = 0; i < N; ++i) {
         const double a = (i + 0.5) * tph::kPi / N;
         for (int m2i = 0; m2i < b->n_m2; ++m2i) {
-            TB[(size_t)i * b->sep_kb + 2 * m2i] = std::cos(2 * m2i * a);
-            TB[(size_t)i * b->sep_kb + 2 * m2i + 1] = -std::sin(2 * m2i * a);
+            TB[(size_t)i * b->sep_kb + 2 * m2i] = std::cos(mstep * m2i * a);
+            TB[(size_t)i * b->sep_kb + 2 * m2i + 1] = -std::sin(mstep * m2i * a);
         }
     }
 
@@ -1480,8 +1550,15 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         tp_basis_destroy(b);
         return rc;
     }
-    // padding of As (l-triple rows >= n_lt, columns >= klength) must read as zero
+    if (b->lam && ((rc = up(&b->d_lamY, lamY)) || (rc = up(&b->d_ktk, ktk)) || (rc = up(&b->d_lament, lament)) ||
+                   (rc = up(&b->d_lamcs, lamcs)))) {
+        tp_basis_destroy(b);
+        return rc;
+    }
+    // padding of As (l-triple rows >= n_lt, columns >= klength) must read as
+    // zero; so must the rows of G past the classes, which k_glam leaves alone
     hipError_t e = hipMemset(b->d_As, 0, sizeof(double) * (size_t)b->nlt_pad * b->kl_pad);
+    if (e == hipSuccess) e = hipMemset(b->d_G, 0, sizeof(double) * (size_t)b->nm_pad64 * b->kl_pad);
     if (e == hipSuccess) e = hipMemset(b->d_counter, 0, sizeof(unsigned));
     if (e != hipSuccess) {
         tp_basis_destroy(b);
@@ -1514,6 +1591,8 @@ int tp_basis_set_precision(tp_basis *b, int mode)
     if (int rc = check_ctx(b->ctx)) return rc;
     if (mode != TP_PREC_FP64 && (b->K > 64 || b->n_c > 1024))
         return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64 and L <= 44");
+    if (mode != TP_PREC_FP64 && b->lam)
+        return tp_fail(TP_E_UNSUPPORTED, "mixed precision needs lambda dq = 0 (the curved Ewald sphere runs in FP64)");
     if ((mode == TP_PREC_FP16X3 || mode == TP_PREC_BF16) && !b->d_Fhi) {
         // F[r][lt] = sum_class Bm[class][r] H[lt][class] in FP64, then scaled by a
         // power of two into [2^13, 2^14) and split into FP16 hi/lo and BF16
@@ -1753,6 +1832,7 @@ int tp_energy_grad(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const doubl
     if (!bc || !r || !E_out || !grad_out || !coeff) return tp_fail(TP_E_ARG, "null argument");
     tp_basis *b = const_cast<tp_basis *>(bc);
     if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
+    if (b->lam) return tp_fail(TP_E_UNSUPPORTED, "the energy gradient needs lambda dq = 0");
     if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "coefficient vector shorter than LMAX^2");
     if (int rc = grad_setup(ctx, b)) return rc;
     if (int rc = upload_coeff(ctx, b, coeff, ld, b->K, b->d_coef, b->d_raw)) return rc;
@@ -1923,6 +2003,7 @@ int tp_two_photons(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t
     if (!bc || !c2_out) return tp_fail(TP_E_ARG, "null argument");
     tp_basis *b = const_cast<tp_basis *>(bc);
     if (b->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
+    if (bc->lam) return tp_fail(TP_E_UNSUPPORTED, "twoPhotons with lambda dq != 0 (alpha_star) is not implemented");
     if (K2 < 1) return tp_fail(TP_E_ARG, "need K2 >= 1");
     const int N = b->N, nl = b->L / b->lstep + 1;
     const size_t lds = two_photons_lds_bytes(b->n_c, N, nl);
@@ -3053,6 +3134,7 @@ int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, cons
     *out = nullptr;
     if (int rc = check_ctx(ctx)) return rc;
     if (!b || !refs || !coeff || !ld || !seeds) return tp_fail(TP_E_ARG, "null argument");
+    if (b->lam) return tp_fail(TP_E_UNSUPPORTED, "batched runs need lambda dq = 0");
     if (b->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
     if (ngroups < 1 || nprop < 3 || nprop > 4096 || hist_cap < 0)
         return tp_fail(TP_E_ARG, "need ngroups >= 1, 3 <= nprop <= 4096, hist_cap >= 0");
