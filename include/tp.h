@@ -67,7 +67,7 @@ void tp_finalize(tp_ctx *ctx);
  * (correlations.jl:94), which needs |K lambda dq| <= 4 pi; such a basis
  * evaluates in FP64 only (tp_c3, tp_energy, tp_state_*, tp_anneal,
  * tp_energy_batch), and mixed precision, batched runs, the energy gradient
- * and tp_two_photons return TP_E_UNSUPPORTED for it. */
+ * and tp_two_photons (use tp_two_photons_ewald) return TP_E_UNSUPPORTED for it. */
 int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq,
                     int for_intensity, tp_basis **out);
 /* The same from a basis the caller already holds: wignerlist[basislen] and
@@ -203,8 +203,8 @@ int tp_hist_add(tp_hist *h, const double *photons, const int64_t *offsets, int n
 int tp_hist_read(tp_hist *h, double *c1, double *c2, double *c3, uint64_t *n1, uint64_t *n2, uint64_t *n3);
 void tp_hist_destroy(tp_hist *h);
 
-/* Two-photon correlation of a structure: twoPhotons (correlations.jl:229-243),
- * lambda = 0.  coeff: K2 shells of ld complex (interleaved) coefficients in the
+/* Two-photon correlation of a structure: twoPhotons (correlations.jl:229-243)
+ * on the flat detector (lambda = 0).  coeff: K2 shells of ld complex (interleaved) coefficients in the
  * seanindex layout; the l range and N are the basis's.  c2_out[N,K2,K2] in the
  * reference's (column-major) layout, c2[a,k2,k1] = sum_{l even <= L}
  * Re(c_k1,l . c_k2,l) P_l(cos alpha_a) / (4 pi), alpha_a = (a + 1/2) pi / N;
@@ -212,6 +212,13 @@ void tp_hist_destroy(tp_hist *h);
  * sum |c2| instead of by 4 pi.  One kernel launch (timed by tp_set_timing). */
 int tp_two_photons(tp_ctx *ctx, const tp_basis *basis, const double *coeff, int64_t ld, int K2, int minimal,
                    int normalize, double *c2_out);
+/* twoPhotons with the basis's lambda and the volume's shell spacing dq
+ * (dq(volume), correlations.jl:231): the series is taken at P_l(cos
+ * alpha_star) (datagen.jl:49-53), cos alpha_star = sin th1 sin th2 cos alpha +
+ * cos th1 cos th2 with th_k = acos(k dq lambda / (4 pi)); needs
+ * |K2 dq lambda| <= 4 pi.  With lambda = 0 it is tp_two_photons. */
+int tp_two_photons_ewald(tp_ctx *ctx, const tp_basis *basis, const double *coeff, int64_t ld, int K2, int minimal,
+                         int normalize, double dq, double *c2_out);
 
 /* A start structure from a two-photon histogram: retrieveSolution
  * (correlations.jl:262-319), lambda = 0, host code (no GPU call).  c2[N,K2dim,K2dim]
@@ -224,6 +231,12 @@ int tp_two_photons(tp_ctx *ctx, const tp_basis *basis, const double *coeff, int6
  * NULL) = the summed shell negativity before that change. */
 int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi, int L, int LMAX,
                          double *coeff_out, int64_t ld, int *flipped, double *negativity);
+/* retrieveSolution(c2, L, LMAX, k2lo:k2hi, qmax, lambda) with any lambda: the
+ * fit's Legendre columns are taken at alpha_star of each shell pair with
+ * dq = qmax / k2hi, and the slices are symmetrised only for lambda = 0
+ * (correlations.jl:282-289).  With lambda = 0 it is tp_retrieve_solution. */
+int tp_retrieve_solution_ewald(const double *c2, int N, int K2dim, int k2lo, int k2hi, int L, int LMAX, double qmax,
+                               double lambda, double *coeff_out, int64_t ld, int *flipped, double *negativity);
 
 /* Rotational fit of a structure to a reference: the scoring inside
  * fitStructures_full / fitStructures_random (data_processing.jl:26-79).

@@ -32,9 +32,18 @@ def alpharange(N):
     return np.linspace(da / 2, np.pi - da / 2, N)
 
 
-def two_photons(coeff, L, LMAX, N, K2, minimal=False, normalize=False):
+def alpha_star(alpha, k1, k2, dq, lam):
+    """datagen.jl:49-53: the angle between the Ewald-sphere points of shells k1,
+    k2 (1-based) whose detector azimuths differ by alpha."""
+    t1 = np.arccos(k1 * dq * lam / (4 * np.pi))
+    t2 = np.arccos(k2 * dq * lam / (4 * np.pi))
+    return np.arccos(np.sin(t1) * np.sin(t2) * np.cos(alpha) + np.cos(t1) * np.cos(t2))
+
+
+def two_photons(coeff, L, LMAX, N, K2, minimal=False, normalize=False, lam=0.0, dq=0.0):
     """c2 (N, K2, K2) indexed [a, k2, k1] (zero-based) from coefficients
-    coeff (>= K2, LMAX^2) complex in the seanindex layout."""
+    coeff (>= K2, LMAX^2) complex in the seanindex layout; lam != 0 takes the
+    Legendre series at alpha_star with the volume's shell spacing dq."""
     c = np.asarray(coeff, dtype=np.complex128)
     al = alpharange(N)
     out = np.zeros((N, K2, K2))
@@ -44,7 +53,8 @@ def two_photons(coeff, L, LMAX, N, K2, minimal=False, normalize=False):
             for l in range(0, L + 1, 2):
                 ix = [seanindex(l, m, LMAX) - 1 for m in range(-l, l + 1)]
                 fac = np.vdot(c[k1, ix], c[k2, ix])       # sum conj(a) b
-                sl = sl + fac * eval_legendre(l, np.cos(al))
+                x = np.cos(al) if lam == 0.0 else np.cos(alpha_star(al, k1 + 1, k2 + 1, dq, lam))
+                sl = sl + fac * eval_legendre(l, x)
             out[:, k2, k1] = sl.real
     return out / np.abs(out).sum() if normalize else out / (4 * np.pi)
 
@@ -93,10 +103,11 @@ def negativity(coeff, LMAX, lo, hi):
     return tot
 
 
-def retrieve_solution(c2, L, LMAX, lo, hi):
+def retrieve_solution(c2, L, LMAX, lo, hi, qmax=1.0, lam=0.0):
     """(coeff (hi, LMAX^2), info): coefficients fitted to c2 ([a, k2, k1]) over
     shells lo..hi (1-based); info holds G and the eigenvalues per l, the
-    negativity before the flip and whether every sign was changed."""
+    negativity before the flip and whether every sign was changed.  lam != 0:
+    Legendre columns at alpha_star with dq = qmax / hi, no symmetrisation."""
     N = c2.shape[0]
     K2 = hi - lo + 1
     assert 2 * L < K2
@@ -108,9 +119,11 @@ def retrieve_solution(c2, L, LMAX, lo, hi):
         for k1 in range(lo, hi + 1):
             for k2 in range(lo, k1 + 1):
                 s = c2[:, k2 - 1, k1 - 1]
-                s = 0.5 * (s + s[::-1])
+                if lam == 0.0:
+                    s = 0.5 * (s + s[::-1])
                 upperL = min(L, 2 * min(k1, k2) - 2)
-                A = np.stack([eval_legendre(lp, np.cos(al)) / (4 * np.pi) for lp in range(0, upperL + 1, 2)], axis=1)
+                x = np.cos(al) if lam == 0.0 else np.cos(alpha_star(al, k1, k2, qmax / hi, lam))
+                A = np.stack([eval_legendre(lp, x) / (4 * np.pi) for lp in range(0, upperL + 1, 2)], axis=1)
                 fac = np.linalg.lstsq(A, s, rcond=None)[0]
                 if l <= upperL:
                     G[k2 - lo, k1 - lo] = G[k1 - lo, k2 - lo] = fac[l // 2]

@@ -100,5 +100,29 @@ def test_retrieve_rejects_bad_ranges(tp):
         tp.retrieveSolution(c2, 6, 8, 10, 1.0)
     with pytest.raises(RuntimeError, match="k2hi"):
         tp.retrieveSolution(c2, 2, 8, range(1, 12), 1.0)
-    with pytest.raises(NotImplementedError):
-        tp.retrieveSolution(c2, 2, 8, 10, 1.0, lam=1.0)
+    with pytest.raises(RuntimeError, match="alpha_star"):
+        tp.retrieveSolution(c2, 2, 8, 10, 20.0, lam=1.0)   # qmax lambda > 4 pi
+
+
+@pytest.mark.parametrize("lam", [2.5, 1.0])
+def test_two_photons_and_retrieve_ewald_match_oracle(tp, c2o, lam):
+    """lambda != 0: twoPhotons at alpha_star with the volume's dq
+    (tp_two_photons_ewald) and retrieveSolution with alpha_star columns
+    (tp_retrieve_solution_ewald), against the oracle (pinned by the rotation
+    average on the Ewald sphere and the round trip in test_oracle_c2.py)."""
+    L, LMAX, N, K2, qmax = 4, 6, 10, 12, 1.2
+    c = random_real_coeff(K2, L, LMAX, 23, c00=4.0)
+    vol = tp.SphericalHarmonicsVolume(c, LMAX, K2, qmax)
+    b = tp.HIP_calculate_basis(L, LMAX, N, K2, lam, vol.dq())
+    got = tp.twoPhotons(vol, b, K2)
+    want = c2o.two_photons(c, L, LMAX, N, K2, lam=lam, dq=vol.dq())
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    rec, info = tp.retrieveSolution(want, L, LMAX, K2, qmax, lam=lam, return_info=True)
+    wrec, winfo = c2o.retrieve_solution(want, L, LMAX, 1, K2, qmax=qmax, lam=lam)
+    assert info["flipped"] == winfo["flipped"]
+    assert np.abs(rec.coeff - wrec).max() <= 1e-8 * np.abs(wrec).max()
+    with pytest.raises(RuntimeError, match="tp_two_photons_ewald"):
+        from tp_amd import check, lib
+        out = np.zeros(N * K2 * K2)
+        cc = np.ascontiguousarray(c)
+        check(lib().tp_two_photons(b.ctx.h, b.h, cc.ctypes.data, LMAX * LMAX, K2, 0, 0, out.ctypes.data), "x")

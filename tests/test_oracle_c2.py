@@ -141,3 +141,50 @@ def test_retrieve_round_trip(c2o, L, LMAX, N, K2, lo, c00):
     if c00 >= 30.0:
         assert not info["flipped"] and info["negativity"] == 0.0
         assert np.all(rec[lo - 1:, 0].real > 0)
+
+
+def test_two_photons_ewald_is_the_rotation_average(c2o):
+    """lambda != 0: c2(a, k2, k1) = (1/8 pi^2) int_SO(3) f_k1(R n1) f_k2(R n2) dR
+    with n1, n2 on the Ewald sphere (polar angles acos(k dq lambda / 4 pi),
+    azimuths 0 and alpha_a); exact quadrature over Euler angles."""
+    L, LMAX, N, K, lam, dq = 4, 5, 6, 3, 2.0, 1.1
+    c = random_real_coeff(K, L, LMAX, 13, c00=2.0)
+    got = c2o.two_photons(c, L, LMAX, N, K, lam=lam, dq=dq)
+    flat = c2o.two_photons(c, L, LMAX, N, K)
+    assert np.abs(got - flat).max() > 1e-3 * np.abs(flat).max()
+    gx, gw = np.polynomial.legendre.leggauss(6)
+    na = 10
+    A, B, G = np.meshgrid(2 * np.pi * np.arange(na) / na, np.arccos(gx), 2 * np.pi * np.arange(na) / na, indexing="ij")
+    W = np.broadcast_to(gw[None, :, None], A.shape) * (2 * np.pi / na) ** 2 / (8 * np.pi ** 2)
+    ca, sa, cb, sb, cg, sg = np.cos(A), np.sin(A), np.cos(B), np.sin(B), np.cos(G), np.sin(G)
+    # R = Rz(A) Ry(B) Rz(G)
+    Rm = np.stack([np.stack([ca * cb * cg - sa * sg, -ca * cb * sg - sa * cg, ca * sb], -1),
+                   np.stack([sa * cb * cg + ca * sg, -sa * cb * sg + ca * cg, sa * sb], -1),
+                   np.stack([-sb * cg, sb * sg, cb], -1)], -2)
+
+    def f(k, n):
+        v = Rm @ n
+        th, ph = np.arccos(np.clip(v[..., 2], -1, 1)), np.arctan2(v[..., 1], v[..., 0])
+        return surface_at(c[k], LMAX, th, ph).real
+
+    for a, alpha in enumerate(c2o.alpharange(N)):
+        for k1 in range(1, K + 1):
+            for k2 in range(1, K + 1):
+                t1, t2 = np.arccos(k1 * dq * lam / (4 * np.pi)), np.arccos(k2 * dq * lam / (4 * np.pi))
+                n1 = np.array([np.sin(t1), 0.0, np.cos(t1)])
+                n2 = np.array([np.sin(t2) * np.cos(alpha), np.sin(t2) * np.sin(alpha), np.cos(t2)])
+                avg = (W * f(k1 - 1, n1) * f(k2 - 1, n2)).sum()
+                assert got[a, k2 - 1, k1 - 1] == pytest.approx(avg, rel=1e-10, abs=1e-12 * np.abs(got).max())
+
+
+def test_retrieve_round_trip_ewald(c2o):
+    """lambda != 0: the fit uses alpha_star columns (dq = qmax / hi) and no
+    symmetrisation; c2 of the retrieved structure equals the c2 it was fitted to."""
+    L, LMAX, N, K2, qmax, lam = 4, 6, 10, 12, 1.2, 3.0
+    c = restricted_coeff(K2, L, LMAX, 17, 4.0)
+    c2 = c2o.two_photons(c, L, LMAX, N, K2, lam=lam, dq=qmax / K2)
+    rec, info = c2o.retrieve_solution(c2, L, LMAX, 1, K2, qmax=qmax, lam=lam)
+    back = c2o.two_photons(rec, L, LMAX, N, K2, lam=lam, dq=qmax / K2)
+    np.testing.assert_allclose(back, c2, rtol=1e-9, atol=1e-10 * np.abs(c2).max())
+    flat, _ = c2o.retrieve_solution(c2, L, LMAX, 1, K2)
+    assert np.abs(c2o.two_photons(flat, L, LMAX, N, K2, lam=lam, dq=qmax / K2) - c2).max() > 1e-6 * np.abs(c2).max()

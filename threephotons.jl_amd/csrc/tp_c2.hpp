@@ -1,6 +1,8 @@
 // tp_c2.hpp -- two-photon correlation of a structure (twoPhotons,
-// correlations.jl:229-243) on the device.  lambda = 0 (flat detector:
-// alpha_star(alpha) = alpha, datagen.jl:49-53).
+// correlations.jl:229-243) on the device.  On the flat detector (lambda = 0)
+// alpha_star(alpha) = alpha and the Legendre table is shared by every shell
+// pair; otherwise the series is taken at alpha_star (datagen.jl:49-53), the
+// angle between the two shells' Ewald-sphere points.
 #pragma once
 
 // Legendre P_l(x), l = 0..lmax, by Bonnet's recurrence
@@ -35,10 +37,12 @@ inline size_t two_photons_lds_bytes(int n_c, int N, int nl)
 // per (k2, a) sums the Legendre series from LDS.  With minimal only k2 <= k1
 // is computed (the rest is written 0, as the reference leaves it).  coef:
 // compact [K2][n_c], order l at coff[l], m = -l..l.  out: [K2][K2][N] (a
-// fastest) == Julia c2[:, k2, k1].
+// fastest) == Julia c2[:, k2, k1].  ewc = dq lambda / (4 pi): 0 on the flat
+// detector; otherwise theta_k = acos(k ewc) and the series is taken at
+// P_l(cos alpha_star), cos alpha_star = sin th1 sin th2 cos alpha + cos th1 cos th2.
 __global__ __launch_bounds__(256) void k_two_photons(const double2 *__restrict__ coef, int n_c,
                                                      const int *__restrict__ coff, int L, int lstep, int N, int K2,
-                                                     int minimal, double scale, double *__restrict__ out)
+                                                     int minimal, double scale, double ewc, double *__restrict__ out)
 {
     extern __shared__ double lds[];
     const int nl = L / lstep + 1;
@@ -73,8 +77,18 @@ __global__ __launch_bounds__(256) void k_two_photons(const double2 *__restrict__
         const int j = t / N, a = t - j * N;
         double v = 0.0;
         if (j < k2n) {
-            const double *d = dl + j * nl, *p = pt + a * nl;
-            for (int li = 0; li < nl; ++li) v += d[li] * p[li];
+            const double *d = dl + j * nl;
+            if (ewc == 0.0) {
+                const double *p = pt + a * nl;
+                for (int li = 0; li < nl; ++li) v += d[li] * p[li];
+            } else {
+                const double t1 = acos((k1 + 1) * ewc), t2 = acos((c0 + j + 1) * ewc);
+                const double al = (a + 0.5) * M_PI / N;
+                const double x = cos(acos(sin(t1) * sin(t2) * cos(al) + cos(t1) * cos(t2)));
+                tp_legendre(L, x, [&](int l, double p) {
+                    if (l % lstep == 0) v += d[l / lstep] * p;
+                });
+            }
             v *= scale;
         }
         o[t] = v;

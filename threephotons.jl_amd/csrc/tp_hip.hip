@@ -1996,15 +1996,16 @@ int tp_hist_read(tp_hist *h, double *c1, double *c2, double *c3, uint64_t *n1_ou
     return 0;
 }
 
-int tp_two_photons(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, int K2, int minimal,
-                   int normalize, double *c2_out)
+// twoPhotons with ewc = dq lambda / (4 pi) (0: the flat detector)
+static int two_photons(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, int K2, int minimal,
+                       int normalize, double ewc, double *c2_out)
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!bc || !c2_out) return tp_fail(TP_E_ARG, "null argument");
     tp_basis *b = const_cast<tp_basis *>(bc);
     if (b->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
-    if (bc->lam) return tp_fail(TP_E_UNSUPPORTED, "twoPhotons with lambda dq != 0 (alpha_star) is not implemented");
     if (K2 < 1) return tp_fail(TP_E_ARG, "need K2 >= 1");
+    if (!(std::fabs(K2 * ewc) <= 1.0)) return tp_fail(TP_E_ARG, "alpha_star undefined: need |K2 dq lambda| <= 4 pi");
     const int N = b->N, nl = b->L / b->lstep + 1;
     const size_t lds = two_photons_lds_bytes(b->n_c, N, nl);
     if (lds > 64 * 1024) return tp_fail(TP_E_UNSUPPORTED, "two-photon tables exceed 64 KiB of LDS (N or L too large)");
@@ -2033,7 +2034,7 @@ int tp_two_photons(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
     hipLaunchKernelGGL(k_two_photons, dim3((unsigned)K2, (unsigned)((K2 + kC2Chunk - 1) / kC2Chunk)), dim3(256), lds, st,
                        b->d_c2coef, b->n_c, b->d_coff, b->L,
-                       b->lstep, N, K2, minimal ? 1 : 0, normalize ? 1.0 : 1.0 / (4.0 * tph::kPi), b->d_c2);
+                       b->lstep, N, K2, minimal ? 1 : 0, normalize ? 1.0 : 1.0 / (4.0 * tph::kPi), ewc, b->d_c2);
     TP_HIP(hipGetLastError());
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     TP_HIP(hipMemcpyAsync(c2_out, b->d_c2, sizeof(double) * (size_t)len, hipMemcpyDeviceToHost, st));
@@ -2051,6 +2052,21 @@ int tp_two_photons(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t
     return 0;
 }
 
+int tp_two_photons(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, int K2, int minimal,
+                   int normalize, double *c2_out)
+{
+    if (bc && bc->lam)
+        return tp_fail(TP_E_UNSUPPORTED, "the basis has lambda dq != 0: use tp_two_photons_ewald with the volume's dq");
+    return two_photons(ctx, bc, coeff, ld, K2, minimal, normalize, 0.0, c2_out);
+}
+
+int tp_two_photons_ewald(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_t ld, int K2, int minimal,
+                         int normalize, double dq, double *c2_out)
+{
+    if (!bc) return tp_fail(TP_E_ARG, "null basis");
+    return two_photons(ctx, bc, coeff, ld, K2, minimal, normalize, dq * bc->lambda / (4 * tph::kPi), c2_out);
+}
+
 int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi, int L, int LMAX, double *coeff_out,
                          int64_t ld, int *flipped, double *negativity)
 {
@@ -2059,6 +2075,22 @@ int tp_retrieve_solution(const double *c2, int N, int K2dim, int k2lo, int k2hi,
     const std::string err = tpc2::retrieve_solution(c2, N, K2dim, k2lo, k2hi, L, LMAX,
                                                     reinterpret_cast<std::complex<double> *>(coeff_out), ld, flipped,
                                                     negativity);
+    if (!err.empty()) return tp_fail(TP_E_ARG, err);
+    return 0;
+}
+
+int tp_retrieve_solution_ewald(const double *c2, int N, int K2dim, int k2lo, int k2hi, int L, int LMAX, double qmax,
+                               double lambda, double *coeff_out, int64_t ld, int *flipped, double *negativity)
+{
+    if (!c2 || !coeff_out) return tp_fail(TP_E_ARG, "null argument");
+    if (N < 1 || K2dim < 1 || LMAX < 1 || k2hi < 1) return tp_fail(TP_E_ARG, "need N, K2, LMAX, k2hi >= 1");
+    // the structure is SphericalHarmonicsVolume(LMAX, k2hi, qmax): dq = qmax / k2hi (correlations.jl:269-271)
+    const double mdq = qmax / k2hi;
+    if (!(std::fabs(k2hi * mdq * lambda / (4 * tph::kPi)) <= 1.0))
+        return tp_fail(TP_E_ARG, "alpha_star undefined: need |qmax lambda| <= 4 pi");
+    const std::string err = tpc2::retrieve_solution(c2, N, K2dim, k2lo, k2hi, L, LMAX,
+                                                    reinterpret_cast<std::complex<double> *>(coeff_out), ld, flipped,
+                                                    negativity, lambda, mdq);
     if (!err.empty()) return tp_fail(TP_E_ARG, err);
     return 0;
 }

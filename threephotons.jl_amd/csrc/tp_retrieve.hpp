@@ -145,7 +145,7 @@ inline double shell_negativity(const std::complex<double> *c, int LMAX, const st
     return neg / tot;
 }
 
-// retrieveSolution (correlations.jl:262-319) with lambda = 0.  c2: Julia
+// retrieveSolution (correlations.jl:262-319).  c2: Julia
 // layout [N][K2dim][K2dim] (c2[a + N (k2-1 + K2dim (k1-1))]); shells lo..hi
 // (1-based) are fitted; out: hi shells of ld complex coefficients (every other
 // entry 0).  Per even l <= L: for k1 in lo..hi, k2 in lo..k1 the symmetrised
@@ -156,10 +156,13 @@ inline double shell_negativity(const std::complex<double> *c, int LMAX, const st
 // V[k, j] in the real basis, m = j - l; real_to_comp (spherical_harmonics.jl:
 // 472-481) maps x to c = Umat(l)^H x.  If the summed shell negativity of the
 // result exceeds 0.33 every coefficient changes sign.  The fit does not
-// depend on l, so each (k1, k2) is fitted once.  Returns an error message, or
-// an empty string; *flipped (if given) reports the sign change.
+// depend on l, so each (k1, k2) is fitted once.  With lambda != 0 the slices
+// are not symmetrised and the columns are P_l'(cos alpha_star) / (4 pi) of the
+// pair (datagen.jl:49-53) with the shell spacing mdq.  Returns an error
+// message, or an empty string; *flipped (if given) reports the sign change.
 inline std::string retrieve_solution(const double *c2, int N, int K2dim, int lo, int hi, int L, int LMAX,
-                                     std::complex<double> *out, int64_t ld, int *flipped, double *negativity)
+                                     std::complex<double> *out, int64_t ld, int *flipped, double *negativity,
+                                     double lambda = 0.0, double mdq = 0.0)
 {
     const int K2 = hi - lo + 1;
     if (lo < 1 || hi > K2dim || K2 < 1) return "need 1 <= k2lo <= k2hi <= K2";
@@ -183,10 +186,25 @@ inline std::string retrieve_solution(const double *c2, int N, int K2dim, int lo,
     for (int k1 = lo; k1 <= hi; ++k1)
         for (int k2 = lo; k2 <= k1; ++k2) {
             const double *sl = c2 + (size_t)N * ((k2 - 1) + (size_t)K2dim * (k1 - 1));
-            for (int a = 0; a < N; ++a) s[a] = 0.5 * (sl[a] + sl[N - 1 - a]);
+            if (lambda == 0.0)   // symmetrised on the flat detector only (correlations.jl:282-285)
+                for (int a = 0; a < N; ++a) s[a] = 0.5 * (sl[a] + sl[N - 1 - a]);
+            else
+                for (int a = 0; a < N; ++a) s[a] = sl[a];
             const int upperL = std::min(L, 2 * std::min(k1, k2) - 2);
             const int nc = upperL / 2 + 1;
             std::vector<double> A(Pl.begin(), Pl.begin() + (size_t)N * nc);
+            if (lambda != 0.0) {
+                const double t1 = std::acos(k1 * mdq * lambda / (4 * tph::kPi));
+                const double t2 = std::acos(k2 * mdq * lambda / (4 * tph::kPi));
+                for (int a = 0; a < N; ++a) {
+                    const double al = (a + 0.5) * tph::kPi / N;
+                    const double x = std::cos(std::acos(std::sin(t1) * std::sin(t2) * std::cos(al) +
+                                                        std::cos(t1) * std::cos(t2)));
+                    tp_legendre(2 * (nc - 1), x, [&](int l, double p) {
+                        if ((l & 1) == 0) A[(size_t)a + (size_t)N * (l / 2)] = p / (4.0 * tph::kPi);
+                    });
+                }
+            }
             lstsq(A, N, nc, s, f);
             const size_t o = (size_t)(k1 - lo) * K2 + (k2 - lo);
             up[o] = upperL;

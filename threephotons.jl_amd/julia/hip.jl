@@ -183,24 +183,26 @@ l range and alpha grid (lambda = 0)."""
 function twoPhotons(volume::SphericalHarmonicsVolume, basis::BasisTypeHip, K2::Int64, minimal::Bool=false, normalize::Bool=false)
     coeff = hip_pack(volume, K2)
     c = zeros(Float64, basis.N, K2, K2)
-    hip_check(ccall((:tp_two_photons, libtp), Cint,
-                    (Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Cint, Cint, Cint, Ptr{Float64}),
-                    hip_ctx, basis.handle, coeff, size(coeff, 1), K2, minimal, normalize, c), "tp_two_photons")
+    # alpha_star with basis.lambda and dq(volume), as the reference's twoPhotons (correlations.jl:231,238)
+    hip_check(ccall((:tp_two_photons_ewald, libtp), Cint,
+                    (Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Cint, Cint, Cint, Cdouble, Ptr{Float64}),
+                    hip_ctx, basis.handle, coeff, size(coeff, 1), K2, minimal, normalize, dq(volume), c),
+              "tp_two_photons_ewald")
     return c
 end
 
-"""retrieveSolution (correlations.jl:262-319) through libtp (host code, lambda = 0 only):
+"""retrieveSolution (correlations.jl:262-319) through libtp (host code, any lambda):
 the start structure whose two-photon correlation matches c2 on K2_range."""
 function retrieveSolution_hip(c2::C2, L::Int64, LMAX::Int64, K2_range::UnitRange{Int64}, qmax::Float64, lambda::Float64=0.0)
-    @assert lambda == 0.0
     N, K2dim, _ = size(c2)
     hi = maximum(K2_range)
     coeff = zeros(Complex{Float64}, LMAX^2, hi)
     flipped = Ref{Cint}(0); neg = Ref{Cdouble}(0.0)
-    hip_check(ccall((:tp_retrieve_solution, libtp), Cint,
-                    (Ptr{Float64}, Cint, Cint, Cint, Cint, Cint, Cint, Ptr{Complex{Float64}}, Int64, Ptr{Cint}, Ptr{Cdouble}),
-                    Array{Float64}(c2), N, K2dim, minimum(K2_range), hi, L, LMAX, coeff, LMAX^2, flipped, neg),
-              "tp_retrieve_solution")
+    hip_check(ccall((:tp_retrieve_solution_ewald, libtp), Cint,
+                    (Ptr{Float64}, Cint, Cint, Cint, Cint, Cint, Cint, Cdouble, Cdouble, Ptr{Complex{Float64}}, Int64,
+                     Ptr{Cint}, Ptr{Cdouble}),
+                    Array{Float64}(c2), N, K2dim, minimum(K2_range), hi, L, LMAX, qmax, lambda, coeff, LMAX^2,
+                    flipped, neg), "tp_retrieve_solution_ewald")
     intensity = SphericalHarmonicsVolume(LMAX, hi, qmax)
     for k = 1:hi intensity.coeff[k][:] = coeff[:, k] end
     return intensity

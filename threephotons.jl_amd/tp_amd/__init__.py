@@ -370,14 +370,24 @@ class PhotonHistogram:
                 "n2": n2.reshape(s2, order="F"), "n3": n3.reshape(s3, order="F")}
 
 
-def twoPhotons(volume, basis, K2, minimal=False, normalize=False):
+def twoPhotons(volume, basis, K2, minimal=False, normalize=False, dq=None):
     """Two-photon correlation c2 (N, K2, K2), indexed [a, k2, k1] (zero-based),
-    of shells 1..K2 of `volume` on the GPU (tp_two_photons; correlations.jl:229-243,
-    lambda = 0).  The l range and N are the basis's."""
+    of shells 1..K2 of `volume` on the GPU (correlations.jl:229-243).  The l
+    range, N and lambda are the basis's; with lambda != 0 the series is taken
+    at alpha_star with the volume's shell spacing (dq(volume), or `dq` for a
+    plain coefficient array; tp_two_photons_ewald), else tp_two_photons."""
     c, ld = _coeff_ptr(volume, basis, K2)
     out = np.zeros(basis.N * K2 * K2)
-    check(lib().tp_two_photons(basis.ctx.h, basis.h, c.ctypes.data, ld, K2, int(bool(minimal)),
-                               int(bool(normalize)), out.ctypes.data), "tp_two_photons")
+    if basis.lam != 0.0:
+        if dq is None:
+            if not isinstance(volume, SphericalHarmonicsVolume):
+                raise ValueError("lambda != 0 needs the volume's shell spacing: pass dq")
+            dq = volume.dq()
+        check(lib().tp_two_photons_ewald(basis.ctx.h, basis.h, c.ctypes.data, ld, K2, int(bool(minimal)),
+                                         int(bool(normalize)), float(dq), out.ctypes.data), "tp_two_photons_ewald")
+    else:
+        check(lib().tp_two_photons(basis.ctx.h, basis.h, c.ctypes.data, ld, K2, int(bool(minimal)),
+                                   int(bool(normalize)), out.ctypes.data), "tp_two_photons")
     return out.reshape((basis.N, K2, K2), order="F")
 
 
@@ -385,9 +395,8 @@ def retrieveSolution(c2, L, LMAX, K2_range, qmax, lam=0.0, return_info=False):
     """A start structure from a two-photon correlation (tp_retrieve_solution;
     correlations.jl:262-319): a SphericalHarmonicsVolume of max(K2_range)
     shells whose c2 matches c2 ([a, k2, k1]) on K2_range.  K2_range is an int
-    (1..K2) or a range of 1-based shells; only lambda = 0 is supported."""
-    if lam != 0.0:
-        raise NotImplementedError("retrieveSolution: only lambda = 0 (flat detector)")
+    (1..K2) or a range of 1-based shells; lam != 0 fits at alpha_star
+    (tp_retrieve_solution_ewald)."""
     rng = range(1, K2_range + 1) if isinstance(K2_range, int) else K2_range
     lo, hi = min(rng), max(rng)
     c2 = np.asarray(c2, dtype=np.float64)
@@ -397,8 +406,13 @@ def retrieveSolution(c2, L, LMAX, K2_range, qmax, lam=0.0, return_info=False):
     flat = np.ascontiguousarray(c2.ravel(order="F"))
     out = np.zeros((hi, LMAX * LMAX), np.complex128)
     flipped, neg = C.c_int(), C.c_double()
-    check(lib().tp_retrieve_solution(flat.ctypes.data, N, K2a, lo, hi, L, LMAX, out.ctypes.data, LMAX * LMAX,
-                                     C.byref(flipped), C.byref(neg)), "tp_retrieve_solution")
+    if lam != 0.0:
+        check(lib().tp_retrieve_solution_ewald(flat.ctypes.data, N, K2a, lo, hi, L, LMAX, float(qmax), float(lam),
+                                               out.ctypes.data, LMAX * LMAX, C.byref(flipped), C.byref(neg)),
+              "tp_retrieve_solution_ewald")
+    else:
+        check(lib().tp_retrieve_solution(flat.ctypes.data, N, K2a, lo, hi, L, LMAX, out.ctypes.data, LMAX * LMAX,
+                                         C.byref(flipped), C.byref(neg)), "tp_retrieve_solution")
     vol = SphericalHarmonicsVolume(out, LMAX, hi, qmax)
     if return_info:
         return vol, {"flipped": bool(flipped.value), "negativity": neg.value}

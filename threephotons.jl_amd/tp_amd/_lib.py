@@ -83,6 +83,8 @@ def lib():
     L.tp_hist_destroy.restype = None
     L.tp_two_photons.argtypes = [vp, vp, vp, i64, i32, i32, i32, vp]
     L.tp_retrieve_solution.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp, i64, vp, vp]
+    L.tp_two_photons_ewald.argtypes = [vp, vp, vp, i64, i32, i32, i32, dbl, vp]
+    L.tp_retrieve_solution_ewald.argtypes = [vp, i32, i32, i32, i32, i32, i32, dbl, dbl, vp, i64, vp, vp]
     L.tp_fit_create.argtypes = [vp, vp, vp, i64, i32, i32, pp]
     L.tp_fit_set_volume.argtypes = [vp, vp, i64]
     L.tp_fit_similarity.argtypes = [vp, i32, vp, vp]
@@ -179,7 +181,8 @@ EXPORTS = [
     "tp_hist_create", "tp_hist_add", "tp_hist_read", "tp_hist_destroy", "tp_two_photons", "tp_retrieve_solution",
     "tp_fit_create", "tp_fit_set_volume", "tp_fit_similarity", "tp_fit_destroy",
     "tp_sh_surface", "tp_sh_coeff", "tp_sph_bessel", "tp_sraar", "tp_pictures", "tp_hist_add_generated",
-    "tp_density_surface", "tp_absolute_square", "tp_structure_from_atoms",
+    "tp_density_surface", "tp_absolute_square", "tp_structure_from_atoms", "tp_two_photons_ewald",
+    "tp_retrieve_solution_ewald",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
