@@ -73,6 +73,8 @@ static uint16_t host_bf16(double x)   // round to nearest even of the float valu
 // ---------------------------------------------------------------------------
 // objects
 // ---------------------------------------------------------------------------
+struct PhTables;
+
 struct tp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -84,7 +86,9 @@ struct tp_ctx {
     double *d_res = nullptr;   // its device address
     tp_batch *eb = nullptr;    // plan (and graph) of the last mixed-precision tp_energy_batch
     std::vector<const void *> eb_sig;
+    std::vector<PhTables *> ph_cache;   // SH / Hankel tables of recent (LMAX, K), newest last
 };
+static void ph_cache_free(tp_ctx *ctx);
 
 // contraction tiles
 constexpr int G1_BM = 64, G1_BN = 32, G1_WGM = 2;   // tile of the gradient's gAs = H^T . gG
@@ -1104,6 +1108,7 @@ void tp_finalize(tp_ctx *ctx)
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     if (ctx->h_res) (void)hipHostFree(ctx->h_res);
     batch_free(ctx->eb);
+    ph_cache_free(ctx);
     delete ctx;
 }
 
@@ -2388,6 +2393,38 @@ static int ph_tables(tp_ctx *ctx, int LMAX, int K, bool bessel, PhTables &t)
     return 0;
 }
 
+// Tables of (LMAX, K) from the context's cache, built on a miss (a table with
+// the Bessel matrices also serves calls without them).  The cache keeps the
+// 8 most recent sizes; the analysis scratch d_G in it is shared by the calls
+// of one context, which never overlap.
+static int ph_get(tp_ctx *ctx, int LMAX, int K, bool bessel, const PhTables **out)
+{
+    auto &c = ctx->ph_cache;
+    for (size_t i = 0; i < c.size(); ++i)
+        if (c[i]->LMAX == LMAX && c[i]->K == K && (!bessel || c[i]->d_Hf)) {
+            *out = c[i];
+            return 0;
+        }
+    PhTables *t = new PhTables();
+    if (int rc = ph_tables(ctx, LMAX, K, bessel, *t)) {
+        delete t;
+        return rc;
+    }
+    if (c.size() >= 8) {
+        delete c.front();
+        c.erase(c.begin());
+    }
+    c.push_back(t);
+    *out = t;
+    return 0;
+}
+
+static void ph_cache_free(tp_ctx *ctx)
+{
+    for (PhTables *t : ctx->ph_cache) delete t;
+    ctx->ph_cache.clear();
+}
+
 // host coefficients (seanindex layout, stride ld) <-> packed (l^2 + l + m)
 static std::vector<double2> ph_pack(const double *coeff, int64_t ld, int K, int LMAX)
 {
@@ -2454,8 +2491,9 @@ int tp_sh_surface(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld,
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!coeff || !surf || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
-    PhTables t;
-    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const PhTables *tcache = nullptr;
+    if (int rc = ph_get(ctx, LMAX, K, false, &tcache)) return rc;
+    const PhTables &t = *tcache;
     const size_t NP = (size_t)4 * LMAX * LMAX;
     DevBuf<double2> c, f;
     if (int rc = dev_alloc(&c.p, (size_t)K * LMAX * LMAX)) return rc;
@@ -2473,8 +2511,9 @@ int tp_sh_coeff(tp_ctx *ctx, int LMAX, int K, const double *surf, double *coeff,
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!coeff || !surf || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
-    PhTables t;
-    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const PhTables *tcache = nullptr;
+    if (int rc = ph_get(ctx, LMAX, K, false, &tcache)) return rc;
+    const PhTables &t = *tcache;
     const size_t NP = (size_t)4 * LMAX * LMAX;
     DevBuf<double2> c, f;
     if (int rc = dev_alloc(&c.p, (size_t)K * LMAX * LMAX)) return rc;
@@ -2493,8 +2532,9 @@ int tp_sph_bessel(tp_ctx *ctx, int LMAX, int K, int forward, const double *coeff
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!coeff || !out || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
-    PhTables t;
-    if (int rc = ph_tables(ctx, LMAX, K, true, t)) return rc;
+    const PhTables *tcache = nullptr;
+    if (int rc = ph_get(ctx, LMAX, K, true, &tcache)) return rc;
+    const PhTables &t = *tcache;
     DevBuf<double2> a, b;
     if (int rc = dev_alloc(&a.p, (size_t)K * LMAX * LMAX)) return rc;
     if (int rc = dev_alloc(&b.p, (size_t)K * LMAX * LMAX)) return rc;
@@ -2509,24 +2549,46 @@ int tp_sph_bessel(tp_ctx *ctx, int LMAX, int K, int forward, const double *coeff
     return 0;
 }
 
-// Atoms (x, y, z, R, B) -> the density kernel's rows; R, B > 0.
-static int str_atoms(const double *atoms, int natoms, std::vector<StrAtom> &out)
+// Atoms (x, y, z, R, B) -> the density kernel's rows, sorted by |a|, and per
+// shell r_k = k rmax / K the range of atoms with ||a| - r_k| < sqrt 5 (plus
+// a margin for rounding): rows followed by the K ranges.  R, B > 0.
+static int str_atoms(const double *atoms, int natoms, int K, double rmax, std::vector<StrAtom> &out,
+                     std::vector<int2> &range)
 {
-    out.resize((size_t)natoms);
+    std::vector<double> rad((size_t)natoms);
+    std::vector<int> ord((size_t)natoms);
     for (int a = 0; a < natoms; ++a) {
         const double *r = atoms + (size_t)5 * a;
         if (!(r[3] > 0.0) || !(r[4] > 0.0)) return tp_fail(TP_E_ARG, "atom form factors need R > 0 and B > 0");
+        rad[(size_t)a] = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        ord[(size_t)a] = a;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return rad[(size_t)x] < rad[(size_t)y]; });
+    out.resize((size_t)natoms);
+    std::vector<double> srad((size_t)natoms);
+    for (int i = 0; i < natoms; ++i) {
+        const double *r = atoms + (size_t)5 * ord[(size_t)i];
         const double R = r[3], s = R * std::sqrt(2.0 * tph::kPi);
-        out[(size_t)a] = StrAtom{r[0], r[1], r[2], r[4] / (s * s * s), 2.0 * R * R};
+        out[(size_t)i] = StrAtom{r[0], r[1], r[2], r[4] / (s * s * s), 2.0 * R * R};
+        srad[(size_t)i] = rad[(size_t)ord[(size_t)i]];
+    }
+    const double reach = std::sqrt(5.0) + 1e-6;
+    range.resize((size_t)K);
+    for (int k = 0; k < K; ++k) {
+        const double rk = (k + 1) * (rmax / K);
+        const int lo = (int)(std::lower_bound(srad.begin(), srad.end(), rk - reach) - srad.begin());
+        const int hi = (int)(std::upper_bound(srad.begin(), srad.end(), rk + reach) - srad.begin());
+        range[(size_t)k] = make_int2(lo, std::max(lo, hi));
     }
     return 0;
 }
 
-static void str_density(tp_ctx *ctx, const StrAtom *d_atoms, int natoms, int LMAX, int K, double rmax, double2 *surf)
+static void str_density(tp_ctx *ctx, const StrAtom *d_atoms, const int2 *d_range, int LMAX, int K, double rmax,
+                        double2 *surf)
 {
     const int64_t n = (int64_t)K * 4 * LMAX * LMAX;
     hipLaunchKernelGGL(k_density_surface, dim3((unsigned)((n + kStrThreads - 1) / kStrThreads)), dim3(kStrThreads), 0,
-                       ctx->stream, d_atoms, natoms, LMAX, K, rmax / K, surf);
+                       ctx->stream, d_atoms, d_range, LMAX, K, rmax / K, surf);
 }
 
 int tp_density_surface(tp_ctx *ctx, const double *atoms, int natoms, int LMAX, int K, double rmax, double *surf)
@@ -2535,17 +2597,21 @@ int tp_density_surface(tp_ctx *ctx, const double *atoms, int natoms, int LMAX, i
     if ((!atoms && natoms > 0) || natoms < 0 || !surf || LMAX < 1 || K < 1 || !(rmax > 0.0))
         return tp_fail(TP_E_ARG, "bad argument");
     std::vector<StrAtom> h;
-    if (int rc = str_atoms(atoms, natoms, h)) return rc;
+    std::vector<int2> hr;
+    if (int rc = str_atoms(atoms, natoms, K, rmax, h, hr)) return rc;
     const size_t n = (size_t)K * 4 * LMAX * LMAX;
     DevBuf<StrAtom> da;
+    DevBuf<int2> dr;
     DevBuf<double2> f;
     if (natoms > 0)
         if (int rc = dev_alloc(&da.p, h.size())) return rc;
+    if (int rc = dev_alloc(&dr.p, hr.size())) return rc;
     if (int rc = dev_alloc(&f.p, n)) return rc;
     if (natoms > 0)
         TP_HIP(hipMemcpyAsync(da.p, h.data(), sizeof(StrAtom) * h.size(), hipMemcpyHostToDevice, ctx->stream));
+    TP_HIP(hipMemcpyAsync(dr.p, hr.data(), sizeof(int2) * hr.size(), hipMemcpyHostToDevice, ctx->stream));
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-    str_density(ctx, da.p, natoms, LMAX, K, rmax, f.p);
+    str_density(ctx, da.p, dr.p, LMAX, K, rmax, f.p);
     TP_HIP(hipGetLastError());
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     TP_HIP(hipMemcpyAsync(surf, f.p, sizeof(double2) * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -2562,8 +2628,9 @@ int tp_absolute_square(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!coeff || !out || ld < (int64_t)LMAX * LMAX) return tp_fail(TP_E_ARG, "bad argument");
-    PhTables t;
-    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const PhTables *tcache = nullptr;
+    if (int rc = ph_get(ctx, LMAX, K, false, &tcache)) return rc;
+    const PhTables &t = *tcache;
     const size_t nq = (size_t)K * LMAX * LMAX, n = (size_t)K * 4 * LMAX * LMAX;
     DevBuf<double2> c, f;
     if (int rc = dev_alloc(&c.p, nq)) return rc;
@@ -2589,22 +2656,27 @@ int tp_structure_from_atoms(tp_ctx *ctx, const double *atoms, int natoms, int LM
     if ((!atoms && natoms > 0) || natoms < 0 || !(rmax > 0.0) || ld < (int64_t)LMAX * LMAX)
         return tp_fail(TP_E_ARG, "bad argument");
     std::vector<StrAtom> h;
-    if (int rc = str_atoms(atoms, natoms, h)) return rc;
-    PhTables t;
-    if (int rc = ph_tables(ctx, LMAX, K, true, t)) return rc;
+    std::vector<int2> hr;
+    if (int rc = str_atoms(atoms, natoms, K, rmax, h, hr)) return rc;
+    const PhTables *tcache = nullptr;
+    if (int rc = ph_get(ctx, LMAX, K, true, &tcache)) return rc;
+    const PhTables &t = *tcache;
     const size_t nq = (size_t)K * LMAX * LMAX, n = (size_t)K * 4 * LMAX * LMAX;
     DevBuf<StrAtom> da;
+    DevBuf<int2> dr;
     DevBuf<double2> f, cd, cf, ci;
     if (natoms > 0)
         if (int rc = dev_alloc(&da.p, h.size())) return rc;
+    if (int rc = dev_alloc(&dr.p, hr.size())) return rc;
     if (int rc = dev_alloc(&f.p, n)) return rc;
     if (int rc = dev_alloc(&cd.p, nq)) return rc;
     if (int rc = dev_alloc(&cf.p, nq)) return rc;
     if (int rc = dev_alloc(&ci.p, nq)) return rc;
     hipStream_t st = ctx->stream;
     if (natoms > 0) TP_HIP(hipMemcpyAsync(da.p, h.data(), sizeof(StrAtom) * h.size(), hipMemcpyHostToDevice, st));
+    TP_HIP(hipMemcpyAsync(dr.p, hr.data(), sizeof(int2) * hr.size(), hipMemcpyHostToDevice, st));
     if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    str_density(ctx, da.p, natoms, LMAX, K, rmax, f.p);   // getDensity: sample
+    str_density(ctx, da.p, dr.p, LMAX, K, rmax, f.p);   // getDensity: sample
     ph_analysis(ctx, t, f.p, cd.p);                       //             and transform
     ph_bessel(ctx, t, cd.p, 1, cf.p);                     // forward
     ph_synth(ctx, t, cf.p, f.p);                          // absoluteSquare
@@ -2639,8 +2711,9 @@ int tp_sraar(tp_ctx *ctx, int LMAX, int K, const double *intensity, int64_t ld, 
     if (int rc = check_ctx(ctx)) return rc;
     if (!intensity || !coeff_out || ld < (int64_t)LMAX * LMAX || iterations < 0)
         return tp_fail(TP_E_ARG, "bad argument");
-    PhTables t;
-    if (int rc = ph_tables(ctx, LMAX, K, true, t)) return rc;
+    const PhTables *tcache = nullptr;
+    if (int rc = ph_get(ctx, LMAX, K, true, &tcache)) return rc;
+    const PhTables &t = *tcache;
     hipStream_t st = ctx->stream;
     const int NP = 4 * LMAX * LMAX;
     const int64_t n = (int64_t)K * NP;
@@ -2723,8 +2796,9 @@ int tp_sraar(tp_ctx *ctx, int LMAX, int K, const double *intensity, int64_t ld, 
 static int pic_surfaces(tp_ctx *ctx, int LMAX, int K, const double *coeff, int64_t ld, DevBuf<double2> &surf,
                         double &maxI)
 {
-    PhTables t;
-    if (int rc = ph_tables(ctx, LMAX, K, false, t)) return rc;
+    const PhTables *tcache = nullptr;
+    if (int rc = ph_get(ctx, LMAX, K, false, &tcache)) return rc;
+    const PhTables &t = *tcache;
     const size_t NP = (size_t)4 * LMAX * LMAX;
     DevBuf<double2> c;
     if (int rc = dev_alloc(&c.p, (size_t)K * LMAX * LMAX)) return rc;

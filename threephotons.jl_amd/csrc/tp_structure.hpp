@@ -23,10 +23,14 @@ struct StrAtom {
 
 // surf[k][j S + i] = sum over atoms with |p - a|^2 < 5 of
 // amp exp(-|p - a|^2 / w), p the grid point (phi_i, theta_j) of shell k + 1;
-// one thread per grid point, atoms in LDS chunks, summed in atom order.
-__global__ __launch_bounds__(kStrThreads) void k_density_surface(const StrAtom *__restrict__ atoms, int natoms,
-                                                                 int LMAX, int K, double dr,
-                                                                 double2 *__restrict__ surf)
+// one thread per grid point, atoms in LDS chunks.  The atoms come sorted by
+// |a|, and range[k] = [first, end) holds every atom with ||a| - r_k| < sqrt 5
+// (the others are farther than sqrt 5 from the whole shell, since
+// |p - a| >= |r_k - |a||); a workgroup loops over the union of its shells'
+// ranges and sums in that order.
+__global__ __launch_bounds__(kStrThreads) void k_density_surface(const StrAtom *__restrict__ atoms,
+                                                                 const int2 *__restrict__ range, int LMAX, int K,
+                                                                 double dr, double2 *__restrict__ surf)
 {
     __shared__ StrAtom sa[kStrAtomChunk];
     const int S = 2 * LMAX, NP = S * S;
@@ -44,9 +48,12 @@ __global__ __launch_bounds__(kStrThreads) void k_density_surface(const StrAtom *
         y = r * sin(phi) * st;
         z = r * cos(theta);
     }
+    const int64_t t0 = (int64_t)blockIdx.x * kStrThreads;
+    const int kf = (int)(t0 / NP), kl = (int)min((int64_t)K - 1, (t0 + kStrThreads - 1) / NP);
+    const int abeg = range[kf].x, aend = range[kl].y;
     double rho = 0.0;
-    for (int a0 = 0; a0 < natoms; a0 += kStrAtomChunk) {
-        const int na = min(kStrAtomChunk, natoms - a0);
+    for (int a0 = abeg; a0 < aend; a0 += kStrAtomChunk) {
+        const int na = min(kStrAtomChunk, aend - a0);
         __syncthreads();
         for (int a = threadIdx.x; a < na; a += kStrThreads) sa[a] = atoms[a0 + a];
         __syncthreads();
