@@ -679,8 +679,8 @@ def main():
         gradient_profile(args.grad_profile)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the k-triple sharding "
-              "design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (sharding over k-triple "
+              "columns is designed, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import tp_amd as tp
