@@ -100,3 +100,26 @@ def test_sh_round_trip_above_lmax_32(tp):
         err = np.abs(back - c).max() / np.abs(c).max()
         say(f"  LMAX={LMAX} round trip {err:.2e}")
         assert err <= 1e-11
+
+
+def test_shell_correlations(tp, so):
+    """ISC / FSC (spherical_harmonics.jl:400-423): a structure's own density
+    against its intensity and Fourier volume gives 1 on every shell; two
+    unrelated volumes give the oracle's surface correlations."""
+    from oracle.c2_oracle import surface
+    from tp_amd import SphericalHarmonicsVolume, structure
+    LMAX, K, rmax = 8, 6, 12.0
+    d, f, i = structure.createSphericalHarmonicsStructure(CRAMBIN, LMAX, K, rmax)
+    assert np.allclose(structure.ISC(d, i), 1.0, atol=1e-12)
+    assert np.allclose(structure.FSC(d, f), 1.0, atol=1e-12)
+    a = random_real_coeff(K, LMAX - 1, LMAX, 5, even=False)
+    b = random_real_coeff(K, LMAX - 1, LMAX, 6, even=False)
+    va, vb = SphericalHarmonicsVolume(a, LMAX, K, 1.0), SphericalHarmonicsVolume(b, LMAX, K, 1.0)
+    isc = structure.shell_correlation_ISC(va, vb, range(2, K + 1))
+    fsc = structure.shell_correlation_FSC(va, vb)
+    for k in range(1, K + 1):
+        sa, sb = surface(a[k - 1], LMAX).ravel(), surface(b[k - 1], LMAX).ravel()
+        if k >= 2:
+            assert isc[k - 2] == pytest.approx(np.corrcoef(sa.real, sb.real)[0, 1], abs=1e-12)
+        want = abs(np.vdot(sb, sa)) / (np.linalg.norm(sa) * np.linalg.norm(sb))
+        assert fsc[k - 1] == pytest.approx(want, abs=1e-12)

@@ -38,3 +38,15 @@ def test_atom_table_form_factors_and_type_one():
     assert np.array_equal(tab[:, 3:], [atomfactorlist["C"], atomfactorlist["S"], atomfactorlist["H"]])
     with pytest.raises(KeyError):
         atom_table([mk("X")])
+
+
+def test_correlation_rules():
+    """cor_FSC: |sum a conj b| / (|a| |b|), 0 at zero norm, invariant under a
+    complex phase; cor_ISC: Pearson of the real parts (utilities.jl:45-59)."""
+    from tp_amd.structure import cor_FSC, cor_ISC
+    rng = np.random.default_rng(1)
+    a = rng.normal(size=50) + 1j * rng.normal(size=50)
+    assert cor_FSC(a, np.zeros(50, complex)) == 0.0
+    assert cor_FSC(a, 3.0 * np.exp(0.7j) * a) == pytest.approx(1.0, abs=1e-14)
+    assert cor_ISC(a, 2.0 * a + 5.0) == pytest.approx(1.0, abs=1e-14)
+    assert cor_ISC(a, -a) == pytest.approx(-1.0, abs=1e-14)

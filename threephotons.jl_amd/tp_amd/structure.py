@@ -7,6 +7,8 @@ Fourier transform and intensity (structure.jl:1-134) on the device.
   absoluteSquare(volume)                      spherical_harmonics.jl:393-397 (tp_absolute_square)
   createSphericalHarmonicsStructure(pdb_or_atoms, LMAX, KMAX, rmax)
                                               structure.jl:124-134 (tp_structure_from_atoms)
+  shell_correlation / _ISC / _FSC, ISC, FSC    spherical_harmonics.jl:400-423: per-shell
+                                              correlations of surfaces synthesised on the device
 
 The atoms' form factors (R, B) come from atomfactorlist (structure.jl:8-17).
 As in getDensity_cartesian, atoms of type "1" add no density; any other type
@@ -131,3 +133,48 @@ def createSphericalHarmonicsStructure(pdb_or_atoms, LMAX, KMAX, rmax, ctx=None):
     q = np.pi / (rmax / KMAX)
     return (SphericalHarmonicsVolume(d, LMAX, KMAX, rmax), SphericalHarmonicsVolume(f, LMAX, KMAX, q),
             SphericalHarmonicsVolume(i, LMAX, KMAX, q))
+
+
+# ---- shell correlations (spherical_harmonics.jl:400-423, utilities.jl:45-59)
+
+def cor_ISC(a, b):
+    """Pearson correlation of the real parts (utilities.jl:45-47)."""
+    return float(np.corrcoef(np.real(a), np.real(b))[0, 1])
+
+
+def cor_FSC(a, b):
+    """|sum a conj(b)| / (|a| |b|), 0 when either norm is at most eps (utilities.jl:50-59)."""
+    na, nb = np.linalg.norm(a), np.linalg.norm(b)
+    if na > np.finfo(float).eps and nb > np.finfo(float).eps:
+        return float(abs(np.sum(a * np.conj(b)) / (na * nb)))
+    return 0.0
+
+
+def shell_correlation(volume1, volume2, K_range=None, cor_method=cor_FSC, ctx=None):
+    """Per-shell correlation of the two volumes' surfaces (synthesised on the device)."""
+    from .phases import getSurfaceVolume
+    s1, s2 = getSurfaceVolume(volume1, ctx), getSurfaceVolume(volume2, ctx)
+    if s1.shape != s2.shape:
+        raise ValueError("volumes of different shapes")
+    ks = range(1, s1.shape[0] + 1) if K_range is None else K_range
+    return np.array([cor_method(s1[k - 1], s2[k - 1]) for k in ks])
+
+
+def shell_correlation_ISC(volume1, volume2, K_range=None, ctx=None):
+    return shell_correlation(volume1, volume2, K_range, cor_ISC, ctx)
+
+
+def shell_correlation_FSC(volume1, volume2, K_range=None, ctx=None):
+    return shell_correlation(volume1, volume2, K_range, cor_FSC, ctx)
+
+
+def ISC(density, intensity, K_range=None, ctx=None):
+    """Intensity shell correlation of a density against an intensity (spherical_harmonics.jl:417-419)."""
+    from .phases import forward
+    return shell_correlation_ISC(intensity, absoluteSquare(forward(density, ctx), ctx), K_range, ctx)
+
+
+def FSC(density, fourier, K_range=None, ctx=None):
+    """Fourier shell correlation of a density against a Fourier volume (spherical_harmonics.jl:421-423)."""
+    from .phases import forward
+    return shell_correlation_FSC(fourier, forward(density, ctx), K_range, ctx)
