@@ -471,6 +471,31 @@ def generate_leg(tp, truth, npic):
     return out
 
 
+def ewald_leg(cfg, tp, start, c3ref, reps):
+    """The plain FP64 evaluation on the curved Ewald sphere: the full-size
+    problem (L = 18, K = 26, N = 32, the same start structure and histogram)
+    with a basis at lambda = 2.5 (the reference's own GPU test value) and
+    dq = pi / 38; synchronous c3 + energy evaluations from the host, stage
+    times with HIP events."""
+    ctx = tp.HIP_init()
+    lam, dq = 2.5, np.pi / 38
+    b = tp.HIP_calculate_basis(cfg["L"], cfg["LMAX"], cfg["N"], cfg["K"], lam, dq)
+    tp.energy(start, b, c3ref)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        E = tp.energy(start, b, c3ref)
+    el = time.perf_counter() - t0
+    ctx.set_timing(True)
+    tp.energy(start, b, c3ref)
+    st = ctx.last_timings()
+    ctx.set_timing(False)
+    out = {"workload": f"L={cfg['L']}, K={cfg['K']}, N={cfg['N']}, lambda={lam}, dq=pi/38", "evals_per_s": reps / el,
+           "ms_per_eval": 1e3 * el / reps, "energy": E,
+           "stage_ms": {"triple": st[0], "glam_G": st[1], "c3_energy": st[2], "final": st[3]}}
+    log(f"curved Ewald sphere: {reps / el:.1f} evals/s ({1e3 * el / reps:.3f} ms), stage ms {np.round(st, 4)}")
+    return out
+
+
 def structure_leg(tp, reps):
     """createSphericalHarmonicsStructure (structure.jl:124-134) of crambin
     (the ATOM records the reference's structure test loads, 670 atoms) at
@@ -666,6 +691,7 @@ def main():
     ap.add_argument("--sraar-iters", type=int, default=1001, help="iterations of the phase-retrieval leg (0: skip)")
     ap.add_argument("--gen-pictures", type=int, default=65536, help="pictures of the histogram-generation leg (0: skip)")
     ap.add_argument("--structure-reps", type=int, default=5, help="timed calls of the structure leg (0: skip)")
+    ap.add_argument("--ewald-reps", type=int, default=50, help="timed evaluations at lambda = 2.5 (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -761,6 +787,7 @@ def main():
     phase = phase_leg(tp, truth, args.sraar_iters) if args.sraar_iters > 0 else None
     gen = generate_leg(tp, truth, args.gen_pictures) if args.gen_pictures > 0 else None
     struct = structure_leg(tp, args.structure_reps) if args.structure_reps > 0 else None
+    ewald = ewald_leg(cfg, tp, start, c3ref, args.ewald_reps) if args.ewald_reps > 0 else None
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:
@@ -830,6 +857,8 @@ def main():
         out["generated_histograms"] = gen
     if struct is not None:
         out["structure"] = struct
+    if ewald is not None:
+        out["curved_ewald"] = ewald
     if not args.no_cpu:
         nthreads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)
