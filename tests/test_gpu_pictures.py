@@ -133,3 +133,16 @@ def test_photon_variance_and_beamstop_shots_match_oracle(tp, pic):
         want = pic.beamstop_shot(qcut, 300, 0.1, 2, i)
         assert len(shots[i]) == len(want) > 0
         assert np.abs(shots[i] - want).max() <= 1e-15
+
+
+def test_empty_pictures(tp):
+    """No incident photons, or no pictures: empty results, no error."""
+    from tp_amd import pictures
+    vol = _volume(tp, 6, 8, 1.0, 3)
+    got, rot = pictures.pointsPerOrientation(vol, 1.0, 3, photons_per_image=0, seed=2)
+    assert all(len(p) == 0 for p in got) and rot.shape == (3, 3, 3)
+    got, _ = pictures.pointsPerOrientation(vol, 1.0, 0, photons_per_image=10, seed=2)
+    assert got == []
+    h = tp.PhotonHistogram(8, 8, 6, 1.0 / 8)
+    assert pictures.generate_histograms(h, vol, 1.0, 4, photons_per_image=0, seed=2) == 0
+    assert h.read()["n1"].sum() == 0

@@ -123,3 +123,17 @@ def test_shell_correlations(tp, so):
             assert isc[k - 2] == pytest.approx(np.corrcoef(sa.real, sb.real)[0, 1], abs=1e-12)
         want = abs(np.vdot(sb, sa)) / (np.linalg.norm(sa) * np.linalg.norm(sb))
         assert fsc[k - 1] == pytest.approx(want, abs=1e-12)
+
+
+def test_structure_edge_cases(tp):
+    """No atoms: zero density, Fourier and intensity volumes; a non-positive
+    form factor is an argument error, not a silent NaN."""
+    from tp_amd import structure
+    d, f, i = structure.createSphericalHarmonicsStructure(np.zeros((0, 5)), 4, 3, 5.0)
+    assert not d.coeff.any() and not f.coeff.any() and not i.coeff.any()
+    assert not structure.getDensitySurfaces(np.zeros((0, 5)), 4, 3, 5.0).any()
+    with pytest.raises(RuntimeError, match="R > 0"):
+        structure.getDensitySurfaces(np.array([[0.0, 0.0, 0.0, 0.0, 6.0]]), 4, 3, 5.0)
+    # one atom farther than sqrt 5 from every shell point adds nothing
+    far = structure.getDensitySurfaces(np.array([[0.0, 0.0, 40.0, 0.7, 6.0]]), 4, 3, 5.0)
+    assert not far.any()
