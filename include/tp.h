@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TP_ABI_VERSION 1
+#define TP_ABI_VERSION 2
 
 #define TP_OK 0
 #define TP_E_ARG (-1)          /* invalid argument or shape */
@@ -112,7 +112,10 @@ int tp_c3(tp_ctx *ctx, const tp_basis *b, const double *coeff, int64_t ld, doubl
  * K3_range = k3lo:k3hi walk of energy() (correlations.jl:211-217) are
  * normalised (q / sum|q|) and kept on the device.  Kr is the shell extent of
  * the c3ref array.  Mirrors the c3ref compression of new_energy
- * (cuda.jl:127-139). */
+ * (cuda.jl:127-139).  A visited slice with sum|q| = 0 or a non-finite entry
+ * has no normalised form (the reference's energy() would produce NaN): the
+ * call fails with TP_E_ARG.  Raw counts go through tp_load_histograms
+ * (loadHistograms' clamp) first. */
 int tp_ref_create(tp_ctx *ctx, const tp_basis *b, const double *c3ref, int Kr, int k3lo, int k3hi,
                   tp_ref **out);
 void tp_ref_destroy(tp_ref *r);
@@ -160,6 +163,68 @@ typedef struct tp_anneal_params {
 
 int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
               double *E_trace, double *Enew_trace, int32_t *acc_trace, double *scalars);
+
+/* The same annealing in resumable pieces, with the reference's stopping rule.
+ * tp_anneal_status is rotate_all_at_once's `state` dict for one resolution L
+ * (determination.jl:281-338): step = state["i"], E, T, stepsizes[l/2 - 1] =
+ * state["stepsizes"][l] (l = 2, 4, ..., L).  The step-i proposal and
+ * acceptance draw from Philox4x32-10 keyed by (seed, i), so a run continued
+ * from a saved status and structure repeats the uninterrupted run bit for bit. */
+#define TP_MAX_LBLOCKS 32
+typedef struct tp_anneal_status {
+    int64_t step;                          /* steps taken so far */
+    double E, T;                           /* current energy and temperature */
+    double E0, T0;                         /* at the start of the run */
+    double stepsizes[TP_MAX_LBLOCKS];      /* index l/2 - 1 */
+    int32_t nl;                            /* L/2 blocks in use */
+    int32_t naccepted;                     /* accepted steps so far */
+    int32_t stopped;                       /* 1 once stepsizes[nl-1] <= min_stepsize */
+    int32_t reserved;
+} tp_anneal_status;
+/* A new run: initial energy, stepsize_l = initial_stepsize (l - 1) and T0 from
+ * p->ntemp proposals (calculate_temperature, determination.jl:244-261, :291-312);
+ * step = 0.  p->nsteps is not used. */
+int tp_anneal_begin(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
+                    tp_anneal_status *st);
+/* Up to max_steps further steps from st (updated in place), all on the
+ * device.  min_stepsize > 0 is the loop condition of rotate_all_at_once,
+ * `while stepsizes[L] > minstepsize` (determination.jl:1 uses 1.5 degrees,
+ * :322): no step is taken once the step size of the largest l is at or below
+ * it, and st->stopped says so.  Traces (may be NULL) get one entry per step
+ * taken; *steps_done (may be NULL) = their number. */
+int tp_anneal_steps(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
+                    tp_anneal_status *st, int max_steps, double min_stepsize, double *E_trace, double *Enew_trace,
+                    int32_t *acc_trace, int32_t *steps_done);
+
+/* Checkpoint of an annealing run: the counterpart of saveState / the
+ * regular_action every 500 steps and the resume of rotation_search
+ * (determination.jl:154-229, 264-278; utilities.jl:278-287), in a versioned
+ * binary file of this library's own (the reference serialises Julia dicts).
+ * It holds the run's parameters, its status, the basis shape it ran on and
+ * nshell shells of ld complex coefficients; the file is written to path.tmp
+ * and renamed, and carries a checksum that tp_checkpoint_load verifies. */
+typedef struct tp_checkpoint_info {
+    tp_anneal_params params;
+    tp_anneal_status status;
+    double min_stepsize;
+    double lambda, dq;
+    int32_t L, LMAX, N, K, k3lo, k3hi;
+    int32_t nshell;                        /* shells stored */
+    int32_t ld;                            /* complex entries per shell stored (>= LMAX^2) */
+} tp_checkpoint_info;
+int tp_checkpoint_save(const char *path, const tp_checkpoint_info *info, const double *coeff);
+/* Reads the header into *info; with coeff != NULL also the coefficients, which
+ * need room for info->nshell * info->ld complex values (at most cap_doubles
+ * doubles).  TP_E_ARG for a missing, truncated, foreign or corrupted file. */
+int tp_checkpoint_load(const char *path, tp_checkpoint_info *info, double *coeff, int64_t cap_doubles);
+
+/* loadHistograms (datagen.jl:432-475) for the triplet histogram: an
+ * NA x NB x Kf x Kf x Kf c3 histogram (column-major, as read from a
+ * histogram file) becomes c3_out[N, 2N, K3, K3, K3]: an old N x N histogram
+ * (NA == NB) is doubled along beta for k1 >= k2 >= k3 (:445-458), every entry
+ * is clamped to >= 1e-30 (:460-462), the first K3 shells are kept and divided
+ * by their total (:465-468).  Host code (no GPU call).  K3 <= Kf. */
+int tp_load_histograms(const double *c3_full, int NA, int NB, int Kf, int K3, double *c3_out);
 
 /* ---- batches: many evaluations in one graph launch ------------------------- */
 
@@ -281,8 +346,10 @@ int tp_pictures(tp_ctx *ctx, int LMAX, int K, double rmax, const double *coeff, 
                 int npictures, int first_picture, double *photons, int32_t *counts, double *rotations);
 /* generateHistogram's inner loop (datagen.jl:391-415) on the device: the
  * pictures of tp_pictures (capacity <= 512) histogrammed into h as
- * tp_hist_add does, without leaving the device; overflow (may be NULL) = the
- * number of accepted photons dropped beyond capacity. */
+ * tp_hist_add does, without leaving the device.  A picture that accepted more
+ * than capacity photons cannot be histogrammed whole and is skipped (none of
+ * its photons, pairs or triplets is counted); overflow (may be NULL) = the
+ * number of pictures skipped. */
 int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double *coeff, int64_t ld,
                           const tp_picture_params *p, int npictures, int first_picture, int64_t *overflow);
 

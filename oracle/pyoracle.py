@@ -81,6 +81,12 @@ def lib():
         L.or_anneal.restype = C.c_int
         L.or_anneal.argtypes = [C.c_void_p, d_p, d_p, C.c_int64, C.c_int, C.c_int, C.c_double, C.c_double,
                                 C.c_double, C.c_double, C.c_uint64, C.c_int, d_p, d_p, i32_p, d_p, C.c_int]
+        L.or_anneal_ex.restype = C.c_int
+        L.or_anneal_ex.argtypes = [C.c_void_p, d_p, d_p, C.c_int64, C.c_int, C.c_int, d_p, d_p, C.c_int,
+                                   C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_int,
+                                   d_p, d_p, i32_p, C.c_int]
+        L.or_load_c3.restype = None
+        L.or_load_c3.argtypes = [d_p, C.c_int, C.c_int, C.c_int, C.c_int, d_p]
         L.or_d_work.restype = C.c_double
         L.or_d_work.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double]
         L.or_rotation_matrix.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, d_p]
@@ -290,6 +296,56 @@ def anneal(basis, c3ref, coeff, nsteps, initial_stepsize, stepsizefactor, initia
     if rc != 0:
         raise ValueError("anneal needs lambda = 0")
     return dict(E=E, E_new=En, accepted=acc, E0=sc[0], T0=sc[1], E_final=sc[2], T_final=sc[3], coeff=c)
+
+
+class AnnealState:
+    """rotate_all_at_once's state for the oracle's resumable driver (or_anneal_ex)."""
+
+    def __init__(self, basis, c3ref, coeff, initial_stepsize, stepsizefactor, initial_temperature_factor,
+                 temperature_decay, seed, ntemp=50, nthreads=0):
+        self.basis = basis
+        self.cref = np.ascontiguousarray(np.asarray(c3ref, dtype=np.float64).ravel(order="F"))
+        self.coeff = np.array(coeff, dtype=np.complex128, copy=True)
+        self.args = (initial_stepsize, stepsizefactor, initial_temperature_factor, temperature_decay, seed, ntemp)
+        self.st = np.zeros(6)
+        self.stepsizes = np.zeros(max(basis.L // 2, 1))
+        self.nthreads = nthreads
+        self._run(1, 0, 0.0)
+
+    def _run(self, begin, n, min_stepsize):
+        c = self.coeff
+        E, En = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        acc = np.zeros(max(n, 1), np.int32)
+        k = lib().or_anneal_ex(self.basis.h, self.cref, c.view(np.float64).reshape(c.shape[0], -1), c.shape[1],
+                               c.shape[0], begin, self.st, self.stepsizes, n, min_stepsize, *self.args, E, En, acc,
+                               self.nthreads)
+        if k < 0:
+            raise ValueError("anneal needs lambda = 0")
+        return dict(E=E[:k], E_new=En[:k], accepted=acc[:k])
+
+    def steps(self, n, min_stepsize=0.0):
+        return self._run(0, n, min_stepsize)
+
+    @property
+    def step(self):
+        return int(self.st[0])
+
+    @property
+    def E(self):
+        return self.st[1]
+
+    @property
+    def T(self):
+        return self.st[2]
+
+
+def load_c3(c3_full, K3):
+    """loadHistograms' triplet histogram (datagen.jl:432-475), see or_load_c3."""
+    a = np.asarray(c3_full, dtype=np.float64)
+    NA, NB, Kf = a.shape[0], a.shape[1], a.shape[2]
+    out = np.zeros(2 * NA * NA * K3 ** 3)
+    lib().or_load_c3(np.ascontiguousarray(a.ravel(order="F")), NA, NB, Kf, K3, out)
+    return out.reshape((NA, 2 * NA, K3, K3, K3), order="F")
 
 
 def d_work(l, m1, m2, beta):

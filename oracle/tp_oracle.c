@@ -1052,6 +1052,97 @@ int or_anneal(const or_basis *b, const double *c3ref, double *coeff /* in/out, K
     return 0;
 }
 
+/* rotate_all_at_once in resumable pieces (determination.jl:281-338):        */
+/* begin != 0 starts a run (E, T from calculate_temperature, step sizes      */
+/* initial_stepsize (l - 1), step 0); otherwise st = {step, E, T, naccepted} */
+/* and stepsizes[] continue a saved run.  Steps until max_steps are done or, */
+/* with min_stepsize > 0, `while stepsizes[L] > minstepsize` (:1, :322)      */
+/* fails before a step.  Returns the steps taken.                            */
+int or_anneal_ex(const or_basis *b, const double *c3ref, double *coeff, int64_t ld, int KMAX, int begin,
+                 double *st /* in/out: step, E, T, naccepted, E0, T0 */, double *stepsizes /* in/out [L/2] */,
+                 int max_steps, double min_stepsize, double initial_stepsize, double stepsizefactor,
+                 double initial_temperature_factor, double temperature_decay, uint64_t seed, int ntemp,
+                 double *E_trace, double *Enew_trace, int *acc_trace, int nthreads)
+{
+    if (b->lambda != 0.0) return -1;
+    int L = b->L, nl = L / 2;
+    double *c3buf = (double *)malloc(sizeof(double) * b->R * b->klength);
+    double *step = (double *)malloc(sizeof(double) * 2 * ld * KMAX);
+    if (begin) {
+        for (int l = 2; l <= L; l += 2) stepsizes[l / 2 - 1] = initial_stepsize * (l - 1);   /* :295 */
+        double E = or_energy_of(b, coeff, ld, c3ref, c3buf, nthreads);
+        double T = 0.0;
+        if (initial_temperature_factor > 0.0 && ntemp > 1) {
+            double s1 = 0, s2 = 0;
+            double *es = (double *)malloc(sizeof(double) * ntemp);
+            for (int i = 0; i < ntemp; ++i) {
+                or_mc_propose(seed, (uint32_t)i, OR_TAG_TEMP, L, stepsizes, coeff, step, ld, b->LMAX, KMAX);
+                es[i] = or_energy_of(b, step, ld, c3ref, c3buf, nthreads);
+                s1 += es[i];
+            }
+            double mean = s1 / ntemp;
+            for (int i = 0; i < ntemp; ++i) s2 += (es[i] - mean) * (es[i] - mean);
+            T = sqrt(s2 / (ntemp - 1)) * initial_temperature_factor;
+            free(es);
+        }
+        st[0] = 0; st[1] = E; st[2] = T; st[3] = 0; st[4] = E; st[5] = T;
+    }
+    int64_t i0 = (int64_t)st[0];
+    double E = st[1], T = st[2];
+    int nacc = (int)st[3], taken = 0;
+    for (int j = 0; j < max_steps; ++j) {
+        if (min_stepsize > 0.0 && !(stepsizes[nl - 1] > min_stepsize)) break;
+        uint32_t i = (uint32_t)(i0 + j + 1);
+        T *= temperature_decay;
+        or_mc_propose(seed, i, OR_TAG_MOVE, L, stepsizes, coeff, step, ld, b->LMAX, KMAX);
+        double En = or_energy_of(b, step, ld, c3ref, c3buf, nthreads);
+        double dE = En - E;
+        double ap = exp(-dE / T);
+        double u = or_uniform(seed, i, OR_TAG_ACC);
+        int acc = 0;
+        if (dE < 0.0 || (dE > 0.0 && u < ap)) {
+            acc = 1;
+            for (int q = 0; q < nl; ++q) stepsizes[q] *= stepsizefactor;
+            E = En;
+            memcpy(coeff, step, sizeof(double) * 2 * ld * KMAX);
+        } else {
+            for (int q = 0; q < nl; ++q) stepsizes[q] /= stepsizefactor;
+        }
+        nacc += acc;
+        E_trace[j] = E; Enew_trace[j] = En; acc_trace[j] = acc;
+        ++taken;
+    }
+    st[0] = (double)(i0 + taken); st[1] = E; st[2] = T; st[3] = nacc;
+    free(c3buf); free(step);
+    return taken;
+}
+
+/* loadHistograms for the triplet histogram (datagen.jl:432-475), as a plain */
+/* restatement: NA x NB x Kf^3 (column-major) -> N x 2N x K3^3; doubling of  */
+/* an N x N histogram for k1 >= k2 >= k3 (:445-458), clamp to >= 1e-30       */
+/* (:460-462), cut to K3 shells, division by the total of |entries|          */
+/* (:465-468).                                                               */
+void or_load_c3(const double *c3_full, int NA, int NB, int Kf, int K3, double *out)
+{
+    int64_t N = NA, R = 2 * N * N, Rf = (int64_t)NA * NB;
+    for (int k1 = 0; k1 < K3; ++k1)
+        for (int k2 = 0; k2 < K3; ++k2)
+            for (int k3 = 0; k3 < K3; ++k3)
+                for (int64_t b = 0; b < 2 * N; ++b)
+                    for (int64_t a = 0; a < N; ++a) {
+                        double v = 0.0;
+                        if (NB == 2 * NA)
+                            v = c3_full[a + N * b + R * (k3 + (int64_t)Kf * (k2 + (int64_t)Kf * k1))];
+                        else if (k1 >= k2 && k2 >= k3)
+                            v = c3_full[a + N * (b % N) + Rf * (k3 + (int64_t)Kf * (k2 + (int64_t)Kf * k1))];
+                        out[a + N * b + R * (k3 + (int64_t)K3 * (k2 + (int64_t)K3 * k1))] = v < 1e-30 ? 1e-30 : v;
+                    }
+    int64_t n = R * K3 * K3 * K3;
+    double t = 0.0;
+    for (int64_t i = 0; i < n; ++i) t += fabs(out[i]);
+    for (int64_t i = 0; i < n; ++i) out[i] /= t;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Wigner-D: d_work / Dlms / rotationMatrix (spherical_harmonics.jl:572-595),*/
 /* explicit sum with factorials (long double in place of BigInt).            */

@@ -80,7 +80,11 @@ def test_capacity_overflow_is_reported(tp):
     with pytest.raises(RuntimeError, match="capacity"):
         pictures.pointsPerOrientation(vol, 1.0, 3, photons_per_image=800, capacity=2, seed=1)
     h = tp.PhotonHistogram(8, 8, 6, 1.0 / 8)
-    assert pictures.generate_histograms(h, vol, 1.0, 3, photons_per_image=800, capacity=2, seed=1) > 0
+    with pytest.raises(RuntimeError, match="capacity"):
+        pictures.generate_histograms(h, vol, 1.0, 3, photons_per_image=800, capacity=2, seed=1)
+    # the overflowed pictures are left out whole: no photon, pair or triplet of them is counted
+    a = h.read()
+    assert a["n1"].sum() == 0 and a["n2"].sum() == 0 and a["n3"].sum() == 0
 
 
 def test_particles_and_noise_match_oracle(tp, pic):

@@ -203,7 +203,8 @@ struct tp_state {
     double2 *d_prop = nullptr;   // [nshell][n_c] proposal buffer
     double *d_scal = nullptr;    // [8]: E, T
     double *d_step = nullptr;    // [L/2] step sizes
-    int *d_sel = nullptr;        // [1] which buffer is current during tp_anneal
+    int *d_sel = nullptr;        // [4] which buffer is current during tp_anneal, then the McArgs
+                                 // control block {stopped, last step, accepted steps}
 };
 
 // ---------------------------------------------------------------------------
@@ -238,12 +239,27 @@ static int upload_coeff(tp_ctx *ctx, tp_basis *b, const double *coeff, int64_t l
     return 0;
 }
 
+// Per-evaluation device scratch of enqueue_eval (by default the basis's own
+// buffers).
+struct EvalScratch {
+    double *As, *G, *part, *blk, *Ainv, *shsc;
+    unsigned *counter;
+    uint16_t *Ahi, *Alo;
+};
+
+static EvalScratch basis_scratch(const tp_basis *b)
+{
+    return EvalScratch{b->d_As, b->d_G, b->d_part, b->d_blk, b->d_Ainv, b->d_shsc, b->d_counter, b->d_Ahi, b->d_Alo};
+}
+
 // Enqueue one evaluation on coefficients (b0, b1, sel, flip): As, G, c3 (+energy).
 // With r == nullptr the raw c3 goes to c3out; otherwise the energy is formed
 // and k_final applies `mc`.
 static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const double2 *b0, const double2 *b1,
-                        const int *sel, int flip, double *c3out, double *Eout, const McArgs &mc, bool force64 = false)
+                        const int *sel, int flip, double *c3out, double *Eout, const McArgs &mc, bool force64 = false,
+                        const EvalScratch *scr = nullptr)
 {
+    const EvalScratch sc = scr ? *scr : basis_scratch(b);
     const bool timing = ctx->timing;
     hipStream_t st = ctx->stream;
     const int prec = force64 ? TP_PREC_FP64 : b->prec;
@@ -251,18 +267,18 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     if (b->use_mfma && prec != TP_PREC_FP64) {
         // mixed precision: compensated FP16 triple products
         hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)b->K), dim3(256), 0, st, b0, b1, sel, flip, b->n_c, b->L,
-                           b->d_shsc);
+                           sc.shsc);
         hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
-                           b->d_shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
-                           b->EmaxLt8, b->Prows, b->d_As, b->kl_pad);
+                           sc.shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
+                           b->EmaxLt8, b->Prows, sc.As, b->kl_pad);
     } else if (b->use_mfma) {
         hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(1024), b->mfma_lds, st, b0, b1, sel, flip,
                            b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_work, b->Lw, b->EmaxLt,
-                           b->Prows, b->d_As, b->kl_pad);
+                           b->Prows, sc.As, b->kl_pad);
     } else {
         hipLaunchKernelGGL(k_triple, dim3((unsigned)b->nchunks), dim3(1024), b->triple_lds, st, b0, b1, sel, flip,
                            b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_chunks, b->d_pairs,
-                           b->npairs, b->PB, b->Gmax, b->Emax, b->d_As, b->kl_pad);
+                           b->npairs, b->PB, b->Gmax, b->Emax, sc.As, b->kl_pad);
     }
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
@@ -274,19 +290,19 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
         const dim3 gs((unsigned)(b->kl_pad / 64), (unsigned)(b->ldk / 64));
         if (prec == TP_PREC_FP16X3) {
-            hipLaunchKernelGGL(k_split_as<1>, gs, dim3(256), 0, st, b->d_As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
-                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, b->d_Ahi, b->d_Alo, b->d_Ainv);
+            hipLaunchKernelGGL(k_split_as<1>, gs, dim3(256), 0, st, sc.As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
+                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, sc.Ahi, sc.Alo, sc.Ainv);
         } else {
-            hipLaunchKernelGGL(k_split_as<2>, gs, dim3(256), 0, st, b->d_As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
-                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, b->d_Ahi, nullptr, b->d_Ainv);
+            hipLaunchKernelGGL(k_split_as<2>, gs, dim3(256), 0, st, sc.As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
+                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, sc.Ahi, nullptr, sc.Ainv);
         }
         TP_HIP(hipGetLastError());
         if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
         const uint16_t *Fh = prec == TP_PREC_FP16X3 ? b->d_Fhi : b->d_Fbf;
 #define TP_G16(MODE, OUT, EN, OUTP)                                                                                \
     hipLaunchKernelGGL((k_gemm16<H_BM, H_BN, MODE, OUT, EN, 0>), grid16, dim3(256), 0, st, Fh, b->d_Flo,          \
-                       (int64_t)b->ldk, b->d_Ahi, b->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, b->d_Ainv, (int)b->R,  \
-                       ncols, b->n_rt16, b->n_ct16, OUTP, b->R, qh, b->d_part, b->kl_pad, nullptr, nullptr)
+                       (int64_t)b->ldk, sc.Ahi, sc.Alo, (int64_t)b->ldk, b->ldk, b->F_inv, sc.Ainv, (int)b->R,  \
+                       ncols, b->n_rt16, b->n_ct16, OUTP, b->R, qh, sc.part, b->kl_pad, nullptr, nullptr)
         if (prec == TP_PREC_FP16X3) {
             if (r && c3out) TP_G16(1, 1, true, c3out);
             else if (r) TP_G16(1, 0, true, nullptr);
@@ -304,15 +320,15 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         // curved Ewald sphere: the class weights depend on the k-triple
         hipLaunchKernelGGL(k_glam, dim3((unsigned)(b->kl_pad / kLamThreads + (b->kl_pad % kLamThreads != 0)),
                                         (unsigned)((b->n_m + kLamClasses - 1) / kLamClasses)),
-                           dim3(kLamThreads), 0, st, b->d_As, b->kl_pad, b->d_ktk, (int)b->klength, b->d_lamY,
-                           b->d_lament, b->d_lamcs, b->n_m, b->d_G, b->kl_pad);
+                           dim3(kLamThreads), 0, st, sc.As, b->kl_pad, b->d_ktk, (int)b->klength, b->d_lamY,
+                           b->d_lament, b->d_lamcs, b->n_m, sc.G, b->kl_pad);
     } else {
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
     // XCD-grouped column tiles: each As column block is fetched from HBM once
     hipLaunchKernelGGL((k_gemm<GF_BM, GF_BN, GF_WGM, 2, false, true>),
                        dim3((unsigned)(8 * ((b->g1_ct + 7) / 8) * b->g1_rt)), dim3(256), 0,
-                       st, b->d_H, b->nm_pad64, b->d_As, b->kl_pad, (int)b->nlt_pad, (int)(b->g1_rt * GF_BM),
-                       (int)b->kl_pad, b->g1_rt, b->g1_ct, b->d_G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
+                       st, b->d_H, b->nm_pad64, sc.As, b->kl_pad, (int)b->nlt_pad, (int)(b->g1_rt * GF_BM),
+                       (int)b->kl_pad, b->g1_rt, b->g1_ct, sc.G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
     }
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
@@ -322,15 +338,15 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         const dim3 grid((unsigned)(b->n_rt * b->n_ct));
         if (r && c3out)
             hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
-                               b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
-                               r->d_qhat, b->d_part, b->kl_pad, b->d_logtab);
+                               sc.G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
+                               r->d_qhat, sc.part, b->kl_pad, b->d_logtab);
         else if (r)
             hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 0, true>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
-                               b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, nullptr, b->R,
-                               r->d_qhat, b->d_part, b->kl_pad, b->d_logtab);
+                               sc.G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, nullptr, b->R,
+                               r->d_qhat, sc.part, b->kl_pad, b->d_logtab);
         else
             hipLaunchKernelGGL((k_gemm<G2_BM, G2_BN, G2_WGM, 1, false>), grid, dim3(256), 0, st, b->d_Bm, b->R_pad,
-                               b->d_G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
+                               sc.G, b->kl_pad, (int)b->nm_pad16, (int)b->R, ncols, b->n_rt, b->n_ct, c3out, b->R,
                                nullptr, nullptr, (int64_t)0, nullptr);
         TP_HIP(hipGetLastError());
         n_rt_fin = b->n_rt;
@@ -339,8 +355,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     const dim3 grid((unsigned)((ncols + 15) / 16), (unsigned)b->njc);
     const int64_t tap = (int64_t)b->n_m * b->jpad;
 #define TP_SEP(OUT, EN, OUTP, QH)                                                                                   \
-    hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), c3sep_lds_bytes(b->sep_kb), st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
-                       tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, b->d_part, b->kl_pad, OUTP, b->R,       \
+    hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
+                       tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, sc.part, b->kl_pad, OUTP, b->R,       \
                        b->d_logtab)
     if (r && c3out) TP_SEP(1, true, c3out, r->d_qhat);
     else if (r) TP_SEP(0, true, nullptr, r->d_qhat);
@@ -353,8 +369,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     if (timing) TP_HIP(hipEventRecord(ctx->ev[3], st));
     if (r) {
         const unsigned nfb = (unsigned)((ncols + kFinalCols - 1) / kFinalCols);
-        hipLaunchKernelGGL(k_final, dim3(nfb), dim3(256), 0, st, b->d_part, n_rt_fin, b->kl_pad, ncols, r->d_qsum,
-                           r->d_tf, b->d_blk, b->d_counter, Eout, mc);
+        hipLaunchKernelGGL(k_final, dim3(nfb), dim3(256), 0, st, sc.part, n_rt_fin, b->kl_pad, ncols, r->d_qsum,
+                           r->d_tf, sc.blk, sc.counter, Eout, mc);
         TP_HIP(hipGetLastError());
     }
     if (timing) TP_HIP(hipEventRecord(ctx->ev[4], st));
@@ -1690,6 +1706,13 @@ int tp_ref_create(tp_ctx *ctx, const tp_basis *b, const double *c3ref, int Kr, i
         const double *q = c3ref + R * ((k3 - 1) + (int64_t)Kr * ((k2 - 1) + (int64_t)Kr * (k1 - 1)));
         double s = 0.0;
         for (int64_t r = 0; r < R; ++r) s += std::fabs(q[r]);
+        if (!(s > 0.0) || !std::isfinite(s)) {
+            char msg[192];
+            std::snprintf(msg, sizeof(msg),
+                          "histogram slice (k1,k2,k3) = (%d,%d,%d) sums to %g: no normalised form; clamp it as "
+                          "loadHistograms does (tp_load_histograms)", k1, k2, k3, s);
+            return tp_fail(TP_E_ARG, msg);
+        }
         double t = 0.0;
         for (int64_t r = 0; r < R; ++r) {
             qhat[(size_t)i * R + r] = q[r] / s;
@@ -2653,8 +2676,8 @@ int tp_structure_from_atoms(tp_ctx *ctx, const double *atoms, int natoms, int LM
                             double *density, double *fourier, double *intensity, int64_t ld)
 {
     if (int rc = check_ctx(ctx)) return rc;
-    if ((!atoms && natoms > 0) || natoms < 0 || !(rmax > 0.0) || ld < (int64_t)LMAX * LMAX)
-        return tp_fail(TP_E_ARG, "bad argument");
+    if ((!atoms && natoms > 0) || natoms < 0 || !(rmax > 0.0) || K < 2 || LMAX < 1 || ld < (int64_t)LMAX * LMAX)
+        return tp_fail(TP_E_ARG, "bad argument (natoms >= 0, rmax > 0, K >= 2, LMAX >= 1, ld >= LMAX^2)");
     std::vector<StrAtom> h;
     std::vector<int2> hr;
     if (int rc = str_atoms(atoms, natoms, K, rmax, h, hr)) return rc;
@@ -2942,13 +2965,14 @@ int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double
                            (double *)nullptr);
         TP_HIP(hipGetLastError());
         if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
+        // a picture that accepted more than capacity photons is skipped whole
         hipLaunchKernelGGL(k_hist_pictures, dim3((unsigned)nb), dim3(kHistThreads), 0, st, dph.p, (const int64_t *)nullptr,
                            dcnt.p, p->capacity, h->dq, h->N, h->K2, h->K3, h->d_n1, h->d_n2, h->d_n3);
         TP_HIP(hipGetLastError());
         if (ctx->timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
         TP_HIP(hipMemcpyAsync(hc.data(), dcnt.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, st));
         TP_HIP(hipStreamSynchronize(st));
-        for (int i = 0; i < nb; ++i) over += std::max(0, hc[i] - p->capacity);
+        for (int i = 0; i < nb; ++i) over += hc[i] > p->capacity ? 1 : 0;
         if (ctx->timing) {
             float a = 0.f, c = 0.f;
             if (hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]) == hipSuccess) t_gen += a;
@@ -2960,7 +2984,7 @@ int tp_hist_add_generated(tp_hist *h, int LMAX, int K, double rmax, const double
         ctx->last_ms[1] = t_hist;
         ctx->last_ms[2] = ctx->last_ms[3] = 0.0;
     }
-    h->pictures += npictures;
+    h->pictures += npictures - over;
     if (overflow) *overflow = over;
     return 0;
 }
@@ -2990,7 +3014,7 @@ int tp_state_create(tp_ctx *ctx, const tp_basis *bc, const double *coeff, int64_
     int rc;
     if ((rc = dev_alloc(&s->d_cur, (size_t)nshell * b->n_c)) || (rc = dev_alloc(&s->d_prop, (size_t)nshell * b->n_c)) ||
         (rc = dev_alloc(&s->d_scal, 8)) || (rc = dev_alloc(&s->d_step, std::max(1, b->L / 2))) ||
-        (rc = dev_alloc(&s->d_sel, 1))) {
+        (rc = dev_alloc(&s->d_sel, 4))) {
         tp_state_destroy(s);
         return rc;
     }
@@ -3052,8 +3076,7 @@ int tp_state_c3(tp_ctx *ctx, const tp_basis *bc, tp_state *s, double *c3_out)
     return 0;
 }
 
-int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
-              double *E_trace, double *Enew_trace, int32_t *acc_trace, double *scalars)
+static int anneal_check(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const tp_state *s, const tp_anneal_params *p)
 {
     if (int rc = check_ctx(ctx)) return rc;
     if (!b || !r || !s || !p) return tp_fail(TP_E_ARG, "null argument");
@@ -3062,97 +3085,354 @@ int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, cons
         return tp_fail(TP_E_ARG, "invalid annealing parameters");
     if (b->L < 2) return tp_fail(TP_E_ARG, "annealing needs L >= 2");
     if (b->L > 62) return tp_fail(TP_E_UNSUPPORTED, "annealing moves need 2L+1 <= 64");
-    const int nl = b->L / 2;
-    hipStream_t st = ctx->stream;
-    std::vector<double> steps(nl);
-    for (int l = 2; l <= b->L; l += 2) steps[l / 2 - 1] = p->initial_stepsize * (l - 1);   // determination.jl:295
-    TP_HIP(hipMemcpyAsync(s->d_step, steps.data(), sizeof(double) * nl, hipMemcpyHostToDevice, st));
-    TP_HIP(hipMemsetAsync(s->d_sel, 0, sizeof(int), st));
-    double *d_tr = nullptr;
-    int *d_acc = nullptr;
-    const int ns = std::max(1, p->nsteps), nt = std::max(1, p->ntemp);
-    int rc = 0;
-    if ((rc = dev_alloc(&d_tr, 2 * (size_t)ns + nt + 4)) || (rc = dev_alloc(&d_acc, ns))) {
-        if (d_tr) (void)hipFree(d_tr);
-        return rc;
-    }
-    auto cleanup = [&]() {
-        (void)hipFree(d_tr);
-        (void)hipFree(d_acc);
-    };
+    return 0;
+}
+
+static McArgs anneal_mc(const tp_basis *b, tp_state *s, const tp_anneal_params *p)
+{
     McArgs mc = mc_none();
     mc.seed = p->seed;
     mc.decay = p->temperature_decay;
     mc.factor = p->stepsizefactor;
-    mc.nl = nl;
+    mc.nl = b->L / 2;
     mc.scal = s->d_scal;
     mc.stepsizes = s->d_step;
     mc.sel = s->d_sel;
-    mc.trE = d_tr;
-    mc.trEn = d_tr + ns;
-    mc.trA = d_acc;
-    mc.temp = d_tr + 2 * ns;
+    return mc;
+}
+
+static void launch_propose(tp_ctx *ctx, const tp_basis *b, tp_state *s, uint64_t seed, uint32_t step, uint32_t tag,
+                           const int *stopped)
+{
+    hipLaunchKernelGGL(k_propose, dim3((unsigned)(b->L / 2), (unsigned)((s->nshell + 3) / 4)), dim3(256), 0,
+                       ctx->stream, s->d_cur, s->d_prop, s->d_sel, s->nshell, b->n_c, b->d_coff, s->d_step, seed, step,
+                       tag, stopped);
+}
+
+int tp_anneal_begin(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
+                    tp_anneal_status *st)
+{
+    if (int rc = anneal_check(ctx, b, r, s, p)) return rc;
+    if (!st) return tp_fail(TP_E_ARG, "null status");
+    const int nl = b->L / 2;
+    hipStream_t stream = ctx->stream;
+    std::memset(st, 0, sizeof(*st));
+    st->nl = nl;
+    for (int l = 2; l <= b->L; l += 2) st->stepsizes[l / 2 - 1] = p->initial_stepsize * (l - 1);   // determination.jl:295
+    TP_HIP(hipMemcpyAsync(s->d_step, st->stepsizes, sizeof(double) * nl, hipMemcpyHostToDevice, stream));
+    TP_HIP(hipMemsetAsync(s->d_sel, 0, sizeof(int) * 4, stream));
+    double *d_temp = nullptr;
+    if (int rc = dev_alloc(&d_temp, (size_t)std::max(1, p->ntemp))) return rc;
+    McArgs mc = anneal_mc(b, s, p);
+    mc.temp = d_temp;
     // initial energy (determination.jl:298) -> scal[0]
     mc.mode = 2;
-    if ((rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 0, nullptr, b->d_out, mc))) {
-        cleanup();
-        return rc;
-    }
+    int rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 0, nullptr, b->d_out, mc);
     // temperature from `ntemp` proposals of the start structure (calculate_temperature, :244-261)
     double T0 = 0.0;
-    if (p->initial_temperature_factor > 0.0 && p->ntemp > 1) {
+    if (!rc && p->initial_temperature_factor > 0.0 && p->ntemp > 1) {
         mc.mode = 1;
-        for (int i = 0; i < p->ntemp; ++i) {
-            hipLaunchKernelGGL(k_propose, dim3(nl, (unsigned)((s->nshell + 3) / 4)), dim3(256), 0, st, s->d_cur, s->d_prop, s->d_sel, s->nshell, b->n_c,
-                               b->d_coff, s->d_step, p->seed, (uint32_t)i, TP_TAG_TEMP);
+        for (int i = 0; i < p->ntemp && !rc; ++i) {
+            launch_propose(ctx, b, s, p->seed, (uint32_t)i, TP_TAG_TEMP, nullptr);
             mc.step = (uint32_t)i;
-            if ((rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 1, nullptr, b->d_out, mc))) {
-                cleanup();
-                return rc;
-            }
+            rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 1, nullptr, b->d_out, mc);
         }
         std::vector<double> et(p->ntemp);
-        TP_HIP(hipMemcpyAsync(et.data(), mc.temp, sizeof(double) * p->ntemp, hipMemcpyDeviceToHost, st));
-        TP_HIP(hipStreamSynchronize(st));
-        double m = 0;
-        for (double v : et) m += v;
-        m /= p->ntemp;
-        double v2 = 0;
-        for (double v : et) v2 += (v - m) * (v - m);
-        T0 = std::sqrt(v2 / (p->ntemp - 1)) * p->initial_temperature_factor;
-    }
-    TP_HIP(hipMemcpyAsync(s->d_scal + 1, &T0, sizeof(double), hipMemcpyHostToDevice, st));
-    double E0 = 0;
-    TP_HIP(hipMemcpyAsync(&E0, s->d_scal, sizeof(double), hipMemcpyDeviceToHost, st));
-    TP_HIP(hipStreamSynchronize(st));
-    // the hot loop: propose, evaluate the proposal buffer, Metropolis in k_final
-    mc.mode = 3;
-    for (int i = 1; i <= p->nsteps; ++i) {
-        hipLaunchKernelGGL(k_propose, dim3(nl, (unsigned)((s->nshell + 3) / 4)), dim3(256), 0, st, s->d_cur, s->d_prop, s->d_sel, s->nshell, b->n_c,
-                           b->d_coff, s->d_step, p->seed, (uint32_t)i, TP_TAG_MOVE);
-        mc.step = (uint32_t)i;
-        if ((rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 1, nullptr, b->d_out, mc))) {
-            cleanup();
-            return rc;
+        if (!rc && (hipMemcpyAsync(et.data(), d_temp, sizeof(double) * p->ntemp, hipMemcpyDeviceToHost, stream) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(stream) != hipSuccess))
+            rc = tp_fail(TP_E_HIP, "temperature probe read failed");
+        if (!rc) {
+            double m = 0;
+            for (double v : et) m += v;
+            m /= p->ntemp;
+            double v2 = 0;
+            for (double v : et) v2 += (v - m) * (v - m);
+            T0 = std::sqrt(v2 / (p->ntemp - 1)) * p->initial_temperature_factor;
         }
     }
-    TP_HIP(hipGetLastError());
-    double sc[2] = {0, 0};
-    int sel = 0;
-    TP_HIP(hipMemcpyAsync(sc, s->d_scal, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
-    TP_HIP(hipMemcpyAsync(&sel, s->d_sel, sizeof(int), hipMemcpyDeviceToHost, st));
-    if (E_trace && p->nsteps > 0)
-        TP_HIP(hipMemcpyAsync(E_trace, mc.trE, sizeof(double) * p->nsteps, hipMemcpyDeviceToHost, st));
-    if (Enew_trace && p->nsteps > 0)
-        TP_HIP(hipMemcpyAsync(Enew_trace, mc.trEn, sizeof(double) * p->nsteps, hipMemcpyDeviceToHost, st));
-    if (acc_trace && p->nsteps > 0)
-        TP_HIP(hipMemcpyAsync(acc_trace, d_acc, sizeof(int) * p->nsteps, hipMemcpyDeviceToHost, st));
-    TP_HIP(hipStreamSynchronize(st));
-    if (sel) std::swap(s->d_cur, s->d_prop);   // the accepted structure is current again
-    if (scalars) {
-        scalars[0] = E0; scalars[1] = T0; scalars[2] = sc[0]; scalars[3] = sc[1];
+    (void)hipFree(d_temp);
+    if (rc) return rc;
+    TP_HIP(hipMemcpyAsync(s->d_scal + 1, &T0, sizeof(double), hipMemcpyHostToDevice, stream));
+    double E0 = 0;
+    TP_HIP(hipMemcpyAsync(&E0, s->d_scal, sizeof(double), hipMemcpyDeviceToHost, stream));
+    TP_HIP(hipStreamSynchronize(stream));
+    st->E = st->E0 = E0;
+    st->T = st->T0 = T0;
+    return 0;
+}
+
+int tp_anneal_steps(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
+                    tp_anneal_status *st, int max_steps, double min_stepsize, double *E_trace, double *Enew_trace,
+                    int32_t *acc_trace, int32_t *steps_done)
+{
+    if (steps_done) *steps_done = 0;
+    if (int rc = anneal_check(ctx, b, r, s, p)) return rc;
+    if (!st) return tp_fail(TP_E_ARG, "null status");
+    const int nl = b->L / 2;
+    if (st->nl != nl) return tp_fail(TP_E_STATE, "status was made for another L");
+    if (max_steps < 0 || st->step < 0 || st->step + (int64_t)max_steps > (int64_t)INT32_MAX)
+        return tp_fail(TP_E_ARG, "step count out of range");
+    // the loop condition is checked before each step (determination.jl:322)
+    if (min_stepsize > 0.0 && st->stepsizes[nl - 1] <= min_stepsize) st->stopped = 1;
+    if (st->stopped || max_steps == 0) return 0;
+    hipStream_t stream = ctx->stream;
+    const int32_t ctl0[4] = {0, 0, (int32_t)st->step, st->naccepted};   // sel, stopped, last step, accepted
+    const double sc0[2] = {st->E, st->T};
+    TP_HIP(hipMemcpyAsync(s->d_sel, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stream));
+    TP_HIP(hipMemcpyAsync(s->d_scal, sc0, sizeof(sc0), hipMemcpyHostToDevice, stream));
+    TP_HIP(hipMemcpyAsync(s->d_step, st->stepsizes, sizeof(double) * nl, hipMemcpyHostToDevice, stream));
+    double *d_tr = nullptr;
+    int *d_acc = nullptr;
+    int rc = 0;
+    if ((rc = dev_alloc(&d_tr, 2 * (size_t)max_steps)) || (rc = dev_alloc(&d_acc, (size_t)max_steps))) {
+        if (d_tr) (void)hipFree(d_tr);
+        return rc;
     }
-    cleanup();
+    McArgs mc = anneal_mc(b, s, p);
+    mc.mode = 3;
+    mc.trE = d_tr;
+    mc.trEn = d_tr + max_steps;
+    mc.trA = d_acc;
+    mc.tr0 = (uint32_t)st->step;
+    mc.minstep = min_stepsize;
+    mc.ctl = s->d_sel + 1;
+    // the hot loop: propose, evaluate the proposal buffer, Metropolis in k_final;
+    // with a stopping rule the host looks at the flag every kCheck steps
+    constexpr int kCheck = 256;
+    const uint32_t first = (uint32_t)st->step + 1, last = (uint32_t)(st->step + max_steps);
+    for (uint32_t i = first; i <= last && !rc; ++i) {
+        launch_propose(ctx, b, s, p->seed, i, TP_TAG_MOVE, min_stepsize > 0.0 ? mc.ctl : nullptr);
+        mc.step = i;
+        rc = enqueue_eval(ctx, b, r, s->d_cur, s->d_prop, s->d_sel, 1, nullptr, b->d_out, mc);
+        if (!rc && min_stepsize > 0.0 && (i - first + 1) % kCheck == 0 && i < last) {
+            int32_t stopped = 0;
+            if (hipMemcpyAsync(&stopped, mc.ctl, sizeof(int32_t), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess)
+                rc = tp_fail(TP_E_HIP, "stopping-rule flag read failed");
+            if (stopped) break;
+        }
+    }
+    if (!rc && hipGetLastError() != hipSuccess) rc = tp_fail(TP_E_HIP, "annealing launch failed");
+    int32_t ctl[4] = {0, 0, 0, 0};
+    double sc[2] = {0, 0};
+    if (!rc && (hipMemcpyAsync(ctl, s->d_sel, sizeof(ctl), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                hipMemcpyAsync(sc, s->d_scal, sizeof(sc), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                hipMemcpyAsync(st->stepsizes, s->d_step, sizeof(double) * nl, hipMemcpyDeviceToHost, stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(stream) != hipSuccess))
+        rc = tp_fail(TP_E_HIP, "annealing state read failed");
+    const int32_t taken = rc ? 0 : (int32_t)(ctl[2] - st->step);
+    if (!rc && taken > 0) {
+        if ((E_trace && hipMemcpy(E_trace, d_tr, sizeof(double) * taken, hipMemcpyDeviceToHost) != hipSuccess) ||
+            (Enew_trace &&
+             hipMemcpy(Enew_trace, d_tr + max_steps, sizeof(double) * taken, hipMemcpyDeviceToHost) != hipSuccess) ||
+            (acc_trace && hipMemcpy(acc_trace, d_acc, sizeof(int32_t) * taken, hipMemcpyDeviceToHost) != hipSuccess))
+            rc = tp_fail(TP_E_HIP, "trace read failed");
+    }
+    (void)hipFree(d_tr);
+    (void)hipFree(d_acc);
+    if (rc) return rc;
+    if (ctl[0]) std::swap(s->d_cur, s->d_prop);   // the accepted structure is current again
+    st->step = ctl[2];
+    st->naccepted = ctl[3];
+    st->stopped = ctl[1];
+    st->E = sc[0];
+    st->T = sc[1];
+    if (min_stepsize > 0.0 && st->stepsizes[nl - 1] <= min_stepsize) st->stopped = 1;
+    if (steps_done) *steps_done = taken;
+    return 0;
+}
+
+int tp_anneal(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s, const tp_anneal_params *p,
+              double *E_trace, double *Enew_trace, int32_t *acc_trace, double *scalars)
+{
+    tp_anneal_status st;
+    if (int rc = tp_anneal_begin(ctx, b, r, s, p, &st)) return rc;
+    if (int rc = tp_anneal_steps(ctx, b, r, s, p, &st, p->nsteps, 0.0, E_trace, Enew_trace, acc_trace, nullptr))
+        return rc;
+    if (scalars) {
+        scalars[0] = st.E0; scalars[1] = st.T0; scalars[2] = st.E; scalars[3] = st.T;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoints (saveState / rotation_search's resume, determination.jl:154-278)
+// File: "TPCKPT\0\0", u32 version, u32 header bytes, the header fields below
+// in this order (little-endian, as written by this x86-64 host code), the
+// coefficients (nshell * ld complex128), then u64 FNV-1a of all bytes before.
+// ---------------------------------------------------------------------------
+}   // extern "C": the serialisation helpers below are C++
+static const char kCkptMagic[8] = {'T', 'P', 'C', 'K', 'P', 'T', 0, 0};
+constexpr uint32_t kCkptVersion = 1;
+
+struct CkptWriter {
+    std::vector<unsigned char> buf;
+    template <class T>
+    void put(const T &v)
+    {
+        const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+        buf.insert(buf.end(), p, p + sizeof(T));
+    }
+};
+
+struct CkptReader {
+    const unsigned char *p, *end;
+    bool ok = true;
+    template <class T>
+    T get()
+    {
+        T v{};
+        if ((size_t)(end - p) < sizeof(T)) {
+            ok = false;
+            return v;
+        }
+        std::memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+        return v;
+    }
+};
+
+static uint64_t fnv1a(const unsigned char *p, size_t n)
+{
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+static void ckpt_header(CkptWriter &w, const tp_checkpoint_info *in)
+{
+    const tp_anneal_params &pa = in->params;
+    const tp_anneal_status &s = in->status;
+    w.put(pa.nsteps); w.put(pa.ntemp); w.put(pa.initial_stepsize); w.put(pa.stepsizefactor);
+    w.put(pa.initial_temperature_factor); w.put(pa.temperature_decay); w.put(pa.seed);
+    w.put(s.step); w.put(s.E); w.put(s.T); w.put(s.E0); w.put(s.T0);
+    for (int i = 0; i < TP_MAX_LBLOCKS; ++i) w.put(s.stepsizes[i]);
+    w.put(s.nl); w.put(s.naccepted); w.put(s.stopped);
+    w.put(in->min_stepsize); w.put(in->lambda); w.put(in->dq);
+    w.put(in->L); w.put(in->LMAX); w.put(in->N); w.put(in->K); w.put(in->k3lo); w.put(in->k3hi);
+    w.put(in->nshell); w.put(in->ld);
+}
+
+static void ckpt_read_header(CkptReader &r, tp_checkpoint_info *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    tp_anneal_params &pa = o->params;
+    tp_anneal_status &s = o->status;
+    pa.nsteps = r.get<int>(); pa.ntemp = r.get<int>(); pa.initial_stepsize = r.get<double>();
+    pa.stepsizefactor = r.get<double>(); pa.initial_temperature_factor = r.get<double>();
+    pa.temperature_decay = r.get<double>(); pa.seed = r.get<uint64_t>();
+    s.step = r.get<int64_t>(); s.E = r.get<double>(); s.T = r.get<double>(); s.E0 = r.get<double>();
+    s.T0 = r.get<double>();
+    for (int i = 0; i < TP_MAX_LBLOCKS; ++i) s.stepsizes[i] = r.get<double>();
+    s.nl = r.get<int32_t>(); s.naccepted = r.get<int32_t>(); s.stopped = r.get<int32_t>();
+    o->min_stepsize = r.get<double>(); o->lambda = r.get<double>(); o->dq = r.get<double>();
+    o->L = r.get<int32_t>(); o->LMAX = r.get<int32_t>(); o->N = r.get<int32_t>(); o->K = r.get<int32_t>();
+    o->k3lo = r.get<int32_t>(); o->k3hi = r.get<int32_t>(); o->nshell = r.get<int32_t>(); o->ld = r.get<int32_t>();
+}
+
+extern "C" {
+int tp_checkpoint_save(const char *path, const tp_checkpoint_info *info, const double *coeff)
+{
+    if (!path || !info || (!coeff && info->nshell > 0)) return tp_fail(TP_E_ARG, "null argument");
+    if (info->nshell < 0 || info->ld < 0 || (int64_t)info->ld < (int64_t)info->LMAX * info->LMAX ||
+        info->status.nl < 0 || info->status.nl > TP_MAX_LBLOCKS)
+        return tp_fail(TP_E_ARG, "checkpoint shape out of range (nshell >= 0, ld >= LMAX^2, nl <= 32)");
+    CkptWriter hw;
+    ckpt_header(hw, info);
+    CkptWriter w;
+    w.buf.insert(w.buf.end(), kCkptMagic, kCkptMagic + 8);
+    w.put(kCkptVersion);
+    w.put((uint32_t)hw.buf.size());
+    w.buf.insert(w.buf.end(), hw.buf.begin(), hw.buf.end());
+    const size_t nd = 2 * (size_t)info->nshell * (size_t)info->ld;
+    const unsigned char *cp = reinterpret_cast<const unsigned char *>(coeff);
+    if (nd) w.buf.insert(w.buf.end(), cp, cp + nd * sizeof(double));
+    w.put(fnv1a(w.buf.data(), w.buf.size()));
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return tp_fail(TP_E_ARG, "cannot open " + tmp + " for writing");
+    const bool ok = std::fwrite(w.buf.data(), 1, w.buf.size(), f) == w.buf.size();
+    if (std::fclose(f) != 0 || !ok) {
+        std::remove(tmp.c_str());
+        return tp_fail(TP_E_ARG, "short write to " + tmp);
+    }
+    if (std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return tp_fail(TP_E_ARG, std::string("cannot rename the checkpoint to ") + path);
+    }
+    return 0;
+}
+
+int tp_checkpoint_load(const char *path, tp_checkpoint_info *info, double *coeff, int64_t cap_doubles)
+{
+    if (!path || !info) return tp_fail(TP_E_ARG, "null argument");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return tp_fail(TP_E_ARG, std::string("cannot open checkpoint ") + path);
+    std::vector<unsigned char> buf;
+    unsigned char chunk[1 << 16];
+    size_t n;
+    while ((n = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + n);
+    std::fclose(f);
+    if (buf.size() < 8 + 4 + 4 + 8 || std::memcmp(buf.data(), kCkptMagic, 8) != 0)
+        return tp_fail(TP_E_ARG, std::string(path) + " is not a checkpoint of this library");
+    uint64_t sum;
+    std::memcpy(&sum, buf.data() + buf.size() - 8, 8);
+    if (fnv1a(buf.data(), buf.size() - 8) != sum) return tp_fail(TP_E_ARG, std::string(path) + ": checksum mismatch");
+    CkptReader r{buf.data() + 8, buf.data() + buf.size() - 8};
+    const uint32_t ver = r.get<uint32_t>(), hlen = r.get<uint32_t>();
+    if (ver != kCkptVersion) return tp_fail(TP_E_ARG, std::string(path) + ": unknown checkpoint version");
+    CkptReader h{r.p, r.p + std::min<size_t>(hlen, (size_t)(r.end - r.p))};
+    ckpt_read_header(h, info);
+    if (!h.ok || !r.ok) return tp_fail(TP_E_ARG, std::string(path) + ": truncated header");
+    r.p += hlen;
+    if (info->nshell < 0 || info->ld < 0) return tp_fail(TP_E_ARG, std::string(path) + ": bad shape");
+    const size_t nd = 2 * (size_t)info->nshell * (size_t)info->ld;
+    if ((size_t)(r.end - r.p) != nd * sizeof(double))
+        return tp_fail(TP_E_ARG, std::string(path) + ": coefficient block has the wrong size");
+    if (coeff) {
+        if ((int64_t)nd > cap_doubles) return tp_fail(TP_E_ARG, "coefficient buffer too small for the checkpoint");
+        std::memcpy(coeff, r.p, nd * sizeof(double));
+    }
+    return 0;
+}
+
+int tp_load_histograms(const double *c3_full, int NA, int NB, int Kf, int K3, double *c3_out)
+{
+    if (!c3_full || !c3_out) return tp_fail(TP_E_ARG, "null argument");
+    if (NA < 1 || Kf < 1 || K3 < 1 || K3 > Kf || (NB != NA && NB != 2 * NA))
+        return tp_fail(TP_E_ARG, "histogram must be N x N or N x 2N x Kf^3 with 1 <= K3 <= Kf");
+    const int64_t N = NA, R = 2 * N * N, Rf = (int64_t)NA * NB;
+    const int64_t nout = R * K3 * K3 * K3;
+    // doubling of an old N x N histogram: only k1 >= k2 >= k3 is filled (datagen.jl:445-458),
+    // every other slice stays 0 and is clamped below
+    for (int k1 = 1; k1 <= K3; ++k1)
+        for (int k2 = 1; k2 <= K3; ++k2)
+            for (int k3 = 1; k3 <= K3; ++k3) {
+                const double *src = c3_full + Rf * ((k3 - 1) + (int64_t)Kf * ((k2 - 1) + (int64_t)Kf * (k1 - 1)));
+                double *dst = c3_out + R * ((k3 - 1) + (int64_t)K3 * ((k2 - 1) + (int64_t)K3 * (k1 - 1)));
+                if (NB == 2 * NA) {
+                    std::memcpy(dst, src, sizeof(double) * R);
+                } else if (k1 >= k2 && k2 >= k3) {
+                    std::memcpy(dst, src, sizeof(double) * Rf);
+                    std::memcpy(dst + Rf, src, sizeof(double) * Rf);
+                } else {
+                    std::fill(dst, dst + R, 0.0);
+                }
+            }
+    // clamp (:460-462); the cut keeps K3 shells, then the global normalisation (:465-468)
+    double tot = 0.0;
+    for (int64_t i = 0; i < nout; ++i) {
+        if (std::isnan(c3_out[i])) return tp_fail(TP_E_ARG, "histogram holds NaN");   // max.(NaN, 1e-30) is NaN
+        const double v = c3_out[i] > 1e-30 ? c3_out[i] : 1e-30;
+        c3_out[i] = v;
+        tot += std::fabs(v);
+    }
+    for (int64_t i = 0; i < nout; ++i) c3_out[i] /= tot;
     return 0;
 }
 

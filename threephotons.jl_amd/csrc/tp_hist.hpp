@@ -45,8 +45,10 @@ __device__ __forceinline__ int hist_mod_floor(double x, int m)
     return (int)(r < 0 ? r + m : r);
 }
 
-// Pictures: photons [off[b], off[b+1]) of ph, or with cnt given the first
-// min(cnt[b], cap) photons of slot b (ph[b cap .. b cap + cap), generated pictures).
+// Pictures: photons [off[b], off[b+1]) of ph, or with cnt given the cnt[b]
+// photons of slot b (ph[b cap .. b cap + cap), generated pictures); a slot
+// whose picture accepted more than cap photons lost some of them and is
+// skipped whole, so that no picture is histogrammed with a subset.
 __global__ __launch_bounds__(kHistThreads) void k_hist_pictures(const double *__restrict__ ph,
                                                                 const int64_t *__restrict__ off,
                                                                 const int *__restrict__ cnt, int cap, double dq,
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist_pictures(const double *__
     __shared__ int s_l;
     const int tid = threadIdx.x;
     const int64_t p0 = cnt ? (int64_t)blockIdx.x * cap : off[blockIdx.x];
-    const int n = cnt ? min(cnt[blockIdx.x], cap) : (int)(off[blockIdx.x + 1] - p0);
+    const int n = cnt ? (cnt[blockIdx.x] > cap ? 0 : cnt[blockIdx.x]) : (int)(off[blockIdx.x + 1] - p0);
     const double da = M_PI / N;
     if (tid == 0) s_l = 0;
     for (int i = tid; i < n; i += kHistThreads) {

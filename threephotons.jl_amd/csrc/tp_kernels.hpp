@@ -891,7 +891,12 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
 // mode 3: Metropolis step (evaluate_step, determination.jl:491-515, with
 //         T *= decay first, rotate_all_at_once :328): on acceptance the step
 //         sizes grow by `factor`, E is replaced and the proposal buffer
-//         becomes current (*sel ^= 1); otherwise step sizes shrink.
+//         becomes current (*sel ^= 1); otherwise step sizes shrink.  With a
+//         control block (ctl != nullptr) a step is skipped whole once ctl[0]
+//         is set; after each step ctl[1] = its index, ctl[2] counts the
+//         accepted steps, and ctl[0] is set when the step size of the largest
+//         l is at or below minstep (the loop condition of rotate_all_at_once,
+//         determination.jl:1,322).  Trace entry of step i: i - 1 - tr0.
 // ---------------------------------------------------------------------------
 struct McArgs {
     int mode;
@@ -902,9 +907,12 @@ struct McArgs {
     double *scal;        // {E, T}
     double *stepsizes;   // [nl]
     int *sel;            // current buffer index
-    double *trE, *trEn;  // [nsteps]
-    int *trA;            // [nsteps]
+    double *trE, *trEn;  // [steps of this call]
+    int *trA;            // [steps of this call]
     double *temp;        // [ntemp]
+    uint32_t tr0;        // step index before the first step of this call
+    double minstep;      // stopping rule (<= 0: none)
+    int *ctl;            // {stopped, last step, accepted steps} or nullptr
 };
 
 constexpr int kFinalCols = 32;   // columns per k_final block (8 threads each)
@@ -966,6 +974,7 @@ __global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, 
     } else if (mc.mode == 2) {
         mc.scal[0] = E;
     } else if (mc.mode == 3) {
+        if (mc.ctl && mc.ctl[0]) return;   // the run has met its stopping rule
         const double T = mc.scal[1] * mc.decay;
         mc.scal[1] = T;
         const double dE = E - mc.scal[0];
@@ -979,9 +988,15 @@ __global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, 
         } else {
             for (int i = 0; i < mc.nl; ++i) mc.stepsizes[i] /= mc.factor;
         }
-        mc.trE[mc.step - 1] = mc.scal[0];
-        mc.trEn[mc.step - 1] = E;
-        mc.trA[mc.step - 1] = acc;
+        const uint32_t t = mc.step - 1 - mc.tr0;
+        mc.trE[t] = mc.scal[0];
+        mc.trEn[t] = E;
+        mc.trA[t] = acc;
+        if (mc.ctl) {
+            mc.ctl[1] = (int)mc.step;
+            mc.ctl[2] += acc;
+            if (mc.minstep > 0.0 && mc.stepsizes[mc.nl - 1] <= mc.minstep) mc.ctl[0] = 1;
+        }
     }
 }
 
@@ -1011,10 +1026,11 @@ __global__ __launch_bounds__(256) void k_propose(double2 *__restrict__ b0, doubl
                                                  const int *__restrict__ sel, int nshell, int n_c,
                                                  const int *__restrict__ coff_l,
                                                  const double *__restrict__ stepsizes, uint64_t seed,
-                                                 uint32_t step, uint32_t tag)
+                                                 uint32_t step, uint32_t tag, const int *__restrict__ stopped)
 {
     __shared__ double uv[2][64];
     __shared__ double sbeta;
+    if (stopped && stopped[0]) return;   // no step follows a met stopping rule
     const int s = sel ? sel[0] : 0;
     const double2 *cur = s ? b1 : b0;
     double2 *prop = s ? b0 : b1;

@@ -29,6 +29,16 @@ __all__ = [
 ]
 
 
+def _digest(a):
+    """64-bit hash of a contiguous array's bytes (xxh3; blake2b where xxhash is absent)."""
+    try:
+        import xxhash
+        return xxhash.xxh3_64_intdigest(memoryview(a).cast("B"))
+    except ImportError:
+        import hashlib
+        return hashlib.blake2b(memoryview(a).cast("B"), digest_size=8).hexdigest()
+
+
 def seanindex(l, m, LMAX):
     """1-based position of (l, m) in a length-LMAX^2 vector (spherical_harmonics.jl:163-170)."""
     B = LMAX
@@ -159,22 +169,24 @@ class BasisTypeHip:
         return int(lib().tp_basis_get_precision(self.h))
 
     def ref(self, c3ref, K3_range=None):
-        """Device copy of the normalised histogram slices visited by energy()'s walk (cached)."""
+        """Device copy of the normalised histogram slices visited by energy()'s walk.
+        Cached by the array's contents (a 64-bit hash of its bytes), so an array
+        changed in place gets a fresh device copy."""
         if K3_range is None:
             K3_range = range(1, self.K + 1)
-        key = (id(c3ref), K3_range.start, K3_range.stop)
-        hit = self._refs.get(key)
-        if hit is not None and hit[1] is c3ref:
-            return hit[0]
-        c3ref = np.asarray(c3ref, dtype=np.float64)
-        if c3ref.ndim != 5 or c3ref.shape[0] != self.N or c3ref.shape[1] != 2 * self.N:
+        arr = np.asarray(c3ref, dtype=np.float64)
+        if arr.ndim != 5 or arr.shape[0] != self.N or arr.shape[1] != 2 * self.N:
             raise ValueError("c3ref must be (N, 2N, Kr, Kr, Kr)")
-        Kr = c3ref.shape[2]
-        flat = np.ascontiguousarray(c3ref.ravel(order="F"))
+        flat = np.ascontiguousarray(arr.ravel(order="F"))
+        key = (_digest(flat), arr.shape, K3_range.start, K3_range.stop)
+        hit = self._refs.get(key)
+        if hit is not None:
+            return hit[0]
+        Kr = arr.shape[2]
         h = C.c_void_p()
         check(lib().tp_ref_create(self.ctx.h, self.h, flat.ctypes.data, Kr, K3_range.start, K3_range.stop - 1,
                                   C.byref(h)), "tp_ref_create")
-        self._refs[key] = (h, c3ref)
+        self._refs[key] = (h, None)
         return h
 
 

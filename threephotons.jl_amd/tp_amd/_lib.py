@@ -127,6 +127,11 @@ def lib():
     L.tp_batch_set_amax.argtypes = [vp, dbl]
     L.tp_batch_set_contraction.argtypes = [vp, i32]
     L.tp_batch_set_family.argtypes = [vp, i32]
+    L.tp_anneal_begin.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.tp_anneal_steps.argtypes = [vp, vp, vp, vp, vp, vp, i32, dbl, vp, vp, vp, vp]
+    L.tp_checkpoint_save.argtypes = [C.c_char_p, vp, vp]
+    L.tp_checkpoint_load.argtypes = [C.c_char_p, vp, vp, i64]
+    L.tp_load_histograms.argtypes = [vp, i32, i32, i32, i32, vp]
     _lib = L
     return L
 
@@ -166,6 +171,44 @@ class AnnealParams(C.Structure):
     ]
 
 
+MAX_LBLOCKS = 32
+
+
+class AnnealStatus(C.Structure):
+    """tp_anneal_status: rotate_all_at_once's state for one L."""
+    _fields_ = [
+        ("step", C.c_int64),
+        ("E", C.c_double),
+        ("T", C.c_double),
+        ("E0", C.c_double),
+        ("T0", C.c_double),
+        ("stepsizes", C.c_double * MAX_LBLOCKS),
+        ("nl", C.c_int32),
+        ("naccepted", C.c_int32),
+        ("stopped", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class CheckpointInfo(C.Structure):
+    """tp_checkpoint_info: header of an annealing checkpoint file."""
+    _fields_ = [
+        ("params", AnnealParams),
+        ("status", AnnealStatus),
+        ("min_stepsize", C.c_double),
+        ("lam", C.c_double),
+        ("dq", C.c_double),
+        ("L", C.c_int32),
+        ("LMAX", C.c_int32),
+        ("N", C.c_int32),
+        ("K", C.c_int32),
+        ("k3lo", C.c_int32),
+        ("k3hi", C.c_int32),
+        ("nshell", C.c_int32),
+        ("ld", C.c_int32),
+    ]
+
+
 # Entry points declared in include/tp.h (checked by the CPU test suite).
 EXPORTS = [
     "tp_abi_version", "tp_last_error", "tp_init", "tp_finalize", "tp_basis_create", "tp_basis_destroy",
@@ -182,7 +225,8 @@ EXPORTS = [
     "tp_fit_create", "tp_fit_set_volume", "tp_fit_similarity", "tp_fit_destroy",
     "tp_sh_surface", "tp_sh_coeff", "tp_sph_bessel", "tp_sraar", "tp_pictures", "tp_hist_add_generated",
     "tp_density_surface", "tp_absolute_square", "tp_structure_from_atoms", "tp_two_photons_ewald",
-    "tp_retrieve_solution_ewald",
+    "tp_retrieve_solution_ewald", "tp_anneal_begin", "tp_anneal_steps", "tp_checkpoint_save", "tp_checkpoint_load",
+    "tp_load_histograms",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3

@@ -55,11 +55,16 @@ def generate_histograms(hist, volume, qcut, npictures, envelope_sigma=None, phot
                         beamstop_width=0.0, capacity=512, seed=0, first_picture=0, particles=1, noise_photons=0,
                         noise_sigma=1.0, incident_photon_variance=0, beamstop_only=False):
     """Add npictures synthetic pictures of `volume` to the PhotonHistogram
-    `hist`; returns the number of accepted photons dropped beyond capacity."""
+    `hist`.  Raises RuntimeError when a picture accepted more photons than
+    `capacity` (as pointsPerOrientation does); such pictures are left out of
+    the histogram, the others are counted."""
     c = _coeff(volume)
     p = _params(qcut, envelope_sigma, photons_per_image, lam, beamstop_width, capacity, seed, particles,
                 noise_photons, noise_sigma, incident_photon_variance, beamstop_only)
     over = C.c_int64()
     check(lib().tp_hist_add_generated(hist.h, volume.LMAX, volume.KMAX, volume.rmax, c.ctypes.data, c.shape[1],
                                       C.byref(p), npictures, first_picture, C.byref(over)), "tp_hist_add_generated")
-    return over.value
+    if over.value > 0:
+        raise RuntimeError(f"{over.value} of {npictures} pictures accepted more photons than capacity {capacity} "
+                           "and were left out of the histogram")
+    return 0
