@@ -127,6 +127,19 @@ void tp_ref_destroy(tp_ref *r);
 int tp_energy(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const double *coeff, int64_t ld,
               int measure, double *E_out);
 
+/* tp_energy for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments, arithmetic and result as
+ * tp_energy (bit for bit), but every per-evaluation device buffer comes from
+ * one static array of 32 MiB per GPU that the library holds until the process
+ * ends: the call makes no hipMalloc, hipFree or memset.  Bases whose scratch
+ * exceeds 32 MiB (the full-size L = 18, K = 26, N = 32 basis needs about 20 MiB
+ * in FP64 and 27 MiB in FP16X3) get TP_E_UNSUPPORTED.  All calls on one GPU
+ * share the array: calls that overlap on one GPU (other threads or
+ * contexts) race and must be serialised by the caller. */
+int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const double *coeff, int64_t ld,
+                      int measure, double *E_out);
+
 /* ---- device-resident structure (the annealing state) -------------------- */
 
 /* Upload nshell shells of coefficients (nshell >= K; the reference rotates all

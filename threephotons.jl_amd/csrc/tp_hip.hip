@@ -10,6 +10,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -75,8 +76,19 @@ static uint16_t host_bf16(double x)   // round to nearest even of the float valu
 // ---------------------------------------------------------------------------
 struct PhTables;
 
+// Device scratch of tp_energy_noalloc, for callers that cannot allocate: one
+// array per GPU, held from the library's load until the process ends.  Host
+// code never uses the name `scratch` as an address: tp_init takes each
+// device's address of it once with hipGetSymbolAddress (g_scratch_ptr).
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;   // 32 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
 struct tp_ctx {
     int device = 0;
+    void *scratch = nullptr;   // this device's address of the static scratch array
     hipStream_t stream = nullptr;
     bool timing = false;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -1081,9 +1093,24 @@ int tp_init(int device, tp_ctx **out)
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n <= 0) return tp_fail(TP_E_HIP, "no HIP device available");
     if (device < 0 || device >= n) return tp_fail(TP_E_ARG, "device ordinal out of range");
+    if (device >= kMaxDevices) return tp_fail(TP_E_UNSUPPORTED, "device ordinal above 63");
     TP_HIP(hipSetDevice(device));
+    void *scr = nullptr;
+    {
+        // once per device, with that device current: its own copy of the array
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[device]) {
+            e = hipGetSymbolAddress(&g_scratch_ptr[device], HIP_SYMBOL(scratch));
+            if (e != hipSuccess) {
+                g_scratch_ptr[device] = nullptr;
+                return tp_fail(TP_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            }
+        }
+        scr = g_scratch_ptr[device];
+    }
     tp_ctx *c = new tp_ctx();
     c->device = device;
+    c->scratch = scr;
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -1755,6 +1782,83 @@ int tp_energy(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *co
     if (int rc = upload_coeff(ctx, b, coeff, ld, b->K, b->d_coef, b->d_raw)) return rc;
     if (int rc = enqueue_eval(ctx, b, r, b->d_coef, nullptr, nullptr, 0, nullptr, ctx->d_res, mc_none())) return rc;
     TP_HIP(hipStreamSynchronize(ctx->stream));
+    *E_out = ctx->h_res[0];
+    record_timing(ctx);
+    return 0;
+}
+
+// The energy with every per-evaluation buffer carved from the static scratch
+// array: the coefficient staging and compact copy, As, G, the energy partial
+// sums and block totals, the k_final counter and, in mixed precision, the
+// shell scales and the 16-bit As.  No allocation, no memset.
+int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *coeff, int64_t ld,
+                      int measure, double *E_out)
+{
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!bc || !r || !E_out || !coeff) return tp_fail(TP_E_ARG, "null argument");
+    if (measure != 0) {   // correlations.jl:210
+        *E_out = 0.0;
+        return 0;
+    }
+    const tp_basis *b = bc;
+    if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
+    if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "coefficient vector shorter than LMAX^2");
+    if (!ctx->scratch) return tp_fail(TP_E_STATE, "no scratch address for this device");
+    const bool mixed16 = b->prec == TP_PREC_FP16X3 || b->prec == TP_PREC_BF16;
+    const int64_t nrt = std::max<int64_t>(std::max(b->n_rt, b->njc), b->n_rt16);
+    const size_t want[] = {
+        sizeof(double2) * (size_t)b->K * ld,                  // 0 raw coefficients
+        sizeof(double2) * (size_t)b->K * b->n_c,              // 1 compact coefficients
+        sizeof(double) * (size_t)b->nlt_pad * b->kl_pad,      // 2 As
+        sizeof(double) * (size_t)b->nm_pad64 * b->kl_pad,     // 3 G
+        sizeof(double) * (size_t)nrt * 2 * b->kl_pad,         // 4 energy partial sums
+        sizeof(double) * (size_t)b->n_fb,                     // 5 block totals
+        sizeof(unsigned),                                     // 6 k_final counter
+        sizeof(double) * (size_t)b->K,                        // 7 shell scales
+        mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 8 As hi
+        mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 9 As lo
+        mixed16 ? sizeof(double) * (size_t)b->kl_pad : 0,              // 10 1 / column scales
+    };
+    constexpr int nbuf = sizeof(want) / sizeof(want[0]);
+    size_t off[nbuf], total = 0;
+    for (int i = 0; i < nbuf; ++i) {
+        off[i] = total;
+        total += (want[i] + 255) / 256 * 256;
+    }
+    if (total > kNoAllocScratchBytes) {
+        char msg[200];
+        std::snprintf(msg, sizeof(msg), "this basis needs %zu bytes of scratch; tp_energy_noalloc has %zu "
+                      "(use tp_energy)", total, kNoAllocScratchBytes);
+        return tp_fail(TP_E_UNSUPPORTED, msg);
+    }
+    char *base = static_cast<char *>(ctx->scratch);
+    double2 *raw = reinterpret_cast<double2 *>(base + off[0]);
+    double2 *coef = reinterpret_cast<double2 *>(base + off[1]);
+    EvalScratch sc{reinterpret_cast<double *>(base + off[2]), reinterpret_cast<double *>(base + off[3]),
+                   reinterpret_cast<double *>(base + off[4]), reinterpret_cast<double *>(base + off[5]),
+                   reinterpret_cast<double *>(base + off[10]), reinterpret_cast<double *>(base + off[7]),
+                   reinterpret_cast<unsigned *>(base + off[6]), reinterpret_cast<uint16_t *>(base + off[8]),
+                   reinterpret_cast<uint16_t *>(base + off[9])};
+    hipStream_t st = ctx->stream;
+    // the array holds the last call's values: As and G padding must read as zero
+    // and the arrival counter must start at zero (As, G and the counter are
+    // adjacent; the 16-bit As follow the shell scales)
+    const int64_t nz = (int64_t)((off[6] + want[6] - off[2] + 7) / 8);
+    hipLaunchKernelGGL(k_zero64, dim3((unsigned)std::min<int64_t>((nz + 255) / 256, 4096)), dim3(256), 0, st,
+                       reinterpret_cast<double *>(base + off[2]), nz);
+    if (mixed16) {
+        const int64_t n16 = (int64_t)((off[9] + want[9] - off[8]) / 8);
+        hipLaunchKernelGGL(k_zero64, dim3((unsigned)std::min<int64_t>((n16 + 255) / 256, 4096)), dim3(256), 0, st,
+                           reinterpret_cast<double *>(base + off[8]), n16);
+    }
+    TP_HIP(hipMemcpyAsync(raw, coeff, want[0], hipMemcpyHostToDevice, st));
+    const int64_t n = (int64_t)b->K * b->n_c;
+    hipLaunchKernelGGL(k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, raw, ld, b->K, b->d_sean0, b->n_c,
+                       coef);
+    TP_HIP(hipGetLastError());
+    if (int rc = enqueue_eval(ctx, b, r, coef, nullptr, nullptr, 0, nullptr, ctx->d_res, mc_none(), false, &sc))
+        return rc;
+    TP_HIP(hipStreamSynchronize(st));
     *E_out = ctx->h_res[0];
     record_timing(ctx);
     return 0;

@@ -74,6 +74,13 @@ __device__ __forceinline__ const double2 *pick(const double2 *b0, const double2 
 }
 
 // Gather the compact (even l <= L) coefficients out of LMAX^2-strided vectors.
+// Zero n doubles (a fill kernel, so that no memset node is needed).
+__global__ __launch_bounds__(256) void k_zero64(double *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
 __global__ void k_pack(const double2 *__restrict__ raw, int64_t ld, int nshell, const int *__restrict__ sean0,
                        int n_c, double2 *__restrict__ coef)
 {
