@@ -18,6 +18,7 @@
 #include "tp_host_math.hpp"
 #include "tp_kernels.hpp"
 #include "tp_triple16.hpp"
+#include "tp_triple_sym.hpp"
 #include "tp_batch.hpp"
 #include "tp_mlp.hpp"
 #include "tp_grad.hpp"
@@ -141,6 +142,15 @@ struct tp_basis {
     int *d_gstart8 = nullptr;           // [n_groups + 1]
     double *d_shsc = nullptr;           // [K] shell scales of the mixed-precision triple products
     int *d_work = nullptr;         // [nwork] lt | half << 16, most expensive first
+    // sorted l-triples (k_triple_sym; the library's own weights, K <= 32)
+    bool use_sym = false;
+    int nsym_work = 0;
+    size_t sym_lds = 0;
+    int *d_sym_work = nullptr;     // [nsym_work] ts | h1 << 16 | h2 << 20, most expensive first
+    int *d_perms = nullptr;        // [n_lt][8] orderings of each sorted l-triple
+    int *d_gorder = nullptr;       // [n_groups] run order of the mu3 groups of sorted l-triples
+    uint32_t *d_symtab = nullptr;  // V entries each permutation stores (k_triple_sym)
+    int *d_symoff = nullptr;       // [6 * 2 * 2 * 2 + 1]
     std::vector<int> h_lt_cnt, h_lt_l3;   // per l-triple: basis entries, l3 (work lists of batches)
     // mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_BF16)
     int prec = TP_PREC_FP64;
@@ -283,6 +293,10 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
                            sc.shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
                            b->EmaxLt8, b->Prows, sc.As, b->kl_pad);
+    } else if (b->use_sym) {
+        hipLaunchKernelGGL(k_triple_sym, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1, sel,
+                           flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
+                           b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad);
     } else if (b->use_mfma) {
         hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(1024), b->mfma_lds, st, b0, b1, sel, flip,
                            b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_work, b->Lw, b->EmaxLt,
@@ -1179,7 +1193,8 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
                     b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2,
-                    b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs};
+                    b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs, b->d_sym_work, b->d_perms, b->d_gorder,
+                    b->d_symtab, b->d_symoff};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1511,6 +1526,104 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         }
     }
 
+    // sorted l-triples for k_triple_sym: each ordered l-triple t maps to the
+    // sorted one ts (l1 >= l2 >= l3) by a permutation sigma, l_i = L_sigma(i);
+    // only for the library's own (permutation-symmetric) 3j weights
+    std::vector<int> sym_work, perms, gorder, symoff;
+    std::vector<uint32_t> symtab;
+    if (!hw && K <= 32 && nlt > 0) {
+        std::unordered_map<int, int> index_of;   // l1 | l2 << 8 | l3 << 16 -> lt
+        for (int t = 0; t < nlt; ++t)
+            index_of[(ltinfo[t].w & 0xffff) | (ltinfo[t].z << 16)] = t;
+        perms.assign((size_t)nlt * 8, 0);
+        gorder.assign((size_t)b->n_groups, 0);
+        bool ok = true;
+        std::vector<int> sorted_ts;
+        for (int t = 0; t < nlt && ok; ++t) {
+            const int l[3] = {ltinfo[t].w & 0xff, (ltinfo[t].w >> 8) & 0xff, ltinfo[t].z};
+            int Ls[3] = {l[0], l[1], l[2]};
+            std::sort(Ls, Ls + 3, [](int a, int c) { return a > c; });
+            const auto it = index_of.find(Ls[0] | (Ls[1] << 8) | (Ls[2] << 16));
+            if (it == index_of.end()) { ok = false; break; }
+            const int ts = it->second;
+            int sig[3];
+            bool used[3] = {false, false, false};
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    if (!used[j] && Ls[j] == l[i]) { sig[i] = j; used[j] = true; break; }
+            int pidx = -1;
+            for (int p = 0; p < 6; ++p)
+                if (kSymPerm[p][0] == sig[0] && kSymPerm[p][1] == sig[1] && kSymPerm[p][2] == sig[2]) pidx = p;
+            int &cnt = perms[(size_t)ts * 8];
+            if (cnt == 0) sorted_ts.push_back(ts);
+            if (cnt >= 6 || pidx < 0) { ok = false; break; }
+            perms[(size_t)ts * 8 + 1 + cnt] = t | (pidx << 24);
+            ++cnt;
+        }
+        const size_t lds = triple_sym_lds(K, 2 * L + 1, [&] {
+            int m = 0;
+            for (int t = 0; t < nlt; ++t) m = std::max(m, lt_entries[t]);
+            return m;
+        }());
+        int maxlds = 0;
+        if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
+            maxlds = 65536;
+        if (ok && lds <= (size_t)maxlds) {
+            const int nh = K > 16 ? 2 : 1;
+            std::vector<std::pair<double, int>> wl;
+            for (int ts : sorted_ts) {
+                const int g0 = ltinfo[ts].y, n3 = 2 * ltinfo[ts].z + 1;
+                // the groups of a chunk run together: largest first, so that a chunk's are alike
+                std::vector<std::pair<int, int>> ge;
+                double s1 = 0.0;
+                for (int u = 0; u < n3; ++u) {
+                    const int E = gstart[g0 + u + 1] - gstart[g0 + u];
+                    ge.push_back({-E, u});
+                }
+                std::stable_sort(ge.begin(), ge.end());
+                for (int u = 0; u < n3; ++u) gorder[(size_t)g0 + u] = ge[u].second;
+                for (int c0 = 0; c0 < n3; c0 += kSymChunk) s1 += (2 * (-ge[c0].first) + 3) / 4;   // slowest wave
+                const double s2 = ((n3 + 1) / 2) * 2.0 * (nh > 1 ? 2 : 1);
+                const double cost = 2.0 * s1 + s2 + 0.25 * perms[(size_t)ts * 8];
+                for (int h1 = 0; h1 < nh; ++h1)
+                    for (int h2 = 0; h2 < nh; ++h2) wl.push_back({-cost, ts | (h1 << 16) | (h2 << 20)});
+            }
+            std::stable_sort(wl.begin(), wl.end());
+            for (auto &w : wl) sym_work.push_back(w.second);
+            // per (permutation, K1 half, K2 half, K3 block): the tile entries
+            // V[K1][K2][K3] that are As[lt][kt] for the sorted k-triple
+            // kt = (k1, k2, k3), k_i = K_sigma(i), in kt order
+            for (int p = 0; p < 6; ++p)
+                for (int h1 = 0; h1 < 2; ++h1)
+                    for (int h2 = 0; h2 < 2; ++h2)
+                        for (int cb = 0; cb < 2; ++cb) {
+                            symoff.push_back((int)symtab.size());
+                            std::vector<std::pair<int, uint32_t>> ents;
+                            for (int a = 0; a < 16; ++a)
+                                for (int c = 0; c < 16; ++c)
+                                    for (int d = 0; d < 16; ++d) {
+                                        const int Kv[3] = {16 * h1 + a + 1, 16 * h2 + c + 1, 16 * cb + d + 1};
+                                        if (Kv[0] > K || Kv[1] > K || Kv[2] > K) continue;
+                                        const int k1 = Kv[kSymPerm[p][0]], k2 = Kv[kSymPerm[p][1]],
+                                                  k3 = Kv[kSymPerm[p][2]];
+                                        if (k1 < k2 || k2 < k3) continue;
+                                        const int kt = (k1 - 1) * k1 * (k1 + 1) / 6 + (k2 - 1) * k2 / 2 + (k3 - 1);
+                                        ents.push_back({kt, (uint32_t)(a * 256 + c * 16 + d) | ((uint32_t)kt << 12)});
+                                    }
+                            std::sort(ents.begin(), ents.end());
+                            for (auto &x : ents) symtab.push_back(x.second);
+                        }
+            symoff.push_back((int)symtab.size());
+            b->nsym_work = (int)sym_work.size();
+            b->sym_lds = lds;
+            b->use_sym = true;
+            if (lds > 65536)
+                (void)hipFuncSetAttribute((const void *)k_triple_sym, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds);
+            if (b->Lw == 0) b->Lw = 2 * L + 1;
+        }
+    }
+
     // padded dims; H and Bm
     b->R_pad = round_up(b->R, G2_BM);
     b->nlt_pad = round_up(nlt, 16);
@@ -1581,7 +1694,11 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     if ((rc = up(&b->d_H, H)) || (rc = up(&b->d_Bm, Bm)) || (rc = up(&b->d_ew, ew)) || (rc = up(&b->d_eidx, eidx)) ||
         (rc = up(&b->d_gstart, gstart)) || (rc = up(&b->d_ltinfo, ltinfo)) || (rc = up(&b->d_chunks, chunks)) ||
         (rc = up(&b->d_pairs, pairs)) || (rc = up(&b->d_sean0, b->sean0)) || (rc = up(&b->d_coff, b->coff)) ||
-        (b->use_mfma && (rc = up(&b->d_work, work))) || (rc = up(&b->d_ew8, ew8)) || (rc = up(&b->d_eidx8, eidx8)) ||
+        (b->use_mfma && (rc = up(&b->d_work, work))) ||
+        (b->use_sym && ((rc = up(&b->d_sym_work, sym_work)) || (rc = up(&b->d_perms, perms)) ||
+                        (rc = up(&b->d_gorder, gorder)) || (rc = up(&b->d_symtab, symtab)) ||
+                        (rc = up(&b->d_symoff, symoff)))) ||
+        (rc = up(&b->d_ew8, ew8)) || (rc = up(&b->d_eidx8, eidx8)) ||
         (rc = up(&b->d_gstart8, gstart8)) || (rc = up(&b->d_m2start, m2start)) || (rc = up(&b->d_TA, TA)) ||
         (rc = up(&b->d_TB, TB)) || (rc = up(&b->d_logtab, logtab))) {
         tp_basis_destroy(b);

@@ -1,0 +1,210 @@
+// tp_triple_sym.hpp -- FP64 triple products on sorted l-triples (included by tp_hip.hip).
+//
+// For even l1 + l2 + l3 the 3j symbol is unchanged by any permutation of its
+// columns, so
+//   As(k1,k2,k3; l1,l2,l3) = sum 3j(l1 m1 l2 m2 l3 m3) Re(c_k1,l1,m1 c_k2,l2,m2 c_k3,l3,m3)
+// (correlations.jl:165-174 with the weights of :116) is the same number for
+// every simultaneous permutation of (k1,k2,k3) and (l1,l2,l3).  The kernel
+// therefore works on the sorted l-triples s = (L1 >= L2 >= L3) only (125 of
+// the 505 at L = 18) and forms, for ALL ordered shell triples,
+//   V_s[K1][K2][K3] = Re sum_mu3 T[K1][K2][mu3] c_K3[L3,mu3],
+//   T[K1][K2][mu3]  = sum_m1 w(L1 m1 L2 m2 L3 mu3) c_K1[L1,m1] c_K2[L2,m2],  m2 = -m1-mu3,
+// then writes As[lt][kt] = V_s[k_sigma^-1(1)..] for each ordered l-triple lt
+// = sigma(s) and each sorted k-triple kt.  Full 16 x 16 (K1, K2) tiles: no
+// triangular k1 >= k2 >= k3 waste in the MFMA tiles, and T (the stage that
+// costs most) is formed once per sorted l-triple instead of once per
+// ordering (0.57 GFLOP of FP64 MFMA work per full-size evaluation against
+// 1.50 for k_triple_mfma).
+//
+// Workgroup = (sorted l-triple, K1 half h1, K2 half h2): 8 waves, two
+// workgroups per CU.  Per chunk of 8 mu3 groups (one per wave, the groups of
+// a chunk of similar size):
+//   stage 1: Tr, Ti (16 x 16 each) over the group's E entries, four MFMAs
+//            per fetch of four entries (w Re a . Re b, w Re a . Im b,
+//            w Im a . -Im b, w Im a . Re b); Tr, Ti -> LDS;
+//   stage 2: V(256 pairs x 32 K3) += [Tr | Ti] . [Re c_K3 ; -Im c_K3] over
+//            the chunk's 16 slots, in MFMA registers across chunks.
+// Then V goes through LDS to the As rows of the l-triple's orderings, each
+// ordering through a host table of the tile entries it stores.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+constexpr int kSymThreads = 512;
+constexpr int kSymChunk = 8;   // mu3 groups per stage-1 pass (one per wave)
+// The six permutations sigma of (0, 1, 2): an ordered l-triple (l1, l2, l3) =
+// (L_sigma(0), L_sigma(1), L_sigma(2)) of the sorted one.
+__host__ __device__ constexpr int kSymPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+// Dynamic LDS of k_triple_sym for order bound Lw = 2L + 1, K shells and at most
+// Emax entries per l-triple.
+static inline size_t triple_sym_lds(int K, int Lw, int Emax)
+{
+    return 16 * sizeof(double2) * 2 * (size_t)Lw + sizeof(double2) * (size_t)K * Lw +
+           (sizeof(double) + sizeof(uint32_t)) * (size_t)Emax + sizeof(int) * (2 * (size_t)Lw + 2) +
+           sizeof(double) * 2 * kSymChunk * 256 + 64;
+}
+
+// work[i] = ts | h1 << 16 | h2 << 20 (ts: index of the sorted l-triple in the
+// ordered list); perms[ts * 8] = number n of orderings, perms[ts * 8 + 1 + q]
+// = lt | p << 24 with p the index of the permutation sigma of ordering q in
+// kSymPerm (k_i = K_sigma(i)); gorder[first group of ts + u] = the mu3 group
+// run u-th; symtab[symoff[((p * 2 + h1) * 2 + h2) * 2 + cb] ..
+// symoff[.. + 1]) = the V entries of K3 block cb that ordering p stores:
+// (K1 - 16 h1 - 1) * 256 + (K2 - 16 h2 - 1) * 16 + (K3 - 16 cb - 1) | kt << 12,
+// in kt order.
+__global__ __launch_bounds__(kSymThreads) void k_triple_sym(const double2 *__restrict__ cb0,
+                                                           const double2 *__restrict__ cb1,
+                                                           const int *__restrict__ sel, int flip, int n_c, int K,
+                                                           const double *__restrict__ ew,
+                                                           const uint32_t *__restrict__ eidx,
+                                                           const int *__restrict__ gstart,
+                                                           const int4 *__restrict__ ltinfo,
+                                                           const int *__restrict__ work, const int *__restrict__ perms,
+                                                           const int *__restrict__ gorder,
+                                                           const uint32_t *__restrict__ symtab,
+                                                           const int *__restrict__ symoff, int Lw,
+                                                           double *__restrict__ As, int64_t ldA)
+{
+    extern __shared__ double2 sm[];
+    const double2 *__restrict__ coef = pick(cb0, cb1, sel, flip);
+    const int wk = work[blockIdx.x];
+    const int ts = wk & 0xffff, h1 = (wk >> 16) & 15, h2 = (wk >> 20) & 15;
+    const int4 li = ltinfo[ts];
+    const int L3 = li.z, L1 = li.w & 0xff, L2 = (li.w >> 8) & 0xff;
+    const int n1 = 2 * L1 + 1, n2 = 2 * L2 + 1, n3 = 2 * L3 + 1;
+    const int goff = li.y;
+    // LDS carve-up (sizes from the order bound Lw so that the layout is fixed)
+    double2 *Ab = sm;                                      // [n1][16]  c_K1[L1, m1], K1 = 16 h1 + i + 1
+    double2 *Bb = Ab + 16 * (size_t)Lw;                    // [n2][16]  c_K2[L2, m2]
+    double2 *Cb = Bb + 16 * (size_t)Lw;                    // [n3][K]   c_K3[L3, mu3]
+    double *Ws = reinterpret_cast<double *>(Cb + (size_t)K * Lw);   // [E] weights
+    const int e0 = gstart[goff], ne = gstart[goff + n3] - e0;
+    uint32_t *Is = reinterpret_cast<uint32_t *>(Ws + ne);           // [E] local (m1 index | m2 index << 16)
+    int *Gs = reinterpret_cast<int *>(Is + ne);                     // [n3 + 1] group starts
+    int *Od = Gs + (n3 + 1);                                        // [n3] run order of the groups
+    double *Tf = reinterpret_cast<double *>(
+        (reinterpret_cast<uintptr_t>(Od + n3) + 15) & ~(uintptr_t)15);   // [2 kSymChunk][256]; V staging later
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int co1 = L1 * (L1 - 1) / 2, co2 = L2 * (L2 - 1) / 2, co3 = L3 * (L3 - 1) / 2;   // compact offsets (even l)
+    const int K1b = 16 * h1, K2b = 16 * h2;
+    for (int x = tid; x < 16 * n1; x += kSymThreads) {
+        const int m = x >> 4, i = x & 15, k = K1b + i;
+        Ab[x] = k < K ? coef[(size_t)k * n_c + co1 + m] : make_double2(0.0, 0.0);
+    }
+    for (int x = tid; x < 16 * n2; x += kSymThreads) {
+        const int m = x >> 4, j = x & 15, k = K2b + j;
+        Bb[x] = k < K ? coef[(size_t)k * n_c + co2 + m] : make_double2(0.0, 0.0);
+    }
+    for (int x = tid; x < K * n3; x += kSymThreads) {
+        const int m = x / K, k = x - m * K;
+        Cb[x] = coef[(size_t)k * n_c + co3 + m];
+    }
+    for (int x = tid; x < ne; x += kSymThreads) {
+        Ws[x] = ew[e0 + x];
+        const uint32_t id = eidx[e0 + x];
+        Is[x] = (uint32_t)((int)(id & 0xffffu) - co1) | ((uint32_t)((int)(id >> 16) - co2) << 16);
+    }
+    for (int x = tid; x <= n3; x += kSymThreads) Gs[x] = gstart[goff + x] - e0;
+    for (int x = tid; x < n3; x += kSymThreads) Od[x] = gorder[goff + x];
+    __syncthreads();
+
+    const int ncb = K > 16 ? 2 : 1;                 // K3 column blocks
+    dbl4 acc[2][2];                                 // [row block rb = wave + 8 r][K3 block]
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[r][c] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    for (int c0 = 0; c0 < n3; c0 += kSymChunk) {
+        const int nu = min(kSymChunk, n3 - c0);
+        // ---- stage 1: this wave's group; k-step = 4 entries, the complex
+        // product as Tr += (w Re a) Re b - (w Im a) Im b, Ti += (w Re a) Im b + (w Im a) Re b
+        {
+            dbl4 tr = dbl4{0.0, 0.0, 0.0, 0.0}, ti = dbl4{0.0, 0.0, 0.0, 0.0};
+            if (wave < nu) {
+                const int u = Od[c0 + wave];
+                const int s = Gs[u], E = Gs[u + 1] - s;
+                const int nks = (E + 3) >> 2;
+#pragma unroll 2
+                for (int ks = 0; ks < nks; ++ks) {
+                    const int e = 4 * ks + lq;
+                    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+                    if (e < E) {
+                        const double w = Ws[s + e];
+                        const uint32_t id = Is[s + e];
+                        const double2 ca = Ab[(int)(id & 0xffffu) * 16 + l16];
+                        const double2 cbv = Bb[(int)(id >> 16) * 16 + l16];
+                        ar = w * ca.x;
+                        ai = w * ca.y;
+                        br = cbv.x;
+                        bi = cbv.y;
+                    }
+                    tr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, tr, 0, 0, 0);
+                    ti = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ti, 0, 0, 0);
+                    tr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, -bi, tr, 0, 0, 0);
+                    ti = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ti, 0, 0, 0);
+                }
+            }
+            // rows K1 = lq + 4 v, columns K2 = l16: pair p = 16 row + column
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int p = 16 * (lq + 4 * v) + l16;
+                Tf[(2 * wave) * 256 + p] = tr[v];
+                Tf[(2 * wave + 1) * 256 + p] = ti[v];
+            }
+        }
+        __syncthreads();
+        // ---- stage 2: slots 2u' (Re T) and 2u'+1 (Im T) of this chunk
+        const int nsl = 2 * nu, nks2 = (nsl + 3) >> 2;
+        for (int ks = 0; ks < nks2; ++ks) {
+            const int slot = 4 * ks + lq;
+            const bool ok = slot < nsl;
+            double bv[2] = {0.0, 0.0};
+            if (ok) {
+                const int m = Od[c0 + (slot >> 1)];
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) {
+                    const int k3 = 16 * cb + l16;
+                    if (cb < ncb && k3 < K) {
+                        const double2 c = Cb[m * K + k3];
+                        bv[cb] = (slot & 1) ? -c.y : c.x;
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const double av = ok ? Tf[slot * 256 + 16 * (wave + 8 * r) + l16] : 0.0;
+                acc[r][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[0], acc[r][0], 0, 0, 0);
+                if (ncb > 1) acc[r][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[1], acc[r][1], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- V to the As rows of every ordering of the l-triple (tables of the
+    // entries each ordering stores, in k-triple order)
+    const int np = perms[ts * 8];
+    for (int cb = 0; cb < ncb; ++cb) {
+        // V tile of K3 block cb: Vs[pair][k3 - 16 cb] (pairs 16 rb + lq + 4 v, k3 = l16)
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Tf[(16 * (wave + 8 * r) + lq + 4 * v) * 16 + l16] = acc[r][cb][v];
+        __syncthreads();
+        for (int q = 0; q < np; ++q) {
+            const int code = perms[ts * 8 + 1 + q];
+            const int64_t row = (int64_t)(code & 0xffffff) * ldA;
+            const int t = (((code >> 24) * 2 + h1) * 2 + h2) * 2 + cb;
+            const int o = symoff[t], n = symoff[t + 1] - o;
+            for (int x = tid; x < n; x += kSymThreads) {
+                const uint32_t ent = symtab[o + x];
+                As[row + (ent >> 12)] = Tf[ent & 4095u];
+            }
+        }
+        __syncthreads();
+    }
+}

@@ -26,6 +26,7 @@ __all__ = [
     "TPError", "build", "lib", "SphericalHarmonicsVolume", "HIP_init", "HipContext", "BasisTypeHip",
     "HIP_calculate_basis", "FullCorrelation_parallized", "energy", "DeviceState", "rotate_all_at_once",
     "intensity_from_atoms", "seanindex", "tripletFactor", "klength", "twoPhotons", "retrieveSolution",
+    "HistogramRef",
 ]
 
 
@@ -168,26 +169,53 @@ class BasisTypeHip:
         """The current precision mode (PREC_* code)."""
         return int(lib().tp_basis_get_precision(self.h))
 
+    def histogram(self, c3ref, K3_range=None):
+        """A device copy of the normalised histogram slices visited by energy()'s
+        walk (tp_ref_create), as a HistogramRef: pass it where a c3ref is
+        expected to skip any host work per call.  It holds the values of
+        c3ref at this call; later changes to the array are not seen by it."""
+        return HistogramRef(self, self.ref(c3ref, K3_range), K3_range)
+
     def ref(self, c3ref, K3_range=None):
-        """Device copy of the normalised histogram slices visited by energy()'s walk.
-        Cached by the array's contents (a 64-bit hash of its bytes), so an array
-        changed in place gets a fresh device copy."""
+        """The tp_ref handle for c3ref (a HistogramRef of this basis, or an
+        array).  Arrays are cached by their contents (a 64-bit hash of their
+        bytes), so an array changed in place gets a fresh device copy."""
+        if isinstance(c3ref, HistogramRef):
+            if c3ref.basis is not self:
+                raise ValueError("HistogramRef belongs to another basis")
+            if K3_range is not None and K3_range != c3ref.K3_range:
+                raise ValueError("HistogramRef was made for another K3_range")
+            return c3ref.h
         if K3_range is None:
             K3_range = range(1, self.K + 1)
         arr = np.asarray(c3ref, dtype=np.float64)
         if arr.ndim != 5 or arr.shape[0] != self.N or arr.shape[1] != 2 * self.N:
             raise ValueError("c3ref must be (N, 2N, Kr, Kr, Kr)")
-        flat = np.ascontiguousarray(arr.ravel(order="F"))
-        key = (_digest(flat), arr.shape, K3_range.start, K3_range.stop)
+        if arr.flags.c_contiguous or arr.flags.f_contiguous:
+            key = (_digest(arr.ravel(order="K")), arr.shape, arr.flags.c_contiguous, K3_range.start, K3_range.stop)
+        else:
+            key = (_digest(np.ascontiguousarray(arr)), arr.shape, True, K3_range.start, K3_range.stop)
         hit = self._refs.get(key)
         if hit is not None:
             return hit[0]
+        flat = np.ascontiguousarray(arr.ravel(order="F"))
         Kr = arr.shape[2]
         h = C.c_void_p()
         check(lib().tp_ref_create(self.ctx.h, self.h, flat.ctypes.data, Kr, K3_range.start, K3_range.stop - 1,
                                   C.byref(h)), "tp_ref_create")
         self._refs[key] = (h, None)
         return h
+
+
+class HistogramRef:
+    """A histogram resident on the device for one basis and K3_range (the
+    compressed c3ref of new_energy, cuda.jl:127-139).  Kept alive by the
+    basis; pass it as c3ref to energy(), DeviceState.energy() and the
+    annealing drivers."""
+
+    def __init__(self, basis, h, K3_range):
+        self.basis, self.h = basis, h
+        self.K3_range = K3_range if K3_range is not None else range(1, basis.K + 1)
 
 
 def HIP_calculate_basis(L, LMAX, N, K, lam=0.0, dq=0.0, forIntensity=True, ctx=None):
