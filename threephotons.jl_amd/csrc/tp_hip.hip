@@ -1582,7 +1582,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
                 }
                 std::stable_sort(ge.begin(), ge.end());
                 for (int u = 0; u < n3; ++u) gorder[(size_t)g0 + u] = ge[u].second;
-                for (int c0 = 0; c0 < n3; c0 += kSymChunk) s1 += (2 * (-ge[c0].first) + 3) / 4;   // slowest wave
+                for (int c0 = 0; c0 < n3; c0 += kSymWaves) s1 += (-ge[c0].first + 3) / 4;   // slowest wave
                 const double s2 = ((n3 + 1) / 2) * 2.0 * (nh > 1 ? 2 : 1);
                 const double cost = 2.0 * s1 + s2 + 0.25 * perms[(size_t)ts * 8];
                 for (int h1 = 0; h1 < nh; ++h1)

@@ -31,19 +31,20 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
-constexpr int kSymThreads = 512;
-constexpr int kSymChunk = 8;   // mu3 groups per stage-1 pass (one per wave)
+constexpr int kSymWaves = 16;                // waves per workgroup: also the mu3 groups of a stage-1 pass
+constexpr int kSymThreads = 64 * kSymWaves;
+constexpr bool kSymGauss = true;             // stage 1 with three real products per complex one
 // The six permutations sigma of (0, 1, 2): an ordered l-triple (l1, l2, l3) =
 // (L_sigma(0), L_sigma(1), L_sigma(2)) of the sorted one.
-__host__ __device__ constexpr int kSymPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+constexpr int kSymPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
 // Dynamic LDS of k_triple_sym for order bound Lw = 2L + 1, K shells and at most
 // Emax entries per l-triple.
-static inline size_t triple_sym_lds(int K, int Lw, int Emax)
+static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves)
 {
     return 16 * sizeof(double2) * 2 * (size_t)Lw + sizeof(double2) * (size_t)K * Lw +
            (sizeof(double) + sizeof(uint32_t)) * (size_t)Emax + sizeof(int) * (2 * (size_t)Lw + 2) +
-           sizeof(double) * 2 * kSymChunk * 256 + 64;
+           sizeof(double) * 2 * NW * 256 + 64;
 }
 
 // work[i] = ts | h1 << 16 | h2 << 20 (ts: index of the sorted l-triple in the
@@ -54,7 +55,8 @@ static inline size_t triple_sym_lds(int K, int Lw, int Emax)
 // symoff[.. + 1]) = the V entries of K3 block cb that ordering p stores:
 // (K1 - 16 h1 - 1) * 256 + (K2 - 16 h2 - 1) * 16 + (K3 - 16 cb - 1) | kt << 12,
 // in kt order.
-__global__ __launch_bounds__(kSymThreads) void k_triple_sym(const double2 *__restrict__ cb0,
+template <int NW, bool G3>
+__global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restrict__ cb0,
                                                            const double2 *__restrict__ cb1,
                                                            const int *__restrict__ sel, int flip, int n_c, int K,
                                                            const double *__restrict__ ew,
@@ -85,46 +87,49 @@ __global__ __launch_bounds__(kSymThreads) void k_triple_sym(const double2 *__res
     int *Gs = reinterpret_cast<int *>(Is + ne);                     // [n3 + 1] group starts
     int *Od = Gs + (n3 + 1);                                        // [n3] run order of the groups
     double *Tf = reinterpret_cast<double *>(
-        (reinterpret_cast<uintptr_t>(Od + n3) + 15) & ~(uintptr_t)15);   // [2 kSymChunk][256]; V staging later
+        (reinterpret_cast<uintptr_t>(Od + n3) + 15) & ~(uintptr_t)15);   // [2 NW][256]; V staging later
+    constexpr int NT = 64 * NW, NR = 16 / NW;   // threads; stage-2 row blocks per wave
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int co1 = L1 * (L1 - 1) / 2, co2 = L2 * (L2 - 1) / 2, co3 = L3 * (L3 - 1) / 2;   // compact offsets (even l)
     const int K1b = 16 * h1, K2b = 16 * h2;
-    for (int x = tid; x < 16 * n1; x += kSymThreads) {
+    for (int x = tid; x < 16 * n1; x += NT) {
         const int m = x >> 4, i = x & 15, k = K1b + i;
         Ab[x] = k < K ? coef[(size_t)k * n_c + co1 + m] : make_double2(0.0, 0.0);
     }
-    for (int x = tid; x < 16 * n2; x += kSymThreads) {
+    for (int x = tid; x < 16 * n2; x += NT) {
         const int m = x >> 4, j = x & 15, k = K2b + j;
         Bb[x] = k < K ? coef[(size_t)k * n_c + co2 + m] : make_double2(0.0, 0.0);
     }
-    for (int x = tid; x < K * n3; x += kSymThreads) {
+    for (int x = tid; x < K * n3; x += NT) {
         const int m = x / K, k = x - m * K;
         Cb[x] = coef[(size_t)k * n_c + co3 + m];
     }
-    for (int x = tid; x < ne; x += kSymThreads) {
+    for (int x = tid; x < ne; x += NT) {
         Ws[x] = ew[e0 + x];
         const uint32_t id = eidx[e0 + x];
         Is[x] = (uint32_t)((int)(id & 0xffffu) - co1) | ((uint32_t)((int)(id >> 16) - co2) << 16);
     }
-    for (int x = tid; x <= n3; x += kSymThreads) Gs[x] = gstart[goff + x] - e0;
-    for (int x = tid; x < n3; x += kSymThreads) Od[x] = gorder[goff + x];
+    for (int x = tid; x <= n3; x += NT) Gs[x] = gstart[goff + x] - e0;
+    for (int x = tid; x < n3; x += NT) Od[x] = gorder[goff + x];
     __syncthreads();
 
     const int ncb = K > 16 ? 2 : 1;                 // K3 column blocks
-    dbl4 acc[2][2];                                 // [row block rb = wave + 8 r][K3 block]
+    dbl4 acc[NR][2];                                // [row block rb = wave + NW r][K3 block]
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int r = 0; r < NR; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc[r][c] = dbl4{0.0, 0.0, 0.0, 0.0};
 
-    for (int c0 = 0; c0 < n3; c0 += kSymChunk) {
-        const int nu = min(kSymChunk, n3 - c0);
-        // ---- stage 1: this wave's group; k-step = 4 entries, the complex
-        // product as Tr += (w Re a) Re b - (w Im a) Im b, Ti += (w Re a) Im b + (w Im a) Re b
+    for (int c0 = 0; c0 < n3; c0 += NW) {
+        const int nu = min(NW, n3 - c0);
+        // ---- stage 1: this wave's group, k-step = 4 entries; with G3 the
+        // complex product as P1 = (w Re a) Re b, P2 = (w Im a) Im b,
+        // P3 = (w (Re a + Im a)) (Re b + Im b): Tr = P1 - P2, Ti = P3 - P1 - P2;
+        // otherwise Tr += (w Re a) Re b - (w Im a) Im b, Ti += (w Re a) Im b + (w Im a) Re b
         {
-            dbl4 tr = dbl4{0.0, 0.0, 0.0, 0.0}, ti = dbl4{0.0, 0.0, 0.0, 0.0};
+            dbl4 p1 = dbl4{0.0, 0.0, 0.0, 0.0}, p2 = dbl4{0.0, 0.0, 0.0, 0.0}, p3 = dbl4{0.0, 0.0, 0.0, 0.0};
             if (wave < nu) {
                 const int u = Od[c0 + wave];
                 const int s = Gs[u], E = Gs[u + 1] - s;
@@ -143,18 +148,24 @@ __global__ __launch_bounds__(kSymThreads) void k_triple_sym(const double2 *__res
                         br = cbv.x;
                         bi = cbv.y;
                     }
-                    tr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, tr, 0, 0, 0);
-                    ti = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ti, 0, 0, 0);
-                    tr = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, -bi, tr, 0, 0, 0);
-                    ti = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ti, 0, 0, 0);
+                    if (G3) {
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, bi, p2, 0, 0, 0);
+                        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar + ai, br + bi, p3, 0, 0, 0);
+                    } else {
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, p2, 0, 0, 0);
+                        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, -bi, p1, 0, 0, 0);
+                        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, p2, 0, 0, 0);
+                    }
                 }
             }
             // rows K1 = lq + 4 v, columns K2 = l16: pair p = 16 row + column
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int p = 16 * (lq + 4 * v) + l16;
-                Tf[(2 * wave) * 256 + p] = tr[v];
-                Tf[(2 * wave + 1) * 256 + p] = ti[v];
+                Tf[(2 * wave) * 256 + p] = G3 ? p1[v] - p2[v] : p1[v];
+                Tf[(2 * wave + 1) * 256 + p] = G3 ? p3[v] - p1[v] - p2[v] : p2[v];
             }
         }
         __syncthreads();
@@ -176,8 +187,8 @@ __global__ __launch_bounds__(kSymThreads) void k_triple_sym(const double2 *__res
                 }
             }
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const double av = ok ? Tf[slot * 256 + 16 * (wave + 8 * r) + l16] : 0.0;
+            for (int r = 0; r < NR; ++r) {
+                const double av = ok ? Tf[slot * 256 + 16 * (wave + NW * r) + l16] : 0.0;
                 acc[r][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[0], acc[r][0], 0, 0, 0);
                 if (ncb > 1) acc[r][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[1], acc[r][1], 0, 0, 0);
             }
@@ -191,16 +202,16 @@ __global__ __launch_bounds__(kSymThreads) void k_triple_sym(const double2 *__res
     for (int cb = 0; cb < ncb; ++cb) {
         // V tile of K3 block cb: Vs[pair][k3 - 16 cb] (pairs 16 rb + lq + 4 v, k3 = l16)
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < NR; ++r)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Tf[(16 * (wave + 8 * r) + lq + 4 * v) * 16 + l16] = acc[r][cb][v];
+            for (int v = 0; v < 4; ++v) Tf[(16 * (wave + NW * r) + lq + 4 * v) * 16 + l16] = acc[r][cb][v];
         __syncthreads();
         for (int q = 0; q < np; ++q) {
             const int code = perms[ts * 8 + 1 + q];
             const int64_t row = (int64_t)(code & 0xffffff) * ldA;
             const int t = (((code >> 24) * 2 + h1) * 2 + h2) * 2 + cb;
             const int o = symoff[t], n = symoff[t + 1] - o;
-            for (int x = tid; x < n; x += kSymThreads) {
+            for (int x = tid; x < n; x += NT) {
                 const uint32_t ent = symtab[o + x];
                 As[row + (ent >> 12)] = Tf[ent & 4095u];
             }
@@ -208,3 +219,6 @@ __global__ __launch_bounds__(kSymThreads) void k_triple_sym(const double2 *__res
         __syncthreads();
     }
 }
+
+// The configuration the library launches.
+#define k_triple_sym (k_triple_sym_t<kSymWaves, kSymGauss>)
