@@ -56,12 +56,30 @@ def lt_n3_sum(L):
     return s
 
 
-# HBM bytes per launch of the plain path's dominant kernel (k_triple_mfma) at
+def sorted_lt_flops(L, K):
+    """Algorithmic FLOP of the triple products on sorted l-triples (k_triple_sym)
+    for K shells: per sorted l-triple s = (L1 >= L2 >= L3), stage 1 forms T for
+    all K^2 ordered shell pairs over its E_s entries (one complex MAC, 8 flop,
+    each) and stage 2 the real part of the mu3 sum for all K^3 ordered shell
+    triples (2 real MACs, 4 flop, per mu3 value); E_s counts the (m1, m2) pairs
+    allowed by the selection rules (|m1 + m2| <= L3)."""
+    e_sum = n3_sum = 0
+    for l1 in range(0, L + 1, 2):
+        for l2 in range(0, l1 + 1, 2):
+            for l3 in range(0, l2 + 1, 2):
+                if l3 < l1 - l2:
+                    continue
+                n3_sum += 2 * l3 + 1
+                e_sum += sum(1 for m1 in range(-l1, l1 + 1) for m2 in range(-l2, l2 + 1) if abs(m1 + m2) <= l3)
+    return 8.0 * K * K * e_sum + 4.0 * K ** 3 * n3_sum
+
+
+# HBM bytes per launch of the plain path's dominant kernel (k_triple_sym) at
 # the full-size workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
 # dispatch), scaled by what the same counters read for a 256 MiB device copy
 # (FETCH_SIZE 0.5x, WRITE_SIZE 1.0x of the bytes moved); the summary with the
 # commands is profiles/hbm_counters_full_L18_K26_N32.md.
-TRIPLE_FETCH_KIB, TRIPLE_WRITE_KIB = 2478.1, 12924.8
+TRIPLE_FETCH_KIB, TRIPLE_WRITE_KIB = 1748.1, 13704.1
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
@@ -482,6 +500,7 @@ def ewald_leg(cfg, tp, start, c3ref, reps):
     ctx = tp.HIP_init()
     lam, dq = 2.5, np.pi / 38
     b = tp.HIP_calculate_basis(cfg["L"], cfg["LMAX"], cfg["N"], cfg["K"], lam, dq)
+    c3ref = b.histogram(c3ref)   # on the device before the timed region
     tp.energy(start, b, c3ref)
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -735,8 +754,10 @@ def main():
     n_m2 = cfg["L"] // 2 + 1
     flops_g2 = (8.0 * n_m * cfg["N"] + 4.0 * n_m2 * basis.R) * basis.klength
     K = cfg["K"]
-    # coupling-form triple products: complex MACs (8 flop) of stage 1 and stage 2
-    flops_tri = 8.0 * (K * (K + 1) // 2 * basis.basislen + basis.klength * lt_n3_sum(cfg["L"]))
+    # triple products on sorted l-triples (the form k_triple_sym executes); the
+    # coupling form over all 505 ordered l-triples is reported beside it
+    flops_tri = sorted_lt_flops(cfg["L"], K)
+    flops_coupling = 8.0 * (K * (K + 1) // 2 * basis.basislen + basis.klength * lt_n3_sum(cfg["L"]))
     achieved = flops_tri / (kern_ms[0] * 1e-3) / 1e12
     log(f"evals/s {evals_per_s:.1f}  stage ms: triple {kern_ms[0]:.4f} G {kern_ms[1]:.4f} c3+energy "
         f"{kern_ms[2]:.4f} final {kern_ms[3]:.4f}  E={E:.10g}")
@@ -816,7 +837,8 @@ def main():
         "config": {
             "workload": "plain FP64 c3 + Bayes energy: 1000 random point atoms (r=26), KMAX=38, L=18, K=26, "
                         "N=32 -> 3276 k-triples x 2048 bins, 505 l-triples; 1e9-count Poisson histogram",
-            "form": "coupling-form triple products; G = H.As and c3 = Bm.G on FP64 MFMA with the energy fused",
+            "form": "triple products on sorted l-triples (3j column symmetry); G = H.As and c3 = Bm.G on FP64 MFMA "
+                    "with the energy fused",
         },
         "mc_steps_per_s": steps_per_s,
         "mc_anneal": {"steps": args.anneal_steps, "temperature_probes": 50, "seconds": t_anneal,
@@ -834,11 +856,14 @@ def main():
                          "evals_per_s": args.steps / elt16, "energy_abs_err_vs_fp64": abs(Et16 - E),
                          "stage_ms": {"triple": kernt16[0], "gemm_G": kernt16[1], "c3_energy": kernt16[2],
                                       "final": kernt16[3]}}},
-        "roofline": {"bound": "mfma", "kernel": "k_triple_mfma (coupling-form triple products, FP64 MFMA)",
+        "roofline": {"bound": "mfma", "kernel": "k_triple_sym (triple products on the 125 sorted l-triples, "
+                                               "FP64 MFMA)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRIPLE_FETCH_KIB * 1024 * FETCH_SCALE + TRIPLE_WRITE_KIB * 1024 * WRITE_SCALE,
                      "flops_per_launch": flops_tri, "launch_ms": kern_ms[0],
+                     "coupling_form_flops": flops_coupling,
+                     "coupling_form_tflops": flops_coupling / (kern_ms[0] * 1e-3) / 1e12,
                      "c3sep_energy_tflops": flops_g2 / (kern_ms[2] * 1e-3) / 1e12,
                      "gemm_G_tflops": flops_g1 / (kern_ms[1] * 1e-3) / 1e12},
     }
@@ -863,8 +888,16 @@ def main():
     if ewald is not None:
         out["curved_ewald"] = ewald
     if not args.no_cpu:
-        nthreads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)
+        # every core this process may run on, but no more threads than the
+        # OpenMP setting of the machine (a GPU box grants its share of the
+        # host's cores and sets OMP_NUM_THREADS to that share)
+        affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        nthreads = min(affinity, omp) if omp > 0 else affinity
+        cb = cpu_baseline(cfg, start, c3ref, args.cpu_cols, nthreads)
+        cb["host"] = {"nproc": os.cpu_count(), "affinity_cores": affinity, "OMP_NUM_THREADS": omp or None,
+                      "threads_used": nthreads, "model": cpu_model()}
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 
