@@ -456,6 +456,27 @@ int tp_batch_history(tp_batch *B, double *Ehist, int n);
  * (w,x,y,z per even l >= 2; slot e = g*nprop + p), energies to E_out[ngroups*nprop].
  * No decision is taken. */
 int tp_batch_eval_quats(tp_batch *B, const double *quat, double *E_out);
+/* Evaluate given rotations of the current states (no decision taken), each
+ * even l >= 2 block of all K_g shells turned by a real rotation D_l acting on
+ * x = Umat(l) c (spherical_harmonics.jl:445-457), as get_MC_step applies its
+ * moves (determination.jl:232-241).  Slot e = g*nprop + p:
+ *   p < ngeo: D_l = exp(-theta[e][l/2-1] G_l) with G_l run g's real
+ *     skew-symmetric generator of block l ((2l+1)^2 doubles, row-major,
+ *     Frobenius norm <= 1), concatenated over l = 2, 4, .., L per run (gen
+ *     [ngroups][sum_l (2l+1)^2]): a rotation along the one-parameter subgroup
+ *     of G_l (e.g. the normalised Riemannian gradient of the energy on
+ *     SO(2l+1)); computed on the device to ~1e-16 (scaling and squaring of
+ *     the degree-12 Taylor polynomial); theta finite;
+ *   p >= ngeo: D_l = I + (cos b - 1)(u u^T + v v^T) + sin b (v u^T - u v^T)
+ *     from uvb[e][l/2-1] = (u[2L+1], v[2L+1], b) (u, v orthonormal, first
+ *     2l+1 entries used): the reference's plane rotation (utilities.jl:180-190).
+ * Energies (the batch's precision) -> E_out[ngroups*nprop].  No counterpart
+ * in the reference; L <= 22. */
+int tp_batch_eval_rotations(tp_batch *B, int ngeo, const double *gen, const double *theta, const double *uvb,
+                            double *E_out);
+/* Replace run g's current structure (its K_g shells, even l <= L entries of
+ * coeff) and its current energy, e.g. after a decision taken by the caller. */
+int tp_batch_set_state(tp_batch *B, int g, const double *coeff, int64_t ld, double E);
 
 /* Rotate the even l >= 2 blocks of nshell shells by quat[L/2][4] (one unit
  * quaternion per l, w,x,y,z) in FP64, through the real Wigner-D recurrence:

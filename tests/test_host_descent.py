@@ -72,3 +72,75 @@ def test_riemannian_direction_is_steepest_descent():
     eps = 1e-5
     fp, fm = f(gm.moved(c, om, eps)), f(gm.moved(c, om, -eps))
     assert (fp - fm) / (2 * eps) == pytest.approx(-n2, rel=1e-5)
+
+
+def test_expm_skew_is_the_exact_exponential():
+    """expm_skew (the device's scaling-and-squaring Taylor form) against the
+    exponential by eigen-decomposition, over angles from 1e-6 to 3 pi:
+    agreement and orthogonality to 1e-13, det 1."""
+    from tp_amd import descent
+    rng = np.random.default_rng(7)
+    for n in (5, 17, 37):
+        A = rng.normal(size=(n, n))
+        A = 0.5 * (A - A.T)
+        A /= np.linalg.norm(A)
+        w, V = np.linalg.eigh(1j * A)
+        for th in (1e-6, 0.01, 0.3, 1.0, 2.5, 3 * np.pi):
+            R = descent.expm_skew(-th * A)
+            ref = ((V * np.exp(1j * th * w)) @ V.conj().T).real
+            assert np.abs(R - ref).max() < 1e-13, (n, th)
+            assert np.abs(R @ R.T - np.eye(n)).max() < 1e-13
+            assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_geodesic_rotations_equal_angle_rotations():
+    """The generator form (what the device receives) gives the rotations of the
+    eigen form per candidate and per l; theta = 0 is the identity."""
+    from tp_amd import descent
+    gm = descent.GeodesicMoves(_B())
+    om = _skews(gm, 4)
+    gen, norms = gm.generators(om)
+    assert gen.size == sum((2 * l + 1) ** 2 for l in gm.ls)
+    assert np.allclose(norms, [np.linalg.norm(om[l]) for l in gm.ls])
+    theta = np.random.default_rng(5).uniform(0, 2, size=(6, len(gm.ls)))
+    theta[0] = 0.0
+    a, b = gm.geodesic_rotations(gen, theta), gm.angle_rotations(om, theta)
+    for l in gm.ls:
+        assert np.abs(a[l] - b[l]).max() < 1e-13
+        assert np.abs(a[l][0] - np.eye(2 * l + 1)).max() < 1e-15
+
+
+def test_plane_draws_are_rotations_of_the_reference_move():
+    """(u, v, b) rows: u, v orthonormal, the plane rotation orthogonal with det 1,
+    turning u by b towards v and leaving the complement fixed."""
+    from tp_amd import descent
+    gm = descent.GeodesicMoves(_B())
+    rng = np.random.default_rng(6)
+    st = np.array([0.3 * (l - 1) for l in gm.ls])
+    q = descent._plane_draws(len(gm.ls), gm.ls, st, rng, 5, 8)
+    assert q.shape == (5, len(gm.ls), 2 * 17 + 1)
+    for s in range(5):
+        for li, l in enumerate(gm.ls):
+            n = 2 * l + 1
+            u, v, b = q[s, li, :n], q[s, li, n:2 * n], q[s, li, 2 * n]
+            assert abs(u @ u - 1) < 1e-14 and abs(v @ v - 1) < 1e-14 and abs(u @ v) < 1e-14
+            R = descent.plane_rotation(u, v, b)
+            assert np.abs(R @ R.T - np.eye(n)).max() < 1e-14
+            assert np.abs(R @ u - (np.cos(b) * u + np.sin(b) * v)).max() < 1e-14
+            z = rng.normal(size=n)
+            z -= (z @ u) * u + (z @ v) * v
+            assert np.abs(R @ z - z).max() < 1e-14
+
+
+def test_move_log_pairs_every_shell_with_every_label():
+    from tp_amd import descent
+    log = descent.MoveLog(10)
+    f = np.arange(3 * 192, dtype=np.float32).reshape(3, 192)
+    t = np.arange(2 * 36, dtype=np.float32).reshape(2, 36)
+    log.add(f, t)
+    X, T = log.arrays()
+    assert X.shape == (6, 192) and T.shape == (6, 36)
+    assert np.array_equal(X[4], f[1]) and np.array_equal(T[4], t[1])
+    log.add(f, t)   # 4 rows of room left
+    assert log.n == 10 and log.full()
+    assert np.array_equal(log.X[9], f[0]) and np.array_equal(log.T[9], t[1])

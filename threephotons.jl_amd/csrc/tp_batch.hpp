@@ -136,6 +136,58 @@ constexpr int kRotShellChunk = 16;
 // quaternion (cos |b|/2, sin |b|/2, 0, 0), the label the move log records.
 // A Haar-random frame is Haar-random in any fixed orthonormal basis, so the
 // real basis used here serves as well as the reference's Umat basis.
+// x <- D x on the l-block of every shell of evaluation d (real basis x =
+// Umat(l) c, spherical_harmonics.jl:445-457): d.src -> d.coef; the l = 2
+// workgroup (li = 0) also copies the l = 0 entries.  D: n x n row-major in LDS.
+__device__ inline void block_rotate(const double *D, int l, int li, const EvalDesc &d, int n_c,
+                                    double2 (*xs)[2 * kRotMaxL + 1], double2 (*xo)[2 * kRotMaxL + 1])
+{
+    const int tid = threadIdx.x;
+    const int n = 2 * l + 1, off = l * (l - 1) / 2;
+    const double h = 0.70710678118654752440;
+    if (li == 0)
+        for (int k = tid; k < d.K; k += 256) d.coef[(size_t)k * n_c] = d.src[(size_t)k * n_c];
+    for (int k0 = 0; k0 < d.K; k0 += kRotShellChunk) {
+        const int nk = min(kRotShellChunk, d.K - k0);
+        for (int i = tid; i < nk * n; i += 256) {
+            const int kk = i / n, m = i % n - l;
+            const double2 *c = d.src + (size_t)(k0 + kk) * n_c + off;
+            const int am = m < 0 ? -m : m;
+            const double s = (am & 1) ? -1.0 : 1.0;
+            const double2 cp = c[l + am], cm = c[l - am];
+            double2 x;
+            if (m == 0) x = cp;
+            else if (m > 0) x = make_double2(h * (s * cp.x + cm.x), h * (s * cp.y + cm.y));
+            else x = make_double2(-h * (s * cp.y - cm.y), h * (s * cp.x - cm.x));   // i (s c_m - c_-m)/sqrt2
+            xs[kk][m + l] = x;
+        }
+        __syncthreads();
+        for (int i = tid; i < nk * n; i += 256) {
+            const int kk = i / n, r = i % n;
+            double ar = 0.0, ai = 0.0;
+            for (int c = 0; c < n; ++c) {
+                const double dv = D[r * n + c];
+                ar += dv * xs[kk][c].x;
+                ai += dv * xs[kk][c].y;
+            }
+            xo[kk][r] = make_double2(ar, ai);
+        }
+        __syncthreads();
+        for (int i = tid; i < nk * n; i += 256) {
+            const int kk = i / n, m = i % n - l;
+            const int am = m < 0 ? -m : m;
+            const double s = (am & 1) ? -1.0 : 1.0;
+            const double2 xp = xo[kk][l + am], xm = xo[kk][l - am];
+            double2 c;
+            if (m == 0) c = xp;
+            else if (m > 0) c = make_double2(h * s * (xp.x + xm.y), h * s * (xp.y - xm.x));   // s (x_m - i x_-m)/sqrt2
+            else c = make_double2(h * (xp.x - xm.y), h * (xp.y + xm.x));                     // (x_m + i x_-m)/sqrt2
+            d.coef[(size_t)(k0 + kk) * n_c + off + m + l] = c;
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict__ ev, double *__restrict__ quat,
                                                       int nl, int n_c, const GroupState *__restrict__ gs, int family,
                                                       int probe)
@@ -249,50 +301,89 @@ __global__ __launch_bounds__(256) void k_rotate_batch(const EvalDesc *__restrict
         }
         __syncthreads();
     }
-    const double *D = Dm[0];   // l is even
-    const int n = 2 * l + 1, off = l * (l - 1) / 2;
-    const double h = 0.70710678118654752440;
-    if (li == 0)
-        for (int k = tid; k < d.K; k += 256) d.coef[(size_t)k * n_c] = d.src[(size_t)k * n_c];
-    for (int k0 = 0; k0 < d.K; k0 += kRotShellChunk) {
-        const int nk = min(kRotShellChunk, d.K - k0);
-        for (int i = tid; i < nk * n; i += 256) {
-            const int kk = i / n, m = i % n - l;
-            const double2 *c = d.src + (size_t)(k0 + kk) * n_c + off;
-            const int am = m < 0 ? -m : m;
-            const double s = (am & 1) ? -1.0 : 1.0;
-            const double2 cp = c[l + am], cm = c[l - am];
-            double2 x;
-            if (m == 0) x = cp;
-            else if (m > 0) x = make_double2(h * (s * cp.x + cm.x), h * (s * cp.y + cm.y));
-            else x = make_double2(-h * (s * cp.y - cm.y), h * (s * cp.x - cm.x));   // i (s c_m - c_-m)/sqrt2
-            xs[kk][m + l] = x;
-        }
-        __syncthreads();
-        for (int i = tid; i < nk * n; i += 256) {
-            const int kk = i / n, r = i % n;
-            double ar = 0.0, ai = 0.0;
-            for (int c = 0; c < n; ++c) {
-                const double dv = D[r * n + c];
-                ar += dv * xs[kk][c].x;
-                ai += dv * xs[kk][c].y;
-            }
-            xo[kk][r] = make_double2(ar, ai);
-        }
-        __syncthreads();
-        for (int i = tid; i < nk * n; i += 256) {
-            const int kk = i / n, m = i % n - l;
-            const int am = m < 0 ? -m : m;
-            const double s = (am & 1) ? -1.0 : 1.0;
-            const double2 xp = xo[kk][l + am], xm = xo[kk][l - am];
-            double2 c;
-            if (m == 0) c = xp;
-            else if (m > 0) c = make_double2(h * s * (xp.x + xm.y), h * s * (xp.y - xm.x));   // s (x_m - i x_-m)/sqrt2
-            else c = make_double2(h * (xp.x - xm.y), h * (xp.y + xm.x));                     // (x_m + i x_-m)/sqrt2
-            d.coef[(size_t)(k0 + kk) * n_c + off + m + l] = c;
-        }
-        __syncthreads();
+    block_rotate(Dm[0], l, li, d, n_c, xs, xo);   // l is even: D^l is in Dm[0]
+}
+
+// Given rotations of a batch's proposals (tp_batch_eval_rotations), one
+// workgroup per (block l, evaluation e).  Rotations are given in the basis
+// x = Umat(l) c of the reference (spherical_harmonics.jl:445-457); the
+// real basis of block_rotate differs from it by the signs s_u = (-1)^u
+// (u > 0), -(-1)^|u| (u < 0), 1 (u = 0), so D is applied as S D S.
+//   slot p < ngeo of run g: D = exp(-theta G) with G run g's real
+//     skew-symmetric generator of block l (row-major, |G|_F <= 1) and
+//     theta = theta[e][li]: A = -theta G / 2^s with |A|_F <= 1/4, the
+//     degree-12 Taylor polynomial of exp(A) by Horner's rule (truncation
+//     below 1/4^13 / 13! < 1e-18 of |exp(A)|), squared s times;
+//   other slots: D = I + (cos b - 1)(u u^T + v v^T) + sin b (v u^T - u v^T)
+//     from uvb[e][li] = (u[n], v[n], b): a rotation by b in the plane (u, v).
+constexpr int kExpDegree = 12;
+
+__device__ inline void lds_matmul(const double *A, const double *B, double *C, int n)
+{
+    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+        const int r = idx / n, c = idx % n;
+        double acc = 0.0;
+        for (int j = 0; j < n; ++j) acc = fma(A[r * n + j], B[j * n + c], acc);
+        C[idx] = acc;
     }
+}
+
+__global__ __launch_bounds__(256) void k_rotation_batch(const EvalDesc *__restrict__ ev, int nl, int n_c, int ngeo,
+                                                        const double *__restrict__ gen, int64_t gstride,
+                                                        const double *__restrict__ theta,
+                                                        const double *__restrict__ uvb, int uvstride)
+{
+    constexpr int NM = (2 * kRotMaxL + 1) * (2 * kRotMaxL + 1);
+    __shared__ double Dm[NM], Am[NM], Tm[NM];
+    __shared__ double sg[2 * kRotMaxL + 1];
+    __shared__ double2 xs[kRotShellChunk][2 * kRotMaxL + 1], xo[kRotShellChunk][2 * kRotMaxL + 1];
+    const int li = blockIdx.x, e = blockIdx.y, l = 2 * (li + 1), n = 2 * l + 1;
+    const int tid = threadIdx.x;
+    const EvalDesc d = ev[e];
+    for (int r = tid; r < n; r += 256) {
+        const int u = r - l, a = u < 0 ? -u : u;
+        const double par = (a & 1) ? -1.0 : 1.0;
+        sg[r] = u > 0 ? par : (u < 0 ? -par : 1.0);
+    }
+    if (d.p < ngeo) {
+        int64_t goff = 0;
+        for (int j = 0; j < li; ++j) goff += (int64_t)(4 * j + 5) * (4 * j + 5);
+        const double *G = gen + (int64_t)d.group * gstride + goff;
+        const double th = theta[(int64_t)e * nl + li];
+        int s = 0;
+        for (double a = fabs(th); a > 0.25 && s < 64; a *= 0.5) ++s;
+        const double sc = -ldexp(th, -s);
+        for (int i = tid; i < n * n; i += 256) {
+            Am[i] = sc * G[i];
+            Dm[i] = (i / n == i % n ? 1.0 : 0.0) + Am[i] * (1.0 / kExpDegree);
+        }
+        __syncthreads();
+        for (int k = kExpDegree - 1; k >= 1; --k) {   // Dm <- I + A Dm / k
+            lds_matmul(Am, Dm, Tm, n);
+            __syncthreads();
+            const double ik = 1.0 / k;
+            for (int i = tid; i < n * n; i += 256) Dm[i] = (i / n == i % n ? 1.0 : 0.0) + Tm[i] * ik;
+            __syncthreads();
+        }
+        for (int q = 0; q < s; ++q) {
+            lds_matmul(Dm, Dm, Tm, n);
+            __syncthreads();
+            for (int i = tid; i < n * n; i += 256) Dm[i] = Tm[i];
+            __syncthreads();
+        }
+        for (int i = tid; i < n * n; i += 256) Dm[i] *= sg[i / n] * sg[i % n];
+    } else {
+        const double *q = uvb + ((int64_t)e * nl + li) * uvstride;
+        const double b = q[2 * n], cm = cos(b) - 1.0, sb = sin(b);
+        __syncthreads();
+        for (int idx = tid; idx < n * n; idx += 256) {
+            const int r = idx / n, c = idx % n;
+            const double ur = q[r], vr = q[n + r], uc = q[c], vc = q[n + c];
+            Dm[idx] = sg[r] * sg[c] * ((r == c ? 1.0 : 0.0) + cm * (ur * uc + vr * vc) + sb * (vr * uc - ur * vc));
+        }
+    }
+    __syncthreads();
+    block_rotate(Dm, l, li, d, n_c, xs, xo);
 }
 
 // ---------------------------------------------------------------------------

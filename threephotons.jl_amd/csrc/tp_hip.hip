@@ -462,12 +462,19 @@ struct tp_batch {
     double noise = 0.0;
     uint16_t *d_Xi = nullptr, *d_Hi[3] = {nullptr, nullptr, nullptr};
     float *d_Yi = nullptr;
+    // given rotations (tp_batch_eval_rotations), allocated on first use
+    double *d_gen = nullptr;       // [ngroups][sum_l (2l+1)^2] generators
+    double *d_theta = nullptr, *d_uvb = nullptr;
 };
 
 static void batch_free(tp_batch *B)
 {
     if (!B) return;
     if (B->ctx) (void)hipSetDevice(B->ctx->device);
+    void *geo[] = {B->d_gen, B->d_theta, B->d_uvb};
+    for (void *p : geo)
+        if (p) (void)hipFree(p);
+    B->d_gen = B->d_theta = B->d_uvb = nullptr;
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
     for (auto &e : B->ev)
@@ -3860,6 +3867,71 @@ int tp_batch_history(tp_batch *B, double *Ehist, int n)
     const int m = std::min(n, B->hist_cap);
     if (m > 0)
         TP_HIP(hipMemcpy(Ehist, B->d_Ehist, sizeof(double) * (size_t)m * B->ngroups, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int tp_batch_eval_rotations(tp_batch *B, int ngeo, const double *gen, const double *theta, const double *uvb,
+                            double *E_out)
+{
+    if (!B || !B->sweep || !E_out || ngeo < 0 || ngeo > B->nprop) return tp_fail(TP_E_ARG, "bad argument");
+    if (ngeo > 0 && (!gen || !theta)) return tp_fail(TP_E_ARG, "ngeo > 0 needs gen and theta");
+    if (ngeo < B->nprop && !uvb) return tp_fail(TP_E_ARG, "plane slots need uvb");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    const int L = B->b->L, nl = B->nl;
+    if (L > kRotMaxL) return tp_fail(TP_E_UNSUPPORTED, "given rotations need L <= 22");
+    int64_t gstride = 0;
+    for (int l = 2; l <= L; l += 2) gstride += (int64_t)(2 * l + 1) * (2 * l + 1);
+    const int uvstride = 2 * (2 * L + 1) + 1;
+    if (ngeo > 0)
+        for (int64_t i = 0; i < (int64_t)B->ne * nl; ++i)
+            if (!std::isfinite(theta[i])) return tp_fail(TP_E_ARG, "theta must be finite");
+    if (!B->d_gen) {
+        int rc;
+        if ((rc = dev_alloc(&B->d_gen, (size_t)B->ngroups * gstride)) ||
+            (rc = dev_alloc(&B->d_theta, (size_t)B->ne * nl)) ||
+            (rc = dev_alloc(&B->d_uvb, (size_t)B->ne * nl * uvstride)))
+            return rc;
+    }
+    hipStream_t st = B->ctx->stream;
+    if (ngeo > 0) {
+        TP_HIP(hipMemcpyAsync(B->d_gen, gen, sizeof(double) * (size_t)B->ngroups * gstride, hipMemcpyHostToDevice, st));
+        TP_HIP(hipMemcpyAsync(B->d_theta, theta, sizeof(double) * (size_t)B->ne * nl, hipMemcpyHostToDevice, st));
+    }
+    if (ngeo < B->nprop)
+        TP_HIP(hipMemcpyAsync(B->d_uvb, uvb, sizeof(double) * (size_t)B->ne * nl * uvstride, hipMemcpyHostToDevice,
+                              st));
+    hipLaunchKernelGGL(k_rotation_batch, dim3((unsigned)nl, (unsigned)B->ne), dim3(256), 0, st, B->d_ev, nl,
+                       B->b->n_c, ngeo, B->d_gen, gstride, B->d_theta, B->d_uvb, uvstride);
+    TP_HIP(hipGetLastError());
+    if (int rc = batch_enqueue_eval(B, st)) return rc;
+    TP_HIP(hipMemcpyAsync(E_out, B->d_E, sizeof(double) * B->ne, hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+int tp_batch_set_state(tp_batch *B, int g, const double *coeff, int64_t ld, double E)
+{
+    if (!B || !B->sweep || g < 0 || g >= B->ngroups || !coeff) return tp_fail(TP_E_ARG, "bad argument");
+    if (int rc = check_ctx(B->ctx)) return rc;
+    const tp_basis *b = B->b;
+    if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "ld shorter than LMAX^2");
+    const int K = B->h_gs[g].K;
+    std::vector<double2> h((size_t)K * b->n_c);
+    for (int k = 0; k < K; ++k)
+        for (int c = 0; c < b->n_c; ++c) {
+            const double *d = coeff + 2 * ((int64_t)k * ld + b->sean0[c]);
+            h[(size_t)k * b->n_c + c] = make_double2(d[0], d[1]);
+        }
+    // on the batch's stream, after the work already queued there
+    hipStream_t st = B->ctx->stream;
+    GroupState s;
+    TP_HIP(hipMemcpyAsync(&s, B->d_gs + g, sizeof(GroupState), hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    s.E = E;
+    TP_HIP(hipMemcpyAsync(B->d_cur + (size_t)g * B->cstride, h.data(), sizeof(double2) * h.size(),
+                          hipMemcpyHostToDevice, st));
+    TP_HIP(hipMemcpyAsync(B->d_gs + g, &s, sizeof(GroupState), hipMemcpyHostToDevice, st));
+    TP_HIP(hipStreamSynchronize(st));
     return 0;
 }
 

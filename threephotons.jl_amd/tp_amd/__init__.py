@@ -484,7 +484,8 @@ class AnnealBatch:
     def __init__(self, basis, c3refs, starts, seeds, nprop=64, hist_cap=0):
         self.basis, self.nprop = basis, nprop
         self.ngroups = n = len(c3refs)
-        self.Ks = [np.asarray(c3refs[g]).shape[2] for g in range(n)]
+        self.Ks = [(c3refs[g].K3_range.stop - 1) if isinstance(c3refs[g], HistogramRef)
+                   else np.asarray(c3refs[g]).shape[2] for g in range(n)]
         refs = (C.c_void_p * n)(*[basis.ref(c3refs[g], range(1, self.Ks[g] + 1)).value for g in range(n)])
         arrs = [_coeff_ptr(starts[g], basis, self.Ks[g]) for g in range(n)]
         self._starts = [a[0][:self.Ks[g]].copy() for g, a in enumerate(arrs)]
@@ -549,6 +550,33 @@ class AnnealBatch:
         E = np.zeros(self.ngroups * self.nprop)
         check(lib().tp_batch_eval_quats(self.h, q.ctypes.data, E.ctypes.data), "tp_batch_eval_quats")
         return E
+
+    def eval_rotations(self, ngeo, gen=None, theta=None, uvb=None):
+        """Energies of given rotations of the current states (tp_batch_eval_rotations):
+        slots p < ngeo turn block l by exp(-theta G_l) with gen (ngroups,
+        sum_l (2l+1)^2) run g's real skew-symmetric generators (row-major,
+        concatenated over l = 2..L, Frobenius norm <= 1) and theta
+        (ngroups*nprop, L/2); the other slots by the plane rotation of uvb
+        (ngroups*nprop, L/2, 2(2L+1)+1) = (u, v, b).  All in the basis
+        x = Umat(l) c."""
+        n, L = self.ngroups * self.nprop, self.basis.L
+        gsz = sum((2 * l + 1) ** 2 for l in range(2, L + 1, 2))
+        E = np.zeros(n)
+        args = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (gen, theta, uvb)]
+        if ngeo > 0 and (args[0] is None or args[0].size != self.ngroups * gsz):
+            raise ValueError("gen must hold ngroups x sum_l (2l+1)^2 doubles")
+        if args[1] is not None and args[1].shape != (n, L // 2):
+            raise ValueError("theta must be (ngroups*nprop, L/2)")
+        if args[2] is not None and args[2].shape != (n, L // 2, 2 * (2 * L + 1) + 1):
+            raise ValueError("uvb must be (ngroups*nprop, L/2, 2(2L+1)+1)")
+        ptr = [None if a is None else a.ctypes.data for a in args]
+        check(lib().tp_batch_eval_rotations(self.h, int(ngeo), *ptr, E.ctypes.data), "tp_batch_eval_rotations")
+        return E
+
+    def set_state(self, g, coeff, E):
+        """Run g's current structure and energy (tp_batch_set_state)."""
+        c = np.ascontiguousarray(coeff, dtype=np.complex128)
+        check(lib().tp_batch_set_state(self.h, int(g), c.ctypes.data, c.shape[1], float(E)), "tp_batch_set_state")
 
 
 def _ab_log(self, capacity):

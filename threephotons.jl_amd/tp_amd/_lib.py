@@ -133,6 +133,8 @@ def lib():
     L.tp_checkpoint_load.argtypes = [C.c_char_p, vp, vp, i64]
     L.tp_load_histograms.argtypes = [vp, i32, i32, i32, i32, vp]
     L.tp_energy_noalloc.argtypes = [vp, vp, vp, vp, i64, i32, C.POINTER(dbl)]
+    L.tp_batch_eval_rotations.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.tp_batch_set_state.argtypes = [vp, i32, vp, i64, dbl]
     _lib = L
     return L
 
@@ -227,7 +229,7 @@ EXPORTS = [
     "tp_sh_surface", "tp_sh_coeff", "tp_sph_bessel", "tp_sraar", "tp_pictures", "tp_hist_add_generated",
     "tp_density_surface", "tp_absolute_square", "tp_structure_from_atoms", "tp_two_photons_ewald",
     "tp_retrieve_solution_ewald", "tp_anneal_begin", "tp_anneal_steps", "tp_checkpoint_save", "tp_checkpoint_load",
-    "tp_load_histograms", "tp_energy_noalloc",
+    "tp_load_histograms", "tp_energy_noalloc", "tp_batch_eval_rotations", "tp_batch_set_state",
 ]
 
 PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3

@@ -166,3 +166,356 @@ def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, se
         if (target is not None and E <= target) or (time_limit is not None and now >= time_limit):
             break
     return {"coeff": c, "E": E, "history": hist, "step_lengths": steps, "accepted": accepted}
+
+
+# ---- learned proposals along the gradient geodesic ----------------------------
+#
+# A proposal turns every l-block of all shells by exp(-theta_l Omega_l / |Omega_l|)
+# (the Riemannian steepest descent of that block on SO(2l+1), unit length) with
+# one angle theta_l per l.  The network (MlpProposer, BF16 kernels) maps one
+# shell's state and the temperature to the nine angles, as the angles of its
+# nine output quaternions; training pairs are (features of each shell,
+# angles of the accepted move) logged from the method's own runs.  Proposals
+# whose angles are drawn at random (log-normal about the current step scale)
+# are the same method without the network.  The reference's move, a rotation
+# of each l-block in a Haar-random 2-plane (utilities.jl:180-190), fills the
+# fallback slots.  All candidates of a step are scored in one batched
+# evaluation; the best is accepted by Metropolis on its FP64 energy.
+
+def shell_features(c, k, T, gnorm, moves):
+    """The network input for shell k: its 190 real-basis coefficients (even
+    l <= 18, Umat(l) c_k,l; scaled to norm 8), log10 T and log10 |Omega|."""
+    parts = [np.array([c[k, 0].real])]
+    for l in moves.ls:
+        parts.append((moves.U[l] @ c[k, moves.idx[l]]).real)
+    x = np.concatenate(parts)
+    n = np.linalg.norm(x)
+    out = np.zeros(192, np.float32)
+    out[:x.size] = 8.0 * x / n if n > 0 else x
+    out[190] = np.log10(max(T, 1e-12))
+    out[191] = np.log10(max(gnorm, 1e-300))
+    return out
+
+
+def angles_of(Y, nl):
+    """theta_l = 2 atan2(|(x, y, z)|, |w|) of the network's quaternion per l."""
+    q = Y[:, :4 * nl].reshape(-1, nl, 4).astype(np.float64)
+    return 2.0 * np.arctan2(np.linalg.norm(q[:, :, 1:], axis=2), np.abs(q[:, :, 0]))
+
+
+def angle_labels(theta):
+    """Training targets: the quaternion (cos theta/2, sin theta/2, 0, 0) per l."""
+    t = np.zeros((theta.shape[0], 36), np.float32)
+    nl = theta.shape[1]
+    t[:, 0:4 * nl:4] = np.cos(theta / 2)
+    t[:, 1:4 * nl:4] = np.sin(theta / 2)
+    return t
+
+
+class GeodesicMoves(GradientMoves):
+    """Rotations exp(-theta_l Omega_l / |Omega_l|) with per-candidate, per-l
+    angles, and draws of the reference's random move."""
+
+    def plane_rotations(self, stepsizes, rng, S):
+        """S draws of the reference's move (get_MC_step, determination.jl:232-241;
+        random_rotation_step, utilities.jl:180-190): per l a rotation by
+        beta ~ N(0, stepsize_l) in the plane of a random orthonormal 2-frame of
+        R^(2l+1), as complex-basis matrices {l: (S, n, n)}."""
+        out = {}
+        for li, l in enumerate(self.ls):
+            n = 2 * l + 1
+            g = rng.normal(size=(S, 2, n))
+            u = g[:, 0] / np.linalg.norm(g[:, 0], axis=1, keepdims=True)
+            v = g[:, 1] - (g[:, 1] * u).sum(1, keepdims=True) * u
+            v /= np.linalg.norm(v, axis=1, keepdims=True)
+            b = rng.normal(size=S) * stepsizes[li]
+            R = np.eye(n)[None] + (np.cos(b) - 1.0)[:, None, None] * (u[:, :, None] * u[:, None, :] +
+                                                                      v[:, :, None] * v[:, None, :]) \
+                + np.sin(b)[:, None, None] * (v[:, :, None] * u[:, None, :] - u[:, :, None] * v[:, None, :])
+            U = self.U[l]
+            out[l] = U.conj().T[None] @ R.astype(np.complex128) @ U[None]
+        return out
+
+    def angle_rotations(self, om, theta):
+        """Complex-basis rotations exp(-theta_l Omega_l / |Omega_l|), {l: (S, n, n)}
+        (by the eigen-decomposition of i Omega_l / |Omega_l|)."""
+        out = {}
+        for li, l in enumerate(self.ls):
+            A = om[l]
+            nA = np.sqrt((A * A).sum())
+            if nA == 0.0:
+                out[l] = np.broadcast_to(np.eye(2 * l + 1, dtype=np.complex128), (theta.shape[0], 2 * l + 1, 2 * l + 1))
+                continue
+            w, V = np.linalg.eigh(1j * A / nA)
+            W = self.U[l].conj().T @ V
+            ph = np.exp(1j * theta[:, li][:, None] * w[None, :])
+            out[l] = (W[None, :, :] * ph[:, None, :]) @ W.conj().T
+        return out
+
+    def generators(self, om):
+        """(gen, norms): Omega_l / |Omega_l|_F (zero where Omega_l = 0), real
+        row-major, concatenated over l as tp_batch_eval_rotations reads them,
+        and the norms |Omega_l|_F."""
+        parts, norms = [], []
+        for l in self.ls:
+            A = om[l]
+            nA = float(np.sqrt((A * A).sum()))
+            norms.append(nA)
+            parts.append((A / nA if nA > 0.0 else np.zeros_like(A)).ravel())
+        return np.concatenate(parts), np.array(norms)
+
+    def geodesic_rotations(self, gen, theta):
+        """Complex-basis rotations exp(-theta_l G_l), {l: (S, n, n)}, for the
+        generators gen of generators() (expm_skew, the device's algorithm)."""
+        out, off = {}, 0
+        for li, l in enumerate(self.ls):
+            n = 2 * l + 1
+            G = gen[off:off + n * n].reshape(n, n)
+            off += n * n
+            R = expm_skew(-theta[:, li][:, None, None] * G[None])
+            U = self.U[l]
+            out[l] = U.conj().T[None] @ R.astype(np.complex128) @ U[None]
+        return out
+
+    def rotate_all(self, c, rots, nshell):
+        """(S, nshell, LMAX^2): shells 1..nshell of c with every l-block turned by
+        rots[l][s]; entries outside even l <= L copied."""
+        S = next(iter(rots.values())).shape[0]
+        out = np.broadcast_to(c[None, :nshell], (S, nshell, c.shape[1])).copy()
+        for l, rot in rots.items():
+            ix = self.idx[l]
+            out[:, :, ix] = c[None, :nshell, ix] @ np.swapaxes(rot, 1, 2)
+        return out
+
+
+def expm_skew(A):
+    """exp(A) of real skew-symmetric matrices (..., n, n): scaled by 2^-s to
+    |A|_F <= 1/4 (s per matrix), the degree-12 Taylor polynomial by Horner's
+    rule, squared s times (the algorithm of the device's k_rotation_batch)."""
+    A = np.asarray(A, dtype=np.float64)
+    shp = A.shape
+    A = A.reshape((-1,) + shp[-2:])
+    n = shp[-1]
+    out = np.empty_like(A)
+    I = np.eye(n)
+    for i in range(A.shape[0]):
+        a, s = float(np.sqrt((A[i] * A[i]).sum())), 0
+        while a > 0.25 and s < 64:
+            a *= 0.5
+            s += 1
+        B = A[i] * 2.0 ** -s
+        X = I + B / 12.0
+        for k in range(11, 0, -1):
+            X = I + (B @ X) / k
+        for _ in range(s):
+            X = X @ X
+        out[i] = X
+    return out.reshape(shp)
+
+
+class MoveLog:
+    """Training pairs of the learned proposer, in preallocated arrays: X
+    (n, 192) float32 features, T (n, 36) float32 targets (angle quaternions)."""
+
+    def __init__(self, capacity):
+        self.X = np.zeros((capacity, 192), np.float32)
+        self.T = np.zeros((capacity, 36), np.float32)
+        self.n = 0
+
+    def add(self, feats, labels):
+        """Every row of labels with every row of feats (all pairs), as far as room is left."""
+        m = min(feats.shape[0] * labels.shape[0], self.X.shape[0] - self.n)
+        if m <= 0:
+            return
+        X = np.broadcast_to(feats[None, :, :], (labels.shape[0],) + feats.shape).reshape(-1, 192)[:m]
+        T = np.broadcast_to(labels[:, None, :], (labels.shape[0], feats.shape[0], 36)).reshape(-1, 36)[:m]
+        self.X[self.n:self.n + m] = X
+        self.T[self.n:self.n + m] = T
+        self.n += m
+
+    def full(self):
+        return self.n >= self.X.shape[0]
+
+    def arrays(self):
+        return self.X[:self.n], self.T[:self.n]
+
+
+def _plane_draws(nl, ls, stepsizes, rng, S, L):
+    """S draws of the reference's move per l (get_MC_step, determination.jl:232-241;
+    random_rotation_step, utilities.jl:180-190): (u, v, b) rows of
+    tp_batch_eval_rotations, u, v a random orthonormal 2-frame of R^(2l+1)
+    and b ~ N(0, stepsize_l)."""
+    width = 2 * (2 * L + 1) + 1
+    out = np.zeros((S, nl, width))
+    for li, l in enumerate(ls):
+        n = 2 * l + 1
+        g = rng.normal(size=(S, 2, n))
+        u = g[:, 0] / np.linalg.norm(g[:, 0], axis=1, keepdims=True)
+        v = g[:, 1] - (g[:, 1] * u).sum(1, keepdims=True) * u
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        out[:, li, :n] = u
+        out[:, li, n:2 * n] = v
+        out[:, li, 2 * n] = rng.normal(size=S) * stepsizes[li]
+    return out
+
+
+def plane_rotation(u, v, b):
+    """I + (cos b - 1)(u u^T + v v^T) + sin b (v u^T - u v^T) (utilities.jl:180-190)."""
+    return np.eye(u.size) + (np.cos(b) - 1.0) * (np.outer(u, u) + np.outer(v, v)) + \
+        np.sin(b) * (np.outer(v, u) - np.outer(u, v))
+
+
+class _Run:
+    """Host state of one run of learned_gradient_runs."""
+
+    def __init__(self, c, ref, K, T, stepsize0, ls):
+        self.c, self.ref, self.K, self.T = c, ref, K, T
+        self.E, self.g, self.scale = None, None, None
+        self.stepsizes = np.array([stepsize0 * (l - 1) for l in ls])
+        self.accepted, self.wins, self.hist = 0, {"geodesic": 0, "plane": 0}, []
+        self.done = False
+
+
+def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nrand=16, noise=0.3, spread=0.7,
+                          T0=0.0, decay=1.0, seed=0, targets=None, time_limit=None, log=None, stepsizefactor=1.01,
+                          initial_stepsize=np.pi / 4, batch=None, Ks=None):
+    """Concurrent annealing runs along the energy gradient, all candidates of
+    a step scored in one batched evaluation.  Run g has the coefficients
+    coeffs[g] (every shell is rotated), the histogram c3refs[g] of K_g shells
+    (walked over K3_range 1:K_g) and per step nlearn geodesic candidates
+    exp(-theta_l Omega_l / |Omega_l|) along its Riemannian steepest-descent
+    direction Omega (tp_energy_grad in FP64, after the state changed) and
+    nrand draws of the reference's random move (get_MC_step,
+    determination.jl:232-241).  The geodesic angles come from the network
+    (mlp; slot j reads the features of shell j K_g / nlearn, with log-normal
+    noise) or, without it, are log-normal about the run's step scale s (angle
+    per unit of gradient norm: s |Omega_l|), which follows the accepted
+    geodesic moves and halves after a rejection; the plane moves' step sizes
+    grow and shrink by stepsizefactor as in the reference
+    (determination.jl:476-489).  All runs' candidates are turned and scored
+    on the device (tp_batch_eval_rotations, the basis's mixed precision); the
+    best of each run is accepted by Metropolis on its FP64 energy with
+    T *= decay per step (evaluate_step, determination.jl:491-515) and handed
+    back to the batch (tp_batch_set_state).  log (a MoveLog) receives, for
+    every geodesic candidate that passes the Metropolis test on its own, the
+    features of every shell of its run with that candidate's angles.  A run
+    stops at targets[g] (energy); all stop at time_limit seconds.  batch: an
+    AnnealBatch of these runs with nlearn + nrand slots (made here, in FP16
+    hi/lo, when not given).  Returns per run {"coeff", "E", "history" [(t, E)],
+    "accepted", "wins"} and the step count."""
+    from . import AnnealBatch, HistogramRef, PREC_FP16X3
+    gm = GeodesicMoves(basis)
+    c3refs = [x if isinstance(x, HistogramRef) else basis.histogram(x, range(1, np.asarray(x).shape[2] + 1))
+              for x in c3refs]
+    rng = np.random.default_rng(seed)
+    G, nl, L = len(coeffs), len(gm.ls), basis.L
+    ncand = nlearn + nrand
+    if ncand < 3:
+        raise ValueError("need nlearn + nrand >= 3")
+    cs = [np.array(x.coeff if hasattr(x, "coeff") else x, dtype=np.complex128, copy=True) for x in coeffs]
+    if batch is None:
+        prec = basis.precision()
+        basis.set_precision(PREC_FP16X3)
+        try:
+            batch = AnnealBatch(basis, list(c3refs), cs, [seed + g for g in range(G)], nprop=ncand)
+        finally:
+            basis.set_precision(prec)
+    B = batch
+    if B.nprop != ncand or B.ngroups != G:
+        raise ValueError("batch must hold the runs with nlearn + nrand slots each")
+    Ks = list(B.Ks)
+    runs = [_Run(cs[g], c3refs[g], Ks[g], T0, initial_stepsize, gm.ls) for g in range(G)]
+    gsz = sum((2 * l + 1) ** 2 for l in gm.ls)
+    width = 2 * (2 * L + 1) + 1
+    gen = np.zeros((G, gsz))
+    theta = np.zeros((G * ncand, nl))
+    uvb = np.zeros((G * ncand, nl, width))
+    norms = np.zeros((G, nl))
+    t0 = time.perf_counter()
+    for r in runs:
+        r.E, r.g = energy_grad(r.c, basis, r.ref, range(1, r.K + 1))
+        r.hist.append((0.0, r.E))
+    for g, r in enumerate(runs):
+        B.set_state(g, r.c, r.E)
+    step = 0
+    for step in range(1, nsteps + 1):
+        feats = [None] * G
+        for g, r in enumerate(runs):
+            if r.g is None:
+                _, r.g = energy_grad(r.c, basis, r.ref, range(1, r.K + 1))
+            gen[g], norms[g] = gm.generators(gm.riemannian(r.c, r.g))
+            if r.scale is None:
+                gn = float(np.sqrt((norms[g] ** 2).sum()))
+                r.scale = 1.0 / gn if gn > 0 else 0.0
+            if mlp is not None or log is not None:
+                gn = float(np.sqrt((norms[g] ** 2).sum()))
+                feats[g] = np.stack([shell_features(r.c, k, r.T, gn, gm) for k in range(r.K)])
+        if nlearn:
+            if mlp is not None:
+                X = np.concatenate([feats[g][[j * runs[g].K // nlearn for j in range(nlearn)]] for g in range(G)])
+                th = angles_of(mlp.forward(X), nl) * np.exp(noise * rng.normal(size=(G * nlearn, nl)))
+            else:
+                th = np.concatenate([(r.scale * norms[g])[None, :] *
+                                     np.exp(spread * rng.normal(size=(nlearn, 1)) +
+                                            0.5 * spread * rng.normal(size=(nlearn, nl)))
+                                     for g, r in enumerate(runs)])
+            theta.reshape(G, ncand, nl)[:, :nlearn] = th.reshape(G, nlearn, nl)
+        if nrand:
+            for g, r in enumerate(runs):
+                uvb[g * ncand + nlearn:(g + 1) * ncand] = _plane_draws(nl, gm.ls, r.stepsizes, rng, nrand, L)
+        Ec = B.eval_rotations(nlearn, gen, theta, uvb).reshape(G, ncand)
+        now = time.perf_counter() - t0
+        for g, r in enumerate(runs):
+            if r.done:
+                continue
+            r.T *= decay
+            if log is not None and nlearn and not log.full():
+                d = Ec[g, :nlearn] - r.E
+                ok = d < 0.0
+                if r.T > 0.0:
+                    ok |= rng.uniform(size=nlearn) < np.exp(-np.maximum(d, 0.0) / r.T)
+                if ok.any():
+                    log.add(feats[g], angle_labels(theta[g * ncand:g * ncand + nlearn][ok]))
+            k = int(np.argmin(Ec[g]))
+            e = g * ncand + k
+            if k < nlearn:
+                rots = gm.geodesic_rotations(gen[g], theta[e:e + 1])
+            else:
+                rots = {}
+                for li, l in enumerate(gm.ls):
+                    n = 2 * l + 1
+                    q = uvb[e, li]
+                    R = plane_rotation(q[:n], q[n:2 * n], q[2 * n])
+                    rots[l] = (gm.U[l].conj().T @ R.astype(np.complex128) @ gm.U[l])[None]
+            cand = gm.rotate_all(r.c, rots, r.c.shape[0])[0]
+            En = _energy64(cand, basis, r.ref, range(1, r.K + 1))
+            dE = En - r.E
+            if dE < 0.0 or (r.T > 0.0 and rng.uniform() < np.exp(-dE / r.T)):
+                r.accepted += 1
+                r.c, r.E, r.g = cand, En, None
+                B.set_state(g, r.c, r.E)
+                if k < nlearn:
+                    r.wins["geodesic"] += 1
+                    nz = norms[g] > 0
+                    if nz.any():
+                        r.scale = float(np.median(theta[e][nz] / norms[g][nz]))
+                else:
+                    r.wins["plane"] += 1
+                    r.stepsizes *= stepsizefactor
+            else:
+                r.scale /= 2.0
+                r.stepsizes /= stepsizefactor
+            r.hist.append((time.perf_counter() - t0, r.E))
+            if targets is not None and targets[g] is not None and r.E <= targets[g]:
+                r.done = True
+        if all(r.done for r in runs) or (time_limit is not None and now >= time_limit):
+            break
+    return [{"coeff": r.c, "E": r.E, "history": r.hist, "accepted": r.accepted, "wins": r.wins} for r in runs], step
+
+
+def learned_gradient_anneal(coeff, basis, c3ref, nsteps, mlp=None, target=None, **kw):
+    """One run of learned_gradient_runs (its keyword arguments), as
+    gradient_anneal returns it plus the moves won by each kind."""
+    res, _ = learned_gradient_runs([coeff], basis, [c3ref], nsteps, mlp=mlp,
+                                   targets=None if target is None else [target], **kw)
+    return res[0]
