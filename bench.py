@@ -607,88 +607,124 @@ def gradient_leg(tp, basis, c3ref, start, plain, nsteps, ncand=16):
     return out
 
 
-def learned_leg(cfg, tp, workload, truth, basis, c3ref, start, plain, data_sweeps, train_steps, cmp_sweeps,
-                tte_sweeps, nprop=64, nlearn=48, seeds=tuple(range(1000, 1020))):
-    """Learned proposer: (1) move log from the concurrent runs' sweeps
-    (random proposals only), (2) device training of the MLP on the log,
-    (3) the same runs continued with and without the learned slots
-    (nlearn of nprop), (4) time-to-energy on the plain run's own problem
-    (K=26, its start structure and histogram) against the plain FP64 anneal."""
+def learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, samples, train_steps, cmp_steps,
+                tte_steps, nprop=64, nlearn=48, seeds=tuple(range(1000, 1020))):
+    """Learned proposer (tp_amd.descent.learned_gradient_runs), 48 of 64 slots
+    per run, the reference's random move in the other 16:
+    (1) move log from runs shaped as the concurrent-runs workload (20 runs,
+        K_r ~ U{10..26} from seeds 1000..1019, 64 candidates per step): greedy
+        gradient steps with random per-l profiles, in chunks of 50 steps from
+        fresh start structures (randomStartStructure(seed + 20 j)), until the
+        log holds `samples` pairs;
+    (2) device training of the MLP on the log;
+    (3) the 20 runs from randomStartStructure(seed), cmp_steps steps each of
+        the batched random-walk sweep (64 plane moves, graph replays; the
+        method without the learned part), the learned proposer, and the
+        learned proposer with random profiles in place of the network's;
+    (4) time-to-energy on the plain run's problem (K=26, its start structure
+        and histogram) against the plain FP64 anneal.
+    Batches and graphs are built before each timed region."""
+    from tp_amd import descent
     Ks = [int(np.random.default_rng(s).integers(10, 27)) for s in seeds]
-    starts = [workload.random_start_structure(truth, cfg["L"], s) for s in seeds]
-    refs = [np.ascontiguousarray(c3ref[:, :, :K, :K, :K]) for K in Ks]
-    basis.set_precision("fp16x3")
-    out = {"nprop": nprop, "nlearn": nlearn}
+    refs = [basis.histogram(np.ascontiguousarray(c3ref[:, :, :K, :K, :K]), range(1, K + 1)) for K in Ks]
+    nrand = nprop - nlearn
+    out = {"nprop": nprop, "nlearn": nlearn, "nrand": nrand, "K": Ks}
     # (1) data
-    B = tp.AnnealBatch(basis, refs, starts, seeds, nprop=nprop, hist_cap=data_sweeps + 8)
-    cap = 1_200_000
-    B.log(cap)
-    B.probe(0.1)
-    t0 = time.perf_counter()
-    B.run(data_sweeps, factor=SWEEP_FACTOR)
+    mlog = descent.MoveLog(samples)
+    steps, j, t0 = 0, 0, time.perf_counter()
+    while not mlog.full() and j < 40:
+        starts = [workload.random_start_structure(truth, cfg["L"], s + 20 * j) for s in seeds]
+        _, n = descent.learned_gradient_runs(starts, basis, refs, 50, nlearn=nlearn, nrand=nrand, seed=j, log=mlog)
+        steps += n
+        j += 1
+        log(f"learned: data chunk {j}: {steps} steps, {mlog.n} samples, {time.perf_counter() - t0:.1f}s")
     t_data = time.perf_counter() - t0
-    nlog = B.log_count()
-    out["data"] = {"sweeps": data_sweeps, "samples": nlog, "seconds": t_data,
-                   "samples_are": "(move that passes Metropolis alone, shell) pairs"}
-    log(f"learned: {nlog} logged samples from {data_sweeps} sweeps in {t_data:.1f}s")
+    out["data"] = {"samples": mlog.n, "steps": steps, "runs": len(seeds), "chunks": j, "seconds": t_data,
+                   "samples_are": "(shell features, per-l step profile) of the 8 best geodesic candidates per run "
+                                  "and step that pass the Metropolis test, one pair per shell of the run"}
     # (2) training
+    X, T = mlog.arrays()
     mlp = tp.MlpProposer(seed=1)
     t0 = time.perf_counter()
-    tr = mlp.train_log(B, train_steps, batch=512, lr=1e-3, seed=2)
+    tr = mlp.train(X, T, train_steps, batch=512, lr=1e-3, seed=2)
     t_train = time.perf_counter() - t0
+    prof = np.log(descent.ratios_of(mlp.forward(X[:4096]), len(range(2, cfg["L"] + 1, 2))))
     out["train"] = {"steps": train_steps, "batch": 512, "seconds": t_train, "samples_seen": train_steps * 512,
-                    "loss_first": float(tr[:20].mean()), "loss_last": float(tr[-20:].mean())}
+                    "loss_first": float(tr[:20].mean()), "loss_last": float(tr[-20:].mean()),
+                    "predicted_log_profile_mean": [float(x) for x in prof.mean(0)],
+                    "predicted_log_profile_std": [float(x) for x in prof.std(0)]}
     log(f"learned: trained {train_steps} steps in {t_train:.2f}s, loss {tr[:20].mean():.4f} -> {tr[-20:].mean():.4f}")
+    del mlog, X, T
+    # (3) the concurrent runs with and without the learned part
+    starts = [workload.random_start_structure(truth, cfg["L"], s) for s in seeds]
+    cmp = {}
+    basis.set_precision("fp16x3")
+    B = tp.AnnealBatch(basis, refs, starts, list(seeds), nprop=nprop, hist_cap=cmp_steps + 8)
+    B.probe(0.1)
+    B.run(0, factor=SWEEP_FACTOR)
+    E0 = np.array([B.group(g)["E"] for g in range(len(seeds))])
+    t0 = time.perf_counter()
+    B.run(cmp_steps, factor=SWEEP_FACTOR)
+    el = time.perf_counter() - t0
+    dE = np.array([B.group(g)["E"] for g in range(len(seeds))]) - E0
+    cmp["random_walk_batched"] = {"ms_per_step": 1e3 * el / cmp_steps, "mean_dE": float(dE.mean()),
+                                  "dE": [float(x) for x in dE],
+                                  "mode": "64 plane moves per run (|b| <= pi/4 Haar frame), best-of-64 + Metropolis "
+                                          "at the probed temperature, one graph replay per step"}
     del B
-    # (3) continuation with / without the network from the same start
-    res = {}
-    for name, nl in (("random", 0), ("learned", nlearn)):
-        Bc = tp.AnnealBatch(basis, refs, starts, seeds, nprop=nprop, hist_cap=cmp_sweeps + 8)
-        Bc.probe(0.1)
-        if nl:
-            Bc.set_proposer(mlp, nl, noise=0.25)
-        Bc.run(1, factor=SWEEP_FACTOR)
+    basis.set_precision("fp64")
+    for name, m in (("learned", mlp), ("random_profile", None)):
+        batch = _geodesic_batch(tp, basis, refs, starts, nprop)
         t0 = time.perf_counter()
-        Bc.run(cmp_sweeps, factor=SWEEP_FACTOR)
+        res, n = descent.learned_gradient_runs(starts, basis, refs, cmp_steps, mlp=m, nlearn=nlearn, nrand=nrand,
+                                               seed=77, batch=batch)
         el = time.perf_counter() - t0
-        st = [Bc.group(g) for g in range(len(Ks))]
-        h = Bc.history()
-        res[name] = {"ms_per_sweep": 1e3 * el / cmp_sweeps, "E_final": [x["E"] for x in st],
-                     "accepted": [x["nacc"] for x in st], "mean_dE": float(np.mean(h[cmp_sweeps] - h[0]))}
-    out["compare"] = res
-    out["compare"]["gain_mean_dE"] = res["learned"]["mean_dE"] / res["random"]["mean_dE"] \
-        if res["random"]["mean_dE"] != 0 else None
-    log(f"learned: mean dE over {cmp_sweeps} sweeps random {res['random']['mean_dE']:.3f} "
-        f"learned {res['learned']['mean_dE']:.3f}")
+        dE = np.array([r["E"] - r["history"][0][1] for r in res])
+        cmp[name] = {"ms_per_step": 1e3 * el / n, "mean_dE": float(dE.mean()), "dE": [float(x) for x in dE],
+                     "geodesic_wins": int(sum(r["wins"]["geodesic"] for r in res)),
+                     "plane_wins": int(sum(r["wins"]["plane"] for r in res))}
+    cmp["steps"] = cmp_steps
+    cmp["gain_vs_random_walk"] = cmp["learned"]["mean_dE"] / cmp["random_walk_batched"]["mean_dE"]
+    cmp["gain_vs_random_profile"] = cmp["learned"]["mean_dE"] / cmp["random_profile"]["mean_dE"]
+    out["compare"] = cmp
+    log(f"learned: mean dE over {cmp_steps} steps: random walk {cmp['random_walk_batched']['mean_dE']:.3f}, "
+        f"learned {cmp['learned']['mean_dE']:.3f}, random profile {cmp['random_profile']['mean_dE']:.3f}")
     # (4) time-to-energy on the plain run's problem
-    E_plain = plain["E_final"]
-    t_plain = plain["seconds"]
+    E_plain, t_plain = plain["E_final"], plain["seconds"]
     tte = {}
-    for name, nl in (("batched_fp16x3_random", 0), ("batched_fp16x3_learned", nlearn)):
-        Bt = tp.AnnealBatch(basis, [c3ref], [start], [1000], nprop=nprop, hist_cap=tte_sweeps + 8)
-        Bt.probe(0.1)
-        if nl:
-            Bt.set_proposer(mlp, nl, noise=0.25)
-        t0 = time.perf_counter()
-        Bt.run(tte_sweeps, factor=SWEEP_FACTOR)
-        el = time.perf_counter() - t0
-        h = Bt.history()[:tte_sweeps, 0]
-        per = el / tte_sweeps
-        tte[name] = {"sweeps": tte_sweeps, "ms_per_sweep": 1e3 * per, "E_final": float(h[-1]),
-                     "t_within_0.1pct": _time_to(h, lambda i: (i + 1) * per, E_plain * 1.001),
-                     "t_reach_plain_final": _time_to(h, lambda i: (i + 1) * per, E_plain)}
+    for name, m in (("learned", mlp), ("random_profile", None)):
+        batch = _geodesic_batch(tp, basis, [H], [start], nprop)
+        r = descent.learned_gradient_anneal(start, basis, H, tte_steps, mlp=m, nlearn=nlearn, nrand=nrand, seed=1,
+                                            batch=batch)
+        h = r["history"]
+        ts, Es = np.array([t for t, _ in h]), np.array([e for _, e in h])
+        tte[name] = {"steps": len(h) - 1, "ms_per_step": 1e3 * ts[-1] / max(1, len(h) - 1), "E_final": float(Es[-1]),
+                     "E_trace": [float(e) for e in Es],
+                     "t_within_0.1pct": _time_to(Es, lambda i: ts[i], E_plain * 1.001),
+                     "t_reach_plain_final": _time_to(Es, lambda i: ts[i], E_plain)}
     Ep = np.asarray(plain["E_trace"])
     per_step = t_plain / len(Ep)
     tte["plain_fp64"] = {"steps": len(Ep), "seconds": t_plain, "E_final": E_plain,
                          "t_within_0.1pct": _time_to(Ep, lambda i: (i + 1) * per_step, E_plain * 1.001),
                          "t_reach_plain_final": t_plain}
-    lf = tte["batched_fp16x3_learned"]["t_reach_plain_final"]
+    lf = tte["learned"]["t_reach_plain_final"]
     tte["speedup_reach_plain_final"] = (t_plain / lf) if lf else None
     tte["E0"] = float(plain["E0"])
     out["time_to_energy"] = tte
-    log(f"learned: time-to-energy {tte}")
-    basis.set_precision("fp64")
+    log(f"learned: reaches the plain run's final {E_plain:.4f} at "
+        f"{1e3 * lf if lf else float('nan'):.2f} ms (plain {1e3 * t_plain:.1f} ms; speedup "
+        f"{tte['speedup_reach_plain_final']}); E after {tte_steps} steps {tte['learned']['E_final']:.4f}")
     return out
+
+
+def _geodesic_batch(tp, basis, refs, starts, nprop):
+    """A batch of nprop slots per run for the learned proposer, made in FP16 hi/lo."""
+    prec = basis.precision()
+    basis.set_precision("fp16x3")
+    try:
+        return tp.AnnealBatch(basis, list(refs), list(starts), list(range(len(refs))), nprop=nprop)
+    finally:
+        basis.set_precision(prec)
 
 
 def main():
@@ -700,11 +736,13 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=1600)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-sweeps", type=int, default=20, help="sweeps of the concurrent-runs leg (0: skip)")
-    ap.add_argument("--learned-data-sweeps", type=int, default=100,
-                    help="sweeps logged for the learned proposer (0: skip the learned leg)")
-    ap.add_argument("--learned-train-steps", type=int, default=2000)
-    ap.add_argument("--learned-cmp-sweeps", type=int, default=40)
-    ap.add_argument("--learned-tte-sweeps", type=int, default=200)
+    ap.add_argument("--learned-samples", type=int, default=1_000_000,
+                    help="logged training pairs of the learned proposer (0: skip the learned leg)")
+    ap.add_argument("--learned-data-sweeps", type=int, default=None,
+                    help="0 skips the learned leg (the older form of --learned-samples 0)")
+    ap.add_argument("--learned-train-steps", type=int, default=4000)
+    ap.add_argument("--learned-cmp-steps", type=int, default=20)
+    ap.add_argument("--learned-tte-steps", type=int, default=40)
     ap.add_argument("--grad-steps", type=int, default=30, help="steps of the gradient-guided moves (0: skip)")
     ap.add_argument("--hist-pictures", type=int, default=32768, help="pictures of the histogram leg (0: skip)")
     ap.add_argument("--c2-reps", type=int, default=20, help="timed twoPhotons calls of the two-photon leg (0: skip)")
@@ -817,9 +855,11 @@ def main():
     if args.grad_steps > 0:
         grad = gradient_leg(tp, basis, H, start, plain, args.grad_steps)
     learned = None
-    if args.learned_data_sweeps > 0:
-        learned = learned_leg(cfg, tp, workload, truth, basis, c3ref, start, plain, args.learned_data_sweeps,
-                              args.learned_train_steps, args.learned_cmp_sweeps, args.learned_tte_sweeps)
+    if args.learned_data_sweeps == 0:
+        args.learned_samples = 0
+    if args.learned_samples > 0:
+        learned = learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, args.learned_samples,
+                              args.learned_train_steps, args.learned_cmp_steps, args.learned_tte_steps)
 
     out = {
         "metric": METRIC,

@@ -20,7 +20,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import PREC_BF16, PREC_FP16X3, PREC_FP64, AnnealParams, TPError, build, check, lib
+from ._lib import PREC_BF16, PREC_FP16X3, PREC_FP64, AnnealParams, TPError, build, check, lib, track
 
 __all__ = [
     "TPError", "build", "lib", "SphericalHarmonicsVolume", "HIP_init", "HipContext", "BasisTypeHip",
@@ -87,11 +87,14 @@ class HipContext:
         check(lib().tp_init(device, C.byref(h)), "tp_init")
         self.h = h
         self.device = device
+        track(self, 2)
 
     def close(self):
         if getattr(self, "h", None):
             lib().tp_finalize(self.h)
             self.h = None
+
+    _release = close
 
     def __del__(self):
         try:
@@ -139,6 +142,7 @@ class BasisTypeHip:
             check(lib().tp_basis_from_host(self.ctx.h, w.ctypes.data, ind.ctypes.data, w.shape[0], pp, npc, N, L,
                                            LMAX, K, float(lam), float(dq), C.byref(h)), "tp_basis_from_host")
         self.h = h
+        track(self, 1)
         self.L, self.LMAX, self.N, self.K, self.lam, self.dq = L, LMAX, N, K, float(lam), float(dq)
         self.lrange = range(0, L + 1, 2 if forIntensity else 1)
         bl, nl, kl, R = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
@@ -146,14 +150,17 @@ class BasisTypeHip:
         self.basislen, self.n_lt, self.klength, self.R = bl.value, nl.value, kl.value, R.value
         self._refs = {}
 
+    def _release(self):
+        for r in getattr(self, "_refs", {}).values():
+            lib().tp_ref_destroy(r[0])
+        self._refs = {}
+        if getattr(self, "h", None):
+            lib().tp_basis_destroy(self.h)
+            self.h = None
+
     def __del__(self):
         try:
-            for r in self._refs.values():
-                lib().tp_ref_destroy(r[0])
-            self._refs = {}
-            if getattr(self, "h", None):
-                lib().tp_basis_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -280,13 +287,17 @@ class DeviceState:
         h = C.c_void_p()
         check(lib().tp_state_create(basis.ctx.h, basis.h, c.ctypes.data, ld, nshell, C.byref(h)), "tp_state_create")
         self.h, self.basis, self.nshell, self.ld = h, basis, nshell, ld
+        track(self, 0)
         self.template = c[:nshell].copy()
+
+    def _release(self):
+        if getattr(self, "h", None):
+            lib().tp_state_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                lib().tp_state_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -382,12 +393,16 @@ class PhotonHistogram:
         h = C.c_void_p()
         check(lib().tp_hist_create(self.ctx.h, N, K2, K3, self.dq, C.byref(h)), "tp_hist_create")
         self.h = h
+        track(self, 0)
+
+    def _release(self):
+        if getattr(self, "h", None):
+            lib().tp_hist_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                lib().tp_hist_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -497,12 +512,16 @@ class AnnealBatch:
         check(lib().tp_batch_create(basis.ctx.h, basis.h, n, nprop, refs, ptrs, lds, sd, hist_cap, C.byref(h)),
               "tp_batch_create")
         self.h, self.hist_cap = h, hist_cap
+        track(self, 0)
+
+    def _release(self):
+        if getattr(self, "h", None):
+            lib().tp_batch_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                lib().tp_batch_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
@@ -629,13 +648,17 @@ class MlpProposer:
         h = C.c_void_p()
         check(lib().tp_mlp_create(self.ctx.h, int(seed), C.byref(h)), "tp_mlp_create")
         self.h = h
+        track(self, 0)
         self.nparam = lib().tp_mlp_nparam()
+
+    def _release(self):
+        if getattr(self, "h", None):
+            lib().tp_mlp_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                lib().tp_mlp_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 

@@ -1,7 +1,9 @@
 """Build and load libtp.so (the gfx950 C-ABI library declared in include/tp.h)."""
+import atexit
 import ctypes as C
 import os
 import subprocess
+import weakref
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))   # threephotons.jl_amd/
 REPO_ROOT = os.path.dirname(PKG_ROOT)
@@ -31,6 +33,36 @@ def build(verbose=False):
 
 
 _lib = None
+
+# Handles still alive when the interpreter exits are released here, before
+# module teardown clears the names their finalizers use (run objects first,
+# then bases, then contexts), so no device object outlives the process's
+# HIP runtime.
+_live = ([], [], [])
+
+
+def track(obj, tier):
+    """Release obj (its _release()) at exit in tier order: 0 runs, batches,
+    networks, states; 1 bases; 2 contexts."""
+    lst = _live[tier]
+    if len(lst) > 4096:
+        lst[:] = [w for w in lst if w() is not None]
+    lst.append(weakref.ref(obj))
+
+
+def _release_all():
+    for lst in _live:
+        for w in lst:
+            o = w()
+            if o is not None:
+                try:
+                    o._release()
+                except Exception:
+                    pass
+        lst.clear()
+
+
+atexit.register(_release_all)
 
 
 class TPError(RuntimeError):

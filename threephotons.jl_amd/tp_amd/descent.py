@@ -172,15 +172,19 @@ def gradient_anneal(coeff, basis, c3ref, nsteps, ncand=16, T0=0.0, decay=1.0, se
 #
 # A proposal turns every l-block of all shells by exp(-theta_l Omega_l / |Omega_l|)
 # (the Riemannian steepest descent of that block on SO(2l+1), unit length) with
-# one angle theta_l per l.  The network (MlpProposer, BF16 kernels) maps one
-# shell's state and the temperature to the nine angles, as the angles of its
-# nine output quaternions; training pairs are (features of each shell,
-# angles of the accepted move) logged from the method's own runs.  Proposals
-# whose angles are drawn at random (log-normal about the current step scale)
-# are the same method without the network.  The reference's move, a rotation
-# of each l-block in a Haar-random 2-plane (utilities.jl:180-190), fills the
-# fallback slots.  All candidates of a step are scored in one batched
-# evaluation; the best is accepted by Metropolis on its FP64 energy.
+# one angle theta_l per l.  A run keeps a step scale s (angle per unit of
+# gradient norm); a proposal's angles are theta_l = s |Omega_l| rho_l c with a
+# log-normal common factor c and a per-l profile rho (geometric mean 1).
+# Without the network rho is log-normal about 1; with it rho is the network's
+# prediction (MlpProposer, BF16 kernels) from one shell's state and the
+# temperature, rho_l = tan(phi_l / 2) of the rotation angle phi_l of its l-th
+# output quaternion, with log-normal noise.  Training pairs are (features of
+# each shell, the profile of each of the best candidates of a step that pass
+# the Metropolis test) logged from the method's own runs.  The reference's
+# move, a rotation of each l-block in a Haar-random 2-plane
+# (utilities.jl:180-190), fills the fallback slots.  All candidates of a step
+# are scored in one batched evaluation; the best is accepted by Metropolis on
+# its FP64 energy.
 
 def shell_features(c, k, T, gnorm, moves):
     """The network input for shell k: its 190 real-basis coefficients (even
@@ -201,6 +205,28 @@ def angles_of(Y, nl):
     """theta_l = 2 atan2(|(x, y, z)|, |w|) of the network's quaternion per l."""
     q = Y[:, :4 * nl].reshape(-1, nl, 4).astype(np.float64)
     return 2.0 * np.arctan2(np.linalg.norm(q[:, :, 1:], axis=2), np.abs(q[:, :, 0]))
+
+
+def ratios_of(Y, nl):
+    """rho_l = tan(phi_l / 2) = |(x, y, z)| / |w| of the network's quaternion per l."""
+    q = Y[:, :4 * nl].reshape(-1, nl, 4).astype(np.float64)
+    return np.linalg.norm(q[:, :, 1:], axis=2) / np.maximum(np.abs(q[:, :, 0]), 1e-300)
+
+
+def ratio_labels(rho):
+    """Training targets: the quaternion (1, rho, 0, 0) / sqrt(1 + rho^2) per l
+    (rotation angle 2 atan rho)."""
+    t = np.zeros((rho.shape[0], 36), np.float32)
+    nl = rho.shape[1]
+    h = 1.0 / np.sqrt(1.0 + rho * rho)
+    t[:, 0:4 * nl:4] = h
+    t[:, 1:4 * nl:4] = rho * h
+    return t
+
+
+def _gmean_normalised(r):
+    """Rows of r divided by their geometric mean."""
+    return r / np.exp(np.log(r).mean(axis=1, keepdims=True))
 
 
 def angle_labels(theta):
@@ -376,9 +402,9 @@ class _Run:
         self.done = False
 
 
-def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nrand=16, noise=0.3, spread=0.7,
-                          T0=0.0, decay=1.0, seed=0, targets=None, time_limit=None, log=None, stepsizefactor=1.01,
-                          initial_stepsize=np.pi / 4, batch=None, Ks=None):
+def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nrand=16, noise=0.25, spread=0.7,
+                          T0=0.0, decay=1.0, seed=0, targets=None, time_limit=None, log=None, log_top=8,
+                          stepsizefactor=1.01, initial_stepsize=np.pi / 4, batch=None, record=None):
     """Concurrent annealing runs along the energy gradient, all candidates of
     a step scored in one batched evaluation.  Run g has the coefficients
     coeffs[g] (every shell is rotated), the histogram c3refs[g] of K_g shells
@@ -386,22 +412,26 @@ def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nr
     exp(-theta_l Omega_l / |Omega_l|) along its Riemannian steepest-descent
     direction Omega (tp_energy_grad in FP64, after the state changed) and
     nrand draws of the reference's random move (get_MC_step,
-    determination.jl:232-241).  The geodesic angles come from the network
-    (mlp; slot j reads the features of shell j K_g / nlearn, with log-normal
-    noise) or, without it, are log-normal about the run's step scale s (angle
-    per unit of gradient norm: s |Omega_l|), which follows the accepted
-    geodesic moves and halves after a rejection; the plane moves' step sizes
-    grow and shrink by stepsizefactor as in the reference
-    (determination.jl:476-489).  All runs' candidates are turned and scored
-    on the device (tp_batch_eval_rotations, the basis's mixed precision); the
-    best of each run is accepted by Metropolis on its FP64 energy with
-    T *= decay per step (evaluate_step, determination.jl:491-515) and handed
-    back to the batch (tp_batch_set_state).  log (a MoveLog) receives, for
-    every geodesic candidate that passes the Metropolis test on its own, the
-    features of every shell of its run with that candidate's angles.  A run
+    determination.jl:232-241).  Geodesic angles: theta_l = s |Omega_l| rho_l
+    exp(spread N) with the run's step scale s and one N(0, 1) draw per
+    candidate; the profile rho (geometric mean 1) is the network's (mlp; slot
+    j reads the features of shell j K_g / nlearn) times exp(noise N_l), or
+    without it exp(spread N_l / 2).  s follows the geometric mean of
+    theta / (s |Omega|) of the accepted geodesic moves and halves after a
+    rejection; the plane moves' step sizes grow and shrink by stepsizefactor
+    as in the reference (determination.jl:476-489).  All runs' candidates are
+    turned and scored on the device (tp_batch_eval_rotations, the basis's
+    mixed precision); the best of each run is accepted by Metropolis on its
+    FP64 energy with T *= decay per step (evaluate_step,
+    determination.jl:491-515) and handed back to the batch
+    (tp_batch_set_state).  log (a MoveLog) receives, for each of the log_top
+    best geodesic candidates that pass the Metropolis test on their own, the
+    features of every shell of its run with that candidate's profile.  A run
     stops at targets[g] (energy); all stop at time_limit seconds.  batch: an
     AnnealBatch of these runs with nlearn + nrand slots (made here, in FP16
-    hi/lo, when not given).  Returns per run {"coeff", "E", "history" [(t, E)],
+    hi/lo, when not given).  record (a list) receives per step and run the
+    generator norms, step scale, geodesic angles, candidate energies and the
+    best slot.  Returns per run {"coeff", "E", "history" [(t, E)],
     "accepted", "wins"} and the step count."""
     from . import AnnealBatch, HistogramRef, PREC_FP16X3
     gm = GeodesicMoves(basis)
@@ -412,7 +442,13 @@ def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nr
     ncand = nlearn + nrand
     if ncand < 3:
         raise ValueError("need nlearn + nrand >= 3")
-    cs = [np.array(x.coeff if hasattr(x, "coeff") else x, dtype=np.complex128, copy=True) for x in coeffs]
+    cs, nrows = [], []
+    for x in coeffs:
+        a = np.asarray(x.coeff if hasattr(x, "coeff") else x, dtype=np.complex128)
+        nrows.append(a.shape[0])
+        c = np.zeros((max(a.shape[0], basis.K), a.shape[1]), np.complex128)   # shells past K_g stay zero
+        c[:a.shape[0]] = a
+        cs.append(c)
     if batch is None:
         prec = basis.precision()
         basis.set_precision(PREC_FP16X3)
@@ -453,13 +489,13 @@ def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nr
         if nlearn:
             if mlp is not None:
                 X = np.concatenate([feats[g][[j * runs[g].K // nlearn for j in range(nlearn)]] for g in range(G)])
-                th = angles_of(mlp.forward(X), nl) * np.exp(noise * rng.normal(size=(G * nlearn, nl)))
+                rho = _gmean_normalised(np.maximum(ratios_of(mlp.forward(X), nl), 1e-3)) * \
+                    np.exp(noise * rng.normal(size=(G * nlearn, nl)))
             else:
-                th = np.concatenate([(r.scale * norms[g])[None, :] *
-                                     np.exp(spread * rng.normal(size=(nlearn, 1)) +
-                                            0.5 * spread * rng.normal(size=(nlearn, nl)))
-                                     for g, r in enumerate(runs)])
-            theta.reshape(G, ncand, nl)[:, :nlearn] = th.reshape(G, nlearn, nl)
+                rho = np.exp(0.5 * spread * rng.normal(size=(G * nlearn, nl)))
+            common = np.exp(spread * rng.normal(size=(G * nlearn, 1)))
+            base = np.repeat(np.array([r.scale for r in runs])[:, None] * norms, nlearn, axis=0)
+            theta.reshape(G, ncand, nl)[:, :nlearn] = (base * rho * common).reshape(G, nlearn, nl)
         if nrand:
             for g, r in enumerate(runs):
                 uvb[g * ncand + nlearn:(g + 1) * ncand] = _plane_draws(nl, gm.ls, r.stepsizes, rng, nrand, L)
@@ -469,15 +505,21 @@ def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nr
             if r.done:
                 continue
             r.T *= decay
-            if log is not None and nlearn and not log.full():
+            nz = norms[g] > 0
+            if log is not None and nlearn and not log.full() and nz.all() and r.scale > 0:
                 d = Ec[g, :nlearn] - r.E
                 ok = d < 0.0
                 if r.T > 0.0:
                     ok |= rng.uniform(size=nlearn) < np.exp(-np.maximum(d, 0.0) / r.T)
-                if ok.any():
-                    log.add(feats[g], angle_labels(theta[g * ncand:g * ncand + nlearn][ok]))
+                top = [j for j in np.argsort(d, kind="stable")[:log_top] if ok[j]]
+                if top:
+                    ratio = theta[g * ncand + np.array(top)] / (r.scale * norms[g])[None, :]
+                    log.add(feats[g], ratio_labels(_gmean_normalised(ratio)))
             k = int(np.argmin(Ec[g]))
             e = g * ncand + k
+            if record is not None:
+                record.append({"step": step, "run": g, "norms": norms[g].copy(), "scale": r.scale, "E": r.E,
+                               "theta": theta[g * ncand:g * ncand + nlearn].copy(), "Ec": Ec[g].copy(), "best": k})
             if k < nlearn:
                 rots = gm.geodesic_rotations(gen[g], theta[e:e + 1])
             else:
@@ -496,9 +538,8 @@ def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nr
                 B.set_state(g, r.c, r.E)
                 if k < nlearn:
                     r.wins["geodesic"] += 1
-                    nz = norms[g] > 0
                     if nz.any():
-                        r.scale = float(np.median(theta[e][nz] / norms[g][nz]))
+                        r.scale = float(np.exp(np.log(theta[e][nz] / norms[g][nz]).mean()))
                 else:
                     r.wins["plane"] += 1
                     r.stepsizes *= stepsizefactor
@@ -510,7 +551,8 @@ def learned_gradient_runs(coeffs, basis, c3refs, nsteps, mlp=None, nlearn=48, nr
                 r.done = True
         if all(r.done for r in runs) or (time_limit is not None and now >= time_limit):
             break
-    return [{"coeff": r.c, "E": r.E, "history": r.hist, "accepted": r.accepted, "wins": r.wins} for r in runs], step
+    return [{"coeff": r.c[:nrows[g]], "E": r.E, "history": r.hist, "accepted": r.accepted, "wins": r.wins}
+            for g, r in enumerate(runs)], step
 
 
 def learned_gradient_anneal(coeff, basis, c3ref, nsteps, mlp=None, target=None, **kw):

@@ -13,6 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import SphericalHarmonicsVolume, check, lib, rotate
+from ._lib import track
 
 
 def euler_quat(theta, phi, gamma):
@@ -62,12 +63,16 @@ class StructureFit:
         check(lib().tp_fit_create(basis.ctx.h, basis.h, ref.ctypes.data, ref.shape[1], self.klo, self.khi,
                                   C.byref(h)), "tp_fit_create")
         self.h = h
+        track(self, 0)
+
+    def _release(self):
+        if getattr(self, "h", None):
+            lib().tp_fit_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:
-            if getattr(self, "h", None):
-                lib().tp_fit_destroy(self.h)
-                self.h = None
+            self._release()
         except Exception:
             pass
 
