@@ -303,16 +303,19 @@ def gradient_profile(ncalls, ncand=16, reps=10):
                       "energy_fp64": t_e64}), flush=True)
 
 
-def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=tuple(range(1000, 1020))):
+def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=tuple(range(1000, 1020)),
+              prec="fp16x3"):
     """Concurrent annealing runs scored together: run r (seed 1000+r) has
     K_r ~ U{10..26} shells (drawn from its seed), starts from
     randomStartStructure(seed) and scores against the full histogram cut to
     K_r shells; every sweep evaluates nprop rotation proposals per run (one
-    graph replay for all runs) in the FP16 hi/lo contraction."""
+    graph replay for all runs) in the FP16 hi/lo contraction, on compensated
+    FP16 triple products (prec fp16x3) or FP64 ones on the sorted l-triples
+    (fp16x3_t64)."""
     Ks = [int(np.random.default_rng(s).integers(10, 27)) for s in seeds]
     starts = [workload.random_start_structure(truth, cfg["L"], s) for s in seeds]
     refs = [np.ascontiguousarray(c3ref[:, :, :K, :K, :K]) for K in Ks]
-    basis.set_precision("fp16x3")
+    basis.set_precision(prec)
     t0 = time.perf_counter()
     B = tp.AnnealBatch(basis, refs, starts, seeds, nprop=nprop, hist_cap=nsweeps + 8)
     t_create = time.perf_counter() - t0
@@ -332,8 +335,9 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
     kl_total = cols
     flops = 2.0 * basis.R * basis.n_lt * kl_total
     tile_ms = tune["tile_ms"]["%dx%d/%s" % (tune["tile"][0], tune["tile"][1], tune["schedule"])]
-    log(f"batch: {len(Ks)} runs K={Ks}, {ne} evals ({cols} k-triple columns) per sweep: {ms_sweep:.2f} ms/sweep, "
-        f"{ne / (ms_sweep * 1e-3):.0f} evals/s; tune {tune}")
+    basis.set_precision("fp64")
+    log(f"batch ({prec}): {len(Ks)} runs K={Ks}, {ne} evals ({cols} k-triple columns) per sweep: "
+        f"{ms_sweep:.2f} ms/sweep, {ne / (ms_sweep * 1e-3):.0f} evals/s; tune {tune}")
     return {
         "runs": len(Ks), "K": Ks, "proposals_per_run": nprop, "evals_per_sweep": ne,
         "kt_columns_per_sweep": cols, "sweeps": nsweeps, "ms_per_sweep": ms_sweep,
@@ -343,8 +347,11 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
         "gemm16_tflops_algorithmic": flops / (tile_ms * 1e-3) / 1e12,
         "triple_ms": min(tune["order_ms"]), "gemm16_ms": tile_ms,
         "E0": E0, "E_final": [s["E"] for s in st], "accepted": [s["nacc"] for s in st],
-        "mode": "fp16x3 contraction, FP64 rotations/energy; random slots = the reference's plane rotation in a "
-                "Haar frame (|b| <= pi/4); best-of-nprop + Metropolis per sweep; angle factor %.2f/sweep" % SWEEP_FACTOR,
+        "precision": prec,
+        "mode": "%s: fp16x3 contraction on %s triple products, FP64 rotations/energy; random slots = the "
+                "reference's plane rotation in a Haar frame (|b| <= pi/4); best-of-nprop + Metropolis per sweep; "
+                "angle factor %.2f/sweep" % (prec, "FP64 (sorted l-triple)" if prec == "fp16x3_t64" else
+                                             "compensated FP16", SWEEP_FACTOR),
     }
 
 
@@ -608,7 +615,7 @@ def gradient_leg(tp, basis, c3ref, start, plain, nsteps, ncand=16):
 
 
 def learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, samples, train_steps, cmp_steps,
-                tte_steps, nprop=64, nlearn=48, seeds=tuple(range(1000, 1020))):
+                tte_steps, nprop=64, nlearn=48, seeds=tuple(range(1000, 1020)), prec="fp16x3"):
     """Learned proposer (tp_amd.descent.learned_gradient_runs), 48 of 64 slots
     per run, the reference's random move in the other 16:
     (1) move log from runs shaped as the concurrent-runs workload (20 runs,
@@ -623,18 +630,21 @@ def learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, samples
         learned proposer with random profiles in place of the network's;
     (4) time-to-energy on the plain run's problem (K=26, its start structure
         and histogram) against the plain FP64 anneal.
-    Batches and graphs are built before each timed region."""
+    Batches (in mixed precision prec) and graphs are built before each
+    timed region."""
     from tp_amd import descent
     Ks = [int(np.random.default_rng(s).integers(10, 27)) for s in seeds]
     refs = [basis.histogram(np.ascontiguousarray(c3ref[:, :, :K, :K, :K]), range(1, K + 1)) for K in Ks]
     nrand = nprop - nlearn
-    out = {"nprop": nprop, "nlearn": nlearn, "nrand": nrand, "K": Ks}
+    out = {"nprop": nprop, "nlearn": nlearn, "nrand": nrand, "K": Ks, "batch_precision": prec}
     # (1) data
     mlog = descent.MoveLog(samples)
     steps, j, t0 = 0, 0, time.perf_counter()
     while not mlog.full() and j < 40:
         starts = [workload.random_start_structure(truth, cfg["L"], s + 20 * j) for s in seeds]
-        _, n = descent.learned_gradient_runs(starts, basis, refs, 50, nlearn=nlearn, nrand=nrand, seed=j, log=mlog)
+        batch = _geodesic_batch(tp, basis, refs, starts, nprop, prec)
+        _, n = descent.learned_gradient_runs(starts, basis, refs, 50, nlearn=nlearn, nrand=nrand, seed=j, log=mlog,
+                                             batch=batch)
         steps += n
         j += 1
         log(f"learned: data chunk {j}: {steps} steps, {mlog.n} samples, {time.perf_counter() - t0:.1f}s")
@@ -658,7 +668,7 @@ def learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, samples
     # (3) the concurrent runs with and without the learned part
     starts = [workload.random_start_structure(truth, cfg["L"], s) for s in seeds]
     cmp = {}
-    basis.set_precision("fp16x3")
+    basis.set_precision(prec)
     B = tp.AnnealBatch(basis, refs, starts, list(seeds), nprop=nprop, hist_cap=cmp_steps + 8)
     B.probe(0.1)
     B.run(0, factor=SWEEP_FACTOR)
@@ -674,7 +684,7 @@ def learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, samples
     del B
     basis.set_precision("fp64")
     for name, m in (("learned", mlp), ("random_profile", None)):
-        batch = _geodesic_batch(tp, basis, refs, starts, nprop)
+        batch = _geodesic_batch(tp, basis, refs, starts, nprop, prec)
         t0 = time.perf_counter()
         res, n = descent.learned_gradient_runs(starts, basis, refs, cmp_steps, mlp=m, nlearn=nlearn, nrand=nrand,
                                                seed=77, batch=batch)
@@ -693,7 +703,7 @@ def learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, samples
     E_plain, t_plain = plain["E_final"], plain["seconds"]
     tte = {}
     for name, m in (("learned", mlp), ("random_profile", None)):
-        batch = _geodesic_batch(tp, basis, [H], [start], nprop)
+        batch = _geodesic_batch(tp, basis, [H], [start], nprop, prec)
         r = descent.learned_gradient_anneal(start, basis, H, tte_steps, mlp=m, nlearn=nlearn, nrand=nrand, seed=1,
                                             batch=batch)
         h = r["history"]
@@ -717,14 +727,14 @@ def learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, samples
     return out
 
 
-def _geodesic_batch(tp, basis, refs, starts, nprop):
-    """A batch of nprop slots per run for the learned proposer, made in FP16 hi/lo."""
-    prec = basis.precision()
-    basis.set_precision("fp16x3")
+def _geodesic_batch(tp, basis, refs, starts, nprop, prec):
+    """A batch of nprop slots per run for the learned proposer, made in mixed precision prec."""
+    old = basis.precision()
+    basis.set_precision(prec)
     try:
         return tp.AnnealBatch(basis, list(refs), list(starts), list(range(len(refs))), nprop=nprop)
     finally:
-        basis.set_precision(prec)
+        basis.set_precision(old)
 
 
 def main():
@@ -839,10 +849,13 @@ def main():
     log(f"fp16x3: evals/s {args.steps / el16:.1f}  stage ms {np.round(kern16, 4)}  |dE| {abs(E16 - E):.3g}  "
         f"anneal {t_anneal16:.2f}s -> {res16['E_final']:.10g}")
 
-    batch = None
+    batch = batch_t64 = None
+    batch_prec = "fp16x3"
     if args.batch_sweeps > 0:
         batch = batch_leg(cfg, tp, workload, truth, basis, c3ref, args.batch_sweeps)
-        basis.set_precision("fp64")
+        batch_t64 = batch_leg(cfg, tp, workload, truth, basis, c3ref, args.batch_sweeps, prec="fp16x3_t64")
+        if batch_t64["ms_per_sweep"] < batch["ms_per_sweep"]:
+            batch_prec = "fp16x3_t64"
     hist = hist_leg(tp, workload, args.hist_pictures) if args.hist_pictures > 0 else None
     c2 = c2_leg(tp, truth, basis, args.c2_reps) if args.c2_reps > 0 else None
     fitr = fit_leg(tp, truth, args.fit_num) if args.fit_num > 0 else None
@@ -859,7 +872,8 @@ def main():
         args.learned_samples = 0
     if args.learned_samples > 0:
         learned = learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, args.learned_samples,
-                              args.learned_train_steps, args.learned_cmp_steps, args.learned_tte_steps)
+                              args.learned_train_steps, args.learned_cmp_steps, args.learned_tte_steps,
+                              prec=batch_prec)
 
     out = {
         "metric": METRIC,
@@ -909,6 +923,7 @@ def main():
     }
     if batch is not None:
         out["batched"] = batch
+        out["batched_fp16x3_t64"] = batch_t64
     if learned is not None:
         out["learned"] = learned
     if grad is not None:

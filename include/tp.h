@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define TP_ABI_VERSION 2
+#define TP_ABI_VERSION 3
 
 #define TP_OK 0
 #define TP_E_ARG (-1)          /* invalid argument or shape */
@@ -92,14 +92,20 @@ int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *
  *                  terms and the log-likelihood sum stay FP64.
  *   TP_PREC_BF16   one BF16 product with FP32 accumulation (comparison mode).
  *   TP_PREC_TRIPLE16 compensated FP16 triple products (as in FP16X3) with the
- *                  FP64 contraction: the fastest single evaluation.
+ *                  FP64 contraction.
+ *   TP_PREC_FP16X3_T64 the FP16X3 contraction on FP64 triple products (the
+ *                  sorted l-triple form of the FP64 path): needs the
+ *                  library's own weights and K <= 32 (TP_E_UNSUPPORTED
+ *                  otherwise).
  * FP16X3, BF16 and TRIPLE16 compute the triple products on FP16 MFMA with
- * hi/lo compensation.  No counterpart in the reference (whose GPU path is FP32
+ * hi/lo compensation.  A batch (tp_batch_create) keeps the mode its basis
+ * had when it was made.  No counterpart in the reference (whose GPU path is FP32
  * end to end, cuda.jl:79-88). */
 #define TP_PREC_FP64 0
 #define TP_PREC_FP16X3 1
 #define TP_PREC_BF16 2
 #define TP_PREC_TRIPLE16 3
+#define TP_PREC_FP16X3_T64 4
 int tp_basis_set_precision(tp_basis *b, int mode);
 int tp_basis_get_precision(const tp_basis *b);
 
@@ -424,8 +430,9 @@ int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, cons
 void tp_batch_destroy(tp_batch *B);
 /* Time the candidate triple-product work orders and contraction kernels on
  * this batch (reps launches each) and keep the fastest.  ms_out[14] (may be
- * NULL): work orders 0 (by cost), 1 (evaluation-major), then tiles 128x64,
- * 64x64, 128x128, 64x128 with parallel row tiles, the same four with
+ * NULL): work orders 0 (by cost), 1 (evaluation-major; 1e30 in
+ * TP_PREC_FP16X3_T64, whose sorted l-triple form has one order), then tiles
+ * 128x64, 64x64, 128x128, 64x128 with parallel row tiles, the same four with
  * row-walking workgroups, and the same four with the row tiles partitioned
  * over the XCDs (1e30 where a shape does not divide by 8); config_out[4] (may
  * be NULL): chosen tile rows, tile columns, work order, schedule (1 parallel,

@@ -41,9 +41,9 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_loader():
     lib = ctypes.CDLL(_lib_path())
     lib.tp_abi_version.restype = ctypes.c_int
-    assert lib.tp_abi_version() == 2
+    assert lib.tp_abi_version() == 3
     from tp_amd import _lib
-    assert _lib.lib().tp_abi_version() == 2
+    assert _lib.lib().tp_abi_version() == 3
 
 
 def test_gfx950_code_object_present():

@@ -73,7 +73,7 @@ def _hist(tp, c, L, LMAX, N, K, seed):
     return c3ref_from_columns(cnt, N, K)
 
 
-@pytest.mark.parametrize("prec", ["fp16x3", "fp64"])
+@pytest.mark.parametrize("prec", ["fp16x3", "fp64", "fp16x3_t64"])
 def test_energy_batch_matches_single(tp, prec):
     L, LMAX, N = 8, 25, 8
     Ks = [3, 9, 5, 12]
@@ -98,9 +98,9 @@ def test_energy_batch_matches_single(tp, prec):
     assert np.all(np.abs(Eb - E64) <= 1e-6 * np.abs(E64))
 
 
-def _anneal_setup(tp, Ks, nprop, hist_cap=0, L=8, LMAX=25, N=8):
+def _anneal_setup(tp, Ks, nprop, hist_cap=0, L=8, LMAX=25, N=8, prec="fp16x3"):
     master = tp.HIP_calculate_basis(L, LMAX, N, max(Ks))
-    master.set_precision("fp16x3")
+    master.set_precision(prec)
     bases = [tp.HIP_calculate_basis(L, LMAX, N, K) for K in Ks]
     truth = [random_real_coeff(K, L, LMAX, seed=90 + K, c00=4.0, decay=0.1) for K in Ks]
     refs = [_hist(tp, truth[i], L, LMAX, N, K, 20 + i) for i, K in enumerate(Ks)]
@@ -111,15 +111,19 @@ def _anneal_setup(tp, Ks, nprop, hist_cap=0, L=8, LMAX=25, N=8):
     return master, bases, refs, starts, B
 
 
-def test_batch_eval_quats_matches_single(tp):
+@pytest.mark.parametrize("prec", ["fp16x3", "fp16x3_t64"])
+def test_batch_eval_quats_matches_single(tp, prec):
+    """A batch's energies equal single evaluations in the same mode (the
+    batch keeps its mode when the basis changes afterwards)."""
     Ks, nprop = [4, 9, 6], 4
     L = 8
-    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop)
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, prec=prec)
+    master.set_precision("fp64")
     rng = np.random.default_rng(9)
     q = np.stack([random_quats(rng, L // 2) for _ in range(len(Ks) * nprop)])
     E = B.eval_quats(q)
     for b in bases:
-        b.set_precision("fp16x3")
+        b.set_precision(prec)
     for g, K in enumerate(Ks):
         for p in range(nprop):
             e = g * nprop + p

@@ -111,12 +111,12 @@ def test_full_anneal_steps_match_oracle(full, oracle):
     assert np.abs(final - ro["coeff"]).max() < 1e-10 * np.abs(ro["coeff"]).max()
 
 
-def _batch(full, nlearn=0, mlp=None):
+def _batch(full, nlearn=0, mlp=None, prec="fp16x3"):
     tp, wl, cfg, truth, basis, c3ref = (full[k] for k in ("tp", "workload", "cfg", "truth", "basis", "c3ref"))
     Ks = [int(np.random.default_rng(s).integers(10, 27)) for s in SEEDS]
     starts = [wl.random_start_structure(truth, cfg["L"], s) for s in SEEDS]
     refs = [np.ascontiguousarray(c3ref[:, :, :K, :K, :K]) for K in Ks]
-    basis.set_precision("fp16x3")
+    basis.set_precision(prec)
     B = tp.AnnealBatch(basis, refs, starts, SEEDS, nprop=NPROP, hist_cap=16)
     return Ks, starts, refs, B
 
@@ -130,10 +130,13 @@ def _fp64_energy(full, c, K, ref):
     return tp.energy(cc, basis, ref, range(1, K + 1))
 
 
-def test_batched_evaluation_matches_fp64(full, oracle):
+@pytest.mark.parametrize("prec", ["fp16x3", "fp16x3_t64"])
+def test_batched_evaluation_matches_fp64(full, oracle, prec):
+    """fp16x3: compensated FP16 triple products; fp16x3_t64: FP64 ones on the
+    sorted l-triples (runs of K > 16 take both shell halves)."""
     tp, cfg, basis = full["tp"], full["cfg"], full["basis"]
-    say("batch: 20 runs x 64 proposals")
-    Ks, starts, refs, B = _batch(full)
+    say(f"batch ({prec}): 20 runs x 64 proposals")
+    Ks, starts, refs, B = _batch(full, prec=prec)
     assert sorted(set(Ks)) == sorted(set(Ks)) and min(Ks) >= 10 and max(Ks) <= 26
     rng = np.random.default_rng(2024)
     q = np.stack([random_quats(rng, cfg["L"] // 2) for _ in range(len(Ks) * NPROP)])

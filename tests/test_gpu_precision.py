@@ -1,5 +1,5 @@
-"""Mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_BF16) against the FP64
-path, which itself matches the oracle to ~1e-15 (test_gpu_parity.py).
+"""Mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_FP16X3_T64 /
+TP_PREC_BF16 / TP_PREC_TRIPLE16) against the FP64 path, which itself matches the oracle to ~1e-15 (test_gpu_parity.py).
 
 Bar (north star): c3 relative error <= 1e-5 against FP64 for the compensated
 FP16 form.  c3 error per k-triple slice = max_r |c3 - c3_64| / max_r |c3_64|."""
@@ -41,6 +41,9 @@ def test_fp16x3_c3_and_energy(tp, L, LMAX, N, K):
     b.set_precision("triple16")
     ct16 = tp.FullCorrelation_parallized(vol, b, return_raw=True)
     Et16 = tp.energy(vol, b, ref)
+    b.set_precision("fp16x3_t64")
+    c64t = tp.FullCorrelation_parallized(vol, b, return_raw=True)
+    E64t = tp.energy(vol, b, ref)
     b.set_precision("fp64")
     again = tp.FullCorrelation_parallized(vol, b, return_raw=True)
     assert np.array_equal(again, c64)            # switching modes leaves the FP64 path untouched
@@ -54,6 +57,10 @@ def test_fp16x3_c3_and_energy(tp, L, LMAX, N, K):
     print(f"   triple16 slice err {et:.2e}, |dE| {abs(Et16 - E64):.2e}")
     assert et < C3_TOL_FP16X3
     assert abs(Et16 - E64) / abs(E64) < 1e-6
+    e64t = slice_err(c64t, c64)                  # the contraction alone in FP16 hi/lo
+    print(f"   fp16x3_t64 slice err {e64t:.2e}, |dE| {abs(E64t - E64):.2e}")
+    assert e64t < C3_TOL_FP16X3
+    assert abs(E64t - E64) / abs(E64) < 1e-6
 
 
 def test_fp16x3_full_size(tp):
@@ -69,7 +76,9 @@ def test_fp16x3_full_size(tp):
         c64 = tp.FullCorrelation_parallized(st, b, return_raw=True)
         b.set_precision("fp16x3")
         c16 = tp.FullCorrelation_parallized(st, b, return_raw=True)
-        errs.append(slice_err(c16, c64))
+        b.set_precision("fp16x3_t64")
+        c16t = tp.FullCorrelation_parallized(st, b, return_raw=True)
+        errs.append((slice_err(c16, c64), slice_err(c16t, c64)))
     b.set_precision("fp64")
-    print("full-size fp16x3 slice errors:", errs)
-    assert max(errs) < C3_TOL_FP16X3
+    print("full-size fp16x3 / fp16x3_t64 slice errors:", errs)
+    assert max(max(e) for e in errs) < C3_TOL_FP16X3

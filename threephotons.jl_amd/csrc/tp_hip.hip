@@ -151,6 +151,7 @@ struct tp_basis {
     int *d_gorder = nullptr;       // [n_groups] run order of the mu3 groups of sorted l-triples
     uint32_t *d_symtab = nullptr;  // V entries each permutation stores (k_triple_sym)
     int *d_symoff = nullptr;       // [6 * 2 * 2 * 2 + 1]
+    std::vector<std::pair<int, double>> h_sym_cost;   // per sorted l-triple: (ts, stage-1 + write cost)
     std::vector<int> h_lt_cnt, h_lt_l3;   // per l-triple: basis entries, l3 (work lists of batches)
     // mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_BF16)
     int prec = TP_PREC_FP64;
@@ -286,14 +287,14 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     hipStream_t st = ctx->stream;
     const int prec = force64 ? TP_PREC_FP64 : b->prec;
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    if (b->use_mfma && prec != TP_PREC_FP64) {
+    if (b->use_mfma && prec != TP_PREC_FP64 && prec != TP_PREC_FP16X3_T64) {
         // mixed precision: compensated FP16 triple products
         hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)b->K), dim3(256), 0, st, b0, b1, sel, flip, b->n_c, b->L,
                            sc.shsc);
         hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
                            sc.shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
                            b->EmaxLt8, b->Prows, sc.As, b->kl_pad);
-    } else if (b->use_sym) {
+    } else if (b->use_sym) {   // FP64 (also TP_PREC_FP16X3_T64, which needs the tables)
         hipLaunchKernelGGL(k_triple_sym, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1, sel,
                            flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
                            b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad);
@@ -310,12 +311,12 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     if (timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     const int ncols = r ? (int)r->ncols : (int)b->klength;
     int n_rt_fin = b->n_rt;
-    if (prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16) {
+    if (prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16 || prec == TP_PREC_FP16X3_T64) {
         // mixed precision: c3 = F . As with 16-bit MFMA operands
         const double *qh = r ? r->d_qhat : nullptr;
         const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
         const dim3 gs((unsigned)(b->kl_pad / 64), (unsigned)(b->ldk / 64));
-        if (prec == TP_PREC_FP16X3) {
+        if (prec != TP_PREC_BF16) {
             hipLaunchKernelGGL(k_split_as<1>, gs, dim3(256), 0, st, sc.As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
                                sel, flip, b->n_c, b->K, b->L, (int)b->klength, sc.Ahi, sc.Alo, sc.Ainv);
         } else {
@@ -324,12 +325,12 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         }
         TP_HIP(hipGetLastError());
         if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
-        const uint16_t *Fh = prec == TP_PREC_FP16X3 ? b->d_Fhi : b->d_Fbf;
+        const uint16_t *Fh = prec != TP_PREC_BF16 ? b->d_Fhi : b->d_Fbf;
 #define TP_G16(MODE, OUT, EN, OUTP)                                                                                \
     hipLaunchKernelGGL((k_gemm16<H_BM, H_BN, MODE, OUT, EN, 0>), grid16, dim3(256), 0, st, Fh, b->d_Flo,          \
                        (int64_t)b->ldk, sc.Ahi, sc.Alo, (int64_t)b->ldk, b->ldk, b->F_inv, sc.Ainv, (int)b->R,  \
                        ncols, b->n_rt16, b->n_ct16, OUTP, b->R, qh, sc.part, b->kl_pad, nullptr, nullptr)
-        if (prec == TP_PREC_FP16X3) {
+        if (prec != TP_PREC_BF16) {
             if (r && c3out) TP_G16(1, 1, true, c3out);
             else if (r) TP_G16(1, 0, true, nullptr);
             else TP_G16(1, 1, false, c3out);
@@ -433,12 +434,15 @@ struct tp_batch {
     std::vector<std::vector<std::pair<double, int2>>> items;   // per evaluation: (cost, {e, lt | h << 16})
     int64_t ncb = 0, ldA = 0, cstride = 0;
     int nwork = 0;
+    int prec = TP_PREC_FP16X3;             // the basis's precision mode when the batch was made
     int gcand = 0, order = 0;              // contraction candidate (kTiles), triple-product work order (autotuned)
     int family = TP_MOVE_PLANE;            // move family of the random slots
     double tune_ms[2 + kNumTiles] = {};
     EvalDesc *d_ev = nullptr;
     GroupState *d_gs = nullptr;
     int2 *d_work = nullptr;
+    int2 *d_swork = nullptr;       // sorted l-triple items (evaluation, ts | h1 << 16 | h2 << 20), by cost
+    int nswork = 0;
     double2 *d_cur = nullptr, *d_prop = nullptr;
     double *d_quat = nullptr, *d_As = nullptr, *d_sc = nullptr, *d_isc = nullptr, *d_part = nullptr;
     double *d_E = nullptr, *d_Ehist = nullptr, *d_shsc = nullptr;   // d_shsc [ne][K] shell scales
@@ -479,7 +483,7 @@ static void batch_free(tp_batch *B)
     if (B->graph) (void)hipGraphDestroy(B->graph);
     for (auto &e : B->ev)
         if (e) (void)hipEventDestroy(e);
-    void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As,
+    void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_swork, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As,
                     B->d_sc,  B->d_isc, B->d_part, B->d_E,    B->d_Ehist,  B->d_Ahi,    B->d_Alo,
                     (void *)B->d_qtab, B->d_nvalid, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn, B->d_shsc,
                     B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi};
@@ -503,6 +507,33 @@ static void batch_items(const tp_basis *b, int e, int K, std::vector<std::pair<d
             out.push_back({cost, make_int2(e, t | (h << 16))});
         }
     }
+}
+
+// Work items of the sorted l-triple form for every evaluation (both K1 and K2
+// halves where the evaluation has more than 16 shells), most expensive first.
+static int batch_sym_work(tp_batch *B)
+{
+    const tp_basis *b = B->b;
+    if (!b->use_sym) return 0;
+    std::vector<std::pair<double, int2>> all;
+    for (int e = 0; e < B->ne; ++e) {
+        const int nh = B->h_ev[e].K > 16 ? 2 : 1;
+        for (const auto &tc : b->h_sym_cost) {
+            const int n3 = 2 * b->h_lt_l3[tc.first] + 1;
+            const double cost = tc.second + ((n3 + 1) / 2) * 2.0 * nh;
+            for (int h1 = 0; h1 < nh; ++h1)
+                for (int h2 = 0; h2 < nh; ++h2) all.push_back({cost, make_int2(e, tc.first | (h1 << 16) | (h2 << 20))});
+        }
+    }
+    std::stable_sort(all.begin(), all.end(), [](const std::pair<double, int2> &a, const std::pair<double, int2> &c) {
+        return a.first > c.first;
+    });
+    std::vector<int2> w(all.size());
+    for (size_t i = 0; i < all.size(); ++i) w[i] = all[i].second;
+    if (int rc = dev_alloc(&B->d_swork, w.size())) return rc;
+    TP_HIP(hipMemcpy(B->d_swork, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice));
+    B->nswork = (int)w.size();
+    return 0;
 }
 
 // Upload the work list in the given order: 0 = all items by decreasing cost,
@@ -539,12 +570,15 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
 {
     *out = nullptr;
     if (!b->use_mfma) return tp_fail(TP_E_UNSUPPORTED, "batches need the MFMA triple products (K <= 27)");
-    if (b->prec != TP_PREC_FP16X3 && b->prec != TP_PREC_BF16)
-        return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs TP_PREC_FP16X3 or TP_PREC_BF16");
+    if (b->prec != TP_PREC_FP16X3 && b->prec != TP_PREC_BF16 && b->prec != TP_PREC_FP16X3_T64)
+        return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs TP_PREC_FP16X3, TP_PREC_FP16X3_T64 or TP_PREC_BF16");
     if (b->R % 128 != 0) return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs 2N^2 to be a multiple of 128");
+    if (b->prec == TP_PREC_FP16X3_T64 && !b->use_sym)
+        return tp_fail(TP_E_UNSUPPORTED, "TP_PREC_FP16X3_T64 needs the sorted l-triple tables");
     tp_batch *B = new tp_batch();
     B->ctx = ctx;
     B->b = b;
+    B->prec = b->prec;
     B->ngroups = ngroups;
     B->nprop = nprop;
     B->ne = ngroups * nprop;
@@ -607,7 +641,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         (rc = dev_alloc(&B->d_E, B->ne)) || (rc = dev_alloc(&B->d_Ehist, (size_t)std::max(1, hist_cap) * ngroups)) ||
         (rc = dev_alloc(&B->d_shsc, (size_t)B->ne * b->K)) ||
         (rc = dev_alloc(&B->d_Ahi, (size_t)B->ldA * b->ldk)) ||
-        (b->prec == TP_PREC_FP16X3 && (rc = dev_alloc(&B->d_Alo, (size_t)B->ldA * b->ldk))) ||
+        (B->prec != TP_PREC_BF16 && (rc = dev_alloc(&B->d_Alo, (size_t)B->ldA * b->ldk))) ||
         (rc = dev_alloc(&B->d_qtab, qtab.size())) || (rc = dev_alloc(&B->d_nvalid, nvalid.size())) ||
         (!sweep && (rc = dev_alloc(&B->d_raw, (size_t)b->K * b->LMAX * b->LMAX)))) {
         batch_free(B);
@@ -631,7 +665,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         batch_free(B);
         return tp_fail(TP_E_HIP, std::string("batch set-up: ") + hipGetErrorString(e));
     }
-    if ((rc = batch_upload_work(B, 0))) {
+    if ((rc = batch_upload_work(B, 0)) || (rc = batch_sym_work(B))) {
         batch_free(B);
         return rc;
     }
@@ -647,7 +681,7 @@ static void batch_gemm_launch(const tp_batch *B, hipStream_t st)
     // WALK = 2: one workgroup per column tile walking all row tiles; 1: one per
     // tile; 3: one per tile, row tiles partitioned over the XCDs
     const dim3 grid((unsigned)(WALK == 2 ? n_ct : n_rt * n_ct));
-    if (b->prec == TP_PREC_FP16X3)
+    if (B->prec != TP_PREC_BF16)
         hipLaunchKernelGGL((k_gemm16<BM, BN, 1, 0, true, WALK>), grid, dim3(256), 0, st, b->d_Fhi, b->d_Flo,
                            (int64_t)b->ldk, B->d_Ahi, B->d_Alo, (int64_t)b->ldk, b->ldk, b->F_inv, B->d_isc, (int)b->R,
                            0, n_rt, n_ct, nullptr, (int64_t)0, nullptr, B->d_part, B->ldA, B->d_qtab, B->d_nvalid);
@@ -693,16 +727,21 @@ static void batch_triple(const tp_batch *B, hipStream_t st)
     const tp_basis *b = B->b;
     hipLaunchKernelGGL(k_colscale_batch, dim3((unsigned)B->ne), dim3(256), 0, st, B->d_ev, b->n_c, b->L, B->d_sc,
                        B->d_isc, B->d_shsc, b->K);
-    hipLaunchKernelGGL(k_triple16_batch, dim3((unsigned)B->nwork), dim3(512), b->t16_lds, st, B->d_ev, B->d_work,
-                       B->d_shsc, b->K, b->n_c, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->Lw, b->EmaxLt8,
-                       b->Prows, B->d_As, B->ldA);
+    if (B->prec == TP_PREC_FP16X3_T64)
+        hipLaunchKernelGGL(k_triple_sym_batch, dim3((unsigned)B->nswork), dim3(kSymThreads), b->sym_lds, st, B->d_ev,
+                           B->d_swork, b->n_c, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_perms, b->d_gorder,
+                           b->d_symtab, b->d_symoff, b->Lw, B->d_As, B->ldA);
+    else
+        hipLaunchKernelGGL(k_triple16_batch, dim3((unsigned)B->nwork), dim3(512), b->t16_lds, st, B->d_ev, B->d_work,
+                           B->d_shsc, b->K, b->n_c, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->Lw, b->EmaxLt8,
+                           b->Prows, B->d_As, B->ldA);
 }
 
 static void batch_split(const tp_batch *B, hipStream_t st)
 {
     const tp_basis *b = B->b;
     const dim3 gs((unsigned)B->ncb, (unsigned)(b->ldk / 64));
-    if (b->prec == TP_PREC_FP16X3)
+    if (B->prec != TP_PREC_BF16)
         hipLaunchKernelGGL(k_split_batch<1>, gs, dim3(256), 0, st, B->d_As, B->ldA, (int)b->n_lt, b->ldk, B->d_sc,
                            B->d_Ahi, B->d_Alo);
     else
@@ -802,7 +841,11 @@ static int batch_autotune(tp_batch *B, int reps)
     hipStream_t st = B->ctx->stream;
     double best = 1e30;
     int best_order = 0;
+    // work orders of the coupling form (the sorted l-triple form has one order)
+    const int norders = B->prec == TP_PREC_FP16X3_T64 ? 1 : 2;
     for (int order = 0; order < 2; ++order) {
+        B->tune_ms[order] = 1e30;
+        if (order >= norders) continue;
         if (int rc = batch_upload_work(B, order)) return rc;
         double ms = 0;
         if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_triple(B, s); }, &ms)) return rc;
@@ -1592,6 +1635,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
                 for (int c0 = 0; c0 < n3; c0 += kSymWaves) s1 += (-ge[c0].first + 3) / 4;   // slowest wave
                 const double s2 = ((n3 + 1) / 2) * 2.0 * (nh > 1 ? 2 : 1);
                 const double cost = 2.0 * s1 + s2 + 0.25 * perms[(size_t)ts * 8];
+                b->h_sym_cost.push_back({ts, 2.0 * s1 + 0.25 * perms[(size_t)ts * 8]});
                 for (int h1 = 0; h1 < nh; ++h1)
                     for (int h2 = 0; h2 < nh; ++h2) wl.push_back({-cost, ts | (h1 << 16) | (h2 << 20)});
             }
@@ -1624,9 +1668,12 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             b->nsym_work = (int)sym_work.size();
             b->sym_lds = lds;
             b->use_sym = true;
-            if (lds > 65536)
+            if (lds > 65536) {
                 (void)hipFuncSetAttribute((const void *)k_triple_sym, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds);
+                (void)hipFuncSetAttribute((const void *)k_triple_sym_batch,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            }
             if (b->Lw == 0) b->Lw = 2 * L + 1;
         }
     }
@@ -1758,14 +1805,18 @@ int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *
 int tp_basis_set_precision(tp_basis *b, int mode)
 {
     if (!b) return tp_fail(TP_E_ARG, "null basis");
-    if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3 && mode != TP_PREC_BF16 && mode != TP_PREC_TRIPLE16)
+    if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3 && mode != TP_PREC_BF16 && mode != TP_PREC_TRIPLE16 &&
+        mode != TP_PREC_FP16X3_T64)
         return tp_fail(TP_E_ARG, "unknown precision mode");
+    if (mode == TP_PREC_FP16X3_T64 && !b->use_sym)
+        return tp_fail(TP_E_UNSUPPORTED, "TP_PREC_FP16X3_T64 needs the sorted l-triple tables (the library's own "
+                                         "weights, K <= 32)");
     if (int rc = check_ctx(b->ctx)) return rc;
     if (mode != TP_PREC_FP64 && (b->K > 64 || b->n_c > 1024))
         return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64 and L <= 44");
     if (mode != TP_PREC_FP64 && b->lam)
         return tp_fail(TP_E_UNSUPPORTED, "mixed precision needs lambda dq = 0 (the curved Ewald sphere runs in FP64)");
-    if ((mode == TP_PREC_FP16X3 || mode == TP_PREC_BF16) && !b->d_Fhi) {
+    if ((mode == TP_PREC_FP16X3 || mode == TP_PREC_BF16 || mode == TP_PREC_FP16X3_T64) && !b->d_Fhi) {
         // F[r][lt] = sum_class Bm[class][r] H[lt][class] in FP64, then scaled by a
         // power of two into [2^13, 2^14) and split into FP16 hi/lo and BF16
         const int64_t nlt = b->n_lt;
@@ -1928,7 +1979,7 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
     if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
     if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "coefficient vector shorter than LMAX^2");
     if (!ctx->scratch) return tp_fail(TP_E_STATE, "no scratch address for this device");
-    const bool mixed16 = b->prec == TP_PREC_FP16X3 || b->prec == TP_PREC_BF16;
+    const bool mixed16 = b->prec == TP_PREC_FP16X3 || b->prec == TP_PREC_BF16 || b->prec == TP_PREC_FP16X3_T64;
     const int64_t nrt = std::max<int64_t>(std::max(b->n_rt, b->njc), b->n_rt16);
     const size_t want[] = {
         sizeof(double2) * (size_t)b->K * ld,                  // 0 raw coefficients
@@ -3970,7 +4021,8 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
         if (bases[i]->prec != bm->prec) return tp_fail(TP_E_ARG, "batch members must share the precision mode");
     }
     hipStream_t st = ctx->stream;
-    if ((bm->prec != TP_PREC_FP16X3 && bm->prec != TP_PREC_BF16) || !bm->use_mfma || bm->R % 128 != 0) {
+    if ((bm->prec != TP_PREC_FP16X3 && bm->prec != TP_PREC_BF16 && bm->prec != TP_PREC_FP16X3_T64) ||
+        !bm->use_mfma || bm->R % 128 != 0) {
         // one evaluation after another on the device
         double *dE = nullptr;
         if (int rc = dev_alloc(&dE, n)) return rc;

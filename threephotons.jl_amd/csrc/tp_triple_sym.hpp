@@ -55,23 +55,18 @@ static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves)
 // symoff[.. + 1]) = the V entries of K3 block cb that ordering p stores:
 // (K1 - 16 h1 - 1) * 256 + (K2 - 16 h2 - 1) * 16 + (K3 - 16 cb - 1) | kt << 12,
 // in kt order.
+// The body of one work item wk on coefficients coef of K shells; As entry
+// (lt, kt) of the evaluation at As[lt * ldA + as_col(kt, cbase, nprop, p0)],
+// entries kt >= klen (shells past K) skipped.
 template <int NW, bool G3>
-__global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restrict__ cb0,
-                                                           const double2 *__restrict__ cb1,
-                                                           const int *__restrict__ sel, int flip, int n_c, int K,
-                                                           const double *__restrict__ ew,
-                                                           const uint32_t *__restrict__ eidx,
-                                                           const int *__restrict__ gstart,
-                                                           const int4 *__restrict__ ltinfo,
-                                                           const int *__restrict__ work, const int *__restrict__ perms,
-                                                           const int *__restrict__ gorder,
-                                                           const uint32_t *__restrict__ symtab,
-                                                           const int *__restrict__ symoff, int Lw,
-                                                           double *__restrict__ As, int64_t ldA)
+__device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K, int wk,
+                                                const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
+                                                const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
+                                                const int *__restrict__ perms, const int *__restrict__ gorder,
+                                                const uint32_t *__restrict__ symtab, const int *__restrict__ symoff,
+                                                int Lw, double *__restrict__ As, int64_t ldA, int cbase, int nprop,
+                                                int p0, uint32_t klen)
 {
-    extern __shared__ double2 sm[];
-    const double2 *__restrict__ coef = pick(cb0, cb1, sel, flip);
-    const int wk = work[blockIdx.x];
     const int ts = wk & 0xffff, h1 = (wk >> 16) & 15, h2 = (wk >> 20) & 15;
     const int4 li = ltinfo[ts];
     const int L3 = li.z, L1 = li.w & 0xff, L2 = (li.w >> 8) & 0xff;
@@ -212,13 +207,59 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
             const int t = (((code >> 24) * 2 + h1) * 2 + h2) * 2 + cb;
             const int o = symoff[t], n = symoff[t + 1] - o;
             for (int x = tid; x < n; x += NT) {
-                const uint32_t ent = symtab[o + x];
-                As[row + (ent >> 12)] = Tf[ent & 4095u];
+                const uint32_t ent = symtab[o + x], kt = ent >> 12;
+                if (kt < klen) As[row + as_col(kt, cbase, nprop, p0)] = Tf[ent & 4095u];
             }
         }
         __syncthreads();
     }
 }
 
+template <int NW, bool G3>
+__global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restrict__ cb0,
+                                                           const double2 *__restrict__ cb1,
+                                                           const int *__restrict__ sel, int flip, int n_c, int K,
+                                                           const double *__restrict__ ew,
+                                                           const uint32_t *__restrict__ eidx,
+                                                           const int *__restrict__ gstart,
+                                                           const int4 *__restrict__ ltinfo,
+                                                           const int *__restrict__ work, const int *__restrict__ perms,
+                                                           const int *__restrict__ gorder,
+                                                           const uint32_t *__restrict__ symtab,
+                                                           const int *__restrict__ symoff, int Lw,
+                                                           double *__restrict__ As, int64_t ldA)
+{
+    extern __shared__ double2 sm[];
+    triple_sym_body<NW, G3>(sm, pick(cb0, cb1, sel, flip), n_c, K, work[blockIdx.x], ew, eidx, gstart, ltinfo, perms,
+                            gorder, symtab, symoff, Lw, As, ldA, 0, 1, 0, 0xffffffffu);
+}
+
+// The same on a batch of evaluations (struct EvalDesc): work item i =
+// (evaluation, ts | h1 << 16 | h2 << 20); the tables are those of the
+// batch's basis (K_b shells): the entries of an evaluation of K <= K_b
+// shells are the ones with kt < K (K + 1) (K + 2) / 6, since a sorted
+// k-triple's index does not depend on the shell count.
+template <int NW, bool G3, typename Desc>
+__global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__restrict__ ev,
+                                                                 const int2 *__restrict__ work, int n_c,
+                                                                 const double *__restrict__ ew,
+                                                                 const uint32_t *__restrict__ eidx,
+                                                                 const int *__restrict__ gstart,
+                                                                 const int4 *__restrict__ ltinfo,
+                                                                 const int *__restrict__ perms,
+                                                                 const int *__restrict__ gorder,
+                                                                 const uint32_t *__restrict__ symtab,
+                                                                 const int *__restrict__ symoff, int Lw,
+                                                                 double *__restrict__ As, int64_t ldA)
+{
+    extern __shared__ double2 sm[];
+    const int2 w = work[blockIdx.x];
+    const Desc &d = ev[w.x];
+    const int K = d.K;
+    triple_sym_body<NW, G3>(sm, d.coef, n_c, K, w.y, ew, eidx, gstart, ltinfo, perms, gorder, symtab, symoff, Lw,
+                            As, ldA, d.cbase, d.nprop, d.p, (uint32_t)(K * (K + 1) * (K + 2) / 6));
+}
+
 // The configuration the library launches.
 #define k_triple_sym (k_triple_sym_t<kSymWaves, kSymGauss>)
+#define k_triple_sym_batch (k_triple_sym_batch_t<kSymWaves, kSymGauss, EvalDesc>)

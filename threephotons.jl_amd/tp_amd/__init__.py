@@ -166,10 +166,11 @@ class BasisTypeHip:
 
     def set_precision(self, mode):
         """'fp64' (default), 'fp16x3' (compensated FP16 MFMA triple products and
-        contraction), 'bf16', or 'triple16' (compensated FP16 triple products,
-        FP64 contraction)."""
+        contraction), 'bf16', 'triple16' (compensated FP16 triple products,
+        FP64 contraction) or 'fp16x3_t64' (FP64 triple products on the sorted
+        l-triples, the fp16x3 contraction)."""
         m = {"fp64": PREC_FP64, "fp16x3": PREC_FP16X3, "bf16": PREC_BF16,
-             "triple16": 3}[mode] if isinstance(mode, str) else mode
+             "triple16": 3, "fp16x3_t64": 4}[mode] if isinstance(mode, str) else mode
         check(lib().tp_basis_set_precision(self.h, m), "tp_basis_set_precision")
 
     def precision(self):

@@ -264,5 +264,5 @@ EXPORTS = [
     "tp_load_histograms", "tp_energy_noalloc", "tp_batch_eval_rotations", "tp_batch_set_state",
 ]
 
-PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16 = 0, 1, 2, 3
+PREC_FP64, PREC_FP16X3, PREC_BF16, PREC_TRIPLE16, PREC_FP16X3_T64 = 0, 1, 2, 3, 4
 MOVE_QUAT, MOVE_PLANE = 0, 1
