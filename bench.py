@@ -240,7 +240,7 @@ def precision_study(n):
     truth, basis, c3ref, _ = setup(cfg, tp, workload)
     c3ref = basis.histogram(c3ref)
     out = {"metric": "mixed-precision c3 / energy error vs FP64", "orientations": n, "config": "full (L=18,K=26,N=32)"}
-    modes = ("fp16x3", "triple16", "bf16")
+    modes = ("fp16x3", "fp16x3_t64", "triple16", "bf16")
     acc = {m: {"slice_max": [], "rms": [], "norm2": [], "dE_abs": [], "dE_rel": []} for m in modes}
     for seed in range(n):
         st = workload.random_start_structure(truth, cfg["L"], seed)
@@ -750,6 +750,8 @@ def main():
                     help="logged training pairs of the learned proposer (0: skip the learned leg)")
     ap.add_argument("--learned-data-sweeps", type=int, default=None,
                     help="0 skips the learned leg (the older form of --learned-samples 0)")
+    ap.add_argument("--learned-precision", default=None, choices=("fp16x3", "fp16x3_t64"),
+                    help="batch mode of the learned leg (default: the faster mode of the batched leg, else fp16x3)")
     ap.add_argument("--learned-train-steps", type=int, default=4000)
     ap.add_argument("--learned-cmp-steps", type=int, default=20)
     ap.add_argument("--learned-tte-steps", type=int, default=40)
@@ -839,6 +841,15 @@ def main():
         state.energy(H)
     elt16 = time.perf_counter() - t0
     kernt16 = stage_times(ctx, state, H, args.steps)
+    basis.set_precision("fp16x3_t64")
+    Et64 = state.energy(H)
+    for _ in range(args.warmup):
+        state.energy(H)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        state.energy(H)
+    elt64 = time.perf_counter() - t0
+    kernt64 = stage_times(ctx, state, H, args.steps)
     basis.set_precision("fp16x3")
     mstate = tp.DeviceState(start, basis)
     t0 = time.perf_counter()
@@ -870,6 +881,8 @@ def main():
     learned = None
     if args.learned_data_sweeps == 0:
         args.learned_samples = 0
+    if args.learned_precision:
+        batch_prec = args.learned_precision
     if args.learned_samples > 0:
         learned = learned_leg(cfg, tp, workload, truth, basis, c3ref, H, start, plain, args.learned_samples,
                               args.learned_train_steps, args.learned_cmp_steps, args.learned_tte_steps,
@@ -909,7 +922,11 @@ def main():
             "triple16": {"mode": "compensated FP16 triple products, FP64 separated-grid contraction",
                          "evals_per_s": args.steps / elt16, "energy_abs_err_vs_fp64": abs(Et16 - E),
                          "stage_ms": {"triple": kernt16[0], "gemm_G": kernt16[1], "c3_energy": kernt16[2],
-                                      "final": kernt16[3]}}},
+                                      "final": kernt16[3]}},
+            "fp16x3_t64": {"mode": "FP64 triple products on the sorted l-triples, fp16x3 contraction",
+                           "evals_per_s": args.steps / elt64, "energy_abs_err_vs_fp64": abs(Et64 - E),
+                           "stage_ms": {"triple": kernt64[0], "split": kernt64[1], "gemm16_c3_energy": kernt64[2],
+                                        "final": kernt64[3]}}},
         "roofline": {"bound": "mfma", "kernel": "k_triple_sym (triple products on the 125 sorted l-triples, "
                                                "FP64 MFMA)",
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

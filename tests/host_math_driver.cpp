@@ -7,6 +7,7 @@
 //   j n x                 -> J_{l+1/2}(x), l = 0..n-1 (bessel_half_all)
 //   l n                   -> worst error (ulp of the exact value, long double
 //                            reference) of tp_fastlog over n arguments
+//   t n                   -> the same for tp_log_series
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -17,7 +18,7 @@
 static double ulp_err(double x, const double *tab)
 {
     const long double ref = logl((long double)x);
-    const double got = tp_fastlog(x, tab);
+    const double got = tab ? tp_fastlog(x, tab) : tp_log_series(x);
     const double rd = (double)ref;
     const double u = std::nextafter(std::fabs(rd), INFINITY) - std::fabs(rd);
     return (double)(fabsl((long double)got - ref) / (long double)u);
@@ -47,11 +48,12 @@ int main()
             std::vector<double> J((size_t)n);
             tph::bessel_half_all(x, n, J.data());
             for (int l = 0; l < n; ++l) std::printf("%.17g%c", J[(size_t)l], l + 1 < n ? ' ' : '\n');
-        } else if (c == 'l') {
+        } else if (c == 'l' || c == 't') {
             int n;
             if (std::scanf("%d", &n) != 1) return 1;
-            double tab[kLogTabSize];
-            tp_fastlog_table(tab);
+            double tabv[kLogTabSize];
+            tp_fastlog_table(tabv);
+            const double *tab = c == 'l' ? tabv : nullptr;
             std::mt19937_64 rng(12345);
             std::uniform_real_distribution<double> U(0.0, 1.0);
             double worst = 0.0;
@@ -61,8 +63,12 @@ int main()
                 case 0: x = std::exp(-69.0 + 138.0 * U(rng)); break;             // 1e-30 .. 1e30
                 case 1: x = 1.0 + (U(rng) - 0.5) * 0.03125; break;               // around 1
                 case 2: x = std::ldexp(1.0 + U(rng), (int)(U(rng) * 80) - 40); break;
-                default: x = std::ldexp(1.0 + (std::floor(U(rng) * 128) + (U(rng) < 0.5 ? 0.0 : 1.0)) / 128.0, 3) *
-                             (1.0 + (U(rng) - 0.5) * 1e-15); break;              // table-interval edges
+                default:
+                    if (tab) x = std::ldexp(1.0 + (std::floor(U(rng) * 128) + (U(rng) < 0.5 ? 0.0 : 1.0)) / 128.0, 3) *
+                                 (1.0 + (U(rng) - 0.5) * 1e-15);                // table-interval edges
+                    else x = std::ldexp(1.4142135623730951 * (1.0 + (U(rng) - 0.5) * 1e-14),
+                                        (int)(U(rng) * 20) - 10);               // the sqrt2 fold
+                    break;
                 }
                 worst = std::fmax(worst, ulp_err(x, tab));
             }

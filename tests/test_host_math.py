@@ -69,6 +69,14 @@ def test_fastlog_ulp(driver):
     assert worst < 4.0, worst
 
 
+def test_log_series_ulp(driver):
+    """The table-free log of the mixed-precision contraction's energy epilogue
+    (tp_log_series) against long double over 1e-30..1e30, around 1 and at
+    the sqrt2 fold of its argument reduction."""
+    worst = float(driver(["t 400000"])[0])
+    assert worst < 4.0, worst
+
+
 def _bessel(driver, n, xs):
     return [np.array([float(v) for v in ln.split()]) for ln in driver([f"j {n} {x:.17g}" for x in xs])[:len(xs)]]
 

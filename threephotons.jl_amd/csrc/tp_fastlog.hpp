@@ -87,3 +87,48 @@ TP_HD inline double tp_fastlog(double x, const double *tab)
     const double ee = (double)e + tab[2 * kLogTab + j];
     return std::fma(ee, tab[3 * kLogTab], tab[kLogTab + j]) + std::fma(ee, tab[3 * kLogTab + 1], p);
 }
+
+// Table-free FP64 natural logarithm for epilogues where a table in LDS would
+// cost occupancy.  For normal positive x = 2^e m, m in [1, 2), m > sqrt2 is
+// halved (e + 1) so that m is in [1/sqrt2, sqrt2); then with f = m - 1
+// (exact) and s = f / (2 + f), |s| <= 0.1716,
+//   log m = 2 atanh s = 2s + s z (2/3 + 2z/5 + 2z^2/7 + ... + 2z^10/23),  z = s^2,
+// the atanh series to z^10 (truncation < z^11 / 23 < 1e-18 of the result),
+// and log x = e ln2_hi + (log m + e ln2_lo) with ln2_hi of 32 significant
+// bits (e ln2_hi exact for |e| < 2^21).  About 35 instructions, a few ulp
+// (checked against long double in tests/test_host_math.py).  Non-positive,
+// infinite, NaN and subnormal arguments go to the library log.
+TP_HD inline double tp_log_series(double x)
+{
+    uint64_t bits;
+    std::memcpy(&bits, &x, 8);
+    const int ex = (int)((bits >> 52) & 0x7FF);
+    if (ex == 0 || ex == 0x7FF || (bits >> 63)) return std::log(x);
+    int e = ex - 1023;
+    uint64_t mb = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;   // m in [1, 2)
+    if (mb > 0x3FF6A09E667F3BCCull) {   // m > sqrt2 (the double just below sqrt2 is 0x3FF6A09E667F3BCC)
+        mb -= 0x0010000000000000ull;    // m / 2
+        ++e;
+    }
+    double m;
+    std::memcpy(&m, &mb, 8);
+    const double f = m - 1.0;
+    const double sv = f / (2.0 + f);
+    const double z = sv * sv;
+    double q = 2.0 / 23.0;
+    q = std::fma(q, z, 2.0 / 21.0);
+    q = std::fma(q, z, 2.0 / 19.0);
+    q = std::fma(q, z, 2.0 / 17.0);
+    q = std::fma(q, z, 2.0 / 15.0);
+    q = std::fma(q, z, 2.0 / 13.0);
+    q = std::fma(q, z, 2.0 / 11.0);
+    q = std::fma(q, z, 2.0 / 9.0);
+    q = std::fma(q, z, 2.0 / 7.0);
+    q = std::fma(q, z, 2.0 / 5.0);
+    q = std::fma(q, z, 2.0 / 3.0);
+    const double lm = std::fma(sv * z, q, 2.0 * sv);
+    constexpr double ln2_hi = 0.6931471803691238;       // ln 2 truncated to 32 significant bits
+    constexpr double ln2_lo = 1.9082149292705877e-10;   // ln 2 - ln2_hi, rounded
+    const double de = (double)e;
+    return std::fma(de, ln2_hi, std::fma(de, ln2_lo, lm));
+}

@@ -899,7 +899,7 @@ struct tp_mlp {
     // captured training step
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    const void *g_src = nullptr;
+    const void *g_src = nullptr, *g_tgt = nullptr, *g_cnt = nullptr;   // the buffers the graph was captured on
     int g_B = 0;
     float g_lr = 0.f;
     uint64_t g_seed = 0;
@@ -1099,7 +1099,8 @@ static int mlp_train(tp_mlp *m, const uint16_t *Xd, const float *Td, const int *
         if (m->exec) { TP_HIP(hipGraphExecDestroy(m->exec)); m->exec = nullptr; }
         if (m->graph) { TP_HIP(hipGraphDestroy(m->graph)); m->graph = nullptr; }
     }
-    if (!m->exec || m->g_src != (const void *)Xd || m->g_B != B || m->g_lr != lr || m->g_seed != seed) {
+    if (!m->exec || m->g_src != (const void *)Xd || m->g_tgt != (const void *)Td || m->g_cnt != (const void *)count ||
+        m->g_B != B || m->g_lr != lr || m->g_seed != seed) {
         if (m->exec) { TP_HIP(hipGraphExecDestroy(m->exec)); m->exec = nullptr; }
         if (m->graph) { TP_HIP(hipGraphDestroy(m->graph)); m->graph = nullptr; }
         TP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1109,6 +1110,8 @@ static int mlp_train(tp_mlp *m, const uint16_t *Xd, const float *Td, const int *
         m->graph = g;
         TP_HIP(hipGraphInstantiate(&m->exec, g, nullptr, nullptr, 0));
         m->g_src = Xd;
+        m->g_tgt = Td;
+        m->g_cnt = count;
         m->g_B = B;
         m->g_lr = lr;
         m->g_seed = seed;
@@ -4248,6 +4251,10 @@ int tp_mlp_train(tp_mlp *m, const float *X, const float *T, int n, int steps, in
         return tp_fail(TP_E_ARG, "bad argument (batch must be a positive multiple of 64)");
     if (int rc = check_ctx(m->ctx)) return rc;
     if (n > m->ndata_cap) {
+        // a graph captured on the old buffers must not outlive them
+        if (m->exec) { (void)hipGraphExecDestroy(m->exec); m->exec = nullptr; }
+        if (m->graph) { (void)hipGraphDestroy(m->graph); m->graph = nullptr; }
+        TP_HIP(hipStreamSynchronize(m->ctx->stream));
         if (m->d_Xdata) (void)hipFree(m->d_Xdata);
         if (m->d_Tdata) (void)hipFree(m->d_Tdata);
         m->d_Xdata = nullptr;

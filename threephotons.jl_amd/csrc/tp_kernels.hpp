@@ -857,7 +857,7 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
                         if (ENERGY) {
                             const double pv = c > 1e-30 ? c : 1e-30;
                             sp[j] += pv;
-                            sq[j] += qcol[j][r] * log(pv);   // (tp_fastlog's table costs this kernel occupancy)
+                            sq[j] += qcol[j][r] * tp_log_series(pv);   // table-free: no LDS for the log
                         }
                     }
                 }

@@ -13,11 +13,13 @@
 #   fitStructures_full_hip(volume, reference, num, K_range, L, basis)              (data_processing.jl:26-46)
 #   sRAAR_hip(intensity, iterations, beta0, beta_max, tau; cutoff_factor)          (phases.jl:31-95)
 #   createSphericalHarmonicsStructure_hip(pdb, LMAX, KMAX, rmax)                    (structure.jl:124-134)
+#   rotate_all_at_once_hip(out, params, state, c3ref; checkpoint)                  (determination.jl:281-338)
+#   loadHistograms_hip(c3_full, K3)                                                  (datagen.jl:432-475, c3 part)
 #
 # Written against Julia 0.6 syntax like the rest of the package.
 
 export BasisTypeHip, HIP_init, HIP_calculate_basis, retrieveSolution_hip, fitStructures_full_hip, sRAAR_hip,
-       createSphericalHarmonicsStructure_hip
+       createSphericalHarmonicsStructure_hip, rotate_all_at_once_hip, loadHistograms_hip
 
 const libtp = get(ENV, "THREEPHOTONS_LIBTP", joinpath(dirname(@__FILE__), "..", "libtp.so"))
 HIP_enabled = false
@@ -105,9 +107,11 @@ function FullCorrelation_parallized(intensity::SphericalHarmonicsVolume, basis::
     return normalize ? t / sum(abs, t) : t
 end
 
-"""Device copy of the normalised c3ref slices of the K3_range walk (cached per histogram)."""
+"""Device copy of the normalised c3ref slices of the K3_range walk, cached by the
+histogram's contents (Base.hash of the array), so an array changed in place gets a
+fresh device copy."""
 function hip_ref(basis::BasisTypeHip, c3ref::C3, K3_range::UnitRange{Int64})
-    key = (object_id(c3ref), K3_range)
+    key = (hash(c3ref), size(c3ref), K3_range)
     if !haskey(basis.refs, key)
         out = Ref{Ptr{Void}}(C_NULL)
         hip_check(ccall((:tp_ref_create, libtp), Cint, (Ptr{Void}, Ptr{Void}, Ptr{Float64}, Cint, Cint, Cint, Ptr{Ptr{Void}}),
@@ -254,7 +258,9 @@ function sRAAR_hip(intensity::SphericalHarmonicsVolume, iterations::Integer, bet
     return density
 end
 
-"""Switch a basis to the mixed-precision contraction (1 = FP16 hi/lo, 2 = BF16, 0 = FP64)."""
+"""Switch a basis's arithmetic (TP_PREC_* of include/tp.h): 0 = FP64, 1 = FP16 hi/lo
+triple products and contraction, 2 = BF16, 3 = FP16 hi/lo triple products with the FP64
+contraction, 4 = FP64 triple products with the FP16 hi/lo contraction."""
 function hip_set_precision(basis::BasisTypeHip, mode::Integer)
     hip_check(ccall((:tp_basis_set_precision, libtp), Cint, (Ptr{Void}, Cint), basis.handle, mode),
               "tp_basis_set_precision")
@@ -329,4 +335,123 @@ function createSphericalHarmonicsStructure_hip(moleculepdb, LMAX::Int64, KMAX::I
         intensity.coeff[k][:] = I[:, k]
     end
     return density, fourier, intensity
+end
+
+# isbits mirrors of tp_anneal_params, tp_anneal_status and tp_checkpoint_info (include/tp.h)
+immutable TpAnnealParams
+    nsteps::Cint
+    ntemp::Cint
+    initial_stepsize::Cdouble
+    stepsizefactor::Cdouble
+    initial_temperature_factor::Cdouble
+    temperature_decay::Cdouble
+    seed::UInt64
+end
+
+immutable TpAnnealStatus
+    step::Int64
+    E::Cdouble
+    T::Cdouble
+    E0::Cdouble
+    T0::Cdouble
+    stepsizes::NTuple{32,Cdouble}
+    nl::Int32
+    naccepted::Int32
+    stopped::Int32
+    reserved::Int32
+end
+
+immutable TpCheckpointInfo
+    params::TpAnnealParams
+    status::TpAnnealStatus
+    min_stepsize::Cdouble
+    lambda::Cdouble
+    dq::Cdouble
+    L::Int32
+    LMAX::Int32
+    N::Int32
+    K::Int32
+    k3lo::Int32
+    k3hi::Int32
+    nshell::Int32
+    ld::Int32
+end
+
+"""rotate_all_at_once (determination.jl:281-338) with every step on the GPU: the
+intensity stays on the device between steps (tp_state), the steps run in pieces of
+`chunk` (tp_anneal_steps) until stepsizes[L] <= minstepsize (determination.jl:1, 322),
+and state["i"], state["E"], state["T"], state["stepsizes"] and state["intensity"] are
+written back after every piece as the reference keeps them.  regular_action's save
+every 500 steps (determination.jl:270-278) becomes a checkpoint file at `checkpoint`
+(tp_checkpoint_save; nothing: none).  A state that already has stepsizes for this L
+continues from its i, E, T and stepsizes, as the reference's continuation does."""
+function rotate_all_at_once_hip(out, params::Dict, state::Dict, c3ref::C3; checkpoint=nothing, chunk::Int=500)
+    state["L"] = params["L"]
+    L = state["L"]
+    basis = HIP_calculate_basis(L, params["LMAX"], params["N"], maximum(params["K3_range"]), params["lambda"],
+                                dq(state["intensity"]))
+    ref = hip_ref(basis, c3ref, params["K3_range"])
+    intensity = state["intensity"]
+    nshell = intensity.KMAX
+    coeff = hip_pack(intensity, nshell)
+    s = Ref{Ptr{Void}}(C_NULL)
+    hip_check(ccall((:tp_state_create, libtp), Cint,
+                    (Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64, Cint, Ptr{Ptr{Void}}),
+                    hip_ctx, basis.handle, coeff, size(coeff, 1), nshell, s), "tp_state_create")
+    p = Ref(TpAnnealParams(0, 50, params["initial_stepsize"], params["stepsizefactor"],
+                           params["initial_temperature_factor"], params["temperature_decay"],
+                           UInt64(get(params, "seed", 0))))
+    nl = div(L, 2)
+    st = Ref{TpAnnealStatus}()
+    try
+        if !haskey(state["stepsizes"], L)
+            state["K"] = maximum(params["K3_range"])
+            hip_check(ccall((:tp_anneal_begin, libtp), Cint,
+                            (Ptr{Void}, Ptr{Void}, Ptr{Void}, Ptr{Void}, Ref{TpAnnealParams}, Ref{TpAnnealStatus}),
+                            hip_ctx, basis.handle, ref, s[], p, st), "tp_anneal_begin")
+        else
+            steps = ntuple(i -> i <= nl ? Float64(state["stepsizes"][2 * i]) : 0.0, 32)
+            st[] = TpAnnealStatus(state["i"], state["E"], state["T"], state["E"], state["T"], steps, nl, 0, 0, 0)
+        end
+        while true
+            done = Ref{Int32}(0)
+            hip_check(ccall((:tp_anneal_steps, libtp), Cint,
+                            (Ptr{Void}, Ptr{Void}, Ptr{Void}, Ptr{Void}, Ref{TpAnnealParams}, Ref{TpAnnealStatus},
+                             Cint, Cdouble, Ptr{Float64}, Ptr{Float64}, Ptr{Int32}, Ref{Int32}),
+                            hip_ctx, basis.handle, ref, s[], p, st, chunk, minstepsize, C_NULL, C_NULL, C_NULL, done),
+                      "tp_anneal_steps")
+            hip_check(ccall((:tp_state_read, libtp), Cint, (Ptr{Void}, Ptr{Void}, Ptr{Complex{Float64}}, Int64),
+                            hip_ctx, s[], coeff, size(coeff, 1)), "tp_state_read")
+            for k = 1:nshell intensity.coeff[k][:] = coeff[:, k] end
+            state["i"], state["E"], state["T"] = st[].step, st[].E, st[].T
+            state["stepsizes"] = Dict(2 * i => st[].stepsizes[i] for i = 1:nl)
+            if checkpoint != nothing
+                info = Ref(TpCheckpointInfo(p[], st[], minstepsize, params["lambda"], dq(intensity), L, params["LMAX"],
+                                            params["N"], maximum(params["K3_range"]), minimum(params["K3_range"]),
+                                            maximum(params["K3_range"]), nshell, size(coeff, 1)))
+                hip_check(ccall((:tp_checkpoint_save, libtp), Cint, (Cstring, Ref{TpCheckpointInfo}, Ptr{Complex{Float64}}),
+                                checkpoint, info, coeff), "tp_checkpoint_save")
+            end
+            println(out, "step $(state["i"]) E $(state["E"]) T $(state["T"]) stepsize $(st[].stepsizes[nl])")
+            flush(out)
+            if st[].stopped != 0 || done[] == 0
+                break
+            end
+        end
+    finally
+        ccall((:tp_state_destroy, libtp), Void, (Ptr{Void},), s[])
+    end
+    return state
+end
+
+"""The c3 part of loadHistograms (datagen.jl:432-475) through libtp: an old N x N
+histogram is doubled along beta, entries are clamped to >= 1e-30, and the first K3
+shells are kept and divided by their total."""
+function loadHistograms_hip(c3_full::C3, K3::Int64)
+    NA, NB, Kf = size(c3_full, 1), size(c3_full, 2), size(c3_full, 3)
+    N = NA
+    out = zeros(Float64, N, 2 * N, K3, K3, K3)
+    hip_check(ccall((:tp_load_histograms, libtp), Cint, (Ptr{Float64}, Cint, Cint, Cint, Cint, Ptr{Float64}),
+                    Array{Float64}(c3_full), NA, NB, Kf, K3, out), "tp_load_histograms")
+    return out
 end
