@@ -133,6 +133,36 @@ def test_state_energy_and_determinism(tp):
     assert np.array_equal(back, c)
 
 
+def test_state_energy_follows_histogram_mode_and_state(tp):
+    """Device-resident energies across alternating histograms, K3_ranges and
+    precision modes, a new histogram after another was dropped, and a state
+    moved by annealing: each equals the direct evaluation of the same
+    structure (tp_energy), bit for bit."""
+    L, LMAX, N, K = 8, 25, 16, 8
+    c = random_real_coeff(K + 2, L, LMAX, seed=5, c00=4.0)
+    b = tp.HIP_calculate_basis(L, LMAX, N, K)
+    rng = np.random.default_rng(6)
+    ra = rng.uniform(0.1, 1, size=(N, 2 * N, K, K, K))
+    rb = rng.uniform(0.1, 1, size=(N, 2 * N, K, K, K))
+    vol = tp.SphericalHarmonicsVolume(c, LMAX, K + 2, 1.0)
+    st = tp.DeviceState(vol, b)
+    Ha, Hb = b.histogram(ra), b.histogram(rb)
+    H2 = b.histogram(ra, range(2, K + 1))
+    for _ in range(2):
+        for H in (Ha, Hb, H2):
+            assert st.energy(H) == tp.energy(vol, b, H)
+    for mode in ("fp16x3", "fp16x3_t64", "fp64"):
+        b.set_precision(mode)
+        assert st.energy(Ha) == tp.energy(vol, b, Ha)
+    del Hb
+    rc = rng.uniform(0.1, 1, size=(N, 2 * N, K, K, K))
+    Hc = b.histogram(rc)
+    assert st.energy(Hc) == tp.energy(vol, b, Hc)
+    tp.rotate_all_at_once(st, b, Ha, 20, seed=3, ntemp=4)
+    moved = tp.SphericalHarmonicsVolume(st.read(), LMAX, K + 2, 1.0)
+    assert st.energy(Ha) == tp.energy(moved, b, Ha)
+
+
 def test_anneal_matches_oracle(tp, oracle):
     """Device-resident annealing vs the oracle driver with the same Philox
     streams: energies within tolerance and identical accept decisions."""
