@@ -924,6 +924,44 @@ struct McArgs {
 
 constexpr int kFinalCols = 32;   // columns per k_final block (8 threads each)
 
+// What follows an evaluation's energy E (one thread): the energy out, and in
+// the Monte Carlo modes the temperature probe (1), the initial energy (2) or
+// the Metropolis decision of step mc.step (3; evaluate_step,
+// determination.jl:491-515).
+__device__ __forceinline__ void final_energy(double E, double *__restrict__ Eout, McArgs mc)
+{
+    Eout[0] = E;
+    if (mc.mode == 1) {
+        mc.temp[mc.step] = E;
+    } else if (mc.mode == 2) {
+        mc.scal[0] = E;
+    } else if (mc.mode == 3) {
+        if (mc.ctl && mc.ctl[0]) return;   // the run has met its stopping rule
+        const double T = mc.scal[1] * mc.decay;
+        mc.scal[1] = T;
+        const double dE = E - mc.scal[0];
+        const double ap = exp(-dE / T);
+        const double u = dev_uniform(mc.seed, mc.step, TP_TAG_ACC);
+        const int acc = (dE < 0.0 || (dE > 0.0 && u < ap)) ? 1 : 0;
+        if (acc) {
+            for (int i = 0; i < mc.nl; ++i) mc.stepsizes[i] *= mc.factor;
+            mc.scal[0] = E;
+            mc.sel[0] ^= 1;
+        } else {
+            for (int i = 0; i < mc.nl; ++i) mc.stepsizes[i] /= mc.factor;
+        }
+        const uint32_t t = mc.step - 1 - mc.tr0;
+        mc.trE[t] = mc.scal[0];
+        mc.trEn[t] = E;
+        mc.trA[t] = acc;
+        if (mc.ctl) {
+            mc.ctl[1] = (int)mc.step;
+            mc.ctl[2] += acc;
+            if (mc.minstep > 0.0 && mc.stepsizes[mc.nl - 1] <= mc.minstep) mc.ctl[0] = 1;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, int n_rt, int64_t ldp, int ncols,
                                                const double *__restrict__ qsum, const double *__restrict__ tf,
                                                double *__restrict__ blockpart, unsigned *__restrict__ counter,
@@ -975,36 +1013,7 @@ __global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, 
     if (tid != 0) return;
     const double E = fin[0];
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    Eout[0] = E;
-    if (mc.mode == 1) {
-        mc.temp[mc.step] = E;
-    } else if (mc.mode == 2) {
-        mc.scal[0] = E;
-    } else if (mc.mode == 3) {
-        if (mc.ctl && mc.ctl[0]) return;   // the run has met its stopping rule
-        const double T = mc.scal[1] * mc.decay;
-        mc.scal[1] = T;
-        const double dE = E - mc.scal[0];
-        const double ap = exp(-dE / T);
-        const double u = dev_uniform(mc.seed, mc.step, TP_TAG_ACC);
-        const int acc = (dE < 0.0 || (dE > 0.0 && u < ap)) ? 1 : 0;
-        if (acc) {
-            for (int i = 0; i < mc.nl; ++i) mc.stepsizes[i] *= mc.factor;
-            mc.scal[0] = E;
-            mc.sel[0] ^= 1;
-        } else {
-            for (int i = 0; i < mc.nl; ++i) mc.stepsizes[i] /= mc.factor;
-        }
-        const uint32_t t = mc.step - 1 - mc.tr0;
-        mc.trE[t] = mc.scal[0];
-        mc.trEn[t] = E;
-        mc.trA[t] = acc;
-        if (mc.ctl) {
-            mc.ctl[1] = (int)mc.step;
-            mc.ctl[2] += acc;
-            if (mc.minstep > 0.0 && mc.stepsizes[mc.nl - 1] <= mc.minstep) mc.ctl[0] = 1;
-        }
-    }
+    final_energy(E, Eout, mc);
 }
 
 // ---------------------------------------------------------------------------
