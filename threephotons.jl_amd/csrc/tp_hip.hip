@@ -140,7 +140,7 @@ struct tp_basis {
     float *d_ew8 = nullptr;             // [..] weights, groups padded to multiples of 8 with zeros
     uint32_t *d_eidx8 = nullptr;        // [..]
     int *d_gstart8 = nullptr;           // [n_groups + 1]
-    double *d_shsc = nullptr;           // [K] shell scales of the mixed-precision triple products
+    double *d_shsc = nullptr;           // [2K] shell scales of the mixed-precision triple products, shell norms
     int *d_work = nullptr;         // [nwork] lt | half << 16, most expensive first
     // sorted l-triples (k_triple_sym; the library's own weights, K <= 32)
     bool use_sym = false;
@@ -287,10 +287,13 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     hipStream_t st = ctx->stream;
     const int prec = force64 ? TP_PREC_FP64 : b->prec;
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    if (b->use_mfma && prec != TP_PREC_FP64 && prec != TP_PREC_FP16X3_T64) {
-        // mixed precision: compensated FP16 triple products
+    const bool t16 = b->use_mfma && prec != TP_PREC_FP64 && prec != TP_PREC_FP16X3_T64;   // compensated FP16 triples
+    const bool c16 = prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16 || prec == TP_PREC_FP16X3_T64;   // FP16 contraction
+    if (t16 || c16)   // shell scales (of the FP16 triple products) and norms (the contraction's column scales)
         hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)b->K), dim3(256), 0, st, b0, b1, sel, flip, b->n_c, b->L,
                            sc.shsc);
+    if (t16) {
+        // mixed precision: compensated FP16 triple products
         hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
                            sc.shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
                            b->EmaxLt8, b->Prows, sc.As, b->kl_pad);
@@ -317,11 +320,11 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
         const dim3 gs((unsigned)(b->kl_pad / 64), (unsigned)(b->ldk / 64));
         if (prec != TP_PREC_BF16) {
-            hipLaunchKernelGGL(k_split_as<1>, gs, dim3(256), 0, st, sc.As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
-                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, sc.Ahi, sc.Alo, sc.Ainv);
+            hipLaunchKernelGGL(k_split_as<1>, gs, dim3(256), 0, st, sc.As, b->kl_pad, (int)b->n_lt, b->ldk,
+                               sc.shsc + b->K, b->K, (int)b->klength, sc.Ahi, sc.Alo, sc.Ainv);
         } else {
-            hipLaunchKernelGGL(k_split_as<2>, gs, dim3(256), 0, st, sc.As, b->kl_pad, (int)b->n_lt, b->ldk, b0, b1,
-                               sel, flip, b->n_c, b->K, b->L, (int)b->klength, sc.Ahi, nullptr, sc.Ainv);
+            hipLaunchKernelGGL(k_split_as<2>, gs, dim3(256), 0, st, sc.As, b->kl_pad, (int)b->n_lt, b->ldk,
+                               sc.shsc + b->K, b->K, (int)b->klength, sc.Ahi, nullptr, sc.Ainv);
         }
         TP_HIP(hipGetLastError());
         if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
@@ -1768,7 +1771,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         (rc = dev_alloc(&b->d_blk, b->n_fb)) ||
         (rc = dev_alloc(&b->d_counter, 1)) || (rc = dev_alloc(&b->d_out, 8)) ||
         (rc = dev_alloc(&b->d_raw, (size_t)K * LMAX * LMAX)) || (rc = dev_alloc(&b->d_coef, (size_t)K * b->n_c)) ||
-        (rc = dev_alloc(&b->d_shsc, (size_t)K))) {
+        (rc = dev_alloc(&b->d_shsc, (size_t)2 * K))) {
         tp_basis_destroy(b);
         return rc;
     }
@@ -1992,7 +1995,7 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
         sizeof(double) * (size_t)nrt * 2 * b->kl_pad,         // 4 energy partial sums
         sizeof(double) * (size_t)b->n_fb,                     // 5 block totals
         sizeof(unsigned),                                     // 6 k_final counter
-        sizeof(double) * (size_t)b->K,                        // 7 shell scales
+        sizeof(double) * (size_t)2 * b->K,                    // 7 shell scales and norms
         mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 8 As hi
         mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 9 As lo
         mixed16 ? sizeof(double) * (size_t)b->kl_pad : 0,              // 10 1 / column scales

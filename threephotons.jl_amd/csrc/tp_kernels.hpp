@@ -600,34 +600,20 @@ __device__ __forceinline__ uint16_t bf16_bits(double x) { return __builtin_bit_c
 //   |As(kt, lt)| <= |c_k1,l1| |c_k2,l2| |c_k3,l3| <= n_k1 n_k2 n_k3,
 // n_k = max_l |c_k,l| (the 3j coupling of three unit vectors has modulus <= 1
 // by the orthogonality of the Clebsch-Gordan map), so |As s| < 2^14.
+// The shell norms n_k come from k_shell_scales (nk[k]), computed once per
+// evaluation.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As, int64_t kl_pad, int nlt, int ldk,
-                                                  const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
-                                                  const int *__restrict__ sel, int flip, int n_c, int K, int L,
-                                                  int klength, uint16_t *__restrict__ Ahi, uint16_t *__restrict__ Alo,
+                                                  const double *__restrict__ nk, int K, int klength,
+                                                  uint16_t *__restrict__ Ahi, uint16_t *__restrict__ Alo,
                                                   double *__restrict__ inv_scale)
 {
-    const double2 *coef = pick(cb0, cb1, sel, flip);
     __shared__ double tile[64][65];
-    __shared__ double s_n2[64 * 12];   // |c_k,l|^2, k < K <= 64, even l <= 22
     __shared__ double s_nk[64];
     __shared__ double s_sc[64];
     const int kt0 = blockIdx.x * 64, lt0 = blockIdx.y * 64;
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
-    const int nl = L / 2 + 1;
-    for (int i = tid; i < K * nl; i += 256) {
-        const int k = i / nl, l = 2 * (i % nl);
-        const double2 *v = coef + (size_t)k * n_c + l * (l - 1) / 2;
-        double s = 0.0;
-        for (int m = 0; m <= 2 * l; ++m) s += v[m].x * v[m].x + v[m].y * v[m].y;
-        s_n2[i] = s;
-    }
-    __syncthreads();
-    if (tid < K) {
-        double m = 0.0;
-        for (int l = 0; l < nl; ++l) m = fmax(m, s_n2[tid * nl + l]);
-        s_nk[tid] = sqrt(m);
-    }
+    if (tid < K) s_nk[tid] = nk[tid];
     __syncthreads();
     if (tid < 64) {
         double s = 1.0;

@@ -28,7 +28,8 @@ __device__ __forceinline__ void split_h(float x, int j, h16x8 &hi, h16x8 &lo)
 }
 
 // One workgroup per shell k: |c_k,l|^2 per block l in parallel, then the
-// power of two s_k.
+// power of two s_k (out[k]) and the norm n_k = max_l |c_k,l| (out[K + k],
+// read by k_split_as for its column scales).
 __global__ __launch_bounds__(256) void k_shell_scales(const double2 *__restrict__ cb0, const double2 *__restrict__ cb1,
                                                       const int *__restrict__ sel, int flip, int n_c, int L,
                                                       double *__restrict__ out)
@@ -51,6 +52,7 @@ __global__ __launch_bounds__(256) void k_shell_scales(const double2 *__restrict_
         for (int l = 0; l < nl; ++l) m = fmax(m, s_l[l]);
         const double nk = sqrt(m);
         out[blockIdx.x] = nk > 0.0 ? ldexp(1.0, 12 - ilogb(nk)) : 1.0;
+        out[gridDim.x + blockIdx.x] = nk;
     }
 }
 
