@@ -88,3 +88,26 @@ def test_sraar_matches_oracle(tp, po, LMAX, K, iters, cut):
         np.testing.assert_allclose(diff, wdiff, rtol=1e-8)
     Koff = int(np.ceil(cut * K))
     assert np.all(got.coeff[Koff:] == 0)
+
+
+def test_sraar_full_size_first_iterations(tp, po):
+    """The benchmark's sRAAR (the 38-shell point-atom intensity, LMAX = 25,
+    beta 0.75 -> 0.9, tau 350, cutoff 0.3) against the oracle over its first
+    20 iterations from the same phases: per-iteration changes within 1e-9 and
+    coefficients within 1e-8.  (The iteration is chaotic on this grid: a
+    difference of one rounding grows about 3 % per iteration, so two correct
+    implementations part after ~150 iterations; 20 stay far inside.)"""
+    from tp_amd import phases, workload
+    cfg = workload.CONFIGS["full"]
+    atoms = workload.atoms_in_ball(cfg["natoms"], cfg["radius"], cfg["atoms_seed"])
+    truth = tp.intensity_from_atoms(atoms, cfg["KMAX"], cfg["qmax"], cfg["LMAX"], cfg["L"], cfg["ntheta"],
+                                    cfg["nphi"])
+    K, LMAX, iters = truth.KMAX, truth.LMAX, 20
+    ph = np.random.default_rng(2024).uniform(0, 2 * np.pi, (K, 4 * LMAX * LMAX))
+    say(f"full size: device sRAAR, {iters} iterations")
+    got, diff = phases.sRAAR(truth, iters, 0.75, 0.9, 350.0, 0.3, phases=ph, return_diff=True)
+    say("full size: oracle sRAAR")
+    want, wdiff = po.sraar(truth.coeff, LMAX, iters, 0.75, 0.9, 350.0, 0.3, ph)
+    say(f"  changes {np.round(diff[:4], 6)} ... {diff[-1]:.6f}")
+    np.testing.assert_allclose(diff, wdiff, rtol=1e-9)
+    assert np.abs(got.coeff - want).max() <= 1e-8 * np.abs(want).max()

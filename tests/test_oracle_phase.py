@@ -75,3 +75,45 @@ def test_sraar_constraints(po):
     assert len(d) == 4 and all(np.isfinite(d))
     out0, d0 = po.sraar(I, LMAX, 0, 0.75, 0.75, 100.0, 0.5, ph)
     assert d0 == [] and np.all(out0[Koff:] == 0)
+
+
+def _sraar_start(po, I, LMAX, ph):
+    amp = np.array([np.sqrt(np.maximum(po.surface(I[k], LMAX).real.ravel(), 0.0)) for k in range(I.shape[0])])
+    x0 = po._backward_surf(amp * np.exp(1j * ph), LMAX)
+    return amp, x0
+
+
+def test_sraar_one_step_at_beta_0_and_1(po):
+    """RAAR x+ = beta/2 (R_S R_M + I) x + (1 - beta) P_M x (phases.jl:68-70 with
+    R_M = 2B - I, P_S = A): one step at beta = 0 is the modulus step P_M x0 =
+    B(x0) whatever the support projection does, and at beta = 1 it is
+    x0 + A(R_M x0) - B(x0).  The first real-space change pins both (a loop with
+    beta and 1 - beta exchanged, a wrong reflection or a dropped term fails)."""
+    LMAX, K = 5, 6
+    I = random_real_coeff(K, LMAX - 1, LMAX, 9, even=True, c00=5.0, decay=0.1)
+    ph = np.random.default_rng(4).uniform(0, 2 * np.pi, (K, 4 * LMAX * LMAX))
+    amp, x0 = _sraar_start(po, I, LMAX, ph)
+    Bx = po._B(x0, amp, LMAX)
+    _, d0 = po.sraar(I, LMAX, 1, 0.0, 0.0, 100.0, 0.5, ph)
+    want0 = np.abs(x0 - Bx).sum() / np.abs(x0).sum()
+    assert d0[0] == pytest.approx(want0, rel=1e-12)
+    x1 = x0 + po._A(2.0 * Bx - x0, 3) - Bx
+    _, d1 = po.sraar(I, LMAX, 1, 1.0, 1.0, 100.0, 0.5, ph)
+    want1 = np.abs(x0 - x1).sum() / np.abs(x0).sum()
+    assert d1[0] == pytest.approx(want1, rel=1e-12)
+    assert abs(want0 - want1) > 1e-3 * want0      # the support projection is active: the two differ
+
+
+def test_sraar_schedule_limits(po):
+    """beta(n) = e beta0 + (1 - e) beta_max with e = exp(-(n/tau)^3): tau -> 0
+    runs beta_max from the first iteration, tau -> infinity keeps beta0."""
+    LMAX, K, n = 5, 6, 5
+    I = random_real_coeff(K, LMAX - 1, LMAX, 11, even=True, c00=5.0, decay=0.1)
+    ph = np.random.default_rng(6).uniform(0, 2 * np.pi, (K, 4 * LMAX * LMAX))
+    fast = po.sraar(I, LMAX, n, 0.2, 0.9, 1e-9, 0.5, ph)
+    slow = po.sraar(I, LMAX, n, 0.2, 0.9, 1e12, 0.5, ph)
+    at_max = po.sraar(I, LMAX, n, 0.9, 0.9, 100.0, 0.5, ph)
+    at_0 = po.sraar(I, LMAX, n, 0.2, 0.2, 100.0, 0.5, ph)
+    assert np.array_equal(fast[0], at_max[0]) and fast[1] == at_max[1]
+    assert np.array_equal(slow[0], at_0[0]) and slow[1] == at_0[1]
+    assert not np.allclose(at_max[0], at_0[0])
