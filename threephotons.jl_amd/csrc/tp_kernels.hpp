@@ -1203,7 +1203,7 @@ constexpr int kSepMaxKB = 24;   // 2 (L/2 + 1) padded to 4, L <= 22
 inline size_t c3sep_lds_bytes(int kb) { return sizeof(double) * (size_t)kb * 16 * 17; }
 
 template <int OUT, bool ENERGY>
-__global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
+__global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
                                                int n_m2, const double *__restrict__ TA, int64_t ta_plane, int jpad,
                                                const double *__restrict__ TB, int kb, int N, int ncols,
                                                const double *__restrict__ qhat, double *__restrict__ part,
@@ -1267,21 +1267,6 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
 #pragma unroll
     for (int q = 0; q < 6; ++q)
         if (4 * q + lq >= kb) tb[q] = 0.0;
-    // histogram values of this wave's stage-B outputs (transposed blocks:
-    // lane -> column c0 + lq + 4v, row i = 16 ib + l16)
-    double qv[MAXB][4];
-    if (ENERGY) {
-#pragma unroll
-        for (int t = 0; t < MAXB; ++t) {
-            const int ob = wave + NW * t;
-            const int jl = ob / nib, ib = ob % nib, j = min(j0 + jl, twoN - 1), i = min(16 * ib + l16, N - 1);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int col = min(c0 + lq + 4 * v, ncols - 1);
-                qv[t][v] = qhat[(int64_t)col * R + i + N * j];
-            }
-        }
-    }
     __syncthreads();
     // ---- stage B: blocks (j local, i block), K = 2 n_m2 (padded to kb), computed
     // transposed (rows = columns kt, cols = i) so that a lane's c3 values run
@@ -1294,6 +1279,14 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
         const int jl = ob / nib, ib = ob % nib;
         const int j = j0 + jl;
         if (j >= twoN) continue;   // wave-uniform
+        // histogram values of this block (transposed: lane -> column c0 + lq + 4v,
+        // row i = 16 ib + l16), requested before its MFMAs
+        double qv[4];
+        if (ENERGY) {
+            const int iq = min(16 * ib + l16, N - 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) qv[v] = qhat[(int64_t)min(c0 + lq + 4 * v, ncols - 1) * R + iq + N * j];
+        }
         dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int q = 0; q < 6; ++q)
@@ -1309,7 +1302,7 @@ __global__ __launch_bounds__(512) void k_c3sep(const double *__restrict__ G, int
                 if (ENERGY) {
                     const double pv = c > 1e-30 ? c : 1e-30;
                     sp[v] += pv;
-                    sq[v] += qv[t][v] * tp_fastlog(pv, s_lt);
+                    sq[v] += qv[v] * tp_fastlog(pv, s_lt);
                 }
             }
         }
