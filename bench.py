@@ -334,7 +334,7 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
     ms_sweep = 1e3 * el / nsweeps
     kl_total = cols
     flops = 2.0 * basis.R * basis.n_lt * kl_total
-    tile_ms = tune["tile_ms"]["%dx%d/%s" % (tune["tile"][0], tune["tile"][1], tune["schedule"])]
+    tile_ms = tune["tile_ms"][tune["candidate"]]
     basis.set_precision("fp64")
     log(f"batch ({prec}): {len(Ks)} runs K={Ks}, {ne} evals ({cols} k-triple columns) per sweep: "
         f"{ms_sweep:.2f} ms/sweep, {ne / (ms_sweep * 1e-3):.0f} evals/s; tune {tune}")

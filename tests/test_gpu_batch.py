@@ -172,23 +172,25 @@ def test_batch_sweeps(tp, family):
     np.testing.assert_allclose(B2.history(), hist, rtol=1e-10)
 
 
-def test_batch_every_contraction_candidate(tp):
+@pytest.mark.parametrize("prec,ncand", [("fp16x3", 12), ("fp16x3_t64", 4)])
+def test_batch_every_contraction_candidate(tp, prec, ncand):
     """Every contraction candidate of the autotuner, forced in turn, gives the
     single evaluations' energies (N = 32: 2048 rows, so the XCD-partitioned
-    schedule applies to every tile)."""
+    schedule applies to every tile of the dense form; the factored form's
+    candidates are k_hgemm_g's column and row-group shapes)."""
     Ks, nprop, L, N = [4, 7, 5], 4, 4, 32
-    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, L=L, LMAX=6, N=N)
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, L=L, LMAX=6, N=N, prec=prec)
     rng = np.random.default_rng(12)
     q = np.stack([random_quats(rng, L // 2) for _ in range(len(Ks) * nprop)])
     for b in bases:
-        b.set_precision("fp16x3")
+        b.set_precision(prec)
     Es = []
     for g in range(len(Ks)):
         for p in range(nprop):
             Es.append(tp.energy(tp.rotate(starts[g], bases[g], q[g * nprop + p]), bases[g], refs[g]))
     Es = np.array(Es)
-    for cand in range(12):
-        print("candidate", cand, flush=True)
+    for cand in range(ncand):
+        print(prec, "candidate", cand, flush=True)
         B.set_contraction(cand)
         E = B.eval_quats(q)
         assert np.all(np.abs(E - Es) <= 1e-10 * np.abs(Es)), (cand, np.abs(E - Es).max())

@@ -476,8 +476,7 @@ __global__ __launch_bounds__(256) void k_split_batch(const double *__restrict__ 
         const double x = tile[q * 16 + i][kr] * s;
         uint16_t hb, lb = 0;
         if (MODE == 1) {
-            hb = f16_bits(x);
-            lb = f16_bits(x - f16_val(hb));
+            f16_split(x, hb, lb);
         } else {
             hb = bf16_bits(x);
         }

@@ -19,6 +19,7 @@
 #include "tp_kernels.hpp"
 #include "tp_triple16.hpp"
 #include "tp_triple_sym.hpp"
+#include "tp_hgemm.hpp"
 #include "tp_batch.hpp"
 #include "tp_mlp.hpp"
 #include "tp_grad.hpp"
@@ -162,6 +163,13 @@ struct tp_basis {
     uint16_t *d_Fhi = nullptr, *d_Flo = nullptr, *d_Fbf = nullptr;   // [R_pad16][ldk]
     uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;                      // [kl_pad][ldk]
     double *d_Ainv = nullptr;      // [kl_pad] 1 / column scale of As
+    // factored mixed-precision contraction (TP_PREC_FP16X3_T64): G = H . As on
+    // FP16 hi/lo (k_hgemm_g), then the FP64 separated grid
+    uint16_t *d_Hhi = nullptr, *d_Hlo = nullptr;   // [kHgRows][ldh16]  H[class][lt] scaled by 1 / H_inv
+    int ldh16 = 0;                 // n_lt rounded up to 128 (the K of k_hgemm_g)
+    double H_inv = 1.0;
+    int nrb16 = 0;                 // row blocks of 16 classes
+    double *d_nrm2 = nullptr;      // [K][kNrmLd] l-block norms of the evaluated shells
     // device tables
     double *d_H = nullptr;         // [nlt_pad][nm_pad64]   H[lt][class]
     double *d_Bm = nullptr;        // [nm_pad16][R_pad]     Bm[class][r]
@@ -268,11 +276,13 @@ struct EvalScratch {
     double *As, *G, *part, *blk, *Ainv, *shsc;
     unsigned *counter;
     uint16_t *Ahi, *Alo;
+    double *nrm2;   // [K][kNrmLd] l-block norms (k_triple_sym -> k_hgemm_g)
 };
 
 static EvalScratch basis_scratch(const tp_basis *b)
 {
-    return EvalScratch{b->d_As, b->d_G, b->d_part, b->d_blk, b->d_Ainv, b->d_shsc, b->d_counter, b->d_Ahi, b->d_Alo};
+    return EvalScratch{b->d_As,      b->d_G,   b->d_part, b->d_blk, b->d_Ainv,
+                       b->d_shsc,    b->d_counter, b->d_Ahi, b->d_Alo, b->d_nrm2};
 }
 
 // Enqueue one evaluation on coefficients (b0, b1, sel, flip): As, G, c3 (+energy).
@@ -288,7 +298,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     const int prec = force64 ? TP_PREC_FP64 : b->prec;
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
     const bool t16 = b->use_mfma && prec != TP_PREC_FP64 && prec != TP_PREC_FP16X3_T64;   // compensated FP16 triples
-    const bool c16 = prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16 || prec == TP_PREC_FP16X3_T64;   // FP16 contraction
+    const bool c16 = prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16;   // dense FP16 contraction F . As
     if (t16 || c16)   // shell scales (of the FP16 triple products) and norms (the contraction's column scales)
         hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)b->K), dim3(256), 0, st, b0, b1, sel, flip, b->n_c, b->L,
                            sc.shsc);
@@ -300,7 +310,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     } else if (b->use_sym) {   // FP64 (also TP_PREC_FP16X3_T64, which needs the tables)
         hipLaunchKernelGGL(k_triple_sym, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1, sel,
                            flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
-                           b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad);
+                           b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad,
+                           prec == TP_PREC_FP16X3_T64 ? sc.nrm2 : nullptr);
     } else if (b->use_mfma) {
         hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(1024), b->mfma_lds, st, b0, b1, sel, flip,
                            b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_work, b->Lw, b->EmaxLt,
@@ -314,7 +325,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     if (timing) TP_HIP(hipEventRecord(ctx->ev[1], st));
     const int ncols = r ? (int)r->ncols : (int)b->klength;
     int n_rt_fin = b->n_rt;
-    if (prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16 || prec == TP_PREC_FP16X3_T64) {
+    if (c16) {
         // mixed precision: c3 = F . As with 16-bit MFMA operands
         const double *qh = r ? r->d_qhat : nullptr;
         const dim3 grid16((unsigned)(b->n_rt16 * b->n_ct16));
@@ -352,6 +363,13 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
                                         (unsigned)((b->n_m + kLamClasses - 1) / kLamClasses)),
                            dim3(kLamThreads), 0, st, sc.As, b->kl_pad, b->d_ktk, (int)b->klength, b->d_lamY,
                            b->d_lament, b->d_lamcs, b->n_m, sc.G, b->kl_pad);
+    } else if (prec == TP_PREC_FP16X3_T64) {
+    // G = H . As on FP16 hi/lo MFMA operands (As split while staged), FP64 out
+    // (16 columns x 4 row blocks per workgroup; rows past the classes up to
+    // nm_pad16 written as zeros for the dense Bm . G form)
+    hipLaunchKernelGGL((k_hgemm_g<16, 4, true, 128>), dim3((unsigned)(b->kl_pad / 16), (unsigned)((b->nrb16 + 3) / 4)),
+                       dim3(256), 0, st, b->d_Hhi, b->d_Hlo, b->ldh16, b->H_inv, sc.As, b->kl_pad, (int)b->n_lt,
+                       b->ldh16, nullptr, sc.nrm2, b->L, b->K, (int)b->nm_pad16, sc.G, b->kl_pad);
     } else {
     // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
     // XCD-grouped column tiles: each As column block is fetched from HBM once
@@ -449,7 +467,8 @@ struct tp_batch {
     double2 *d_cur = nullptr, *d_prop = nullptr;
     double *d_quat = nullptr, *d_As = nullptr, *d_sc = nullptr, *d_isc = nullptr, *d_part = nullptr;
     double *d_E = nullptr, *d_Ehist = nullptr, *d_shsc = nullptr;   // d_shsc [ne][K] shell scales
-    uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;
+    uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;   // dense F . As (TP_PREC_FP16X3, TP_PREC_BF16)
+    double *d_G = nullptr;                         // [16 nrb16][ldA] G = H . As (TP_PREC_FP16X3_T64)
     const double **d_qtab = nullptr;
     int *d_nvalid = nullptr;
     double2 *d_raw = nullptr;      // staging of host coefficients (plain evaluations)
@@ -489,7 +508,7 @@ static void batch_free(tp_batch *B)
     void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_swork, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As,
                     B->d_sc,  B->d_isc, B->d_part, B->d_E,    B->d_Ehist,  B->d_Ahi,    B->d_Alo,
                     (void *)B->d_qtab, B->d_nvalid, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn, B->d_shsc,
-                    B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi};
+                    B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi, B->d_G};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete B;
@@ -524,8 +543,11 @@ static int batch_sym_work(tp_batch *B)
         for (const auto &tc : b->h_sym_cost) {
             const int n3 = 2 * b->h_lt_l3[tc.first] + 1;
             const double cost = tc.second + ((n3 + 1) / 2) * 2.0 * nh;
+            const int4 li = b->h_ltinfo[tc.first];
+            const bool sym12 = (li.w & 0xff) == ((li.w >> 8) & 0xff);   // items h1 < h2 stored by their mirrors
             for (int h1 = 0; h1 < nh; ++h1)
-                for (int h2 = 0; h2 < nh; ++h2) all.push_back({cost, make_int2(e, tc.first | (h1 << 16) | (h2 << 20))});
+                for (int h2 = 0; h2 < nh; ++h2)
+                    if (!(sym12 && h1 < h2)) all.push_back({cost, make_int2(e, tc.first | (h1 << 16) | (h2 << 20))});
         }
     }
     std::stable_sort(all.begin(), all.end(), [](const std::pair<double, int2> &a, const std::pair<double, int2> &c) {
@@ -575,9 +597,12 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     if (!b->use_mfma) return tp_fail(TP_E_UNSUPPORTED, "batches need the MFMA triple products (K <= 27)");
     if (b->prec != TP_PREC_FP16X3 && b->prec != TP_PREC_BF16 && b->prec != TP_PREC_FP16X3_T64)
         return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs TP_PREC_FP16X3, TP_PREC_FP16X3_T64 or TP_PREC_BF16");
-    if (b->R % 128 != 0) return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs 2N^2 to be a multiple of 128");
-    if (b->prec == TP_PREC_FP16X3_T64 && !b->use_sym)
-        return tp_fail(TP_E_UNSUPPORTED, "TP_PREC_FP16X3_T64 needs the sorted l-triple tables");
+    const bool fac = b->prec == TP_PREC_FP16X3_T64;   // factored contraction G = H . As, c3 = Bm . G
+    if (!fac && b->R % 128 != 0)
+        return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs 2N^2 to be a multiple of 128");
+    if (fac && (!b->use_sym || !b->use_sep || !b->d_Hhi))
+        return tp_fail(TP_E_UNSUPPORTED, "TP_PREC_FP16X3_T64 batches need the sorted l-triple tables and the "
+                                         "separated grid (N <= 32)");
     tp_batch *B = new tp_batch();
     B->ctx = ctx;
     B->b = b;
@@ -643,8 +668,9 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         (rc = dev_alloc(&B->d_isc, B->ldA)) || (rc = dev_alloc(&B->d_part, nrt_max * 2 * B->ldA)) ||
         (rc = dev_alloc(&B->d_E, B->ne)) || (rc = dev_alloc(&B->d_Ehist, (size_t)std::max(1, hist_cap) * ngroups)) ||
         (rc = dev_alloc(&B->d_shsc, (size_t)B->ne * b->K)) ||
-        (rc = dev_alloc(&B->d_Ahi, (size_t)B->ldA * b->ldk)) ||
-        (B->prec != TP_PREC_BF16 && (rc = dev_alloc(&B->d_Alo, (size_t)B->ldA * b->ldk))) ||
+        (!fac && (rc = dev_alloc(&B->d_Ahi, (size_t)B->ldA * b->ldk))) ||
+        (!fac && B->prec != TP_PREC_BF16 && (rc = dev_alloc(&B->d_Alo, (size_t)B->ldA * b->ldk))) ||
+        (fac && (rc = dev_alloc(&B->d_G, (size_t)16 * b->nrb16 * B->ldA))) ||
         (rc = dev_alloc(&B->d_qtab, qtab.size())) || (rc = dev_alloc(&B->d_nvalid, nvalid.size())) ||
         (!sweep && (rc = dev_alloc(&B->d_raw, (size_t)b->K * b->LMAX * b->LMAX)))) {
         batch_free(B);
@@ -718,9 +744,53 @@ static void batch_gemm(const tp_batch *B, hipStream_t st, int cand)
     }
 }
 
+// Factored contraction candidates (TP_PREC_FP16X3_T64): k_hgemm_g with 64
+// columns x all row blocks, 32 columns x all row blocks, 64 columns x 3 row
+// blocks, 16 columns x 4 row blocks per workgroup; then k_c3sep over the
+// batch's columns.
+constexpr int kNumFac = 4;
+static const int kFacTiles[kNumFac][1] = {{64}, {32}, {64}, {16}};
+
+template <int BN, int NRB>
+static void batch_hgemm_launch(const tp_batch *B, hipStream_t st)
+{
+    const tp_basis *b = B->b;
+    const unsigned ny = (unsigned)((b->nrb16 + NRB - 1) / NRB);
+    hipLaunchKernelGGL((k_hgemm_g<BN, NRB, false>), dim3((unsigned)(B->ldA / BN), ny), dim3(256), 0, st, b->d_Hhi,
+                       b->d_Hlo, b->ldh16, b->H_inv, B->d_As, B->ldA, (int)b->n_lt, b->ldh16, B->d_sc, nullptr, b->L,
+                       b->K, b->n_m, B->d_G, B->ldA);
+}
+
+// rows (classes) per workgroup and columns of factored candidate cand
+static void fac_shape(const tp_basis *b, int cand, int *rows, int *cols)
+{
+    const int nrb = b->nrb16;
+    const int nr[kNumFac] = {nrb <= 9 ? 9 : 12, nrb <= 10 ? 10 : 12, 3, 4};
+    *rows = 16 * nr[cand];
+    *cols = kFacTiles[cand][0];
+}
+
+static void batch_contract_fac(const tp_batch *B, hipStream_t st, int cand)
+{
+    const tp_basis *b = B->b;
+    const int nrb = b->nrb16;
+    switch (cand) {
+    case 0: if (nrb <= 9) batch_hgemm_launch<64, 9>(B, st); else batch_hgemm_launch<64, 12>(B, st); break;
+    case 1: if (nrb <= 10) batch_hgemm_launch<32, 10>(B, st); else batch_hgemm_launch<32, 12>(B, st); break;
+    case 2: batch_hgemm_launch<64, 3>(B, st); break;
+    default: batch_hgemm_launch<16, 4>(B, st); break;
+    }
+    const int64_t tap = (int64_t)b->n_m * b->jpad;
+    hipLaunchKernelGGL((k_c3sep<0, true, true>), dim3((unsigned)(B->ldA / 16), (unsigned)b->njc), dim3(512),
+                       c3sep_lds_bytes(b->sep_kb), st, B->d_G, B->ldA, b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad,
+                       b->d_TB, b->sep_kb, b->N, 0, nullptr, B->d_part, B->ldA, nullptr, (int64_t)0, b->d_logtab,
+                       B->d_qtab, B->d_nvalid);
+}
+
 // energy-sum rows the chosen contraction leaves in part
 static int batch_part_rows(const tp_batch *B)
 {
+    if (B->prec == TP_PREC_FP16X3_T64) return B->b->njc;
     return kTiles[B->gcand][2] == 2 ? 1 : (int)(B->b->R_pad16 / kTiles[B->gcand][0]);
 }
 
@@ -761,8 +831,12 @@ static void batch_final(const tp_batch *B, hipStream_t st)
 static int batch_enqueue_eval(const tp_batch *B, hipStream_t st)
 {
     batch_triple(B, st);
-    batch_split(B, st);
-    batch_gemm(B, st, B->gcand);
+    if (B->prec == TP_PREC_FP16X3_T64) {
+        batch_contract_fac(B, st, B->gcand);
+    } else {
+        batch_split(B, st);
+        batch_gemm(B, st, B->gcand);
+    }
     batch_final(B, st);
     TP_HIP(hipGetLastError());
     return 0;
@@ -857,9 +931,23 @@ static int batch_autotune(tp_batch *B, int reps)
     }
     if (int rc = batch_upload_work(B, best_order)) return rc;
     batch_triple(B, st);
-    batch_split(B, st);
     best = 1e30;
     int best_t = 0;
+    if (B->prec == TP_PREC_FP16X3_T64) {
+        for (int t = 0; t < kNumTiles; ++t) {
+            B->tune_ms[2 + t] = 1e30;
+            if (t >= kNumFac) continue;
+            double ms = 0;
+            if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_contract_fac(B, s, t); }, &ms)) return rc;
+            B->tune_ms[2 + t] = ms;
+            if (ms < best) { best = ms; best_t = t; }
+        }
+        B->gcand = best_t;
+        TP_HIP(hipStreamSynchronize(st));
+        if (B->exec) return batch_capture(B, B->g_decay, B->g_factor);
+        return 0;
+    }
+    batch_split(B, st);
     const int nct64 = (int)(B->ldA / 64);
     for (int t = 0; t < kNumTiles; ++t) {
         double ms = 0;
@@ -1250,7 +1338,7 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
                     b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2,
                     b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs, b->d_sym_work, b->d_perms, b->d_gorder,
-                    b->d_symtab, b->d_symoff};
+                    b->d_symtab, b->d_symoff, b->d_Hhi, b->d_Hlo, b->d_nrm2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete b;
@@ -1642,8 +1730,11 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
                 const double s2 = ((n3 + 1) / 2) * 2.0 * (nh > 1 ? 2 : 1);
                 const double cost = 2.0 * s1 + s2 + 0.25 * perms[(size_t)ts * 8];
                 b->h_sym_cost.push_back({ts, 2.0 * s1 + 0.25 * perms[(size_t)ts * 8]});
+                // with l1 = l2 the items h1 < h2 are stored by their mirror images
+                const bool sym12 = (ltinfo[ts].w & 0xff) == ((ltinfo[ts].w >> 8) & 0xff);
                 for (int h1 = 0; h1 < nh; ++h1)
-                    for (int h2 = 0; h2 < nh; ++h2) wl.push_back({-cost, ts | (h1 << 16) | (h2 << 20)});
+                    for (int h2 = 0; h2 < nh; ++h2)
+                        if (!(sym12 && h1 < h2)) wl.push_back({-cost, ts | (h1 << 16) | (h2 << 20)});
             }
             std::stable_sort(wl.begin(), wl.end());
             for (auto &w : wl) sym_work.push_back(w.second);
@@ -1771,7 +1862,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         (rc = dev_alloc(&b->d_blk, b->n_fb)) ||
         (rc = dev_alloc(&b->d_counter, 1)) || (rc = dev_alloc(&b->d_out, 8)) ||
         (rc = dev_alloc(&b->d_raw, (size_t)K * LMAX * LMAX)) || (rc = dev_alloc(&b->d_coef, (size_t)K * b->n_c)) ||
-        (rc = dev_alloc(&b->d_shsc, (size_t)2 * K))) {
+        (rc = dev_alloc(&b->d_shsc, (size_t)2 * K)) || (rc = dev_alloc(&b->d_nrm2, (size_t)K * kNrmLd))) {
         tp_basis_destroy(b);
         return rc;
     }
@@ -1822,7 +1913,52 @@ int tp_basis_set_precision(tp_basis *b, int mode)
         return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64 and L <= 44");
     if (mode != TP_PREC_FP64 && b->lam)
         return tp_fail(TP_E_UNSUPPORTED, "mixed precision needs lambda dq = 0 (the curved Ewald sphere runs in FP64)");
-    if ((mode == TP_PREC_FP16X3 || mode == TP_PREC_BF16 || mode == TP_PREC_FP16X3_T64) && !b->d_Fhi) {
+    if (mode == TP_PREC_FP16X3_T64 && !b->d_Hhi) {
+        // H[class][lt] (FP64 on the host) scaled by a power of two into
+        // [2^13, 2^14) and split into FP16 hi/lo for k_hgemm_g
+        if (b->n_m > 16 * kHgMaxRb || b->L / 2 >= kNrmLd)
+            return tp_fail(TP_E_UNSUPPORTED, "TP_PREC_FP16X3_T64 supports at most 192 classes (L <= 20)");
+        // the column scales come from the l-block norms that the work items of
+        // the l-triples (l, l, l) write: every even l <= L needs one
+        for (int l = 0; l <= b->L; l += 2) {
+            bool found = false;
+            for (const int4 &t : b->h_ltinfo)
+                found = found || (t.z == l && (t.w & 0xff) == l && ((t.w >> 8) & 0xff) == l);
+            if (!found) return tp_fail(TP_E_UNSUPPORTED, "TP_PREC_FP16X3_T64 needs the l-triple (l, l, l) of every l");
+        }
+        const int nrb = (b->n_m + 15) / 16;
+        double hmax = 0.0;
+        for (int64_t t = 0; t < b->n_lt; ++t)
+            for (int c = 0; c < b->n_m; ++c) hmax = std::max(hmax, std::fabs(b->h_H[(size_t)t * b->nm_pad64 + c]));
+        const double sH = hmax > 0.0 ? std::ldexp(1.0, 13 - std::ilogb(hmax)) : 1.0;
+        const int ldh = (int)round_up(b->n_lt, 128);
+        std::vector<uint16_t> hh((size_t)kHgRows * ldh, 0), hl(hh.size(), 0);
+        for (int c = 0; c < b->n_m; ++c)
+            for (int64_t t = 0; t < b->n_lt; ++t) {
+                const double x = b->h_H[(size_t)t * b->nm_pad64 + c] * sH;
+                const size_t i = (size_t)c * ldh + t;
+                hh[i] = host_f16(x);
+                hl[i] = host_f16(x - host_f16_val(hh[i]));
+            }
+        uint16_t *dh = nullptr, *dl = nullptr;
+        if (int rc = dev_alloc(&dh, hh.size())) return rc;
+        if (int rc = dev_alloc(&dl, hl.size())) {
+            (void)hipFree(dh);
+            return rc;
+        }
+        if (hipMemcpy(dh, hh.data(), sizeof(uint16_t) * hh.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dl, hl.data(), sizeof(uint16_t) * hl.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(dh);
+            (void)hipFree(dl);
+            return tp_fail(TP_E_HIP, "hipMemcpy failed");
+        }
+        b->d_Hhi = dh;
+        b->d_Hlo = dl;
+        b->H_inv = 1.0 / sH;
+        b->nrb16 = nrb;
+        b->ldh16 = ldh;
+    }
+    if ((mode == TP_PREC_FP16X3 || mode == TP_PREC_BF16) && !b->d_Fhi) {
         // F[r][lt] = sum_class Bm[class][r] H[lt][class] in FP64, then scaled by a
         // power of two into [2^13, 2^14) and split into FP16 hi/lo and BF16
         const int64_t nlt = b->n_lt;
@@ -1985,7 +2121,7 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
     if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
     if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "coefficient vector shorter than LMAX^2");
     if (!ctx->scratch) return tp_fail(TP_E_STATE, "no scratch address for this device");
-    const bool mixed16 = b->prec == TP_PREC_FP16X3 || b->prec == TP_PREC_BF16 || b->prec == TP_PREC_FP16X3_T64;
+    const bool mixed16 = b->prec == TP_PREC_FP16X3 || b->prec == TP_PREC_BF16;   // 16-bit copies of As
     const int64_t nrt = std::max<int64_t>(std::max(b->n_rt, b->njc), b->n_rt16);
     const size_t want[] = {
         sizeof(double2) * (size_t)b->K * ld,                  // 0 raw coefficients
@@ -1999,6 +2135,7 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
         mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 8 As hi
         mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 9 As lo
         mixed16 ? sizeof(double) * (size_t)b->kl_pad : 0,              // 10 1 / column scales
+        sizeof(double) * (size_t)b->K * kNrmLd,               // 11 l-block norms (FP16X3_T64)
     };
     constexpr int nbuf = sizeof(want) / sizeof(want[0]);
     size_t off[nbuf], total = 0;
@@ -2019,7 +2156,7 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
                    reinterpret_cast<double *>(base + off[4]), reinterpret_cast<double *>(base + off[5]),
                    reinterpret_cast<double *>(base + off[10]), reinterpret_cast<double *>(base + off[7]),
                    reinterpret_cast<unsigned *>(base + off[6]), reinterpret_cast<uint16_t *>(base + off[8]),
-                   reinterpret_cast<uint16_t *>(base + off[9])};
+                   reinterpret_cast<uint16_t *>(base + off[9]), reinterpret_cast<double *>(base + off[11])};
     hipStream_t st = ctx->stream;
     // the array holds the last call's values: As and G padding must read as zero
     // and the arrival counter must start at zero (As, G and the counter are
@@ -3855,7 +3992,11 @@ int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out)
     if (int rc = batch_autotune(B, reps)) return rc;
     if (ms_out)
         for (int i = 0; i < 2 + kNumTiles; ++i) ms_out[i] = B->tune_ms[i];
-    if (config_out) {
+    if (config_out && B->prec == TP_PREC_FP16X3_T64) {
+        fac_shape(B->b, B->gcand, &config_out[0], &config_out[1]);
+        config_out[2] = B->order;
+        config_out[3] = 4 + B->gcand;
+    } else if (config_out) {
         config_out[0] = kTiles[B->gcand][0];
         config_out[1] = kTiles[B->gcand][1];
         config_out[2] = B->order;
@@ -4028,7 +4169,8 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
     }
     hipStream_t st = ctx->stream;
     if ((bm->prec != TP_PREC_FP16X3 && bm->prec != TP_PREC_BF16 && bm->prec != TP_PREC_FP16X3_T64) ||
-        !bm->use_mfma || bm->R % 128 != 0) {
+        !bm->use_mfma || (bm->prec != TP_PREC_FP16X3_T64 && bm->R % 128 != 0) ||
+        (bm->prec == TP_PREC_FP16X3_T64 && (!bm->use_sym || !bm->use_sep))) {
         // one evaluation after another on the device
         double *dE = nullptr;
         if (int rc = dev_alloc(&dE, n)) return rc;
@@ -4373,7 +4515,9 @@ int tp_batch_set_contraction(tp_batch *B, int cand)
 {
     if (!B || cand < 0 || cand >= kNumTiles) return tp_fail(TP_E_ARG, "contraction candidate out of range");
     if (int rc = check_ctx(B->ctx)) return rc;
-    if (kTiles[cand][2] == 3) {
+    if (B->prec == TP_PREC_FP16X3_T64 && cand >= kNumFac)
+        return tp_fail(TP_E_ARG, "TP_PREC_FP16X3_T64 batches have contraction candidates 0..3");
+    if (B->prec != TP_PREC_FP16X3_T64 && kTiles[cand][2] == 3) {
         const int nrt = (int)(B->b->R_pad16 / kTiles[cand][0]), nct = (int)(B->ldA / kTiles[cand][1]);
         if (nrt % 8 != 0 || (nrt * nct) % 8 != 0)
             return tp_fail(TP_E_UNSUPPORTED, "the XCD-partitioned schedule needs row and tile counts divisible by 8");

@@ -242,6 +242,9 @@ __device__ __forceinline__ S1Ops s1_fetch(int ks, int E, int s, int K, int k1s, 
     return o;
 }
 
+// Row length of the per-shell l-block norms |c_k,l|^2 (nrm2[k][l/2], even l <= 30).
+constexpr int kNrmLd = 16;
+
 // Column of k-triple kt in an As row: kt-tile j = kt / 64 of an evaluation
 // sits in 64-column block cbase + j * nprop + p (evaluations of one batch
 // group interleaved by tile; a single evaluation has cbase = 0, nprop = 1,
@@ -593,6 +596,17 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint16_t f16_bits(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 __device__ __forceinline__ double f16_val(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }
 __device__ __forceinline__ uint16_t bf16_bits(double x) { return __builtin_bit_cast(uint16_t, (__bf16)(float)x); }
+// x = hi + lo + e with |e| <= 2^-22 |x| (x scaled into the FP16 range): x is
+// rounded to FP32 first and split there with the hardware conversions (the
+// direct double -> half conversion is a long software sequence on gfx950);
+// xf - hi is exact in FP32.
+__device__ __forceinline__ void f16_split(double x, uint16_t &hb, uint16_t &lb)
+{
+    const float xf = (float)x;
+    const _Float16 h = (_Float16)xf;
+    hb = __builtin_bit_cast(uint16_t, h);
+    lb = __builtin_bit_cast(uint16_t, (_Float16)(xf - (float)h));
+}
 
 // As [lt][kt] (FP64) -> 16-bit copies in [kt][lt] layout (lt contiguous, row
 // length ldk), one 64 x 64 tile per block, with a per-column power-of-two
@@ -646,8 +660,7 @@ __global__ __launch_bounds__(256) void k_split_as(const double *__restrict__ As,
             const double x = tile[q * 16 + i][kr] * s;
             uint16_t hb, lb = 0;
             if (MODE == 1) {
-                hb = f16_bits(x);
-                lb = f16_bits(x - f16_val(hb));
+                f16_split(x, hb, lb);
             } else {
                 hb = bf16_bits(x);
             }
@@ -1202,13 +1215,15 @@ constexpr int kSepMaxKB = 24;   // 2 (L/2 + 1) padded to 4, L <= 22
 // Dynamic LDS of k_c3sep: CS[kb][16][17] doubles.
 inline size_t c3sep_lds_bytes(int kb) { return sizeof(double) * (size_t)kb * 16 * 17; }
 
-template <int OUT, bool ENERGY>
+template <int OUT, bool ENERGY, bool BATCH = false>
 __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
                                                int n_m2, const double *__restrict__ TA, int64_t ta_plane, int jpad,
                                                const double *__restrict__ TB, int kb, int N, int ncols,
                                                const double *__restrict__ qhat, double *__restrict__ part,
                                                int64_t ldp, double *__restrict__ out, int64_t ldo,
-                                               const double *__restrict__ logtab)
+                                               const double *__restrict__ logtab,
+                                               const double *const *__restrict__ qtab = nullptr,
+                                               const int *__restrict__ nvalid = nullptr)
 {
     __shared__ double s_lt[ENERGY ? kLogTabSize : 1];
     __shared__ double red[8][2][16];
@@ -1222,6 +1237,15 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
     // stage A wait for them.  Three workgroups fit a CU (dynamic LDS sized kb).
     constexpr int NW = 8, MAXB = 4;   // stage-B blocks per wave: 16 j x (N/16) i-blocks / 8 waves (N <= 32)
     const int c0 = blockIdx.x * 16, jc = blockIdx.y, j0 = jc * 16;
+    // BATCH: the 16 columns lie in one 64-column block of a batch, whose
+    // histogram columns start at qtab[block] (nvalid[block] of them are real)
+    const int nv = BATCH ? nvalid[c0 >> 6] - (c0 & 63) : ncols - c0;
+    const double *qbase = nullptr;
+    if (ENERGY) {
+        if (!BATCH) qbase = qhat + (int64_t)c0 * 2 * N * N;
+        else if (nv > 0) qbase = qtab[c0 >> 6] + (int64_t)(c0 & 63) * 2 * N * N;
+        else qbase = qtab[c0 >> 6];   // no real column: loads from the block's base, results unused
+    }
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar table loads
     const int l16 = lane & 15, lq = lane >> 4;
@@ -1285,7 +1309,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
         if (ENERGY) {
             const int iq = min(16 * ib + l16, N - 1);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) qv[v] = qhat[(int64_t)min(c0 + lq + 4 * v, ncols - 1) * R + iq + N * j];
+            for (int v = 0; v < 4; ++v) qv[v] = qbase[(int64_t)max(0, min(lq + 4 * v, nv - 1)) * R + iq + N * j];
         }
         dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1295,7 +1319,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int col = c0 + lq + 4 * v;
-            if (col < ncols && i < N) {
+            if (lq + 4 * v < nv && i < N) {
                 const int r = i + N * j;
                 const double c = acc[v];
                 if (OUT == 1) out[(int64_t)col * ldo + r] = c;
@@ -1322,7 +1346,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
             }
         }
         __syncthreads();
-        if (tid < 16 && c0 + tid < ncols) {
+        if (tid < 16 && tid < nv) {
             double a = 0.0, bsum = 0.0;
 #pragma unroll
             for (int w = 0; w < NW; ++w) {

@@ -65,7 +65,7 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                                                 const int *__restrict__ perms, const int *__restrict__ gorder,
                                                 const uint32_t *__restrict__ symtab, const int *__restrict__ symoff,
                                                 int Lw, double *__restrict__ As, int64_t ldA, int cbase, int nprop,
-                                                int p0, uint32_t klen)
+                                                int p0, uint32_t klen, double *__restrict__ nrm2)
 {
     const int ts = wk & 0xffff, h1 = (wk >> 16) & 15, h2 = (wk >> 20) & 15;
     const int4 li = ltinfo[ts];
@@ -109,6 +109,18 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     for (int x = tid; x <= n3; x += NT) Gs[x] = gstart[goff + x] - e0;
     for (int x = tid; x < n3; x += NT) Od[x] = gorder[goff + x];
     __syncthreads();
+    // the work item (l, l, l) with h1 = h2 = 0 holds the l-blocks of every
+    // shell: their squared norms, nrm2[k][l/2] (column scales of the FP16
+    // contraction)
+    if (nrm2 && h1 == 0 && h2 == 0 && L1 == L3 && L2 == L3)
+        for (int k = tid; k < K; k += NT) {
+            double s2 = 0.0;
+            for (int m = 0; m < n3; ++m) {
+                const double2 c = Cb[m * K + k];
+                s2 += c.x * c.x + c.y * c.y;
+            }
+            nrm2[k * kNrmLd + L3 / 2] = s2;
+        }
 
     const int ncb = K > 16 ? 2 : 1;                 // K3 column blocks
     dbl4 acc[NR][2];                                // [row block rb = wave + NW r][K3 block]
@@ -192,8 +204,14 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     }
 
     // ---- V to the As rows of every ordering of the l-triple (tables of the
-    // entries each ordering stores, in k-triple order)
+    // entries each ordering stores, in k-triple order).  With L1 = L2, T and
+    // V are symmetric in (K1, K2) (the 3j symbol is unchanged by exchanging
+    // its first two columns, l1 + l2 + l3 even), so the work lists hold no
+    // item with h1 < h2: item (h1, h2), h1 > h2, also stores the entries of
+    // tile (h2, h1), V(K1, K2, K3) = V(K2, K1, K3) read at the exchanged
+    // local index.
     const int np = perms[ts * 8];
+    const bool mirror = L1 == L2 && h1 > h2;
     for (int cb = 0; cb < ncb; ++cb) {
         // V tile of K3 block cb: Vs[pair][k3 - 16 cb] (pairs 16 rb + lq + 4 v, k3 = l16)
 #pragma unroll
@@ -209,6 +227,15 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
             for (int x = tid; x < n; x += NT) {
                 const uint32_t ent = symtab[o + x], kt = ent >> 12;
                 if (kt < klen) As[row + as_col(kt, cbase, nprop, p0)] = Tf[ent & 4095u];
+            }
+            if (mirror) {
+                const int tm = (((code >> 24) * 2 + h2) * 2 + h1) * 2 + cb;
+                const int om = symoff[tm], nm = symoff[tm + 1] - om;
+                for (int x = tid; x < nm; x += NT) {
+                    const uint32_t ent = symtab[om + x], kt = ent >> 12, e = ent & 4095u;
+                    const uint32_t ex = ((e >> 4) & 15u) * 256u + (e >> 8) * 16u + (e & 15u);
+                    if (kt < klen) As[row + as_col(kt, cbase, nprop, p0)] = Tf[ex];
+                }
             }
         }
         __syncthreads();
@@ -227,11 +254,12 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
                                                            const int *__restrict__ gorder,
                                                            const uint32_t *__restrict__ symtab,
                                                            const int *__restrict__ symoff, int Lw,
-                                                           double *__restrict__ As, int64_t ldA)
+                                                           double *__restrict__ As, int64_t ldA,
+                                                           double *__restrict__ nrm2)
 {
     extern __shared__ double2 sm[];
     triple_sym_body<NW, G3>(sm, pick(cb0, cb1, sel, flip), n_c, K, work[blockIdx.x], ew, eidx, gstart, ltinfo, perms,
-                            gorder, symtab, symoff, Lw, As, ldA, 0, 1, 0, 0xffffffffu);
+                            gorder, symtab, symoff, Lw, As, ldA, 0, 1, 0, 0xffffffffu, nrm2);
 }
 
 // The same on a batch of evaluations (struct EvalDesc): work item i =
@@ -257,7 +285,7 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__re
     const Desc &d = ev[w.x];
     const int K = d.K;
     triple_sym_body<NW, G3>(sm, d.coef, n_c, K, w.y, ew, eidx, gstart, ltinfo, perms, gorder, symtab, symoff, Lw,
-                            As, ldA, d.cbase, d.nprop, d.p, (uint32_t)(K * (K + 1) * (K + 2) / 6));
+                            As, ldA, d.cbase, d.nprop, d.p, (uint32_t)(K * (K + 1) * (K + 2) / 6), nullptr);
 }
 
 // The configuration the library launches.

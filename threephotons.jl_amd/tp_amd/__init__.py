@@ -168,7 +168,8 @@ class BasisTypeHip:
         """'fp64' (default), 'fp16x3' (compensated FP16 MFMA triple products and
         contraction), 'bf16', 'triple16' (compensated FP16 triple products,
         FP64 contraction) or 'fp16x3_t64' (FP64 triple products on the sorted
-        l-triples, the fp16x3 contraction)."""
+        l-triples, G = H.As on compensated FP16 MFMA, the FP64 separated-grid
+        contraction c3 = Bm.G and energy)."""
         m = {"fp64": PREC_FP64, "fp16x3": PREC_FP16X3, "bf16": PREC_BF16,
              "triple16": 3, "fp16x3_t64": 4}[mode] if isinstance(mode, str) else mode
         check(lib().tp_basis_set_precision(self.h, m), "tp_basis_set_precision")
@@ -530,13 +531,22 @@ class AnnealBatch:
         ms = np.zeros(14)
         cfg = np.zeros(4, np.int32)
         check(lib().tp_batch_autotune(self.h, reps, ms.ctypes.data, cfg.ctypes.data), "tp_batch_autotune")
+        if int(cfg[3]) >= 4:
+            # factored contraction (fp16x3_t64): k_hgemm_g candidates (all row
+            # blocks x 64 or 32 columns, 3 row blocks x 64 columns, 4 row blocks
+            # x 16 columns per workgroup), each followed by k_c3sep
+            names = ["allrows64/factored", "allrows32/factored", "rows3x64/factored", "rows4x16/factored"]
+            tile_ms = {n: float(v) for n, v in zip(names, ms[2:6])}
+            return dict(order_ms=ms[:2].tolist(), tile_ms=tile_ms, tile=(int(cfg[0]), int(cfg[1])),
+                        order=int(cfg[2]), schedule="factored", candidate=names[int(cfg[3]) - 4])
         names = [f"{t}/{s}" for s in ("rows", "walk", "xcdrows") for t in ("128x64", "64x64", "128x128", "64x128")]
         return dict(order_ms=ms[:2].tolist(), tile_ms=dict(zip(names, ms[2:].tolist())),
                     tile=(int(cfg[0]), int(cfg[1])), order=int(cfg[2]),
-                    schedule={1: "rows", 2: "walk", 3: "xcdrows"}[int(cfg[3])])
+                    schedule={1: "rows", 2: "walk", 3: "xcdrows"}[int(cfg[3])],
+                    candidate="%dx%d/%s" % (int(cfg[0]), int(cfg[1]), {1: "rows", 2: "walk", 3: "xcdrows"}[int(cfg[3])]))
 
     def set_contraction(self, cand):
-        """Fix the contraction candidate (0..11 in autotune()'s tile_ms order)."""
+        """Fix the contraction candidate (0..11 in autotune()'s tile_ms order; 0..3 for fp16x3_t64)."""
         check(lib().tp_batch_set_contraction(self.h, int(cand)), "tp_batch_set_contraction")
 
     def probe(self, temperature_factor=0.1):
