@@ -8,6 +8,10 @@
 //   l n                   -> worst error (ulp of the exact value, long double
 //                            reference) of tp_fastlog over n arguments
 //   t n                   -> the same for tp_log_series
+//   e n                   -> worst error of the energy epilogues' tp_fastlog_e
+//                            in ulp of max(|log x|, 1) over n arguments in
+//                            1e-30..1e30 and around 1, and its count of
+//                            results that differ from tp_fastlog away from 1
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -73,6 +77,26 @@ int main()
                 worst = std::fmax(worst, ulp_err(x, tab));
             }
             std::printf("%.6g\n", worst);
+        } else if (c == 'e') {
+            int n;
+            if (std::scanf("%d", &n) != 1) return 1;
+            double tab[kLogTabSize];
+            tp_fastlog_table(tab);
+            const LogConsts k = tp_fastlog_consts(tab);
+            std::mt19937_64 rng(777);
+            std::uniform_real_distribution<double> U(0.0, 1.0);
+            double worst = 0.0;
+            long differ = 0;
+            for (int i = 0; i < n; ++i) {
+                const double x = i % 2 ? std::exp(-69.0 + 138.0 * U(rng)) : 1.0 + (U(rng) - 0.5) * 0.03;
+                const double got = tp_fastlog_e(x, tab, k);
+                const long double ref = logl((long double)x);
+                const double big = std::fmax(std::fabs((double)ref), 1.0);
+                const double u = std::nextafter(big, INFINITY) - big;   // ulp of max(|log x|, 1)
+                worst = std::fmax(worst, (double)(fabsl((long double)got - ref) / (long double)u));
+                if (std::fabs(x - 1.0) >= 0.0078125 && got != tp_fastlog(x, tab)) ++differ;
+            }
+            std::printf("%.6g %ld\n", worst, differ);
         } else if (c == 'g') {
             int n;
             if (std::scanf("%d", &n) != 1) return 1;

@@ -69,6 +69,16 @@ def test_fastlog_ulp(driver):
     assert worst < 4.0, worst
 
 
+def test_fastlog_energy_form(driver):
+    """The branch-free log of the FP64 energy epilogues (tp_fastlog_e): within
+    4 ulp of max(|log x|, 1) of the long double log (an absolute bound, which
+    is what the energy sum sees) over 1e-30..1e30 and around 1, and equal to
+    tp_fastlog bit for bit away from 1."""
+    worst, differ = driver(["e 400000"])[0].split()
+    assert float(worst) < 4.0, worst
+    assert int(differ) == 0
+
+
 def test_log_series_ulp(driver):
     """The table-free log of the mixed-precision contraction's energy epilogue
     (tp_log_series) against long double over 1e-30..1e30, around 1 and at

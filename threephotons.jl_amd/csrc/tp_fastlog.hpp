@@ -5,9 +5,10 @@
 // function in IEEE floating-point arithmetic", ACM TOMS 16(4), 1990.
 // For normal positive x = 2^e m, m in [1, 2): j = the top 7 bits of m's
 // fraction, c_j = 1 + (j + 1/2)/128, inv_j = RN(1/c_j), t_j = -log(inv_j),
-// and for c_j >= sqrt2 the reduction is to m/2 in [1/sqrt2, 1): t_j -= ln2,
-// a_j = 1 (so e ln2 and t_j never cancel by more than a factor 2); tables
-// built on the host in long double by tp_fastlog_table, with ln 2 split as
+// and for c_j >= sqrt2 (j >= 53) the reduction is to m/2 in [1/sqrt2, 1):
+// t_j -= ln2, a_j = 1 (so e ln2 and t_j never cancel by more than a factor
+// 2); tables built on the host in long double by tp_fastlog_table, (inv_j,
+// t_j) adjacent (one 16-byte LDS read per logarithm), with ln 2 split as
 // hi + lo, hi having 21 significant bits so that (e + a_j) hi is exact;
 // r = m inv_j - 1 by one FMA (|r| < 2^-8, rounding error < 2^-60), and
 //   log x = (e + a_j) ln2 + t_j + log1p(r),
@@ -29,7 +30,9 @@
 #endif
 
 constexpr int kLogTab = 128;
-constexpr int kLogTabSize = 3 * kLogTab + 2;   // inv_j, t_j, a_j, ln2 hi, ln2 lo
+constexpr int kLogUpJ = 53;                    // c_j >= sqrt2 exactly from j = 53: 128 (sqrt2 - 1) - 1/2 = 52.52
+constexpr int kLogPoly = 7;                    // log1p coefficients -1/2, 1/3, .., -1/8
+constexpr int kLogTabSize = 2 * kLogTab + 2 + kLogPoly;   // (inv_j, t_j) pairs, ln2 hi, ln2 lo, coefficients
 
 inline void tp_fastlog_table(double *tab)
 {
@@ -37,21 +40,22 @@ inline void tp_fastlog_table(double *tab)
     for (int j = 0; j < kLogTab; ++j) {
         const long double c = 1.0L + (j + 0.5L) / kLogTab;
         const double inv = (double)(1.0L / c);
-        const bool up = c >= sqrtl(2.0L);
-        tab[j] = inv;
-        tab[kLogTab + j] = (double)(-logl((long double)inv) - (up ? ln2 : 0.0L));
-        tab[2 * kLogTab + j] = up ? 1.0 : 0.0;
+        const bool up = j >= kLogUpJ;
+        tab[2 * j] = inv;
+        tab[2 * j + 1] = (double)(-logl((long double)inv) - (up ? ln2 : 0.0L));
     }
     double hi = (double)ln2;
     uint64_t b;
     std::memcpy(&b, &hi, 8);
     b &= ~0xFFFFFFFFull;   // keep 21 significant bits
     std::memcpy(&hi, &b, 8);
-    tab[3 * kLogTab] = hi;
-    tab[3 * kLogTab + 1] = (double)(ln2 - (long double)hi);
+    tab[2 * kLogTab] = hi;
+    tab[2 * kLogTab + 1] = (double)(ln2 - (long double)hi);
+    for (int i = 0; i < kLogPoly; ++i) tab[2 * kLogTab + 2 + i] = (i % 2 ? 1.0 : -1.0) / (i + 2);
 }
 
-TP_HD inline double tp_fastlog(double x, const double *tab)
+// ln2hi, ln2lo: tab[2 kLogTab], tab[2 kLogTab + 1], read once by the caller.
+TP_HD inline double tp_fastlog(double x, const double *tab, double ln2hi, double ln2lo)
 {
     uint64_t bits;
     std::memcpy(&bits, &x, 8);
@@ -75,7 +79,8 @@ TP_HD inline double tp_fastlog(double x, const double *tab)
     double m;
     std::memcpy(&m, &mb, 8);
     const int j = (int)((bits >> 45) & 0x7F);
-    const double r = std::fma(m, tab[j], -1.0);
+    const double inv = tab[2 * j], tj = tab[2 * j + 1];
+    const double r = std::fma(m, inv, -1.0);
     double p = -0.125;
     p = std::fma(p, r, 1.0 / 7.0);
     p = std::fma(p, r, -1.0 / 6.0);
@@ -84,8 +89,60 @@ TP_HD inline double tp_fastlog(double x, const double *tab)
     p = std::fma(p, r, 1.0 / 3.0);
     p = std::fma(p, r, -0.5);
     p = std::fma(p * r, r, r);
-    const double ee = (double)e + tab[2 * kLogTab + j];
-    return std::fma(ee, tab[3 * kLogTab], tab[kLogTab + j]) + std::fma(ee, tab[3 * kLogTab + 1], p);
+    const double ee = (double)e + (j >= kLogUpJ ? 1.0 : 0.0);
+    return std::fma(ee, ln2hi, tj) + std::fma(ee, ln2lo, p);
+}
+
+TP_HD inline double tp_fastlog(double x, const double *tab)
+{
+    return tp_fastlog(x, tab, tab[2 * kLogTab], tab[2 * kLogTab + 1]);
+}
+
+// Constants of tp_fastlog_e, read once by a kernel from the table in global
+// memory (wave-uniform loads: they stay in scalar registers instead of being
+// rebuilt in vector registers at every call).
+struct LogConsts {
+    double ln2hi, ln2lo, c[kLogPoly];
+};
+
+TP_HD inline LogConsts tp_fastlog_consts(const double *tab)
+{
+    LogConsts k;
+    k.ln2hi = tab[2 * kLogTab];
+    k.ln2lo = tab[2 * kLogTab + 1];
+    for (int i = 0; i < kLogPoly; ++i) k.c[i] = tab[2 * kLogTab + 2 + i];
+    return k;
+}
+
+// The energy epilogues' logarithm: tp_fastlog's main path without branches,
+// for finite x >= 1e-30 (the energy clamps its argument there; +inf returns
+// +inf).  It equals tp_fastlog bit for bit except for |x - 1| < 2^-7, where it
+// keeps the table reduction instead of summing log1p(x - 1): there the
+// absolute error stays within a few 2^-53 (what the energy sum sees), the
+// relative error is not bounded.
+TP_HD inline double tp_fastlog_e(double x, const double *tab, const LogConsts &k)
+{
+    uint64_t bits;
+    std::memcpy(&bits, &x, 8);
+    const int ex = (int)((bits >> 52) & 0x7FF);
+    const int e = ex - 1023;
+    const uint64_t mb = (bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;
+    double m;
+    std::memcpy(&m, &mb, 8);
+    const int j = (int)((bits >> 45) & 0x7F);
+    const double inv = tab[2 * j], tj = tab[2 * j + 1];
+    const double r = std::fma(m, inv, -1.0);
+    double p = k.c[6];
+    p = std::fma(p, r, k.c[5]);
+    p = std::fma(p, r, k.c[4]);
+    p = std::fma(p, r, k.c[3]);
+    p = std::fma(p, r, k.c[2]);
+    p = std::fma(p, r, k.c[1]);
+    p = std::fma(p, r, k.c[0]);
+    p = std::fma(p * r, r, r);
+    const double ee = (double)e + (j >= kLogUpJ ? 1.0 : 0.0);
+    const double y = std::fma(ee, k.ln2hi, tj) + std::fma(ee, k.ln2lo, p);
+    return ex == 0x7FF ? x : y;
 }
 
 // Table-free FP64 natural logarithm for epilogues where a table in LDS would

@@ -146,7 +146,7 @@ struct tp_basis {
     // sorted l-triples (k_triple_sym; the library's own weights, K <= 32)
     bool use_sym = false;
     int nsym_work = 0;
-    size_t sym_lds = 0;
+    size_t sym_lds = 0, sym_lds_b = 0;   // dynamic LDS of k_triple_sym, k_triple_sym_batch
     int *d_sym_work = nullptr;     // [nsym_work] ts | h1 << 16 | h2 << 20, most expensive first
     int *d_perms = nullptr;        // [n_lt][8] orderings of each sorted l-triple
     int *d_gorder = nullptr;       // [n_groups] run order of the mu3 groups of sorted l-triples
@@ -801,7 +801,7 @@ static void batch_triple(const tp_batch *B, hipStream_t st)
     hipLaunchKernelGGL(k_colscale_batch, dim3((unsigned)B->ne), dim3(256), 0, st, B->d_ev, b->n_c, b->L, B->d_sc,
                        B->d_isc, B->d_shsc, b->K);
     if (B->prec == TP_PREC_FP16X3_T64)
-        hipLaunchKernelGGL(k_triple_sym_batch, dim3((unsigned)B->nswork), dim3(kSymThreads), b->sym_lds, st, B->d_ev,
+        hipLaunchKernelGGL(k_triple_sym_batch, dim3((unsigned)B->nswork), dim3(kSymThreadsBatch), b->sym_lds_b, st, B->d_ev,
                            B->d_swork, b->n_c, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_perms, b->d_gorder,
                            b->d_symtab, b->d_symoff, b->Lw, B->d_As, B->ldA);
     else
@@ -1704,11 +1704,10 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             perms[(size_t)ts * 8 + 1 + cnt] = t | (pidx << 24);
             ++cnt;
         }
-        const size_t lds = triple_sym_lds(K, 2 * L + 1, [&] {
-            int m = 0;
-            for (int t = 0; t < nlt; ++t) m = std::max(m, lt_entries[t]);
-            return m;
-        }());
+        int emax_lt = 0;
+        for (int t = 0; t < nlt; ++t) emax_lt = std::max(emax_lt, lt_entries[t]);
+        const size_t lds = triple_sym_lds(K, 2 * L + 1, emax_lt);
+        const size_t lds_b = triple_sym_lds(K, 2 * L + 1, emax_lt, kSymWavesBatch);
         int maxlds = 0;
         if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
             maxlds = 65536;
@@ -1764,13 +1763,14 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             symoff.push_back((int)symtab.size());
             b->nsym_work = (int)sym_work.size();
             b->sym_lds = lds;
+            b->sym_lds_b = lds_b;
             b->use_sym = true;
-            if (lds > 65536) {
+            if (lds > 65536)
                 (void)hipFuncSetAttribute((const void *)k_triple_sym, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds);
+            if (lds_b > 65536)
                 (void)hipFuncSetAttribute((const void *)k_triple_sym_batch,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            }
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b);
             if (b->Lw == 0) b->Lw = 2 * L + 1;
         }
     }

@@ -1225,13 +1225,14 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
                                                const double *const *__restrict__ qtab = nullptr,
                                                const int *__restrict__ nvalid = nullptr)
 {
-    __shared__ double s_lt[ENERGY ? kLogTabSize : 1];
+    __shared__ __align__(16) double s_lt[ENERGY ? kLogTabSize : 1];
     __shared__ double red[8][2][16];
     extern __shared__ double cs_dyn[];
     // CS[k][j local][col] = cs_dyn[(k * 16 + j) * 17 + col] (padded row)
     auto CS = [&](int k, int r, int c) -> double & { return cs_dyn[(k * 16 + r) * 17 + c]; };
     if (ENERGY)   // ordered before the epilogue by the barrier after stage A
         for (int i = threadIdx.x; i < kLogTabSize; i += 512) s_lt[i] = logtab[i];
+    const LogConsts lk = tp_fastlog_consts(logtab);   // uniform: scalar loads
     // 8 waves.  The histogram values (HBM) are requested after stage A's
     // operands: loads complete in order, so issuing them first would make
     // stage A wait for them.  Three workgroups fit a CU (dynamic LDS sized kb).
@@ -1326,7 +1327,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
                 if (ENERGY) {
                     const double pv = c > 1e-30 ? c : 1e-30;
                     sp[v] += pv;
-                    sq[v] += qv[v] * tp_fastlog(pv, s_lt);
+                    sq[v] += qv[v] * tp_fastlog_e(pv, s_lt, lk);
                 }
             }
         }
