@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <random>
 #include <cmath>
 #include <cstdio>
@@ -98,6 +99,7 @@ struct tp_ctx {
     double *h_pin = nullptr;   // pinned host landing buffer for scalar results
     double *h_res = nullptr;   // pinned, mapped, coherent host buffer the final kernel writes the energy into
     double *d_res = nullptr;   // its device address
+    uint64_t res_seq = 0;      // sequence number of the last synchronous evaluation (h_res[1] as uint64)
     tp_batch *eb = nullptr;    // plan (and graph) of the last mixed-precision tp_energy_batch
     std::vector<const void *> eb_sig;
     std::vector<PhTables *> ph_cache;   // SH / Hankel tables of recent (LMAX, K), newest last
@@ -430,6 +432,25 @@ static McArgs mc_none()
     McArgs m;
     std::memset(&m, 0, sizeof(m));
     return m;
+}
+
+// Wait for the synchronous evaluation numbered seq: k_final stores the energy
+// in mapped host memory and then, with a system-scope release, the sequence
+// number after it, which the host polls (a few microseconds sooner than the
+// stream's completion signal).  Past a time limit, or with timing events
+// recorded, the stream is synchronised instead (which also reports errors).
+static int wait_result(tp_ctx *ctx, uint64_t seq)
+{
+    if (!ctx->timing) {
+        const volatile uint64_t *hp = reinterpret_cast<const volatile uint64_t *>(ctx->h_res + 1);
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint64_t spin = 0;; ++spin) {
+            if (__atomic_load_n(const_cast<const uint64_t *>(hp), __ATOMIC_ACQUIRE) == seq) return 0;
+            if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) break;
+        }
+    }
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
 }
 
 static void record_timing(tp_ctx *ctx)
@@ -2097,8 +2118,11 @@ int tp_energy(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *co
     tp_basis *b = const_cast<tp_basis *>(bc);
     if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
     if (int rc = upload_coeff(ctx, b, coeff, ld, b->K, b->d_coef, b->d_raw)) return rc;
-    if (int rc = enqueue_eval(ctx, b, r, b->d_coef, nullptr, nullptr, 0, nullptr, ctx->d_res, mc_none())) return rc;
-    TP_HIP(hipStreamSynchronize(ctx->stream));
+    McArgs mc = mc_none();
+    mc.seqp = reinterpret_cast<uint64_t *>(ctx->d_res + 1);
+    mc.seq = ++ctx->res_seq;
+    if (int rc = enqueue_eval(ctx, b, r, b->d_coef, nullptr, nullptr, 0, nullptr, ctx->d_res, mc)) return rc;
+    if (int rc = wait_result(ctx, mc.seq)) return rc;
     *E_out = ctx->h_res[0];
     record_timing(ctx);
     return 0;
@@ -3476,8 +3500,11 @@ int tp_state_energy(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, tp_state *s
     if (int rc = check_ctx(ctx)) return rc;
     if (!b || !r || !s || !E_out) return tp_fail(TP_E_ARG, "null argument");
     if (b->ctx != ctx || r->b != b || s->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
-    if (int rc = enqueue_eval(ctx, b, r, s->d_cur, nullptr, nullptr, 0, nullptr, ctx->d_res, mc_none())) return rc;
-    TP_HIP(hipStreamSynchronize(ctx->stream));
+    McArgs mc = mc_none();
+    mc.seqp = reinterpret_cast<uint64_t *>(ctx->d_res + 1);
+    mc.seq = ++ctx->res_seq;
+    if (int rc = enqueue_eval(ctx, b, r, s->d_cur, nullptr, nullptr, 0, nullptr, ctx->d_res, mc)) return rc;
+    if (int rc = wait_result(ctx, mc.seq)) return rc;
     *E_out = ctx->h_res[0];
     record_timing(ctx);
     return 0;

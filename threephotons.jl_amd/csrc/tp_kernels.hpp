@@ -919,6 +919,8 @@ struct McArgs {
     uint32_t tr0;        // step index before the first step of this call
     double minstep;      // stopping rule (<= 0: none)
     int *ctl;            // {stopped, last step, accepted steps} or nullptr
+    uint64_t *seqp;      // mode 0: after E, seq is stored here with system-scope release (host polls it)
+    uint64_t seq;
 };
 
 constexpr int kFinalCols = 32;   // columns per k_final block (8 threads each)
@@ -930,6 +932,7 @@ constexpr int kFinalCols = 32;   // columns per k_final block (8 threads each)
 __device__ __forceinline__ void final_energy(double E, double *__restrict__ Eout, McArgs mc)
 {
     Eout[0] = E;
+    if (mc.seqp) __hip_atomic_store(mc.seqp, mc.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     if (mc.mode == 1) {
         mc.temp[mc.step] = E;
     } else if (mc.mode == 2) {
