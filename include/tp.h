@@ -93,10 +93,13 @@ int tp_basis_info(const tp_basis *b, int64_t *basislen, int64_t *n_lt, int64_t *
  *   TP_PREC_BF16   one BF16 product with FP32 accumulation (comparison mode).
  *   TP_PREC_TRIPLE16 compensated FP16 triple products (as in FP16X3) with the
  *                  FP64 contraction.
- *   TP_PREC_FP16X3_T64 the FP16X3 contraction on FP64 triple products (the
- *                  sorted l-triple form of the FP64 path): needs the
- *                  library's own weights and K <= 32 (TP_E_UNSUPPORTED
- *                  otherwise).
+ *   TP_PREC_FP16X3_T64 the factored mixed form: FP64 triple products (the
+ *                  sorted l-triple form of the FP64 path), G = H.As with H
+ *                  and As split into FP16 hi/lo pairs after power-of-two
+ *                  scaling (hi.hi + (hi.lo + lo.hi), FP32 accumulation),
+ *                  then c3 = Bm.G and the energy in FP64 on the separated
+ *                  grid: needs the library's own weights, K <= 32 and
+ *                  L <= 20 (TP_E_UNSUPPORTED otherwise).
  * FP16X3, BF16 and TRIPLE16 compute the triple products on FP16 MFMA with
  * hi/lo compensation.  A batch (tp_batch_create) keeps the mode its basis
  * had when it was made.  No counterpart in the reference (whose GPU path is FP32
@@ -431,12 +434,18 @@ void tp_batch_destroy(tp_batch *B);
 /* Time the candidate triple-product work orders and contraction kernels on
  * this batch (reps launches each) and keep the fastest.  ms_out[14] (may be
  * NULL): work orders 0 (by cost), 1 (evaluation-major; 1e30 in
- * TP_PREC_FP16X3_T64, whose sorted l-triple form has one order), then tiles
- * 128x64, 64x64, 128x128, 64x128 with parallel row tiles, the same four with
- * row-walking workgroups, and the same four with the row tiles partitioned
- * over the XCDs (1e30 where a shape does not divide by 8); config_out[4] (may
- * be NULL): chosen tile rows, tile columns, work order, schedule (1 parallel,
- * 2 row-walking, 3 XCD-partitioned rows). */
+ * TP_PREC_FP16X3_T64, whose sorted l-triple form has one order), then the
+ * contraction candidates.  Dense form (FP16X3, BF16): tiles 128x64, 64x64,
+ * 128x128, 64x128 with parallel row tiles, the same four with row-walking
+ * workgroups, and the same four with the row tiles partitioned over the XCDs
+ * (1e30 where a shape does not divide by 8); config_out[4] (may be NULL):
+ * chosen tile rows, tile columns, work order, schedule (1 parallel, 2
+ * row-walking, 3 XCD-partitioned rows).  Factored form (FP16X3_T64):
+ * ms_out[2..5] for the G = H.As kernel with all class rows x 64 columns, all
+ * rows x 32 columns, 3 row blocks x 64 columns and 4 row blocks x 16
+ * columns per workgroup (each with the separated-grid stage that follows;
+ * ms_out[6..13] = 1e30); config_out: class rows and columns per workgroup,
+ * work order, 4 + candidate. */
 int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out);
 /* Initial energies and temperatures: slot 0 of every run is the identity, T0 =
  * temperature_factor * std(energies of the other slots) (calculate_temperature,
@@ -447,7 +456,7 @@ int tp_batch_set_family(tp_batch *B, int family);
 /* Angle bound of every run's random proposals, as a fraction of pi/4
  * (0 < amax <= 1; 1 at creation); it then adapts by `factor` per sweep. */
 int tp_batch_set_amax(tp_batch *B, double amax);
-/* Use contraction candidate `cand` (0..11, the order of tp_batch_autotune's
+/* Use contraction candidate `cand` (0..11, 0..3 in TP_PREC_FP16X3_T64; the order of tp_batch_autotune's
  * ms_out[2..13]) instead of the tuned one, re-capturing the sweep graph. */
 int tp_batch_set_contraction(tp_batch *B, int cand);
 /* nsweeps sweeps (graph replays). */
