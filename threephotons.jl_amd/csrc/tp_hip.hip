@@ -309,11 +309,15 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
                            sc.shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
                            b->EmaxLt8, b->Prows, sc.As, b->kl_pad);
-    } else if (b->use_sym) {   // FP64 (also TP_PREC_FP16X3_T64, which needs the tables)
-        hipLaunchKernelGGL(k_triple_sym, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1, sel,
-                           flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
-                           b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad,
-                           prec == TP_PREC_FP16X3_T64 ? sc.nrm2 : nullptr);
+    } else if (b->use_sym) {   // FP64; TP_PREC_FP16X3_T64 with its stage 2 on FP16 MFMA
+        if (prec == TP_PREC_FP16X3_T64)
+            hipLaunchKernelGGL(k_triple_sym16, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1,
+                               sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
+                               b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad, sc.nrm2);
+        else
+            hipLaunchKernelGGL(k_triple_sym, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1,
+                               sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
+                               b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad, nullptr);
     } else if (b->use_mfma) {
         hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(1024), b->mfma_lds, st, b0, b1, sel, flip,
                            b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_work, b->Lw, b->EmaxLt,
@@ -1786,9 +1790,12 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             b->sym_lds = lds;
             b->sym_lds_b = lds_b;
             b->use_sym = true;
-            if (lds > 65536)
+            if (lds > 65536) {
                 (void)hipFuncSetAttribute((const void *)k_triple_sym, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds);
+                (void)hipFuncSetAttribute((const void *)k_triple_sym16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)lds);
+            }
             if (lds_b > 65536)
                 (void)hipFuncSetAttribute((const void *)k_triple_sym_batch,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b);

@@ -592,6 +592,7 @@ typedef _Float16 h16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b16x8 __attribute__((ext_vector_type(8)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f16_bits(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 __device__ __forceinline__ double f16_val(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }
@@ -602,7 +603,11 @@ __device__ __forceinline__ uint16_t bf16_bits(double x) { return __builtin_bit_c
 // xf - hi is exact in FP32.
 __device__ __forceinline__ void f16_split(double x, uint16_t &hb, uint16_t &lb)
 {
-    const float xf = (float)x;
+    float xf = (float)x;
+    // xf is materialised, so that the compiler cannot fold the two roundings
+    // into one FP64 -> FP16 conversion (a long integer sequence on gfx950):
+    // the conversions stay v_cvt_f32_f64 and v_cvt_f16_f32
+    __asm__("" : "+v"(xf));
     const _Float16 h = (_Float16)xf;
     hb = __builtin_bit_cast(uint16_t, h);
     lb = __builtin_bit_cast(uint16_t, (_Float16)(xf - (float)h));

@@ -63,7 +63,23 @@ static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves)
 // The body of one work item wk on coefficients coef of K shells; As entry
 // (lt, kt) of the evaluation at As[lt * ldA + as_col(kt, cbase, nprop, p0)],
 // entries kt >= klen (shells past K) skipped.
-template <int NW, bool G3>
+//
+// S16 (the mixed-precision mode TP_PREC_FP16X3_T64): stage 2 on FP16 MFMA.
+// T (FP64, from stage 1) and c_K3 are scaled by powers of two from the
+// l-block norms of the work item's own shells, r_a(K1) = 2^-(1 + ilogb |c_K1,L1|)
+// and so on, so that |T| r_a r_b < 1 (the Clebsch-Gordan map is unitary:
+// |T(K1,K2,mu3)| <= |c_K1,L1| |c_K2,L2|) and |c_K3| r_c < 1; both are taken
+// times 2^13, split into FP16 hi + lo (22 significant bits), and V
+// accumulates hi.hi, hi.lo and lo.hi on v_mfma_f32_16x16x32_f16 in FP32, one
+// K step per chunk of 16 mu3 groups (32 slots: Re T and Im T).  V is
+// unscaled exactly (powers of two) on its way to As.  LDS: c_K3 split once per
+// item, in place of Cb ({hi Re | hi -Im << 16, lo Re | lo -Im << 16} per
+// entry); T of a chunk as 8-byte {hi Re | hi Im << 16, lo Re | lo Im << 16}
+// per (pair, group), 128 bytes per pair, the 16-byte pairs of groups (2j,
+// 2j + 1) of pair p at position j ^ ((p >> 1) & 7), so that the stage-1
+// writes (one group, 64 pairs) and the stage-2 reads (16 pairs, 4 groups per
+// lane) both spread over the banks.
+template <int NW, bool G3, bool S16 = false>
 __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K, int wk,
                                                 const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
                                                 const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
@@ -86,8 +102,11 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     uint32_t *Is = reinterpret_cast<uint32_t *>(Ws + ne);           // [E] local (m1 index | m2 index << 16)
     int *Gs = reinterpret_cast<int *>(Is + ne);                     // [n3 + 1] group starts
     int *Od = Gs + (n3 + 1);                                        // [n3] run order of the groups
-    double *Tf = reinterpret_cast<double *>(
-        (reinterpret_cast<uintptr_t>(Od + n3) + 15) & ~(uintptr_t)15);   // [2 NW][256]; V staging later
+    // [2 NW][256]; V staging later.  Its 16-byte aligned offset is formed on
+    // sm itself (no integer round trip), so its accesses stay LDS accesses
+    // instead of flat ones.
+    const size_t tf_off = (reinterpret_cast<const char *>(Od + n3) - reinterpret_cast<const char *>(sm) + 15) & ~(size_t)15;
+    double *Tf = reinterpret_cast<double *>(reinterpret_cast<char *>(sm) + tf_off);
     constexpr int NT = 64 * NW, NR = 16 / NW;   // threads; stage-2 row blocks per wave
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -128,22 +147,73 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
         }
 
     const int ncb = K > 16 ? 2 : 1;                 // K3 column blocks
+    // S16: the work item's scales r_a (K1 = K1b + i), r_b (K2 = K2b + j), r_c (K3), and their inverses
+    __shared__ double s_rs[S16 ? 64 : 1], s_ri[S16 ? 64 : 1];
+    uint2 *Cu = reinterpret_cast<uint2 *>(Cb);   // S16: c_K3 split, [n3][K]
+    uint2 *T2 = reinterpret_cast<uint2 *>(Tf);   // S16: T of a chunk, [256 pairs][16 groups]
+    if constexpr (S16) {
+        // squared norms of the 32 + K shell blocks (K1 block, K2 block, K3),
+        // TPC consecutive lanes per block, each summing every TPC-th m
+        constexpr int TPC = NT / 64;
+        const int col = tid / TPC, sub = tid % TPC;
+        const double2 *base = col < 16 ? Ab + col : col < 32 ? Bb + (col - 16) : Cb + (col - 32);
+        const int n = col < 16 ? n1 : col < 32 ? n2 : n3, ld = col < 32 ? 16 : K;
+        double s2 = 0.0;
+        if (col < 32 + K)
+            for (int m = sub; m < n; m += TPC) {
+                const double2 c = base[m * ld];
+                s2 += c.x * c.x + c.y * c.y;
+            }
+#pragma unroll
+        for (int o = 1; o < TPC; o <<= 1) s2 += __shfl_xor(s2, o);
+        if (sub == 0 && col < 32 + K) {
+            const int ex = s2 > 0.0 ? 1 + ilogb(sqrt(s2)) : 0;
+            s_rs[col] = ldexp(1.0, -ex);
+            s_ri[col] = ldexp(1.0, ex);
+        }
+        __syncthreads();
+        // c_K3 (r_c 2^13) split in place: entry x's 8 bytes overwrite half of
+        // double2 entry x / 2, read in this round or an earlier one
+        const int nck = n3 * K;
+        for (int x0 = 0; x0 < nck; x0 += NT) {
+            const int x = x0 + tid;
+            uint2 u = make_uint2(0u, 0u);
+            if (x < nck) {
+                const double2 c = Cb[x];
+                const double sc = s_rs[32 + x % K] * 8192.0;
+                uint16_t hr, lr, hi, li;
+                f16_split(c.x * sc, hr, lr);
+                f16_split(-c.y * sc, hi, li);
+                u = make_uint2(hr | (uint32_t)hi << 16, lr | (uint32_t)li << 16);
+            }
+            __syncthreads();
+            if (x < nck) Cu[x] = u;
+        }
+        __syncthreads();
+    }
     dbl4 acc[NR][2];                                // [row block rb = wave + NW r][K3 block]
+    f32x4 a16[NR][2];                               // the same in FP32 (S16)
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int c = 0; c < 2; ++c) acc[r][c] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < 2; ++c) {
+            acc[r][c] = dbl4{0.0, 0.0, 0.0, 0.0};
+            a16[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
 
-    for (int c0 = 0; c0 < n3; c0 += NW) {
-        const int nu = min(NW, n3 - c0);
+    constexpr int CH = S16 ? 16 : NW;   // mu3 groups per chunk
+    for (int c0 = 0; c0 < n3; c0 += CH) {
+        const int nu = min(CH, n3 - c0);
+        for (int r1 = 0; r1 < CH / NW; ++r1) {
+        const int gi = wave + NW * r1;   // this wave's group within the chunk
         // ---- stage 1: this wave's group, k-step = 4 entries; with G3 the
         // complex product as P1 = (w Re a) Re b, P2 = (w Im a) Im b,
         // P3 = (w (Re a + Im a)) (Re b + Im b): Tr = P1 - P2, Ti = P3 - P1 - P2;
         // otherwise Tr += (w Re a) Re b - (w Im a) Im b, Ti += (w Re a) Im b + (w Im a) Re b
         {
             dbl4 p1 = dbl4{0.0, 0.0, 0.0, 0.0}, p2 = dbl4{0.0, 0.0, 0.0, 0.0}, p3 = dbl4{0.0, 0.0, 0.0, 0.0};
-            if (wave < nu) {
-                const int u = Od[c0 + wave];
+            if (gi < nu) {
+                const int u = Od[c0 + gi];
                 const int s = Gs[u], E = Gs[u + 1] - s;
                 const int nks = (E + 3) >> 2;
 #pragma unroll 2
@@ -173,14 +243,62 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 }
             }
             // rows K1 = lq + 4 v, columns K2 = l16: pair p = 16 row + column
+            if constexpr (S16) {
+                const double sbv = s_rs[16 + l16] * 8192.0;
+                const int gpos = 2 * ((gi >> 1) ^ ((l16 >> 1) & 7)) + (gi & 1);   // (p >> 1) & 7 = (l16 >> 1) & 7
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int p = 16 * (lq + 4 * v) + l16;
-                Tf[(2 * wave) * 256 + p] = G3 ? p1[v] - p2[v] : p1[v];
-                Tf[(2 * wave + 1) * 256 + p] = G3 ? p3[v] - p1[v] - p2[v] : p2[v];
+                for (int v = 0; v < 4; ++v) {
+                    const int p = 16 * (lq + 4 * v) + l16;
+                    const double sc = s_rs[lq + 4 * v] * sbv;
+                    uint16_t hr, lr, hi, li;
+                    f16_split((G3 ? p1[v] - p2[v] : p1[v]) * sc, hr, lr);
+                    f16_split((G3 ? p3[v] - p1[v] - p2[v] : p2[v]) * sc, hi, li);
+                    T2[p * 16 + gpos] = make_uint2(hr | (uint32_t)hi << 16, lr | (uint32_t)li << 16);
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int p = 16 * (lq + 4 * v) + l16;
+                    Tf[(2 * gi) * 256 + p] = G3 ? p1[v] - p2[v] : p1[v];
+                    Tf[(2 * gi + 1) * 256 + p] = G3 ? p3[v] - p1[v] - p2[v] : p2[v];
+                }
             }
         }
+        }
         __syncthreads();
+        if constexpr (S16) {
+            // ---- stage 2 on FP16 MFMA: the chunk's 32 slots are one K step,
+            // lane (l16, lq) holding slots 8 lq .. 8 lq + 7 = (Re, Im) of groups
+            // 4 lq .. 4 lq + 3: A = T of pair 16 rb + l16, B = [Re c_K3 ; -Im c_K3]
+            const int sw = (l16 >> 1) & 7;
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                if (cb >= ncb) break;
+                const int k3 = 16 * cb + l16;
+                u32x4 bh, bl;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int g = 4 * lq + q;
+                    const uint2 u = g < nu && k3 < K ? Cu[Od[c0 + g] * K + k3] : make_uint2(0u, 0u);
+                    bh[q] = u.x;
+                    bl[q] = u.y;
+                }
+                const h16x8 hbh = __builtin_bit_cast(h16x8, bh), hbl = __builtin_bit_cast(h16x8, bl);
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const uint2 *trow = T2 + (16 * (wave + NW * r) + l16) * 16;
+                    const uint4 q1 = *reinterpret_cast<const uint4 *>(trow + 2 * ((2 * lq) ^ sw));
+                    const uint4 q2 = *reinterpret_cast<const uint4 *>(trow + 2 * ((2 * lq + 1) ^ sw));
+                    const h16x8 ah = __builtin_bit_cast(h16x8, u32x4{q1.x, q1.z, q2.x, q2.z});
+                    const h16x8 al = __builtin_bit_cast(h16x8, u32x4{q1.y, q1.w, q2.y, q2.w});
+                    a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, hbh, a16[r][cb], 0, 0, 0);
+                    a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, hbl, a16[r][cb], 0, 0, 0);
+                    a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, hbh, a16[r][cb], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         // ---- stage 2: slots 2u' (Re T) and 2u'+1 (Im T) of this chunk
         const int nsl = 2 * nu, nks2 = (nsl + 3) >> 2;
         for (int ks = 0; ks < nks2; ++ks) {
@@ -218,11 +336,21 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     const int np = perms[ts * 8];
     const bool mirror = L1 == L2 && h1 > h2;
     for (int cb = 0; cb < ncb; ++cb) {
-        // V tile of K3 block cb: Vs[pair][k3 - 16 cb] (pairs 16 rb + lq + 4 v, k3 = l16)
+        // V tile of K3 block cb: Vs[pair][k3 - 16 cb] (pairs 16 rb + lq + 4 v, k3 = l16;
+        // S16: pairs 16 rb + 4 lq + v, unscaled by 2^-26 / (r_a r_b r_c))
 #pragma unroll
         for (int r = 0; r < NR; ++r)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Tf[(16 * (wave + NW * r) + lq + 4 * v) * 16 + l16] = acc[r][cb][v];
+            for (int v = 0; v < 4; ++v) {
+                if constexpr (S16) {
+                    const int p = 16 * (wave + NW * r) + 4 * lq + v, k3 = 16 * cb + l16;
+                    const double un =
+                        k3 < K ? s_ri[p >> 4] * s_ri[16 + (p & 15)] * s_ri[32 + k3] * 0x1p-26 : 0.0;
+                    Tf[p * 16 + l16] = (double)a16[r][cb][v] * un;
+                } else {
+                    Tf[(16 * (wave + NW * r) + lq + 4 * v) * 16 + l16] = acc[r][cb][v];
+                }
+            }
         __syncthreads();
         for (int q = 0; q < np; ++q) {
             const int code = perms[ts * 8 + 1 + q];
@@ -247,7 +375,7 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     }
 }
 
-template <int NW, bool G3>
+template <int NW, bool G3, bool S16 = false>
 __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restrict__ cb0,
                                                            const double2 *__restrict__ cb1,
                                                            const int *__restrict__ sel, int flip, int n_c, int K,
@@ -263,8 +391,8 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
                                                            double *__restrict__ nrm2)
 {
     extern __shared__ double2 sm[];
-    triple_sym_body<NW, G3>(sm, pick(cb0, cb1, sel, flip), n_c, K, work[blockIdx.x], ew, eidx, gstart, ltinfo, perms,
-                            gorder, symtab, symoff, Lw, As, ldA, 0, 1, 0, 0xffffffffu, nrm2);
+    triple_sym_body<NW, G3, S16>(sm, pick(cb0, cb1, sel, flip), n_c, K, work[blockIdx.x], ew, eidx, gstart, ltinfo,
+                                 perms, gorder, symtab, symoff, Lw, As, ldA, 0, 1, 0, 0xffffffffu, nrm2);
 }
 
 // The same on a batch of evaluations (struct EvalDesc): work item i =
@@ -272,7 +400,7 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
 // batch's basis (K_b shells): the entries of an evaluation of K <= K_b
 // shells are the ones with kt < K (K + 1) (K + 2) / 6, since a sorted
 // k-triple's index does not depend on the shell count.
-template <int NW, bool G3, typename Desc>
+template <int NW, bool G3, bool S16, typename Desc>
 __global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__restrict__ ev,
                                                                  const int2 *__restrict__ work, int n_c,
                                                                  const double *__restrict__ ew,
@@ -289,10 +417,14 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__re
     const int2 w = work[blockIdx.x];
     const Desc &d = ev[w.x];
     const int K = d.K;
-    triple_sym_body<NW, G3>(sm, d.coef, n_c, K, w.y, ew, eidx, gstart, ltinfo, perms, gorder, symtab, symoff, Lw,
+    triple_sym_body<NW, G3, S16>(sm, d.coef, n_c, K, w.y, ew, eidx, gstart, ltinfo, perms, gorder, symtab, symoff, Lw,
                             As, ldA, d.cbase, d.nprop, d.p, (uint32_t)(K * (K + 1) * (K + 2) / 6), nullptr);
 }
 
 // The configuration the library launches.
 #define k_triple_sym (k_triple_sym_t<kSymWaves, kSymGauss>)
-#define k_triple_sym_batch (k_triple_sym_batch_t<kSymWavesBatch, kSymGauss, EvalDesc>)
+// TP_PREC_FP16X3_T64 takes stage 2 on FP16 MFMA, in batches (14.9 ms against
+// 15.6 ms per 1,280-evaluation sweep with the FP64 stage 2) and, so that a
+// batch gives what single evaluations give, in single evaluations.
+#define k_triple_sym16 (k_triple_sym_t<kSymWaves, kSymGauss, true>)
+#define k_triple_sym_batch (k_triple_sym_batch_t<kSymWavesBatch, kSymGauss, true, EvalDesc>)
