@@ -39,7 +39,7 @@ __global__ __launch_bounds__(512) void k_c3sep_bwd(const double *__restrict__ G,
                                                    const double *__restrict__ TB, int kb, int N, int ncols,
                                                    const double *__restrict__ qhat, const double *__restrict__ qsum,
                                                    const double *__restrict__ tf, const double *__restrict__ part,
-                                                   int64_t ldp, int njc, int n_m, double *__restrict__ gGp,
+                                                   int64_t ldp, int nprt, int n_m, double *__restrict__ gGp,
                                                    int64_t ldgg, int nm_pad)
 {
     extern __shared__ double cs_dyn[];
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(512) void k_c3sep_bwd(const double *__restrict__ G,
     const bool cok = col < ncols;
     const int colc = min(col, ncols - 1);
     double S = 0.0;
-    for (int q = 0; q < njc; ++q) S += part[(int64_t)(2 * q) * ldp + colc];
+    for (int q = 0; q < nprt; ++q) S += part[(int64_t)(2 * q) * ldp + colc];   // the forward's nprt partial rows
     const double tfc = tf[colc], qsc = qsum[colc];
     __syncthreads();
     // ---- forward stage B in the orientation (i, col) and D; this wave's

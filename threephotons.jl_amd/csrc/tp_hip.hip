@@ -109,8 +109,8 @@ static void ph_cache_free(tp_ctx *ctx);
 // contraction tiles
 constexpr int G1_BM = 64, G1_BN = 32, G1_WGM = 2;   // tile of the gradient's gAs = H^T . gG
 constexpr int kNumTiles = 12;                       // contraction candidates of the batch autotuner
-constexpr int GF_BM = 32, GF_BN = 32, GF_WGM = 2;   // G = H . As: rows = classes (136 at L = 18), small tiles
-                                                    // so that the launch has several waves of workgroups
+constexpr int kDgRB = 3, kDgPF = 4;                 // G = H . As (k_dgemm_g): row blocks per workgroup, k-steps
+                                                    // of operand prefetch
 constexpr int G2_BM = 64, G2_BN = 64, G2_WGM = 4;   // c3 = Bm . G (+ energy)
 constexpr int KL_ALIGN = 128;                       // column padding (multiple of G1_BN, G2_BN and H_BN)
 constexpr int kMfmaMaxK = 27;                       // k_triple_mfma: at most 16 row blocks of pairs per half
@@ -132,7 +132,7 @@ struct tp_basis {
     LamEnt *d_lament = nullptr;    // entries of every class, class-major
     int *d_lamcs = nullptr;        // [n_m + 1] first entry of each class
     int64_t R_pad = 0, nlt_pad = 0, kl_pad = 0, nm_pad64 = 0, nm_pad16 = 0;
-    int n_rt = 0, n_ct = 0, g1_rt = 0, g1_ct = 0, n_fb = 0;
+    int n_rt = 0, n_ct = 0;
     int n_groups = 0, nchunks = 0, Gmax = 0, Emax = 0, npairs = 0, PB = 0;
     size_t triple_lds = 0;
     // MFMA triple products (K <= kMfmaMaxK): work list of (lt, k1 half)
@@ -178,6 +178,10 @@ struct tp_basis {
     // separated grid (k_c3sep): classes of one m2 contiguous
     int n_m2 = 0, jpad = 0, sep_kb = 0, njc = 0;
     bool use_sep = false;          // N <= 32, 2(L/2+1) <= 24 and <= 24 classes per m2
+    // energies on the folded grid (k_c3sep FOLD): lambda dq = 0 (even m2, mu3)
+    // and even N; njf j blocks of 16 cover its N values of j
+    bool fold = false;
+    int njf = 0;
     int *d_m2start = nullptr;      // [n_m2 + 1]
     double *d_TA = nullptr;        // [2][n_m][jpad]  cos / sin(mu3 b_j)
     double *d_TB = nullptr;        // [round_up(N,16)][sep_kb]  cos(m2 a_i), -sin(m2 a_i)
@@ -194,8 +198,6 @@ struct tp_basis {
     double *d_As = nullptr;        // [nlt_pad][kl_pad]
     double *d_G = nullptr;         // [nm_pad64][kl_pad]
     double *d_part = nullptr;      // [n_rt][2][kl_pad]
-    double *d_blk = nullptr;       // [n_fb] block totals of k_final
-    unsigned *d_counter = nullptr; // [1] k_final arrival counter (zero between launches)
     double *d_out = nullptr;       // [8] scalar results
     double2 *d_raw = nullptr;      // [K][LMAX^2] staging of host coefficients
     double2 *d_coef = nullptr;     // [K][n_c] compact coefficients of host-input calls
@@ -225,6 +227,7 @@ struct tp_ref {
     int klo = 1, khi = 1;      // K3_range of the walk
     int64_t ncols = 0;
     double *d_qhat = nullptr;  // [ncols][R]
+    double *d_qfold = nullptr; // [ncols][N^2/2] the histogram on the folded grid (bases with fold)
     double *d_qsum = nullptr;  // [ncols]
     double *d_tf = nullptr;    // [ncols]
 };
@@ -275,16 +278,14 @@ static int upload_coeff(tp_ctx *ctx, tp_basis *b, const double *coeff, int64_t l
 // Per-evaluation device scratch of enqueue_eval (by default the basis's own
 // buffers).
 struct EvalScratch {
-    double *As, *G, *part, *blk, *Ainv, *shsc;
-    unsigned *counter;
+    double *As, *G, *part, *Ainv, *shsc;
     uint16_t *Ahi, *Alo;
     double *nrm2;   // [K][kNrmLd] l-block norms (k_triple_sym -> k_hgemm_g)
 };
 
 static EvalScratch basis_scratch(const tp_basis *b)
 {
-    return EvalScratch{b->d_As,      b->d_G,   b->d_part, b->d_blk, b->d_Ainv,
-                       b->d_shsc,    b->d_counter, b->d_Ahi, b->d_Alo, b->d_nrm2};
+    return EvalScratch{b->d_As, b->d_G, b->d_part, b->d_Ainv, b->d_shsc, b->d_Ahi, b->d_Alo, b->d_nrm2};
 }
 
 // Enqueue one evaluation on coefficients (b0, b1, sel, flip): As, G, c3 (+energy).
@@ -377,12 +378,13 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
                        dim3(256), 0, st, b->d_Hhi, b->d_Hlo, b->ldh16, b->H_inv, sc.As, b->kl_pad, (int)b->n_lt,
                        b->ldh16, nullptr, sc.nrm2, b->L, b->K, (int)b->nm_pad16, sc.G, b->kl_pad);
     } else {
-    // G[class][kt] = sum_lt H[lt][class] As[lt][kt]
-    // XCD-grouped column tiles: each As column block is fetched from HBM once
-    hipLaunchKernelGGL((k_gemm<GF_BM, GF_BN, GF_WGM, 2, false, true>),
-                       dim3((unsigned)(8 * ((b->g1_ct + 7) / 8) * b->g1_rt)), dim3(256), 0,
-                       st, b->d_H, b->nm_pad64, sc.As, b->kl_pad, (int)b->nlt_pad, (int)(b->g1_rt * GF_BM),
-                       (int)b->kl_pad, b->g1_rt, b->g1_ct, sc.G, b->kl_pad, nullptr, nullptr, (int64_t)0, nullptr);
+    // G[class][kt] = sum_lt H[lt][class] As[lt][kt]: 16 columns x 48 classes per
+    // workgroup, K split over its waves (rows up to nm_pad16 written; those
+    // past the classes are zero)
+    const int ncb = (int)(b->kl_pad / 16), nrt = (int)((b->nm_pad16 + 16 * kDgRB - 1) / (16 * kDgRB));
+    hipLaunchKernelGGL((k_dgemm_g<kDgRB, kDgPF>), dim3((unsigned)(8 * ((ncb + 7) / 8) * nrt)), dim3(256), 0, st,
+                       b->d_H, b->nm_pad64, (int)b->nm_pad64, sc.As, b->kl_pad, (int)b->nlt_pad, ncb, nrt, sc.G,
+                       b->kl_pad, (int)b->nm_pad64);
     }
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
@@ -412,19 +414,27 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
                        tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, sc.part, b->kl_pad, OUTP, b->R,       \
                        b->d_logtab)
-    if (r && c3out) TP_SEP(1, true, c3out, r->d_qhat);
-    else if (r) TP_SEP(0, true, nullptr, r->d_qhat);
-    else TP_SEP(1, false, c3out, nullptr);
+    if (r && !c3out && b->fold && r->d_qfold) {
+        // the energy on the folded grid: njf j blocks
+        hipLaunchKernelGGL((k_c3sep<0, true, false, true>), dim3(grid.x, (unsigned)b->njf), dim3(512),
+                           c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, tap,
+                           b->jpad, b->d_TB, b->sep_kb, b->N, ncols, r->d_qfold, sc.part, b->kl_pad, nullptr, b->R,
+                           b->d_logtab);
+        n_rt_fin = b->njf;
+    } else {
+        if (r && c3out) TP_SEP(1, true, c3out, r->d_qhat);
+        else if (r) TP_SEP(0, true, nullptr, r->d_qhat);
+        else TP_SEP(1, false, c3out, nullptr);
+        n_rt_fin = b->njc;
+    }
 #undef TP_SEP
     TP_HIP(hipGetLastError());
-    n_rt_fin = b->njc;
     }
     }
     if (timing) TP_HIP(hipEventRecord(ctx->ev[3], st));
     if (r) {
-        const unsigned nfb = (unsigned)((ncols + kFinalCols - 1) / kFinalCols);
-        hipLaunchKernelGGL(k_final, dim3(nfb), dim3(256), 0, st, sc.part, n_rt_fin, b->kl_pad, ncols, r->d_qsum,
-                           r->d_tf, sc.blk, sc.counter, Eout, mc);
+        hipLaunchKernelGGL(k_final, dim3(1), dim3(kFinalThreads), 0, st, sc.part, n_rt_fin, b->kl_pad, ncols,
+                           r->d_qsum, r->d_tf, Eout, mc);
         TP_HIP(hipGetLastError());
     }
     if (timing) TP_HIP(hipEventRecord(ctx->ev[4], st));
@@ -482,6 +492,7 @@ struct tp_batch {
     int nwork = 0;
     int prec = TP_PREC_FP16X3;             // the basis's precision mode when the batch was made
     int gcand = 0, order = 0;              // contraction candidate (kTiles), triple-product work order (autotuned)
+    bool fold = false;                     // TP_PREC_FP16X3_T64 energies on the folded grid (k_c3sep FOLD)
     int family = TP_MOVE_PLANE;            // move family of the random slots
     double tune_ms[2 + kNumTiles] = {};
     EvalDesc *d_ev = nullptr;
@@ -638,6 +649,8 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     B->nl = b->L / 2;
     B->sweep = sweep;
     B->hist_cap = hist_cap;
+    B->fold = fac && b->fold;
+    for (int g = 0; g < ngroups; ++g) B->fold = B->fold && refs[g]->d_qfold;
     B->cstride = (int64_t)b->K * b->n_c;
     B->refs.assign(refs, refs + ngroups);
     // column blocks
@@ -664,7 +677,10 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         for (int64_t j = 0; j < nt; ++j)
             for (int p = 0; p < nprop; ++p) {
                 const int64_t nv = std::max<int64_t>(0, std::min<int64_t>(64, r->ncols - j * 64));
-                qtab.push_back(r->d_qhat + (nv > 0 ? j * 64 * b->R : 0));
+                if (B->fold)   // folded columns of N^2/2 values
+                    qtab.push_back(r->d_qfold + (nv > 0 ? j * 64 * (b->R / 4) : 0));
+                else
+                    qtab.push_back(r->d_qhat + (nv > 0 ? j * 64 * b->R : 0));
                 nvalid.push_back((int)nv);
             }
         GroupState s;
@@ -677,7 +693,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         cb += nt * nprop;
     }
     while (cb % 2) {   // even block count: 128-column tiles
-        qtab.push_back(refs[0]->d_qhat);
+        qtab.push_back(B->fold ? refs[0]->d_qfold : refs[0]->d_qhat);
         nvalid.push_back(0);
         ++cb;
     }
@@ -806,16 +822,22 @@ static void batch_contract_fac(const tp_batch *B, hipStream_t st, int cand)
     default: batch_hgemm_launch<16, 4>(B, st); break;
     }
     const int64_t tap = (int64_t)b->n_m * b->jpad;
-    hipLaunchKernelGGL((k_c3sep<0, true, true>), dim3((unsigned)(B->ldA / 16), (unsigned)b->njc), dim3(512),
-                       c3sep_lds_bytes(b->sep_kb), st, B->d_G, B->ldA, b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad,
-                       b->d_TB, b->sep_kb, b->N, 0, nullptr, B->d_part, B->ldA, nullptr, (int64_t)0, b->d_logtab,
-                       B->d_qtab, B->d_nvalid);
+    if (B->fold)
+        hipLaunchKernelGGL((k_c3sep<0, true, true, true>), dim3((unsigned)(B->ldA / 16), (unsigned)b->njf), dim3(512),
+                           c3sep_lds_bytes(b->sep_kb), st, B->d_G, B->ldA, b->d_m2start, b->n_m2, b->d_TA, tap,
+                           b->jpad, b->d_TB, b->sep_kb, b->N, 0, nullptr, B->d_part, B->ldA, nullptr, (int64_t)0,
+                           b->d_logtab, B->d_qtab, B->d_nvalid);
+    else
+        hipLaunchKernelGGL((k_c3sep<0, true, true>), dim3((unsigned)(B->ldA / 16), (unsigned)b->njc), dim3(512),
+                           c3sep_lds_bytes(b->sep_kb), st, B->d_G, B->ldA, b->d_m2start, b->n_m2, b->d_TA, tap,
+                           b->jpad, b->d_TB, b->sep_kb, b->N, 0, nullptr, B->d_part, B->ldA, nullptr, (int64_t)0,
+                           b->d_logtab, B->d_qtab, B->d_nvalid);
 }
 
 // energy-sum rows the chosen contraction leaves in part
 static int batch_part_rows(const tp_batch *B)
 {
-    if (B->prec == TP_PREC_FP16X3_T64) return B->b->njc;
+    if (B->prec == TP_PREC_FP16X3_T64) return B->fold ? B->b->njf : B->b->njc;
     return kTiles[B->gcand][2] == 2 ? 1 : (int)(B->b->R_pad16 / kTiles[B->gcand][0]);
 }
 
@@ -1357,8 +1379,7 @@ void tp_basis_destroy(tp_basis *b)
     if (!b) return;
     if (b->ctx) (void)hipSetDevice(b->ctx->device);
     void *ptrs[] = {b->d_H,   b->d_Bm,  b->d_ew,   b->d_eidx,    b->d_gstart, b->d_ltinfo, b->d_chunks,
-                    b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_blk,
-                    b->d_counter, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
+                    b->d_pairs, b->d_sean0, b->d_coff, b->d_As, b->d_G, b->d_part, b->d_out, b->d_raw, b->d_coef, b->d_c3, b->d_work, b->d_Fhi, b->d_Flo,
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
                     b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2,
@@ -1811,9 +1832,6 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     b->nm_pad16 = round_up(b->n_m, 16);
     b->n_rt = (int)(b->R_pad / G2_BM);
     b->n_ct = (int)(b->kl_pad / G2_BN);
-    b->g1_rt = (int)((b->n_m + GF_BM - 1) / GF_BM);   // rows of G past them stay unwritten (never read)
-    b->g1_ct = (int)(b->kl_pad / GF_BN);
-    b->n_fb = (int)((b->kl_pad + kFinalCols - 1) / kFinalCols);
     b->ldk = (int)round_up(nlt, 64);
     b->R_pad16 = round_up(b->R, H_BM);
     b->n_rt16 = (int)(b->R_pad16 / H_BM);
@@ -1842,6 +1860,8 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     int max_cls = 0;
     for (int i = 0; i < b->n_m2; ++i) max_cls = std::max(max_cls, m2start[i + 1] - m2start[i]);
     b->use_sep = N <= 32 && b->sep_kb <= kSepMaxKB && max_cls <= 24;
+    b->fold = b->use_sep && !b->lam && N % 2 == 0;
+    b->njf = (int)round_up(N, 16) / 16;
     std::vector<double> TA((size_t)2 * b->n_m * b->jpad, 0.0);
     for (int c = 0; c < b->n_m; ++c)
         for (int j = 0; j < 2 * N; ++j) {
@@ -1887,8 +1907,7 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     if ((rc = dev_alloc(&b->d_As, (size_t)b->nlt_pad * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_G, (size_t)b->nm_pad64 * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_part, (size_t)std::max(b->n_rt, b->njc) * 2 * b->kl_pad)) ||
-        (rc = dev_alloc(&b->d_blk, b->n_fb)) ||
-        (rc = dev_alloc(&b->d_counter, 1)) || (rc = dev_alloc(&b->d_out, 8)) ||
+        (rc = dev_alloc(&b->d_out, 8)) ||
         (rc = dev_alloc(&b->d_raw, (size_t)K * LMAX * LMAX)) || (rc = dev_alloc(&b->d_coef, (size_t)K * b->n_c)) ||
         (rc = dev_alloc(&b->d_shsc, (size_t)2 * K)) || (rc = dev_alloc(&b->d_nrm2, (size_t)K * kNrmLd))) {
         tp_basis_destroy(b);
@@ -1903,7 +1922,6 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     // zero; so must the rows of G past the classes, which k_glam leaves alone
     hipError_t e = hipMemset(b->d_As, 0, sizeof(double) * (size_t)b->nlt_pad * b->kl_pad);
     if (e == hipSuccess) e = hipMemset(b->d_G, 0, sizeof(double) * (size_t)b->nm_pad64 * b->kl_pad);
-    if (e == hipSuccess) e = hipMemset(b->d_counter, 0, sizeof(unsigned));
     if (e != hipSuccess) {
         tp_basis_destroy(b);
         return tp_fail(TP_E_HIP, "hipMemset failed");
@@ -2051,6 +2069,7 @@ void tp_ref_destroy(tp_ref *r)
     if (!r) return;
     if (r->b && r->b->ctx) (void)hipSetDevice(r->b->ctx->device);
     if (r->d_qhat) (void)hipFree(r->d_qhat);
+    if (r->d_qfold) (void)hipFree(r->d_qfold);
     if (r->d_qsum) (void)hipFree(r->d_qsum);
     if (r->d_tf) (void)hipFree(r->d_tf);
     delete r;
@@ -2109,6 +2128,29 @@ int tp_ref_create(tp_ctx *ctx, const tp_basis *b, const double *c3ref, int Kr, i
         tp_ref_destroy(r);
         return tp_fail(TP_E_HIP, "histogram upload failed");
     }
+    if (b->fold) {
+        // each orbit {(i, j), (i, j + N), (N-1-i, 2N-1-j), (N-1-i, N-1-j)} of
+        // i < N/2, j < N summed into bin i + (N/2) j of the folded column
+        const int N = b->N, Nh = N / 2;
+        const int64_t RF = (int64_t)Nh * N;
+        std::vector<double> qf((size_t)(ncols * RF));
+        for (int64_t c = 0; c < ncols; ++c) {
+            const double *q = qhat.data() + (size_t)(c * R);
+            for (int j = 0; j < N; ++j)
+                for (int i = 0; i < Nh; ++i)
+                    qf[(size_t)(c * RF + i + (int64_t)Nh * j)] =
+                        q[i + (int64_t)N * j] + q[i + (int64_t)N * (j + N)] + q[(N - 1 - i) + (int64_t)N * (2 * N - 1 - j)] +
+                        q[(N - 1 - i) + (int64_t)N * (N - 1 - j)];
+        }
+        if (dev_alloc(&r->d_qfold, qf.size())) {
+            tp_ref_destroy(r);
+            return TP_E_OOM;
+        }
+        if (hipMemcpy(r->d_qfold, qf.data(), sizeof(double) * qf.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            tp_ref_destroy(r);
+            return tp_fail(TP_E_HIP, "histogram upload failed");
+        }
+    }
     *out = r;
     return 0;
 }
@@ -2137,8 +2179,8 @@ int tp_energy(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *co
 
 // The energy with every per-evaluation buffer carved from the static scratch
 // array: the coefficient staging and compact copy, As, G, the energy partial
-// sums and block totals, the k_final counter and, in mixed precision, the
-// shell scales and the 16-bit As.  No allocation, no memset.
+// sums, the shell scales and norms and, in mixed precision, the 16-bit As.
+// No allocation, no memset.
 int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const double *coeff, int64_t ld,
                       int measure, double *E_out)
 {
@@ -2160,13 +2202,11 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
         sizeof(double) * (size_t)b->nlt_pad * b->kl_pad,      // 2 As
         sizeof(double) * (size_t)b->nm_pad64 * b->kl_pad,     // 3 G
         sizeof(double) * (size_t)nrt * 2 * b->kl_pad,         // 4 energy partial sums
-        sizeof(double) * (size_t)b->n_fb,                     // 5 block totals
-        sizeof(unsigned),                                     // 6 k_final counter
-        sizeof(double) * (size_t)2 * b->K,                    // 7 shell scales and norms
-        mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 8 As hi
-        mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 9 As lo
-        mixed16 ? sizeof(double) * (size_t)b->kl_pad : 0,              // 10 1 / column scales
-        sizeof(double) * (size_t)b->K * kNrmLd,               // 11 l-block norms (FP16X3_T64)
+        sizeof(double) * (size_t)2 * b->K,                    // 5 shell scales and norms
+        mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 6 As hi
+        mixed16 ? sizeof(uint16_t) * (size_t)b->kl_pad * b->ldk : 0,   // 7 As lo
+        mixed16 ? sizeof(double) * (size_t)b->kl_pad : 0,              // 8 1 / column scales
+        sizeof(double) * (size_t)b->K * kNrmLd,               // 9 l-block norms (FP16X3_T64)
     };
     constexpr int nbuf = sizeof(want) / sizeof(want[0]);
     size_t off[nbuf], total = 0;
@@ -2183,22 +2223,20 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
     char *base = static_cast<char *>(ctx->scratch);
     double2 *raw = reinterpret_cast<double2 *>(base + off[0]);
     double2 *coef = reinterpret_cast<double2 *>(base + off[1]);
-    EvalScratch sc{reinterpret_cast<double *>(base + off[2]), reinterpret_cast<double *>(base + off[3]),
-                   reinterpret_cast<double *>(base + off[4]), reinterpret_cast<double *>(base + off[5]),
-                   reinterpret_cast<double *>(base + off[10]), reinterpret_cast<double *>(base + off[7]),
-                   reinterpret_cast<unsigned *>(base + off[6]), reinterpret_cast<uint16_t *>(base + off[8]),
-                   reinterpret_cast<uint16_t *>(base + off[9]), reinterpret_cast<double *>(base + off[11])};
+    EvalScratch sc{reinterpret_cast<double *>(base + off[2]),   reinterpret_cast<double *>(base + off[3]),
+                   reinterpret_cast<double *>(base + off[4]),   reinterpret_cast<double *>(base + off[8]),
+                   reinterpret_cast<double *>(base + off[5]),   reinterpret_cast<uint16_t *>(base + off[6]),
+                   reinterpret_cast<uint16_t *>(base + off[7]), reinterpret_cast<double *>(base + off[9])};
     hipStream_t st = ctx->stream;
     // the array holds the last call's values: As and G padding must read as zero
-    // and the arrival counter must start at zero (As, G and the counter are
-    // adjacent; the 16-bit As follow the shell scales)
-    const int64_t nz = (int64_t)((off[6] + want[6] - off[2] + 7) / 8);
+    // (As, G and the partial sums are adjacent; so are the two 16-bit As)
+    const int64_t nz = (int64_t)((off[4] + want[4] - off[2] + 7) / 8);
     hipLaunchKernelGGL(k_zero64, dim3((unsigned)std::min<int64_t>((nz + 255) / 256, 4096)), dim3(256), 0, st,
                        reinterpret_cast<double *>(base + off[2]), nz);
     if (mixed16) {
-        const int64_t n16 = (int64_t)((off[9] + want[9] - off[8]) / 8);
+        const int64_t n16 = (int64_t)((off[7] + want[7] - off[6]) / 8);
         hipLaunchKernelGGL(k_zero64, dim3((unsigned)std::min<int64_t>((n16 + 255) / 256, 4096)), dim3(256), 0, st,
-                           reinterpret_cast<double *>(base + off[8]), n16);
+                           reinterpret_cast<double *>(base + off[6]), n16);
     }
     TP_HIP(hipMemcpyAsync(raw, coeff, want[0], hipMemcpyHostToDevice, st));
     const int64_t n = (int64_t)b->K * b->n_c;
@@ -2285,9 +2323,11 @@ static int enqueue_grad(tp_ctx *ctx, tp_basis *b, const tp_ref *r, const double2
     hipStream_t st = ctx->stream;
     const int ncols = (int)r->ncols;
     const int64_t tap = (int64_t)b->n_m * b->jpad;
+    // the forward left its partial sums in njf rows (folded grid) or njc
+    const int fwd_rows = b->fold && r->d_qfold ? b->njf : b->njc;
     hipLaunchKernelGGL(k_c3sep_bwd, dim3((unsigned)((ncols + 15) / 16), (unsigned)b->njc), dim3(512),
                        c3sep_lds_bytes(b->sep_kb), st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad,
-                       b->d_TB, b->sep_kb, b->N, ncols, r->d_qhat, r->d_qsum, r->d_tf, b->d_part, b->kl_pad, b->njc,
+                       b->d_TB, b->sep_kb, b->N, ncols, r->d_qhat, r->d_qsum, r->d_tf, b->d_part, b->kl_pad, fwd_rows,
                        b->n_m, b->d_gGp, b->kl_pad, (int)b->nm_pad64);
     TP_HIP(hipGetLastError());
     const int n_rt = b->nlt_pad64 / G1_BM, n_ct = (int)(b->kl_pad / G1_BN);

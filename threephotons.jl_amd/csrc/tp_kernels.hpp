@@ -414,6 +414,88 @@ __global__ __launch_bounds__(1024) void k_triple_mfma(const double2 *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// G = H . As of one evaluation in FP64 (the class-weight contraction of the
+// reference's B . PA, correlations.jl:179-193, before the separated grid):
+// G[class][kt] = sum_lt H[lt][class] As[lt][kt], H K-major (ld ldh, rows
+// >= hrows read as zero), K = kdim l-triples (a multiple of 16, padding rows
+// zero).  Workgroup = 16 k-triple columns x 16 RB classes; its 4 waves take
+// a quarter of K each, so they share no operand: the main loop stages
+// nothing through LDS and has no barrier, each wave's RB accumulators are
+// independent MFMA chains, and its operand loads (1 As + RB H values per lane
+// and k-step, from L2) run PF k-steps ahead.  The four partial tiles are
+// summed in LDS in wave order (deterministic).  Workgroups of one column
+// block are dispatched to one XCD (block id mod 8), whose L2 then serves its
+// As columns to all row tiles.
+// ---------------------------------------------------------------------------
+template <int RB, int PF>
+__global__ __launch_bounds__(256) void k_dgemm_g(const double *__restrict__ H, int64_t ldh, int hrows,
+                                                 const double *__restrict__ As, int64_t ldA, int kdim, int ncb,
+                                                 int nrt, double *__restrict__ G, int64_t ldg, int grows)
+{
+    __shared__ double red[4][RB][4][64];   // [wave][row block][v][lane]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, lq = lane >> 4;
+    const int cpx = (ncb + 7) >> 3, idx = blockIdx.x >> 3;
+    const int cb = (blockIdx.x & 7) * cpx + idx / nrt, rt = idx % nrt;
+    if (cb >= ncb) return;   // whole workgroup, before the barrier
+    const int c0 = cb * 16, r0 = rt * 16 * RB;
+    const int kq = kdim >> 2, nks = kq >> 2;   // l-triples and k-steps of this wave
+    const double *pb = As + (int64_t)(wave * kq + lq) * ldA + c0 + l16;
+    // H rows at or past hrows: loaded from the last row (in range) and zeroed
+    // where used, without a branch
+    const double *pa[RB];
+    bool rok[RB];
+#pragma unroll
+    for (int t = 0; t < RB; ++t) {
+        const int row = r0 + 16 * t + l16;
+        rok[t] = row < hrows;
+        pa[t] = H + (int64_t)(wave * kq + lq) * ldh + min(row, hrows - 1);
+    }
+    double qa[PF][RB], qb[PF];
+    auto load = [&](int slot, int ks) {
+        const int64_t o = (int64_t)(4 * ks);
+        qb[slot] = pb[o * ldA];
+#pragma unroll
+        for (int t = 0; t < RB; ++t) qa[slot][t] = pa[t][o * ldh];   // masked where used, so never waited on here
+    };
+    // slot p holds k-step k0 + p; prefetch indices are clamped to the last
+    // k-step (in range; the extra loads go unused)
+#pragma unroll
+    for (int p = 0; p < PF; ++p) load(p, min(p, nks - 1));
+    dbl4 acc[RB];
+#pragma unroll
+    for (int t = 0; t < RB; ++t) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    // k-steps past nks (the last group of PF) run with zero operands: no
+    // remainder branch, which would make the compiler move the accumulators
+    // between register files in every k-step
+    for (int k0 = 0; k0 < nks; k0 += PF) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            const bool kok = k0 + p < nks;
+            const double b = qb[p];
+            double a[RB];
+#pragma unroll
+            for (int t = 0; t < RB; ++t) a[t] = rok[t] && kok ? qa[p][t] : 0.0;
+            load(p, min(k0 + p + PF, nks - 1));
+#pragma unroll
+            for (int t = 0; t < RB; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, acc[t], 0, 0, 0);
+        }
+    }
+    // partial tiles: rows 16 t + lq + 4 v, column l16
+#pragma unroll
+    for (int t = 0; t < RB; ++t)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[wave][t][v][lane] = acc[t][v];
+    __syncthreads();
+    for (int i = tid; i < RB * 256; i += 256) {
+        const int t = i >> 8, v = (i >> 6) & 3, ln = i & 63;
+        const double g = ((red[0][t][v][ln] + red[1][t][v][ln]) + red[2][t][v][ln]) + red[3][t][v][ln];
+        const int row = r0 + 16 * t + (ln >> 4) + 4 * v;
+        if (row < grows) G[(int64_t)row * ldg + c0 + (ln & 15)] = g;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 MFMA GEMM  C[M x N] = A[M x K] . B[K x N]  (v_mfma_f64_16x16x4f64).
 // A is stored K-major (A[k*lda + m]), B as B[k*ldb + n]; K is a multiple of
 // 16 and the padded operands hold zeros.  A BM x BN tile per workgroup of 4
@@ -894,9 +976,12 @@ __global__ __launch_bounds__(256) void k_gemm16(const uint16_t *__restrict__ Ahi
 // ---------------------------------------------------------------------------
 // Energy finish and the annealing step.
 //   E_kt = tF * (qsum * log(S) - Q),  S = sum_rt part0, Q = sum_rt part1,
-// summed per block in a fixed tree, block totals summed in block order by the
-// last block to arrive (one agent-scope fence and atomic per block), so E is
-// bitwise reproducible.  The counter is back at zero at kernel end.
+// in one workgroup of kFinalThreads: thread t sums the columns t, t +
+// kFinalThreads, ..., then a fixed tree, so E is bitwise reproducible.  One
+// workgroup needs no cross-workgroup ordering: with one per 32 columns and
+// the last to arrive summing the block totals, the agent-scope release
+// fence each block needs (an L2 write-back on each XCD) made the kernel 8 us
+// at 3,276 columns, against 3 us.
 // mode 0: E -> Eout;  mode 1: also E -> temp[step] (temperature probe);
 // mode 2: also scal[0] = E (initial energy);
 // mode 3: Metropolis step (evaluate_step, determination.jl:491-515, with
@@ -928,7 +1013,7 @@ struct McArgs {
     uint64_t seq;
 };
 
-constexpr int kFinalCols = 32;   // columns per k_final block (8 threads each)
+constexpr int kFinalThreads = 1024;   // k_final's one workgroup
 
 // What follows an evaluation's energy E (one thread): the energy out, and in
 // the Monte Carlo modes the temperature probe (1), the initial energy (2) or
@@ -969,57 +1054,31 @@ __device__ __forceinline__ void final_energy(double E, double *__restrict__ Eout
     }
 }
 
-__global__ __launch_bounds__(256) void k_final(const double *__restrict__ part, int n_rt, int64_t ldp, int ncols,
-                                               const double *__restrict__ qsum, const double *__restrict__ tf,
-                                               double *__restrict__ blockpart, unsigned *__restrict__ counter,
-                                               double *__restrict__ Eout, McArgs mc)
+__global__ __launch_bounds__(kFinalThreads) void k_final(const double *__restrict__ part, int n_rt, int64_t ldp,
+                                                         int ncols, const double *__restrict__ qsum,
+                                                         const double *__restrict__ tf, double *__restrict__ Eout,
+                                                         McArgs mc)
 {
-    __shared__ double rs[8][kFinalCols], rq[8][kFinalCols], re[kFinalCols];
-    const int tid = threadIdx.x, c = tid % kFinalCols, g = tid / kFinalCols;
-    const int kt = blockIdx.x * kFinalCols + c;
-    double S = 0.0, Q = 0.0;
-    if (kt < ncols)
-        for (int q = g; q < n_rt; q += 8) {
+    __shared__ double fin[kFinalThreads / 64];
+    const int tid = threadIdx.x;
+    double e = 0.0;
+    for (int kt = tid; kt < ncols; kt += kFinalThreads) {
+        double S = 0.0, Q = 0.0;
+#pragma unroll 4
+        for (int q = 0; q < n_rt; ++q) {
             S += part[((int64_t)q * 2 + 0) * ldp + kt];
             Q += part[((int64_t)q * 2 + 1) * ldp + kt];
         }
-    rs[g][c] = S;
-    rq[g][c] = Q;
-    __syncthreads();
-    if (tid < kFinalCols) {
-        double S2 = 0.0, Q2 = 0.0;
-        for (int q = 0; q < 8; ++q) {
-            S2 += rs[q][tid];
-            Q2 += rq[q][tid];
-        }
-        re[tid] = kt < ncols ? tf[kt] * (qsum[kt] * log(S2) - Q2) : 0.0;
+        e += tf[kt] * (qsum[kt] * log(S) - Q);
     }
-    __shared__ int s_last;
-    __shared__ double fin[256];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o);
+    if ((tid & 63) == 0) fin[tid >> 6] = e;
     __syncthreads();
-    if (tid == 0) {
-        double eb = 0.0;
-        for (int q = 0; q < kFinalCols; ++q) eb += re[q];
-        blockpart[blockIdx.x] = eb;
-        __threadfence();
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = prev == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    // last block: all block totals, read in parallel, summed in a fixed tree
-    __threadfence();
-    double e = 0.0;
-    for (unsigned b = tid; b < gridDim.x; b += 256) e += blockpart[b];
-    fin[tid] = e;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (tid < s) fin[tid] += fin[tid + s];
-        __syncthreads();
-    }
     if (tid != 0) return;
-    const double E = fin[0];
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double E = 0.0;
+#pragma unroll
+    for (int w = 0; w < kFinalThreads / 64; ++w) E += fin[w];
     final_energy(E, Eout, mc);
 }
 
@@ -1217,13 +1276,25 @@ __global__ void k_intensity_coeff(const double2 *__restrict__ G, const double *_
 // TB[i][2 m2i + cs] = cos(m2 a_i) / -sin(m2 a_i) (ld kb).  Energy partial
 // sums of the workgroup's rows go to part row blockIdx.y (as k_gemm does per
 // row tile); OUT = 1 also writes c3 (column-major, ld ldo).
+//
+// FOLD (energy only, even N, every class with even m2 and even mu3, i.e.
+// lambda dq = 0): on the grids a_i = (i + 1/2) pi / N, b_j = (j + 1/2) pi / N
+// (alpharange, alpharange_2pi; correlations.jl:247-256) c3 takes the same
+// value on the four bins (i, j), (i, j + N), (N-1-i, 2N-1-j), (N-1-i, N-1-j):
+// b + pi changes mu3 b by a multiple of 2 pi, and (pi - a, 2 pi - b) turns
+// m2 a + mu3 b into a multiple of 2 pi minus itself.  The four bins of every
+// orbit are distinct, and the orbits of i < N/2, j < N cover the grid once,
+// so the kernel forms c3 on those N^2/2 bins only: S gets 4 p per bin and Q
+// the folded histogram's sum of the orbit's four q values (qhat [column]
+// [i + (N/2) j]), the same sums as over the whole grid.  A quarter of stage
+// B, of the logarithms and of the histogram reads, half of stage A.
 // ---------------------------------------------------------------------------
 constexpr int kSepMaxKB = 24;   // 2 (L/2 + 1) padded to 4, L <= 22
 
 // Dynamic LDS of k_c3sep: CS[kb][16][17] doubles.
 inline size_t c3sep_lds_bytes(int kb) { return sizeof(double) * (size_t)kb * 16 * 17; }
 
-template <int OUT, bool ENERGY, bool BATCH = false>
+template <int OUT, bool ENERGY, bool BATCH = false, bool FOLD = false>
 __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
                                                int n_m2, const double *__restrict__ TA, int64_t ta_plane, int jpad,
                                                const double *__restrict__ TB, int kb, int N, int ncols,
@@ -1233,6 +1304,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
                                                const double *const *__restrict__ qtab = nullptr,
                                                const int *__restrict__ nvalid = nullptr)
 {
+    static_assert(!FOLD || (ENERGY && OUT == 0), "the folded grid forms the energy only");
     __shared__ __align__(16) double s_lt[ENERGY ? kLogTabSize : 1];
     __shared__ double red[8][2][16];
     extern __shared__ double cs_dyn[];
@@ -1246,20 +1318,21 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
     // stage A wait for them.  Three workgroups fit a CU (dynamic LDS sized kb).
     constexpr int NW = 8, MAXB = 4;   // stage-B blocks per wave: 16 j x (N/16) i-blocks / 8 waves (N <= 32)
     const int c0 = blockIdx.x * 16, jc = blockIdx.y, j0 = jc * 16;
+    // bins: i < NI, j < NJ; histogram column of RQ values, bin (i, j) at i + NI j
+    const int NI = FOLD ? N / 2 : N, NJ = FOLD ? N : 2 * N, RQ = NI * NJ;
     // BATCH: the 16 columns lie in one 64-column block of a batch, whose
     // histogram columns start at qtab[block] (nvalid[block] of them are real)
     const int nv = BATCH ? nvalid[c0 >> 6] - (c0 & 63) : ncols - c0;
     const double *qbase = nullptr;
     if (ENERGY) {
-        if (!BATCH) qbase = qhat + (int64_t)c0 * 2 * N * N;
-        else if (nv > 0) qbase = qtab[c0 >> 6] + (int64_t)(c0 & 63) * 2 * N * N;
+        if (!BATCH) qbase = qhat + (int64_t)c0 * RQ;
+        else if (nv > 0) qbase = qtab[c0 >> 6] + (int64_t)(c0 & 63) * RQ;
         else qbase = qtab[c0 >> 6];   // no real column: loads from the block's base, results unused
     }
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar table loads
     const int l16 = lane & 15, lq = lane >> 4;
-    const int R = 2 * N * N, twoN = 2 * N;
-    const int nib = (N + 15) / 16, nob = 16 * nib;
+    const int nib = (NI + 15) / 16, nob = 16 * nib;
     // padding rows of CS (k' >= 2 n_m2) stay zero
     for (int i = tid; i < (kb - 2 * n_m2) * 256; i += 512) {
         const int k = 2 * n_m2 + i / 256, r = (i / 16) % 16, c = i % 16;
@@ -1311,14 +1384,14 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
         if (ob >= nob) break;
         const int jl = ob / nib, ib = ob % nib;
         const int j = j0 + jl;
-        if (j >= twoN) continue;   // wave-uniform
+        if (j >= NJ) continue;   // wave-uniform
         // histogram values of this block (transposed: lane -> column c0 + lq + 4v,
         // row i = 16 ib + l16), requested before its MFMAs
         double qv[4];
         if (ENERGY) {
-            const int iq = min(16 * ib + l16, N - 1);
+            const int iq = min(16 * ib + l16, NI - 1);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) qv[v] = qbase[(int64_t)max(0, min(lq + 4 * v, nv - 1)) * R + iq + N * j];
+            for (int v = 0; v < 4; ++v) qv[v] = qbase[(int64_t)max(0, min(lq + 4 * v, nv - 1)) * RQ + iq + NI * j];
         }
         dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1328,7 +1401,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int col = c0 + lq + 4 * v;
-            if (lq + 4 * v < nv && i < N) {
+            if (lq + 4 * v < nv && i < NI) {
                 const int r = i + N * j;
                 const double c = acc[v];
                 if (OUT == 1) out[(int64_t)col * ldo + r] = c;
@@ -1362,7 +1435,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
                 a += red[w][0][tid];
                 bsum += red[w][1][tid];
             }
-            part[((int64_t)jc * 2 + 0) * ldp + c0 + tid] = a;
+            part[((int64_t)jc * 2 + 0) * ldp + c0 + tid] = FOLD ? 4.0 * a : a;   // FOLD: four bins per orbit
             part[((int64_t)jc * 2 + 1) * ldp + c0 + tid] = bsum;
         }
     }
