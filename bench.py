@@ -46,6 +46,15 @@ def folded_classes(L):
     return n
 
 
+def sep_grid_flops(L, N):
+    """FP64 flops per k-triple column of k_c3sep on the folded grid (lambda = 0,
+    even N: the N^2/2 orbit representatives i < N/2, j < N): stage A (C, S over
+    mu3 at the N values of beta) 4 n_m N, stage B (over m2 = 0, 2, .., L at
+    each representative bin) 4 n_m2 N^2/2."""
+    n_m, n_m2 = folded_classes(L), L // 2 + 1
+    return 4.0 * n_m * N + 2.0 * n_m2 * N * N
+
+
 def lt_n3_sum(L):
     """Sum over l-triples (even l, triangle rule) of 2 l3 + 1."""
     s = 0
@@ -334,8 +343,8 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
     ms_sweep = 1e3 * el / nsweeps
     if prec == "fp16x3_t64":
         # factored contraction: G = H.As, then the separated grid (stages A and B)
-        n_m, n_m2, N = folded_classes(cfg["L"]), cfg["L"] // 2 + 1, cfg["N"]
-        flops = (2.0 * n_m * basis.n_lt + 8.0 * n_m * N + 4.0 * n_m2 * basis.R) * cols
+        n_m = folded_classes(cfg["L"])
+        flops = (2.0 * n_m * basis.n_lt + sep_grid_flops(cfg["L"], cfg["N"])) * cols
         contraction = "k_hgemm_g (G = H.As, FP16 hi/lo MFMA) + k_c3sep (c3 = Bm.G, FP64 separated grid, energy)"
     else:
         flops = 2.0 * basis.R * basis.n_lt * cols
@@ -812,10 +821,8 @@ def main():
     kern_ms = stage_times(ctx, state, H, args.steps)
     n_m = folded_classes(cfg["L"])
     flops_g1 = 2.0 * n_m * basis.n_lt * basis.klength         # G = H . As
-    # c3 = Bm . G on the separated (alpha, beta) grid: stage A (C, S over mu3 at
-    # 2N values of beta) and stage B (over m2 = 0, 2, .., L at every bin)
-    n_m2 = cfg["L"] // 2 + 1
-    flops_g2 = (8.0 * n_m * cfg["N"] + 4.0 * n_m2 * basis.R) * basis.klength
+    # c3 = Bm . G and the energy on the folded separated (alpha, beta) grid
+    flops_g2 = sep_grid_flops(cfg["L"], cfg["N"]) * basis.klength
     K = cfg["K"]
     # triple products on sorted l-triples (the form k_triple_sym executes); the
     # coupling form over all 505 ordered l-triples is reported beside it

@@ -251,6 +251,54 @@ def test_c3_fp32_tables_gap(oracle):
     assert 1e-9 < e < 1e-5
 
 
+def _orbits(N):
+    """Bins r = i + N j of the orbits {(i, j), (i, j+N), (N-1-i, 2N-1-j),
+    (N-1-i, N-1-j)} of the representatives i < N/2, j < N (even N)."""
+    i, j = np.meshgrid(np.arange(N // 2), np.arange(N), indexing="ij")
+    return [a.ravel() + N * b.ravel() for a, b in
+            ((i, j), (i, j + N), (N - 1 - i, 2 * N - 1 - j), (N - 1 - i, N - 1 - j))]
+
+
+def test_c3_lambda0_folded_grid(oracle):
+    """lambda = 0: every class has even m2 and mu3, so on the midpoint grids
+    c3 takes one value on each orbit of four bins, the orbits partition the
+    2N^2 bins, and the energy from the N^2/2 representatives with the folded
+    histogram (S: 4 p per bin; Q: the orbit's summed q) is the full-grid
+    energy (the folding of the library's k_c3sep FOLD)."""
+    L, LMAX, N, K = 8, 10, 8, 3
+    c = random_real_coeff(K, L, LMAX, 21, c00=5.0)
+    b = oracle.Basis(L, LMAX, N, K)
+    c3 = b.c3_factored(c)
+    orb = _orbits(N)
+    allbins = np.sort(np.concatenate(orb))
+    assert np.array_equal(allbins, np.arange(2 * N * N))
+    scale = np.abs(c3).max()
+    for o in orb[1:]:
+        assert np.abs(c3[o] - c3[orb[0]]).max() <= 1e-13 * scale
+    # the energy from the representatives and the folded histogram
+    ref_cols = np.random.default_rng(5).uniform(0.1, 1.0, size=c3.shape)
+    ref = c3ref_from_columns(ref_cols, N, K)
+    E_full = _energy(oracle, c3, N, ref, K)
+    E_fold = 0.0
+    for i, (k1, k2, k3) in enumerate(ktriples(K)):
+        q = ref_cols[:, i] / ref_cols[:, i].sum()
+        p = np.maximum(c3[orb[0], i], 1e-30)
+        qf = sum(q[o] for o in orb)
+        S, Q = 4.0 * p.sum(), (qf * np.log(p)).sum()
+        E_fold += oracle.tripletFactor(k1, k2, k3) * (q.sum() * math.log(S) - Q)
+    assert E_fold == pytest.approx(E_full, rel=1e-13)
+
+
+def test_c3_curved_ewald_no_folding(oracle):
+    """lambda dq != 0 brings odd m2 and mu3: c3 differs across an orbit, so
+    the library keeps the full grid there."""
+    L, LMAX, N, K = 6, 8, 4, 2
+    c = random_real_coeff(K, L, LMAX, 22, c00=5.0)
+    c3 = oracle.Basis(L, LMAX, N, K, lam=2.5, dq=1.0).c3_dense(c)
+    orb = _orbits(N)
+    assert np.abs(c3[orb[1]] - c3[orb[0]]).max() > 1e-6 * np.abs(c3).max()
+
+
 # ---------------------------------------------------------------- energy
 def _energy(oracle, c3raw, N, c3ref, K, lo=1):
     return oracle.energy(c3raw, N, c3ref, c3ref.shape[2], lo, K)
