@@ -88,7 +88,7 @@ def sorted_lt_flops(L, K):
 # dispatch), scaled by what the same counters read for a 256 MiB device copy
 # (FETCH_SIZE 0.5x, WRITE_SIZE 1.0x of the bytes moved); the summary with the
 # commands is profiles/hbm_counters_full_L18_K26_N32.md.
-TRIPLE_FETCH_KIB, TRIPLE_WRITE_KIB = 1679.2, 13708.1
+TRIPLE_FETCH_KIB, TRIPLE_WRITE_KIB = 1683.8, 13708.3
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
