@@ -73,9 +73,10 @@ def _hist(tp, c, L, LMAX, N, K, seed):
     return c3ref_from_columns(cnt, N, K)
 
 
-@pytest.mark.parametrize("prec", ["fp16x3", "fp64", "fp16x3_t64"])
-def test_energy_batch_matches_single(tp, prec):
-    L, LMAX, N = 8, 25, 8
+@pytest.mark.parametrize("prec,N", [("fp16x3", 8), ("fp64", 8), ("fp16x3_t64", 8),
+                                    ("fp16x3_t64", 7)])   # odd N: the full (alpha, beta) grid, no folding
+def test_energy_batch_matches_single(tp, prec, N):
+    L, LMAX = 8, 25
     Ks = [3, 9, 5, 12]
     starts = [random_real_coeff(K, L, LMAX, seed=40 + K, c00=4.0, decay=0.1) for K in Ks]
     refs = [_hist(tp, starts[i], L, LMAX, N, K, 70 + i) for i, K in enumerate(Ks)]
