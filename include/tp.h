@@ -433,9 +433,9 @@ int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, cons
 void tp_batch_destroy(tp_batch *B);
 /* Time the candidate triple-product work orders and contraction kernels on
  * this batch (reps launches each) and keep the fastest.  ms_out[14] (may be
- * NULL): work orders 0 (by cost), 1 (evaluation-major; 1e30 in
- * TP_PREC_FP16X3_T64, whose sorted l-triple form has one order), then the
- * contraction candidates.  Dense form (FP16X3, BF16): tiles 128x64, 64x64,
+ * NULL): work orders 0 (by cost), 1 (evaluation-major; in TP_PREC_FP16X3_T64
+ * the sorted l-triple items of each evaluation on one XCD, one evaluation
+ * after another), then the contraction candidates.  Dense form (FP16X3, BF16): tiles 128x64, 64x64,
  * 128x128, 64x128 with parallel row tiles, the same four with row-walking
  * workgroups, and the same four with the row tiles partitioned over the XCDs
  * (1e30 where a shape does not divide by 8); config_out[4] (may be NULL):

@@ -149,12 +149,13 @@ struct tp_basis {
     bool use_sym = false;
     int nsym_work = 0;
     size_t sym_lds = 0, sym_lds_b = 0;   // dynamic LDS of k_triple_sym, k_triple_sym_batch
-    int *d_sym_work = nullptr;     // [nsym_work] ts | h1 << 16 | h2 << 20, most expensive first
+    SymItem *d_sym_work = nullptr; // [nsym_work] work items, most expensive first
     int *d_perms = nullptr;        // [n_lt][8] orderings of each sorted l-triple
     int *d_gorder = nullptr;       // [n_groups] run order of the mu3 groups of sorted l-triples
     uint32_t *d_symtab = nullptr;  // V entries each permutation stores (k_triple_sym)
     int *d_symoff = nullptr;       // [6 * 2 * 2 * 2 + 1]
     std::vector<std::pair<int, double>> h_sym_cost;   // per sorted l-triple: (ts, stage-1 + write cost)
+    std::vector<SymItem> h_sym_item;                  // per l-triple: its work item with h1 = h2 = 0
     std::vector<int> h_lt_cnt, h_lt_l3;   // per l-triple: basis entries, l3 (work lists of batches)
     // mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_BF16)
     int prec = TP_PREC_FP64;
@@ -313,11 +314,11 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     } else if (b->use_sym) {   // FP64; TP_PREC_FP16X3_T64 with its stage 2 on FP16 MFMA
         if (prec == TP_PREC_FP16X3_T64)
             hipLaunchKernelGGL(k_triple_sym16, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1,
-                               sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
+                               sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_sym_work,
                                b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad, sc.nrm2);
         else
             hipLaunchKernelGGL(k_triple_sym, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1,
-                               sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_sym_work,
+                               sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_sym_work,
                                b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad, nullptr);
     } else if (b->use_mfma) {
         hipLaunchKernelGGL(k_triple_mfma, dim3((unsigned)b->nwork), dim3(1024), b->mfma_lds, st, b0, b1, sel, flip,
@@ -493,12 +494,13 @@ struct tp_batch {
     int prec = TP_PREC_FP16X3;             // the basis's precision mode when the batch was made
     int gcand = 0, order = 0;              // contraction candidate (kTiles), triple-product work order (autotuned)
     bool fold = false;                     // TP_PREC_FP16X3_T64 energies on the folded grid (k_c3sep FOLD)
+    int sym_order = 0;                     // work order of the sorted l-triple items (batch_sym_work)
     int family = TP_MOVE_PLANE;            // move family of the random slots
     double tune_ms[2 + kNumTiles] = {};
     EvalDesc *d_ev = nullptr;
     GroupState *d_gs = nullptr;
     int2 *d_work = nullptr;
-    int2 *d_swork = nullptr;       // sorted l-triple items (evaluation, ts | h1 << 16 | h2 << 20), by cost
+    SymItem *d_swork = nullptr;    // sorted l-triple work items of every evaluation, by cost
     int nswork = 0;
     double2 *d_cur = nullptr, *d_prop = nullptr;
     double *d_quat = nullptr, *d_As = nullptr, *d_sc = nullptr, *d_isc = nullptr, *d_part = nullptr;
@@ -568,14 +570,21 @@ static void batch_items(const tp_basis *b, int e, int K, std::vector<std::pair<d
 }
 
 // Work items of the sorted l-triple form for every evaluation (both K1 and K2
-// halves where the evaluation has more than 16 shells), most expensive first.
-static int batch_sym_work(tp_batch *B)
+// halves where the evaluation has more than 16 shells).  order 0: all items by
+// decreasing cost.  order 1: evaluation-major per XCD -- the evaluations are
+// dealt to the 8 XCDs (workgroup i runs on XCD i mod 8) balancing their
+// cost, each XCD runs its evaluations one after another, an evaluation's
+// items heaviest first, so the coefficients of the one or two evaluations in
+// flight on an XCD stay in its L2; queues are padded to equal length with
+// empty items (e = -1).
+static int batch_sym_work(tp_batch *B, int order)
 {
     const tp_basis *b = B->b;
     if (!b->use_sym) return 0;
-    std::vector<std::pair<double, int2>> all;
+    std::vector<std::vector<std::pair<double, SymItem>>> per((size_t)B->ne);
+    std::vector<double> ecost((size_t)B->ne, 0.0);
     for (int e = 0; e < B->ne; ++e) {
-        const int nh = B->h_ev[e].K > 16 ? 2 : 1;
+        const int K = B->h_ev[e].K, nh = K > 16 ? 2 : 1;
         for (const auto &tc : b->h_sym_cost) {
             const int n3 = 2 * b->h_lt_l3[tc.first] + 1;
             const double cost = tc.second + ((n3 + 1) / 2) * 2.0 * nh;
@@ -583,17 +592,58 @@ static int batch_sym_work(tp_batch *B)
             const bool sym12 = (li.w & 0xff) == ((li.w >> 8) & 0xff);   // items h1 < h2 stored by their mirrors
             for (int h1 = 0; h1 < nh; ++h1)
                 for (int h2 = 0; h2 < nh; ++h2)
-                    if (!(sym12 && h1 < h2)) all.push_back({cost, make_int2(e, tc.first | (h1 << 16) | (h2 << 20))});
+                    if (!(sym12 && h1 < h2)) {
+                        SymItem it = b->h_sym_item[tc.first];
+                        it.e = e;
+                        it.K = K;
+                        it.tsx |= h1 << 11 | h2 << 12;
+                        per[(size_t)e].push_back({cost, it});
+                        ecost[(size_t)e] += cost;
+                    }
         }
+        std::stable_sort(per[(size_t)e].begin(), per[(size_t)e].end(),
+                         [](const std::pair<double, SymItem> &a, const std::pair<double, SymItem> &c) {
+                             return a.first > c.first;
+                         });
     }
-    std::stable_sort(all.begin(), all.end(), [](const std::pair<double, int2> &a, const std::pair<double, int2> &c) {
-        return a.first > c.first;
-    });
-    std::vector<int2> w(all.size());
-    for (size_t i = 0; i < all.size(); ++i) w[i] = all[i].second;
-    if (int rc = dev_alloc(&B->d_swork, w.size())) return rc;
-    TP_HIP(hipMemcpy(B->d_swork, w.data(), sizeof(int2) * w.size(), hipMemcpyHostToDevice));
+    std::vector<SymItem> w;
+    if (order == 0) {
+        std::vector<std::pair<double, SymItem>> all;
+        for (auto &v : per) all.insert(all.end(), v.begin(), v.end());
+        std::stable_sort(all.begin(), all.end(),
+                         [](const std::pair<double, SymItem> &a, const std::pair<double, SymItem> &c) {
+                             return a.first > c.first;
+                         });
+        for (auto &x : all) w.push_back(x.second);
+    } else {
+        std::vector<int> eo((size_t)B->ne);
+        for (int e = 0; e < B->ne; ++e) eo[(size_t)e] = e;
+        std::stable_sort(eo.begin(), eo.end(), [&](int a, int c) { return ecost[(size_t)a] > ecost[(size_t)c]; });
+        std::vector<std::vector<SymItem>> q(8);
+        double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int e : eo) {
+            int x = 0;
+            for (int y = 1; y < 8; ++y)
+                if (load[y] < load[x]) x = y;
+            for (auto &it : per[(size_t)e]) q[(size_t)x].push_back(it.second);
+            load[x] += ecost[(size_t)e];
+        }
+        size_t len = 0;
+        for (auto &v : q) len = std::max(len, v.size());
+        SymItem empty{};
+        empty.e = -1;
+        for (size_t i = 0; i < len; ++i)
+            for (int x = 0; x < 8; ++x) w.push_back(i < q[(size_t)x].size() ? q[(size_t)x][i] : empty);
+    }
+    if (B->d_swork && (int)w.size() != B->nswork) {
+        (void)hipFree(B->d_swork);
+        B->d_swork = nullptr;
+    }
+    if (!B->d_swork)
+        if (int rc = dev_alloc(&B->d_swork, w.size())) return rc;
+    TP_HIP(hipMemcpy(B->d_swork, w.data(), sizeof(SymItem) * w.size(), hipMemcpyHostToDevice));
     B->nswork = (int)w.size();
+    B->sym_order = order;
     return 0;
 }
 
@@ -735,7 +785,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         batch_free(B);
         return tp_fail(TP_E_HIP, std::string("batch set-up: ") + hipGetErrorString(e));
     }
-    if ((rc = batch_upload_work(B, 0)) || (rc = batch_sym_work(B))) {
+    if ((rc = batch_upload_work(B, 0)) || (rc = batch_sym_work(B, 0))) {
         batch_free(B);
         return rc;
     }
@@ -849,7 +899,8 @@ static void batch_triple(const tp_batch *B, hipStream_t st)
                        B->d_isc, B->d_shsc, b->K);
     if (B->prec == TP_PREC_FP16X3_T64)
         hipLaunchKernelGGL(k_triple_sym_batch, dim3((unsigned)B->nswork), dim3(kSymThreadsBatch), b->sym_lds_b, st, B->d_ev,
-                           B->d_swork, b->n_c, b->d_ew, b->d_eidx, b->d_gstart, b->d_ltinfo, b->d_perms, b->d_gorder,
+                           B->d_swork, B->d_prop, (int64_t)B->cstride, b->n_c, b->d_ew, b->d_eidx, b->d_gstart,
+                           b->d_perms, b->d_gorder,
                            b->d_symtab, b->d_symoff, b->Lw, B->d_As, B->ldA);
     else
         hipLaunchKernelGGL(k_triple16_batch, dim3((unsigned)B->nwork), dim3(512), b->t16_lds, st, B->d_ev, B->d_work,
@@ -965,18 +1016,17 @@ static int batch_autotune(tp_batch *B, int reps)
     hipStream_t st = B->ctx->stream;
     double best = 1e30;
     int best_order = 0;
-    // work orders of the coupling form (the sorted l-triple form has one order)
-    const int norders = B->prec == TP_PREC_FP16X3_T64 ? 1 : 2;
+    // work orders: of the coupling form, or (FP16X3_T64) of the sorted l-triple form
+    const bool sym = B->prec == TP_PREC_FP16X3_T64;
     for (int order = 0; order < 2; ++order) {
         B->tune_ms[order] = 1e30;
-        if (order >= norders) continue;
-        if (int rc = batch_upload_work(B, order)) return rc;
+        if (int rc = sym ? batch_sym_work(B, order) : batch_upload_work(B, order)) return rc;
         double ms = 0;
         if (int rc = batch_time(B, reps, [&](hipStream_t s) { batch_triple(B, s); }, &ms)) return rc;
         B->tune_ms[order] = ms;
         if (ms < best) { best = ms; best_order = order; }
     }
-    if (int rc = batch_upload_work(B, best_order)) return rc;
+    if (int rc = sym ? batch_sym_work(B, best_order) : batch_upload_work(B, best_order)) return rc;
     batch_triple(B, st);
     best = 1e30;
     int best_t = 0;
@@ -1719,9 +1769,10 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     // sorted l-triples for k_triple_sym: each ordered l-triple t maps to the
     // sorted one ts (l1 >= l2 >= l3) by a permutation sigma, l_i = L_sigma(i);
     // only for the library's own (permutation-symmetric) 3j weights
-    std::vector<int> sym_work, perms, gorder, symoff;
+    std::vector<int> perms, gorder, symoff;
+    std::vector<SymItem> sym_work;
     std::vector<uint32_t> symtab;
-    if (!hw && K <= 32 && nlt > 0) {
+    if (!hw && K <= 32 && nlt > 0 && nlt < 2048) {   // sym_tsx: 11 bits of l-triple index
         std::unordered_map<int, int> index_of;   // l1 | l2 << 8 | l3 << 16 -> lt
         for (int t = 0; t < nlt; ++t)
             index_of[(ltinfo[t].w & 0xffff) | (ltinfo[t].z << 16)] = t;
@@ -1782,7 +1833,22 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
                         if (!(sym12 && h1 < h2)) wl.push_back({-cost, ts | (h1 << 16) | (h2 << 20)});
             }
             std::stable_sort(wl.begin(), wl.end());
-            for (auto &w : wl) sym_work.push_back(w.second);
+            // the work items: each l-triple's record with the item's halves
+            b->h_sym_item.assign((size_t)nlt, SymItem{});
+            for (int t = 0; t < nlt; ++t) {
+                const int g0 = ltinfo[t].y, n3 = 2 * ltinfo[t].z + 1;
+                SymItem &it = b->h_sym_item[(size_t)t];
+                it.tsx = sym_tsx(t, 0, 0, ltinfo[t].w & 0xff, (ltinfo[t].w >> 8) & 0xff, ltinfo[t].z);
+                it.e0 = gstart[g0];
+                it.ne = gstart[g0 + n3] - gstart[g0];
+                it.goff = g0;
+                it.K = K;
+            }
+            for (auto &w : wl) {
+                SymItem it = b->h_sym_item[(size_t)(w.second & 0xffff)];
+                it.tsx |= ((w.second >> 16) & 15) << 11 | ((w.second >> 20) & 15) << 12;
+                sym_work.push_back(it);
+            }
             // per (permutation, K1 half, K2 half, K3 block): the tile entries
             // V[K1][K2][K3] that are As[lt][kt] for the sorted k-triple
             // kt = (k1, k2, k3), k_i = K_sigma(i), in kt order
@@ -4068,7 +4134,7 @@ int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out)
         for (int i = 0; i < 2 + kNumTiles; ++i) ms_out[i] = B->tune_ms[i];
     if (config_out && B->prec == TP_PREC_FP16X3_T64) {
         fac_shape(B->b, B->gcand, &config_out[0], &config_out[1]);
-        config_out[2] = B->order;
+        config_out[2] = B->sym_order;
         config_out[3] = 4 + B->gcand;
     } else if (config_out) {
         config_out[0] = kTiles[B->gcand][0];

@@ -52,8 +52,20 @@ static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves)
            sizeof(double) * 2 * NW * 256 + 64;
 }
 
-// work[i] = ts | h1 << 16 | h2 << 20 (ts: index of the sorted l-triple in the
-// ordered list); perms[ts * 8] = number n of orderings, perms[ts * 8 + 1 + q]
+// A work item: everything its operand loads need, in one 32-byte record, so
+// that they all issue after one (uniform) load: ts = index of the sorted
+// l-triple in the ordered list, its l's and the item's K1 / K2 halves packed
+// in tsx (sym_tsx), its entries ew/eidx[e0 .. e0 + ne) and mu3 groups from
+// goff (gstart, gorder); e and K: the evaluation of a batch and its shells.
+struct SymItem {
+    int e, tsx, e0, ne, goff, K, pad0, pad1;
+};
+__host__ __device__ constexpr int sym_tsx(int ts, int h1, int h2, int L1, int L2, int L3)
+{
+    return ts | h1 << 11 | h2 << 12 | L1 << 13 | L2 << 18 | L3 << 23;   // ts < 2048, l <= 31
+}
+
+// perms[ts * 8] = number n of orderings, perms[ts * 8 + 1 + q]
 // = lt | p << 24 with p the index of the permutation sigma of ordering q in
 // kSymPerm (k_i = K_sigma(i)); gorder[first group of ts + u] = the mu3 group
 // run u-th; symtab[symoff[((p * 2 + h1) * 2 + h2) * 2 + cb] ..
@@ -80,25 +92,24 @@ static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves)
 // writes (one group, 64 pairs) and the stage-2 reads (16 pairs, 4 groups per
 // lane) both spread over the banks.
 template <int NW, bool G3, bool S16 = false>
-__device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K, int wk,
-                                                const double *__restrict__ ew, const uint32_t *__restrict__ eidx,
-                                                const int *__restrict__ gstart, const int4 *__restrict__ ltinfo,
+__device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K,
+                                                const SymItem &it, const double *__restrict__ ew,
+                                                const uint32_t *__restrict__ eidx, const int *__restrict__ gstart,
                                                 const int *__restrict__ perms, const int *__restrict__ gorder,
                                                 const uint32_t *__restrict__ symtab, const int *__restrict__ symoff,
                                                 int Lw, double *__restrict__ As, int64_t ldA, int cbase, int nprop,
                                                 int p0, uint32_t klen, double *__restrict__ nrm2)
 {
-    const int ts = wk & 0xffff, h1 = (wk >> 16) & 15, h2 = (wk >> 20) & 15;
-    const int4 li = ltinfo[ts];
-    const int L3 = li.z, L1 = li.w & 0xff, L2 = (li.w >> 8) & 0xff;
+    const int ts = it.tsx & 2047, h1 = (it.tsx >> 11) & 1, h2 = (it.tsx >> 12) & 1;
+    const int L1 = (it.tsx >> 13) & 31, L2 = (it.tsx >> 18) & 31, L3 = (it.tsx >> 23) & 31;
     const int n1 = 2 * L1 + 1, n2 = 2 * L2 + 1, n3 = 2 * L3 + 1;
-    const int goff = li.y;
+    const int goff = it.goff;
     // LDS carve-up (sizes from the order bound Lw so that the layout is fixed)
     double2 *Ab = sm;                                      // [n1][16]  c_K1[L1, m1], K1 = 16 h1 + i + 1
     double2 *Bb = Ab + 16 * (size_t)Lw;                    // [n2][16]  c_K2[L2, m2]
     double2 *Cb = Bb + 16 * (size_t)Lw;                    // [n3][K]   c_K3[L3, mu3]
     double *Ws = reinterpret_cast<double *>(Cb + (size_t)K * Lw);   // [E] weights
-    const int e0 = gstart[goff], ne = gstart[goff + n3] - e0;
+    const int e0 = it.e0, ne = it.ne;
     uint32_t *Is = reinterpret_cast<uint32_t *>(Ws + ne);           // [E] local (m1 index | m2 index << 16)
     int *Gs = reinterpret_cast<int *>(Is + ne);                     // [n3 + 1] group starts
     int *Od = Gs + (n3 + 1);                                        // [n3] run order of the groups
@@ -382,8 +393,8 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
                                                            const double *__restrict__ ew,
                                                            const uint32_t *__restrict__ eidx,
                                                            const int *__restrict__ gstart,
-                                                           const int4 *__restrict__ ltinfo,
-                                                           const int *__restrict__ work, const int *__restrict__ perms,
+                                                           const SymItem *__restrict__ items,
+                                                           const int *__restrict__ perms,
                                                            const int *__restrict__ gorder,
                                                            const uint32_t *__restrict__ symtab,
                                                            const int *__restrict__ symoff, int Lw,
@@ -391,22 +402,23 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
                                                            double *__restrict__ nrm2)
 {
     extern __shared__ double2 sm[];
-    triple_sym_body<NW, G3, S16>(sm, pick(cb0, cb1, sel, flip), n_c, K, work[blockIdx.x], ew, eidx, gstart, ltinfo,
-                                 perms, gorder, symtab, symoff, Lw, As, ldA, 0, 1, 0, 0xffffffffu, nrm2);
+    const SymItem it = items[blockIdx.x];
+    triple_sym_body<NW, G3, S16>(sm, pick(cb0, cb1, sel, flip), n_c, K, it, ew, eidx, gstart, perms, gorder, symtab,
+                                 symoff, Lw, As, ldA, 0, 1, 0, 0xffffffffu, nrm2);
 }
 
-// The same on a batch of evaluations (struct EvalDesc): work item i =
-// (evaluation, ts | h1 << 16 | h2 << 20); the tables are those of the
-// batch's basis (K_b shells): the entries of an evaluation of K <= K_b
-// shells are the ones with kt < K (K + 1) (K + 2) / 6, since a sorted
-// k-triple's index does not depend on the shell count.
+// The same on a batch of evaluations (struct EvalDesc): item.e is the
+// evaluation (coefficients at coef + e cstride), item.K its shells; the
+// tables are those of the batch's basis (K_b shells): the entries of an
+// evaluation of K <= K_b shells are the ones with kt < K (K + 1) (K + 2) / 6,
+// since a sorted k-triple's index does not depend on the shell count.
 template <int NW, bool G3, bool S16, typename Desc>
 __global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__restrict__ ev,
-                                                                 const int2 *__restrict__ work, int n_c,
-                                                                 const double *__restrict__ ew,
+                                                                 const SymItem *__restrict__ items,
+                                                                 const double2 *__restrict__ coef, int64_t cstride,
+                                                                 int n_c, const double *__restrict__ ew,
                                                                  const uint32_t *__restrict__ eidx,
                                                                  const int *__restrict__ gstart,
-                                                                 const int4 *__restrict__ ltinfo,
                                                                  const int *__restrict__ perms,
                                                                  const int *__restrict__ gorder,
                                                                  const uint32_t *__restrict__ symtab,
@@ -414,11 +426,13 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__re
                                                                  double *__restrict__ As, int64_t ldA)
 {
     extern __shared__ double2 sm[];
-    const int2 w = work[blockIdx.x];
-    const Desc &d = ev[w.x];
-    const int K = d.K;
-    triple_sym_body<NW, G3, S16>(sm, d.coef, n_c, K, w.y, ew, eidx, gstart, ltinfo, perms, gorder, symtab, symoff, Lw,
-                            As, ldA, d.cbase, d.nprop, d.p, (uint32_t)(K * (K + 1) * (K + 2) / 6), nullptr);
+    const SymItem it = items[blockIdx.x];
+    if (it.e < 0) return;       // padding of an XCD queue (whole workgroup, before any barrier)
+    const Desc &d = ev[it.e];   // column placement, needed only at the write-out
+    const int K = it.K;
+    triple_sym_body<NW, G3, S16>(sm, coef + it.e * cstride, n_c, K, it, ew, eidx, gstart, perms, gorder, symtab,
+                                 symoff, Lw, As, ldA, d.cbase, d.nprop, d.p, (uint32_t)(K * (K + 1) * (K + 2) / 6),
+                                 nullptr);
 }
 
 // The configuration the library launches.
