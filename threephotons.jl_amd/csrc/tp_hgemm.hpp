@@ -22,7 +22,9 @@
 // (KS = 128 for one evaluation: few slices, so little serial load latency).
 // Column scales: from the shell norms (SINGLE: one evaluation,
 // nrm2[k][l/2] = |c_k,l|^2 written by k_triple_sym) or from sc[] (batches:
-// k_colscale_batch).
+// k_colscale_batch).  Batches (A32) keep As in FP32: f16_split rounds its
+// argument to FP32 first, and a power-of-two scale commutes with that
+// rounding, so the split of (double) As32 x sc equals that of As x sc.
 #pragma once
 
 #include "tp_kernels.hpp"
@@ -45,12 +47,13 @@ __device__ __forceinline__ double hg_kt_scale(int kt, const double *nk)
     return bnd > 0.0 ? ldexp(1.0, 13 - ilogb(bnd)) : 1.0;
 }
 
-template <int BN, int NRB, bool SINGLE, int KS = 32>
+template <int BN, int NRB, bool SINGLE, int KS = 32, bool A32 = false>
 __global__ __launch_bounds__(256) void k_hgemm_g(const uint16_t *__restrict__ Hhi, const uint16_t *__restrict__ Hlo,
                                                  int ldh, double h_inv, const double *__restrict__ As, int64_t ldA,
                                                  int nlt, int kpad, const double *__restrict__ sc,
                                                  const double *__restrict__ nrm2, int L, int K, int grows,
-                                                 double *__restrict__ G, int64_t ldg)
+                                                 double *__restrict__ G, int64_t ldg,
+                                                 const float *__restrict__ As32 = nullptr)
 {
     constexpr int CBN = BN / 16, WPC = 4 / CBN, NI = NRB / WPC;
     constexpr int NROW = NRB * 16;
@@ -75,6 +78,7 @@ __global__ __launch_bounds__(256) void k_hgemm_g(const uint16_t *__restrict__ Hh
     // ---- slice loads (registers), then LDS
     u16x8 pa[QA];
     double pb[8];
+    float pbf[8];   // A32: the prefetched FP32 values, converted where staged (not waited on at the load)
     auto loadA = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
@@ -93,7 +97,8 @@ __global__ __launch_bounds__(256) void k_hgemm_g(const uint16_t *__restrict__ Hh
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int lt = k0 + 8 * bg + i;
-                pb[i] = lt < nlt ? As[(int64_t)lt * ldA + c0 + bc] : 0.0;
+                if constexpr (A32) pbf[i] = lt < nlt ? As32[(int64_t)lt * ldA + c0 + bc] : 0.f;
+                else pb[i] = lt < nlt ? As[(int64_t)lt * ldA + c0 + bc] : 0.0;
             }
         }
     };
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(256) void k_hgemm_g(const uint16_t *__restrict__ Hh
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 uint16_t hb, lb;
-                f16_split(pb[i] * bsc, hb, lb);
+                f16_split((A32 ? (double)pbf[i] : pb[i]) * bsc, hb, lb);
                 h[i] = hb;
                 l[i] = lb;
             }

@@ -504,6 +504,7 @@ struct tp_batch {
     int nswork = 0;
     double2 *d_cur = nullptr, *d_prop = nullptr;
     double *d_quat = nullptr, *d_As = nullptr, *d_sc = nullptr, *d_isc = nullptr, *d_part = nullptr;
+    float *d_As32 = nullptr;       // FP16X3_T64: As in FP32 [n_lt][ldA] (k_triple_sym_batch -> k_hgemm_g)
     double *d_E = nullptr, *d_Ehist = nullptr, *d_shsc = nullptr;   // d_shsc [ne][K] shell scales
     uint16_t *d_Ahi = nullptr, *d_Alo = nullptr;   // dense F . As (TP_PREC_FP16X3, TP_PREC_BF16)
     double *d_G = nullptr;                         // [16 nrb16][ldA] G = H . As (TP_PREC_FP16X3_T64)
@@ -543,7 +544,7 @@ static void batch_free(tp_batch *B)
     if (B->graph) (void)hipGraphDestroy(B->graph);
     for (auto &e : B->ev)
         if (e) (void)hipEventDestroy(e);
-    void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_swork, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As,
+    void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_swork, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As, B->d_As32,
                     B->d_sc,  B->d_isc, B->d_part, B->d_E,    B->d_Ehist,  B->d_Ahi,    B->d_Alo,
                     (void *)B->d_qtab, B->d_nvalid, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn, B->d_shsc,
                     B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi, B->d_G};
@@ -755,7 +756,8 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         (rc = dev_alloc(&B->d_prop, (size_t)B->ne * B->cstride)) ||
         (sweep && (rc = dev_alloc(&B->d_cur, (size_t)ngroups * B->cstride))) ||
         (rc = dev_alloc(&B->d_quat, (size_t)B->ne * std::max(1, B->nl) * 4)) ||
-        (rc = dev_alloc(&B->d_As, (size_t)b->n_lt * B->ldA)) || (rc = dev_alloc(&B->d_sc, B->ldA)) ||
+        (!fac && (rc = dev_alloc(&B->d_As, (size_t)b->n_lt * B->ldA))) ||
+        (fac && (rc = dev_alloc(&B->d_As32, (size_t)b->n_lt * B->ldA))) || (rc = dev_alloc(&B->d_sc, B->ldA)) ||
         (rc = dev_alloc(&B->d_isc, B->ldA)) || (rc = dev_alloc(&B->d_part, nrt_max * 2 * B->ldA)) ||
         (rc = dev_alloc(&B->d_E, B->ne)) || (rc = dev_alloc(&B->d_Ehist, (size_t)std::max(1, hist_cap) * ngroups)) ||
         (rc = dev_alloc(&B->d_shsc, (size_t)B->ne * b->K)) ||
@@ -778,7 +780,8 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     if (e == hipSuccess) e = hipMemcpy(B->d_qtab, qtab.data(), sizeof(double *) * qtab.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(B->d_nvalid, nvalid.data(), sizeof(int) * nvalid.size(), hipMemcpyHostToDevice);
     // As columns that no evaluation writes (tile padding) must read as zero
-    if (e == hipSuccess) e = hipMemset(B->d_As, 0, sizeof(double) * (size_t)b->n_lt * B->ldA);
+    if (e == hipSuccess && B->d_As) e = hipMemset(B->d_As, 0, sizeof(double) * (size_t)b->n_lt * B->ldA);
+    if (e == hipSuccess && B->d_As32) e = hipMemset(B->d_As32, 0, sizeof(float) * (size_t)b->n_lt * B->ldA);
     if (e == hipSuccess) e = hipEventCreate(&B->ev[0]);
     if (e == hipSuccess) e = hipEventCreate(&B->ev[1]);
     if (e != hipSuccess) {
@@ -847,9 +850,9 @@ static void batch_hgemm_launch(const tp_batch *B, hipStream_t st)
 {
     const tp_basis *b = B->b;
     const unsigned ny = (unsigned)((b->nrb16 + NRB - 1) / NRB);
-    hipLaunchKernelGGL((k_hgemm_g<BN, NRB, false>), dim3((unsigned)(B->ldA / BN), ny), dim3(256), 0, st, b->d_Hhi,
-                       b->d_Hlo, b->ldh16, b->H_inv, B->d_As, B->ldA, (int)b->n_lt, b->ldh16, B->d_sc, nullptr, b->L,
-                       b->K, b->n_m, B->d_G, B->ldA);
+    hipLaunchKernelGGL((k_hgemm_g<BN, NRB, false, 32, true>), dim3((unsigned)(B->ldA / BN), ny), dim3(256), 0, st,
+                       b->d_Hhi, b->d_Hlo, b->ldh16, b->H_inv, nullptr, B->ldA, (int)b->n_lt, b->ldh16, B->d_sc,
+                       nullptr, b->L, b->K, b->n_m, B->d_G, B->ldA, B->d_As32);
 }
 
 // rows (classes) per workgroup and columns of factored candidate cand
@@ -901,7 +904,7 @@ static void batch_triple(const tp_batch *B, hipStream_t st)
         hipLaunchKernelGGL(k_triple_sym_batch, dim3((unsigned)B->nswork), dim3(kSymThreadsBatch), b->sym_lds_b, st, B->d_ev,
                            B->d_swork, B->d_prop, (int64_t)B->cstride, b->n_c, b->d_ew, b->d_eidx, b->d_gstart,
                            b->d_perms, b->d_gorder,
-                           b->d_symtab, b->d_symoff, b->Lw, B->d_As, B->ldA);
+                           b->d_symtab, b->d_symoff, b->Lw, B->d_As32, B->ldA);
     else
         hipLaunchKernelGGL(k_triple16_batch, dim3((unsigned)B->nwork), dim3(512), b->t16_lds, st, B->d_ev, B->d_work,
                            B->d_shsc, b->K, b->n_c, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->Lw, b->EmaxLt8,

@@ -91,14 +91,15 @@ __host__ __device__ constexpr int sym_tsx(int ts, int h1, int h2, int L1, int L2
 // 2j + 1) of pair p at position j ^ ((p >> 1) & 7), so that the stage-1
 // writes (one group, 64 pairs) and the stage-2 reads (16 pairs, 4 groups per
 // lane) both spread over the banks.
-template <int NW, bool G3, bool S16 = false>
+template <int NW, bool G3, bool S16 = false, bool A32 = false>
 __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K,
                                                 const SymItem &it, const double *__restrict__ ew,
                                                 const uint32_t *__restrict__ eidx, const int *__restrict__ gstart,
                                                 const int *__restrict__ perms, const int *__restrict__ gorder,
                                                 const uint32_t *__restrict__ symtab, const int *__restrict__ symoff,
                                                 int Lw, double *__restrict__ As, int64_t ldA, int cbase, int nprop,
-                                                int p0, uint32_t klen, double *__restrict__ nrm2)
+                                                int p0, uint32_t klen, double *__restrict__ nrm2,
+                                                float *__restrict__ As32 = nullptr)
 {
     const int ts = it.tsx & 2047, h1 = (it.tsx >> 11) & 1, h2 = (it.tsx >> 12) & 1;
     const int L1 = (it.tsx >> 13) & 31, L2 = (it.tsx >> 18) & 31, L3 = (it.tsx >> 23) & 31;
@@ -370,7 +371,10 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
             const int o = symoff[t], n = symoff[t + 1] - o;
             for (int x = tid; x < n; x += NT) {
                 const uint32_t ent = symtab[o + x], kt = ent >> 12;
-                if (kt < klen) As[row + as_col(kt, cbase, nprop, p0)] = Tf[ent & 4095u];
+                if (kt < klen) {
+                    if constexpr (A32) As32[row + as_col(kt, cbase, nprop, p0)] = (float)Tf[ent & 4095u];
+                    else As[row + as_col(kt, cbase, nprop, p0)] = Tf[ent & 4095u];
+                }
             }
             if (mirror) {
                 const int tm = (((code >> 24) * 2 + h2) * 2 + h1) * 2 + cb;
@@ -378,7 +382,10 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 for (int x = tid; x < nm; x += NT) {
                     const uint32_t ent = symtab[om + x], kt = ent >> 12, e = ent & 4095u;
                     const uint32_t ex = ((e >> 4) & 15u) * 256u + (e >> 8) * 16u + (e & 15u);
-                    if (kt < klen) As[row + as_col(kt, cbase, nprop, p0)] = Tf[ex];
+                    if (kt < klen) {
+                        if constexpr (A32) As32[row + as_col(kt, cbase, nprop, p0)] = (float)Tf[ex];
+                        else As[row + as_col(kt, cbase, nprop, p0)] = Tf[ex];
+                    }
                 }
             }
         }
@@ -408,7 +415,9 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
 }
 
 // The same on a batch of evaluations (struct EvalDesc): item.e is the
-// evaluation (coefficients at coef + e cstride), item.K its shells; the
+// evaluation (coefficients at coef + e cstride), item.K its shells; As in FP32
+// (its only reader, k_hgemm_g, rounds As x (a power of two) to FP32 before
+// the FP16 split, which gives the same bits from the FP32 value); the
 // tables are those of the batch's basis (K_b shells): the entries of an
 // evaluation of K <= K_b shells are the ones with kt < K (K + 1) (K + 2) / 6,
 // since a sorted k-triple's index does not depend on the shell count.
@@ -423,16 +432,16 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__re
                                                                  const int *__restrict__ gorder,
                                                                  const uint32_t *__restrict__ symtab,
                                                                  const int *__restrict__ symoff, int Lw,
-                                                                 double *__restrict__ As, int64_t ldA)
+                                                                 float *__restrict__ As32, int64_t ldA)
 {
     extern __shared__ double2 sm[];
     const SymItem it = items[blockIdx.x];
     if (it.e < 0) return;       // padding of an XCD queue (whole workgroup, before any barrier)
     const Desc &d = ev[it.e];   // column placement, needed only at the write-out
     const int K = it.K;
-    triple_sym_body<NW, G3, S16>(sm, coef + it.e * cstride, n_c, K, it, ew, eidx, gstart, perms, gorder, symtab,
-                                 symoff, Lw, As, ldA, d.cbase, d.nprop, d.p, (uint32_t)(K * (K + 1) * (K + 2) / 6),
-                                 nullptr);
+    triple_sym_body<NW, G3, S16, true>(sm, coef + it.e * cstride, n_c, K, it, ew, eidx, gstart, perms, gorder,
+                                       symtab, symoff, Lw, nullptr, ldA, d.cbase, d.nprop, d.p,
+                                       (uint32_t)(K * (K + 1) * (K + 2) / 6), nullptr, As32);
 }
 
 // The configuration the library launches.
