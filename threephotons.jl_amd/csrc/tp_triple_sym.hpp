@@ -76,7 +76,10 @@ __host__ __device__ constexpr int sym_tsx(int ts, int h1, int h2, int L1, int L2
 // (lt, kt) of the evaluation at As[lt * ldA + as_col(kt, cbase, nprop, p0)],
 // entries kt >= klen (shells past K) skipped.
 //
-// S16 (the mixed-precision mode TP_PREC_FP16X3_T64): stage 2 on FP16 MFMA.
+// S16 (the mixed-precision mode TP_PREC_FP16X3_T64): stage 1 on FP32 MFMA
+// (v_mfma_f32_16x16x4f32, the FP64 form's operand shape at twice the FP64
+// rate; c_K1, c_K2 and the weights staged in FP32, T accumulated in FP32,
+// rows 4 (lane / 16) + v), stage 2 on FP16 MFMA.
 // T (FP64, from stage 1) and c_K3 are scaled by powers of two from the
 // l-block norms of the work item's own shells, r_a(K1) = 2^-(1 + ilogb |c_K1,L1|)
 // and so on, so that |T| r_a r_b < 1 (the Clebsch-Gordan map is unitary:
@@ -125,20 +128,27 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     const int l16 = lane & 15, lq = lane >> 4;
     const int co1 = L1 * (L1 - 1) / 2, co2 = L2 * (L2 - 1) / 2, co3 = L3 * (L3 - 1) / 2;   // compact offsets (even l)
     const int K1b = 16 * h1, K2b = 16 * h2;
+    float2 *Abf = reinterpret_cast<float2 *>(Ab), *Bbf = reinterpret_cast<float2 *>(Bb);   // S16: FP32 copies
+    float *Wsf = reinterpret_cast<float *>(Ws);
     for (int x = tid; x < 16 * n1; x += NT) {
         const int m = x >> 4, i = x & 15, k = K1b + i;
-        Ab[x] = k < K ? coef[(size_t)k * n_c + co1 + m] : make_double2(0.0, 0.0);
+        const double2 c = k < K ? coef[(size_t)k * n_c + co1 + m] : make_double2(0.0, 0.0);
+        if constexpr (S16) Abf[x] = make_float2((float)c.x, (float)c.y);
+        else Ab[x] = c;
     }
     for (int x = tid; x < 16 * n2; x += NT) {
         const int m = x >> 4, j = x & 15, k = K2b + j;
-        Bb[x] = k < K ? coef[(size_t)k * n_c + co2 + m] : make_double2(0.0, 0.0);
+        const double2 c = k < K ? coef[(size_t)k * n_c + co2 + m] : make_double2(0.0, 0.0);
+        if constexpr (S16) Bbf[x] = make_float2((float)c.x, (float)c.y);
+        else Bb[x] = c;
     }
     for (int x = tid; x < K * n3; x += NT) {
         const int m = x / K, k = x - m * K;
         Cb[x] = coef[(size_t)k * n_c + co3 + m];
     }
     for (int x = tid; x < ne; x += NT) {
-        Ws[x] = ew[e0 + x];
+        if constexpr (S16) Wsf[x] = (float)ew[e0 + x];
+        else Ws[x] = ew[e0 + x];
         const uint32_t id = eidx[e0 + x];
         Is[x] = (uint32_t)((int)(id & 0xffffu) - co1) | ((uint32_t)((int)(id >> 16) - co2) << 16);
     }
@@ -168,12 +178,16 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
         // TPC consecutive lanes per block, each summing every TPC-th m
         constexpr int TPC = NT / 64;
         const int col = tid / TPC, sub = tid % TPC;
-        const double2 *base = col < 16 ? Ab + col : col < 32 ? Bb + (col - 16) : Cb + (col - 32);
-        const int n = col < 16 ? n1 : col < 32 ? n2 : n3, ld = col < 32 ? 16 : K;
+        const int n = col < 16 ? n1 : col < 32 ? n2 : n3;
         double s2 = 0.0;
-        if (col < 32 + K)
+        if (col < 32)
             for (int m = sub; m < n; m += TPC) {
-                const double2 c = base[m * ld];
+                const float2 c = col < 16 ? Abf[m * 16 + col] : Bbf[m * 16 + col - 16];
+                s2 += (double)c.x * c.x + (double)c.y * c.y;
+            }
+        else if (col < 32 + K)
+            for (int m = sub; m < n; m += TPC) {
+                const double2 c = Cb[m * K + col - 32];
                 s2 += c.x * c.x + c.y * c.y;
             }
 #pragma unroll
@@ -222,7 +236,48 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
         // complex product as P1 = (w Re a) Re b, P2 = (w Im a) Im b,
         // P3 = (w (Re a + Im a)) (Re b + Im b): Tr = P1 - P2, Ti = P3 - P1 - P2;
         // otherwise Tr += (w Re a) Re b - (w Im a) Im b, Ti += (w Re a) Im b + (w Im a) Re b
-        {
+        if constexpr (S16) {
+            // FP32: rows K1 = 4 lq + v of the accumulators
+            f32x4 q1 = f32x4{0.f, 0.f, 0.f, 0.f}, q2 = q1, q3 = q1;
+            if (gi < nu) {
+                const int u = Od[c0 + gi];
+                const int s = Gs[u], E = Gs[u + 1] - s;
+                const int nks = (E + 3) >> 2;
+#pragma unroll 2
+                for (int ks = 0; ks < nks; ++ks) {
+                    const int e = 4 * ks + lq;
+                    float ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
+                    if (e < E) {
+                        const float w = Wsf[s + e];
+                        const uint32_t id = Is[s + e];
+                        const float2 ca = Abf[(int)(id & 0xffffu) * 16 + l16];
+                        const float2 cbv = Bbf[(int)(id >> 16) * 16 + l16];
+                        ar = w * ca.x;
+                        ai = w * ca.y;
+                        br = cbv.x;
+                        bi = cbv.y;
+                    }
+                    q1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar, br, q1, 0, 0, 0);
+                    q2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ai, bi, q2, 0, 0, 0);
+                    q3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar + ai, br + bi, q3, 0, 0, 0);
+                }
+            }
+            // pair p = 16 (4 lq + v) + l16; (p >> 1) & 7 = (l16 >> 1) & 7
+            const float sbv = (float)(s_rs[16 + l16] * 8192.0);
+            const int gpos = 2 * ((gi >> 1) ^ ((l16 >> 1) & 7)) + (gi & 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int p = 16 * (4 * lq + v) + l16;
+                const float sc = (float)s_rs[4 * lq + v] * sbv;
+                const float tr = (q1[v] - q2[v]) * sc, ti = (q3[v] - q1[v] - q2[v]) * sc;
+                const _Float16 hr = (_Float16)tr, hi = (_Float16)ti;
+                const _Float16 lr = (_Float16)(tr - (float)hr), li = (_Float16)(ti - (float)hi);
+                T2[p * 16 + gpos] = make_uint2((uint32_t)__builtin_bit_cast(uint16_t, hr) |
+                                                   (uint32_t)__builtin_bit_cast(uint16_t, hi) << 16,
+                                               (uint32_t)__builtin_bit_cast(uint16_t, lr) |
+                                                   (uint32_t)__builtin_bit_cast(uint16_t, li) << 16);
+            }
+        } else {
             dbl4 p1 = dbl4{0.0, 0.0, 0.0, 0.0}, p2 = dbl4{0.0, 0.0, 0.0, 0.0}, p3 = dbl4{0.0, 0.0, 0.0, 0.0};
             if (gi < nu) {
                 const int u = Od[c0 + gi];
@@ -255,25 +310,11 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 }
             }
             // rows K1 = lq + 4 v, columns K2 = l16: pair p = 16 row + column
-            if constexpr (S16) {
-                const double sbv = s_rs[16 + l16] * 8192.0;
-                const int gpos = 2 * ((gi >> 1) ^ ((l16 >> 1) & 7)) + (gi & 1);   // (p >> 1) & 7 = (l16 >> 1) & 7
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int p = 16 * (lq + 4 * v) + l16;
-                    const double sc = s_rs[lq + 4 * v] * sbv;
-                    uint16_t hr, lr, hi, li;
-                    f16_split((G3 ? p1[v] - p2[v] : p1[v]) * sc, hr, lr);
-                    f16_split((G3 ? p3[v] - p1[v] - p2[v] : p2[v]) * sc, hi, li);
-                    T2[p * 16 + gpos] = make_uint2(hr | (uint32_t)hi << 16, lr | (uint32_t)li << 16);
-                }
-            } else {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int p = 16 * (lq + 4 * v) + l16;
-                    Tf[(2 * gi) * 256 + p] = G3 ? p1[v] - p2[v] : p1[v];
-                    Tf[(2 * gi + 1) * 256 + p] = G3 ? p3[v] - p1[v] - p2[v] : p2[v];
-                }
+            for (int v = 0; v < 4; ++v) {
+                const int p = 16 * (lq + 4 * v) + l16;
+                Tf[(2 * gi) * 256 + p] = G3 ? p1[v] - p2[v] : p1[v];
+                Tf[(2 * gi + 1) * 256 + p] = G3 ? p3[v] - p1[v] - p2[v] : p2[v];
             }
         }
         }
