@@ -148,7 +148,7 @@ struct tp_basis {
     // sorted l-triples (k_triple_sym; the library's own weights, K <= 32)
     bool use_sym = false;
     int nsym_work = 0;
-    size_t sym_lds = 0, sym_lds_b = 0;   // dynamic LDS of k_triple_sym, k_triple_sym_batch
+    size_t sym_lds = 0, sym16_lds = 0, sym_lds_b = 0;   // dynamic LDS of k_triple_sym, k_triple_sym16, k_triple_sym_batch
     SymItem *d_sym_work = nullptr; // [nsym_work] work items, most expensive first
     int *d_perms = nullptr;        // [n_lt][8] orderings of each sorted l-triple
     int *d_gorder = nullptr;       // [n_groups] run order of the mu3 groups of sorted l-triples
@@ -313,7 +313,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
                            b->EmaxLt8, b->Prows, sc.As, b->kl_pad);
     } else if (b->use_sym) {   // FP64; TP_PREC_FP16X3_T64 with its stage 2 on FP16 MFMA
         if (prec == TP_PREC_FP16X3_T64)
-            hipLaunchKernelGGL(k_triple_sym16, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1,
+            hipLaunchKernelGGL(k_triple_sym16, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym16_lds, st, b0, b1,
                                sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_sym_work,
                                b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad, sc.nrm2);
         else
@@ -1807,7 +1807,8 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         int emax_lt = 0;
         for (int t = 0; t < nlt; ++t) emax_lt = std::max(emax_lt, lt_entries[t]);
         const size_t lds = triple_sym_lds(K, 2 * L + 1, emax_lt);
-        const size_t lds_b = triple_sym_lds(K, 2 * L + 1, emax_lt, kSymWavesBatch);
+        const size_t lds16 = triple_sym_lds(K, 2 * L + 1, emax_lt, kSymWaves, true);
+        const size_t lds_b = triple_sym_lds(K, 2 * L + 1, emax_lt, kSymWavesBatch, true);
         int maxlds = 0;
         if (hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device) != hipSuccess)
             maxlds = 65536;
@@ -1878,14 +1879,15 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             symoff.push_back((int)symtab.size());
             b->nsym_work = (int)sym_work.size();
             b->sym_lds = lds;
+            b->sym16_lds = lds16;
             b->sym_lds_b = lds_b;
             b->use_sym = true;
-            if (lds > 65536) {
+            if (lds > 65536)
                 (void)hipFuncSetAttribute((const void *)k_triple_sym, hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)lds);
+            if (lds16 > 65536)
                 (void)hipFuncSetAttribute((const void *)k_triple_sym16, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)lds);
-            }
+                                          (int)lds16);
             if (lds_b > 65536)
                 (void)hipFuncSetAttribute((const void *)k_triple_sym_batch,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_b);

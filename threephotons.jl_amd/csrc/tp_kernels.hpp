@@ -675,6 +675,8 @@ typedef __bf16 b16x8 __attribute__((ext_vector_type(8)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h16x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint16_t f16_bits(double x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 __device__ __forceinline__ double f16_val(uint16_t b) { return (double)__builtin_bit_cast(_Float16, b); }

@@ -29,6 +29,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
 
 constexpr int kSymWaves = 16;                // waves per workgroup: also the mu3 groups of a stage-1 pass
@@ -43,13 +44,18 @@ constexpr bool kSymGauss = true;             // stage 1 with three real products
 // (L_sigma(0), L_sigma(1), L_sigma(2)) of the sorted one.
 constexpr int kSymPerm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
-// Dynamic LDS of k_triple_sym for order bound Lw = 2L + 1, K shells and at most
-// Emax entries per l-triple.
-static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves)
+// Dynamic LDS of the sorted-triple kernels for order bound Lw = 2L + 1, K
+// shells and at most Emax entries per l-triple: FP64 staging, or (s16, the
+// mixed mode) FP32 staging with T chunks of NW groups and V staged in FP32.
+static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves, bool s16 = false)
 {
-    return 16 * sizeof(double2) * 2 * (size_t)Lw + sizeof(double2) * (size_t)K * Lw +
-           (sizeof(double) + sizeof(uint32_t)) * (size_t)Emax + sizeof(int) * (2 * (size_t)Lw + 2) +
-           sizeof(double) * 2 * NW * 256 + 64;
+    if (!s16)
+        return 16 * sizeof(double2) * 2 * (size_t)Lw + sizeof(double2) * (size_t)K * Lw +
+               (sizeof(double) + sizeof(uint32_t)) * (size_t)Emax + sizeof(int) * (2 * (size_t)Lw + 2) +
+               sizeof(double) * 2 * NW * 256 + 64;
+    const size_t tf = std::max((size_t)NW * 256 * sizeof(uint2), (size_t)256 * 16 * sizeof(float));
+    return 16 * sizeof(float2) * 2 * (size_t)Lw + sizeof(float2) * (size_t)K * Lw +
+           (sizeof(float) + sizeof(uint32_t)) * (size_t)Emax + sizeof(int) * (2 * (size_t)Lw + 2) + tf + 64;
 }
 
 // A work item: everything its operand loads need, in one 32-byte record, so
@@ -85,15 +91,17 @@ __host__ __device__ constexpr int sym_tsx(int ts, int h1, int h2, int L1, int L2
 // and so on, so that |T| r_a r_b < 1 (the Clebsch-Gordan map is unitary:
 // |T(K1,K2,mu3)| <= |c_K1,L1| |c_K2,L2|) and |c_K3| r_c < 1; both are taken
 // times 2^13, split into FP16 hi + lo (22 significant bits), and V
-// accumulates hi.hi, hi.lo and lo.hi on v_mfma_f32_16x16x32_f16 in FP32, one
-// K step per chunk of 16 mu3 groups (32 slots: Re T and Im T).  V is
-// unscaled exactly (powers of two) on its way to As.  LDS: c_K3 split once per
-// item, in place of Cb ({hi Re | hi -Im << 16, lo Re | lo -Im << 16} per
-// entry); T of a chunk as 8-byte {hi Re | hi Im << 16, lo Re | lo Im << 16}
-// per (pair, group), 128 bytes per pair, the 16-byte pairs of groups (2j,
-// 2j + 1) of pair p at position j ^ ((p >> 1) & 7), so that the stage-1
-// writes (one group, 64 pairs) and the stage-2 reads (16 pairs, 4 groups per
-// lane) both spread over the banks.
+// accumulates hi.hi, hi.lo and lo.hi in FP32 on v_mfma_f32_16x16x16f16, one
+// K step per 8 mu3 groups (16 slots: Re T and Im T) of a chunk of NW groups
+// (one per wave).  V is unscaled exactly (powers
+// of two) and staged in FP32 on its way to As.  LDS: c_K3 staged in FP32 and
+// split in place ({hi Re | hi -Im << 16, lo Re | lo -Im << 16} per entry);
+// T of a chunk as 8-byte {hi Re | hi Im << 16, lo Re | lo Im << 16} per
+// (pair, group), the 16-byte pairs of groups (2j, 2j + 1) of pair p at
+// position j ^ ((p >> SH) & (NW/2 - 1)) (SH = 1 for 16 groups, 2 for 8), so
+// that the stage-1 writes (one group, 64 pairs) and the stage-2 reads (16
+// pairs per lane group) both spread over the banks.  About 42 KB at the full
+// size for 8 waves: three workgroups per CU.
 template <int NW, bool G3, bool S16 = false, bool A32 = false>
 __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K,
                                                 const SymItem &it, const double *__restrict__ ew,
@@ -108,28 +116,38 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     const int L1 = (it.tsx >> 13) & 31, L2 = (it.tsx >> 18) & 31, L3 = (it.tsx >> 23) & 31;
     const int n1 = 2 * L1 + 1, n2 = 2 * L2 + 1, n3 = 2 * L3 + 1;
     const int goff = it.goff;
-    // LDS carve-up (sizes from the order bound Lw so that the layout is fixed)
-    double2 *Ab = sm;                                      // [n1][16]  c_K1[L1, m1], K1 = 16 h1 + i + 1
-    double2 *Bb = Ab + 16 * (size_t)Lw;                    // [n2][16]  c_K2[L2, m2]
-    double2 *Cb = Bb + 16 * (size_t)Lw;                    // [n3][K]   c_K3[L3, mu3]
-    double *Ws = reinterpret_cast<double *>(Cb + (size_t)K * Lw);   // [E] weights
+    // LDS carve-up (sizes from the order bound Lw so that the layout is fixed;
+    // pointers formed on sm itself, no integer round trip, so that their
+    // accesses stay LDS accesses instead of flat ones).  FP64: Ab, Bb, Cb
+    // double2 and Ws double; S16: their FP32 copies (Cb then split in place).
+    constexpr size_t CE = S16 ? sizeof(float2) : sizeof(double2), WE = S16 ? sizeof(float) : sizeof(double);
+    char *const pA = reinterpret_cast<char *>(sm);
+    char *const pB = pA + 16 * CE * (size_t)Lw;
+    char *const pC = pB + 16 * CE * (size_t)Lw;
+    char *const pW = pC + CE * (size_t)K * Lw;
+    double2 *Ab = reinterpret_cast<double2 *>(pA);         // [n1][16]  c_K1[L1, m1], K1 = 16 h1 + i + 1
+    double2 *Bb = reinterpret_cast<double2 *>(pB);         // [n2][16]  c_K2[L2, m2]
+    double2 *Cb = reinterpret_cast<double2 *>(pC);         // [n3][K]   c_K3[L3, mu3]
+    double *Ws = reinterpret_cast<double *>(pW);           // [E] weights
+    float2 *Abf = reinterpret_cast<float2 *>(pA), *Bbf = reinterpret_cast<float2 *>(pB);   // S16
+    float2 *Cf = reinterpret_cast<float2 *>(pC);
+    uint2 *Cu = reinterpret_cast<uint2 *>(pC);             // S16: c_K3 split, [n3][K]
+    float *Wsf = reinterpret_cast<float *>(pW);
     const int e0 = it.e0, ne = it.ne;
-    uint32_t *Is = reinterpret_cast<uint32_t *>(Ws + ne);           // [E] local (m1 index | m2 index << 16)
-    int *Gs = reinterpret_cast<int *>(Is + ne);                     // [n3 + 1] group starts
-    int *Od = Gs + (n3 + 1);                                        // [n3] run order of the groups
-    // [2 NW][256]; V staging later.  Its 16-byte aligned offset is formed on
-    // sm itself (no integer round trip), so its accesses stay LDS accesses
-    // instead of flat ones.
-    const size_t tf_off = (reinterpret_cast<const char *>(Od + n3) - reinterpret_cast<const char *>(sm) + 15) & ~(size_t)15;
-    double *Tf = reinterpret_cast<double *>(reinterpret_cast<char *>(sm) + tf_off);
+    uint32_t *Is = reinterpret_cast<uint32_t *>(pW + WE * (size_t)ne);   // [E] local (m1 index | m2 index << 16)
+    int *Gs = reinterpret_cast<int *>(Is + ne);                          // [n3 + 1] group starts
+    int *Od = Gs + (n3 + 1);                                             // [n3] run order of the groups
+    // FP64: [2 NW][256] T slots, then the V staging; S16: T chunk, then the V staging in FP32
+    const size_t tf_off = (reinterpret_cast<const char *>(Od + n3) - pA + 15) & ~(size_t)15;
+    double *Tf = reinterpret_cast<double *>(pA + tf_off);
+    uint2 *T2 = reinterpret_cast<uint2 *>(pA + tf_off);
+    float *Tff = reinterpret_cast<float *>(pA + tf_off);
     constexpr int NT = 64 * NW, NR = 16 / NW;   // threads; stage-2 row blocks per wave
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int l16 = lane & 15, lq = lane >> 4;
     const int co1 = L1 * (L1 - 1) / 2, co2 = L2 * (L2 - 1) / 2, co3 = L3 * (L3 - 1) / 2;   // compact offsets (even l)
     const int K1b = 16 * h1, K2b = 16 * h2;
-    float2 *Abf = reinterpret_cast<float2 *>(Ab), *Bbf = reinterpret_cast<float2 *>(Bb);   // S16: FP32 copies
-    float *Wsf = reinterpret_cast<float *>(Ws);
     for (int x = tid; x < 16 * n1; x += NT) {
         const int m = x >> 4, i = x & 15, k = K1b + i;
         const double2 c = k < K ? coef[(size_t)k * n_c + co1 + m] : make_double2(0.0, 0.0);
@@ -144,7 +162,9 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     }
     for (int x = tid; x < K * n3; x += NT) {
         const int m = x / K, k = x - m * K;
-        Cb[x] = coef[(size_t)k * n_c + co3 + m];
+        const double2 c = coef[(size_t)k * n_c + co3 + m];
+        if constexpr (S16) Cf[x] = make_float2((float)c.x, (float)c.y);
+        else Cb[x] = c;
     }
     for (int x = tid; x < ne; x += NT) {
         if constexpr (S16) Wsf[x] = (float)ew[e0 + x];
@@ -157,12 +177,12 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     __syncthreads();
     // the work item (l, l, l) with h1 = h2 = 0 holds the l-blocks of every
     // shell: their squared norms, nrm2[k][l/2] (column scales of the FP16
-    // contraction)
+    // contraction), from the FP64 coefficients
     if (nrm2 && h1 == 0 && h2 == 0 && L1 == L3 && L2 == L3)
         for (int k = tid; k < K; k += NT) {
             double s2 = 0.0;
             for (int m = 0; m < n3; ++m) {
-                const double2 c = Cb[m * K + k];
+                const double2 c = S16 ? coef[(size_t)k * n_c + co3 + m] : Cb[m * K + k];
                 s2 += c.x * c.x + c.y * c.y;
             }
             nrm2[k * kNrmLd + L3 / 2] = s2;
@@ -171,8 +191,6 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     const int ncb = K > 16 ? 2 : 1;                 // K3 column blocks
     // S16: the work item's scales r_a (K1 = K1b + i), r_b (K2 = K2b + j), r_c (K3), and their inverses
     __shared__ double s_rs[S16 ? 64 : 1], s_ri[S16 ? 64 : 1];
-    uint2 *Cu = reinterpret_cast<uint2 *>(Cb);   // S16: c_K3 split, [n3][K]
-    uint2 *T2 = reinterpret_cast<uint2 *>(Tf);   // S16: T of a chunk, [256 pairs][16 groups]
     if constexpr (S16) {
         // squared norms of the 32 + K shell blocks (K1 block, K2 block, K3),
         // TPC consecutive lanes per block, each summing every TPC-th m
@@ -180,15 +198,10 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
         const int col = tid / TPC, sub = tid % TPC;
         const int n = col < 16 ? n1 : col < 32 ? n2 : n3;
         double s2 = 0.0;
-        if (col < 32)
+        if (col < 32 + K)
             for (int m = sub; m < n; m += TPC) {
-                const float2 c = col < 16 ? Abf[m * 16 + col] : Bbf[m * 16 + col - 16];
+                const float2 c = col < 16 ? Abf[m * 16 + col] : col < 32 ? Bbf[m * 16 + col - 16] : Cf[m * K + col - 32];
                 s2 += (double)c.x * c.x + (double)c.y * c.y;
-            }
-        else if (col < 32 + K)
-            for (int m = sub; m < n; m += TPC) {
-                const double2 c = Cb[m * K + col - 32];
-                s2 += c.x * c.x + c.y * c.y;
             }
 #pragma unroll
         for (int o = 1; o < TPC; o <<= 1) s2 += __shfl_xor(s2, o);
@@ -198,22 +211,14 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
             s_ri[col] = ldexp(1.0, ex);
         }
         __syncthreads();
-        // c_K3 (r_c 2^13) split in place: entry x's 8 bytes overwrite half of
-        // double2 entry x / 2, read in this round or an earlier one
-        const int nck = n3 * K;
-        for (int x0 = 0; x0 < nck; x0 += NT) {
-            const int x = x0 + tid;
-            uint2 u = make_uint2(0u, 0u);
-            if (x < nck) {
-                const double2 c = Cb[x];
-                const double sc = s_rs[32 + x % K] * 8192.0;
-                uint16_t hr, lr, hi, li;
-                f16_split(c.x * sc, hr, lr);
-                f16_split(-c.y * sc, hi, li);
-                u = make_uint2(hr | (uint32_t)hi << 16, lr | (uint32_t)li << 16);
-            }
-            __syncthreads();
-            if (x < nck) Cu[x] = u;
+        // c_K3 (r_c 2^13) split in place, entry by entry (the same 8 bytes)
+        for (int x = tid; x < n3 * K; x += NT) {
+            const float2 c = Cf[x];
+            const double sc = s_rs[32 + x % K] * 8192.0;
+            uint16_t hr, lr, hi, li;
+            f16_split((double)c.x * sc, hr, lr);
+            f16_split(-(double)c.y * sc, hi, li);
+            Cu[x] = make_uint2(hr | (uint32_t)hi << 16, lr | (uint32_t)li << 16);
         }
         __syncthreads();
     }
@@ -227,11 +232,13 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
             a16[r][c] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
 
-    constexpr int CH = S16 ? 16 : NW;   // mu3 groups per chunk
-    for (int c0 = 0; c0 < n3; c0 += CH) {
-        const int nu = min(CH, n3 - c0);
-        for (int r1 = 0; r1 < CH / NW; ++r1) {
-        const int gi = wave + NW * r1;   // this wave's group within the chunk
+    // S16: the T chunk's swizzle, pair p's 16-byte pair of groups j at j ^ ((p >> SH) & FM)
+    constexpr int SH = NW == 16 ? 1 : 2, FM = NW / 2 - 1;
+    static_assert(!S16 || NW == 16 || NW == 8, "the FP16 stage 2 takes 16 or 8 groups per chunk");
+    const int sw = (l16 >> SH) & FM;   // (p >> SH) & FM for the pairs p = 16 x + l16 a lane touches
+    for (int c0 = 0; c0 < n3; c0 += NW) {
+        const int nu = min(NW, n3 - c0);
+        const int gi = wave;   // this wave's group within the chunk
         // ---- stage 1: this wave's group, k-step = 4 entries; with G3 the
         // complex product as P1 = (w Re a) Re b, P2 = (w Im a) Im b,
         // P3 = (w (Re a + Im a)) (Re b + Im b): Tr = P1 - P2, Ti = P3 - P1 - P2;
@@ -262,9 +269,9 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                     q3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ar + ai, br + bi, q3, 0, 0, 0);
                 }
             }
-            // pair p = 16 (4 lq + v) + l16; (p >> 1) & 7 = (l16 >> 1) & 7
+            // pair p = 16 (4 lq + v) + l16
             const float sbv = (float)(s_rs[16 + l16] * 8192.0);
-            const int gpos = 2 * ((gi >> 1) ^ ((l16 >> 1) & 7)) + (gi & 1);
+            const int gpos = 2 * ((gi >> 1) ^ sw) + (gi & 1);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int p = 16 * (4 * lq + v) + l16;
@@ -272,7 +279,7 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 const float tr = (q1[v] - q2[v]) * sc, ti = (q3[v] - q1[v] - q2[v]) * sc;
                 const _Float16 hr = (_Float16)tr, hi = (_Float16)ti;
                 const _Float16 lr = (_Float16)(tr - (float)hr), li = (_Float16)(ti - (float)hi);
-                T2[p * 16 + gpos] = make_uint2((uint32_t)__builtin_bit_cast(uint16_t, hr) |
+                T2[p * NW + gpos] = make_uint2((uint32_t)__builtin_bit_cast(uint16_t, hr) |
                                                    (uint32_t)__builtin_bit_cast(uint16_t, hi) << 16,
                                                (uint32_t)__builtin_bit_cast(uint16_t, lr) |
                                                    (uint32_t)__builtin_bit_cast(uint16_t, li) << 16);
@@ -317,36 +324,40 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 Tf[(2 * gi + 1) * 256 + p] = G3 ? p3[v] - p1[v] - p2[v] : p2[v];
             }
         }
-        }
         __syncthreads();
         if constexpr (S16) {
-            // ---- stage 2 on FP16 MFMA: the chunk's 32 slots are one K step,
-            // lane (l16, lq) holding slots 8 lq .. 8 lq + 7 = (Re, Im) of groups
-            // 4 lq .. 4 lq + 3: A = T of pair 16 rb + l16, B = [Re c_K3 ; -Im c_K3]
-            const int sw = (l16 >> 1) & 7;
+            // ---- stage 2 on FP16 MFMA (v_mfma_f32_16x16x16f16): each 8 groups of
+            // the chunk (16 slots) are one K step, lane (l16, lq) holding slots
+            // 4 lq .. 4 lq + 3 = (Re, Im) of groups 2 lq, 2 lq + 1 of the 8; B =
+            // [Re c_K3 ; -Im c_K3] of those groups, A = T of pair 16 rb + l16.
+            // With 16 waves the two halves of a chunk run in this order, so every
+            // accumulator sees the same MFMA sequence as with 8 waves (batches
+            // give the energies of single evaluations bit for bit).
 #pragma unroll
             for (int cb = 0; cb < 2; ++cb) {
                 if (cb >= ncb) break;
                 const int k3 = 16 * cb + l16;
-                u32x4 bh, bl;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int g = 4 * lq + q;
-                    const uint2 u = g < nu && k3 < K ? Cu[Od[c0 + g] * K + k3] : make_uint2(0u, 0u);
-                    bh[q] = u.x;
-                    bl[q] = u.y;
-                }
-                const h16x8 hbh = __builtin_bit_cast(h16x8, bh), hbl = __builtin_bit_cast(h16x8, bl);
+                for (int hh = 0; hh < NW / 8; ++hh) {
+                    u32x2 bh, bl;
 #pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    const uint2 *trow = T2 + (16 * (wave + NW * r) + l16) * 16;
-                    const uint4 q1 = *reinterpret_cast<const uint4 *>(trow + 2 * ((2 * lq) ^ sw));
-                    const uint4 q2 = *reinterpret_cast<const uint4 *>(trow + 2 * ((2 * lq + 1) ^ sw));
-                    const h16x8 ah = __builtin_bit_cast(h16x8, u32x4{q1.x, q1.z, q2.x, q2.z});
-                    const h16x8 al = __builtin_bit_cast(h16x8, u32x4{q1.y, q1.w, q2.y, q2.w});
-                    a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, hbh, a16[r][cb], 0, 0, 0);
-                    a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, hbl, a16[r][cb], 0, 0, 0);
-                    a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, hbh, a16[r][cb], 0, 0, 0);
+                    for (int q = 0; q < 2; ++q) {
+                        const int g = 8 * hh + 2 * lq + q;
+                        const uint2 u = g < nu && k3 < K ? Cu[Od[c0 + g] * K + k3] : make_uint2(0u, 0u);
+                        bh[q] = u.x;
+                        bl[q] = u.y;
+                    }
+                    const h16x4 hbh = __builtin_bit_cast(h16x4, bh), hbl = __builtin_bit_cast(h16x4, bl);
+#pragma unroll
+                    for (int r = 0; r < NR; ++r) {
+                        const uint2 *trow = T2 + (16 * (wave + NW * r) + l16) * NW;
+                        const uint4 q1 = *reinterpret_cast<const uint4 *>(trow + 2 * ((4 * hh + lq) ^ sw));
+                        const h16x4 ah = __builtin_bit_cast(h16x4, u32x2{q1.x, q1.z});
+                        const h16x4 al = __builtin_bit_cast(h16x4, u32x2{q1.y, q1.w});
+                        a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x16f16(al, hbh, a16[r][cb], 0, 0, 0);
+                        a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, hbl, a16[r][cb], 0, 0, 0);
+                        a16[r][cb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ah, hbh, a16[r][cb], 0, 0, 0);
+                    }
                 }
             }
             __syncthreads();
@@ -388,9 +399,15 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     // local index.
     const int np = perms[ts * 8];
     const bool mirror = L1 == L2 && h1 > h2;
+    auto vstore = [&](int64_t row, uint32_t kt, uint32_t idx) {
+        const int64_t col = as_col(kt, cbase, nprop, p0);
+        if constexpr (!S16) As[row + col] = Tf[idx];
+        else if constexpr (A32) As32[row + col] = Tff[idx];
+        else As[row + col] = (double)Tff[idx];
+    };
     for (int cb = 0; cb < ncb; ++cb) {
         // V tile of K3 block cb: Vs[pair][k3 - 16 cb] (pairs 16 rb + lq + 4 v, k3 = l16;
-        // S16: pairs 16 rb + 4 lq + v, unscaled by 2^-26 / (r_a r_b r_c))
+        // S16: pairs 16 rb + 4 lq + v, unscaled by 2^-26 / (r_a r_b r_c), in FP32)
 #pragma unroll
         for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -399,7 +416,7 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                     const int p = 16 * (wave + NW * r) + 4 * lq + v, k3 = 16 * cb + l16;
                     const double un =
                         k3 < K ? s_ri[p >> 4] * s_ri[16 + (p & 15)] * s_ri[32 + k3] * 0x1p-26 : 0.0;
-                    Tf[p * 16 + l16] = (double)a16[r][cb][v] * un;
+                    Tff[p * 16 + l16] = (float)((double)a16[r][cb][v] * un);
                 } else {
                     Tf[(16 * (wave + NW * r) + lq + 4 * v) * 16 + l16] = acc[r][cb][v];
                 }
@@ -412,10 +429,7 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
             const int o = symoff[t], n = symoff[t + 1] - o;
             for (int x = tid; x < n; x += NT) {
                 const uint32_t ent = symtab[o + x], kt = ent >> 12;
-                if (kt < klen) {
-                    if constexpr (A32) As32[row + as_col(kt, cbase, nprop, p0)] = (float)Tf[ent & 4095u];
-                    else As[row + as_col(kt, cbase, nprop, p0)] = Tf[ent & 4095u];
-                }
+                if (kt < klen) vstore(row, kt, ent & 4095u);
             }
             if (mirror) {
                 const int tm = (((code >> 24) * 2 + h2) * 2 + h1) * 2 + cb;
@@ -423,10 +437,7 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 for (int x = tid; x < nm; x += NT) {
                     const uint32_t ent = symtab[om + x], kt = ent >> 12, e = ent & 4095u;
                     const uint32_t ex = ((e >> 4) & 15u) * 256u + (e >> 8) * 16u + (e & 15u);
-                    if (kt < klen) {
-                        if constexpr (A32) As32[row + as_col(kt, cbase, nprop, p0)] = (float)Tf[ex];
-                        else As[row + as_col(kt, cbase, nprop, p0)] = Tf[ex];
-                    }
+                    if (kt < klen) vstore(row, kt, ex);
                 }
             }
         }
