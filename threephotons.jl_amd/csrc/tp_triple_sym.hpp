@@ -55,7 +55,7 @@ static inline size_t triple_sym_lds(int K, int Lw, int Emax, int NW = kSymWaves,
                sizeof(double) * 2 * NW * 256 + 64;
     const size_t tf = std::max((size_t)NW * 256 * sizeof(uint2), (size_t)256 * 16 * sizeof(float));
     return 16 * sizeof(float2) * 2 * (size_t)Lw + sizeof(float2) * (size_t)K * Lw +
-           (sizeof(float) + sizeof(uint32_t)) * (size_t)Emax + sizeof(int) * (2 * (size_t)Lw + 2) + tf + 64;
+           (sizeof(float) + sizeof(uint16_t)) * (size_t)Emax + sizeof(int) * (2 * (size_t)Lw + 2) + tf + 32;
 }
 
 // A work item: everything its operand loads need, in one 32-byte record, so
@@ -100,8 +100,9 @@ __host__ __device__ constexpr int sym_tsx(int ts, int h1, int h2, int L1, int L2
 // (pair, group), the 16-byte pairs of groups (2j, 2j + 1) of pair p at
 // position j ^ ((p >> SH) & (NW/2 - 1)) (SH = 1 for 16 groups, 2 for 8), so
 // that the stage-1 writes (one group, 64 pairs) and the stage-2 reads (16
-// pairs per lane group) both spread over the banks.  About 42 KB at the full
-// size for 8 waves: three workgroups per CU.
+// pairs per lane group) both spread over the banks; an entry's local m1 and
+// m2 indices in 2 bytes, the item's scales as exponents.  About 40 KB at the
+// full size for 8 waves: four workgroups per CU.
 template <int NW, bool G3, bool S16 = false, bool A32 = false>
 __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__restrict__ coef, int n_c, int K,
                                                 const SymItem &it, const double *__restrict__ ew,
@@ -135,7 +136,9 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
     float *Wsf = reinterpret_cast<float *>(pW);
     const int e0 = it.e0, ne = it.ne;
     uint32_t *Is = reinterpret_cast<uint32_t *>(pW + WE * (size_t)ne);   // [E] local (m1 index | m2 index << 16)
-    int *Gs = reinterpret_cast<int *>(Is + ne);                          // [n3 + 1] group starts
+    uint16_t *Is16 = reinterpret_cast<uint16_t *>(pW + WE * (size_t)ne); // S16: [E] m1 index | m2 index << 8
+    constexpr size_t IE = S16 ? sizeof(uint16_t) : sizeof(uint32_t);
+    int *Gs = reinterpret_cast<int *>(pW + (WE + IE) * (size_t)ne + (S16 ? 2 * (ne & 1) : 0));   // [n3 + 1] group starts
     int *Od = Gs + (n3 + 1);                                             // [n3] run order of the groups
     // FP64: [2 NW][256] T slots, then the V staging; S16: T chunk, then the V staging in FP32
     const size_t tf_off = (reinterpret_cast<const char *>(Od + n3) - pA + 15) & ~(size_t)15;
@@ -170,7 +173,10 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
         if constexpr (S16) Wsf[x] = (float)ew[e0 + x];
         else Ws[x] = ew[e0 + x];
         const uint32_t id = eidx[e0 + x];
-        Is[x] = (uint32_t)((int)(id & 0xffffu) - co1) | ((uint32_t)((int)(id >> 16) - co2) << 16);
+        if constexpr (S16)
+            Is16[x] = (uint16_t)(((int)(id & 0xffffu) - co1) | (((int)(id >> 16) - co2) << 8));   // local indices < 2L + 1 <= 45
+        else
+            Is[x] = (uint32_t)((int)(id & 0xffffu) - co1) | ((uint32_t)((int)(id >> 16) - co2) << 16);
     }
     for (int x = tid; x <= n3; x += NT) Gs[x] = gstart[goff + x] - e0;
     for (int x = tid; x < n3; x += NT) Od[x] = gorder[goff + x];
@@ -189,8 +195,8 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
         }
 
     const int ncb = K > 16 ? 2 : 1;                 // K3 column blocks
-    // S16: the work item's scales r_a (K1 = K1b + i), r_b (K2 = K2b + j), r_c (K3), and their inverses
-    __shared__ double s_rs[S16 ? 64 : 1], s_ri[S16 ? 64 : 1];
+    // S16: the work item's scales r_a (K1 = K1b + i), r_b (K2 = K2b + j), r_c (K3) = 2^-s_ex
+    __shared__ int s_ex[S16 ? 64 : 1];
     if constexpr (S16) {
         // squared norms of the 32 + K shell blocks (K1 block, K2 block, K3),
         // TPC consecutive lanes per block, each summing every TPC-th m
@@ -205,16 +211,12 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
             }
 #pragma unroll
         for (int o = 1; o < TPC; o <<= 1) s2 += __shfl_xor(s2, o);
-        if (sub == 0 && col < 32 + K) {
-            const int ex = s2 > 0.0 ? 1 + ilogb(sqrt(s2)) : 0;
-            s_rs[col] = ldexp(1.0, -ex);
-            s_ri[col] = ldexp(1.0, ex);
-        }
+        if (sub == 0 && col < 32 + K) s_ex[col] = s2 > 0.0 ? 1 + ilogb(sqrt(s2)) : 0;
         __syncthreads();
         // c_K3 (r_c 2^13) split in place, entry by entry (the same 8 bytes)
         for (int x = tid; x < n3 * K; x += NT) {
             const float2 c = Cf[x];
-            const double sc = s_rs[32 + x % K] * 8192.0;
+            const double sc = ldexp(1.0, 13 - s_ex[32 + x % K]);
             uint16_t hr, lr, hi, li;
             f16_split((double)c.x * sc, hr, lr);
             f16_split(-(double)c.y * sc, hi, li);
@@ -256,9 +258,9 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                     float ar = 0.f, ai = 0.f, br = 0.f, bi = 0.f;
                     if (e < E) {
                         const float w = Wsf[s + e];
-                        const uint32_t id = Is[s + e];
-                        const float2 ca = Abf[(int)(id & 0xffffu) * 16 + l16];
-                        const float2 cbv = Bbf[(int)(id >> 16) * 16 + l16];
+                        const uint32_t id = Is16[s + e];
+                        const float2 ca = Abf[(int)(id & 0xffu) * 16 + l16];
+                        const float2 cbv = Bbf[(int)(id >> 8) * 16 + l16];
                         ar = w * ca.x;
                         ai = w * ca.y;
                         br = cbv.x;
@@ -270,12 +272,12 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 }
             }
             // pair p = 16 (4 lq + v) + l16
-            const float sbv = (float)(s_rs[16 + l16] * 8192.0);
+            const int exb = 13 - s_ex[16 + l16];
             const int gpos = 2 * ((gi >> 1) ^ sw) + (gi & 1);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int p = 16 * (4 * lq + v) + l16;
-                const float sc = (float)s_rs[4 * lq + v] * sbv;
+                const float sc = ldexpf(1.f, exb - s_ex[4 * lq + v]);
                 const float tr = (q1[v] - q2[v]) * sc, ti = (q3[v] - q1[v] - q2[v]) * sc;
                 const _Float16 hr = (_Float16)tr, hi = (_Float16)ti;
                 const _Float16 lr = (_Float16)(tr - (float)hr), li = (_Float16)(ti - (float)hi);
@@ -415,7 +417,7 @@ __device__ __forceinline__ void triple_sym_body(double2 *sm, const double2 *__re
                 if constexpr (S16) {
                     const int p = 16 * (wave + NW * r) + 4 * lq + v, k3 = 16 * cb + l16;
                     const double un =
-                        k3 < K ? s_ri[p >> 4] * s_ri[16 + (p & 15)] * s_ri[32 + k3] * 0x1p-26 : 0.0;
+                        k3 < K ? ldexp(1.0, s_ex[p >> 4] + s_ex[16 + (p & 15)] + s_ex[32 + k3] - 26) : 0.0;
                     Tff[p * 16 + l16] = (float)((double)a16[r][cb][v] * un);
                 } else {
                     Tf[(16 * (wave + NW * r) + lq + 4 * v) * 16 + l16] = acc[r][cb][v];
@@ -474,7 +476,7 @@ __global__ __launch_bounds__(64 * NW) void k_triple_sym_t(const double2 *__restr
 // evaluation of K <= K_b shells are the ones with kt < K (K + 1) (K + 2) / 6,
 // since a sorted k-triple's index does not depend on the shell count.
 template <int NW, bool G3, bool S16, typename Desc>
-__global__ __launch_bounds__(64 * NW) void k_triple_sym_batch_t(const Desc *__restrict__ ev,
+__global__ __launch_bounds__(64 * NW, S16 ? 4 : 2) void k_triple_sym_batch_t(const Desc *__restrict__ ev,
                                                                  const SymItem *__restrict__ items,
                                                                  const double2 *__restrict__ coef, int64_t cstride,
                                                                  int n_c, const double *__restrict__ ew,
