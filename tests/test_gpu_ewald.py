@@ -31,12 +31,16 @@ def say(*a):
 
 CASES = [
     # (L, LMAX, N, K, lambda, dq): the separated-grid contraction with odd m (N <= 32, small L),
-    # two k1 halves of the triple products (K = 20), the GEMM contraction (N > 32)
+    # two k1 halves of the triple products (K = 20), the GEMM contraction (N > 32), the wide
+    # separated-grid instance (more than 24 classes per m2: L = 14, and L = 18 with its 40 rows
+    # of (m2, cos/sin) at N = 32)
     (4, 6, 4, 3, 2.5, 1.0),
     (6, 9, 5, 5, 1.0, 0.8),
     (4, 6, 32, 4, 2.0, 1.0),
     (6, 8, 4, 20, 0.5, 0.3),
     (4, 6, 40, 3, 2.0, 1.0),
+    (14, 16, 24, 6, 1.5, 0.1),
+    (18, 19, 32, 5, 2.5, 0.08),
 ]
 
 

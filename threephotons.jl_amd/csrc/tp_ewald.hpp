@@ -14,63 +14,86 @@
 // The triple invariants As and the contraction c3 = Bm . G are unchanged.
 #pragma once
 
-constexpr int kLamThreads = 256;
-constexpr int kLamClasses = 1;    // classes per workgroup (grid y)
+constexpr int kLamMaxThreads = 1024;
 
-struct LamEnt {
-    double hw;        // folded 3j weight
-    int lt;           // l-triple (row of As)
-    int y1, y2, y3;   // (l, m) rows of the sphPlm table, times K
-    int fl;           // 1: l3 differs from the previous entry's, 2: (l2, l3) differs
+// One entry of a class: its folded 3j weight, the l-triple (row of As), the
+// rows l1 / lstep, l2 / lstep, l3 / lstep of the class's staged sphPlm values,
+// and two flags: 1 when l3 differs from the previous entry's, 2 when (l2, l3)
+// does (the first entry of a class carries both).
+struct LamRec {
+    double hw;
+    uint32_t a;   // lt | flags << 16
+    uint32_t b;   // r1 | r2 << 8 | r3 << 16
 };
 
-// One thread per column kt of [0, ldg) (columns past klength get zeros), the
-// classes [kLamClasses blockIdx.y, + kLamClasses) of G.  ktk[kt] = k1 | k2 << 10
-// | k3 << 20; Y[y + k - 1] = sphPlm(l, m, theta_k) for the (l, m) row y: the
-// shells of one (l, m) are contiguous, so the lanes of a wave (consecutive
-// k-triples, mostly differing in k3) read one or two cache lines.
-__global__ __launch_bounds__(kLamThreads) void k_glam(const double *__restrict__ As, int64_t lda,
-                                                      const int *__restrict__ ktk, int klength,
-                                                      const double *__restrict__ Y,
-                                                      const LamEnt *__restrict__ ent, const int *__restrict__ cstart,
-                                                      int n_m, double *__restrict__ G, int64_t ldg)
+// Dynamic LDS of k_glam: the class's sphPlm rows [3][nr][K] and its entries.
+inline size_t glam_lds_bytes(int nr, int K, int max_ent)
 {
-    const int kt = blockIdx.x * kLamThreads + threadIdx.x;
+    return sizeof(double) * (size_t)((3 * nr * K + 1) & ~1) + sizeof(LamRec) * (size_t)max_ent;
+}
+
+// One workgroup per (block of blockDim.x columns, class c): thread -> column
+// kt (columns in [klength, ldg) get zeros).  The class fixes m1 = -(m2 + mu3),
+// m2 and mu3, so the sphPlm factors it needs are the rows (l, m1), (l, m2),
+// (l, mu3) for l = 0, lstep, ..., L of every shell: they and the class's
+// entries are staged in LDS, and the only global load per entry is the As
+// value.  Y[(l (2L+1) + m + L) K + k - 1] = sphPlm(l, m, theta_k) at K shells;
+// ktk[kt] = k1 | k2 << 10 | k3 << 20.  With the blocks in x and the classes
+// in y, workgroup i runs on XCD i % 8, so at 4 column blocks every XCD reads
+// one block of As (3.4 MB at L = 18, K = 26: it stays in that XCD's L2).
+__global__ __launch_bounds__(kLamMaxThreads) void k_glam(const double *__restrict__ As, int64_t lda,
+                                                         const int *__restrict__ ktk, int klength,
+                                                         const double *__restrict__ Y, int L, int lstep, int K,
+                                                         const int2 *__restrict__ cm, const LamRec *__restrict__ ent,
+                                                         const int *__restrict__ cstart, double *__restrict__ G,
+                                                         int64_t ldg)
+{
+    extern __shared__ double lam_sm[];
+    const int c = blockIdx.y, W = 2 * L + 1, nr = L / lstep + 1, nt = blockDim.x;
+    const int eb = cstart[c], n = cstart[c + 1] - eb;
+    const int2 mm = cm[c];   // (m2, mu3)
+    const int m1 = -(mm.x + mm.y);
+    double *Ys = lam_sm;   // [3][nr][K]: rows of m1, m2, mu3
+    LamRec *se = reinterpret_cast<LamRec *>(lam_sm + ((3 * nr * K + 1) & ~1));
+    for (int i = threadIdx.x; i < 3 * nr * K; i += nt) {
+        const int w = i / (nr * K), r = (i / K) % nr, k = i % K, l = r * lstep;
+        const int m = w == 0 ? m1 : (w == 1 ? mm.x : mm.y);
+        Ys[i] = (m >= -l && m <= l) ? Y[(size_t)(l * W + m + L) * K + k] : 0.0;
+    }
+    for (int i = threadIdx.x; i < n; i += nt) se[i] = ent[eb + i];
+    __syncthreads();
+    const int kt = blockIdx.x * nt + threadIdx.x;
     if (kt >= ldg) return;   // no barriers below
-    const int c0 = blockIdx.y * kLamClasses, c1 = min(c0 + kLamClasses, n_m);
     if (kt >= klength) {
-        for (int c = c0; c < c1; ++c) G[(int64_t)c * ldg + kt] = 0.0;
+        G[(int64_t)c * ldg + kt] = 0.0;
         return;
     }
     const int p = ktk[kt];
-    const double *Y1 = Y + ((p & 1023) - 1);
-    const double *Y2 = Y + (((p >> 10) & 1023) - 1);
-    const double *Y3 = Y + (((p >> 20) & 1023) - 1);
-    for (int c = c0; c < c1; ++c) {
-        // a class's entries run sorted by (l3, l2, l1): the k3 factor and the
-        // (k2, k3) product are reloaded only when l3 or (l2, l3) changes (the
-        // first entry carries both flags); two accumulators keep two
-        // independent chains of loads and FMAs in flight
-        double acc0 = 0.0, acc1 = 0.0, y3 = 0.0, y23 = 0.0;
-        const int e1 = cstart[c + 1];
-        int e = cstart[c];
-        for (; e + 1 < e1; e += 2) {
-            const LamEnt a = ent[e], b = ent[e + 1];   // the same for every lane
-            if (a.fl & 1) y3 = Y3[a.y3];
-            if (a.fl & 2) y23 = Y2[a.y2] * y3;
-            const double ta = a.hw * Y1[a.y1] * y23;
-            if (b.fl & 1) y3 = Y3[b.y3];
-            if (b.fl & 2) y23 = Y2[b.y2] * y3;
-            const double tb = b.hw * Y1[b.y1] * y23;
-            acc0 += ta * As[(int64_t)a.lt * lda + kt];
-            acc1 += tb * As[(int64_t)b.lt * lda + kt];
-        }
-        if (e < e1) {
-            const LamEnt a = ent[e];
-            if (a.fl & 1) y3 = Y3[a.y3];
-            if (a.fl & 2) y23 = Y2[a.y2] * y3;
-            acc0 += a.hw * Y1[a.y1] * y23 * As[(int64_t)a.lt * lda + kt];
-        }
-        G[(int64_t)c * ldg + kt] = acc0 + acc1;
+    const double *Y1 = Ys + ((p & 1023) - 1);
+    const double *Y2 = Ys + nr * K + (((p >> 10) & 1023) - 1);
+    const double *Y3 = Ys + 2 * nr * K + (((p >> 20) & 1023) - 1);
+    const double *Ak = As + kt;
+    // entries run sorted by (l3, l2, l1): the k3 factor and the (k2, k3)
+    // product are reloaded only when l3 or (l2, l3) changes; two
+    // accumulators keep two independent chains of loads and FMAs in flight
+    double acc0 = 0.0, acc1 = 0.0, y3 = 0.0, y23 = 0.0;
+    int e = 0;
+    for (; e + 1 < n; e += 2) {
+        const LamRec a = se[e], b = se[e + 1];   // the same for every lane
+        if (a.a & 0x10000u) y3 = Y3[(int)(a.b >> 16) * K];
+        if (a.a & 0x20000u) y23 = Y2[(int)((a.b >> 8) & 255u) * K] * y3;
+        const double ta = a.hw * Y1[(int)(a.b & 255u) * K] * y23;
+        if (b.a & 0x10000u) y3 = Y3[(int)(b.b >> 16) * K];
+        if (b.a & 0x20000u) y23 = Y2[(int)((b.b >> 8) & 255u) * K] * y3;
+        const double tb = b.hw * Y1[(int)(b.b & 255u) * K] * y23;
+        acc0 += ta * Ak[(int64_t)(a.a & 0xffffu) * lda];
+        acc1 += tb * Ak[(int64_t)(b.a & 0xffffu) * lda];
     }
+    if (e < n) {
+        const LamRec a = se[e];
+        if (a.a & 0x10000u) y3 = Y3[(int)(a.b >> 16) * K];
+        if (a.a & 0x20000u) y23 = Y2[(int)((a.b >> 8) & 255u) * K] * y3;
+        acc0 += a.hw * Y1[(int)(a.b & 255u) * K] * y23 * Ak[(int64_t)(a.a & 0xffffu) * lda];
+    }
+    G[(int64_t)c * ldg + kt] = acc0 + acc1;
 }

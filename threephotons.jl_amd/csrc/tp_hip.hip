@@ -129,8 +129,11 @@ struct tp_basis {
     int lam_yrow = 0;
     double *d_lamY = nullptr;      // [(L+1)(2L+1)][K] sphPlm(l, m, theta_k)
     int *d_ktk = nullptr;          // [klength] k1 | k2 << 10 | k3 << 20
-    LamEnt *d_lament = nullptr;    // entries of every class, class-major
+    LamRec *d_lament = nullptr;    // entries of every class, class-major
     int *d_lamcs = nullptr;        // [n_m + 1] first entry of each class
+    int2 *d_lamcm = nullptr;       // [n_m] (m2, mu3) of each class
+    size_t lam_lds = 0;            // k_glam's dynamic LDS
+    int lam_nt = 0;                // k_glam's workgroup size (columns per workgroup)
     int64_t R_pad = 0, nlt_pad = 0, kl_pad = 0, nm_pad64 = 0, nm_pad16 = 0;
     int n_rt = 0, n_ct = 0;
     int n_groups = 0, nchunks = 0, Gmax = 0, Emax = 0, npairs = 0, PB = 0;
@@ -179,6 +182,7 @@ struct tp_basis {
     // separated grid (k_c3sep): classes of one m2 contiguous
     int n_m2 = 0, jpad = 0, sep_kb = 0, njc = 0;
     bool use_sep = false;          // N <= 32, 2(L/2+1) <= 24 and <= 24 classes per m2
+    bool sep_wide = false;         // otherwise the wide k_c3sep: N <= 32, 2(L+1) <= 40 (odd m, lambda dq != 0)
     // energies on the folded grid (k_c3sep FOLD): lambda dq = 0 (even m2, mu3)
     // and even N; njf j blocks of 16 cover its N values of j
     bool fold = false;
@@ -367,10 +371,9 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     } else {
     if (b->lam) {
         // curved Ewald sphere: the class weights depend on the k-triple
-        hipLaunchKernelGGL(k_glam, dim3((unsigned)(b->kl_pad / kLamThreads + (b->kl_pad % kLamThreads != 0)),
-                                        (unsigned)((b->n_m + kLamClasses - 1) / kLamClasses)),
-                           dim3(kLamThreads), 0, st, sc.As, b->kl_pad, b->d_ktk, (int)b->klength, b->d_lamY,
-                           b->d_lament, b->d_lamcs, b->n_m, sc.G, b->kl_pad);
+        hipLaunchKernelGGL(k_glam, dim3((unsigned)((b->kl_pad + b->lam_nt - 1) / b->lam_nt), (unsigned)b->n_m),
+                           dim3((unsigned)b->lam_nt), b->lam_lds, st, sc.As, b->kl_pad, b->d_ktk, (int)b->klength,
+                           b->d_lamY, b->L, b->lstep, b->K, b->d_lamcm, b->d_lament, b->d_lamcs, sc.G, b->kl_pad);
     } else if (prec == TP_PREC_FP16X3_T64) {
     // G = H . As on FP16 hi/lo MFMA operands (As split while staged), FP64 out
     // (16 columns x 4 row blocks per workgroup; rows past the classes up to
@@ -389,7 +392,7 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     }
     TP_HIP(hipGetLastError());
     if (timing) TP_HIP(hipEventRecord(ctx->ev[2], st));
-    if (!b->use_sep) {
+    if (!b->use_sep && !b->sep_wide) {
         // c3 = Bm . G as a GEMM (k_c3sep keeps at most 32 values of alpha per
         // workgroup and at most 24 classes per m2 and 24 rows of (m2, cos/sin))
         const dim3 grid((unsigned)(b->n_rt * b->n_ct));
@@ -415,7 +418,22 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     hipLaunchKernelGGL((k_c3sep<OUT, EN>), grid, dim3(512), c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, \
                        tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, QH, sc.part, b->kl_pad, OUTP, b->R,       \
                        b->d_logtab)
-    if (r && !c3out && b->fold && r->d_qfold) {
+    if (b->sep_wide) {
+        // odd-m classes (curved Ewald sphere): the wide instance
+        if (r && c3out) hipLaunchKernelGGL((k_c3sep<1, true, false, false, kSepWideQB>), grid, dim3(512),
+                                           c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2,
+                                           b->d_TA, tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, r->d_qhat, sc.part,
+                                           b->kl_pad, c3out, b->R, b->d_logtab);
+        else if (r) hipLaunchKernelGGL((k_c3sep<0, true, false, false, kSepWideQB>), grid, dim3(512),
+                                       c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA,
+                                       tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, r->d_qhat, sc.part, b->kl_pad,
+                                       nullptr, b->R, b->d_logtab);
+        else hipLaunchKernelGGL((k_c3sep<1, false, false, false, kSepWideQB>), grid, dim3(512),
+                                c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, tap,
+                                b->jpad, b->d_TB, b->sep_kb, b->N, ncols, nullptr, sc.part, b->kl_pad, c3out, b->R,
+                                b->d_logtab);
+        n_rt_fin = b->njc;
+    } else if (r && !c3out && b->fold && r->d_qfold) {
         // the energy on the folded grid: njf j blocks
         hipLaunchKernelGGL((k_c3sep<0, true, false, true>), dim3(grid.x, (unsigned)b->njf), dim3(512),
                            c3sep_lds_bytes(b->sep_kb), st, sc.G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, tap,
@@ -1436,7 +1454,7 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
                     b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2,
-                    b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs, b->d_sym_work, b->d_perms, b->d_gorder,
+                    b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs, b->d_lamcm, b->d_sym_work, b->d_perms, b->d_gorder,
                     b->d_symtab, b->d_symoff, b->d_Hhi, b->d_Hlo, b->d_nrm2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1596,7 +1614,13 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     }
     // curved Ewald sphere: per-class entries over the l-triples, the shells'
     // sphPlm rows and the k-triple decoding of k_glam
+    struct LamEnt {   // host form of an entry, sorted before packing into LamRec
+        double hw;
+        int lt, y1, y2, y3, fl;
+    };
     std::vector<LamEnt> lament;
+    std::vector<LamRec> lamrec;
+    std::vector<int2> lamcm;
     std::vector<int> lamcs, ktk;
     std::vector<double> lamY;
     if (b->lam) {
@@ -1639,6 +1663,20 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             }
         }
         lamcs.push_back((int)lament.size());
+        // packed: rows l / lstep of the class's staged sphPlm values
+        int max_ent = 0;
+        for (int c = 0; c < b->n_m; ++c) {
+            max_ent = std::max(max_ent, lamcs[c + 1] - lamcs[c]);
+            lamcm.push_back(make_int2(cls_m2[c], cls_mu3[c]));
+        }
+        const auto row = [&](int y) { return (uint32_t)(y / K / W / b->lstep); };
+        for (const LamEnt &x : lament)
+            lamrec.push_back(LamRec{x.hw, (uint32_t)x.lt | ((uint32_t)x.fl << 16),
+                                    row(x.y1) | (row(x.y2) << 8) | (row(x.y3) << 16)});
+        b->lam_lds = glam_lds_bytes(L / b->lstep + 1, K, max_ent);
+        if (b->lam_lds > 65536)
+            (void)hipFuncSetAttribute((const void *)k_glam, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->lam_lds);
         for (int k1 = 1; k1 <= K; ++k1)
             for (int k2 = 1; k2 <= k1; ++k2)
                 for (int k3 = 1; k3 <= k2; ++k3) ktk.push_back(k1 | (k2 << 10) | (k3 << 20));
@@ -1899,6 +1937,10 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     b->R_pad = round_up(b->R, G2_BM);
     b->nlt_pad = round_up(nlt, 16);
     b->kl_pad = round_up(b->klength, KL_ALIGN);
+    {   // k_glam: the columns in equal blocks of at most kLamMaxThreads
+        const int64_t nblk = (b->kl_pad + kLamMaxThreads - 1) / kLamMaxThreads;
+        b->lam_nt = (int)round_up((b->kl_pad + nblk - 1) / nblk, 64);
+    }
     b->nm_pad64 = round_up(b->n_m, G1_BM);
     b->nm_pad16 = round_up(b->n_m, 16);
     b->n_rt = (int)(b->R_pad / G2_BM);
@@ -1931,6 +1973,16 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     int max_cls = 0;
     for (int i = 0; i < b->n_m2; ++i) max_cls = std::max(max_cls, m2start[i + 1] - m2start[i]);
     b->use_sep = N <= 32 && b->sep_kb <= kSepMaxKB && max_cls <= 24;
+    b->sep_wide = !b->use_sep && N <= 32 && b->sep_kb <= kSepWideMaxKB;
+    if (b->sep_wide && c3sep_lds_bytes(b->sep_kb) > 65536) {
+        const int lb = (int)c3sep_lds_bytes(b->sep_kb);
+        (void)hipFuncSetAttribute((const void *)k_c3sep<1, true, false, false, kSepWideQB>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lb);
+        (void)hipFuncSetAttribute((const void *)k_c3sep<0, true, false, false, kSepWideQB>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lb);
+        (void)hipFuncSetAttribute((const void *)k_c3sep<1, false, false, false, kSepWideQB>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lb);
+    }
     b->fold = b->use_sep && !b->lam && N % 2 == 0;
     b->njf = (int)round_up(N, 16) / 16;
     std::vector<double> TA((size_t)2 * b->n_m * b->jpad, 0.0);
@@ -1984,8 +2036,8 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         tp_basis_destroy(b);
         return rc;
     }
-    if (b->lam && ((rc = up(&b->d_lamY, lamY)) || (rc = up(&b->d_ktk, ktk)) || (rc = up(&b->d_lament, lament)) ||
-                   (rc = up(&b->d_lamcs, lamcs)))) {
+    if (b->lam && ((rc = up(&b->d_lamY, lamY)) || (rc = up(&b->d_ktk, ktk)) || (rc = up(&b->d_lament, lamrec)) ||
+                   (rc = up(&b->d_lamcs, lamcs)) || (rc = up(&b->d_lamcm, lamcm)))) {
         tp_basis_destroy(b);
         return rc;
     }

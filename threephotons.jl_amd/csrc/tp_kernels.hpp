@@ -1292,12 +1292,18 @@ __global__ void k_intensity_coeff(const double2 *__restrict__ G, const double *_
 // B, of the logarithms and of the histogram reads, half of stage A.
 // ---------------------------------------------------------------------------
 constexpr int kSepMaxKB = 24;   // 2 (L/2 + 1) padded to 4, L <= 22
+// The wide instance (QB = kSepWideQB) takes the odd-m classes of the curved
+// Ewald sphere: 2 (L + 1) rows of (m2, cos/sin) padded to 4, L <= 19, and
+// any number of classes per m2 (stage A runs them in chunks of 24).  Its LDS
+// image (87 KB at L = 18) leaves one workgroup per CU.
+constexpr int kSepWideQB = 10;
+constexpr int kSepWideMaxKB = 4 * kSepWideQB;
 
 // Dynamic LDS of k_c3sep: CS[kb][16][17] doubles.
 inline size_t c3sep_lds_bytes(int kb) { return sizeof(double) * (size_t)kb * 16 * 17; }
 
-template <int OUT, bool ENERGY, bool BATCH = false, bool FOLD = false>
-__global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
+template <int OUT, bool ENERGY, bool BATCH = false, bool FOLD = false, int QB = 6>
+__global__ __launch_bounds__(512, QB > 6 ? 1 : 3) void k_c3sep(const double *__restrict__ G, int64_t ldg, const int *__restrict__ m2start,
                                                int n_m2, const double *__restrict__ TA, int64_t ta_plane, int jpad,
                                                const double *__restrict__ TB, int kb, int N, int ncols,
                                                const double *__restrict__ qhat, double *__restrict__ part,
@@ -1307,6 +1313,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
                                                const int *__restrict__ nvalid = nullptr)
 {
     static_assert(!FOLD || (ENERGY && OUT == 0), "the folded grid forms the energy only");
+    static_assert(QB == 6 || (QB == kSepWideQB && !FOLD && !BATCH), "wide instance: one evaluation, full grid");
     __shared__ __align__(16) double s_lt[ENERGY ? kLogTabSize : 1];
     __shared__ double red[8][2][16];
     extern __shared__ double cs_dyn[];
@@ -1340,23 +1347,27 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
         const int k = 2 * n_m2 + i / 256, r = (i / 16) % 16, c = i % 16;
         CS(k, r, c) = 0.0;
     }
-    // ---- stage A: blocks (m2i, cs), rows j local, K = classes of m2 (<= 24)
+    // ---- stage A: blocks (m2i, cs), rows j local, K = classes of m2 (<= 24;
+    // the wide instance: any number, in chunks of 24)
     for (int b = wave; b < 2 * n_m2; b += NW) {
         const int m2i = b >> 1, cs = b & 1;
-        const int cb = m2start[m2i], ce = m2start[m2i + 1];
+        const int cb0 = m2start[m2i], ce = m2start[m2i + 1];
         const double *ta = TA + (int64_t)cs * ta_plane;
-        double av[6], gv[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int cls = cb + 4 * q + lq;
-            const bool ok = cls < ce;
-            av[q] = ok ? ta[(int64_t)cls * jpad + j0 + l16] : 0.0;
-            gv[q] = ok ? G[(int64_t)cls * ldg + c0 + l16] : 0.0;
-        }
         dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int cb = cb0; cb < ce; cb += 24) {
+            double av[6], gv[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (cb + 4 * q < ce) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], gv[q], acc, 0, 0, 0);
+            for (int q = 0; q < 6; ++q) {
+                const int cls = cb + 4 * q + lq;
+                const bool ok = cls < ce;
+                av[q] = ok ? ta[(int64_t)cls * jpad + j0 + l16] : 0.0;
+                gv[q] = ok ? G[(int64_t)cls * ldg + c0 + l16] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (cb + 4 * q < ce) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], gv[q], acc, 0, 0, 0);
+            if constexpr (QB == 6) break;   // at most 24 classes per m2
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) CS(b, lq + 4 * v, l16) = acc[v];
     }
@@ -1366,14 +1377,14 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
     // with out-of-range values zeroed afterwards: loads under a branch would
     // make the first stage-B MFMA wait for every histogram load.
     const int ibw = wave % nib;
-    double tb[6];
+    double tb[QB];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
+    for (int q = 0; q < QB; ++q) {
         const int kk = 4 * q + lq;
         tb[q] = TB[(int64_t)(16 * ibw + l16) * kb + min(kk, kb - 1)];
     }
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
+    for (int q = 0; q < QB; ++q)
         if (4 * q + lq >= kb) tb[q] = 0.0;
     __syncthreads();
     // ---- stage B: blocks (j local, i block), K = 2 n_m2 (padded to kb), computed
@@ -1397,7 +1408,7 @@ __global__ __launch_bounds__(512, 3) void k_c3sep(const double *__restrict__ G, 
         }
         dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
+        for (int q = 0; q < QB; ++q)
             if (4 * q < kb) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(CS(4 * q + lq, jl, l16), tb[q], acc, 0, 0, 0);
         const int i = 16 * ib + l16;
 #pragma unroll
