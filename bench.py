@@ -943,8 +943,9 @@ def main():
                          "evals_per_s": args.steps / elt16, "energy_abs_err_vs_fp64": abs(Et16 - E),
                          "stage_ms": {"triple": kernt16[0], "gemm_G": kernt16[1], "c3_energy": kernt16[2],
                                       "final": kernt16[3]}},
-            "fp16x3_t64": {"mode": "FP64 triple products on the sorted l-triples, G = H.As on FP16 hi/lo MFMA "
-                                   "(k_hgemm_g), c3 = Bm.G and the energy on the FP64 separated grid",
+            "fp16x3_t64": {"mode": "triple products on the sorted l-triples (FP32 MFMA stage 1, FP16 hi/lo MFMA "
+                                   "stage 2), G = H.As on FP16 hi/lo MFMA (k_hgemm_g), c3 = Bm.G and the energy "
+                                   "on the FP64 separated grid",
                            "evals_per_s": args.steps / elt64, "energy_abs_err_vs_fp64": abs(Et64 - E),
                            "stage_ms": {"triple": kernt64[0], "gemm_G_fp16x3": kernt64[1],
                                         "c3sep_energy": kernt64[2], "final": kernt64[3]}}},

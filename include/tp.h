@@ -65,8 +65,9 @@ void tp_finalize(tp_ctx *ctx);
  * (lrange = 0:2:L).  lambda dq != 0 is the curved Ewald sphere: the
  * Legendre factors are taken at theta_k = acos(k lambda dq / (4 pi))
  * (correlations.jl:94), which needs |K lambda dq| <= 4 pi; such a basis
- * evaluates in FP64 only (tp_c3, tp_energy, tp_state_*, tp_anneal,
- * tp_energy_batch), and mixed precision, batched runs, the energy gradient
+ * evaluates in FP64 or TP_PREC_FP16X3_T64 (mixed-precision triple products,
+ * G and c3 in FP64) through tp_c3, tp_energy, tp_state_*, tp_anneal and
+ * tp_energy_batch; the other mixed modes, batched runs, the energy gradient
  * and tp_two_photons (use tp_two_photons_ewald) return TP_E_UNSUPPORTED for it. */
 int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq,
                     int for_intensity, tp_basis **out);

@@ -87,6 +87,36 @@ def test_reference_gpu_test_on_crambin(tp, oracle):
     assert abs(E - Eo) / abs(Eo) < TOL
 
 
+@pytest.mark.parametrize("L,LMAX,N,K,lam,dq", [(4, 6, 32, 4, 2.0, 1.0), (18, 19, 32, 5, 2.5, 0.08)])
+def test_fp16x3_t64_on_the_curved_sphere(tp, oracle, L, LMAX, N, K, lam, dq):
+    """FP16X3_T64 at lambda dq != 0: the mixed-precision triple products (FP32 stage 1,
+    FP16 hi/lo stage 2) feed the FP64 k_glam and separated-grid contraction.  Bars:
+    c3 within 1e-5 of the oracle (the mixed-precision bar) and the energy within 1e-9
+    relative; tp_energy_batch (one evaluation after another here) gives the same energies."""
+    say("case", L, LMAX, N, K, lam, dq)
+    c = random_real_coeff(K, L, LMAX, seed=L * 10 + K, c00=4.0)
+    b = tp.HIP_calculate_basis(L, LMAX, N, K, lam, dq)
+    ob = oracle.Basis(L, LMAX, N, K, lam=lam, dq=dq)
+    vol = tp.SphericalHarmonicsVolume(c, LMAX, K, 1.0)
+    want = ob.c3_dense(c, nthreads=8)
+    p = np.maximum(want, 1e-30)
+    rng = np.random.default_rng(3)
+    ref = c3ref_from_columns(rng.poisson(1e4 * p / p.sum() * p.shape[1]).astype(float), N, K)
+    ref = np.maximum(ref, 1e-30)
+    ref /= ref.sum()
+    E64 = tp.energy(vol, b, ref)
+    b.set_precision("fp16x3_t64")
+    got = tp.FullCorrelation_parallized(vol, b, return_raw=True)
+    err = rel_err(got, want)
+    E = tp.energy(vol, b, ref)
+    Eo = oracle.energy(want, N, ref, K, 1, K)
+    say(f"  c3 {err:.2e}, energy vs FP64 {abs(E - E64) / abs(E64):.2e}, vs oracle {abs(E - Eo) / abs(Eo):.2e}")
+    assert 0 < err < 1e-5
+    assert abs(E - Eo) / abs(Eo) < 1e-9
+    Eb = tp.energy_batch([vol, vol], [b, b], [ref, ref])
+    assert np.array_equal(np.asarray(Eb), np.array([E, E]))
+
+
 def test_curved_basis_limits(tp):
     b = tp.HIP_calculate_basis(4, 6, 4, 3, 2.5, 1.0)
     for mode in ("fp16x3", "bf16", "triple16"):

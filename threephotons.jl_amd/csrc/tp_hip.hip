@@ -319,7 +319,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         if (prec == TP_PREC_FP16X3_T64)
             hipLaunchKernelGGL(k_triple_sym16, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym16_lds, st, b0, b1,
                                sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_sym_work,
-                               b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad, sc.nrm2);
+                               b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad,
+                               b->lam ? nullptr : sc.nrm2);   // norms: column scales of k_hgemm_g
         else
             hipLaunchKernelGGL(k_triple_sym, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym_lds, st, b0, b1,
                                sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_sym_work,
@@ -2080,9 +2081,10 @@ int tp_basis_set_precision(tp_basis *b, int mode)
     if (int rc = check_ctx(b->ctx)) return rc;
     if (mode != TP_PREC_FP64 && (b->K > 64 || b->n_c > 1024))
         return tp_fail(TP_E_UNSUPPORTED, "mixed-precision contraction supports K <= 64 and L <= 44");
-    if (mode != TP_PREC_FP64 && b->lam)
-        return tp_fail(TP_E_UNSUPPORTED, "mixed precision needs lambda dq = 0 (the curved Ewald sphere runs in FP64)");
-    if (mode == TP_PREC_FP16X3_T64 && !b->d_Hhi) {
+    if (mode != TP_PREC_FP64 && mode != TP_PREC_FP16X3_T64 && b->lam)
+        return tp_fail(TP_E_UNSUPPORTED, "on the curved Ewald sphere (lambda dq != 0) the precision modes are FP64 "
+                                         "and FP16X3_T64");
+    if (mode == TP_PREC_FP16X3_T64 && !b->d_Hhi && !b->lam) {   // lambda dq != 0: G from k_glam in FP64
         // H[class][lt] (FP64 on the host) scaled by a power of two into
         // [2^13, 2^14) and split into FP16 hi/lo for k_hgemm_g
         if (b->n_m > 16 * kHgMaxRb || b->L / 2 >= kNrmLd)
@@ -4367,7 +4369,7 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
     hipStream_t st = ctx->stream;
     if ((bm->prec != TP_PREC_FP16X3 && bm->prec != TP_PREC_BF16 && bm->prec != TP_PREC_FP16X3_T64) ||
         !bm->use_mfma || (bm->prec != TP_PREC_FP16X3_T64 && bm->R % 128 != 0) ||
-        (bm->prec == TP_PREC_FP16X3_T64 && (!bm->use_sym || !bm->use_sep))) {
+        (bm->prec == TP_PREC_FP16X3_T64 && (!bm->use_sym || !bm->use_sep)) || bm->lam) {
         // one evaluation after another on the device
         double *dE = nullptr;
         if (int rc = dev_alloc(&dE, n)) return rc;

@@ -167,9 +167,12 @@ class BasisTypeHip:
     def set_precision(self, mode):
         """'fp64' (default), 'fp16x3' (compensated FP16 MFMA triple products and
         contraction), 'bf16', 'triple16' (compensated FP16 triple products,
-        FP64 contraction) or 'fp16x3_t64' (FP64 triple products on the sorted
-        l-triples, G = H.As on compensated FP16 MFMA, the FP64 separated-grid
-        contraction c3 = Bm.G and energy)."""
+        FP64 contraction) or 'fp16x3_t64' (triple products on the sorted
+        l-triples, FP32 MFMA stage 1 and compensated FP16 MFMA stage 2,
+        G = H.As on compensated FP16 MFMA, the FP64 separated-grid
+        contraction c3 = Bm.G and energy).  On the curved Ewald sphere
+        (lambda dq != 0) the modes are 'fp64' and 'fp16x3_t64', whose G is
+        then formed in FP64 (k_glam)."""
         m = {"fp64": PREC_FP64, "fp16x3": PREC_FP16X3, "bf16": PREC_BF16,
              "triple16": 3, "fp16x3_t64": 4}[mode] if isinstance(mode, str) else mode
         check(lib().tp_basis_set_precision(self.h, m), "tp_basis_set_precision")
