@@ -65,22 +65,25 @@ def test_rotate_invariances(tp):
     assert rel_err(tp.FullCorrelation_parallized(rot_l, b, return_raw=True), c3) > 1e-4
 
 
-def _hist(tp, c, L, LMAX, N, K, seed):
-    b = tp.HIP_calculate_basis(L, LMAX, N, K)
+def _hist(tp, c, L, LMAX, N, K, seed, lam=0.0, dq=0.0):
+    b = tp.HIP_calculate_basis(L, LMAX, N, K, lam, dq)
     c64 = tp.FullCorrelation_parallized(c, b, return_raw=True)
     p = np.maximum(c64, 1e-30)
     cnt = np.random.default_rng(seed).poisson(1e6 * p / p.sum()).astype(float) + 1e-30
     return c3ref_from_columns(cnt, N, K)
 
 
-@pytest.mark.parametrize("prec,N", [("fp16x3", 8), ("fp64", 8), ("fp16x3_t64", 8),
-                                    ("fp16x3_t64", 7)])   # odd N: the full (alpha, beta) grid, no folding
-def test_energy_batch_matches_single(tp, prec, N):
-    L, LMAX = 8, 25
+@pytest.mark.parametrize("prec,N,L,lam", [("fp16x3", 8, 8, 0.0), ("fp64", 8, 8, 0.0), ("fp16x3_t64", 8, 8, 0.0),
+                                         ("fp16x3_t64", 7, 8, 0.0),    # odd N: the full (alpha, beta) grid
+                                         ("fp16x3_t64", 8, 8, 2.5),    # curved Ewald sphere (k_glam)
+                                         ("fp16x3_t64", 8, 14, 1.5)])  # ... with the wide separated grid
+def test_energy_batch_matches_single(tp, prec, N, L, lam):
+    LMAX = 25
+    dq = 0.3 if lam else 0.0   # |K lambda dq| <= 4 pi at K = 12
     Ks = [3, 9, 5, 12]
     starts = [random_real_coeff(K, L, LMAX, seed=40 + K, c00=4.0, decay=0.1) for K in Ks]
-    refs = [_hist(tp, starts[i], L, LMAX, N, K, 70 + i) for i, K in enumerate(Ks)]
-    bases = [tp.HIP_calculate_basis(L, LMAX, N, K) for K in Ks]
+    refs = [_hist(tp, starts[i], L, LMAX, N, K, 70 + i, lam, dq) for i, K in enumerate(Ks)]
+    bases = [tp.HIP_calculate_basis(L, LMAX, N, K, lam, dq) for K in Ks]
     for b in bases:
         b.set_precision(prec)
     rng = np.random.default_rng(1)
@@ -99,12 +102,12 @@ def test_energy_batch_matches_single(tp, prec, N):
     assert np.all(np.abs(Eb - E64) <= 1e-6 * np.abs(E64))
 
 
-def _anneal_setup(tp, Ks, nprop, hist_cap=0, L=8, LMAX=25, N=8, prec="fp16x3"):
-    master = tp.HIP_calculate_basis(L, LMAX, N, max(Ks))
+def _anneal_setup(tp, Ks, nprop, hist_cap=0, L=8, LMAX=25, N=8, prec="fp16x3", lam=0.0, dq=0.0):
+    master = tp.HIP_calculate_basis(L, LMAX, N, max(Ks), lam, dq)
     master.set_precision(prec)
-    bases = [tp.HIP_calculate_basis(L, LMAX, N, K) for K in Ks]
+    bases = [tp.HIP_calculate_basis(L, LMAX, N, K, lam, dq) for K in Ks]
     truth = [random_real_coeff(K, L, LMAX, seed=90 + K, c00=4.0, decay=0.1) for K in Ks]
-    refs = [_hist(tp, truth[i], L, LMAX, N, K, 20 + i) for i, K in enumerate(Ks)]
+    refs = [_hist(tp, truth[i], L, LMAX, N, K, 20 + i, lam, dq) for i, K in enumerate(Ks)]
     rng = np.random.default_rng(4)
     starts = [tp.rotate(truth[i], bases[i], random_quats(rng, L // 2)) for i in range(len(Ks))]
     B = tp.AnnealBatch(master, refs, starts, seeds=[1000 + g for g in range(len(Ks))], nprop=nprop,
@@ -112,13 +115,14 @@ def _anneal_setup(tp, Ks, nprop, hist_cap=0, L=8, LMAX=25, N=8, prec="fp16x3"):
     return master, bases, refs, starts, B
 
 
-@pytest.mark.parametrize("prec", ["fp16x3", "fp16x3_t64"])
-def test_batch_eval_quats_matches_single(tp, prec):
+@pytest.mark.parametrize("prec,lam", [("fp16x3", 0.0), ("fp16x3_t64", 0.0), ("fp16x3_t64", 2.5)])
+def test_batch_eval_quats_matches_single(tp, prec, lam):
     """A batch's energies equal single evaluations in the same mode (the
-    batch keeps its mode when the basis changes afterwards)."""
+    batch keeps its mode when the basis changes afterwards); lam != 0: the
+    curved Ewald sphere."""
     Ks, nprop = [4, 9, 6], 4
     L = 8
-    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, prec=prec)
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, prec=prec, lam=lam, dq=0.3 if lam else 0.0)
     master.set_precision("fp64")
     rng = np.random.default_rng(9)
     q = np.stack([random_quats(rng, L // 2) for _ in range(len(Ks) * nprop)])
@@ -131,6 +135,29 @@ def test_batch_eval_quats_matches_single(tp, prec):
             c = tp.rotate(starts[g], bases[g], q[e])
             Es = tp.energy(c, bases[g], refs[g])
             assert abs(E[e] - Es) <= 1e-10 * abs(Es), (g, p, E[e], Es)
+
+
+def test_batch_sweeps_curved_sphere(tp):
+    """Annealing sweeps of a batch on the curved Ewald sphere (lambda dq != 0,
+    FP16X3_T64: k_glam over the batch's columns): every run's energy equals a
+    single evaluation of its state, and the runs descend."""
+    Ks, nprop, nsw = [5, 9, 7], 8, 10
+    master, bases, refs, starts, B = _anneal_setup(tp, Ks, nprop, hist_cap=nsw, prec="fp16x3_t64", lam=2.5,
+                                                   dq=0.3)
+    B.probe(0.1)
+    g0 = [B.group(g) for g in range(len(Ks))]
+    for b in bases:
+        b.set_precision("fp16x3_t64")
+    for g in range(len(Ks)):
+        Es = tp.energy(starts[g], bases[g], refs[g])
+        assert abs(g0[g]["E"] - Es) <= 1e-10 * abs(Es)
+    B.run(nsw, decay=0.999, factor=1.01)
+    for g in range(len(Ks)):
+        st = B.group(g)
+        assert st["nsweep"] == nsw
+        Es = tp.energy(B.coeff(g), bases[g], refs[g])
+        assert abs(st["E"] - Es) <= 1e-10 * abs(Es)
+        assert st["E"] <= g0[g]["E"] + 5 * g0[g]["T"]
 
 
 @pytest.mark.parametrize("family", ["plane", "quat"])

@@ -33,20 +33,27 @@ inline size_t glam_lds_bytes(int nr, int K, int max_ent)
 }
 
 // One workgroup per (block of blockDim.x columns, class c): thread -> column
-// kt (columns in [klength, ldg) get zeros).  The class fixes m1 = -(m2 + mu3),
-// m2 and mu3, so the sphPlm factors it needs are the rows (l, m1), (l, m2),
-// (l, mu3) for l = 0, lstep, ..., L of every shell: they and the class's
-// entries are staged in LDS, and the only global load per entry is the As
-// value.  Y[(l (2L+1) + m + L) K + k - 1] = sphPlm(l, m, theta_k) at K shells;
+// (columns past klength, and in a batch the columns past a block's nvalid,
+// get zeros).  The class fixes m1 = -(m2 + mu3), m2 and mu3, so the sphPlm
+// factors it needs are the rows (l, m1), (l, m2), (l, mu3) for l = 0, lstep,
+// ..., L of every shell: they and the class's entries are staged in LDS, and
+// the only global load per entry is the As value.
+// Y[(l (2L+1) + m + L) K + k - 1] = sphPlm(l, m, theta_k) at K shells;
 // ktk[kt] = k1 | k2 << 10 | k3 << 20.  With the blocks in x and the classes
 // in y, workgroup i runs on XCD i % 8, so at 4 column blocks every XCD reads
 // one block of As (3.4 MB at L = 18, K = 26: it stays in that XCD's L2).
-__global__ __launch_bounds__(kLamMaxThreads) void k_glam(const double *__restrict__ As, int64_t lda,
-                                                         const int *__restrict__ ktk, int klength,
-                                                         const double *__restrict__ Y, int L, int lstep, int K,
-                                                         const int2 *__restrict__ cm, const LamRec *__restrict__ ent,
-                                                         const int *__restrict__ cstart, double *__restrict__ G,
-                                                         int64_t ldg)
+// BATCH: the columns of a batch (As in FP32, as k_triple_sym_batch writes
+// it); 64-column block j holds the k-triples bkt0[j] .. bkt0[j] + nvalid[j]
+// - 1 of one evaluation (a run's k-triples are the first ones of the basis's
+// order, k1 >= k2 >= k3 with k1 outermost).
+template <bool BATCH, typename AT>
+__global__ __launch_bounds__(kLamMaxThreads) void k_glam_t(const AT *__restrict__ As, int64_t lda,
+                                                           const int *__restrict__ ktk, int klength,
+                                                           const double *__restrict__ Y, int L, int lstep, int K,
+                                                           const int2 *__restrict__ cm, const LamRec *__restrict__ ent,
+                                                           const int *__restrict__ cstart, double *__restrict__ G,
+                                                           int64_t ldg, const int *__restrict__ nvalid,
+                                                           const int *__restrict__ bkt0)
 {
     extern __shared__ double lam_sm[];
     const int c = blockIdx.y, W = 2 * L + 1, nr = L / lstep + 1, nt = blockDim.x;
@@ -62,17 +69,24 @@ __global__ __launch_bounds__(kLamMaxThreads) void k_glam(const double *__restric
     }
     for (int i = threadIdx.x; i < n; i += nt) se[i] = ent[eb + i];
     __syncthreads();
-    const int kt = blockIdx.x * nt + threadIdx.x;
-    if (kt >= ldg) return;   // no barriers below
-    if (kt >= klength) {
-        G[(int64_t)c * ldg + kt] = 0.0;
+    const int col = blockIdx.x * nt + threadIdx.x;
+    if (col >= ldg) return;   // no barriers below
+    int kt = col;
+    if constexpr (BATCH) {
+        const int loc = col & 63;
+        kt = loc < nvalid[col >> 6] ? bkt0[col >> 6] + loc : -1;
+    } else if (kt >= klength) {
+        kt = -1;
+    }
+    if (kt < 0) {
+        G[(int64_t)c * ldg + col] = 0.0;
         return;
     }
     const int p = ktk[kt];
     const double *Y1 = Ys + ((p & 1023) - 1);
     const double *Y2 = Ys + nr * K + (((p >> 10) & 1023) - 1);
     const double *Y3 = Ys + 2 * nr * K + (((p >> 20) & 1023) - 1);
-    const double *Ak = As + kt;
+    const AT *Ak = As + col;
     // entries run sorted by (l3, l2, l1): the k3 factor and the (k2, k3)
     // product are reloaded only when l3 or (l2, l3) changes; two
     // accumulators keep two independent chains of loads and FMAs in flight
@@ -86,14 +100,17 @@ __global__ __launch_bounds__(kLamMaxThreads) void k_glam(const double *__restric
         if (b.a & 0x10000u) y3 = Y3[(int)(b.b >> 16) * K];
         if (b.a & 0x20000u) y23 = Y2[(int)((b.b >> 8) & 255u) * K] * y3;
         const double tb = b.hw * Y1[(int)(b.b & 255u) * K] * y23;
-        acc0 += ta * Ak[(int64_t)(a.a & 0xffffu) * lda];
-        acc1 += tb * Ak[(int64_t)(b.a & 0xffffu) * lda];
+        acc0 += ta * (double)Ak[(int64_t)(a.a & 0xffffu) * lda];
+        acc1 += tb * (double)Ak[(int64_t)(b.a & 0xffffu) * lda];
     }
     if (e < n) {
         const LamRec a = se[e];
         if (a.a & 0x10000u) y3 = Y3[(int)(a.b >> 16) * K];
         if (a.a & 0x20000u) y23 = Y2[(int)((a.b >> 8) & 255u) * K] * y3;
-        acc0 += a.hw * Y1[(int)(a.b & 255u) * K] * y23 * Ak[(int64_t)(a.a & 0xffffu) * lda];
+        acc0 += a.hw * Y1[(int)(a.b & 255u) * K] * y23 * (double)Ak[(int64_t)(a.a & 0xffffu) * lda];
     }
-    G[(int64_t)c * ldg + kt] = acc0 + acc1;
+    G[(int64_t)c * ldg + col] = acc0 + acc1;
 }
+
+#define k_glam (k_glam_t<false, double>)
+#define k_glam_batch (k_glam_t<true, float>)

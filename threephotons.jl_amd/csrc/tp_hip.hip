@@ -374,7 +374,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         // curved Ewald sphere: the class weights depend on the k-triple
         hipLaunchKernelGGL(k_glam, dim3((unsigned)((b->kl_pad + b->lam_nt - 1) / b->lam_nt), (unsigned)b->n_m),
                            dim3((unsigned)b->lam_nt), b->lam_lds, st, sc.As, b->kl_pad, b->d_ktk, (int)b->klength,
-                           b->d_lamY, b->L, b->lstep, b->K, b->d_lamcm, b->d_lament, b->d_lamcs, sc.G, b->kl_pad);
+                           b->d_lamY, b->L, b->lstep, b->K, b->d_lamcm, b->d_lament, b->d_lamcs, sc.G, b->kl_pad,
+                           nullptr, nullptr);
     } else if (prec == TP_PREC_FP16X3_T64) {
     // G = H . As on FP16 hi/lo MFMA operands (As split while staged), FP64 out
     // (16 columns x 4 row blocks per workgroup; rows past the classes up to
@@ -529,6 +530,7 @@ struct tp_batch {
     double *d_G = nullptr;                         // [16 nrb16][ldA] G = H . As (TP_PREC_FP16X3_T64)
     const double **d_qtab = nullptr;
     int *d_nvalid = nullptr;
+    int *d_bkt0 = nullptr;         // curved Ewald sphere: first k-triple of each 64-column block
     double2 *d_raw = nullptr;      // staging of host coefficients (plain evaluations)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -565,7 +567,7 @@ static void batch_free(tp_batch *B)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {B->d_ev,  B->d_gs,  B->d_work, B->d_swork, B->d_cur,  B->d_prop,   B->d_quat,   B->d_As, B->d_As32,
                     B->d_sc,  B->d_isc, B->d_part, B->d_E,    B->d_Ehist,  B->d_Ahi,    B->d_Alo,
-                    (void *)B->d_qtab, B->d_nvalid, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn, B->d_shsc,
+                    (void *)B->d_qtab, B->d_nvalid, B->d_bkt0, B->d_raw, B->d_Xlog, B->d_Tlog, B->d_logn, B->d_shsc,
                     B->d_Xi, B->d_Hi[0], B->d_Hi[1], B->d_Hi[2], B->d_Yi, B->d_G};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -706,7 +708,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     const bool fac = b->prec == TP_PREC_FP16X3_T64;   // factored contraction G = H . As, c3 = Bm . G
     if (!fac && b->R % 128 != 0)
         return tp_fail(TP_E_UNSUPPORTED, "batched contraction needs 2N^2 to be a multiple of 128");
-    if (fac && (!b->use_sym || !b->use_sep || !b->d_Hhi))
+    if (fac && (!b->use_sym || !(b->use_sep || (b->lam && b->sep_wide)) || (!b->lam && !b->d_Hhi)))
         return tp_fail(TP_E_UNSUPPORTED, "TP_PREC_FP16X3_T64 batches need the sorted l-triple tables and the "
                                          "separated grid (N <= 32)");
     tp_batch *B = new tp_batch();
@@ -725,7 +727,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     B->refs.assign(refs, refs + ngroups);
     // column blocks
     std::vector<const double *> qtab;
-    std::vector<int> nvalid;
+    std::vector<int> nvalid, bkt0;
     int64_t cb = 0;
     for (int g = 0; g < ngroups; ++g) {
         const tp_ref *r = refs[g];
@@ -752,6 +754,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
                 else
                     qtab.push_back(r->d_qhat + (nv > 0 ? j * 64 * b->R : 0));
                 nvalid.push_back((int)nv);
+                bkt0.push_back((int)(j * 64));
             }
         GroupState s;
         std::memset(&s, 0, sizeof(s));
@@ -765,6 +768,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     while (cb % 2) {   // even block count: 128-column tiles
         qtab.push_back(B->fold ? refs[0]->d_qfold : refs[0]->d_qhat);
         nvalid.push_back(0);
+        bkt0.push_back(0);
         ++cb;
     }
     B->ncb = cb;
@@ -782,7 +786,8 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         (rc = dev_alloc(&B->d_shsc, (size_t)B->ne * b->K)) ||
         (!fac && (rc = dev_alloc(&B->d_Ahi, (size_t)B->ldA * b->ldk))) ||
         (!fac && B->prec != TP_PREC_BF16 && (rc = dev_alloc(&B->d_Alo, (size_t)B->ldA * b->ldk))) ||
-        (fac && (rc = dev_alloc(&B->d_G, (size_t)16 * b->nrb16 * B->ldA))) ||
+        (fac && (rc = dev_alloc(&B->d_G, (size_t)(b->lam ? b->n_m : 16 * b->nrb16) * B->ldA))) ||
+        (b->lam && (rc = dev_alloc(&B->d_bkt0, bkt0.size()))) ||
         (rc = dev_alloc(&B->d_qtab, qtab.size())) || (rc = dev_alloc(&B->d_nvalid, nvalid.size())) ||
         (!sweep && (rc = dev_alloc(&B->d_raw, (size_t)b->K * b->LMAX * b->LMAX)))) {
         batch_free(B);
@@ -798,6 +803,8 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     hipError_t e = hipMemcpy(B->d_ev, B->h_ev.data(), sizeof(EvalDesc) * B->ne, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(B->d_qtab, qtab.data(), sizeof(double *) * qtab.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(B->d_nvalid, nvalid.data(), sizeof(int) * nvalid.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && B->d_bkt0)
+        e = hipMemcpy(B->d_bkt0, bkt0.data(), sizeof(int) * bkt0.size(), hipMemcpyHostToDevice);
     // As columns that no evaluation writes (tile padding) must read as zero
     if (e == hipSuccess && B->d_As) e = hipMemset(B->d_As, 0, sizeof(double) * (size_t)b->n_lt * B->ldA);
     if (e == hipSuccess && B->d_As32) e = hipMemset(B->d_As32, 0, sizeof(float) * (size_t)b->n_lt * B->ldA);
@@ -886,6 +893,27 @@ static void fac_shape(const tp_basis *b, int cand, int *rows, int *cols)
 static void batch_contract_fac(const tp_batch *B, hipStream_t st, int cand)
 {
     const tp_basis *b = B->b;
+    if (b->lam) {
+        // curved Ewald sphere: G per column (k_glam, FP64), then c3 = Bm . G
+        // on the separated grid (the wide instance for the odd-m classes)
+        hipLaunchKernelGGL(k_glam_batch, dim3((unsigned)((B->ldA + b->lam_nt - 1) / b->lam_nt), (unsigned)b->n_m),
+                           dim3((unsigned)b->lam_nt), b->lam_lds, st, B->d_As32, B->ldA, b->d_ktk, (int)b->klength,
+                           b->d_lamY, b->L, b->lstep, b->K, b->d_lamcm, b->d_lament, b->d_lamcs, B->d_G, B->ldA,
+                           B->d_nvalid, B->d_bkt0);
+        const int64_t tap = (int64_t)b->n_m * b->jpad;
+        const dim3 grid((unsigned)(B->ldA / 16), (unsigned)b->njc);
+        if (b->sep_wide)
+            hipLaunchKernelGGL((k_c3sep<0, true, true, false, kSepWideQB>), grid, dim3(512),
+                               c3sep_lds_bytes(b->sep_kb), st, B->d_G, B->ldA, b->d_m2start, b->n_m2, b->d_TA, tap,
+                               b->jpad, b->d_TB, b->sep_kb, b->N, 0, nullptr, B->d_part, B->ldA, nullptr, (int64_t)0,
+                               b->d_logtab, B->d_qtab, B->d_nvalid);
+        else
+            hipLaunchKernelGGL((k_c3sep<0, true, true>), grid, dim3(512), c3sep_lds_bytes(b->sep_kb), st, B->d_G,
+                               B->ldA, b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad, b->d_TB, b->sep_kb, b->N, 0,
+                               nullptr, B->d_part, B->ldA, nullptr, (int64_t)0, b->d_logtab, B->d_qtab,
+                               B->d_nvalid);
+        return;
+    }
     const int nrb = b->nrb16;
     switch (cand) {
     case 0: if (nrb <= 9) batch_hgemm_launch<64, 9>(B, st); else batch_hgemm_launch<64, 12>(B, st); break;
@@ -1675,9 +1703,12 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             lamrec.push_back(LamRec{x.hw, (uint32_t)x.lt | ((uint32_t)x.fl << 16),
                                     row(x.y1) | (row(x.y2) << 8) | (row(x.y3) << 16)});
         b->lam_lds = glam_lds_bytes(L / b->lstep + 1, K, max_ent);
-        if (b->lam_lds > 65536)
+        if (b->lam_lds > 65536) {
             (void)hipFuncSetAttribute((const void *)k_glam, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)b->lam_lds);
+            (void)hipFuncSetAttribute((const void *)k_glam_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)b->lam_lds);
+        }
         for (int k1 = 1; k1 <= K; ++k1)
             for (int k2 = 1; k2 <= k1; ++k2)
                 for (int k3 = 1; k3 <= k2; ++k3) ktk.push_back(k1 | (k2 << 10) | (k3 << 20));
@@ -1982,6 +2013,8 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         (void)hipFuncSetAttribute((const void *)k_c3sep<0, true, false, false, kSepWideQB>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lb);
         (void)hipFuncSetAttribute((const void *)k_c3sep<1, false, false, false, kSepWideQB>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lb);
+        (void)hipFuncSetAttribute((const void *)k_c3sep<0, true, true, false, kSepWideQB>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lb);
     }
     b->fold = b->use_sep && !b->lam && N % 2 == 0;
@@ -4141,7 +4174,9 @@ int tp_batch_create(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, cons
     *out = nullptr;
     if (int rc = check_ctx(ctx)) return rc;
     if (!b || !refs || !coeff || !ld || !seeds) return tp_fail(TP_E_ARG, "null argument");
-    if (b->lam) return tp_fail(TP_E_UNSUPPORTED, "batched runs need lambda dq = 0");
+    if (b->lam && b->prec != TP_PREC_FP16X3_T64)
+        return tp_fail(TP_E_UNSUPPORTED, "batched runs on the curved Ewald sphere (lambda dq != 0) need "
+                                         "TP_PREC_FP16X3_T64");
     if (b->ctx != ctx) return tp_fail(TP_E_STATE, "basis belongs to another context");
     if (ngroups < 1 || nprop < 3 || nprop > 4096 || hist_cap < 0)
         return tp_fail(TP_E_ARG, "need ngroups >= 1, 3 <= nprop <= 4096, hist_cap >= 0");
@@ -4369,7 +4404,7 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
     hipStream_t st = ctx->stream;
     if ((bm->prec != TP_PREC_FP16X3 && bm->prec != TP_PREC_BF16 && bm->prec != TP_PREC_FP16X3_T64) ||
         !bm->use_mfma || (bm->prec != TP_PREC_FP16X3_T64 && bm->R % 128 != 0) ||
-        (bm->prec == TP_PREC_FP16X3_T64 && (!bm->use_sym || !bm->use_sep)) || bm->lam) {
+        (bm->prec == TP_PREC_FP16X3_T64 && (!bm->use_sym || !(bm->use_sep || (bm->lam && bm->sep_wide))))) {
         // one evaluation after another on the device
         double *dE = nullptr;
         if (int rc = dev_alloc(&dE, n)) return rc;

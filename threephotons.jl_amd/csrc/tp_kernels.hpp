@@ -1313,7 +1313,7 @@ __global__ __launch_bounds__(512, QB > 6 ? 1 : 3) void k_c3sep(const double *__r
                                                const int *__restrict__ nvalid = nullptr)
 {
     static_assert(!FOLD || (ENERGY && OUT == 0), "the folded grid forms the energy only");
-    static_assert(QB == 6 || (QB == kSepWideQB && !FOLD && !BATCH), "wide instance: one evaluation, full grid");
+    static_assert(QB == 6 || (QB == kSepWideQB && !FOLD), "wide instance: the full grid");
     __shared__ __align__(16) double s_lt[ENERGY ? kLogTabSize : 1];
     __shared__ double red[8][2][16];
     extern __shared__ double cs_dyn[];
