@@ -514,15 +514,19 @@ def generate_leg(tp, truth, npic):
     return out
 
 
-def ewald_leg(cfg, tp, start, c3ref, reps):
+def ewald_leg(cfg, tp, start, c3ref, reps, workload=None, truth=None, batch_sweeps=0):
     """The plain FP64 evaluation on the curved Ewald sphere: the full-size
     problem (L = 18, K = 26, N = 32, the same start structure and histogram)
     with a basis at lambda = 2.5 (the reference's own GPU test value) and
     dq = pi / 38; synchronous c3 + energy evaluations from the host, stage
-    times with HIP events."""
+    times with HIP events.  With batch_sweeps > 0 also the batched form on
+    the same basis (FP16X3_T64): the 20 runs of the batched leg (K_r from
+    seeds 1000..1019, 64 proposals each) scored against the full histogram
+    cut to K_r shells, sweeps timed after one untimed sweep."""
     ctx = tp.HIP_init()
     lam, dq = 2.5, np.pi / 38
     b = tp.HIP_calculate_basis(cfg["L"], cfg["LMAX"], cfg["N"], cfg["K"], lam, dq)
+    c3ref_full = c3ref
     c3ref = b.histogram(c3ref)   # on the device before the timed region
     tp.energy(start, b, c3ref)
     t0 = time.perf_counter()
@@ -537,6 +541,36 @@ def ewald_leg(cfg, tp, start, c3ref, reps):
            "ms_per_eval": 1e3 * el / reps, "energy": E,
            "stage_ms": {"triple": st[0], "glam_G": st[1], "c3_energy": st[2], "final": st[3]}}
     log(f"curved Ewald sphere: {reps / el:.1f} evals/s ({1e3 * el / reps:.3f} ms), stage ms {np.round(st, 4)}")
+    # the same evaluation in FP16X3_T64 (mixed-precision triple products)
+    b.set_precision("fp16x3_t64")
+    E16 = tp.energy(start, b, c3ref)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        tp.energy(start, b, c3ref)
+    el16 = time.perf_counter() - t0
+    out["fp16x3_t64"] = {"evals_per_s": reps / el16, "energy_abs_err_vs_fp64": abs(E16 - E)}
+    if batch_sweeps > 0:
+        seeds = tuple(range(1000, 1020))
+        Ks = [int(np.random.default_rng(s).integers(10, 27)) for s in seeds]
+        starts = [workload.random_start_structure(truth, cfg["L"], s) for s in seeds]
+        refs = [np.ascontiguousarray(c3ref_full[:, :, :K, :K, :K]) for K in Ks]
+        nprop = 64
+        B = tp.AnnealBatch(b, refs, starts, seeds, nprop=nprop, hist_cap=batch_sweeps + 8)
+        B.probe(0.1)
+        E0 = [B.group(g)["E"] for g in range(len(Ks))]
+        B.run(1, factor=SWEEP_FACTOR)   # graph capture, untimed
+        t0 = time.perf_counter()
+        B.run(batch_sweeps, factor=SWEEP_FACTOR)
+        elb = time.perf_counter() - t0
+        ne = len(Ks) * nprop
+        out["batched_fp16x3_t64"] = {
+            "runs": len(Ks), "K": Ks, "proposals_per_run": nprop, "evals_per_sweep": ne, "sweeps": batch_sweeps,
+            "ms_per_sweep": 1e3 * elb / batch_sweeps, "evals_per_s": ne * batch_sweeps / elb,
+            "E0": E0, "E_final": [B.group(g)["E"] for g in range(len(Ks))]}
+        log(f"curved Ewald sphere, batched: {1e3 * elb / batch_sweeps:.2f} ms/sweep, "
+            f"{ne * batch_sweeps / elb:.0f} evals/s")
+        del B
+    b.set_precision("fp64")
     return out
 
 
@@ -779,6 +813,8 @@ def main():
     ap.add_argument("--gen-pictures", type=int, default=65536, help="pictures of the histogram-generation leg (0: skip)")
     ap.add_argument("--structure-reps", type=int, default=5, help="timed calls of the structure leg (0: skip)")
     ap.add_argument("--ewald-reps", type=int, default=50, help="timed evaluations at lambda = 2.5 (0: skip)")
+    ap.add_argument("--ewald-batch-sweeps", type=int, default=5,
+                    help="timed sweeps of the batched runs at lambda = 2.5 (0: skip)")
     ap.add_argument("--grad-profile", type=int, default=0,
                     help="instead of the benchmark: time the parts of a gradient move, then this many gradient calls")
     ap.add_argument("--precision-study", type=int, default=0,
@@ -893,7 +929,8 @@ def main():
     phase = phase_leg(tp, truth, args.sraar_iters) if args.sraar_iters > 0 else None
     gen = generate_leg(tp, truth, args.gen_pictures) if args.gen_pictures > 0 else None
     struct = structure_leg(tp, args.structure_reps) if args.structure_reps > 0 else None
-    ewald = ewald_leg(cfg, tp, start, c3ref, args.ewald_reps) if args.ewald_reps > 0 else None
+    ewald = (ewald_leg(cfg, tp, start, c3ref, args.ewald_reps, workload, truth, args.ewald_batch_sweeps)
+             if args.ewald_reps > 0 else None)
     plain = {"E_final": res["E_final"], "E0": res["E0"], "seconds": t_anneal, "E_trace": res["E"]}
     grad = None
     if args.grad_steps > 0:

@@ -45,7 +45,10 @@ inline size_t glam_lds_bytes(int nr, int K, int max_ent)
 // BATCH: the columns of a batch (As in FP32, as k_triple_sym_batch writes
 // it); 64-column block j holds the k-triples bkt0[j] .. bkt0[j] + nvalid[j]
 // - 1 of one evaluation (a run's k-triples are the first ones of the basis's
-// order, k1 >= k2 >= k3 with k1 outermost).
+// order, k1 >= k2 >= k3 with k1 outermost).  A batch has thousands of column
+// blocks, so its grid has the classes in x and the column blocks in y: the
+// workgroups running at one time share one column block of As, which every
+// XCD then holds in its L2, and As is read from HBM about once.
 template <bool BATCH, typename AT>
 __global__ __launch_bounds__(kLamMaxThreads) void k_glam_t(const AT *__restrict__ As, int64_t lda,
                                                            const int *__restrict__ ktk, int klength,
@@ -56,7 +59,8 @@ __global__ __launch_bounds__(kLamMaxThreads) void k_glam_t(const AT *__restrict_
                                                            const int *__restrict__ bkt0)
 {
     extern __shared__ double lam_sm[];
-    const int c = blockIdx.y, W = 2 * L + 1, nr = L / lstep + 1, nt = blockDim.x;
+    const int c = BATCH ? blockIdx.x : blockIdx.y, cblk = BATCH ? blockIdx.y : blockIdx.x;
+    const int W = 2 * L + 1, nr = L / lstep + 1, nt = blockDim.x;
     const int eb = cstart[c], n = cstart[c + 1] - eb;
     const int2 mm = cm[c];   // (m2, mu3)
     const int m1 = -(mm.x + mm.y);
@@ -69,7 +73,7 @@ __global__ __launch_bounds__(kLamMaxThreads) void k_glam_t(const AT *__restrict_
     }
     for (int i = threadIdx.x; i < n; i += nt) se[i] = ent[eb + i];
     __syncthreads();
-    const int col = blockIdx.x * nt + threadIdx.x;
+    const int col = cblk * nt + threadIdx.x;
     if (col >= ldg) return;   // no barriers below
     int kt = col;
     if constexpr (BATCH) {

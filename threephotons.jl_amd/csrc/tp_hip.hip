@@ -896,7 +896,7 @@ static void batch_contract_fac(const tp_batch *B, hipStream_t st, int cand)
     if (b->lam) {
         // curved Ewald sphere: G per column (k_glam, FP64), then c3 = Bm . G
         // on the separated grid (the wide instance for the odd-m classes)
-        hipLaunchKernelGGL(k_glam_batch, dim3((unsigned)((B->ldA + b->lam_nt - 1) / b->lam_nt), (unsigned)b->n_m),
+        hipLaunchKernelGGL(k_glam_batch, dim3((unsigned)b->n_m, (unsigned)((B->ldA + b->lam_nt - 1) / b->lam_nt)),
                            dim3((unsigned)b->lam_nt), b->lam_lds, st, B->d_As32, B->ldA, b->d_ktk, (int)b->klength,
                            b->d_lamY, b->L, b->lstep, b->K, b->d_lamcm, b->d_lament, b->d_lamcs, B->d_G, B->ldA,
                            B->d_nvalid, B->d_bkt0);
