@@ -66,9 +66,10 @@ void tp_finalize(tp_ctx *ctx);
  * Legendre factors are taken at theta_k = acos(k lambda dq / (4 pi))
  * (correlations.jl:94), which needs |K lambda dq| <= 4 pi; such a basis
  * evaluates in FP64 or TP_PREC_FP16X3_T64 (mixed-precision triple products,
- * G and c3 in FP64) through tp_c3, tp_energy, tp_state_*, tp_anneal and
- * tp_energy_batch; the other mixed modes, batched runs, the energy gradient
- * and tp_two_photons (use tp_two_photons_ewald) return TP_E_UNSUPPORTED for it. */
+ * G and c3 in FP64) through tp_c3, tp_energy, tp_state_*, tp_anneal,
+ * tp_energy_batch, tp_energy_grad (FP64) and the batched runs (tp_batch_*,
+ * TP_PREC_FP16X3_T64); the other mixed modes and tp_two_photons (use
+ * tp_two_photons_ewald) return TP_E_UNSUPPORTED for it. */
 int tp_basis_create(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda, double dq,
                     int for_intensity, tp_basis **out);
 /* The same from a basis the caller already holds: wignerlist[basislen] and
@@ -270,7 +271,8 @@ int tp_energy_batch(tp_ctx *ctx, int n, const tp_basis *const *bases, const tp_r
  * coeff.  No counterpart in the reference, which evaluates energy()
  * (correlations.jl:201-226) only; it is that function's chain rule through
  * the contraction and the triple products (:165-193).  Needs N <= 32, L <= 22
- * and K <= 27 (TP_E_UNSUPPORTED otherwise). */
+ * and K <= 27, and on the curved Ewald sphere L <= 19 (TP_E_UNSUPPORTED
+ * otherwise). */
 int tp_energy_grad(tp_ctx *ctx, const tp_basis *basis, const tp_ref *ref, const double *coeff, int64_t ld,
                    double *E_out, double *grad_out);
 

@@ -89,6 +89,51 @@ def test_energy_grad_rotation_fd(tp):
         assert an == pytest.approx(fd, rel=1e-5, abs=1e-7 * abs(E))
 
 
+@pytest.mark.parametrize("L,LMAX,N,K,lam,dq", [
+    (8, 10, 8, 6, 2.5, 0.3),      # separated grid with odd m
+    (14, 16, 16, 6, 1.5, 0.3),    # the wide layout (> 24 classes per m2), two j blocks
+    (18, 19, 32, 5, 2.5, 0.08),   # the wide layout at L = 18, N = 32
+])
+def test_energy_grad_curved_sphere_fd(tp, L, LMAX, N, K, lam, dq):
+    """The gradient on the curved Ewald sphere (lambda dq != 0: the contraction's
+    backward in the odd-m layout, k_glam run backwards) against central
+    differences of the GPU energy (pinned to the oracle's dense form at
+    lambda dq != 0 in test_gpu_ewald.py; the oracle's chain rule covers lambda
+    = 0 only): along rotations of l-blocks and along a random direction with
+    the real-intensity structure.  The energy returned equals tp.energy."""
+    from scipy.linalg import expm
+    from tp_amd import workload
+    say(f"case L={L} N={N} K={K} lambda={lam} dq={dq}")
+    c = random_real_coeff(K, L, LMAX, 5, c00=6.0, decay=0.05)
+    other = random_real_coeff(K, L, LMAX, 6, c00=6.0, decay=0.05)
+    b = tp.HIP_calculate_basis(L, LMAX, N, K, lam, dq)
+    p = np.maximum(tp.FullCorrelation_parallized(other, b, return_raw=True), 1e-30)
+    ref = c3ref_from_columns(p / p.sum(), N, K)
+    E, g = tp.energy_grad(c, b, ref)
+    assert E == tp.energy(c, b, ref)
+    assert np.abs(g).max() > 0
+    rng = np.random.default_rng(2)
+    eps = 1e-5
+    for l in (2, L):
+        n = 2 * l + 1
+        A = rng.normal(size=(n, n))
+        A = A - A.T
+
+        def rot(t):
+            return workload.rotate_l(c, l, expm(t * A), LMAX)
+
+        fd = (tp.energy(rot(eps), b, ref) - tp.energy(rot(-eps), b, ref)) / (2 * eps)
+        d = (rot(eps) - rot(-eps)) / (2 * eps)
+        an = float(np.real(np.vdot(g, d)))
+        say(f"  rotation l={l}: analytic {an:.10g}  fd {fd:.10g}")
+        assert an == pytest.approx(fd, rel=1e-5, abs=1e-7 * abs(E))
+    d = random_real_coeff(K, L, LMAX, 11, c00=1.0, decay=0.05)
+    fd = (tp.energy(c + eps * d, b, ref) - tp.energy(c - eps * d, b, ref)) / (2 * eps)
+    an = float(np.real(np.vdot(g, d)))
+    say(f"  random direction: analytic {an:.10g}  fd {fd:.10g}")
+    assert an == pytest.approx(fd, rel=1e-5, abs=1e-7 * abs(E))
+
+
 def test_gradient_moves_beat_random_walk(tp):
     """Gradient-guided moves (tp_amd.descent) only ever lower the energy at
     T = 0 and, for the same number of energy evaluations (a gradient counted

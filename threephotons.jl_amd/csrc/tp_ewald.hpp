@@ -16,32 +16,38 @@
 
 constexpr int kLamMaxThreads = 1024;
 
-// One entry of a class: its folded 3j weight, the l-triple (row of As), the
-// rows l1 / lstep, l2 / lstep, l3 / lstep of the class's staged sphPlm values,
-// and two flags: 1 when l3 differs from the previous entry's, 2 when (l2, l3)
-// does (the first entry of a class carries both).
+// One term of an output row (forward: a class of G; backward: an l-triple of
+// dE/dAs): its folded 3j weight, the input row (forward: the l-triple, a row
+// of As; backward: the class, a row of dE/dG), the positions r1, r2, r3 of
+// its three sphPlm rows (k1, k2 and k3 factors) among the rows staged for
+// the output row, and two flags: 1 when r3 differs from the previous
+// entry's, 2 when (r2, r3) does (the first entry of a row carries both).
 struct LamRec {
     double hw;
-    uint32_t a;   // lt | flags << 16
+    uint32_t a;   // input row | flags << 16
     uint32_t b;   // r1 | r2 << 8 | r3 << 16
 };
 
-// Dynamic LDS of k_glam: the class's sphPlm rows [3][nr][K] and its entries.
-inline size_t glam_lds_bytes(int nr, int K, int max_ent)
+// Dynamic LDS of k_glam: an output row's staged sphPlm rows and its entries.
+inline size_t glam_lds_bytes(int max_rows, int K, int max_ent)
 {
-    return sizeof(double) * (size_t)((3 * nr * K + 1) & ~1) + sizeof(LamRec) * (size_t)max_ent;
+    return sizeof(double) * (size_t)((max_rows * K + 1) & ~1) + sizeof(LamRec) * (size_t)max_ent;
 }
 
-// One workgroup per (block of blockDim.x columns, class c): thread -> column
-// (columns past klength, and in a batch the columns past a block's nvalid,
-// get zeros).  The class fixes m1 = -(m2 + mu3), m2 and mu3, so the sphPlm
-// factors it needs are the rows (l, m1), (l, m2), (l, mu3) for l = 0, lstep,
-// ..., L of every shell: they and the class's entries are staged in LDS, and
-// the only global load per entry is the As value.
-// Y[(l (2L+1) + m + L) K + k - 1] = sphPlm(l, m, theta_k) at K shells;
-// ktk[kt] = k1 | k2 << 10 | k3 << 20.  With the blocks in x and the classes
-// in y, workgroup i runs on XCD i % 8, so at 4 column blocks every XCD reads
-// one block of As (3.4 MB at L = 18, K = 26: it stays in that XCD's L2).
+// One workgroup per (block of blockDim.x columns, output row o): thread ->
+// column (columns past klength, and in a batch the columns past a block's
+// nvalid, get zeros).
+//   Out[o][kt] = sum over entries e of o: hw_e Y[k1][r1_e] Y[k2][r2_e] Y[k3][r3_e] In[in_e][kt].
+// Forward (G = H_kt As): o is a class, which fixes m1 = -(m2 + mu3), m2 and
+// mu3, so its sphPlm rows are (l, m1), (l, m2), (l, mu3) for l = 0, lstep,
+// ..., L.  Backward (dE/dAs = H_kt^T dE/dG): o is an l-triple, whose rows are
+// (l1, m), (l2, m), (l3, mu) for every m.  The rows (yrows[ystart[o]] ..,
+// indices y = l (2L+1) + m + L of Y[y K + k - 1] = sphPlm(l, m, theta_k))
+// and the row's entries are staged in LDS, so the only global load per
+// entry is the In value.  ktk[kt] = k1 | k2 << 10 | k3 << 20.  With the
+// blocks in x and the rows in y, workgroup i runs on XCD i % 8, so at 4
+// column blocks every XCD reads one block of In (3.4 MB of As at L = 18,
+// K = 26: it stays in that XCD's L2).
 // BATCH: the columns of a batch (As in FP32, as k_triple_sym_batch writes
 // it); 64-column block j holds the k-triples bkt0[j] .. bkt0[j] + nvalid[j]
 // - 1 of one evaluation (a run's k-triples are the first ones of the basis's
@@ -52,25 +58,20 @@ inline size_t glam_lds_bytes(int nr, int K, int max_ent)
 template <bool BATCH, typename AT>
 __global__ __launch_bounds__(kLamMaxThreads) void k_glam_t(const AT *__restrict__ As, int64_t lda,
                                                            const int *__restrict__ ktk, int klength,
-                                                           const double *__restrict__ Y, int L, int lstep, int K,
-                                                           const int2 *__restrict__ cm, const LamRec *__restrict__ ent,
+                                                           const double *__restrict__ Y, int K,
+                                                           const int *__restrict__ yrows, const int *__restrict__ ystart,
+                                                           int max_rows, const LamRec *__restrict__ ent,
                                                            const int *__restrict__ cstart, double *__restrict__ G,
                                                            int64_t ldg, const int *__restrict__ nvalid,
                                                            const int *__restrict__ bkt0)
 {
     extern __shared__ double lam_sm[];
-    const int c = BATCH ? blockIdx.x : blockIdx.y, cblk = BATCH ? blockIdx.y : blockIdx.x;
-    const int W = 2 * L + 1, nr = L / lstep + 1, nt = blockDim.x;
+    const int c = BATCH ? blockIdx.x : blockIdx.y, cblk = BATCH ? blockIdx.y : blockIdx.x, nt = blockDim.x;
     const int eb = cstart[c], n = cstart[c + 1] - eb;
-    const int2 mm = cm[c];   // (m2, mu3)
-    const int m1 = -(mm.x + mm.y);
-    double *Ys = lam_sm;   // [3][nr][K]: rows of m1, m2, mu3
-    LamRec *se = reinterpret_cast<LamRec *>(lam_sm + ((3 * nr * K + 1) & ~1));
-    for (int i = threadIdx.x; i < 3 * nr * K; i += nt) {
-        const int w = i / (nr * K), r = (i / K) % nr, k = i % K, l = r * lstep;
-        const int m = w == 0 ? m1 : (w == 1 ? mm.x : mm.y);
-        Ys[i] = (m >= -l && m <= l) ? Y[(size_t)(l * W + m + L) * K + k] : 0.0;
-    }
+    const int y0 = ystart[c], ny = ystart[c + 1] - y0;
+    double *Ys = lam_sm;   // [ny][K]
+    LamRec *se = reinterpret_cast<LamRec *>(lam_sm + ((max_rows * K + 1) & ~1));
+    for (int i = threadIdx.x; i < ny * K; i += nt) Ys[i] = Y[(size_t)yrows[y0 + i / K] * K + i % K];
     for (int i = threadIdx.x; i < n; i += nt) se[i] = ent[eb + i];
     __syncthreads();
     const int col = cblk * nt + threadIdx.x;
@@ -88,8 +89,8 @@ __global__ __launch_bounds__(kLamMaxThreads) void k_glam_t(const AT *__restrict_
     }
     const int p = ktk[kt];
     const double *Y1 = Ys + ((p & 1023) - 1);
-    const double *Y2 = Ys + nr * K + (((p >> 10) & 1023) - 1);
-    const double *Y3 = Ys + 2 * nr * K + (((p >> 20) & 1023) - 1);
+    const double *Y2 = Ys + (((p >> 10) & 1023) - 1);
+    const double *Y3 = Ys + (((p >> 20) & 1023) - 1);
     const AT *Ak = As + col;
     // entries run sorted by (l3, l2, l1): the k3 factor and the (k2, k3)
     // product are reloaded only when l3 or (l2, l3) changes; two
@@ -118,3 +119,15 @@ __global__ __launch_bounds__(kLamMaxThreads) void k_glam_t(const AT *__restrict_
 
 #define k_glam (k_glam_t<false, double>)
 #define k_glam_batch (k_glam_t<true, float>)
+
+// out[i] = sum_b in[b blk_stride + i], i < n, blocks added in order: the njc
+// partial row blocks of dE/dG that the backward contraction leaves.
+__global__ __launch_bounds__(256) void k_sum_blocks(const double *__restrict__ in, int nblk, int64_t blk_stride,
+                                                    int64_t n, double *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += in[b * blk_stride + i];
+        out[i] = s;
+    }
+}

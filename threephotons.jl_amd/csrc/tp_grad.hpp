@@ -6,7 +6,9 @@
 //   D[r,kt]   = dE/dc3[r,kt] = tF (qsum / S - qhat_r / c3_r)  where c3_r > 1e-30
 //               (0 where the clamp max(c3, 1e-30) is active), S = sum_r max(c3_r, 1e-30)
 //   gG        = Bm^T D   (k_c3sep_bwd: the separated grid run backwards)
-//   gAs       = H^T gG   (FP64 MFMA GEMM, k_gemm)
+//   gAs       = H^T gG   (FP64 MFMA GEMM, k_gemm; on the curved Ewald sphere
+//                         H depends on the k-triple: k_glam with the terms
+//                         grouped by l-triple, after the row blocks of gG are summed)
 //   gamma_k   = dE/dRe c_k + i dE/dIm c_k  (k_triple_bwd + k_grad_reduce)
 //
 // With As[lt,kt] = Re sum_mu3 T_{k1k2}[lt,mu3] c_k3[l3,mu3] and
@@ -32,7 +34,11 @@
 // cross-wave sum.  The partial over this workgroup's 16 values of j goes to
 // row block jc of gGp; the GEMM that follows sums the njc blocks.
 // LDS: CS / PS [kb][16][17] doubles (dynamic, c3sep_lds_bytes(kb)).
+// QB = kSepWideQB: the wide layout of k_c3sep (odd-m classes of the curved
+// Ewald sphere: classes per m2 in chunks of 24, kb <= 40, three 16-row
+// blocks of PS).
 // ---------------------------------------------------------------------------
+template <int QB = 6>
 __global__ __launch_bounds__(512) void k_c3sep_bwd(const double *__restrict__ G, int64_t ldg,
                                                    const int *__restrict__ m2start, int n_m2,
                                                    const double *__restrict__ TA, int64_t ta_plane, int jpad,
@@ -58,20 +64,23 @@ __global__ __launch_bounds__(512) void k_c3sep_bwd(const double *__restrict__ G,
     // ---- forward stage A (as k_c3sep)
     for (int b = wave; b < 2 * n_m2; b += NW) {
         const int m2i = b >> 1, cs = b & 1;
-        const int cb = m2start[m2i], ce = m2start[m2i + 1];
+        const int cb0 = m2start[m2i], ce = m2start[m2i + 1];
         const double *ta = TA + (int64_t)cs * ta_plane;
-        double av[6], gv[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const int cls = cb + 4 * q + lq;
-            const bool ok = cls < ce;
-            av[q] = ok ? ta[(int64_t)cls * jpad + j0 + l16] : 0.0;
-            gv[q] = ok ? G[(int64_t)cls * ldg + c0 + l16] : 0.0;
-        }
         dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int cb = cb0; cb < ce; cb += 24) {
+            double av[6], gv[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-            if (cb + 4 * q < ce) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], gv[q], acc, 0, 0, 0);
+            for (int q = 0; q < 6; ++q) {
+                const int cls = cb + 4 * q + lq;
+                const bool ok = cls < ce;
+                av[q] = ok ? ta[(int64_t)cls * jpad + j0 + l16] : 0.0;
+                gv[q] = ok ? G[(int64_t)cls * ldg + c0 + l16] : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                if (cb + 4 * q < ce) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], gv[q], acc, 0, 0, 0);
+            if constexpr (QB == 6) break;   // at most 24 classes per m2
+        }
 #pragma unroll
         for (int v = 0; v < 4; ++v) CS(b, lq + 4 * v, l16) = acc[v];
     }
@@ -94,7 +103,7 @@ __global__ __launch_bounds__(512) void k_c3sep_bwd(const double *__restrict__ G,
             dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
             if (ib < nib) {
 #pragma unroll
-                for (int q = 0; q < 6; ++q)
+                for (int q = 0; q < QB; ++q)
                     if (4 * q < kb) {
                         const int kk = 4 * q + lq;
                         const double a = TB[(int64_t)(16 * ib + l16) * kb + min(kk, kb - 1)];
@@ -120,7 +129,7 @@ __global__ __launch_bounds__(512) void k_c3sep_bwd(const double *__restrict__ G,
     for (int js = 0; js < 2; ++js) {
         const int jl = wave + NW * js;
 #pragma unroll
-        for (int kblk = 0; kblk < 2; ++kblk) {
+        for (int kblk = 0; kblk < (4 * QB + 15) / 16; ++kblk) {
             dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int ib = 0; ib < 2; ++ib) {

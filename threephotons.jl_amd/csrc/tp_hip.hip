@@ -131,8 +131,18 @@ struct tp_basis {
     int *d_ktk = nullptr;          // [klength] k1 | k2 << 10 | k3 << 20
     LamRec *d_lament = nullptr;    // entries of every class, class-major
     int *d_lamcs = nullptr;        // [n_m + 1] first entry of each class
-    int2 *d_lamcm = nullptr;       // [n_m] (m2, mu3) of each class
+    int *d_lamyr = nullptr;        // sphPlm rows staged per class: (l, m1), (l, m2), (l, mu3) for every l
+    int *d_lamys = nullptr;        // [n_m + 1] first staged row of each class
+    int lam_rows = 0;              // most rows staged for one class
     size_t lam_lds = 0;            // k_glam's dynamic LDS
+    // the gradient: the same terms grouped by l-triple (dE/dAs = H_kt^T dE/dG)
+    LamRec *d_lambent = nullptr;   // entries of every l-triple, l-triple-major
+    int *d_lambcs = nullptr;       // [n_lt + 1] first entry of each l-triple
+    int *d_lambyr = nullptr;       // sphPlm rows staged per l-triple: (l1, m), (l2, m), (l3, mu) for every m
+    int *d_lambys = nullptr;       // [n_lt + 1] first staged row of each l-triple
+    int lamb_rows = 0;
+    size_t lamb_lds = 0;
+    double *d_gG = nullptr;        // [n_m][kl_pad] dE/dG (the sum of the backward contraction's row blocks)
     int lam_nt = 0;                // k_glam's workgroup size (columns per workgroup)
     int64_t R_pad = 0, nlt_pad = 0, kl_pad = 0, nm_pad64 = 0, nm_pad16 = 0;
     int n_rt = 0, n_ct = 0;
@@ -374,7 +384,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         // curved Ewald sphere: the class weights depend on the k-triple
         hipLaunchKernelGGL(k_glam, dim3((unsigned)((b->kl_pad + b->lam_nt - 1) / b->lam_nt), (unsigned)b->n_m),
                            dim3((unsigned)b->lam_nt), b->lam_lds, st, sc.As, b->kl_pad, b->d_ktk, (int)b->klength,
-                           b->d_lamY, b->L, b->lstep, b->K, b->d_lamcm, b->d_lament, b->d_lamcs, sc.G, b->kl_pad,
+                           b->d_lamY, b->K, b->d_lamyr, b->d_lamys, b->lam_rows, b->d_lament, b->d_lamcs, sc.G,
+                           b->kl_pad,
                            nullptr, nullptr);
     } else if (prec == TP_PREC_FP16X3_T64) {
     // G = H . As on FP16 hi/lo MFMA operands (As split while staged), FP64 out
@@ -898,7 +909,8 @@ static void batch_contract_fac(const tp_batch *B, hipStream_t st, int cand)
         // on the separated grid (the wide instance for the odd-m classes)
         hipLaunchKernelGGL(k_glam_batch, dim3((unsigned)b->n_m, (unsigned)((B->ldA + b->lam_nt - 1) / b->lam_nt)),
                            dim3((unsigned)b->lam_nt), b->lam_lds, st, B->d_As32, B->ldA, b->d_ktk, (int)b->klength,
-                           b->d_lamY, b->L, b->lstep, b->K, b->d_lamcm, b->d_lament, b->d_lamcs, B->d_G, B->ldA,
+                           b->d_lamY, b->K, b->d_lamyr, b->d_lamys, b->lam_rows, b->d_lament, b->d_lamcs, B->d_G,
+                           B->ldA,
                            B->d_nvalid, B->d_bkt0);
         const int64_t tap = (int64_t)b->n_m * b->jpad;
         const dim3 grid((unsigned)(B->ldA / 16), (unsigned)b->njc);
@@ -1483,7 +1495,8 @@ void tp_basis_destroy(tp_basis *b)
                     b->d_Fbf, b->d_Ahi, b->d_Alo, b->d_Ainv, b->d_shsc, b->d_ew8, b->d_eidx8, b->d_gstart8,
                     b->d_m2start, b->d_TA, b->d_TB, b->d_logtab, b->d_HT4, b->d_gGp, b->d_gA, b->d_gpart,
                     b->d_grad, b->d_lstart, b->d_llist, b->d_bwd_order, b->d_c2coef, b->d_c2raw, b->d_c2,
-                    b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs, b->d_lamcm, b->d_sym_work, b->d_perms, b->d_gorder,
+                    b->d_lamY, b->d_ktk, b->d_lament, b->d_lamcs, b->d_lamyr, b->d_lamys,
+                    b->d_lambent, b->d_lambcs, b->d_lambyr, b->d_lambys, b->d_gG, b->d_sym_work, b->d_perms, b->d_gorder,
                     b->d_symtab, b->d_symoff, b->d_Hhi, b->d_Hlo, b->d_nrm2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1649,7 +1662,8 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
     };
     std::vector<LamEnt> lament;
     std::vector<LamRec> lamrec;
-    std::vector<int2> lamcm;
+    std::vector<int> lamyr, lamys, lambcs, lambyr, lambys;
+    std::vector<LamRec> lambrec;
     std::vector<int> lamcs, ktk;
     std::vector<double> lamY;
     if (b->lam) {
@@ -1692,23 +1706,74 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
             }
         }
         lamcs.push_back((int)lament.size());
-        // packed: rows l / lstep of the class's staged sphPlm values
+        // packed, forward: class c stages the rows (l, m1), (l, m2), (l, mu3)
+        // for l = 0, lstep, ..., L (nr rows each; a row with |m| > l is zero)
+        const int nr = L / b->lstep + 1;
         int max_ent = 0;
         for (int c = 0; c < b->n_m; ++c) {
             max_ent = std::max(max_ent, lamcs[c + 1] - lamcs[c]);
-            lamcm.push_back(make_int2(cls_m2[c], cls_mu3[c]));
+            lamys.push_back((int)lamyr.size());
+            const int ms[3] = {-(cls_m2[c] + cls_mu3[c]), cls_m2[c], cls_mu3[c]};
+            for (int w = 0; w < 3; ++w)
+                for (int l = 0; l <= L; l += b->lstep) lamyr.push_back(l * W + ms[w] + L);
         }
-        const auto row = [&](int y) { return (uint32_t)(y / K / W / b->lstep); };
+        lamys.push_back((int)lamyr.size());
+        b->lam_rows = 3 * nr;
+        const auto lof = [&](int y) { return y / K / W; };          // l of a table row y K
+        const auto mof = [&](int y) { return y / K % W - L; };      // m of a table row y K
         for (const LamEnt &x : lament)
             lamrec.push_back(LamRec{x.hw, (uint32_t)x.lt | ((uint32_t)x.fl << 16),
-                                    row(x.y1) | (row(x.y2) << 8) | (row(x.y3) << 16)});
-        b->lam_lds = glam_lds_bytes(L / b->lstep + 1, K, max_ent);
+                                    (uint32_t)(lof(x.y1) / b->lstep) | ((uint32_t)(nr + lof(x.y2) / b->lstep) << 8) |
+                                        ((uint32_t)(2 * nr + lof(x.y3) / b->lstep) << 16)});
+        b->lam_lds = glam_lds_bytes(b->lam_rows, K, max_ent);
+        // backward: l-triple t stages (l1, m), (l2, m), (l3, mu) for every m;
+        // its terms are the forward ones of every class, sorted by (r3, r2, r1)
+        std::vector<std::vector<std::pair<int, LamRec>>> byt(nlt);
+        for (int c = 0; c < b->n_m; ++c)
+            for (int e = lamcs[c]; e < lamcs[c + 1]; ++e) {
+                const LamEnt &x = lament[e];
+                const int l1 = lof(x.y1), l2 = lof(x.y2), l3 = lof(x.y3);
+                const uint32_t r1 = (uint32_t)(mof(x.y1) + l1), r2 = (uint32_t)(2 * l1 + 1 + mof(x.y2) + l2),
+                               r3 = (uint32_t)(2 * l1 + 1 + 2 * l2 + 1 + mof(x.y3) + l3);
+                byt[x.lt].push_back({0, LamRec{x.hw, (uint32_t)c, r1 | (r2 << 8) | (r3 << 16)}});
+            }
+        int maxb_ent = 0;
+        for (int t = 0; t < nlt; ++t) {
+            const int l3 = ltinfo[t].z, l1 = ltinfo[t].w & 255, l2 = ltinfo[t].w >> 8;
+            lambys.push_back((int)lambyr.size());
+            for (int m = -l1; m <= l1; ++m) lambyr.push_back(l1 * W + m + L);
+            for (int m = -l2; m <= l2; ++m) lambyr.push_back(l2 * W + m + L);
+            for (int m = -l3; m <= l3; ++m) lambyr.push_back(l3 * W + m + L);
+            b->lamb_rows = std::max(b->lamb_rows, 2 * (l1 + l2 + l3) + 3);
+            auto &v = byt[t];
+            std::sort(v.begin(), v.end(), [](const std::pair<int, LamRec> &x, const std::pair<int, LamRec> &y) {
+                const uint32_t a3 = x.second.b >> 16, b3 = y.second.b >> 16;
+                const uint32_t a2 = (x.second.b >> 8) & 255u, b2 = (y.second.b >> 8) & 255u;
+                if (a3 != b3) return a3 < b3;
+                if (a2 != b2) return a2 < b2;
+                return (x.second.b & 255u) != (y.second.b & 255u) ? (x.second.b & 255u) < (y.second.b & 255u)
+                                                                  : x.second.a < y.second.a;
+            });
+            lambcs.push_back((int)lambrec.size());
+            for (size_t i = 0; i < v.size(); ++i) {
+                LamRec rrec = v[i].second;
+                const bool n3 = i == 0 || (rrec.b >> 16) != (v[i - 1].second.b >> 16);
+                const bool n2 = n3 || ((rrec.b >> 8) & 255u) != ((v[i - 1].second.b >> 8) & 255u);
+                rrec.a |= (uint32_t)((n3 ? 1 : 0) | (n2 ? 2 : 0)) << 16;
+                lambrec.push_back(rrec);
+            }
+            maxb_ent = std::max(maxb_ent, (int)v.size());
+        }
+        lambcs.push_back((int)lambrec.size());
+        lambys.push_back((int)lambyr.size());
+        b->lamb_lds = glam_lds_bytes(b->lamb_rows, K, maxb_ent);
         if (b->lam_lds > 65536) {
-            (void)hipFuncSetAttribute((const void *)k_glam, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)b->lam_lds);
             (void)hipFuncSetAttribute((const void *)k_glam_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)b->lam_lds);
         }
+        if (std::max(b->lam_lds, b->lamb_lds) > 65536)
+            (void)hipFuncSetAttribute((const void *)k_glam, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)std::max(b->lam_lds, b->lamb_lds));
         for (int k1 = 1; k1 <= K; ++k1)
             for (int k2 = 1; k2 <= k1; ++k2)
                 for (int k3 = 1; k3 <= k2; ++k3) ktk.push_back(k1 | (k2 << 10) | (k3 << 20));
@@ -2071,7 +2136,9 @@ static int basis_build(tp_ctx *ctx, int L, int LMAX, int N, int K, double lambda
         return rc;
     }
     if (b->lam && ((rc = up(&b->d_lamY, lamY)) || (rc = up(&b->d_ktk, ktk)) || (rc = up(&b->d_lament, lamrec)) ||
-                   (rc = up(&b->d_lamcs, lamcs)) || (rc = up(&b->d_lamcm, lamcm)))) {
+                   (rc = up(&b->d_lamcs, lamcs)) || (rc = up(&b->d_lamyr, lamyr)) || (rc = up(&b->d_lamys, lamys)) ||
+                   (rc = up(&b->d_lambent, lambrec)) || (rc = up(&b->d_lambcs, lambcs)) ||
+                   (rc = up(&b->d_lambyr, lambyr)) || (rc = up(&b->d_lambys, lambys)))) {
         tp_basis_destroy(b);
         return rc;
     }
@@ -2415,7 +2482,7 @@ int tp_energy_noalloc(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const do
 static int grad_setup(tp_ctx *ctx, tp_basis *b)
 {
     if (b->grad_ready) return 0;
-    if (!b->use_sep || !b->use_mfma)
+    if (!(b->use_sep || (b->lam && b->sep_wide)) || !b->use_mfma)
         return tp_fail(TP_E_UNSUPPORTED, "the energy gradient needs N <= 32, L <= 22 and K <= 27");
     b->gLw = 2 * b->L + 1;
     b->gEmax = b->EmaxLt;
@@ -2430,11 +2497,17 @@ static int grad_setup(tp_ctx *ctx, tp_basis *b)
                                    (int)b->bwd_lds));
     b->nlt_pad64 = (int)round_up(b->n_lt, 64);
     const int nlt = (int)b->n_lt, nmp = (int)b->nm_pad64, njc = b->njc, K = b->K;
-    std::vector<double> HT4((size_t)njc * nmp * b->nlt_pad64, 0.0);
-    for (int jc = 0; jc < njc; ++jc)
-        for (int c = 0; c < nmp; ++c)
-            for (int t = 0; t < nlt; ++t)
-                HT4[((size_t)jc * nmp + c) * b->nlt_pad64 + t] = b->h_H[(size_t)t * b->nm_pad64 + c];
+    // H^T for each of the njc row blocks of dE/dG (lambda dq = 0; on the
+    // curved Ewald sphere the blocks are summed and k_glam runs backwards)
+    std::vector<double> HT4(b->lam ? 0 : (size_t)njc * nmp * b->nlt_pad64, 0.0);
+    if (!b->lam)
+        for (int jc = 0; jc < njc; ++jc)
+            for (int c = 0; c < nmp; ++c)
+                for (int t = 0; t < nlt; ++t)
+                    HT4[((size_t)jc * nmp + c) * b->nlt_pad64 + t] = b->h_H[(size_t)t * b->nm_pad64 + c];
+    if (b->lam && c3sep_lds_bytes(b->sep_kb) > 65536)
+        TP_HIP(hipFuncSetAttribute((const void *)k_c3sep_bwd<kSepWideQB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)c3sep_lds_bytes(b->sep_kb)));
     // reduction lists: for each even l, the (l-triple, slot) blocks whose slot has order l
     const int nl = b->L / 2 + 1;
     std::vector<int> lstart(nl + 1, 0), llist;
@@ -2459,13 +2532,17 @@ static int grad_setup(tp_ctx *ctx, tp_basis *b)
     std::vector<int> order(nlt);
     for (int t = 0; t < nlt; ++t) order[t] = cost[t].second;
     int rc = 0;
-    if ((rc = dev_alloc(&b->d_HT4, HT4.size())) || (rc = dev_alloc(&b->d_gGp, (size_t)njc * nmp * b->kl_pad)) ||
+    if ((!b->lam && (rc = dev_alloc(&b->d_HT4, HT4.size()))) ||
+        (b->lam && (rc = dev_alloc(&b->d_gG, (size_t)b->n_m * b->kl_pad))) ||
+        (rc = dev_alloc(&b->d_gGp, (size_t)njc * nmp * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_gA, (size_t)b->nlt_pad64 * b->kl_pad)) ||
         (rc = dev_alloc(&b->d_gpart, (size_t)nlt * 3 * K * b->gLw)) || (rc = dev_alloc(&b->d_grad, (size_t)K * b->n_c)) ||
         (rc = dev_alloc(&b->d_lstart, lstart.size())) || (rc = dev_alloc(&b->d_bwd_order, order.size())) ||
         (rc = dev_alloc(&b->d_llist, llist.size())))
         return rc;
-    TP_HIP(hipMemcpy(b->d_HT4, HT4.data(), sizeof(double) * HT4.size(), hipMemcpyHostToDevice));
+    if (!b->lam) TP_HIP(hipMemcpy(b->d_HT4, HT4.data(), sizeof(double) * HT4.size(), hipMemcpyHostToDevice));
+    // rows of dE/dAs past the l-triples (read by no kernel; k_glam writes n_lt rows) stay 0
+    TP_HIP(hipMemset(b->d_gA, 0, sizeof(double) * (size_t)b->nlt_pad64 * b->kl_pad));
     TP_HIP(hipMemset(b->d_gGp, 0, sizeof(double) * (size_t)njc * nmp * b->kl_pad));   // rows / columns never written stay 0
     TP_HIP(hipMemcpy(b->d_lstart, lstart.data(), sizeof(int) * lstart.size(), hipMemcpyHostToDevice));
     TP_HIP(hipMemcpy(b->d_bwd_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice));
@@ -2483,16 +2560,35 @@ static int enqueue_grad(tp_ctx *ctx, tp_basis *b, const tp_ref *r, const double2
     const int64_t tap = (int64_t)b->n_m * b->jpad;
     // the forward left its partial sums in njf rows (folded grid) or njc
     const int fwd_rows = b->fold && r->d_qfold ? b->njf : b->njc;
-    hipLaunchKernelGGL(k_c3sep_bwd, dim3((unsigned)((ncols + 15) / 16), (unsigned)b->njc), dim3(512),
-                       c3sep_lds_bytes(b->sep_kb), st, b->d_G, b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad,
-                       b->d_TB, b->sep_kb, b->N, ncols, r->d_qhat, r->d_qsum, r->d_tf, b->d_part, b->kl_pad, fwd_rows,
-                       b->n_m, b->d_gGp, b->kl_pad, (int)b->nm_pad64);
+    const dim3 gbw((unsigned)((ncols + 15) / 16), (unsigned)b->njc);
+    if (b->use_sep)
+        hipLaunchKernelGGL(k_c3sep_bwd<6>, gbw, dim3(512), c3sep_lds_bytes(b->sep_kb), st, b->d_G, b->kl_pad,
+                           b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols, r->d_qhat,
+                           r->d_qsum, r->d_tf, b->d_part, b->kl_pad, fwd_rows, b->n_m, b->d_gGp, b->kl_pad,
+                           (int)b->nm_pad64);
+    else   // the wide layout (odd-m classes, lambda dq != 0)
+        hipLaunchKernelGGL(k_c3sep_bwd<kSepWideQB>, gbw, dim3(512), c3sep_lds_bytes(b->sep_kb), st, b->d_G,
+                           b->kl_pad, b->d_m2start, b->n_m2, b->d_TA, tap, b->jpad, b->d_TB, b->sep_kb, b->N, ncols,
+                           r->d_qhat, r->d_qsum, r->d_tf, b->d_part, b->kl_pad, fwd_rows, b->n_m, b->d_gGp,
+                           b->kl_pad, (int)b->nm_pad64);
     TP_HIP(hipGetLastError());
-    const int n_rt = b->nlt_pad64 / G1_BM, n_ct = (int)(b->kl_pad / G1_BN);
-    hipLaunchKernelGGL((k_gemm<G1_BM, G1_BN, G1_WGM, 2, false>), dim3((unsigned)(n_rt * n_ct)), dim3(256), 0, st,
-                       b->d_HT4, (int64_t)b->nlt_pad64, b->d_gGp, b->kl_pad, (int)(b->njc * b->nm_pad64),
-                       b->nlt_pad64, (int)b->kl_pad, n_rt, n_ct, b->d_gA, b->kl_pad, nullptr, nullptr, (int64_t)0,
-                       nullptr);
+    if (b->lam) {
+        // dE/dG = the sum of the njc row blocks; dE/dAs[lt][kt] = sum over the
+        // terms of lt of hw Y Y Y dE/dG[class][kt] (k_glam with the transposed tables)
+        const int64_t n = (int64_t)b->n_m * b->kl_pad;
+        hipLaunchKernelGGL(k_sum_blocks, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0, st,
+                           b->d_gGp, b->njc, (int64_t)b->nm_pad64 * b->kl_pad, n, b->d_gG);
+        hipLaunchKernelGGL(k_glam, dim3((unsigned)((b->kl_pad + b->lam_nt - 1) / b->lam_nt), (unsigned)b->n_lt),
+                           dim3((unsigned)b->lam_nt), b->lamb_lds, st, b->d_gG, b->kl_pad, b->d_ktk, (int)b->klength,
+                           b->d_lamY, b->K, b->d_lambyr, b->d_lambys, b->lamb_rows, b->d_lambent, b->d_lambcs,
+                           b->d_gA, b->kl_pad, nullptr, nullptr);
+    } else {
+        const int n_rt = b->nlt_pad64 / G1_BM, n_ct = (int)(b->kl_pad / G1_BN);
+        hipLaunchKernelGGL((k_gemm<G1_BM, G1_BN, G1_WGM, 2, false>), dim3((unsigned)(n_rt * n_ct)), dim3(256), 0, st,
+                           b->d_HT4, (int64_t)b->nlt_pad64, b->d_gGp, b->kl_pad, (int)(b->njc * b->nm_pad64),
+                           b->nlt_pad64, (int)b->kl_pad, n_rt, n_ct, b->d_gA, b->kl_pad, nullptr, nullptr, (int64_t)0,
+                           nullptr);
+    }
     TP_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_triple_bwd, dim3((unsigned)b->n_lt), dim3(1024), b->bwd_lds, st, coef, b->n_c, b->K, b->d_ew,
                        b->d_eidx, b->d_gstart, b->d_ltinfo, b->gLw, b->gEmax, b->d_gA, b->kl_pad, (int)b->klength,
@@ -2511,7 +2607,6 @@ int tp_energy_grad(tp_ctx *ctx, const tp_basis *bc, const tp_ref *r, const doubl
     if (!bc || !r || !E_out || !grad_out || !coeff) return tp_fail(TP_E_ARG, "null argument");
     tp_basis *b = const_cast<tp_basis *>(bc);
     if (b->ctx != ctx || r->b != b) return tp_fail(TP_E_STATE, "objects belong to another context/basis");
-    if (b->lam) return tp_fail(TP_E_UNSUPPORTED, "the energy gradient needs lambda dq = 0");
     if (ld < (int64_t)b->LMAX * b->LMAX) return tp_fail(TP_E_ARG, "coefficient vector shorter than LMAX^2");
     if (int rc = grad_setup(ctx, b)) return rc;
     if (int rc = upload_coeff(ctx, b, coeff, ld, b->K, b->d_coef, b->d_raw)) return rc;
