@@ -318,9 +318,10 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
     K_r ~ U{10..26} shells (drawn from its seed), starts from
     randomStartStructure(seed) and scores against the full histogram cut to
     K_r shells; every sweep evaluates nprop rotation proposals per run (one
-    graph replay for all runs) in the FP16 hi/lo contraction, on compensated
-    FP16 triple products (prec fp16x3) or FP64 ones on the sorted l-triples
-    (fp16x3_t64)."""
+    graph replay for all runs) on the sorted l-triples' mixed-precision
+    triple products (FP32 stage 1, FP16 hi/lo stage 2), then the dense FP16
+    hi/lo contraction c3 = F.As (prec fp16x3) or the factored one, G = H.As on
+    FP16 hi/lo MFMA and the FP64 separated grid (fp16x3_t64)."""
     Ks = [int(np.random.default_rng(s).integers(10, 27)) for s in seeds]
     starts = [workload.random_start_structure(truth, cfg["L"], s) for s in seeds]
     refs = [np.ascontiguousarray(c3ref[:, :, :K, :K, :K]) for K in Ks]
@@ -366,8 +367,8 @@ def batch_leg(cfg, tp, workload, truth, basis, c3ref, nsweeps, nprop=64, seeds=t
         "precision": prec,
         "mode": "%s: fp16x3 contraction on %s triple products, FP64 rotations/energy; random slots = the "
                 "reference's plane rotation in a Haar frame (|b| <= pi/4); best-of-nprop + Metropolis per sweep; "
-                "angle factor %.2f/sweep" % (prec, "FP64 (sorted l-triple)" if prec == "fp16x3_t64" else
-                                             "compensated FP16", SWEEP_FACTOR),
+                "angle factor %.2f/sweep" % (prec, "sorted l-triple (FP32 stage 1, FP16 hi/lo stage 2)",
+                                             SWEEP_FACTOR),
     }
 
 
@@ -970,7 +971,8 @@ def main():
         "kernel_ms": {"triple": kern_ms[0], "gemm_G": kern_ms[1], "c3sep_energy": kern_ms[2],
                       "final": kern_ms[3]},
         "mixed_precision": {
-            "mode": "fp16x3: c3 = F.As on FP16 MFMA, hi.hi + (hi.lo + lo.hi), FP32 accumulation, FP64 energy",
+            "mode": "fp16x3: sorted l-triple products (FP32 stage 1, FP16 hi/lo stage 2), c3 = F.As on FP16 MFMA, "
+                    "hi.hi + (hi.lo + lo.hi), FP32 accumulation, FP64 energy",
             "evals_per_s": args.steps / el16, "ms_per_eval": 1e3 * el16 / args.steps,
             "stage_ms": {"triple": kern16[0], "split": kern16[1], "gemm16_c3_energy": kern16[2], "final": kern16[3]},
             "gemm16_tflops_algorithmic": 2.0 * basis.R * basis.n_lt * basis.klength / (kern16[2] * 1e-3) / 1e12,

@@ -453,9 +453,10 @@ __global__ __launch_bounds__(256) void k_colscale_batch(const EvalDesc *__restri
     }
 }
 
-// As [lt][col] (FP64) -> scaled 16-bit copies [col][lt], one 64 x 64 tile per block.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_split_batch(const double *__restrict__ As, int64_t ldA, int nlt, int ldk,
+// As [lt][col] (FP64, or FP32 from k_triple_sym_batch) -> scaled 16-bit
+// copies [col][lt], one 64 x 64 tile per block.
+template <int MODE, typename AT = double>
+__global__ __launch_bounds__(256) void k_split_batch(const AT *__restrict__ As, int64_t ldA, int nlt, int ldk,
                                                      const double *__restrict__ sc, uint16_t *__restrict__ Ahi,
                                                      uint16_t *__restrict__ Alo)
 {
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) void k_split_batch(const double *__restrict__ 
     const int tid = threadIdx.x, c = tid & 63, g = tid >> 6;
     for (int r = g; r < 64; r += 4) {
         const int lt = lt0 + r;
-        tile[r][c] = lt < nlt ? As[(int64_t)lt * ldA + c0 + c] : 0.0;
+        tile[r][c] = lt < nlt ? (double)As[(int64_t)lt * ldA + c0 + c] : 0.0;
     }
     __syncthreads();
     const int kr = tid >> 2, q = tid & 3;

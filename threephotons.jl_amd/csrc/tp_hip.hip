@@ -315,8 +315,11 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
     hipStream_t st = ctx->stream;
     const int prec = force64 ? TP_PREC_FP64 : b->prec;
     if (timing) TP_HIP(hipEventRecord(ctx->ev[0], st));
-    const bool t16 = b->use_mfma && prec != TP_PREC_FP64 && prec != TP_PREC_FP16X3_T64;   // compensated FP16 triples
     const bool c16 = prec == TP_PREC_FP16X3 || prec == TP_PREC_BF16;   // dense FP16 contraction F . As
+    // compensated FP16 triple products (k_triple16): TRIPLE16, and the dense
+    // modes on bases without the sorted l-triple tables (which otherwise take
+    // the mixed sorted-triple kernel, as FP16X3_T64 and their batches do)
+    const bool t16 = b->use_mfma && (prec == TP_PREC_TRIPLE16 || (c16 && !b->use_sym));
     if (t16 || c16)   // shell scales (of the FP16 triple products) and norms (the contraction's column scales)
         hipLaunchKernelGGL(k_shell_scales, dim3((unsigned)b->K), dim3(256), 0, st, b0, b1, sel, flip, b->n_c, b->L,
                            sc.shsc);
@@ -325,8 +328,8 @@ static int enqueue_eval(tp_ctx *ctx, const tp_basis *b, const tp_ref *r, const d
         hipLaunchKernelGGL(k_triple16, dim3((unsigned)b->nwork), dim3(512), b->t16_lds, st, b0, b1, sel, flip,
                            sc.shsc, b->n_c, b->K, b->d_ew8, b->d_eidx8, b->d_gstart8, b->d_ltinfo, b->d_work, b->Lw,
                            b->EmaxLt8, b->Prows, sc.As, b->kl_pad);
-    } else if (b->use_sym) {   // FP64; TP_PREC_FP16X3_T64 with its stage 2 on FP16 MFMA
-        if (prec == TP_PREC_FP16X3_T64)
+    } else if (b->use_sym) {   // FP64; the mixed modes with stage 1 on FP32 and stage 2 on FP16 MFMA
+        if (prec != TP_PREC_FP64)
             hipLaunchKernelGGL(k_triple_sym16, dim3((unsigned)b->nsym_work), dim3(kSymThreads), b->sym16_lds, st, b0, b1,
                                sel, flip, b->n_c, b->K, b->d_ew, b->d_eidx, b->d_gstart, b->d_sym_work,
                                b->d_perms, b->d_gorder, b->d_symtab, b->d_symoff, b->Lw, sc.As, b->kl_pad,
@@ -525,6 +528,7 @@ struct tp_batch {
     int prec = TP_PREC_FP16X3;             // the basis's precision mode when the batch was made
     int gcand = 0, order = 0;              // contraction candidate (kTiles), triple-product work order (autotuned)
     bool fold = false;                     // TP_PREC_FP16X3_T64 energies on the folded grid (k_c3sep FOLD)
+    bool sym = false;                      // triple products on the sorted l-triples (k_triple_sym_batch, As in FP32)
     int sym_order = 0;                     // work order of the sorted l-triple items (batch_sym_work)
     int family = TP_MOVE_PLANE;            // move family of the random slots
     double tune_ms[2 + kNumTiles] = {};
@@ -733,6 +737,7 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
     B->sweep = sweep;
     B->hist_cap = hist_cap;
     B->fold = fac && b->fold;
+    B->sym = b->use_sym;
     for (int g = 0; g < ngroups; ++g) B->fold = B->fold && refs[g]->d_qfold;
     B->cstride = (int64_t)b->K * b->n_c;
     B->refs.assign(refs, refs + ngroups);
@@ -790,8 +795,8 @@ static int batch_build(tp_ctx *ctx, const tp_basis *b, int ngroups, int nprop, c
         (rc = dev_alloc(&B->d_prop, (size_t)B->ne * B->cstride)) ||
         (sweep && (rc = dev_alloc(&B->d_cur, (size_t)ngroups * B->cstride))) ||
         (rc = dev_alloc(&B->d_quat, (size_t)B->ne * std::max(1, B->nl) * 4)) ||
-        (!fac && (rc = dev_alloc(&B->d_As, (size_t)b->n_lt * B->ldA))) ||
-        (fac && (rc = dev_alloc(&B->d_As32, (size_t)b->n_lt * B->ldA))) || (rc = dev_alloc(&B->d_sc, B->ldA)) ||
+        (!B->sym && (rc = dev_alloc(&B->d_As, (size_t)b->n_lt * B->ldA))) ||
+        (B->sym && (rc = dev_alloc(&B->d_As32, (size_t)b->n_lt * B->ldA))) || (rc = dev_alloc(&B->d_sc, B->ldA)) ||
         (rc = dev_alloc(&B->d_isc, B->ldA)) || (rc = dev_alloc(&B->d_part, nrt_max * 2 * B->ldA)) ||
         (rc = dev_alloc(&B->d_E, B->ne)) || (rc = dev_alloc(&B->d_Ehist, (size_t)std::max(1, hist_cap) * ngroups)) ||
         (rc = dev_alloc(&B->d_shsc, (size_t)B->ne * b->K)) ||
@@ -959,7 +964,7 @@ static void batch_triple(const tp_batch *B, hipStream_t st)
     const tp_basis *b = B->b;
     hipLaunchKernelGGL(k_colscale_batch, dim3((unsigned)B->ne), dim3(256), 0, st, B->d_ev, b->n_c, b->L, B->d_sc,
                        B->d_isc, B->d_shsc, b->K);
-    if (B->prec == TP_PREC_FP16X3_T64)
+    if (B->sym)
         hipLaunchKernelGGL(k_triple_sym_batch, dim3((unsigned)B->nswork), dim3(kSymThreadsBatch), b->sym_lds_b, st, B->d_ev,
                            B->d_swork, B->d_prop, (int64_t)B->cstride, b->n_c, b->d_ew, b->d_eidx, b->d_gstart,
                            b->d_perms, b->d_gorder,
@@ -974,7 +979,13 @@ static void batch_split(const tp_batch *B, hipStream_t st)
 {
     const tp_basis *b = B->b;
     const dim3 gs((unsigned)B->ncb, (unsigned)(b->ldk / 64));
-    if (B->prec != TP_PREC_BF16)
+    if (B->sym && B->prec != TP_PREC_BF16)
+        hipLaunchKernelGGL((k_split_batch<1, float>), gs, dim3(256), 0, st, B->d_As32, B->ldA, (int)b->n_lt, b->ldk,
+                           B->d_sc, B->d_Ahi, B->d_Alo);
+    else if (B->sym)
+        hipLaunchKernelGGL((k_split_batch<2, float>), gs, dim3(256), 0, st, B->d_As32, B->ldA, (int)b->n_lt, b->ldk,
+                           B->d_sc, B->d_Ahi, nullptr);
+    else if (B->prec != TP_PREC_BF16)
         hipLaunchKernelGGL(k_split_batch<1>, gs, dim3(256), 0, st, B->d_As, B->ldA, (int)b->n_lt, b->ldk, B->d_sc,
                            B->d_Ahi, B->d_Alo);
     else
@@ -1078,8 +1089,8 @@ static int batch_autotune(tp_batch *B, int reps)
     hipStream_t st = B->ctx->stream;
     double best = 1e30;
     int best_order = 0;
-    // work orders: of the coupling form, or (FP16X3_T64) of the sorted l-triple form
-    const bool sym = B->prec == TP_PREC_FP16X3_T64;
+    // work orders: of the sorted l-triple form, or (bases from a caller's weights) of the coupling form
+    const bool sym = B->sym;
     for (int order = 0; order < 2; ++order) {
         B->tune_ms[order] = 1e30;
         if (int rc = sym ? batch_sym_work(B, order) : batch_upload_work(B, order)) return rc;
@@ -4328,7 +4339,7 @@ int tp_batch_autotune(tp_batch *B, int reps, double *ms_out, int *config_out)
     } else if (config_out) {
         config_out[0] = kTiles[B->gcand][0];
         config_out[1] = kTiles[B->gcand][1];
-        config_out[2] = B->order;
+        config_out[2] = B->sym ? B->sym_order : B->order;
         config_out[3] = kTiles[B->gcand][2];
     }
     return 0;
