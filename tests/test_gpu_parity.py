@@ -1,6 +1,6 @@
 """Parity of the gfx950 path (libtp.so through its C ABI) with the CPU oracle.
 
-Bar (north star): c3 and the Bayes energy within 1e-10 relative of the FP64
+Bar: c3 and the Bayes energy within 1e-10 relative of the FP64
 oracle; histogram bin indexing exact.  c3 error is max|gpu - oracle| /
 max|oracle| over the whole (R, klength) matrix."""
 import math

@@ -1,7 +1,7 @@
 """Mixed-precision contraction (TP_PREC_FP16X3 / TP_PREC_FP16X3_T64 /
 TP_PREC_BF16 / TP_PREC_TRIPLE16) against the FP64 path, which itself matches the oracle to ~1e-15 (test_gpu_parity.py).
 
-Bar (north star): c3 relative error <= 1e-5 against FP64 for the compensated
+Bar: c3 relative error <= 1e-5 against FP64 for the compensated
 FP16 form.  c3 error per k-triple slice = max_r |c3 - c3_64| / max_r |c3_64|."""
 import numpy as np
 import pytest
